@@ -1,0 +1,98 @@
+// Host-side, value-independent analysis of the Newton-Raphson Jacobian and
+// of its batched multifrontal LU.  Runs once per NewtonSolver::initialize().
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+namespace cupf {
+
+// ---------------------------------------------------------------------------
+// Jacobian structure.  Restates cuPF jacobian_analysis.cpp:257-445:
+// x = [Va@pv | Va@pq | Vm@pq] and F = [P@pv | P@pq | Q@pq] share one index
+// space; the CSR pattern has sorted, de-duplicated rows; map11..22[k] give the
+// CSR slot each Ybus entry k contributes to (or -1), diag11..22[bus] the
+// diagonal-block slots.
+// ---------------------------------------------------------------------------
+struct JacobianStructure {
+    int n_bus = 0, n_pv = 0, n_pq = 0, dim = 0, nnz = 0;
+    std::vector<int> pv, pq, pvpq;        // pvpq = pv || pq
+    std::vector<int> row_pvpq, row_pq;    // bus -> J index (or -1)
+    std::vector<int> row_ptr, col_idx;    // CSR pattern of J
+    std::vector<int> map11, map21, map12, map22;   // [nnzY]
+    std::vector<int> diag11, diag21, diag12, diag22;  // [n_bus]
+    std::vector<int> row_bus;             // J row -> bus
+    std::vector<uint8_t> row_is_q;        // J row -> 1 for a Q (Vm) row
+};
+
+JacobianStructure analyze_jacobian(int n_bus, const int* ybus_indptr, const int* ybus_indices,
+                                   int nnzY, const int* pv, int n_pv, const int* pq, int n_pq);
+
+// ---------------------------------------------------------------------------
+// Symbolic LU (symmetric-pattern multifrontal).  Given a column order
+// `perm` (perm[new] = old), the pieces below follow the reference's
+// custom_linear_solver/src/analyze: symmetric pattern, elimination tree,
+// postorder, column counts, fill pattern (symbolic/elimination_tree.cpp),
+// relaxed panels (symbolic/supernode.cpp:104-141), fronts and the extend-add
+// map (symbolic/multifrontal.cpp:224-320).
+// ---------------------------------------------------------------------------
+struct Symbolic {
+    int n = 0;
+    std::vector<int> perm, iperm;         // perm[new] = old
+    std::vector<int> sym_cp, sym_ri;      // permuted A+A^T (no diagonal), CSC
+    std::vector<int> parent, post, colcount;
+    std::vector<int> Lp, Li;              // fill pattern of L (incl. diagonal)
+    // relaxed panels
+    int cap = 8;
+    int num_panels = 0;
+    std::vector<int> panel_first, panel_ncols, panel_width, panel_of;
+    // fronts
+    std::vector<int> panel_parent, front_ptr, front_rows, asm_ptr, asm_idx;
+};
+
+// Off-diagonal symmetric pattern of a square CSR pattern, as CSC with sorted
+// rows and no duplicates.
+void symmetric_pattern(int n, const int* rp, const int* ci, std::vector<int>& cp,
+                       std::vector<int>& ri);
+
+std::vector<int> etree(int n, const int* cp, const int* ri);
+std::vector<int> postorder(const std::vector<int>& parent);
+std::vector<int> column_counts(int n, const int* cp, const int* ri, const std::vector<int>& parent,
+                               const std::vector<int>& post);
+void fill_pattern(int n, const int* cp, const int* ri, const std::vector<int>& parent,
+                  std::vector<int>& Lp, std::vector<int>& Li);
+
+// Fill-reducing order of the Jacobian: minimum (weighted) degree on the bus
+// graph (PV bus weight 1, PQ bus weight 2), expanded to J indices and then
+// relabelled by the elimination-tree postorder so panels are contiguous.
+std::vector<int> order_jacobian(const JacobianStructure& js);
+
+// Full symbolic pipeline for a given order (no reordering is done here).
+Symbolic analyze_symbolic(int n, const int* rp, const int* ci, const std::vector<int>& perm,
+                          int cap);
+
+// ---------------------------------------------------------------------------
+// Lowered plan for the batched GPU factorization.  Fronts are processed in
+// panel-id order (a postorder: every child precedes its parent).  Each front
+// is a dense fsz x fsz row-major block in a per-scenario arena.
+// ---------------------------------------------------------------------------
+struct LuPlan {
+    int n = 0, nnz = 0, num_fronts = 0;
+    long long arena = 0;                  // elements per scenario
+    std::vector<int> perm, iperm;
+    std::vector<int> fsz, nc;             // per front
+    std::vector<long long> off;           // per front arena offset
+    std::vector<int> front_ptr, front_rows;   // global (new) row indices
+    std::vector<int> child_ptr, child_idx;    // children per front
+    std::vector<int> asm_ptr, asm_loc;    // per front: CB row -> parent-local index
+    std::vector<int> ent_ptr, ent_src, ent_dst;  // A entries: CSR slot -> local pos
+    // work figures
+    double flops = 0.0;                   // LU flops per scenario
+    double factor_bytes_min = 0.0;        // elements touched once: sum fsz^2
+    int max_fsz = 0, max_nc = 0;
+    long long lnz = 0;                    // nnz(L) incl. diagonal (symbolic)
+};
+
+LuPlan lower_plan(const Symbolic& s, int n, const int* rp, const int* ci);
+
+}  // namespace cupf

@@ -1,0 +1,406 @@
+// extern "C" boundary of libcupf_hip.so (declared in include/cupf_hip.h).
+// Every entry point catches the C++ exceptions of the solver and turns them
+// into a status code plus a thread-local message.
+#include "../../include/cupf_hip.h"
+
+#include "analyze.hpp"
+#include "newton_solver.hpp"
+
+#include <complex>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+
+struct cupf_solver {
+    NewtonSolver* impl;
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+template <typename F>
+int guarded(F&& f) {
+    try {
+        f();
+        g_last_error.clear();
+        return CUPF_OK;
+    } catch (const std::invalid_argument& e) {
+        g_last_error = e.what();
+        return CUPF_INVALID_ARGUMENT;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return CUPF_RUNTIME_ERROR;
+    } catch (...) {
+        g_last_error = "unknown error";
+        return CUPF_RUNTIME_ERROR;
+    }
+}
+
+NewtonSolver& solver_of(cupf_solver* s) {
+    if (!s || !s->impl) throw std::invalid_argument("null solver handle");
+    return *s->impl;
+}
+
+NRConfig to_cfg(const cupf_nr_config* c) {
+    NRConfig r;
+    if (c) { r.tolerance = c->tolerance; r.max_iter = c->max_iter; }
+    return r;
+}
+
+SolveOptions to_sopt(const cupf_solve_options* c) {
+    SolveOptions r;
+    if (c) {
+        r.prepare_adjoint_cache = c->prepare_adjoint_cache != 0;
+        r.adjoint_cache_mode = static_cast<AdjointCacheMode>(c->adjoint_cache_mode);
+        r.allow_explicit_transpose_fallback = c->allow_explicit_transpose_fallback != 0;
+    }
+    return r;
+}
+
+AdjointOptions to_aopt(const cupf_adjoint_options* c) {
+    AdjointOptions r;
+    if (c) {
+        r.reuse_forward_factorization = c->reuse_forward_factorization != 0;
+        r.allow_refactorize = c->allow_refactorize != 0;
+        r.require_cached_factorization = c->require_cached_factorization != 0;
+        r.allow_refactorize_for_backward = c->allow_refactorize_for_backward != 0;
+        r.allow_inexact_last_newton_factorization = c->allow_inexact_last_newton_factorization != 0;
+        r.use_transpose_solve = c->use_transpose_solve != 0;
+        r.allow_explicit_transpose_fallback = c->allow_explicit_transpose_fallback != 0;
+        r.compute_load_gradients = c->compute_load_gradients != 0;
+        r.check_residual = c->check_residual != 0;
+    }
+    return r;
+}
+
+NewtonOptions to_nopt(const cupf_newton_options* c) {
+    NewtonOptions r;
+    if (!c) return r;
+    r.backend = static_cast<BackendKind>(c->backend);
+    r.compute = static_cast<ComputePolicy>(c->compute);
+    r.cuda_linear_solver = static_cast<CudaLinearSolverKind>(c->cuda_linear_solver);
+    r.cpu_linear_solver = static_cast<CpuLinearSolverKind>(c->cpu_linear_solver);
+    r.cuda_jacobian = static_cast<CudaJacobianKind>(c->cuda_jacobian);
+    r.cudss.use_matching = c->cudss_use_matching != 0;
+    r.cudss.matching_alg = static_cast<CuDSSAlgorithm>(c->cudss_matching_alg);
+    r.cudss.auto_pivot_epsilon = c->cudss_auto_pivot_epsilon != 0;
+    r.cudss.pivot_epsilon = c->cudss_pivot_epsilon;
+    r.custom.precision = static_cast<CustomPrecision>(c->custom.precision);
+    r.custom.matching = static_cast<CustomMatchingMode>(c->custom.matching);
+    r.custom.pivot_strategy = static_cast<CustomPivotStrategy>(c->custom.pivot_strategy);
+    r.custom.serial_nd = c->custom.serial_nd != 0;
+    r.custom.max_panel_width = c->custom.max_panel_width;
+    r.custom.shift_retry_epsilon = c->custom.shift_retry_epsilon;
+    r.custom.refine_steps = c->custom.refine_steps;
+    r.use_cuda_graph = c->use_cuda_graph != 0;
+    return r;
+}
+
+void fill_info(const AdjointResult& a, cupf_adjoint_info* o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->n_bus = a.n_bus; o->batch_size = a.batch_size; o->dimF = a.dimF;
+    o->success = a.success; o->used_adjoint_cache = a.used_adjoint_cache;
+    o->adjoint_cache_matches_final_state = a.adjoint_cache_matches_final_state;
+    o->reused_forward_factorization = a.reused_forward_factorization;
+    o->reused_final_state_factorization = a.reused_final_state_factorization;
+    o->refactorized_for_backward = a.refactorized_for_backward;
+    o->used_explicit_transpose = a.used_explicit_transpose;
+    o->used_python_scipy = a.used_python_scipy;
+    o->includes_host_device_transfer = a.includes_host_device_transfer;
+    o->zero_copy = a.zero_copy; o->torch_extension_zero_copy = a.torch_extension_zero_copy;
+    o->raw_pointer_api_used = a.raw_pointer_api_used;
+    o->current_stream_integrated = a.current_stream_integrated;
+    o->jt_symbolic_analyzed_at_initialize = a.jt_symbolic_analyzed_at_initialize;
+    o->jt_values_transposed_on_device = a.jt_values_transposed_on_device;
+    o->jt_factorized_during_forward_cache = a.jt_factorized_during_forward_cache;
+    o->jt_refactorized_during_backward = a.jt_refactorized_during_backward;
+    o->host_roundtrip_for_jt_transpose = a.host_roundtrip_for_jt_transpose;
+    o->jt_residual_norm = a.jt_residual_norm; o->solve_time_ms = a.solve_time_ms;
+    o->transpose_solve_time_ms = a.transpose_solve_time_ms;
+    o->factorization_time_ms = a.factorization_time_ms; o->total_time_ms = a.total_time_ms;
+    std::strncpy(o->backend, a.backend.c_str(), sizeof(o->backend) - 1);
+    std::strncpy(o->transpose_solve_backend, a.transpose_solve_backend.c_str(),
+                 sizeof(o->transpose_solve_backend) - 1);
+}
+
+YbusView make_view(int32_t rows, int32_t cols, const int32_t* indptr, const int32_t* indices,
+                   const double* data, int32_t nnz) {
+    if (rows <= 0 || cols <= 0) throw std::invalid_argument("Ybus rows/cols must be positive");
+    if (!indptr) throw std::invalid_argument("Ybus indptr must not be null");
+    return YbusView{indptr, indices, reinterpret_cast<const std::complex<double>*>(data), rows,
+                    cols, nnz};
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* cupf_last_error(void) { return g_last_error.c_str(); }
+
+void cupf_default_nr_config(cupf_nr_config* o) {
+    NRConfig d;
+    o->tolerance = d.tolerance;
+    o->max_iter = d.max_iter;
+}
+
+void cupf_default_solve_options(cupf_solve_options* o) {
+    SolveOptions d;
+    o->prepare_adjoint_cache = d.prepare_adjoint_cache;
+    o->adjoint_cache_mode = int32_t(d.adjoint_cache_mode);
+    o->allow_explicit_transpose_fallback = d.allow_explicit_transpose_fallback;
+}
+
+void cupf_default_newton_options(cupf_newton_options* o) {
+    NewtonOptions d;
+    std::memset(o, 0, sizeof(*o));
+    o->backend = int32_t(d.backend);
+    o->compute = int32_t(d.compute);
+    o->cuda_linear_solver = int32_t(d.cuda_linear_solver);
+    o->cpu_linear_solver = int32_t(d.cpu_linear_solver);
+    o->cuda_jacobian = int32_t(d.cuda_jacobian);
+    o->cudss_use_matching = d.cudss.use_matching;
+    o->cudss_matching_alg = int32_t(d.cudss.matching_alg);
+    o->cudss_auto_pivot_epsilon = d.cudss.auto_pivot_epsilon;
+    o->cudss_pivot_epsilon = d.cudss.pivot_epsilon;
+    o->custom.precision = int32_t(d.custom.precision);
+    o->custom.matching = int32_t(d.custom.matching);
+    o->custom.pivot_strategy = int32_t(d.custom.pivot_strategy);
+    o->custom.serial_nd = d.custom.serial_nd;
+    o->custom.max_panel_width = d.custom.max_panel_width;
+    o->custom.shift_retry_epsilon = d.custom.shift_retry_epsilon;
+    o->custom.refine_steps = d.custom.refine_steps;
+    o->use_cuda_graph = d.use_cuda_graph;
+}
+
+void cupf_default_adjoint_options(cupf_adjoint_options* o) {
+    AdjointOptions d;
+    o->reuse_forward_factorization = d.reuse_forward_factorization;
+    o->allow_refactorize = d.allow_refactorize;
+    o->require_cached_factorization = d.require_cached_factorization;
+    o->allow_refactorize_for_backward = d.allow_refactorize_for_backward;
+    o->allow_inexact_last_newton_factorization = d.allow_inexact_last_newton_factorization;
+    o->use_transpose_solve = d.use_transpose_solve;
+    o->allow_explicit_transpose_fallback = d.allow_explicit_transpose_fallback;
+    o->compute_load_gradients = d.compute_load_gradients;
+    o->check_residual = d.check_residual;
+}
+
+int cupf_solver_create(const cupf_newton_options* options, cupf_solver** out) {
+    return guarded([&] {
+        if (!out) throw std::invalid_argument("out must not be null");
+        *out = nullptr;
+        auto* s = new cupf_solver{nullptr};
+        try {
+            s->impl = new NewtonSolver(to_nopt(options));
+        } catch (...) {
+            delete s;
+            throw;
+        }
+        *out = s;
+    });
+}
+
+int cupf_solver_destroy(cupf_solver* solver) {
+    return guarded([&] {
+        if (!solver) return;
+        delete solver->impl;
+        delete solver;
+    });
+}
+
+int cupf_solver_initialize(cupf_solver* solver, int32_t rows, int32_t cols,
+                           const int32_t* indptr, const int32_t* indices, const double* data,
+                           int32_t nnz, const int32_t* pv, int32_t n_pv, const int32_t* pq,
+                           int32_t n_pq) {
+    return guarded([&] {
+        solver_of(solver).initialize(make_view(rows, cols, indptr, indices, data, nnz), pv, n_pv,
+                                     pq, n_pq);
+    });
+}
+
+int cupf_solver_solve_batch(cupf_solver* solver, int32_t rows, int32_t cols,
+                            const int32_t* indptr, const int32_t* indices, const double* data,
+                            int32_t nnz, const double* sbus, int64_t sbus_stride,
+                            const double* V0, int64_t V0_stride, int32_t batch,
+                            const int32_t* pv, int32_t n_pv, const int32_t* pq, int32_t n_pq,
+                            const cupf_nr_config* config, const cupf_solve_options* options,
+                            double* V_out, int32_t* iterations, double* final_mismatch,
+                            uint8_t* converged, uint8_t* singular) {
+    return guarded([&] {
+        NRBatchResult r;
+        solver_of(solver).solve_batch(make_view(rows, cols, indptr, indices, data, nnz),
+                                      reinterpret_cast<const std::complex<double>*>(sbus),
+                                      sbus_stride,
+                                      reinterpret_cast<const std::complex<double>*>(V0),
+                                      V0_stride, batch, pv, n_pv, pq, n_pq, to_cfg(config),
+                                      to_sopt(options), r);
+        if (V_out) std::memcpy(V_out, r.V.data(), r.V.size() * sizeof(std::complex<double>));
+        for (int b = 0; b < batch; ++b) {
+            if (iterations) iterations[b] = r.iterations[b];
+            if (final_mismatch) final_mismatch[b] = r.final_mismatch[b];
+            if (converged) converged[b] = r.converged[b];
+            if (singular) singular[b] = r.singular[b];
+        }
+    });
+}
+
+int cupf_solver_solve_adjoint(cupf_solver* solver, const double* grad_va, int64_t grad_va_stride,
+                              const double* grad_vm, int64_t grad_vm_stride, int32_t batch,
+                              const int32_t* pv, int32_t n_pv, const int32_t* pq, int32_t n_pq,
+                              const cupf_adjoint_options* options, double* lambda,
+                              double* grad_load_p, double* grad_load_q, cupf_adjoint_info* info) {
+    return guarded([&] {
+        AdjointResult r;
+        solver_of(solver).solve_adjoint(grad_va, grad_va_stride, grad_vm, grad_vm_stride, batch,
+                                        pv, n_pv, pq, n_pq, to_aopt(options), r);
+        if (lambda) std::memcpy(lambda, r.lambda.data(), r.lambda.size() * sizeof(double));
+        if (grad_load_p && !r.grad_load_p.empty())
+            std::memcpy(grad_load_p, r.grad_load_p.data(), r.grad_load_p.size() * sizeof(double));
+        if (grad_load_q && !r.grad_load_q.empty())
+            std::memcpy(grad_load_q, r.grad_load_q.data(), r.grad_load_q.size() * sizeof(double));
+        fill_info(r, info);
+    });
+}
+
+int cupf_solver_solve_torch_forward(cupf_solver* solver, const void* sbus_base_re,
+                                    const void* sbus_base_im, const void* load_p,
+                                    const void* load_q, const void* v0_va, const void* v0_vm,
+                                    void* va_out, void* vm_out, int32_t batch, int32_t n_bus,
+                                    const char* dtype, const cupf_nr_config* config,
+                                    const cupf_solve_options* options, void* stream,
+                                    cupf_adjoint_info* info) {
+    return guarded([&] {
+        AdjointResult r;
+        solver_of(solver).solve_torch_forward(sbus_base_re, sbus_base_im, load_p, load_q, v0_va,
+                                              v0_vm, va_out, vm_out, batch, n_bus, dtype,
+                                              to_cfg(config), to_sopt(options), stream, r);
+        fill_info(r, info);
+    });
+}
+
+int cupf_solver_solve_torch_backward(cupf_solver* solver, const void* grad_va, const void* grad_vm,
+                                     void* grad_load_p, void* grad_load_q, int32_t batch,
+                                     int32_t n_bus, const char* dtype,
+                                     const cupf_adjoint_options* options, void* stream,
+                                     cupf_adjoint_info* info) {
+    return guarded([&] {
+        AdjointResult r;
+        solver_of(solver).solve_torch_backward(grad_va, grad_vm, grad_load_p, grad_load_q, batch,
+                                               n_bus, dtype, to_aopt(options), stream, r);
+        fill_info(r, info);
+    });
+}
+
+int cupf_solver_get_stats(cupf_solver* solver, cupf_solver_stats* out) {
+    return guarded([&] {
+        const NewtonSolverStats& s = solver_of(solver).stats();
+        out->last_iterations = s.last_iterations;
+        out->last_batch = s.last_batch;
+        out->factorizations = s.factorizations;
+        out->last_solve_ms = s.last_solve_ms;
+        out->dim = s.dim; out->nnz_j = s.nnz_j; out->num_fronts = s.num_fronts;
+        out->max_front = s.max_front; out->max_pivots = s.max_pivots;
+        out->arena_elems = s.arena_elems; out->lnz = s.lnz; out->factor_flops = s.factor_flops;
+    });
+}
+
+int cupf_solver_debug_residual(cupf_solver* solver, const double* sbus, const double* V,
+                               int32_t batch, double* F, double* norm) {
+    return guarded([&] {
+        solver_of(solver).debug_residual(reinterpret_cast<const std::complex<double>*>(sbus),
+                                         reinterpret_cast<const std::complex<double>*>(V), batch,
+                                         F, norm);
+    });
+}
+
+int cupf_solver_debug_jacobian(cupf_solver* solver, const double* V, int32_t batch, double* J) {
+    return guarded([&] {
+        solver_of(solver).debug_jacobian(reinterpret_cast<const std::complex<double>*>(V), batch, J);
+    });
+}
+
+int cupf_solver_debug_factor_solve(cupf_solver* solver, const double* A, const double* rhs,
+                                   int32_t batch, int32_t transpose, double* x,
+                                   uint8_t* singular) {
+    return guarded([&] {
+        solver_of(solver).debug_factor_solve(A, rhs, batch, transpose != 0, x, singular);
+    });
+}
+
+int cupf_solver_debug_pattern(cupf_solver* solver, int32_t* row_ptr, int32_t* col_idx,
+                              int32_t* perm) {
+    return guarded([&] { solver_of(solver).debug_pattern(row_ptr, col_idx, perm); });
+}
+
+int cupf_analyze_jacobian(int32_t n_bus, const int32_t* indptr, const int32_t* indices,
+                          const int32_t* pv, int32_t n_pv, const int32_t* pq, int32_t n_pq,
+                          int32_t* dim, int32_t* nnz, int32_t* row_ptr, int32_t* col_idx,
+                          int32_t* map11, int32_t* map21, int32_t* map12, int32_t* map22,
+                          int32_t* diag11, int32_t* diag21, int32_t* diag12, int32_t* diag22) {
+    return guarded([&] {
+        if (!indptr || !indices) throw std::invalid_argument("null Ybus pattern");
+        const cupf::JacobianStructure js =
+            cupf::analyze_jacobian(n_bus, indptr, indices, indptr[n_bus], pv, n_pv, pq, n_pq);
+        *dim = js.dim;
+        *nnz = js.nnz;
+        if (!col_idx) return;
+        auto cp = [](const std::vector<int>& v, int32_t* o) { if (o) std::memcpy(o, v.data(), v.size() * sizeof(int)); };
+        cp(js.row_ptr, row_ptr); cp(js.col_idx, col_idx);
+        cp(js.map11, map11); cp(js.map21, map21); cp(js.map12, map12); cp(js.map22, map22);
+        cp(js.diag11, diag11); cp(js.diag21, diag21); cp(js.diag12, diag12); cp(js.diag22, diag22);
+    });
+}
+
+int cupf_order_jacobian(int32_t n_bus, const int32_t* indptr, const int32_t* indices,
+                        const int32_t* pv, int32_t n_pv, const int32_t* pq, int32_t n_pq,
+                        int32_t* perm) {
+    return guarded([&] {
+        if (!indptr || !indices || !perm) throw std::invalid_argument("null argument");
+        const cupf::JacobianStructure js =
+            cupf::analyze_jacobian(n_bus, indptr, indices, indptr[n_bus], pv, n_pv, pq, n_pq);
+        const auto p = cupf::order_jacobian(js);
+        std::memcpy(perm, p.data(), p.size() * sizeof(int));
+    });
+}
+
+int cupf_symbolic(int32_t n, const int32_t* row_ptr, const int32_t* col_idx, const int32_t* perm,
+                  int32_t cap, int64_t* sizes, int32_t* parent, int32_t* post, int32_t* colcount,
+                  int32_t* Lp, int32_t* Li, int32_t* panel_first, int32_t* panel_ncols,
+                  int32_t* panel_width, int32_t* panel_of, int32_t* panel_parent,
+                  int32_t* front_ptr, int32_t* front_rows, int32_t* asm_ptr, int32_t* asm_idx) {
+    return guarded([&] {
+        if (!row_ptr || !col_idx || !perm || !sizes) throw std::invalid_argument("null argument");
+        const std::vector<int> pv(perm, perm + n);
+        const cupf::Symbolic s = cupf::analyze_symbolic(n, row_ptr, col_idx, pv, cap);
+        sizes[0] = (int64_t)s.Li.size();
+        sizes[1] = s.num_panels;
+        sizes[2] = (int64_t)s.front_rows.size();
+        sizes[3] = (int64_t)s.asm_idx.size();
+        if (!Li) return;
+        auto cp = [](const std::vector<int>& v, int32_t* o) { if (o) std::memcpy(o, v.data(), v.size() * sizeof(int)); };
+        cp(s.parent, parent); cp(s.post, post); cp(s.colcount, colcount); cp(s.Lp, Lp);
+        cp(s.Li, Li); cp(s.panel_first, panel_first); cp(s.panel_ncols, panel_ncols);
+        cp(s.panel_width, panel_width); cp(s.panel_of, panel_of); cp(s.panel_parent, panel_parent);
+        cp(s.front_ptr, front_ptr); cp(s.front_rows, front_rows); cp(s.asm_ptr, asm_ptr);
+        cp(s.asm_idx, asm_idx);
+    });
+}
+
+int cupf_plan_figures(int32_t n, const int32_t* row_ptr, const int32_t* col_idx,
+                      const int32_t* perm, int32_t cap, double* out5) {
+    return guarded([&] {
+        if (!row_ptr || !col_idx || !perm || !out5) throw std::invalid_argument("null argument");
+        const std::vector<int> pv(perm, perm + n);
+        const cupf::Symbolic s = cupf::analyze_symbolic(n, row_ptr, col_idx, pv, cap);
+        const cupf::LuPlan p = cupf::lower_plan(s, n, row_ptr, col_idx);
+        out5[0] = p.flops;
+        out5[1] = p.factor_bytes_min;
+        out5[2] = p.max_fsz;
+        out5[3] = p.num_fronts;
+        out5[4] = double(p.lnz);
+    });
+}
+
+}  // extern "C"

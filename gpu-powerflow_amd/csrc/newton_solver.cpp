@@ -1,0 +1,762 @@
+// NewtonSolver driver: host orchestration of the HIP kernels.
+//
+// Loop semantics follow cuPF core/newton_solver.cpp:497-558: each iteration
+// evaluates the residual and its infinity norm first and stops before the
+// Jacobian when the batch-wide worst norm is within tolerance; `iterations`
+// counts residual evaluations and is reported batch-wide; a non-finite norm
+// raises std::runtime_error; converged[b] = final_mismatch[b] <= tolerance.
+#include "newton_solver.hpp"
+
+#include "analyze.hpp"
+#include "hip_common.hpp"
+#include "kernels.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+
+using namespace cupf;
+
+namespace {
+
+void validate_batch_args(int32_t batch_size, int64_t stride, int32_t n_bus, const char* name) {
+    if (batch_size <= 0)
+        throw std::invalid_argument("NewtonSolver::solve_batch(): batch_size must be positive");
+    if (stride < n_bus)
+        throw std::invalid_argument(std::string("NewtonSolver::solve_batch(): ") + name +
+                                    " stride must be at least n_bus");
+}
+
+}  // namespace
+
+struct NewtonSolver::Impl {
+    NewtonOptions opt;
+    bool initialized = false;
+    NewtonSolverStats st;
+
+    // precision profile
+    bool front64 = true;   // fronts in FP64
+    bool dx32 = false;     // Newton step stored in FP32 (ComputePolicy::FP32)
+    int refine = 0;        // FP64 refinement steps of the step
+    bool static_piv = true;
+    double piv_eps = 1e-8;
+
+    // host structure
+    int n = 0, nnzY = 0;
+    std::vector<int> yp, yi;
+    std::vector<int> pv, pq;
+    JacobianStructure js;
+    Symbolic sym;
+    LuPlan plan;
+    std::vector<int> zero_slots;
+    std::vector<int> jt_rp, jt_ci, jt_map;
+    std::vector<long long> uoff;
+
+    // device structure
+    DeviceBuffer d_yp, d_yi, d_y, d_row_pvpq, d_row_pq, d_m11, d_m21, d_m12, d_m22, d_zero,
+        d_row_bus, d_row_isq, d_jrp, d_jci, d_jtrp, d_jtci, d_jtmap;
+    DeviceBuffer d_fsz, d_nc, d_off, d_uoff, d_fptr, d_frows, d_cptr, d_cidx, d_aptr, d_aloc,
+        d_eptr, d_esrc, d_edst, d_perm;
+    kern::NrDevice nr;
+    kern::LuDevice lu;
+
+    // batch buffers
+    int capB = 0;
+    DeviceBuffer V, Va, Vm, S, F, normF, bmax, J, arena, work, upd, dx, ddx, res, sing, lam, g,
+        io_a, io_b;
+    hipStream_t own = nullptr;
+    hipStream_t s = nullptr;   // stream of the current call
+
+    // adjoint cache
+    bool cache = false;
+    int cache_B = 0;
+    int last_B = 0;
+
+    size_t tsize() const { return front64 ? sizeof(double) : sizeof(float); }
+    size_t xsize() const { return dx32 ? sizeof(float) : sizeof(double); }
+
+    void ensure_batch(int B) {
+        if (B <= capB) return;
+        const size_t nb = size_t(B);
+        V.allocate(nb * n * sizeof(double2));
+        Va.allocate(nb * n * sizeof(double));
+        Vm.allocate(nb * n * sizeof(double));
+        S.allocate(nb * n * sizeof(double2));
+        F.allocate(nb * js.dim * sizeof(double));
+        normF.allocate(nb * sizeof(double));
+        bmax.allocate(sizeof(double));
+        J.allocate(nb * js.nnz * sizeof(double));
+        arena.allocate(nb * size_t(plan.arena) * tsize());
+        work.allocate(nb * js.dim * tsize());
+        upd.allocate(nb * std::max<size_t>(1, size_t(lu.upd_total)) * tsize());
+        dx.allocate(nb * js.dim * xsize());
+        ddx.allocate(nb * js.dim * xsize());
+        res.allocate(nb * js.dim * sizeof(double));
+        sing.allocate(nb * sizeof(int));
+        lam.allocate(nb * js.dim * sizeof(double));
+        g.allocate(nb * js.dim * sizeof(double));
+        io_a.allocate(nb * n * sizeof(double2));
+        io_b.allocate(nb * n * sizeof(double2));
+        capB = B;
+    }
+
+    // ---- one linear step: factor J (current values) and solve J x = rhs ----
+    void factor(int B) {
+        if (front64)
+            kern::lu_factor<double, double>(lu, J.as<double>(), js.nnz, B, arena.as<double>(),
+                                            piv_eps, static_piv ? 1 : 0, sing.as<int>(), s);
+        else
+            kern::lu_factor<double, float>(lu, J.as<double>(), js.nnz, B, arena.as<float>(),
+                                           piv_eps, static_piv ? 1 : 0, sing.as<int>(), s);
+        ++st.factorizations;
+    }
+
+    // rhs (FP64, [B][dim]) -> x; transposed systems use the J^T pattern for
+    // the refinement residual.
+    template <typename TX>
+    void solve_into(int B, const double* rhs, TX* x, TX* xtmp, bool transpose) {
+        if (front64) {
+            kern::lu_solve<double, double, TX>(lu, arena.as<double>(), rhs, js.dim, B,
+                                               work.as<double>(), upd.as<double>(), x, js.dim,
+                                               transpose, s);
+            return;
+        }
+        kern::lu_solve<float, double, TX>(lu, arena.as<float>(), rhs, js.dim, B, work.as<float>(),
+                                          upd.as<float>(), x, js.dim, transpose, s);
+        for (int it = 0; it < refine; ++it) {
+            if (transpose)
+                kern::refine_residual<TX>(d_jtrp.as<int>(), d_jtci.as<int>(), d_jtmap.as<int>(),
+                                          J.as<double>(), js.dim, js.nnz, rhs, x, B,
+                                          res.as<double>(), s);
+            else
+                kern::refine_residual<TX>(d_jrp.as<int>(), d_jci.as<int>(), nullptr,
+                                          J.as<double>(), js.dim, js.nnz, rhs, x, B,
+                                          res.as<double>(), s);
+            kern::lu_solve<float, double, TX>(lu, arena.as<float>(), res.as<double>(), js.dim, B,
+                                              work.as<float>(), upd.as<float>(), xtmp, js.dim,
+                                              transpose, s);
+            kern::axpy_update<TX>(xtmp, x, (long long)B * js.dim, s);
+        }
+    }
+
+    void newton_step(int B) {
+        factor(B);
+        if (dx32) {
+            solve_into<float>(B, F.as<double>(), dx.as<float>(), ddx.as<float>(), false);
+            kern::voltage_update<float>(nr, dx.as<float>(), B, Va.as<double>(), Vm.as<double>(),
+                                        V.as<double2>(), s);
+        } else {
+            solve_into<double>(B, F.as<double>(), dx.as<double>(), ddx.as<double>(), false);
+            kern::voltage_update<double>(nr, dx.as<double>(), B, Va.as<double>(),
+                                         Vm.as<double>(), V.as<double2>(), s);
+        }
+    }
+
+    double read_batch_max(int B) {
+        kern::batch_max(normF.as<double>(), B, bmax.as<double>(), s);
+        double h = 0.0;
+        CUPF_HIP(hipMemcpyAsync(&h, bmax.get(), sizeof(double), hipMemcpyDeviceToHost, s));
+        CUPF_HIP(hipStreamSynchronize(s));
+        return h;
+    }
+
+    int run_nr(int B, const NRConfig& cfg) {
+        int completed = 0;
+        for (int it = 0; it < cfg.max_iter; ++it) {
+            completed = it + 1;
+            kern::residual(nr, V.as<double2>(), S.as<double2>(), B, F.as<double>(),
+                           normF.as<double>(), s);
+            const double m = read_batch_max(B);
+            if (!std::isfinite(m))
+                throw std::runtime_error("NewtonSolver: mismatch norm is not finite");
+            if (m <= cfg.tolerance) break;
+            kern::jacobian_impl<double>(nr, V.as<double2>(), B, J.as<double>(),
+                                        d_zero.as<int>(), int(zero_slots.size()), s);
+            newton_step(B);
+        }
+        return completed;
+    }
+
+    void prepare_cache(int B) {
+        kern::jacobian_impl<double>(nr, V.as<double2>(), B, J.as<double>(), d_zero.as<int>(),
+                                    int(zero_slots.size()), s);
+        factor(B);
+        cache = true;
+        cache_B = B;
+    }
+
+    void check_ybus(const YbusView& y) const {
+        if (y.rows != n || y.cols != n || y.nnz != nnzY)
+            throw std::runtime_error("NewtonSolver: Ybus dimensions do not match initialize()");
+        if (!y.indptr || !y.indices || !y.data)
+            throw std::invalid_argument("NewtonSolver: Ybus arrays must not be null");
+        if (!std::equal(yp.begin(), yp.end(), y.indptr) ||
+            !std::equal(yi.begin(), yi.end(), y.indices))
+            throw std::runtime_error(
+                "NewtonSolver: Ybus sparsity pattern changed since initialize()");
+    }
+
+    void check_pvpq(const int32_t* pv_, int32_t n_pv, const int32_t* pq_, int32_t n_pq) const {
+        if (n_pv != int(pv.size()) || n_pq != int(pq.size()) ||
+            (n_pv > 0 && (!pv_ || !std::equal(pv.begin(), pv.end(), pv_))) ||
+            (n_pq > 0 && (!pq_ || !std::equal(pq.begin(), pq.end(), pq_))))
+            throw std::invalid_argument("NewtonSolver: pv/pq must match initialize()");
+    }
+
+    void upload_ybus_values(const YbusView& y) {
+        CUPF_HIP(hipMemcpyAsync(d_y.get(), y.data, size_t(nnzY) * sizeof(double2),
+                                hipMemcpyHostToDevice, s));
+    }
+};
+
+// ===========================================================================
+
+NewtonSolver::NewtonSolver(const NewtonOptions& options) : impl_(new Impl) {
+    if (options.use_cuda_graph && options.backend != BackendKind::CUDA)
+        throw std::invalid_argument("NewtonSolver: use_cuda_graph=true requires backend=CUDA");
+    if (options.backend == BackendKind::CPU)
+        throw std::invalid_argument(
+            "NewtonSolver: this build has only the GPU (HIP) backend; BackendKind.CPU is the "
+            "reference KLU path and is not provided");
+    if (options.cuda_linear_solver != CudaLinearSolverKind::Custom)
+        throw std::invalid_argument(
+            "NewtonSolver: cuDSS is not available on ROCm; use CudaLinearSolverKind.Custom");
+    if (options.custom.matching != CustomMatchingMode::None)
+        throw std::invalid_argument("NewtonSolver: structural matching is not supported");
+    if (options.custom.max_panel_width < 1)
+        throw std::invalid_argument("NewtonSolver: custom.max_panel_width must be >= 1");
+    if (options.custom.refine_steps < 0)
+        throw std::invalid_argument("NewtonSolver: custom.refine_steps must be >= 0");
+    Impl& m = *impl_;
+    m.opt = options;
+    switch (options.compute) {
+        case ComputePolicy::FP64:
+            m.front64 = true; m.dx32 = false; m.refine = 0;
+            break;
+        case ComputePolicy::Mixed:
+            m.front64 = options.custom.precision == CustomPrecision::FP64;
+            m.dx32 = false;
+            m.refine = m.front64 ? 0 : options.custom.refine_steps;
+            break;
+        case ComputePolicy::FP32:
+            m.front64 = false; m.dx32 = true; m.refine = 0;
+            break;
+    }
+    m.static_piv = options.custom.pivot_strategy == CustomPivotStrategy::StaticDiagonalShift &&
+                   options.custom.shift_retry_epsilon > 0.0;
+    m.piv_eps = options.custom.shift_retry_epsilon;
+    CUPF_HIP(hipStreamCreate(&m.own));
+    m.s = m.own;
+}
+
+NewtonSolver::~NewtonSolver() {
+    if (impl_ && impl_->own) {
+        (void)hipStreamSynchronize(impl_->own);
+        (void)hipStreamDestroy(impl_->own);
+    }
+}
+
+const NewtonSolverStats& NewtonSolver::stats() const { return impl_->st; }
+
+void NewtonSolver::initialize(const YbusView& ybus, const int32_t* pv, int32_t n_pv,
+                              const int32_t* pq, int32_t n_pq) {
+    Impl& m = *impl_;
+    if (ybus.rows <= 0 || ybus.cols != ybus.rows)
+        throw std::invalid_argument("NewtonSolver::initialize(): Ybus must be square");
+    if (!ybus.indptr || (ybus.nnz > 0 && !ybus.indices))
+        throw std::invalid_argument("NewtonSolver::initialize(): Ybus indptr/indices must not be null");
+    if (ybus.indptr[0] != 0 || ybus.indptr[ybus.rows] != ybus.nnz)
+        throw std::invalid_argument("NewtonSolver::initialize(): Ybus indptr is inconsistent with nnz");
+    for (int r = 0; r < ybus.rows; ++r)
+        if (ybus.indptr[r + 1] < ybus.indptr[r])
+            throw std::invalid_argument("NewtonSolver::initialize(): Ybus indptr must be non-decreasing");
+    m.initialized = false;
+    m.cache = false;
+    m.n = ybus.rows;
+    m.nnzY = ybus.nnz;
+    m.yp.assign(ybus.indptr, ybus.indptr + m.n + 1);
+    m.yi.assign(ybus.indices, ybus.indices + m.nnzY);
+    m.js = analyze_jacobian(m.n, m.yp.data(), m.yi.data(), m.nnzY, pv, n_pv, pq, n_pq);
+    m.pv.assign(pv, pv + n_pv);
+    m.pq.assign(pq, pq + n_pq);
+
+    // J slots reached by no Ybus entry (only for a structurally unsymmetric Ybus)
+    {
+        std::vector<char> hit(m.js.nnz, 0);
+        for (auto* mp : {&m.js.map11, &m.js.map21, &m.js.map12, &m.js.map22})
+            for (int v : *mp)
+                if (v >= 0) hit[v] = 1;
+        m.zero_slots.clear();
+        for (int k = 0; k < m.js.nnz; ++k)
+            if (!hit[k]) m.zero_slots.push_back(k);
+    }
+
+    const std::vector<int> perm = order_jacobian(m.js);
+    m.sym = analyze_symbolic(m.js.dim, m.js.row_ptr.data(), m.js.col_idx.data(), perm,
+                             m.opt.custom.max_panel_width);
+    m.plan = lower_plan(m.sym, m.js.dim, m.js.row_ptr.data(), m.js.col_idx.data());
+
+    // update-vector offsets (solve)
+    m.uoff.assign(m.plan.num_fronts + 1, 0);
+    for (int p = 0; p < m.plan.num_fronts; ++p)
+        m.uoff[p + 1] = m.uoff[p] + (m.plan.fsz[p] - m.plan.nc[p]);
+
+    // J^T pattern with value map (refinement residual of transposed solves)
+    {
+        const int d = m.js.dim;
+        m.jt_rp.assign(d + 1, 0);
+        for (int k = 0; k < m.js.nnz; ++k) ++m.jt_rp[m.js.col_idx[k] + 1];
+        for (int c = 0; c < d; ++c) m.jt_rp[c + 1] += m.jt_rp[c];
+        m.jt_ci.assign(m.js.nnz, 0);
+        m.jt_map.assign(m.js.nnz, 0);
+        std::vector<int> nx(m.jt_rp.begin(), m.jt_rp.end() - 1);
+        for (int r = 0; r < d; ++r)
+            for (int k = m.js.row_ptr[r]; k < m.js.row_ptr[r + 1]; ++k) {
+                const int pos = nx[m.js.col_idx[k]]++;
+                m.jt_ci[pos] = r;
+                m.jt_map[pos] = k;
+            }
+    }
+
+    hipStream_t s = m.own;
+    m.s = s;
+    m.d_yp = upload_vector(m.yp, s);
+    m.d_yi = upload_vector(m.yi, s);
+    m.d_y.allocate(size_t(m.nnzY) * sizeof(double2));
+    if (ybus.data)
+        CUPF_HIP(hipMemcpyAsync(m.d_y.get(), ybus.data, size_t(m.nnzY) * sizeof(double2),
+                                hipMemcpyHostToDevice, s));
+    m.d_row_pvpq = upload_vector(m.js.row_pvpq, s);
+    m.d_row_pq = upload_vector(m.js.row_pq, s);
+    m.d_m11 = upload_vector(m.js.map11, s);
+    m.d_m21 = upload_vector(m.js.map21, s);
+    m.d_m12 = upload_vector(m.js.map12, s);
+    m.d_m22 = upload_vector(m.js.map22, s);
+    m.d_zero = upload_vector(m.zero_slots, s);
+    m.d_row_bus = upload_vector(m.js.row_bus, s);
+    m.d_row_isq = upload_vector(m.js.row_is_q, s);
+    m.d_jrp = upload_vector(m.js.row_ptr, s);
+    m.d_jci = upload_vector(m.js.col_idx, s);
+    m.d_jtrp = upload_vector(m.jt_rp, s);
+    m.d_jtci = upload_vector(m.jt_ci, s);
+    m.d_jtmap = upload_vector(m.jt_map, s);
+
+    const LuPlan& P = m.plan;
+    m.d_fsz = upload_vector(P.fsz, s);
+    m.d_nc = upload_vector(P.nc, s);
+    m.d_off = upload_vector(P.off, s);
+    m.d_uoff = upload_vector(m.uoff, s);
+    m.d_fptr = upload_vector(P.front_ptr, s);
+    m.d_frows = upload_vector(P.front_rows, s);
+    m.d_cptr = upload_vector(P.child_ptr, s);
+    m.d_cidx = upload_vector(P.child_idx, s);
+    m.d_aptr = upload_vector(P.asm_ptr, s);
+    m.d_aloc = upload_vector(P.asm_loc, s);
+    m.d_eptr = upload_vector(P.ent_ptr, s);
+    m.d_esrc = upload_vector(P.ent_src, s);
+    m.d_edst = upload_vector(P.ent_dst, s);
+    m.d_perm = upload_vector(P.perm, s);
+    CUPF_HIP(hipStreamSynchronize(s));
+
+    kern::NrDevice& nr = m.nr;
+    nr.n_bus = m.n; nr.n_pv = n_pv; nr.n_pq = n_pq; nr.dim = m.js.dim;
+    nr.nnzY = m.nnzY; nr.nnzJ = m.js.nnz;
+    nr.yp = m.d_yp.as<int>(); nr.yi = m.d_yi.as<int>(); nr.y = m.d_y.as<double2>();
+    nr.y_stride = 0;
+    nr.row_pvpq = m.d_row_pvpq.as<int>(); nr.row_pq = m.d_row_pq.as<int>();
+    nr.map11 = m.d_m11.as<int>(); nr.map21 = m.d_m21.as<int>();
+    nr.map12 = m.d_m12.as<int>(); nr.map22 = m.d_m22.as<int>();
+
+    kern::LuDevice& lu = m.lu;
+    lu.n = m.js.dim; lu.nnz = m.js.nnz; lu.P = P.num_fronts; lu.max_fsz = P.max_fsz;
+    lu.arena = P.arena; lu.upd_total = m.uoff[P.num_fronts];
+    lu.fsz = m.d_fsz.as<int>(); lu.nc = m.d_nc.as<int>(); lu.off = m.d_off.as<long long>();
+    lu.uoff = m.d_uoff.as<long long>(); lu.front_ptr = m.d_fptr.as<int>();
+    lu.front_rows = m.d_frows.as<int>(); lu.child_ptr = m.d_cptr.as<int>();
+    lu.child_idx = m.d_cidx.as<int>(); lu.asm_ptr = m.d_aptr.as<int>();
+    lu.asm_loc = m.d_aloc.as<int>(); lu.ent_ptr = m.d_eptr.as<int>();
+    lu.ent_src = m.d_esrc.as<int>(); lu.ent_dst = m.d_edst.as<int>(); lu.perm = m.d_perm.as<int>();
+
+    m.capB = 0;   // batch buffers sized lazily for the new structure
+    m.st = NewtonSolverStats{};
+    m.st.dim = m.js.dim; m.st.nnz_j = m.js.nnz; m.st.num_fronts = P.num_fronts;
+    m.st.max_front = P.max_fsz; m.st.max_pivots = P.max_nc; m.st.arena_elems = P.arena;
+    m.st.lnz = P.lnz; m.st.factor_flops = P.flops;
+    m.initialized = true;
+}
+
+void NewtonSolver::solve(const YbusView& ybus, const std::complex<double>* sbus,
+                         const std::complex<double>* V0, const int32_t* pv, int32_t n_pv,
+                         const int32_t* pq, int32_t n_pq, const NRConfig& config,
+                         const SolveOptions& solve_options, NRResult& result) {
+    NRBatchResult br;
+    solve_batch(ybus, sbus, ybus.rows, V0, ybus.rows, 1, pv, n_pv, pq, n_pq, config,
+                solve_options, br);
+    result = {};
+    result.V = std::move(br.V);
+    result.iterations = br.iterations.empty() ? 0 : br.iterations[0];
+    result.final_mismatch = br.final_mismatch.empty() ? 0.0 : br.final_mismatch[0];
+    result.converged = !br.converged.empty() && br.converged[0] != 0;
+}
+
+void NewtonSolver::solve_batch(const YbusView& ybus, const std::complex<double>* sbus,
+                               int64_t sbus_stride, const std::complex<double>* V0,
+                               int64_t V0_stride, int32_t batch_size, const int32_t* pv,
+                               int32_t n_pv, const int32_t* pq, int32_t n_pq,
+                               const NRConfig& config, const SolveOptions& solve_options,
+                               NRBatchResult& result) {
+    Impl& m = *impl_;
+    if (!m.initialized)
+        throw std::runtime_error("NewtonSolver::solve_batch(): initialize() must be called first");
+    validate_batch_args(batch_size, sbus_stride, ybus.rows, "sbus");
+    validate_batch_args(batch_size, V0_stride, ybus.rows, "V0");
+    if (!sbus || !V0) throw std::invalid_argument("NewtonSolver::solve_batch(): null input");
+    if (config.max_iter < 0) throw std::invalid_argument("NRConfig.max_iter must be >= 0");
+    m.check_ybus(ybus);
+    m.check_pvpq(pv, n_pv, pq, n_pq);
+    const auto t0 = std::chrono::steady_clock::now();
+    m.s = m.own;
+    m.cache = false;
+    const int B = batch_size;
+    m.ensure_batch(B);
+    m.upload_ybus_values(ybus);
+    const size_t row = size_t(m.n) * sizeof(double2);
+    CUPF_HIP(hipMemcpy2DAsync(m.S.get(), row, sbus, size_t(sbus_stride) * sizeof(double2), row, B,
+                              hipMemcpyHostToDevice, m.s));
+    CUPF_HIP(hipMemcpy2DAsync(m.io_a.get(), row, V0, size_t(V0_stride) * sizeof(double2), row, B,
+                              hipMemcpyHostToDevice, m.s));
+    kern::seed_state(m.nr, m.io_a.as<double2>(), m.n, B, m.V.as<double2>(), m.Va.as<double>(),
+                     m.Vm.as<double>(), m.s);
+    CUPF_HIP(hipMemsetAsync(m.sing.get(), 0, size_t(B) * sizeof(int), m.s));
+
+    const int iters = m.run_nr(B, config);
+    m.last_B = B;
+    if (solve_options.prepare_adjoint_cache &&
+        solve_options.adjoint_cache_mode != AdjointCacheMode::None)
+        m.prepare_cache(B);
+
+    result = {};
+    result.n_bus = m.n;
+    result.batch_size = B;
+    result.V.resize(size_t(B) * m.n);
+    CUPF_HIP(hipMemcpyAsync(result.V.data(), m.V.get(), size_t(B) * row, hipMemcpyDeviceToHost,
+                            m.s));
+    result.final_mismatch.resize(B);
+    CUPF_HIP(hipMemcpyAsync(result.final_mismatch.data(), m.normF.get(), size_t(B) * sizeof(double),
+                            hipMemcpyDeviceToHost, m.s));
+    std::vector<int> sing(B);
+    CUPF_HIP(hipMemcpyAsync(sing.data(), m.sing.get(), size_t(B) * sizeof(int),
+                            hipMemcpyDeviceToHost, m.s));
+    CUPF_HIP(hipStreamSynchronize(m.s));
+    result.iterations.assign(B, iters);
+    result.converged.resize(B);
+    result.singular.resize(B);
+    for (int b = 0; b < B; ++b) {
+        result.converged[b] = uint8_t(result.final_mismatch[b] <= config.tolerance ? 1 : 0);
+        result.singular[b] = uint8_t(sing[b] != 0);
+    }
+    m.st.last_iterations = iters;
+    m.st.last_batch = B;
+    m.st.last_solve_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+void NewtonSolver::solve_adjoint(const double* grad_va, int64_t grad_va_stride,
+                                 const double* grad_vm, int64_t grad_vm_stride,
+                                 int32_t batch_size, const int32_t* pv, int32_t n_pv,
+                                 const int32_t* pq, int32_t n_pq, const AdjointOptions& options,
+                                 AdjointResult& result) {
+    Impl& m = *impl_;
+    if (!m.initialized || m.last_B == 0)
+        throw std::runtime_error(
+            "NewtonSolver::solve_adjoint(): initialize() and solve()/solve_batch() must be called first");
+    if (options.reuse_forward_factorization && !options.allow_refactorize)
+        throw std::runtime_error(
+            "NewtonSolver::solve_adjoint(): forward factorization reuse is not available for the "
+            "final converged state; enable allow_refactorize");
+    if (batch_size <= 0)
+        throw std::invalid_argument("NewtonSolver::solve_adjoint(): batch_size must be positive");
+    if (batch_size != m.last_B)
+        throw std::invalid_argument(
+            "NewtonSolver::solve_adjoint(): batch_size does not match the last forward solve");
+    if (!grad_va || !grad_vm)
+        throw std::invalid_argument("NewtonSolver::solve_adjoint(): grad_va/grad_vm must not be null");
+    if (grad_va_stride < m.n || grad_vm_stride < m.n)
+        throw std::invalid_argument("NewtonSolver::solve_adjoint(): grad strides must be at least n_bus");
+    m.check_pvpq(pv, n_pv, pq, n_pq);
+    const auto t0 = std::chrono::steady_clock::now();
+    m.s = m.own;
+    const int B = batch_size;
+    result = {};
+    result.backend = m.front64 ? "hip_multifrontal_fp64" : "hip_multifrontal_fp32_refined";
+    result.transpose_solve_backend = "hip_multifrontal_transpose_solve";
+    const bool cache_ok = m.cache && m.cache_B == B;
+    result.used_adjoint_cache = cache_ok;
+    result.adjoint_cache_matches_final_state = cache_ok;
+    if (!cache_ok) {
+        if (options.require_cached_factorization ||
+            !(options.allow_refactorize && options.allow_refactorize_for_backward))
+            throw std::runtime_error(
+                "NewtonSolver::solve_adjoint(): missing exact final-state adjoint cache");
+        m.prepare_cache(B);
+        result.refactorized_for_backward = true;
+        result.jt_refactorized_during_backward = true;
+    } else {
+        result.reused_final_state_factorization = true;
+        result.jt_factorized_during_forward_cache = true;
+    }
+    const size_t row = size_t(m.n) * sizeof(double);
+    CUPF_HIP(hipMemcpy2DAsync(m.io_a.get(), row, grad_va, size_t(grad_va_stride) * sizeof(double),
+                              row, B, hipMemcpyHostToDevice, m.s));
+    CUPF_HIP(hipMemcpy2DAsync(m.io_b.get(), row, grad_vm, size_t(grad_vm_stride) * sizeof(double),
+                              row, B, hipMemcpyHostToDevice, m.s));
+    kern::adjoint_gather<double>(m.nr, m.d_row_bus.as<int>(), m.d_row_isq.as<unsigned char>(),
+                                 m.io_a.as<double>(), m.n, m.io_b.as<double>(), m.n, B,
+                                 m.g.as<double>(), m.s);
+    m.solve_into<double>(B, m.g.as<double>(), m.lam.as<double>(), m.ddx.as<double>(), true);
+    result.n_bus = m.n;
+    result.batch_size = B;
+    result.dimF = m.js.dim;
+    result.lambda.resize(size_t(B) * m.js.dim);
+    CUPF_HIP(hipMemcpyAsync(result.lambda.data(), m.lam.get(), result.lambda.size() * sizeof(double),
+                            hipMemcpyDeviceToHost, m.s));
+    if (options.compute_load_gradients) {
+        kern::adjoint_project<double>(m.nr, m.lam.as<double>(), B, m.io_a.as<double>(),
+                                      m.io_b.as<double>(), m.s);
+        result.grad_load_p.resize(size_t(B) * m.n);
+        result.grad_load_q.resize(size_t(B) * m.n);
+        CUPF_HIP(hipMemcpyAsync(result.grad_load_p.data(), m.io_a.get(), size_t(B) * row,
+                                hipMemcpyDeviceToHost, m.s));
+        CUPF_HIP(hipMemcpyAsync(result.grad_load_q.data(), m.io_b.get(), size_t(B) * row,
+                                hipMemcpyDeviceToHost, m.s));
+    }
+    if (options.check_residual) {
+        kern::refine_residual<double>(m.d_jtrp.as<int>(), m.d_jtci.as<int>(), m.d_jtmap.as<int>(),
+                                      m.J.as<double>(), m.js.dim, m.js.nnz, m.g.as<double>(),
+                                      m.lam.as<double>(), B, m.res.as<double>(), m.s);
+        std::vector<double> r(size_t(B) * m.js.dim), gh(size_t(B) * m.js.dim);
+        CUPF_HIP(hipMemcpyAsync(r.data(), m.res.get(), r.size() * sizeof(double),
+                                hipMemcpyDeviceToHost, m.s));
+        CUPF_HIP(hipMemcpyAsync(gh.data(), m.g.get(), gh.size() * sizeof(double),
+                                hipMemcpyDeviceToHost, m.s));
+        CUPF_HIP(hipStreamSynchronize(m.s));
+        long double rr = 0, gg = 0;
+        for (size_t i = 0; i < r.size(); ++i) {
+            rr += (long double)r[i] * r[i];
+            gg += (long double)gh[i] * gh[i];
+        }
+        const long double den = std::max(std::sqrt(std::max(gg, (long double)1e-60)), (long double)1e-30);
+        result.jt_residual_norm = double(std::sqrt(rr) / den);
+    }
+    CUPF_HIP(hipStreamSynchronize(m.s));
+    result.success = true;
+    result.total_time_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    result.solve_time_ms = result.total_time_ms;
+    result.transpose_solve_time_ms = result.total_time_ms;
+}
+
+// ---------------------------------------------------------------------------
+// Device-pointer (torch) entry points.
+// ---------------------------------------------------------------------------
+namespace {
+bool dtype_is(const char* dtype, const char* want) { return dtype && std::strcmp(dtype, want) == 0; }
+}  // namespace
+
+void NewtonSolver::solve_torch_forward(const void* sbus_base_re, const void* sbus_base_im,
+                                       const void* load_p, const void* load_q, const void* v0_va,
+                                       const void* v0_vm, void* va_out, void* vm_out,
+                                       int32_t batch_size, int32_t n_bus, const char* dtype,
+                                       const NRConfig& config, const SolveOptions& solve_options,
+                                       void* stream, AdjointResult& result) {
+    Impl& m = *impl_;
+    if (!m.initialized)
+        throw std::runtime_error("NewtonSolver::solve_torch_forward(): initialize() must be called first");
+    if (batch_size <= 0 || n_bus != m.n)
+        throw std::invalid_argument("NewtonSolver::solve_torch_forward(): bad batch/n_bus");
+    const bool want64 = m.opt.compute == ComputePolicy::FP64;
+    if (!dtype_is(dtype, want64 ? "float64" : "float32"))
+        throw std::invalid_argument(std::string("NewtonSolver: tensors must be ") +
+                                    (want64 ? "float64 for ComputePolicy.FP64"
+                                            : "float32 for ComputePolicy.FP32/Mixed"));
+    if (!sbus_base_re || !sbus_base_im || !load_p || !load_q || !v0_va || !v0_vm || !va_out || !vm_out)
+        throw std::invalid_argument("NewtonSolver::solve_torch_forward(): null tensor pointer");
+    m.s = stream ? static_cast<hipStream_t>(stream) : m.own;
+    m.cache = false;
+    const int B = batch_size;
+    m.ensure_batch(B);
+    if (want64)
+        kern::torch_inputs<double>(m.n, B, (const double*)sbus_base_re, (const double*)sbus_base_im,
+                                   (const double*)load_p, (const double*)load_q,
+                                   (const double*)v0_va, (const double*)v0_vm, m.S.as<double2>(),
+                                   m.V.as<double2>(), m.Va.as<double>(), m.Vm.as<double>(), m.s);
+    else
+        kern::torch_inputs<float>(m.n, B, (const float*)sbus_base_re, (const float*)sbus_base_im,
+                                  (const float*)load_p, (const float*)load_q, (const float*)v0_va,
+                                  (const float*)v0_vm, m.S.as<double2>(), m.V.as<double2>(),
+                                  m.Va.as<double>(), m.Vm.as<double>(), m.s);
+    CUPF_HIP(hipMemsetAsync(m.sing.get(), 0, size_t(B) * sizeof(int), m.s));
+    const int iters = m.run_nr(B, config);
+    m.last_B = B;
+    if (solve_options.prepare_adjoint_cache &&
+        solve_options.adjoint_cache_mode != AdjointCacheMode::None)
+        m.prepare_cache(B);
+    if (want64)
+        kern::torch_outputs<double>(m.n, B, m.Va.as<double>(), m.Vm.as<double>(), (double*)va_out,
+                                    (double*)vm_out, m.s);
+    else
+        kern::torch_outputs<float>(m.n, B, m.Va.as<double>(), m.Vm.as<double>(), (float*)va_out,
+                                   (float*)vm_out, m.s);
+    result = {};
+    result.success = true;
+    result.n_bus = m.n;
+    result.batch_size = B;
+    result.dimF = m.js.dim;
+    result.zero_copy = true;
+    result.torch_extension_zero_copy = true;
+    result.raw_pointer_api_used = true;
+    result.current_stream_integrated = stream != nullptr;
+    result.used_adjoint_cache = m.cache;
+    result.jt_factorized_during_forward_cache = m.cache;
+    result.backend = m.front64 ? "hip_multifrontal_fp64" : "hip_multifrontal_fp32_refined";
+    m.st.last_iterations = iters;
+    m.st.last_batch = B;
+    m.s = m.own;
+}
+
+void NewtonSolver::solve_torch_backward(const void* grad_va, const void* grad_vm,
+                                        void* grad_load_p, void* grad_load_q, int32_t batch_size,
+                                        int32_t n_bus, const char* dtype,
+                                        const AdjointOptions& options, void* stream,
+                                        AdjointResult& result) {
+    Impl& m = *impl_;
+    if (!m.initialized || m.last_B == 0)
+        throw std::runtime_error("NewtonSolver::solve_torch_backward(): no forward solve to differentiate");
+    if (batch_size != m.last_B || n_bus != m.n)
+        throw std::invalid_argument(
+            "NewtonSolver::solve_torch_backward(): batch_size does not match the last forward solve");
+    const bool want64 = m.opt.compute == ComputePolicy::FP64;
+    if (!dtype_is(dtype, want64 ? "float64" : "float32"))
+        throw std::invalid_argument("NewtonSolver::solve_torch_backward(): dtype does not match the forward");
+    if (!grad_va || !grad_vm || !grad_load_p || !grad_load_q)
+        throw std::invalid_argument("NewtonSolver::solve_torch_backward(): null tensor pointer");
+    m.s = stream ? static_cast<hipStream_t>(stream) : m.own;
+    const int B = batch_size;
+    const bool cache_ok = m.cache && m.cache_B == B;
+    if (!cache_ok) {
+        if (options.require_cached_factorization ||
+            !(options.allow_refactorize && options.allow_refactorize_for_backward))
+            throw std::runtime_error(
+                "NewtonSolver::solve_torch_backward(): missing exact final-state adjoint cache");
+        m.prepare_cache(B);
+    }
+    if (want64)
+        kern::adjoint_gather<double>(m.nr, m.d_row_bus.as<int>(), m.d_row_isq.as<unsigned char>(),
+                                     (const double*)grad_va, m.n, (const double*)grad_vm, m.n, B,
+                                     m.g.as<double>(), m.s);
+    else
+        kern::adjoint_gather<float>(m.nr, m.d_row_bus.as<int>(), m.d_row_isq.as<unsigned char>(),
+                                    (const float*)grad_va, m.n, (const float*)grad_vm, m.n, B,
+                                    m.g.as<double>(), m.s);
+    m.solve_into<double>(B, m.g.as<double>(), m.lam.as<double>(), m.ddx.as<double>(), true);
+    if (want64)
+        kern::adjoint_project<double>(m.nr, m.lam.as<double>(), B, (double*)grad_load_p,
+                                      (double*)grad_load_q, m.s);
+    else
+        kern::adjoint_project<float>(m.nr, m.lam.as<double>(), B, (float*)grad_load_p,
+                                     (float*)grad_load_q, m.s);
+    result = {};
+    result.success = true;
+    result.n_bus = m.n;
+    result.batch_size = B;
+    result.dimF = m.js.dim;
+    result.used_adjoint_cache = cache_ok;
+    result.adjoint_cache_matches_final_state = true;
+    result.reused_final_state_factorization = cache_ok;
+    result.refactorized_for_backward = !cache_ok;
+    result.zero_copy = true;
+    result.torch_extension_zero_copy = true;
+    result.raw_pointer_api_used = true;
+    result.current_stream_integrated = stream != nullptr;
+    result.backend = m.front64 ? "hip_multifrontal_fp64" : "hip_multifrontal_fp32_refined";
+    result.transpose_solve_backend = "hip_multifrontal_transpose_solve";
+    m.s = m.own;
+}
+
+// ---------------------------------------------------------------------------
+// Diagnostics.
+// ---------------------------------------------------------------------------
+void NewtonSolver::debug_residual(const std::complex<double>* sbus, const std::complex<double>* V,
+                                  int32_t batch, double* F, double* norm) {
+    Impl& m = *impl_;
+    if (!m.initialized) throw std::runtime_error("debug_residual: not initialized");
+    m.s = m.own;
+    m.ensure_batch(batch);
+    const size_t row = size_t(m.n) * sizeof(double2);
+    CUPF_HIP(hipMemcpyAsync(m.S.get(), sbus, batch * row, hipMemcpyHostToDevice, m.s));
+    CUPF_HIP(hipMemcpyAsync(m.V.get(), V, batch * row, hipMemcpyHostToDevice, m.s));
+    kern::residual(m.nr, m.V.as<double2>(), m.S.as<double2>(), batch, m.F.as<double>(),
+                   m.normF.as<double>(), m.s);
+    CUPF_HIP(hipMemcpyAsync(F, m.F.get(), size_t(batch) * m.js.dim * sizeof(double),
+                            hipMemcpyDeviceToHost, m.s));
+    CUPF_HIP(hipMemcpyAsync(norm, m.normF.get(), size_t(batch) * sizeof(double),
+                            hipMemcpyDeviceToHost, m.s));
+    CUPF_HIP(hipStreamSynchronize(m.s));
+}
+
+void NewtonSolver::debug_jacobian(const std::complex<double>* V, int32_t batch, double* Jout) {
+    Impl& m = *impl_;
+    if (!m.initialized) throw std::runtime_error("debug_jacobian: not initialized");
+    m.s = m.own;
+    m.ensure_batch(batch);
+    const size_t row = size_t(m.n) * sizeof(double2);
+    CUPF_HIP(hipMemcpyAsync(m.V.get(), V, batch * row, hipMemcpyHostToDevice, m.s));
+    kern::jacobian_impl<double>(m.nr, m.V.as<double2>(), batch, m.J.as<double>(),
+                                m.d_zero.as<int>(), int(m.zero_slots.size()), m.s);
+    CUPF_HIP(hipMemcpyAsync(Jout, m.J.get(), size_t(batch) * m.js.nnz * sizeof(double),
+                            hipMemcpyDeviceToHost, m.s));
+    CUPF_HIP(hipStreamSynchronize(m.s));
+}
+
+void NewtonSolver::debug_factor_solve(const double* A, const double* rhs, int32_t batch,
+                                      bool transpose, double* x, uint8_t* singular) {
+    Impl& m = *impl_;
+    if (!m.initialized) throw std::runtime_error("debug_factor_solve: not initialized");
+    m.s = m.own;
+    m.ensure_batch(batch);
+    CUPF_HIP(hipMemcpyAsync(m.J.get(), A, size_t(batch) * m.js.nnz * sizeof(double),
+                            hipMemcpyHostToDevice, m.s));
+    CUPF_HIP(hipMemcpyAsync(m.g.get(), rhs, size_t(batch) * m.js.dim * sizeof(double),
+                            hipMemcpyHostToDevice, m.s));
+    CUPF_HIP(hipMemsetAsync(m.sing.get(), 0, size_t(batch) * sizeof(int), m.s));
+    m.factor(batch);
+    if (m.dx32) {
+        m.solve_into<float>(batch, m.g.as<double>(), m.dx.as<float>(), m.ddx.as<float>(), transpose);
+        std::vector<float> xf(size_t(batch) * m.js.dim);
+        CUPF_HIP(hipMemcpyAsync(xf.data(), m.dx.get(), xf.size() * sizeof(float),
+                                hipMemcpyDeviceToHost, m.s));
+        CUPF_HIP(hipStreamSynchronize(m.s));
+        for (size_t i = 0; i < xf.size(); ++i) x[i] = xf[i];
+    } else {
+        m.solve_into<double>(batch, m.g.as<double>(), m.lam.as<double>(), m.ddx.as<double>(),
+                             transpose);
+        CUPF_HIP(hipMemcpyAsync(x, m.lam.get(), size_t(batch) * m.js.dim * sizeof(double),
+                                hipMemcpyDeviceToHost, m.s));
+    }
+    std::vector<int> sg(batch);
+    CUPF_HIP(hipMemcpyAsync(sg.data(), m.sing.get(), size_t(batch) * sizeof(int),
+                            hipMemcpyDeviceToHost, m.s));
+    CUPF_HIP(hipStreamSynchronize(m.s));
+    for (int b = 0; b < batch; ++b) singular[b] = uint8_t(sg[b] != 0);
+}
+
+void NewtonSolver::debug_pattern(int32_t* row_ptr, int32_t* col_idx, int32_t* perm) const {
+    const Impl& m = *impl_;
+    if (!m.initialized) throw std::runtime_error("debug_pattern: not initialized");
+    std::copy(m.js.row_ptr.begin(), m.js.row_ptr.end(), row_ptr);
+    std::copy(m.js.col_idx.begin(), m.js.col_idx.end(), col_idx);
+    std::copy(m.plan.perm.begin(), m.plan.perm.end(), perm);
+}

@@ -1,0 +1,79 @@
+// NewtonSolver: the batched Newton-Raphson power-flow driver.  The public
+// methods and their argument meaning mirror cuPF/cpp/inc/newton_solver/core/
+// newton_solver.hpp:26-102 (the torch entry points take raw device pointers
+// and a stream instead of relying on a thread-local current stream).
+#pragma once
+
+#include "newton_solver_types.hpp"
+
+#include <cstdint>
+#include <memory>
+
+struct NewtonSolverStats {
+    int32_t last_iterations = 0;
+    int32_t last_batch = 0;
+    int32_t factorizations = 0;
+    double last_solve_ms = 0.0;
+    // symbolic / plan figures
+    int32_t dim = 0, nnz_j = 0, num_fronts = 0, max_front = 0, max_pivots = 0;
+    int64_t arena_elems = 0, lnz = 0;
+    double factor_flops = 0.0;
+};
+
+class NewtonSolver {
+public:
+    explicit NewtonSolver(const NewtonOptions& options = {});
+    ~NewtonSolver();
+    NewtonSolver(const NewtonSolver&) = delete;
+    NewtonSolver& operator=(const NewtonSolver&) = delete;
+
+    void initialize(const YbusView& ybus, const int32_t* pv, int32_t n_pv, const int32_t* pq,
+                    int32_t n_pq);
+
+    void solve(const YbusView& ybus, const std::complex<double>* sbus,
+               const std::complex<double>* V0, const int32_t* pv, int32_t n_pv,
+               const int32_t* pq, int32_t n_pq, const NRConfig& config,
+               const SolveOptions& solve_options, NRResult& result);
+
+    void solve_batch(const YbusView& ybus, const std::complex<double>* sbus, int64_t sbus_stride,
+                     const std::complex<double>* V0, int64_t V0_stride, int32_t batch_size,
+                     const int32_t* pv, int32_t n_pv, const int32_t* pq, int32_t n_pq,
+                     const NRConfig& config, const SolveOptions& solve_options,
+                     NRBatchResult& result);
+
+    void solve_adjoint(const double* grad_va, int64_t grad_va_stride, const double* grad_vm,
+                       int64_t grad_vm_stride, int32_t batch_size, const int32_t* pv,
+                       int32_t n_pv, const int32_t* pq, int32_t n_pq,
+                       const AdjointOptions& options, AdjointResult& result);
+
+    // Device-pointer entry points (cuPF torch_bridge.cpp:209-482).  dtype is
+    // "float64" or "float32"; `stream` is a hipStream_t (nullptr = default).
+    void solve_torch_forward(const void* sbus_base_re, const void* sbus_base_im,
+                             const void* load_p, const void* load_q, const void* v0_va,
+                             const void* v0_vm, void* va_out, void* vm_out, int32_t batch_size,
+                             int32_t n_bus, const char* dtype, const NRConfig& config,
+                             const SolveOptions& solve_options, void* stream,
+                             AdjointResult& result);
+    void solve_torch_backward(const void* grad_va, const void* grad_vm, void* grad_load_p,
+                              void* grad_load_q, int32_t batch_size, int32_t n_bus,
+                              const char* dtype, const AdjointOptions& options, void* stream,
+                              AdjointResult& result);
+
+    const NewtonSolverStats& stats() const;
+
+    // Diagnostics used by the parity tests: run single stages on host data.
+    // Residual F [B][dim] and norms [B] at voltages V [B][n].
+    void debug_residual(const std::complex<double>* sbus, const std::complex<double>* V,
+                        int32_t batch, double* F, double* norm);
+    // Jacobian values [B][nnzJ] (CSR order of the analysed pattern) at V.
+    void debug_jacobian(const std::complex<double>* V, int32_t batch, double* J);
+    // Factor A [B][nnzJ] (CSR values) and solve A x = rhs (or A^T x = rhs).
+    void debug_factor_solve(const double* A, const double* rhs, int32_t batch, bool transpose,
+                            double* x, uint8_t* singular);
+    // Jacobian pattern and fill-reducing order of the last initialize().
+    void debug_pattern(int32_t* row_ptr, int32_t* col_idx, int32_t* perm) const;
+
+private:
+    struct Impl;
+    std::unique_ptr<Impl> impl_;
+};

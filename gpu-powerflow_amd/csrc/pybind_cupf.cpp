@@ -1,0 +1,526 @@
+// _cupf: Python binding of the MI355X solver.  The module surface (enums,
+// option/result classes, NewtonSolver methods, argument names and defaults)
+// mirrors cuPF/bindings/pybind_cupf.cpp:162-656.  Every call goes through
+// the C ABI of libcupf_hip.so (include/cupf_hip.h); C status codes become the
+// reference's exception classes (ValueError for std::invalid_argument,
+// RuntimeError for std::runtime_error).
+#include <pybind11/complex.h>
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include "../../include/cupf_hip.h"
+#include "newton_solver_types.hpp"
+
+#include <complex>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+
+namespace py = pybind11;
+
+namespace {
+
+void check(int rc) {
+    if (rc == CUPF_OK) return;
+    const std::string msg = cupf_last_error();
+    if (rc == CUPF_INVALID_ARGUMENT) throw std::invalid_argument(msg);
+    throw std::runtime_error(msg);
+}
+
+cupf_nr_config c_cfg(const NRConfig& c) { return {c.tolerance, c.max_iter}; }
+
+cupf_solve_options c_sopt(const SolveOptions& s) {
+    return {s.prepare_adjoint_cache ? 1 : 0, int32_t(s.adjoint_cache_mode),
+            s.allow_explicit_transpose_fallback ? 1 : 0};
+}
+
+cupf_adjoint_options c_aopt(const AdjointOptions& a) {
+    return {a.reuse_forward_factorization, a.allow_refactorize, a.require_cached_factorization,
+            a.allow_refactorize_for_backward, a.allow_inexact_last_newton_factorization,
+            a.use_transpose_solve, a.allow_explicit_transpose_fallback, a.compute_load_gradients,
+            a.check_residual};
+}
+
+cupf_newton_options c_nopt(const NewtonOptions& o) {
+    cupf_newton_options c;
+    std::memset(&c, 0, sizeof(c));
+    c.backend = int32_t(o.backend);
+    c.compute = int32_t(o.compute);
+    c.cuda_linear_solver = int32_t(o.cuda_linear_solver);
+    c.cpu_linear_solver = int32_t(o.cpu_linear_solver);
+    c.cuda_jacobian = int32_t(o.cuda_jacobian);
+    c.cudss_use_matching = o.cudss.use_matching;
+    c.cudss_matching_alg = int32_t(o.cudss.matching_alg);
+    c.cudss_auto_pivot_epsilon = o.cudss.auto_pivot_epsilon;
+    c.cudss_pivot_epsilon = o.cudss.pivot_epsilon;
+    c.custom.precision = int32_t(o.custom.precision);
+    c.custom.matching = int32_t(o.custom.matching);
+    c.custom.pivot_strategy = int32_t(o.custom.pivot_strategy);
+    c.custom.serial_nd = o.custom.serial_nd;
+    c.custom.max_panel_width = o.custom.max_panel_width;
+    c.custom.shift_retry_epsilon = o.custom.shift_retry_epsilon;
+    c.custom.refine_steps = o.custom.refine_steps;
+    c.use_cuda_graph = o.use_cuda_graph;
+    return c;
+}
+
+void info_to_result(const cupf_adjoint_info& i, AdjointResult& r) {
+    r.n_bus = i.n_bus; r.batch_size = i.batch_size; r.dimF = i.dimF;
+    r.success = i.success; r.used_adjoint_cache = i.used_adjoint_cache;
+    r.adjoint_cache_matches_final_state = i.adjoint_cache_matches_final_state;
+    r.reused_forward_factorization = i.reused_forward_factorization;
+    r.reused_final_state_factorization = i.reused_final_state_factorization;
+    r.refactorized_for_backward = i.refactorized_for_backward;
+    r.used_explicit_transpose = i.used_explicit_transpose;
+    r.used_python_scipy = i.used_python_scipy;
+    r.includes_host_device_transfer = i.includes_host_device_transfer;
+    r.zero_copy = i.zero_copy; r.torch_extension_zero_copy = i.torch_extension_zero_copy;
+    r.raw_pointer_api_used = i.raw_pointer_api_used;
+    r.current_stream_integrated = i.current_stream_integrated;
+    r.jt_symbolic_analyzed_at_initialize = i.jt_symbolic_analyzed_at_initialize;
+    r.jt_values_transposed_on_device = i.jt_values_transposed_on_device;
+    r.jt_factorized_during_forward_cache = i.jt_factorized_during_forward_cache;
+    r.jt_refactorized_during_backward = i.jt_refactorized_during_backward;
+    r.host_roundtrip_for_jt_transpose = i.host_roundtrip_for_jt_transpose;
+    r.jt_residual_norm = i.jt_residual_norm; r.solve_time_ms = i.solve_time_ms;
+    r.transpose_solve_time_ms = i.transpose_solve_time_ms;
+    r.factorization_time_ms = i.factorization_time_ms; r.total_time_ms = i.total_time_ms;
+    r.backend = i.backend;
+    r.transpose_solve_backend = i.transpose_solve_backend;
+}
+
+using I32 = py::array_t<int32_t, py::array::c_style | py::array::forcecast>;
+using C128 = py::array_t<std::complex<double>, py::array::c_style | py::array::forcecast>;
+using F64 = py::array_t<double, py::array::c_style | py::array::forcecast>;
+
+// Validation of a numpy CSR Ybus (pybind_cupf.cpp:70-95).
+void check_ybus(const I32& indptr, const I32& indices, const C128& data, int32_t rows,
+                int32_t cols) {
+    if (rows <= 0 || cols <= 0) throw std::invalid_argument("Ybus rows/cols must be positive");
+    if (indptr.ndim() != 1 || indices.ndim() != 1 || data.ndim() != 1)
+        throw std::invalid_argument("Ybus indptr, indices, and data must be 1D arrays");
+    if (indptr.size() != py::ssize_t(rows + 1))
+        throw std::invalid_argument("Ybus indptr length must be rows + 1");
+    if (indices.size() != data.size())
+        throw std::invalid_argument("Ybus indices and data must have the same length");
+}
+
+const double* cplx(const C128& a) { return reinterpret_cast<const double*>(a.data()); }
+
+template <typename T>
+py::array_t<T> vec_to_numpy(const std::vector<T>& v) {
+    py::array_t<T> out(v.size());
+    if (!v.empty()) std::memcpy(out.mutable_data(), v.data(), v.size() * sizeof(T));
+    return out;
+}
+
+py::array_t<double> mat_to_numpy(const std::vector<double>& v, int32_t r, int32_t c) {
+    py::array_t<double> out({r, c});
+    if (!v.empty()) std::memcpy(out.mutable_data(), v.data(), v.size() * sizeof(double));
+    return out;
+}
+
+class PySolver {
+public:
+    explicit PySolver(const NewtonOptions& o) {
+        const cupf_newton_options c = c_nopt(o);
+        check(cupf_solver_create(&c, &h_));
+    }
+    ~PySolver() { if (h_) cupf_solver_destroy(h_); }
+    cupf_solver* h() const { return h_; }
+    int32_t n_bus = 0;
+    int32_t dim = 0;
+    int32_t last_batch = 0;
+
+private:
+    cupf_solver* h_ = nullptr;
+};
+
+NRBatchResult run_batch(PySolver& self, const I32& indptr, const I32& indices, const C128& data,
+                        int32_t rows, int32_t cols, const C128& sbus, const C128& v0,
+                        int32_t batch, const I32& pv, const I32& pq, const NRConfig& config,
+                        const SolveOptions& so) {
+    NRBatchResult r;
+    r.n_bus = rows;
+    r.batch_size = batch;
+    r.V.resize(size_t(batch) * rows);
+    r.iterations.resize(batch);
+    r.final_mismatch.resize(batch);
+    r.converged.resize(batch);
+    r.singular.resize(batch);
+    const cupf_nr_config c = c_cfg(config);
+    const cupf_solve_options s = c_sopt(so);
+    {
+        py::gil_scoped_release nogil;
+        check(cupf_solver_solve_batch(self.h(), rows, cols, indptr.data(), indices.data(),
+                                      cplx(data), int32_t(data.size()), cplx(sbus), rows,
+                                      cplx(v0), rows, batch, pv.data(), int32_t(pv.size()),
+                                      pq.data(), int32_t(pq.size()), &c, &s,
+                                      reinterpret_cast<double*>(r.V.data()), r.iterations.data(),
+                                      r.final_mismatch.data(), r.converged.data(),
+                                      r.singular.data()));
+    }
+    self.last_batch = batch;
+    return r;
+}
+
+}  // namespace
+
+PYBIND11_MODULE(_cupf, m) {
+    m.doc() = "cuPF for MI355X: batched GPU Newton-Raphson power flow (HIP/CDNA4)";
+
+    py::enum_<BackendKind>(m, "BackendKind", "Compute backend (CUDA names the GPU/HIP backend)")
+        .value("CPU", BackendKind::CPU)
+        .value("CUDA", BackendKind::CUDA)
+        .export_values();
+    py::enum_<ComputePolicy>(m, "ComputePolicy", "Internal precision policy")
+        .value("FP64", ComputePolicy::FP64)
+        .value("FP32", ComputePolicy::FP32)
+        .value("Mixed", ComputePolicy::Mixed)
+        .export_values();
+    py::enum_<CudaLinearSolverKind>(m, "CudaLinearSolverKind", "GPU linear solver")
+        .value("CuDSS", CudaLinearSolverKind::CuDSS)
+        .value("Custom", CudaLinearSolverKind::Custom)
+        .export_values();
+    py::enum_<CustomPrecision>(m, "CustomPrecision", "Front precision (TF32 = 3xBF16 MFMA split)")
+        .value("FP64", CustomPrecision::FP64)
+        .value("FP32", CustomPrecision::FP32)
+        .value("TF32", CustomPrecision::TF32)
+        .export_values();
+    py::enum_<CustomMatchingMode>(m, "CustomMatchingMode")
+        .value("None", CustomMatchingMode::None)
+        .value("Structural", CustomMatchingMode::Structural)
+        .export_values();
+    py::enum_<CustomPivotStrategy>(m, "CustomPivotStrategy")
+        .value("None", CustomPivotStrategy::None)
+        .value("StaticDiagonalShift", CustomPivotStrategy::StaticDiagonalShift)
+        .export_values();
+    py::enum_<CpuLinearSolverKind>(m, "CpuLinearSolverKind")
+        .value("KLU", CpuLinearSolverKind::KLU)
+        .value("UMFPACK", CpuLinearSolverKind::UMFPACK)
+        .export_values();
+    py::enum_<CudaJacobianKind>(m, "CudaJacobianKind")
+        .value("Edge", CudaJacobianKind::Edge)
+        .value("EdgeAtomic", CudaJacobianKind::EdgeAtomic)
+        .value("VertexWarp", CudaJacobianKind::VertexWarp)
+        .export_values();
+    py::enum_<AdjointCacheMode>(m, "AdjointCacheMode")
+        .value("None", AdjointCacheMode::None)
+        .value("FinalStateFactorization", AdjointCacheMode::FinalStateFactorization)
+        .value("ReuseLastNewtonFactorizationIfExact",
+               AdjointCacheMode::ReuseLastNewtonFactorizationIfExact)
+        .export_values();
+    py::enum_<CuDSSAlgorithm>(m, "CuDSSAlgorithm")
+        .value("DEFAULT", CuDSSAlgorithm::Default)
+        .value("ALG_1", CuDSSAlgorithm::Alg1)
+        .value("ALG_2", CuDSSAlgorithm::Alg2)
+        .value("ALG_3", CuDSSAlgorithm::Alg3)
+        .value("ALG_4", CuDSSAlgorithm::Alg4)
+        .value("ALG_5", CuDSSAlgorithm::Alg5)
+        .export_values();
+
+    py::class_<NRConfig>(m, "NRConfig", "Newton-Raphson convergence settings")
+        .def(py::init<>())
+        .def_readwrite("tolerance", &NRConfig::tolerance)
+        .def_readwrite("max_iter", &NRConfig::max_iter);
+
+    py::class_<CuDSSOptions>(m, "CuDSSOptions")
+        .def(py::init<>())
+        .def_readwrite("use_matching", &CuDSSOptions::use_matching)
+        .def_readwrite("matching_alg", &CuDSSOptions::matching_alg)
+        .def_readwrite("auto_pivot_epsilon", &CuDSSOptions::auto_pivot_epsilon)
+        .def_readwrite("pivot_epsilon", &CuDSSOptions::pivot_epsilon);
+
+    py::class_<CustomSolverConfig>(m, "CustomSolverConfig")
+        .def(py::init<>())
+        .def_readwrite("precision", &CustomSolverConfig::precision)
+        .def_readwrite("matching", &CustomSolverConfig::matching)
+        .def_readwrite("pivot_strategy", &CustomSolverConfig::pivot_strategy)
+        .def_readwrite("serial_nd", &CustomSolverConfig::serial_nd)
+        .def_readwrite("max_panel_width", &CustomSolverConfig::max_panel_width)
+        .def_readwrite("shift_retry_epsilon", &CustomSolverConfig::shift_retry_epsilon)
+        .def_readwrite("refine_steps", &CustomSolverConfig::refine_steps);
+
+    py::class_<NewtonOptions>(m, "NewtonOptions")
+        .def(py::init<>())
+        .def_readwrite("backend", &NewtonOptions::backend)
+        .def_readwrite("compute", &NewtonOptions::compute)
+        .def_readwrite("cpu_linear_solver", &NewtonOptions::cpu_linear_solver)
+        .def_readwrite("cuda_jacobian", &NewtonOptions::cuda_jacobian)
+        .def_readwrite("cuda_linear_solver", &NewtonOptions::cuda_linear_solver)
+        .def_readwrite("cudss", &NewtonOptions::cudss)
+        .def_readwrite("custom", &NewtonOptions::custom)
+        .def_readwrite("use_cuda_graph", &NewtonOptions::use_cuda_graph);
+
+    py::class_<SolveOptions>(m, "SolveOptions")
+        .def(py::init<>())
+        .def_readwrite("prepare_adjoint_cache", &SolveOptions::prepare_adjoint_cache)
+        .def_readwrite("adjoint_cache_mode", &SolveOptions::adjoint_cache_mode)
+        .def_readwrite("allow_explicit_transpose_fallback",
+                       &SolveOptions::allow_explicit_transpose_fallback);
+
+    py::class_<AdjointOptions>(m, "AdjointOptions")
+        .def(py::init<>())
+        .def_readwrite("reuse_forward_factorization", &AdjointOptions::reuse_forward_factorization)
+        .def_readwrite("allow_refactorize", &AdjointOptions::allow_refactorize)
+        .def_readwrite("require_cached_factorization", &AdjointOptions::require_cached_factorization)
+        .def_readwrite("allow_refactorize_for_backward",
+                       &AdjointOptions::allow_refactorize_for_backward)
+        .def_readwrite("allow_inexact_last_newton_factorization",
+                       &AdjointOptions::allow_inexact_last_newton_factorization)
+        .def_readwrite("use_transpose_solve", &AdjointOptions::use_transpose_solve)
+        .def_readwrite("allow_explicit_transpose_fallback",
+                       &AdjointOptions::allow_explicit_transpose_fallback)
+        .def_readwrite("compute_load_gradients", &AdjointOptions::compute_load_gradients)
+        .def_readwrite("check_residual", &AdjointOptions::check_residual);
+
+    py::class_<NRResult>(m, "NRResult")
+        .def_readonly("V", &NRResult::V)
+        .def_property_readonly("V_numpy", [](const NRResult& r) { return vec_to_numpy(r.V); })
+        .def_readonly("iterations", &NRResult::iterations)
+        .def_readonly("final_mismatch", &NRResult::final_mismatch)
+        .def_readonly("converged", &NRResult::converged);
+
+    py::class_<NRBatchResult>(m, "NRBatchResult")
+        .def_readonly("V", &NRBatchResult::V)
+        .def_property_readonly("V_numpy",
+                               [](const NRBatchResult& r) {
+                                   py::array_t<std::complex<double>> out({r.batch_size, r.n_bus});
+                                   if (!r.V.empty())
+                                       std::memcpy(out.mutable_data(), r.V.data(),
+                                                   r.V.size() * sizeof(std::complex<double>));
+                                   return out;
+                               })
+        .def_readonly("n_bus", &NRBatchResult::n_bus)
+        .def_readonly("batch_size", &NRBatchResult::batch_size)
+        .def_readonly("iterations", &NRBatchResult::iterations)
+        .def_property_readonly("iterations_numpy",
+                               [](const NRBatchResult& r) { return vec_to_numpy(r.iterations); })
+        .def_readonly("final_mismatch", &NRBatchResult::final_mismatch)
+        .def_property_readonly("final_mismatch_numpy",
+                               [](const NRBatchResult& r) { return vec_to_numpy(r.final_mismatch); })
+        .def_readonly("converged", &NRBatchResult::converged)
+        .def_property_readonly("converged_numpy",
+                               [](const NRBatchResult& r) { return vec_to_numpy(r.converged); })
+        .def_readonly("singular", &NRBatchResult::singular)
+        .def_property_readonly("singular_numpy",
+                               [](const NRBatchResult& r) { return vec_to_numpy(r.singular); });
+
+    py::class_<AdjointResult>(m, "AdjointResult")
+        .def_readonly("lambda", &AdjointResult::lambda)
+        .def_readonly("lambda_", &AdjointResult::lambda)
+        .def_property_readonly("lambda_numpy",
+                               [](const AdjointResult& r) {
+                                   return mat_to_numpy(r.lambda, r.batch_size, r.dimF);
+                               })
+        .def_readonly("grad_load_p", &AdjointResult::grad_load_p)
+        .def_property_readonly("grad_load_p_numpy",
+                               [](const AdjointResult& r) {
+                                   return mat_to_numpy(r.grad_load_p, r.batch_size, r.n_bus);
+                               })
+        .def_readonly("grad_load_q", &AdjointResult::grad_load_q)
+        .def_property_readonly("grad_load_q_numpy",
+                               [](const AdjointResult& r) {
+                                   return mat_to_numpy(r.grad_load_q, r.batch_size, r.n_bus);
+                               })
+        .def_readonly("n_bus", &AdjointResult::n_bus)
+        .def_readonly("batch_size", &AdjointResult::batch_size)
+        .def_readonly("dimF", &AdjointResult::dimF)
+        .def_readonly("success", &AdjointResult::success)
+        .def_readonly("used_adjoint_cache", &AdjointResult::used_adjoint_cache)
+        .def_readonly("adjoint_cache_matches_final_state",
+                      &AdjointResult::adjoint_cache_matches_final_state)
+        .def_readonly("reused_forward_factorization", &AdjointResult::reused_forward_factorization)
+        .def_readonly("reused_final_state_factorization",
+                      &AdjointResult::reused_final_state_factorization)
+        .def_readonly("refactorized_for_backward", &AdjointResult::refactorized_for_backward)
+        .def_readonly("used_explicit_transpose", &AdjointResult::used_explicit_transpose)
+        .def_readonly("used_python_scipy", &AdjointResult::used_python_scipy)
+        .def_readonly("includes_host_device_transfer", &AdjointResult::includes_host_device_transfer)
+        .def_readonly("zero_copy", &AdjointResult::zero_copy)
+        .def_readonly("torch_extension_zero_copy", &AdjointResult::torch_extension_zero_copy)
+        .def_readonly("raw_pointer_api_used", &AdjointResult::raw_pointer_api_used)
+        .def_readonly("current_stream_integrated", &AdjointResult::current_stream_integrated)
+        .def_readonly("jt_symbolic_analyzed_at_initialize",
+                      &AdjointResult::jt_symbolic_analyzed_at_initialize)
+        .def_readonly("jt_values_transposed_on_device", &AdjointResult::jt_values_transposed_on_device)
+        .def_readonly("jt_factorized_during_forward_cache",
+                      &AdjointResult::jt_factorized_during_forward_cache)
+        .def_readonly("jt_refactorized_during_backward",
+                      &AdjointResult::jt_refactorized_during_backward)
+        .def_readonly("host_roundtrip_for_jt_transpose",
+                      &AdjointResult::host_roundtrip_for_jt_transpose)
+        .def_readonly("jt_residual_norm", &AdjointResult::jt_residual_norm)
+        .def_readonly("solve_time_ms", &AdjointResult::solve_time_ms)
+        .def_readonly("transpose_solve_time_ms", &AdjointResult::transpose_solve_time_ms)
+        .def_readonly("factorization_time_ms", &AdjointResult::factorization_time_ms)
+        .def_readonly("total_time_ms", &AdjointResult::total_time_ms)
+        .def_readonly("backend", &AdjointResult::backend)
+        .def_readonly("transpose_solve_backend", &AdjointResult::transpose_solve_backend)
+        .def_readonly("sign_convention", &AdjointResult::sign_convention);
+
+    py::class_<PySolver>(m, "NewtonSolver")
+        .def(py::init<const NewtonOptions&>(), py::arg("options") = NewtonOptions{})
+        .def(
+            "initialize",
+            [](PySolver& self, I32 indptr, I32 indices, C128 data, int32_t rows, int32_t cols,
+               I32 pv, I32 pq) {
+                check_ybus(indptr, indices, data, rows, cols);
+                py::gil_scoped_release nogil;
+                check(cupf_solver_initialize(self.h(), rows, cols, indptr.data(), indices.data(),
+                                             cplx(data), int32_t(data.size()), pv.data(),
+                                             int32_t(pv.size()), pq.data(), int32_t(pq.size())));
+                self.n_bus = rows;
+                self.dim = int32_t(pv.size() + 2 * pq.size());
+            },
+            py::arg("indptr"), py::arg("indices"), py::arg("data"), py::arg("rows"),
+            py::arg("cols"), py::arg("pv"), py::arg("pq"))
+        .def(
+            "solve",
+            [](PySolver& self, I32 indptr, I32 indices, C128 data, int32_t rows, int32_t cols,
+               C128 sbus, C128 v0, I32 pv, I32 pq, const NRConfig& config,
+               const SolveOptions& so) {
+                if (sbus.ndim() != 1 || v0.ndim() != 1)
+                    throw std::invalid_argument("sbus and v0 must be 1D arrays for solve()");
+                if (sbus.size() != rows || v0.size() != rows)
+                    throw std::invalid_argument("sbus/v0 length must match Ybus rows");
+                check_ybus(indptr, indices, data, rows, cols);
+                NRBatchResult br =
+                    run_batch(self, indptr, indices, data, rows, cols, sbus, v0, 1, pv, pq, config, so);
+                NRResult r;
+                r.V = std::move(br.V);
+                r.iterations = br.iterations[0];
+                r.final_mismatch = br.final_mismatch[0];
+                r.converged = br.converged[0] != 0;
+                return r;
+            },
+            py::arg("indptr"), py::arg("indices"), py::arg("data"), py::arg("rows"),
+            py::arg("cols"), py::arg("sbus"), py::arg("v0"), py::arg("pv"), py::arg("pq"),
+            py::arg("config") = NRConfig{}, py::arg("solve_options") = SolveOptions{})
+        .def(
+            "solve_batch",
+            [](PySolver& self, I32 indptr, I32 indices, C128 data, int32_t rows, int32_t cols,
+               C128 sbus, C128 v0, I32 pv, I32 pq, const NRConfig& config,
+               const SolveOptions& so) {
+                if (sbus.ndim() != 2 || v0.ndim() != 2)
+                    throw std::invalid_argument(
+                        "sbus and v0 must be 2D arrays [batch, n_bus] for solve_batch()");
+                if (sbus.shape(0) != v0.shape(0) || sbus.shape(1) != rows || v0.shape(1) != rows)
+                    throw std::invalid_argument("sbus/v0 shape must be [batch, rows]");
+                check_ybus(indptr, indices, data, rows, cols);
+                return run_batch(self, indptr, indices, data, rows, cols, sbus, v0,
+                                 int32_t(sbus.shape(0)), pv, pq, config, so);
+            },
+            py::arg("indptr"), py::arg("indices"), py::arg("data"), py::arg("rows"),
+            py::arg("cols"), py::arg("sbus"), py::arg("v0"), py::arg("pv"), py::arg("pq"),
+            py::arg("config") = NRConfig{}, py::arg("solve_options") = SolveOptions{})
+        .def(
+            "solve_adjoint",
+            [](PySolver& self, F64 grad_va, F64 grad_vm, I32 pv, I32 pq,
+               const AdjointOptions& options) {
+                if (grad_va.ndim() != grad_vm.ndim())
+                    throw std::invalid_argument("grad_va and grad_vm must have the same rank");
+                int32_t batch = 1, n_bus = 0;
+                if (grad_va.ndim() == 1) {
+                    if (grad_va.shape(0) != grad_vm.shape(0))
+                        throw std::invalid_argument("1D grad_va/grad_vm shapes must match");
+                    n_bus = int32_t(grad_va.shape(0));
+                } else if (grad_va.ndim() == 2) {
+                    if (grad_va.shape(0) != grad_vm.shape(0) || grad_va.shape(1) != grad_vm.shape(1))
+                        throw std::invalid_argument("2D grad_va/grad_vm shapes must match");
+                    batch = int32_t(grad_va.shape(0));
+                    n_bus = int32_t(grad_va.shape(1));
+                } else {
+                    throw std::invalid_argument(
+                        "grad_va and grad_vm must be 1D [n_bus] or 2D [batch, n_bus]");
+                }
+                if (n_bus != self.n_bus)
+                    throw std::invalid_argument("grad_va/grad_vm length must match n_bus");
+                AdjointResult r;
+                r.lambda.resize(size_t(batch) * self.dim);
+                r.grad_load_p.resize(size_t(batch) * n_bus);
+                r.grad_load_q.resize(size_t(batch) * n_bus);
+                const cupf_adjoint_options o = c_aopt(options);
+                cupf_adjoint_info info;
+                {
+                    py::gil_scoped_release nogil;
+                    check(cupf_solver_solve_adjoint(self.h(), grad_va.data(), n_bus, grad_vm.data(),
+                                                    n_bus, batch, pv.data(), int32_t(pv.size()),
+                                                    pq.data(), int32_t(pq.size()), &o,
+                                                    r.lambda.data(), r.grad_load_p.data(),
+                                                    r.grad_load_q.data(), &info));
+                }
+                info_to_result(info, r);
+                if (!options.compute_load_gradients) {
+                    r.grad_load_p.clear();
+                    r.grad_load_q.clear();
+                }
+                return r;
+            },
+            py::arg("grad_va"), py::arg("grad_vm"), py::arg("pv"), py::arg("pq"),
+            py::arg("options") = AdjointOptions{})
+        .def(
+            "_solve_torch_raw",
+            [](PySolver& self, uintptr_t sbus_base_re, uintptr_t sbus_base_im, uintptr_t load_p,
+               uintptr_t load_q, uintptr_t v0_va, uintptr_t v0_vm, uintptr_t va_out,
+               uintptr_t vm_out, int32_t batch, int32_t n_bus, const std::string& dtype,
+               const NRConfig& config, const SolveOptions& so, uintptr_t stream) {
+                const cupf_nr_config c = c_cfg(config);
+                const cupf_solve_options s = c_sopt(so);
+                cupf_adjoint_info info;
+                {
+                    py::gil_scoped_release nogil;
+                    check(cupf_solver_solve_torch_forward(
+                        self.h(), (const void*)sbus_base_re, (const void*)sbus_base_im,
+                        (const void*)load_p, (const void*)load_q, (const void*)v0_va,
+                        (const void*)v0_vm, (void*)va_out, (void*)vm_out, batch, n_bus,
+                        dtype.c_str(), &c, &s, (void*)stream, &info));
+                }
+                self.last_batch = batch;
+                AdjointResult r;
+                info_to_result(info, r);
+                return r;
+            })
+        .def(
+            "_solve_adjoint_torch_raw",
+            [](PySolver& self, uintptr_t grad_va, uintptr_t grad_vm, uintptr_t grad_p,
+               uintptr_t grad_q, int32_t batch, int32_t n_bus, const std::string& dtype,
+               const AdjointOptions& options, uintptr_t stream) {
+                const cupf_adjoint_options o = c_aopt(options);
+                cupf_adjoint_info info;
+                {
+                    py::gil_scoped_release nogil;
+                    check(cupf_solver_solve_torch_backward(
+                        self.h(), (const void*)grad_va, (const void*)grad_vm, (void*)grad_p,
+                        (void*)grad_q, batch, n_bus, dtype.c_str(), &o, (void*)stream, &info));
+                }
+                AdjointResult r;
+                info_to_result(info, r);
+                return r;
+            })
+        .def("stats",
+             [](PySolver& self) {
+                 cupf_solver_stats s;
+                 check(cupf_solver_get_stats(self.h(), &s));
+                 py::dict d;
+                 d["last_iterations"] = s.last_iterations;
+                 d["last_batch"] = s.last_batch;
+                 d["factorizations"] = s.factorizations;
+                 d["last_solve_ms"] = s.last_solve_ms;
+                 d["dim"] = s.dim;
+                 d["nnz_j"] = s.nnz_j;
+                 d["num_fronts"] = s.num_fronts;
+                 d["max_front"] = s.max_front;
+                 d["max_pivots"] = s.max_pivots;
+                 d["arena_elems"] = s.arena_elems;
+                 d["lnz"] = s.lnz;
+                 d["factor_flops"] = s.factor_flops;
+                 return d;
+             })
+        .def_property_readonly("n_bus", [](const PySolver& s) { return s.n_bus; })
+        .def_property_readonly("dimF", [](const PySolver& s) { return s.dim; })
+        .def_property_readonly("last_batch", [](const PySolver& s) { return s.last_batch; })
+        .def_property_readonly("handle",
+                               [](const PySolver& s) { return reinterpret_cast<uintptr_t>(s.h()); });
+}

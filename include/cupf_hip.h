@@ -1,0 +1,220 @@
+/*
+ * cupf_hip.h -- C ABI of the MI355X batched Newton-Raphson power-flow solver
+ * (libcupf_hip.so).  Plain pointers and sizes only; complex arrays are
+ * interleaved (re, im) float64 pairs; every call returns a status code and
+ * cupf_last_error() holds the message of the last failure on this thread.
+ *
+ * The entry points replace, one for one, what the reference's pybind module
+ * binds (cuPF/bindings/pybind_cupf.cpp) on top of its C++ NewtonSolver
+ * (cuPF/cpp/inc/newton_solver/core/newton_solver.hpp:26-102).  Status codes
+ * carry the reference's exception classes: CUPF_INVALID_ARGUMENT for
+ * std::invalid_argument (bad arguments, strides, option combinations) and
+ * CUPF_RUNTIME_ERROR for std::runtime_error (state and numerics).
+ */
+#ifndef CUPF_HIP_H
+#define CUPF_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    CUPF_OK = 0,
+    CUPF_INVALID_ARGUMENT = 1,
+    CUPF_RUNTIME_ERROR = 2
+};
+
+/* Enum values follow the declaration order of the reference enums
+ * (newton_solver_types.hpp:41-147). */
+enum { CUPF_BACKEND_CPU = 0, CUPF_BACKEND_GPU = 1 };
+enum { CUPF_COMPUTE_FP64 = 0, CUPF_COMPUTE_FP32 = 1, CUPF_COMPUTE_MIXED = 2 };
+enum { CUPF_PRECISION_FP64 = 0, CUPF_PRECISION_FP32 = 1, CUPF_PRECISION_TF32 = 2 };
+
+typedef struct cupf_solver cupf_solver;
+
+/* NRConfig (newton_solver_types.hpp:35-38) */
+typedef struct {
+    double tolerance;
+    int32_t max_iter;
+} cupf_nr_config;
+
+/* SolveOptions (newton_solver_types.hpp:47-51) */
+typedef struct {
+    int32_t prepare_adjoint_cache;
+    int32_t adjoint_cache_mode;              /* 0 None, 1 FinalState, 2 ReuseLastIfExact */
+    int32_t allow_explicit_transpose_fallback;
+} cupf_solve_options;
+
+/* CustomSolverConfig (newton_solver_types.hpp:152-159) + refine_steps */
+typedef struct {
+    int32_t precision;
+    int32_t matching;
+    int32_t pivot_strategy;
+    int32_t serial_nd;
+    int32_t max_panel_width;
+    double shift_retry_epsilon;
+    int32_t refine_steps;
+} cupf_custom_config;
+
+/* NewtonOptions (newton_solver_types.hpp:167-192) with CuDSSOptions inlined */
+typedef struct {
+    int32_t backend;
+    int32_t compute;
+    int32_t cuda_linear_solver;              /* 0 CuDSS, 1 Custom */
+    int32_t cpu_linear_solver;
+    int32_t cuda_jacobian;
+    int32_t cudss_use_matching;
+    int32_t cudss_matching_alg;
+    int32_t cudss_auto_pivot_epsilon;
+    double cudss_pivot_epsilon;
+    cupf_custom_config custom;
+    int32_t use_cuda_graph;
+} cupf_newton_options;
+
+/* AdjointOptions (newton_solver_types.hpp:240-250) */
+typedef struct {
+    int32_t reuse_forward_factorization;
+    int32_t allow_refactorize;
+    int32_t require_cached_factorization;
+    int32_t allow_refactorize_for_backward;
+    int32_t allow_inexact_last_newton_factorization;
+    int32_t use_transpose_solve;
+    int32_t allow_explicit_transpose_fallback;
+    int32_t compute_load_gradients;
+    int32_t check_residual;
+} cupf_adjoint_options;
+
+/* Scalar part of AdjointResult (newton_solver_types.hpp:252-290); the arrays
+ * are caller buffers of the calls below. */
+typedef struct {
+    int32_t n_bus, batch_size, dimF;
+    int32_t success, used_adjoint_cache, adjoint_cache_matches_final_state,
+        reused_forward_factorization, reused_final_state_factorization,
+        refactorized_for_backward, used_explicit_transpose, used_python_scipy,
+        includes_host_device_transfer, zero_copy, torch_extension_zero_copy,
+        raw_pointer_api_used, current_stream_integrated, jt_symbolic_analyzed_at_initialize,
+        jt_values_transposed_on_device, jt_factorized_during_forward_cache,
+        jt_refactorized_during_backward, host_roundtrip_for_jt_transpose;
+    double jt_residual_norm, solve_time_ms, transpose_solve_time_ms, factorization_time_ms,
+        total_time_ms;
+    char backend[64];
+    char transpose_solve_backend[64];
+} cupf_adjoint_info;
+
+/* Solver figures (no reference counterpart; diagnostics). */
+typedef struct {
+    int32_t last_iterations, last_batch, factorizations;
+    double last_solve_ms;
+    int32_t dim, nnz_j, num_fronts, max_front, max_pivots;
+    int64_t arena_elems, lnz;
+    double factor_flops;
+} cupf_solver_stats;
+
+const char* cupf_last_error(void);
+
+void cupf_default_nr_config(cupf_nr_config* out);
+void cupf_default_solve_options(cupf_solve_options* out);
+void cupf_default_newton_options(cupf_newton_options* out);
+void cupf_default_adjoint_options(cupf_adjoint_options* out);
+
+/* NewtonSolver(const NewtonOptions&)  (newton_solver.hpp:318) */
+int cupf_solver_create(const cupf_newton_options* options, cupf_solver** out);
+/* ~NewtonSolver()  (newton_solver.hpp:319) */
+int cupf_solver_destroy(cupf_solver* solver);
+
+/* initialize(ybus, pv, n_pv, pq, n_pq)  (newton_solver.hpp:321-323) */
+int cupf_solver_initialize(cupf_solver* solver, int32_t rows, int32_t cols,
+                           const int32_t* indptr, const int32_t* indices, const double* data,
+                           int32_t nnz, const int32_t* pv, int32_t n_pv, const int32_t* pq,
+                           int32_t n_pq);
+
+/* solve_batch(...)  (newton_solver.hpp:334-344).  sbus/V0: [batch][stride]
+ * complex; V_out: [batch][rows] complex; iterations/final_mismatch/
+ * converged/singular: [batch] (singular may be NULL). */
+int cupf_solver_solve_batch(cupf_solver* solver, int32_t rows, int32_t cols,
+                            const int32_t* indptr, const int32_t* indices, const double* data,
+                            int32_t nnz, const double* sbus, int64_t sbus_stride,
+                            const double* V0, int64_t V0_stride, int32_t batch,
+                            const int32_t* pv, int32_t n_pv, const int32_t* pq, int32_t n_pq,
+                            const cupf_nr_config* config, const cupf_solve_options* options,
+                            double* V_out, int32_t* iterations, double* final_mismatch,
+                            uint8_t* converged, uint8_t* singular);
+
+/* solve_adjoint(...)  (newton_solver.hpp:346-354).  lambda: [batch][dimF];
+ * grad_load_p/q: [batch][n_bus] (may be NULL when not computed). */
+int cupf_solver_solve_adjoint(cupf_solver* solver, const double* grad_va, int64_t grad_va_stride,
+                              const double* grad_vm, int64_t grad_vm_stride, int32_t batch,
+                              const int32_t* pv, int32_t n_pv, const int32_t* pq, int32_t n_pq,
+                              const cupf_adjoint_options* options, double* lambda,
+                              double* grad_load_p, double* grad_load_q, cupf_adjoint_info* info);
+
+/* solve_torch_forward(...)  (newton_solver.hpp:367-380; torch_bridge.cpp:309-482).
+ * Device pointers; dtype "float64" (FP64) or "float32" (FP32/Mixed); stream is
+ * a hipStream_t (NULL = the solver's own stream). */
+int cupf_solver_solve_torch_forward(cupf_solver* solver, const void* sbus_base_re,
+                                    const void* sbus_base_im, const void* load_p,
+                                    const void* load_q, const void* v0_va, const void* v0_vm,
+                                    void* va_out, void* vm_out, int32_t batch, int32_t n_bus,
+                                    const char* dtype, const cupf_nr_config* config,
+                                    const cupf_solve_options* options, void* stream,
+                                    cupf_adjoint_info* info);
+
+/* solve_torch_backward(...)  (newton_solver.hpp:357-365; torch_bridge.cpp:209-303) */
+int cupf_solver_solve_torch_backward(cupf_solver* solver, const void* grad_va, const void* grad_vm,
+                                     void* grad_load_p, void* grad_load_q, int32_t batch,
+                                     int32_t n_bus, const char* dtype,
+                                     const cupf_adjoint_options* options, void* stream,
+                                     cupf_adjoint_info* info);
+
+int cupf_solver_get_stats(cupf_solver* solver, cupf_solver_stats* out);
+
+/* ---- Diagnostics for parity tests (no reference counterpart) ---------- */
+int cupf_solver_debug_residual(cupf_solver* solver, const double* sbus, const double* V,
+                               int32_t batch, double* F, double* norm);
+int cupf_solver_debug_jacobian(cupf_solver* solver, const double* V, int32_t batch, double* J);
+int cupf_solver_debug_factor_solve(cupf_solver* solver, const double* A, const double* rhs,
+                                   int32_t batch, int32_t transpose, double* x,
+                                   uint8_t* singular);
+int cupf_solver_debug_pattern(cupf_solver* solver, int32_t* row_ptr, int32_t* col_idx,
+                              int32_t* perm);
+
+/* ---- Host analysis (no GPU needed) ------------------------------------ */
+/* Jacobian pattern + scatter maps; the reference's make_jacobian_indexing /
+ * JacobianPatternGenerator / JacobianMapBuilder (jacobian_analysis.cpp:257-445).
+ * Pass col_idx == NULL to query dim and nnz first. */
+int cupf_analyze_jacobian(int32_t n_bus, const int32_t* indptr, const int32_t* indices,
+                          const int32_t* pv, int32_t n_pv, const int32_t* pq, int32_t n_pq,
+                          int32_t* dim, int32_t* nnz, int32_t* row_ptr, int32_t* col_idx,
+                          int32_t* map11, int32_t* map21, int32_t* map12, int32_t* map22,
+                          int32_t* diag11, int32_t* diag21, int32_t* diag12, int32_t* diag22);
+
+/* Fill-reducing, postordered column order of a Jacobian (perm[new] = old). */
+int cupf_order_jacobian(int32_t n_bus, const int32_t* indptr, const int32_t* indices,
+                        const int32_t* pv, int32_t n_pv, const int32_t* pq, int32_t n_pq,
+                        int32_t* perm);
+
+/* Symbolic multifrontal analysis of a square CSR pattern under perm with a
+ * relaxed-panel cap: the integer outputs of the reference's Etree,
+ * Postorder, ColumnCounts, FillPattern, RelaxedPanels and
+ * ComputeMultifrontalSymbolic (custom_linear_solver/src/analyze/symbolic).
+ * Call once with Li == NULL to get sizes = {lnz, num_panels, len(front_rows),
+ * len(asm_idx)}; then pass arrays of those sizes (n, n+1, ... as named). */
+int cupf_symbolic(int32_t n, const int32_t* row_ptr, const int32_t* col_idx,
+                  const int32_t* perm, int32_t cap, int64_t* sizes, int32_t* parent,
+                  int32_t* post, int32_t* colcount, int32_t* Lp, int32_t* Li,
+                  int32_t* panel_first, int32_t* panel_ncols, int32_t* panel_width,
+                  int32_t* panel_of, int32_t* panel_parent, int32_t* front_ptr,
+                  int32_t* front_rows, int32_t* asm_ptr, int32_t* asm_idx);
+
+/* Work figures of the lowered plan for (pattern, perm, cap):
+ * out = {flops per scenario, sum fsz^2, max fsz, number of fronts, nnz(L)}. */
+int cupf_plan_figures(int32_t n, const int32_t* row_ptr, const int32_t* col_idx,
+                      const int32_t* perm, int32_t cap, double* out5);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CUPF_HIP_H */
