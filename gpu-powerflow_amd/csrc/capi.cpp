@@ -306,6 +306,30 @@ int cupf_solver_get_stats(cupf_solver* solver, cupf_solver_stats* out) {
     });
 }
 
+int cupf_solver_solve_batch_device(cupf_solver* solver, const void* sbus, int64_t sbus_stride,
+                                   const void* V0, int64_t V0_stride, int32_t batch,
+                                   const cupf_nr_config* config,
+                                   const cupf_solve_options* options, void* V_out,
+                                   int32_t* iterations, double* final_mismatch,
+                                   uint8_t* converged, void* stream) {
+    return guarded([&] {
+        solver_of(solver).solve_batch_device(sbus, sbus_stride, V0, V0_stride, batch,
+                                             to_cfg(config), to_sopt(options), V_out, iterations,
+                                             final_mismatch, converged, stream);
+    });
+}
+
+int cupf_solver_set_stage_timing(cupf_solver* solver, int32_t on) {
+    return guarded([&] { solver_of(solver).set_stage_timing(on != 0); });
+}
+
+int cupf_solver_collect_stage_times(cupf_solver* solver, double* total_ms, int64_t* launches) {
+    return guarded([&] {
+        if (!total_ms || !launches) throw std::invalid_argument("null output");
+        solver_of(solver).collect_stage_times(total_ms, launches);
+    });
+}
+
 int cupf_solver_debug_residual(cupf_solver* solver, const double* sbus, const double* V,
                                int32_t batch, double* F, double* norm) {
     return guarded([&] {

@@ -30,9 +30,63 @@ void validate_batch_args(int32_t batch_size, int64_t stride, int32_t n_bus, cons
                                     " stride must be at least n_bus");
 }
 
+// Pairs of HIP events recorded around each stage on the stream it runs on.
+struct StageTimer {
+    static constexpr int kStages = 5;
+    bool on = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pool;
+    std::vector<int> stage_of;
+    size_t used = 0;
+    ~StageTimer() {
+        for (auto& e : pool) {
+            (void)hipEventDestroy(e.first);
+            (void)hipEventDestroy(e.second);
+        }
+    }
+    int begin(int stage, hipStream_t s) {
+        if (!on) return -1;
+        if (used == pool.size()) {
+            std::pair<hipEvent_t, hipEvent_t> e;
+            CUPF_HIP(hipEventCreate(&e.first));
+            CUPF_HIP(hipEventCreate(&e.second));
+            pool.push_back(e);
+            stage_of.push_back(stage);
+        }
+        stage_of[used] = stage;
+        CUPF_HIP(hipEventRecord(pool[used].first, s));
+        return int(used++);
+    }
+    void end(int idx, hipStream_t s) {
+        if (idx >= 0) CUPF_HIP(hipEventRecord(pool[idx].second, s));
+    }
+    void collect(double* ms, int64_t* count) {
+        for (int k = 0; k < kStages; ++k) { ms[k] = 0.0; count[k] = 0; }
+        for (size_t i = 0; i < used; ++i) {
+            CUPF_HIP(hipEventSynchronize(pool[i].second));
+            float t = 0.f;
+            CUPF_HIP(hipEventElapsedTime(&t, pool[i].first, pool[i].second));
+            ms[stage_of[i]] += t;
+            count[stage_of[i]] += 1;
+        }
+        used = 0;
+    }
+};
+
+struct StageScope {
+    StageTimer& t;
+    hipStream_t s;
+    int idx;
+    StageScope(StageTimer& timer, int stage, hipStream_t stream)
+        : t(timer), s(stream), idx(timer.begin(stage, stream)) {}
+    ~StageScope() { t.end(idx, s); }
+};
+
+enum { kStResidual = 0, kStJacobian = 1, kStFactor = 2, kStSolve = 3, kStUpdate = 4 };
+
 }  // namespace
 
 struct NewtonSolver::Impl {
+    StageTimer timer;
     NewtonOptions opt;
     bool initialized = false;
     NewtonSolverStats st;
@@ -143,13 +197,24 @@ struct NewtonSolver::Impl {
     }
 
     void newton_step(int B) {
-        factor(B);
+        {
+            StageScope st_(timer, kStFactor, s);
+            factor(B);
+        }
         if (dx32) {
-            solve_into<float>(B, F.as<double>(), dx.as<float>(), ddx.as<float>(), false);
+            {
+                StageScope st_(timer, kStSolve, s);
+                solve_into<float>(B, F.as<double>(), dx.as<float>(), ddx.as<float>(), false);
+            }
+            StageScope st_(timer, kStUpdate, s);
             kern::voltage_update<float>(nr, dx.as<float>(), B, Va.as<double>(), Vm.as<double>(),
                                         V.as<double2>(), s);
         } else {
-            solve_into<double>(B, F.as<double>(), dx.as<double>(), ddx.as<double>(), false);
+            {
+                StageScope st_(timer, kStSolve, s);
+                solve_into<double>(B, F.as<double>(), dx.as<double>(), ddx.as<double>(), false);
+            }
+            StageScope st_(timer, kStUpdate, s);
             kern::voltage_update<double>(nr, dx.as<double>(), B, Va.as<double>(),
                                          Vm.as<double>(), V.as<double2>(), s);
         }
@@ -167,14 +232,20 @@ struct NewtonSolver::Impl {
         int completed = 0;
         for (int it = 0; it < cfg.max_iter; ++it) {
             completed = it + 1;
-            kern::residual(nr, V.as<double2>(), S.as<double2>(), B, F.as<double>(),
-                           normF.as<double>(), s);
+            {
+                StageScope st_(timer, kStResidual, s);
+                kern::residual(nr, V.as<double2>(), S.as<double2>(), B, F.as<double>(),
+                               normF.as<double>(), s);
+            }
             const double m = read_batch_max(B);
             if (!std::isfinite(m))
                 throw std::runtime_error("NewtonSolver: mismatch norm is not finite");
             if (m <= cfg.tolerance) break;
-            kern::jacobian_impl<double>(nr, V.as<double2>(), B, J.as<double>(),
-                                        d_zero.as<int>(), int(zero_slots.size()), s);
+            {
+                StageScope st_(timer, kStJacobian, s);
+                kern::jacobian_impl<double>(nr, V.as<double2>(), B, J.as<double>(),
+                                            d_zero.as<int>(), int(zero_slots.size()), s);
+            }
             newton_step(B);
         }
         return completed;
@@ -260,6 +331,61 @@ NewtonSolver::~NewtonSolver() {
 }
 
 const NewtonSolverStats& NewtonSolver::stats() const { return impl_->st; }
+
+void NewtonSolver::set_stage_timing(bool on) {
+    impl_->timer.on = on;
+    impl_->timer.used = 0;
+}
+
+void NewtonSolver::collect_stage_times(double* total_ms, int64_t* launches) {
+    impl_->timer.collect(total_ms, launches);
+}
+
+void NewtonSolver::solve_batch_device(const void* sbus, int64_t sbus_stride, const void* V0,
+                                      int64_t V0_stride, int32_t batch_size,
+                                      const NRConfig& config, const SolveOptions& solve_options,
+                                      void* V_out, int32_t* iterations, double* final_mismatch,
+                                      uint8_t* converged, void* stream) {
+    Impl& m = *impl_;
+    if (!m.initialized)
+        throw std::runtime_error("NewtonSolver::solve_batch_device(): initialize() must be called first");
+    validate_batch_args(batch_size, sbus_stride, m.n, "sbus");
+    validate_batch_args(batch_size, V0_stride, m.n, "V0");
+    if (!sbus || !V0 || !V_out)
+        throw std::invalid_argument("NewtonSolver::solve_batch_device(): null device pointer");
+    if (config.max_iter < 0) throw std::invalid_argument("NRConfig.max_iter must be >= 0");
+    const auto t0 = std::chrono::steady_clock::now();
+    m.s = stream ? static_cast<hipStream_t>(stream) : m.own;
+    m.cache = false;
+    const int B = batch_size;
+    m.ensure_batch(B);
+    const size_t row = size_t(m.n) * sizeof(double2);
+    CUPF_HIP(hipMemcpy2DAsync(m.S.get(), row, sbus, size_t(sbus_stride) * sizeof(double2), row, B,
+                              hipMemcpyDeviceToDevice, m.s));
+    kern::seed_state(m.nr, static_cast<const double2*>(V0), V0_stride, B, m.V.as<double2>(),
+                     m.Va.as<double>(), m.Vm.as<double>(), m.s);
+    CUPF_HIP(hipMemsetAsync(m.sing.get(), 0, size_t(B) * sizeof(int), m.s));
+    const int iters = m.run_nr(B, config);
+    m.last_B = B;
+    if (solve_options.prepare_adjoint_cache &&
+        solve_options.adjoint_cache_mode != AdjointCacheMode::None)
+        m.prepare_cache(B);
+    CUPF_HIP(hipMemcpyAsync(V_out, m.V.get(), size_t(B) * row, hipMemcpyDeviceToDevice, m.s));
+    std::vector<double> nf(B);
+    CUPF_HIP(hipMemcpyAsync(nf.data(), m.normF.get(), size_t(B) * sizeof(double),
+                            hipMemcpyDeviceToHost, m.s));
+    CUPF_HIP(hipStreamSynchronize(m.s));
+    for (int b = 0; b < B; ++b) {
+        if (iterations) iterations[b] = iters;
+        if (final_mismatch) final_mismatch[b] = nf[b];
+        if (converged) converged[b] = uint8_t(nf[b] <= config.tolerance ? 1 : 0);
+    }
+    m.st.last_iterations = iters;
+    m.st.last_batch = B;
+    m.st.last_solve_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    m.s = m.own;
+}
 
 void NewtonSolver::initialize(const YbusView& ybus, const int32_t* pv, int32_t n_pv,
                               const int32_t* pq, int32_t n_pq) {

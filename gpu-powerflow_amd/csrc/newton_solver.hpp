@@ -59,6 +59,21 @@ public:
                               const char* dtype, const AdjointOptions& options, void* stream,
                               AdjointResult& result);
 
+    // Device-resident batch solve (no host staging): sbus/V0/V_out are device
+    // complex128 arrays [batch][stride] / [batch][n_bus]; Ybus values are the
+    // ones of the last initialize()/solve.  iterations/final_mismatch/converged
+    // are host arrays [batch].
+    void solve_batch_device(const void* sbus, int64_t sbus_stride, const void* V0,
+                            int64_t V0_stride, int32_t batch_size, const NRConfig& config,
+                            const SolveOptions& solve_options, void* V_out, int32_t* iterations,
+                            double* final_mismatch, uint8_t* converged, void* stream);
+
+    // HIP-event timing of the stages on the stream they run on.
+    // Stage order: residual, jacobian, factor, solve (with refinement),
+    // voltage update.  collect_stage_times() synchronises and resets.
+    void set_stage_timing(bool on);
+    void collect_stage_times(double* total_ms, int64_t* launches);
+
     const NewtonSolverStats& stats() const;
 
     // Diagnostics used by the parity tests: run single stages on host data.

@@ -499,6 +499,45 @@ PYBIND11_MODULE(_cupf, m) {
                 info_to_result(info, r);
                 return r;
             })
+        .def(
+            "solve_batch_device",
+            [](PySolver& self, uintptr_t sbus, int64_t sbus_stride, uintptr_t v0,
+               int64_t v0_stride, int32_t batch, const NRConfig& config, const SolveOptions& so,
+               uintptr_t v_out, uintptr_t stream) {
+                NRBatchResult r;
+                r.n_bus = self.n_bus;
+                r.batch_size = batch;
+                r.iterations.resize(batch);
+                r.final_mismatch.resize(batch);
+                r.converged.resize(batch);
+                const cupf_nr_config c = c_cfg(config);
+                const cupf_solve_options s = c_sopt(so);
+                {
+                    py::gil_scoped_release nogil;
+                    check(cupf_solver_solve_batch_device(
+                        self.h(), (const void*)sbus, sbus_stride, (const void*)v0, v0_stride,
+                        batch, &c, &s, (void*)v_out, r.iterations.data(), r.final_mismatch.data(),
+                        r.converged.data(), (void*)stream));
+                }
+                self.last_batch = batch;
+                return r;
+            },
+            py::arg("sbus_ptr"), py::arg("sbus_stride"), py::arg("v0_ptr"), py::arg("v0_stride"),
+            py::arg("batch"), py::arg("config") = NRConfig{},
+            py::arg("solve_options") = SolveOptions{}, py::arg("v_out_ptr") = 0,
+            py::arg("stream") = 0)
+        .def("set_stage_timing",
+             [](PySolver& self, bool on) { check(cupf_solver_set_stage_timing(self.h(), on)); })
+        .def("collect_stage_times",
+             [](PySolver& self) {
+                 double ms[5];
+                 int64_t cnt[5];
+                 check(cupf_solver_collect_stage_times(self.h(), ms, cnt));
+                 py::dict d;
+                 const char* names[5] = {"residual", "jacobian", "factor", "solve", "voltage_update"};
+                 for (int k = 0; k < 5; ++k) d[names[k]] = py::make_tuple(ms[k], cnt[k]);
+                 return d;
+             })
         .def("stats",
              [](PySolver& self) {
                  cupf_solver_stats s;

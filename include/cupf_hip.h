@@ -170,6 +170,26 @@ int cupf_solver_solve_torch_backward(cupf_solver* solver, const void* grad_va, c
 
 int cupf_solver_get_stats(cupf_solver* solver, cupf_solver_stats* out);
 
+/* Device-resident solve_batch (no reference counterpart; the reference's
+ * solve_batch stages host arrays, cuda_batched_storage.cpp:235-358): sbus and
+ * V0 are device complex128 [batch][stride], V_out device complex128
+ * [batch][n_bus]; Ybus values are those of the last initialize()/solve;
+ * iterations/final_mismatch/converged are host [batch] (may be NULL);
+ * stream is a hipStream_t (NULL = the solver's own stream). */
+int cupf_solver_solve_batch_device(cupf_solver* solver, const void* sbus, int64_t sbus_stride,
+                                   const void* V0, int64_t V0_stride, int32_t batch,
+                                   const cupf_nr_config* config,
+                                   const cupf_solve_options* options, void* V_out,
+                                   int32_t* iterations, double* final_mismatch,
+                                   uint8_t* converged, void* stream);
+
+/* HIP-event stage timing (no reference counterpart; the reference's
+ * ENABLE_TIMING ScopedTimer, utils/timer.hpp:52-283, syncs per stage).
+ * Stages: 0 residual, 1 jacobian, 2 factor, 3 solve+refine, 4 voltage update.
+ * collect synchronises, writes total_ms[5] and launches[5], and resets. */
+int cupf_solver_set_stage_timing(cupf_solver* solver, int32_t on);
+int cupf_solver_collect_stage_times(cupf_solver* solver, double* total_ms, int64_t* launches);
+
 /* ---- Diagnostics for parity tests (no reference counterpart) ---------- */
 int cupf_solver_debug_residual(cupf_solver* solver, const double* sbus, const double* V,
                                int32_t batch, double* F, double* norm);
