@@ -413,11 +413,14 @@ Symbolic analyze_symbolic(int n, const int* rp, const int* ci, const std::vector
 // ===========================================================================
 // Lowering to the per-scenario front arena.
 // ===========================================================================
-LuPlan lower_plan(const Symbolic& s, int n, const int* rp, const int* ci) {
+LuPlan lower_plan(const Symbolic& s, int n, const int* rp, const int* ci, int tier0_max,
+                  int tier1_max) {
     LuPlan L;
     L.n = n;
     L.nnz = rp[n];
     L.num_fronts = s.num_panels;
+    L.tier0_max = tier0_max;
+    L.tier1_max = tier1_max;
     L.perm = s.perm;
     L.iperm = s.iperm;
     L.front_ptr = s.front_ptr;
@@ -425,23 +428,48 @@ LuPlan lower_plan(const Symbolic& s, int n, const int* rp, const int* ci) {
     const int P = s.num_panels;
     L.fsz.resize(P);
     L.nc.resize(P);
-    L.off.resize(P + 1);
-    long long off = 0;
+    L.tier.resize(P);
+    L.level.assign(P, 0);
+    L.off.resize(P);
+    L.uoff.assign(P + 1, 0);
     for (int p = 0; p < P; ++p) {
-        L.fsz[p] = s.front_ptr[p + 1] - s.front_ptr[p];
+        const int f = s.front_ptr[p + 1] - s.front_ptr[p];
+        L.fsz[p] = f;
         L.nc[p] = s.panel_ncols[p];
-        L.off[p] = off;
-        off += (long long)L.fsz[p] * L.fsz[p];
-        L.max_fsz = std::max(L.max_fsz, L.fsz[p]);
+        L.tier[p] = f <= tier0_max ? 0 : (f <= tier1_max ? 1 : 2);
+        const long long f2 = (long long)f * f;
+        if (L.tier[p] == 0) { L.off[p] = L.arena0; L.arena0 += f2; }
+        else { L.off[p] = L.arena1; L.arena1 += f2; }
+        L.uoff[p + 1] = L.uoff[p] + (f - L.nc[p]);
+        L.max_fsz = std::max(L.max_fsz, f);
         L.max_nc = std::max(L.max_nc, L.nc[p]);
         for (int k = 0; k < L.nc[p]; ++k) {
-            const double m = L.fsz[p] - k - 1;
+            const double m = f - k - 1;
             L.flops += m + 2.0 * m * m;
         }
-        L.factor_bytes_min += double(L.fsz[p]) * L.fsz[p];
+        L.factor_bytes_min += double(f2);
     }
-    L.off[P] = off;
-    L.arena = off;
+    L.upd_total = L.uoff[P];
+    // levels: children precede parents in panel order
+    for (int p = 0; p < P; ++p) {
+        const int par = s.panel_parent[p];
+        if (par >= 0) L.level[par] = std::max(L.level[par], L.level[p] + 1);
+    }
+    for (int p = 0; p < P; ++p) L.num_levels = std::max(L.num_levels, L.level[p] + 1);
+    const int G = L.num_levels * 3;
+    L.group_ptr.assign(G + 1, 0);
+    L.group_max_fsz.assign(G, 0);
+    for (int p = 0; p < P; ++p) {
+        const int g = L.level[p] * 3 + L.tier[p];
+        ++L.group_ptr[g + 1];
+        L.group_max_fsz[g] = std::max(L.group_max_fsz[g], L.fsz[p]);
+    }
+    for (int g = 0; g < G; ++g) L.group_ptr[g + 1] += L.group_ptr[g];
+    L.group_fronts.assign(P, -1);
+    {
+        std::vector<int> nx(L.group_ptr.begin(), L.group_ptr.end() - 1);
+        for (int p = 0; p < P; ++p) L.group_fronts[nx[L.level[p] * 3 + L.tier[p]]++] = p;
+    }
     L.lnz = s.Lp[n];
     for (int p = 0; p < P; ++p)
         for (int k = s.asm_ptr[p]; k < s.asm_ptr[p + 1]; ++k)

@@ -72,27 +72,44 @@ Symbolic analyze_symbolic(int n, const int* rp, const int* ci, const std::vector
                           int cap);
 
 // ---------------------------------------------------------------------------
-// Lowered plan for the batched GPU factorization.  Fronts are processed in
-// panel-id order (a postorder: every child precedes its parent).  Each front
-// is a dense fsz x fsz row-major block in a per-scenario arena.
+// Lowered plan for the batched GPU factorization and solve.
+//
+// Fronts are grouped by tree level (1 + max child level) and, inside a
+// level, by tier:
+//   tier 0 (fsz <= tier0_max): one lane per scenario; the front lives in a
+//          scenario-interleaved arena, element e of scenario b at
+//          (off + e) * Bp + b, Bp the batch padded to 64;
+//   tier 1 (fsz <= tier1_max): one workgroup per (front, scenario), the
+//          front assembled and factored in LDS, stored scenario-major at
+//          b * arena1 + off + e;
+//   tier 2 (larger): one workgroup per (front, scenario), blocked dense LU
+//          in the scenario-major arena.
+// Every front is a dense fsz x fsz row-major block.
 // ---------------------------------------------------------------------------
 struct LuPlan {
-    int n = 0, nnz = 0, num_fronts = 0;
-    long long arena = 0;                  // elements per scenario
+    int n = 0, nnz = 0, num_fronts = 0, num_levels = 0;
+    int tier0_max = 16, tier1_max = 128;
+    long long arena0 = 0, arena1 = 0;     // elements per scenario, per layout
     std::vector<int> perm, iperm;
-    std::vector<int> fsz, nc;             // per front
-    std::vector<long long> off;           // per front arena offset
+    std::vector<int> fsz, nc, tier, level;    // per front
+    std::vector<long long> off;           // per front offset in its tier's arena
     std::vector<int> front_ptr, front_rows;   // global (new) row indices
     std::vector<int> child_ptr, child_idx;    // children per front
     std::vector<int> asm_ptr, asm_loc;    // per front: CB row -> parent-local index
     std::vector<int> ent_ptr, ent_src, ent_dst;  // A entries: CSR slot -> local pos
+    std::vector<long long> uoff;          // per front offset of its update vector
+    long long upd_total = 0;
+    std::vector<int> group_ptr;           // [num_levels * 3 + 1] into group_fronts
+    std::vector<int> group_fronts;        // fronts of (level, tier) groups
+    std::vector<int> group_max_fsz;       // [num_levels * 3]
     // work figures
     double flops = 0.0;                   // LU flops per scenario
-    double factor_bytes_min = 0.0;        // elements touched once: sum fsz^2
+    double factor_bytes_min = 0.0;        // sum fsz^2
     int max_fsz = 0, max_nc = 0;
     long long lnz = 0;                    // nnz(L) incl. diagonal (symbolic)
 };
 
-LuPlan lower_plan(const Symbolic& s, int n, const int* rp, const int* ci);
+LuPlan lower_plan(const Symbolic& s, int n, const int* rp, const int* ci, int tier0_max = 16,
+                  int tier1_max = 128);
 
 }  // namespace cupf

@@ -1,24 +1,31 @@
 // Batched multifrontal LU for CDNA4 (gfx950).
 //
-// Factor: one workgroup (256 lanes) per scenario walks the fronts in
-// postorder, so no launch per level and no cross-workgroup synchronisation
-// is needed; the batch supplies the parallelism (B workgroups).  Each front
-// is assembled by its owner (zero, original entries, then every child's
-// contribution block gathered in a fixed order: deterministic, no atomics).
-//   * fronts with fsz^2 elements fitting the LDS budget are assembled and
-//     factored entirely in LDS, then written to the arena once;
-//   * larger fronts are factored in place in the arena by a blocked
-//     right-looking LU: the column panel (rows x kb) is staged in LDS and
-//     factored there, U12 is solved column-per-lane, and the trailing block
-//     is updated in 64x64 tiles with 4x4 register micro-tiles.
-// No pivoting: a pivot with |p| <= eps is replaced by sign(p)*eps and the
-// scenario's singular flag is raised (reference GuardedPivot,
-// custom_linear_solver/src/factorize/front_ops.cuh:146-160, but the flag is
-// returned to the caller here).
+// The fronts of one tree level are independent; the host launches one kernel
+// per (level, tier) group, children before parents.  Three tiers:
 //
-// Solve: one wave per scenario.  Each of the four triangular sweeps uses the
-// form that reads the row-major fronts along rows (coalesced): dot products
-// for L and U, axpy over rows of U for U^T and of L for L^T.
+//   tier 0  fsz <= 16: batch-SIMT.  One wave = one front x 64 scenarios, one
+//           lane per scenario; the front lives in the scenario-interleaved
+//           arena (element e of scenario b at (off + e) * Bp + b), so every
+//           access of the wave is one contiguous segment and all control flow
+//           and index loads are wave-uniform.  No synchronisation at all.
+//   tier 1  fsz <= tier1_max: one workgroup per (front, scenario); the front
+//           is assembled and factored in LDS and stored scenario-major.
+//   tier 2  larger: one workgroup per (front, scenario); blocked right-looking
+//           LU in the scenario-major arena with the column panel staged and
+//           factored in LDS, U12 solved column-per-lane and the trailing block
+//           updated in 64x64 tiles of 4x4 register micro-tiles.
+//
+// Assembly is owner-computes: a front zeroes itself, stores its original
+// entries and gathers each child's contribution block in a fixed order
+// (deterministic; no atomics).  No pivoting: |pivot| <= eps becomes
+// sign(pivot)*eps and the scenario's singular flag is raised (reference
+// GuardedPivot, custom_linear_solver/src/factorize/front_ops.cuh:146-160; the
+// flag is returned to the caller here).
+//
+// Workgroup-per-scenario grids are 1-D and decoded so that each of the 8 XCDs
+// (blocks are dispatched round-robin over XCDs) owns a contiguous range of
+// scenarios of a front: neighbouring scenarios share the cache lines of the
+// interleaved arrays inside one XCD's L2.
 #include "kernels.hpp"
 #include "hip_common.hpp"
 
@@ -27,8 +34,10 @@ namespace {
 
 constexpr int kFactorBlock = 256;
 constexpr int kLdsBytes = 64 * 1024;
-constexpr int kKbMax = 16;
+constexpr int kKbMax = 32;
 constexpr int kTile = 64;
+constexpr int kTier0Max = 16;
+constexpr int kXcd = 8;
 
 template <typename T>
 __device__ inline T guard_pivot(T piv, double eps, int static_piv, int& flag) {
@@ -43,345 +52,599 @@ __device__ inline T guard_pivot(T piv, double eps, int static_piv, int& flag) {
     return piv;
 }
 
-template <typename TJ, typename T>
-__global__ __launch_bounds__(kFactorBlock) void k_factor(LuDevice d, const TJ* __restrict__ A,
-                                                         long long a_stride,
-                                                         T* __restrict__ arena_all, double eps,
-                                                         int static_piv, int* __restrict__ sing) {
-    constexpr int kLdsT = kLdsBytes / int(sizeof(T));
-    __shared__ T lds[kLdsT];
-    __shared__ T stage[kKbMax * kTile];
-    __shared__ int sflag;
-    const int b = blockIdx.x;
-    const int tid = threadIdx.x;
-    const TJ* Ab = A + (long long)b * a_stride;
-    T* arena = arena_all + (long long)b * d.arena;
+// Flat block id -> (front index in the group, scenario); see file comment.
+__device__ inline void decode_block(int Bp, int& fi, int& b) {
+    const long long lin = blockIdx.x;
+    const long long per = (long long)gridDim.x / kXcd;   // gridDim.x = count * Bp, Bp % 64 == 0
+    const long long w = (lin % kXcd) * per + lin / kXcd;
+    fi = int(w / Bp);
+    b = int(w % Bp);
+}
+
+// Pointer to element 0 of front c's contribution-block source for scenario b
+// plus the element stride (interleaved tier-0 fronts step by Bp).
+template <typename T>
+__device__ inline const T* front_ptr(const LuDevice& d, const Arenas<T>& ar, int c, int b, int Bp,
+                                     long long& stride) {
+    if (d.tier[c] == 0) {
+        stride = Bp;
+        return ar.a0 + d.off[c] * Bp + b;
+    }
+    stride = 1;
+    return ar.a1 + (long long)b * d.arena1 + d.off[c];
+}
+
+// ===========================================================================
+// Factor, tier 0 (lane per scenario)
+// ===========================================================================
+template <typename T>
+__global__ __launch_bounds__(kLanes) void k_factor_t0(LuDevice d, const int* __restrict__ fronts,
+                                                      const double* __restrict__ A, int Bp,
+                                                      Arenas<T> ar, double eps, int static_piv,
+                                                      int* __restrict__ sing) {
+    const int p = fronts[blockIdx.x];
+    const int b = blockIdx.y * kLanes + threadIdx.x;
+    const int fs = d.fsz[p], ncp = d.nc[p];
+    T* G = ar.a0 + d.off[p] * Bp + b;
+    const long long s = Bp;
+    for (int e = 0; e < fs * fs; ++e) G[e * s] = T(0);
+    for (int e = d.ent_ptr[p]; e < d.ent_ptr[p + 1]; ++e)
+        G[d.ent_dst[e] * s] = T(A[(long long)d.ent_src[e] * Bp + b]);
+    for (int q = d.child_ptr[p]; q < d.child_ptr[p + 1]; ++q) {
+        const int c = d.child_idx[q];
+        const int cfs = d.fsz[c], cnc = d.nc[c], cuc = cfs - cnc;
+        long long cs;
+        const T* C = front_ptr(d, ar, c, b, Bp, cs);
+        const int* loc = d.asm_loc + d.asm_ptr[c];
+        for (int a = 0; a < cuc; ++a) {
+            const int ra = loc[a] * fs;
+            const T* crow = C + (long long)((cnc + a) * cfs + cnc) * cs;
+            for (int bb = 0; bb < cuc; ++bb) G[(ra + loc[bb]) * s] += crow[bb * cs];
+        }
+    }
     int flag = 0;
-    if (tid == 0) sflag = 0;
+    for (int k = 0; k < ncp; ++k) {
+        const T piv = G[(k * fs + k) * s];
+        const T g = guard_pivot(piv, eps, static_piv, flag);
+        if (g != piv) G[(k * fs + k) * s] = g;
+        for (int i = k + 1; i < fs; ++i) {
+            const T l = G[(i * fs + k) * s] / g;
+            G[(i * fs + k) * s] = l;
+            for (int j = k + 1; j < fs; ++j) G[(i * fs + j) * s] -= l * G[(k * fs + j) * s];
+        }
+    }
+    if (flag) atomicOr(sing + b, 1);
+}
 
-    for (int p = 0; p < d.P; ++p) {
-        const int fs = d.fsz[p];
-        const int ncp = d.nc[p];
-        T* G = arena + d.off[p];
-        const bool small = fs * fs <= kLdsT;
-        T* F = small ? lds : G;
-        const int fs2 = fs * fs;
+// ===========================================================================
+// Factor, tier 1 (front in LDS) and tier 2 (blocked, in the arena)
+// ===========================================================================
+template <typename T>
+__device__ inline void assemble_front(const LuDevice& d, const Arenas<T>& ar, const double* A,
+                                      int Bp, int p, int b, int fs, T* F) {
+    const int tid = threadIdx.x;
+    for (int e = tid; e < fs * fs; e += kFactorBlock) F[e] = T(0);
+    __syncthreads();
+    for (int e = d.ent_ptr[p] + tid; e < d.ent_ptr[p + 1]; e += kFactorBlock)
+        F[d.ent_dst[e]] = T(A[(long long)d.ent_src[e] * Bp + b]);
+    __syncthreads();
+    for (int q = d.child_ptr[p]; q < d.child_ptr[p + 1]; ++q) {
+        const int c = d.child_idx[q];
+        const int cfs = d.fsz[c], cnc = d.nc[c], cuc = cfs - cnc;
+        long long cs;
+        const T* C = front_ptr(d, ar, c, b, Bp, cs);
+        const int* loc = d.asm_loc + d.asm_ptr[c];
+        for (int e = tid; e < cuc * cuc; e += kFactorBlock) {
+            const int a = e / cuc, bb = e - a * cuc;
+            F[loc[a] * fs + loc[bb]] += C[(long long)((cnc + a) * cfs + cnc + bb) * cs];
+        }
+        __syncthreads();
+    }
+}
 
-        // ---- assemble ---------------------------------------------------
-        for (int e = tid; e < fs2; e += kFactorBlock) F[e] = T(0);
+template <typename T>
+__global__ __launch_bounds__(kFactorBlock) void k_factor_t1(LuDevice d,
+                                                            const int* __restrict__ fronts,
+                                                            const double* __restrict__ A, int Bp,
+                                                            Arenas<T> ar, double eps,
+                                                            int static_piv,
+                                                            int* __restrict__ sing) {
+    extern __shared__ unsigned char smem_raw[];
+    T* F = reinterpret_cast<T*>(smem_raw);
+    int fi, b;
+    decode_block(Bp, fi, b);
+    const int p = fronts[fi];
+    const int fs = d.fsz[p], ncp = d.nc[p];
+    const int tid = threadIdx.x;
+    assemble_front(d, ar, A, Bp, p, b, fs, F);
+    int flag = 0;
+    for (int k = 0; k < ncp; ++k) {
+        const T piv = F[k * fs + k];
+        const T g = guard_pivot(piv, eps, static_piv, flag);
         __syncthreads();
-        for (int e = d.ent_ptr[p] + tid; e < d.ent_ptr[p + 1]; e += kFactorBlock)
-            F[d.ent_dst[e]] = T(Ab[d.ent_src[e]]);
+        if (tid == 0 && g != piv) F[k * fs + k] = g;
+        const int m = fs - k - 1;
+        for (int i = tid; i < m; i += kFactorBlock) F[(k + 1 + i) * fs + k] /= g;
         __syncthreads();
-        for (int q = d.child_ptr[p]; q < d.child_ptr[p + 1]; ++q) {
-            const int c = d.child_idx[q];
-            const int cfs = d.fsz[c], cnc = d.nc[c], cuc = cfs - cnc;
-            const T* C = arena + d.off[c] + (long long)cnc * cfs + cnc;
-            const int* loc = d.asm_loc + d.asm_ptr[c];
-            for (int e = tid; e < cuc * cuc; e += kFactorBlock) {
-                const int a = e / cuc, bb = e - a * cuc;
-                F[loc[a] * fs + loc[bb]] += C[(long long)a * cfs + bb];
+        for (int e = tid; e < m * m; e += kFactorBlock) {
+            const int i = k + 1 + e / m, j = k + 1 + e % m;
+            F[i * fs + j] -= F[i * fs + k] * F[k * fs + j];
+        }
+        __syncthreads();
+    }
+    T* G = ar.a1 + (long long)b * d.arena1 + d.off[p];
+    for (int e = tid; e < fs * fs; e += kFactorBlock) G[e] = F[e];
+    if (flag) atomicOr(sing + b, 1);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kFactorBlock) void k_factor_t2(LuDevice d,
+                                                            const int* __restrict__ fronts,
+                                                            const double* __restrict__ A, int Bp,
+                                                            Arenas<T> ar, double eps,
+                                                            int static_piv,
+                                                            int* __restrict__ sing) {
+    constexpr int kLdsT = kLdsBytes / int(sizeof(T));
+    constexpr int kKb = kKbMax < kLdsT / 512 ? kKbMax : kLdsT / 512;   // >= 16
+    __shared__ T P[kLdsT];
+    __shared__ T stage[kKb * kTile];
+    int fi, b;
+    decode_block(Bp, fi, b);
+    const int p = fronts[fi];
+    const int fs = d.fsz[p], ncp = d.nc[p];
+    const int tid = threadIdx.x;
+    T* G = ar.a1 + (long long)b * d.arena1 + d.off[p];
+    assemble_front(d, ar, A, Bp, p, b, fs, G);
+    int flag = 0;
+    int kb = kLdsT / fs;
+    if (kb > kKb) kb = kKb;
+    for (int k0 = 0; k0 < ncp; k0 += kb) {
+        const int kbb = min(kb, ncp - k0);
+        const int rows = fs - k0;
+        for (int e = tid; e < rows * kbb; e += kFactorBlock) {
+            const int r = e / kbb, c = e - r * kbb;
+            P[e] = G[(long long)(k0 + r) * fs + k0 + c];
+        }
+        __syncthreads();
+        for (int c = 0; c < kbb; ++c) {
+            const T piv = P[c * kbb + c];
+            const T g = guard_pivot(piv, eps, static_piv, flag);
+            __syncthreads();
+            if (tid == 0 && g != piv) P[c * kbb + c] = g;
+            for (int r = c + 1 + tid; r < rows; r += kFactorBlock) P[r * kbb + c] /= g;
+            __syncthreads();
+            const int w = kbb - c - 1;
+            if (w > 0) {
+                const int nr = rows - c - 1;
+                for (int e = tid; e < nr * w; e += kFactorBlock) {
+                    const int r = c + 1 + e / w, cc = c + 1 + e % w;
+                    P[r * kbb + cc] -= P[r * kbb + c] * P[c * kbb + cc];
+                }
             }
             __syncthreads();
         }
-
-        if (small) {
-            // ---- whole front in LDS: right-looking rank-1 steps ------------
-            for (int k = 0; k < ncp; ++k) {
-                const T piv = F[k * fs + k];
-                const T g = guard_pivot(piv, eps, static_piv, flag);
-                __syncthreads();
-                if (tid == 0 && g != piv) F[k * fs + k] = g;
-                const int m = fs - k - 1;
-                for (int i = tid; i < m; i += kFactorBlock) F[(k + 1 + i) * fs + k] /= g;
-                __syncthreads();
-                for (int e = tid; e < m * m; e += kFactorBlock) {
-                    const int i = k + 1 + e / m, j = k + 1 + e % m;
-                    F[i * fs + j] -= F[i * fs + k] * F[k * fs + j];
-                }
-                __syncthreads();
-            }
-            for (int e = tid; e < fs2; e += kFactorBlock) G[e] = F[e];
-            __syncthreads();
-            continue;
+        for (int e = tid; e < rows * kbb; e += kFactorBlock) {
+            const int r = e / kbb, c = e - r * kbb;
+            G[(long long)(k0 + r) * fs + k0 + c] = P[e];
         }
-
-        // ---- large front, blocked in the arena ---------------------------
-        int kb = kLdsT / fs;
-        if (kb > kKbMax) kb = kKbMax;
-        for (int k0 = 0; k0 < ncp; k0 += kb) {
-            const int kbb = min(kb, ncp - k0);
-            const int rows = fs - k0;
-            T* P = lds;   // rows x kbb, row-major
-            for (int e = tid; e < rows * kbb; e += kFactorBlock) {
-                const int r = e / kbb, c = e - r * kbb;
-                P[e] = G[(long long)(k0 + r) * fs + k0 + c];
+        // U12 = L11^{-1} A12, one column per lane
+        for (int j = k0 + kbb + tid; j < fs; j += kFactorBlock) {
+            T u[kKb];
+#pragma unroll
+            for (int r = 0; r < kKb; ++r)
+                if (r < kbb) u[r] = G[(long long)(k0 + r) * fs + j];
+#pragma unroll
+            for (int r = 1; r < kKb; ++r) {
+                if (r < kbb) {
+                    T acc = u[r];
+#pragma unroll
+                    for (int c = 0; c < kKb; ++c)
+                        if (c < r) acc -= P[r * kbb + c] * u[c];
+                    u[r] = acc;
+                }
             }
-            __syncthreads();
-            for (int c = 0; c < kbb; ++c) {
-                const T piv = P[c * kbb + c];
-                const T g = guard_pivot(piv, eps, static_piv, flag);
-                __syncthreads();
-                if (tid == 0 && g != piv) P[c * kbb + c] = g;
-                for (int r = c + 1 + tid; r < rows; r += kFactorBlock) P[r * kbb + c] /= g;
-                __syncthreads();
-                const int w = kbb - c - 1;
-                if (w > 0) {
-                    const int nr = rows - c - 1;
-                    for (int e = tid; e < nr * w; e += kFactorBlock) {
-                        const int r = c + 1 + e / w, cc = c + 1 + e % w;
-                        P[r * kbb + cc] -= P[r * kbb + c] * P[c * kbb + cc];
-                    }
+#pragma unroll
+            for (int r = 0; r < kKb; ++r)
+                if (r < kbb) G[(long long)(k0 + r) * fs + j] = u[r];
+        }
+        __syncthreads();
+        // trailing update A22 -= L21 U12 over [t0, fs)^2
+        const int t0 = k0 + kbb;
+        const int tn = fs - t0;
+        if (tn > 0) {
+            const int nt = (tn + kTile - 1) / kTile;
+            const int tx = tid & 15, ty = tid >> 4;
+            for (int t = 0; t < nt * nt; ++t) {
+                const int i0 = t0 + (t / nt) * kTile;
+                const int j0 = t0 + (t % nt) * kTile;
+                for (int e = tid; e < kbb * kTile; e += kFactorBlock) {
+                    const int c = e / kTile, jj = e - c * kTile;
+                    stage[e] = (j0 + jj < fs) ? G[(long long)(k0 + c) * fs + j0 + jj] : T(0);
                 }
                 __syncthreads();
-            }
-            for (int e = tid; e < rows * kbb; e += kFactorBlock) {
-                const int r = e / kbb, c = e - r * kbb;
-                G[(long long)(k0 + r) * fs + k0 + c] = P[e];
-            }
-            // U12 = L11^{-1} A12, one column per lane.
-            for (int j = k0 + kbb + tid; j < fs; j += kFactorBlock) {
-                T u[kKbMax];
+                T acc[4][4];
 #pragma unroll
-                for (int r = 0; r < kKbMax; ++r)
-                    if (r < kbb) u[r] = G[(long long)(k0 + r) * fs + j];
+                for (int r = 0; r < 4; ++r)
 #pragma unroll
-                for (int r = 1; r < kKbMax; ++r) {
-                    if (r < kbb) {
-                        T acc = u[r];
-#pragma unroll
-                        for (int c = 0; c < kKbMax; ++c)
-                            if (c < r) acc -= P[r * kbb + c] * u[c];
-                        u[r] = acc;
-                    }
-                }
-#pragma unroll
-                for (int r = 0; r < kKbMax; ++r)
-                    if (r < kbb) G[(long long)(k0 + r) * fs + j] = u[r];
-            }
-            __syncthreads();
-            // Trailing update A22 -= L21 U12 over [t0, fs)^2.
-            const int t0 = k0 + kbb;
-            const int tn = fs - t0;
-            if (tn > 0) {
-                const int nt = (tn + kTile - 1) / kTile;
-                const int tx = tid & 15, ty = tid >> 4;
-                for (int t = 0; t < nt * nt; ++t) {
-                    const int i0 = t0 + (t / nt) * kTile;
-                    const int j0 = t0 + (t % nt) * kTile;
-                    for (int e = tid; e < kbb * kTile; e += kFactorBlock) {
-                        const int c = e / kTile, jj = e - c * kTile;
-                        stage[e] = (j0 + jj < fs) ? G[(long long)(k0 + c) * fs + j0 + jj] : T(0);
-                    }
-                    __syncthreads();
-                    T acc[4][4];
-#pragma unroll
-                    for (int r = 0; r < 4; ++r)
-#pragma unroll
-                        for (int q = 0; q < 4; ++q) acc[r][q] = T(0);
-                    for (int c = 0; c < kbb; ++c) {
-                        T l[4], u[4];
-#pragma unroll
-                        for (int r = 0; r < 4; ++r) {
-                            const int i = i0 + ty + 16 * r;
-                            l[r] = (i < fs) ? P[(i - k0) * kbb + c] : T(0);
-                        }
-#pragma unroll
-                        for (int q = 0; q < 4; ++q) u[q] = stage[c * kTile + tx + 16 * q];
-#pragma unroll
-                        for (int r = 0; r < 4; ++r)
-#pragma unroll
-                            for (int q = 0; q < 4; ++q) acc[r][q] += l[r] * u[q];
-                    }
+                    for (int q = 0; q < 4; ++q) acc[r][q] = T(0);
+                for (int c = 0; c < kbb; ++c) {
+                    T l[4], u[4];
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
                         const int i = i0 + ty + 16 * r;
-                        if (i >= fs) continue;
-#pragma unroll
-                        for (int q = 0; q < 4; ++q) {
-                            const int j = j0 + tx + 16 * q;
-                            if (j < fs) G[(long long)i * fs + j] -= acc[r][q];
-                        }
+                        l[r] = (i < fs) ? P[(i - k0) * kbb + c] : T(0);
                     }
-                    __syncthreads();
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) u[q] = stage[c * kTile + tx + 16 * q];
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) acc[r][q] += l[r] * u[q];
                 }
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int i = i0 + ty + 16 * r;
+                    if (i >= fs) continue;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const int j = j0 + tx + 16 * q;
+                        if (j < fs) G[(long long)i * fs + j] -= acc[r][q];
+                    }
+                }
+                __syncthreads();
             }
-            __syncthreads();
         }
+        __syncthreads();
     }
-    if (flag) atomicOr(&sflag, 1);
-    __syncthreads();
-    if (tid == 0) sing[b] = sflag;
+    if (flag) atomicOr(sing + b, 1);
 }
 
-// ---------------------------------------------------------------------------
-// Solve: one 64-lane wave per scenario (block == wave, so __syncthreads is a
-// wave barrier).
-// ---------------------------------------------------------------------------
-template <typename T>
-__device__ inline T wave_sum(T v) {
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
-
-// out[i] -= sum_k M[i*ld + k] * v[k] for i < nrow, k < ncol.
-template <typename T>
-__device__ inline void gemv_rows(const T* __restrict__ M, int ld, int nrow, int ncol,
-                                 const T* v, T* out, int lane) {
-    if (ncol >= 32) {
-        for (int i = 0; i < nrow; ++i) {
-            T part = T(0);
-            for (int k = lane; k < ncol; k += 64) part += M[(long long)i * ld + k] * v[k];
-            part = wave_sum(part);
-            if (lane == 0) out[i] -= part;
+// ===========================================================================
+// Solve, tier 0 (lane per scenario): w kept in LDS as w[l][lane]
+// ===========================================================================
+template <typename T, bool TRANS>
+__global__ __launch_bounds__(kLanes) void k_fwd_t0(LuDevice d, const int* __restrict__ fronts,
+                                                   Arenas<T> ar, const double* __restrict__ rhs,
+                                                   T* __restrict__ y, T* __restrict__ upd,
+                                                   int Bp) {
+    __shared__ T wsh[kTier0Max * kLanes];
+    const int lane = threadIdx.x;
+    T* w = wsh + lane;
+    const int p = fronts[blockIdx.x];
+    const int b = blockIdx.y * kLanes + lane;
+    const int fs = d.fsz[p], ncp = d.nc[p], uc = fs - ncp;
+    const int* fr = d.front_rows + d.front_ptr[p];
+    const T* G = ar.a0 + d.off[p] * Bp + b;
+    const long long s = Bp;
+    for (int l = 0; l < fs; ++l)
+        w[l * kLanes] = (l < ncp) ? T(rhs[(long long)d.perm[fr[l]] * Bp + b]) : T(0);
+    for (int q = d.child_ptr[p]; q < d.child_ptr[p + 1]; ++q) {
+        const int c = d.child_idx[q];
+        const int cuc = d.fsz[c] - d.nc[c];
+        const int* loc = d.asm_loc + d.asm_ptr[c];
+        const T* u = upd + d.uoff[c] * Bp + b;
+        for (int a = 0; a < cuc; ++a) w[loc[a] * kLanes] += u[a * s];
+    }
+    for (int k = 0; k < ncp; ++k) {
+        T yk = w[k * kLanes];
+        if (TRANS) {
+            yk /= G[(k * fs + k) * s];
+            w[k * kLanes] = yk;
         }
-    } else {
-        for (int i = lane; i < nrow; i += 64) {
+        for (int i = k + 1; i < fs; ++i)
+            w[i * kLanes] -= (TRANS ? G[(k * fs + i) * s] : G[(i * fs + k) * s]) * yk;
+    }
+    for (int k = 0; k < ncp; ++k) y[(long long)fr[k] * Bp + b] = w[k * kLanes];
+    T* u = upd + d.uoff[p] * Bp + b;
+    for (int a = 0; a < uc; ++a) u[a * s] = w[(ncp + a) * kLanes];
+}
+
+template <typename T, bool TRANS>
+__global__ __launch_bounds__(kLanes) void k_bwd_t0(LuDevice d, const int* __restrict__ fronts,
+                                                   Arenas<T> ar, T* __restrict__ y, int Bp) {
+    __shared__ T wsh[kTier0Max * kLanes];
+    const int lane = threadIdx.x;
+    T* w = wsh + lane;
+    const int p = fronts[blockIdx.x];
+    const int b = blockIdx.y * kLanes + lane;
+    const int fs = d.fsz[p], ncp = d.nc[p];
+    const int* fr = d.front_rows + d.front_ptr[p];
+    const T* G = ar.a0 + d.off[p] * Bp + b;
+    const long long s = Bp;
+    for (int l = 0; l < fs; ++l) w[l * kLanes] = y[(long long)fr[l] * Bp + b];
+    for (int k = 0; k < ncp; ++k) {
+        T acc = T(0);
+        for (int j = ncp; j < fs; ++j)
+            acc += (TRANS ? G[(j * fs + k) * s] : G[(k * fs + j) * s]) * w[j * kLanes];
+        w[k * kLanes] -= acc;
+    }
+    for (int i = ncp - 1; i >= 0; --i) {
+        T acc = w[i * kLanes];
+        for (int k = i + 1; k < ncp; ++k)
+            acc -= (TRANS ? G[(k * fs + i) * s] : G[(i * fs + k) * s]) * w[k * kLanes];
+        w[i * kLanes] = TRANS ? acc : acc / G[(i * fs + i) * s];
+    }
+    for (int k = 0; k < ncp; ++k) y[(long long)fr[k] * Bp + b] = w[k * kLanes];
+}
+
+// ===========================================================================
+// Solve, tiers 1-2: one wave per (front, scenario), blocked dense sweeps
+// ===========================================================================
+// Unit lower L11 then w[nc..fs) -= L21 w[0..nc)   (G row-major, ld fs)
+template <typename T>
+__device__ inline void sweep_fwd_L(const T* __restrict__ G, int fs, int nc, T* w, int lane) {
+    for (int i0 = 0; i0 < nc; i0 += kLanes) {
+        const int nb = min(kLanes, nc - i0);
+        const int i = i0 + lane;
+        T wi = T(0);
+        if (lane < nb) {
+            const T* row = G + (long long)i * fs;
             T acc = T(0);
-            for (int k = 0; k < ncol; ++k) acc += M[(long long)i * ld + k] * v[k];
-            out[i] -= acc;
+            for (int k = 0; k < i0; ++k) acc += row[k] * w[k];
+            wi = w[i] - acc;
         }
+        for (int k = 0; k < nb; ++k) {
+            const T yk = __shfl(wi, k, kLanes);
+            if (lane > k && lane < nb) wi -= G[(long long)i * fs + i0 + k] * yk;
+        }
+        if (lane < nb) w[i] = wi;
+        __syncthreads();
+    }
+    for (int r = nc + lane; r < fs; r += kLanes) {
+        const T* row = G + (long long)r * fs;
+        T acc = T(0);
+        for (int k = 0; k < nc; ++k) acc += row[k] * w[k];
+        w[r] -= acc;
+    }
+    __syncthreads();
+}
+
+// U11^T (lower, non-unit) then w[nc..fs) -= U12^T w[0..nc)
+template <typename T>
+__device__ inline void sweep_fwd_Ut(const T* __restrict__ G, int fs, int nc, T* w, int lane) {
+    for (int i0 = 0; i0 < nc; i0 += kLanes) {
+        const int nb = min(kLanes, nc - i0);
+        const int i = i0 + lane;
+        T wi = T(0);
+        if (lane < nb) {
+            T acc = T(0);
+            for (int k = 0; k < i0; ++k) acc += G[(long long)k * fs + i] * w[k];
+            wi = w[i] - acc;
+        }
+        for (int k = 0; k < nb; ++k) {
+            const T dk = G[(long long)(i0 + k) * fs + i0 + k];
+            const T yk = __shfl(wi, k, kLanes) / dk;
+            if (lane == k) wi = yk;
+            if (lane > k && lane < nb) wi -= G[(long long)(i0 + k) * fs + i] * yk;
+        }
+        if (lane < nb) w[i] = wi;
+        __syncthreads();
+    }
+    for (int r = nc + lane; r < fs; r += kLanes) {
+        T acc = T(0);
+        for (int k = 0; k < nc; ++k) acc += G[(long long)k * fs + r] * w[k];
+        w[r] -= acc;
+    }
+    __syncthreads();
+}
+
+// w[0..nc) -= U12 w[nc..fs) then U11 back substitution
+template <typename T>
+__device__ inline void sweep_bwd_U(const T* __restrict__ G, int fs, int nc, T* w, int lane) {
+    for (int k = lane; k < nc; k += kLanes) {
+        const T* row = G + (long long)k * fs;
+        T acc = T(0);
+        for (int j = nc; j < fs; ++j) acc += row[j] * w[j];
+        w[k] -= acc;
+    }
+    __syncthreads();
+    for (int i1 = nc; i1 > 0; i1 -= kLanes) {
+        const int i0 = max(0, i1 - kLanes);
+        const int nb = i1 - i0;
+        const int i = i0 + lane;
+        T wi = T(0);
+        if (lane < nb) {
+            const T* row = G + (long long)i * fs;
+            T acc = T(0);
+            for (int k = i1; k < nc; ++k) acc += row[k] * w[k];
+            wi = w[i] - acc;
+        }
+        for (int k = nb - 1; k >= 0; --k) {
+            const T dk = G[(long long)(i0 + k) * fs + i0 + k];
+            const T xk = __shfl(wi, k, kLanes) / dk;
+            if (lane == k) wi = xk;
+            if (lane < k) wi -= G[(long long)i * fs + i0 + k] * xk;
+        }
+        if (lane < nb) w[i] = wi;
+        __syncthreads();
     }
 }
 
-template <typename T, typename TR, typename TX, bool TRANS>
-__global__ __launch_bounds__(64) void k_solve(LuDevice d, const T* __restrict__ arena_all,
-                                              const TR* __restrict__ rhs, long long rs,
-                                              T* __restrict__ work_all, T* __restrict__ upd_all,
-                                              TX* __restrict__ x, long long xs) {
+// w[0..nc) -= L21^T w[nc..fs) then unit L11^T back substitution
+template <typename T>
+__device__ inline void sweep_bwd_Lt(const T* __restrict__ G, int fs, int nc, T* w, int lane) {
+    for (int k = lane; k < nc; k += kLanes) {
+        T acc = T(0);
+        for (int j = nc; j < fs; ++j) acc += G[(long long)j * fs + k] * w[j];
+        w[k] -= acc;
+    }
+    __syncthreads();
+    for (int i1 = nc; i1 > 0; i1 -= kLanes) {
+        const int i0 = max(0, i1 - kLanes);
+        const int nb = i1 - i0;
+        const int i = i0 + lane;
+        T wi = T(0);
+        if (lane < nb) {
+            T acc = T(0);
+            for (int k = i1; k < nc; ++k) acc += G[(long long)k * fs + i] * w[k];
+            wi = w[i] - acc;
+        }
+        for (int k = nb - 1; k >= 0; --k) {
+            const T xk = __shfl(wi, k, kLanes);
+            if (lane < k) wi -= G[(long long)(i0 + k) * fs + i] * xk;
+        }
+        if (lane < nb) w[i] = wi;
+        __syncthreads();
+    }
+}
+
+template <typename T, bool TRANS>
+__global__ __launch_bounds__(kLanes) void k_fwd_t12(LuDevice d, const int* __restrict__ fronts,
+                                                    Arenas<T> ar, const double* __restrict__ rhs,
+                                                    T* __restrict__ y, T* __restrict__ upd,
+                                                    int Bp) {
     extern __shared__ unsigned char smem_raw[];
     T* w = reinterpret_cast<T*>(smem_raw);
-    const int b = blockIdx.x;
     const int lane = threadIdx.x;
-    const T* arena = arena_all + (long long)b * d.arena;
-    T* y = work_all + (long long)b * d.n;
-    T* upd = upd_all + (long long)b * d.upd_total;
-    const TR* rb = rhs + (long long)b * rs;
-
-    // ---- forward: L (or U^T) -------------------------------------------
-    for (int p = 0; p < d.P; ++p) {
-        const int fs = d.fsz[p], ncp = d.nc[p], uc = fs - ncp;
-        const int* fr = d.front_rows + d.front_ptr[p];
-        const T* G = arena + d.off[p];
-        for (int l = lane; l < fs; l += 64) w[l] = (l < ncp) ? T(rb[d.perm[fr[l]]]) : T(0);
-        __syncthreads();
-        for (int q = d.child_ptr[p]; q < d.child_ptr[p + 1]; ++q) {
-            const int c = d.child_idx[q];
-            const int cuc = d.fsz[c] - d.nc[c];
-            const int* loc = d.asm_loc + d.asm_ptr[c];
-            const T* uc_src = upd + d.uoff[c];
-            for (int a = lane; a < cuc; a += 64) w[loc[a]] += uc_src[a];
-            __syncthreads();
-        }
-        if (!TRANS) {
-            // unit lower L11, dot form along rows
-            for (int i = 1; i < ncp; ++i) {
-                T part = T(0);
-                for (int k = lane; k < i; k += 64) part += G[(long long)i * fs + k] * w[k];
-                part = wave_sum(part);
-                if (lane == 0) w[i] -= part;
-                __syncthreads();
-            }
-            if (uc > 0) gemv_rows(G + (long long)ncp * fs, fs, uc, ncp, w, w + ncp, lane);
-        } else {
-            // U^T: column k of U^T is row k of U (contiguous)
-            for (int k = 0; k < ncp; ++k) {
-                const T yk = w[k] / G[(long long)k * fs + k];
-                __syncthreads();
-                if (lane == 0) w[k] = yk;
-                for (int i = k + 1 + lane; i < fs; i += 64) w[i] -= G[(long long)k * fs + i] * yk;
-                __syncthreads();
-            }
-        }
-        __syncthreads();
-        for (int k = lane; k < ncp; k += 64) y[fr[k]] = w[k];
-        for (int a = lane; a < uc; a += 64) upd[d.uoff[p] + a] = w[ncp + a];
+    int fi, b;
+    decode_block(Bp, fi, b);
+    const int p = fronts[fi];
+    const int fs = d.fsz[p], ncp = d.nc[p], uc = fs - ncp;
+    const int* fr = d.front_rows + d.front_ptr[p];
+    const T* G = ar.a1 + (long long)b * d.arena1 + d.off[p];
+    for (int l = lane; l < fs; l += kLanes)
+        w[l] = (l < ncp) ? T(rhs[(long long)d.perm[fr[l]] * Bp + b]) : T(0);
+    __syncthreads();
+    for (int q = d.child_ptr[p]; q < d.child_ptr[p + 1]; ++q) {
+        const int c = d.child_idx[q];
+        const int cuc = d.fsz[c] - d.nc[c];
+        const int* loc = d.asm_loc + d.asm_ptr[c];
+        const T* u = upd + d.uoff[c] * Bp + b;
+        for (int a = lane; a < cuc; a += kLanes) w[loc[a]] += u[(long long)a * Bp];
         __syncthreads();
     }
+    if (TRANS) sweep_fwd_Ut(G, fs, ncp, w, lane);
+    else sweep_fwd_L(G, fs, ncp, w, lane);
+    for (int k = lane; k < ncp; k += kLanes) y[(long long)fr[k] * Bp + b] = w[k];
+    T* u = upd + d.uoff[p] * Bp + b;
+    for (int a = lane; a < uc; a += kLanes) u[(long long)a * Bp] = w[ncp + a];
+}
 
-    // ---- backward: U (or L^T); y is overwritten by x in place -----------
-    for (int p = d.P - 1; p >= 0; --p) {
-        const int fs = d.fsz[p], ncp = d.nc[p], uc = fs - ncp;
-        const int* fr = d.front_rows + d.front_ptr[p];
-        const T* G = arena + d.off[p];
-        for (int l = lane; l < fs; l += 64) w[l] = y[fr[l]];
-        __syncthreads();
-        if (!TRANS) {
-            if (uc > 0) gemv_rows(G + ncp, fs, ncp, uc, w + ncp, w, lane);
-            __syncthreads();
-            for (int i = ncp - 1; i >= 0; --i) {
-                T part = T(0);
-                for (int k = i + 1 + lane; k < ncp; k += 64) part += G[(long long)i * fs + k] * w[k];
-                part = wave_sum(part);
-                if (lane == 0) w[i] = (w[i] - part) / G[(long long)i * fs + i];
-                __syncthreads();
-            }
-        } else {
-            // L^T: w_k -= sum_a L(ncp+a, k) x_a, axpy over rows of L21
-            for (int a = 0; a < uc; ++a) {
-                const T xa = w[ncp + a];
-                const T* row = G + (long long)(ncp + a) * fs;
-                for (int k = lane; k < ncp; k += 64) w[k] -= row[k] * xa;
-            }
-            __syncthreads();
-            for (int j = ncp - 1; j >= 1; --j) {
-                const T xj = w[j];
-                const T* row = G + (long long)j * fs;
-                for (int i = lane; i < j; i += 64) w[i] -= row[i] * xj;
-                __syncthreads();
-            }
-        }
-        __syncthreads();
-        for (int k = lane; k < ncp; k += 64) y[fr[k]] = w[k];
-        __syncthreads();
-    }
-    for (int k = lane; k < d.n; k += 64) x[(long long)b * xs + d.perm[k]] = TX(y[k]);
+template <typename T, bool TRANS>
+__global__ __launch_bounds__(kLanes) void k_bwd_t12(LuDevice d, const int* __restrict__ fronts,
+                                                    Arenas<T> ar, T* __restrict__ y, int Bp) {
+    extern __shared__ unsigned char smem_raw[];
+    T* w = reinterpret_cast<T*>(smem_raw);
+    const int lane = threadIdx.x;
+    int fi, b;
+    decode_block(Bp, fi, b);
+    const int p = fronts[fi];
+    const int fs = d.fsz[p], ncp = d.nc[p];
+    const int* fr = d.front_rows + d.front_ptr[p];
+    const T* G = ar.a1 + (long long)b * d.arena1 + d.off[p];
+    for (int l = lane; l < fs; l += kLanes) w[l] = y[(long long)fr[l] * Bp + b];
+    __syncthreads();
+    if (TRANS) sweep_bwd_Lt(G, fs, ncp, w, lane);
+    else sweep_bwd_U(G, fs, ncp, w, lane);
+    for (int k = lane; k < ncp; k += kLanes) y[(long long)fr[k] * Bp + b] = w[k];
+}
+
+template <typename T, typename TX>
+__global__ __launch_bounds__(256) void k_scatter(const int* __restrict__ perm, int n,
+                                                 const T* __restrict__ y, int Bp,
+                                                 TX* __restrict__ x) {
+    const int lane = threadIdx.x & (kLanes - 1), w = threadIdx.x / kLanes;
+    const int k = blockIdx.x * 4 + w;
+    const int b = blockIdx.y * kLanes + lane;
+    if (k >= n) return;
+    x[(long long)perm[k] * Bp + b] = TX(y[(long long)k * Bp + b]);
 }
 
 }  // namespace
 
-template <typename TJ, typename T>
-void lu_factor(const LuDevice& d, const TJ* A, long long a_stride, int B, T* arena,
-               double pivot_eps, int static_pivoting, int* sing, hipStream_t s) {
-    constexpr int kLdsT = kLdsBytes / int(sizeof(T));
-    if (d.max_fsz > kLdsT)
-        throw std::runtime_error("lu_factor: front larger than the LDS panel budget");
-    hipLaunchKernelGGL((k_factor<TJ, T>), dim3(B), dim3(kFactorBlock), 0, s, d, A, a_stride,
-                       arena, pivot_eps, static_pivoting, sing);
+template <typename T>
+void lu_factor_group(const LuDevice& d, int tier, const int* fronts, int count, int max_fsz,
+                     const double* A, int Bp, Arenas<T> ar, double pivot_eps, int static_piv,
+                     int* sing, hipStream_t s) {
+    if (count <= 0) return;
+    if (tier == 0) {
+        if (max_fsz > kTier0Max) throw std::runtime_error("lu_factor_group: tier-0 front too large");
+        hipLaunchKernelGGL(k_factor_t0<T>, dim3(count, Bp / kLanes), dim3(kLanes), 0, s, d, fronts,
+                           A, Bp, ar, pivot_eps, static_piv, sing);
+    } else if (tier == 1) {
+        const size_t shmem = size_t(max_fsz) * max_fsz * sizeof(T);
+        if (shmem > size_t(kLdsBytes)) throw std::runtime_error("lu_factor_group: tier-1 front exceeds LDS");
+        hipLaunchKernelGGL(k_factor_t1<T>, dim3(unsigned(count) * unsigned(Bp)), dim3(kFactorBlock),
+                           shmem, s, d, fronts, A, Bp, ar, pivot_eps, static_piv, sing);
+    } else {
+        if (max_fsz > kLdsBytes / int(sizeof(T)))
+            throw std::runtime_error("lu_factor_group: front exceeds the LDS panel budget");
+        hipLaunchKernelGGL(k_factor_t2<T>, dim3(unsigned(count) * unsigned(Bp)), dim3(kFactorBlock),
+                           0, s, d, fronts, A, Bp, ar, pivot_eps, static_piv, sing);
+    }
     CUPF_HIP(hipGetLastError());
 }
 
-template <typename T, typename TR, typename TX>
-void lu_solve(const LuDevice& d, const T* arena, const TR* rhs, long long rhs_stride, int B,
-              T* work, T* upd, TX* x, long long x_stride, bool transpose, hipStream_t s) {
-    const size_t shmem = size_t(d.max_fsz) * sizeof(T);
-    if (shmem > 64 * 1024) throw std::runtime_error("lu_solve: front exceeds the LDS budget");
-    if (transpose)
-        hipLaunchKernelGGL((k_solve<T, TR, TX, true>), dim3(B), dim3(64), shmem, s, d, arena, rhs,
-                           rhs_stride, work, upd, x, x_stride);
-    else
-        hipLaunchKernelGGL((k_solve<T, TR, TX, false>), dim3(B), dim3(64), shmem, s, d, arena,
-                           rhs, rhs_stride, work, upd, x, x_stride);
+template <typename T>
+void lu_solve_forward_group(const LuDevice& d, int tier, const int* fronts, int count,
+                            int max_fsz, Arenas<T> ar, const double* rhs, T* y, T* upd, int Bp,
+                            bool transpose, hipStream_t s) {
+    if (count <= 0) return;
+    if (tier == 0) {
+        if (transpose)
+            hipLaunchKernelGGL((k_fwd_t0<T, true>), dim3(count, Bp / kLanes), dim3(kLanes), 0, s, d,
+                               fronts, ar, rhs, y, upd, Bp);
+        else
+            hipLaunchKernelGGL((k_fwd_t0<T, false>), dim3(count, Bp / kLanes), dim3(kLanes), 0, s,
+                               d, fronts, ar, rhs, y, upd, Bp);
+    } else {
+        const size_t shmem = size_t(max_fsz) * sizeof(T);
+        const dim3 grid(unsigned(count) * unsigned(Bp));
+        if (transpose)
+            hipLaunchKernelGGL((k_fwd_t12<T, true>), grid, dim3(kLanes), shmem, s, d, fronts, ar,
+                               rhs, y, upd, Bp);
+        else
+            hipLaunchKernelGGL((k_fwd_t12<T, false>), grid, dim3(kLanes), shmem, s, d, fronts, ar,
+                               rhs, y, upd, Bp);
+    }
     CUPF_HIP(hipGetLastError());
 }
 
-template void lu_factor<double, double>(const LuDevice&, const double*, long long, int, double*,
-                                        double, int, int*, hipStream_t);
-template void lu_factor<double, float>(const LuDevice&, const double*, long long, int, float*,
-                                       double, int, int*, hipStream_t);
-template void lu_factor<float, float>(const LuDevice&, const float*, long long, int, float*,
-                                      double, int, int*, hipStream_t);
-template void lu_solve<double, double, double>(const LuDevice&, const double*, const double*,
-                                               long long, int, double*, double*, double*,
-                                               long long, bool, hipStream_t);
-template void lu_solve<double, double, float>(const LuDevice&, const double*, const double*,
-                                              long long, int, double*, double*, float*,
-                                              long long, bool, hipStream_t);
-template void lu_solve<float, double, float>(const LuDevice&, const float*, const double*,
-                                             long long, int, float*, float*, float*, long long,
-                                             bool, hipStream_t);
-template void lu_solve<float, double, double>(const LuDevice&, const float*, const double*,
-                                              long long, int, float*, float*, double*, long long,
-                                              bool, hipStream_t);
+template <typename T>
+void lu_solve_backward_group(const LuDevice& d, int tier, const int* fronts, int count,
+                             int max_fsz, Arenas<T> ar, T* y, int Bp, bool transpose,
+                             hipStream_t s) {
+    if (count <= 0) return;
+    if (tier == 0) {
+        if (transpose)
+            hipLaunchKernelGGL((k_bwd_t0<T, true>), dim3(count, Bp / kLanes), dim3(kLanes), 0, s, d,
+                               fronts, ar, y, Bp);
+        else
+            hipLaunchKernelGGL((k_bwd_t0<T, false>), dim3(count, Bp / kLanes), dim3(kLanes), 0, s,
+                               d, fronts, ar, y, Bp);
+    } else {
+        const size_t shmem = size_t(max_fsz) * sizeof(T);
+        const dim3 grid(unsigned(count) * unsigned(Bp));
+        if (transpose)
+            hipLaunchKernelGGL((k_bwd_t12<T, true>), grid, dim3(kLanes), shmem, s, d, fronts, ar, y,
+                               Bp);
+        else
+            hipLaunchKernelGGL((k_bwd_t12<T, false>), grid, dim3(kLanes), shmem, s, d, fronts, ar,
+                               y, Bp);
+    }
+    CUPF_HIP(hipGetLastError());
+}
+
+template <typename T, typename TX>
+void lu_solve_scatter(const LuDevice& d, const T* y, int Bp, TX* x, hipStream_t s) {
+    const dim3 grid((d.n + 3) / 4, Bp / kLanes);
+    hipLaunchKernelGGL((k_scatter<T, TX>), grid, dim3(256), 0, s, d.perm, d.n, y, Bp, x);
+    CUPF_HIP(hipGetLastError());
+}
+
+#define CUPF_LU_INST(T)                                                                          \
+    template void lu_factor_group<T>(const LuDevice&, int, const int*, int, int, const double*,  \
+                                     int, Arenas<T>, double, int, int*, hipStream_t);            \
+    template void lu_solve_forward_group<T>(const LuDevice&, int, const int*, int, int,           \
+                                            Arenas<T>, const double*, T*, T*, int, bool,         \
+                                            hipStream_t);                                         \
+    template void lu_solve_backward_group<T>(const LuDevice&, int, const int*, int, int,          \
+                                             Arenas<T>, T*, int, bool, hipStream_t);             \
+    template void lu_solve_scatter<T, double>(const LuDevice&, const T*, int, double*,            \
+                                              hipStream_t);                                       \
+    template void lu_solve_scatter<T, float>(const LuDevice&, const T*, int, float*, hipStream_t);
+
+CUPF_LU_INST(double)
+CUPF_LU_INST(float)
 
 }  // namespace cupf::kern

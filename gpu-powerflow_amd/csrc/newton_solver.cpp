@@ -107,20 +107,20 @@ struct NewtonSolver::Impl {
     LuPlan plan;
     std::vector<int> zero_slots;
     std::vector<int> jt_rp, jt_ci, jt_map;
-    std::vector<long long> uoff;
 
     // device structure
     DeviceBuffer d_yp, d_yi, d_y, d_row_pvpq, d_row_pq, d_m11, d_m21, d_m12, d_m22, d_zero,
         d_row_bus, d_row_isq, d_jrp, d_jci, d_jtrp, d_jtci, d_jtmap;
-    DeviceBuffer d_fsz, d_nc, d_off, d_uoff, d_fptr, d_frows, d_cptr, d_cidx, d_aptr, d_aloc,
-        d_eptr, d_esrc, d_edst, d_perm;
+    DeviceBuffer d_fsz, d_nc, d_tier, d_off, d_uoff, d_fptr, d_frows, d_cptr, d_cidx, d_aptr,
+        d_aloc, d_eptr, d_esrc, d_edst, d_perm, d_groups;
     kern::NrDevice nr;
     kern::LuDevice lu;
 
-    // batch buffers
-    int capB = 0;
-    DeviceBuffer V, Va, Vm, S, F, normF, bmax, J, arena, work, upd, dx, ddx, res, sing, lam, g,
-        io_a, io_b;
+    // batch buffers (scenario-interleaved, Bp = batch padded to 64)
+    int capBp = 0, capB = 0;
+    int B = 0, Bp = 0;
+    DeviceBuffer V, Va, Vm, S, F, normF, bmax, J, arena0, arena1, ywork, upd, dx, ddx, res, sing,
+        lam, g, stage_a, stage_b;
     hipStream_t own = nullptr;
     hipStream_t s = nullptr;   // stream of the current call
 
@@ -132,95 +132,144 @@ struct NewtonSolver::Impl {
     size_t tsize() const { return front64 ? sizeof(double) : sizeof(float); }
     size_t xsize() const { return dx32 ? sizeof(float) : sizeof(double); }
 
-    void ensure_batch(int B) {
-        if (B <= capB) return;
-        const size_t nb = size_t(B);
-        V.allocate(nb * n * sizeof(double2));
-        Va.allocate(nb * n * sizeof(double));
-        Vm.allocate(nb * n * sizeof(double));
-        S.allocate(nb * n * sizeof(double2));
-        F.allocate(nb * js.dim * sizeof(double));
-        normF.allocate(nb * sizeof(double));
-        bmax.allocate(sizeof(double));
-        J.allocate(nb * js.nnz * sizeof(double));
-        arena.allocate(nb * size_t(plan.arena) * tsize());
-        work.allocate(nb * js.dim * tsize());
-        upd.allocate(nb * std::max<size_t>(1, size_t(lu.upd_total)) * tsize());
-        dx.allocate(nb * js.dim * xsize());
-        ddx.allocate(nb * js.dim * xsize());
-        res.allocate(nb * js.dim * sizeof(double));
-        sing.allocate(nb * sizeof(int));
-        lam.allocate(nb * js.dim * sizeof(double));
-        g.allocate(nb * js.dim * sizeof(double));
-        io_a.allocate(nb * n * sizeof(double2));
-        io_b.allocate(nb * n * sizeof(double2));
-        capB = B;
+    void ensure_batch(int batch) {
+        B = batch;
+        Bp = kern::pad_batch(batch);
+        if (Bp > capBp) {
+            const size_t nb = size_t(Bp);
+            V.allocate(nb * n * sizeof(double2));
+            Va.allocate(nb * n * sizeof(double));
+            Vm.allocate(nb * n * sizeof(double));
+            S.allocate(nb * n * sizeof(double2));
+            F.allocate(nb * js.dim * sizeof(double));
+            normF.allocate(nb * sizeof(double));
+            bmax.allocate(sizeof(double));
+            J.allocate(nb * js.nnz * sizeof(double));
+            arena0.allocate(nb * std::max<size_t>(1, size_t(plan.arena0)) * tsize());
+            arena1.allocate(nb * std::max<size_t>(1, size_t(plan.arena1)) * tsize());
+            ywork.allocate(nb * js.dim * tsize());
+            upd.allocate(nb * std::max<size_t>(1, size_t(plan.upd_total)) * tsize());
+            dx.allocate(nb * js.dim * xsize());
+            ddx.allocate(nb * js.dim * xsize());
+            res.allocate(nb * js.dim * sizeof(double));
+            sing.allocate(nb * sizeof(int));
+            lam.allocate(nb * js.dim * sizeof(double));
+            g.allocate(nb * js.dim * sizeof(double));
+            capBp = Bp;
+        }
+        if (batch > capB) {
+            const size_t big = size_t(batch) * std::max(n, js.dim) * sizeof(double2);
+            stage_a.allocate(big);
+            stage_b.allocate(big);
+            capB = batch;
+        }
     }
 
-    // ---- one linear step: factor J (current values) and solve J x = rhs ----
-    void factor(int B) {
-        if (front64)
-            kern::lu_factor<double, double>(lu, J.as<double>(), js.nnz, B, arena.as<double>(),
-                                            piv_eps, static_piv ? 1 : 0, sing.as<int>(), s);
-        else
-            kern::lu_factor<double, float>(lu, J.as<double>(), js.nnz, B, arena.as<float>(),
-                                           piv_eps, static_piv ? 1 : 0, sing.as<int>(), s);
+    const int* group(int level, int tier, int& count, int& max_fsz) const {
+        const int gi = level * 3 + tier;
+        count = plan.group_ptr[gi + 1] - plan.group_ptr[gi];
+        max_fsz = plan.group_max_fsz[gi];
+        return d_groups.as<int>() + plan.group_ptr[gi];
+    }
+
+    // ---- factor the current J values ------------------------------------
+    template <typename T>
+    void factor_t(kern::Arenas<T> ar) {
+        for (int L = 0; L < plan.num_levels; ++L)
+            for (int t = 0; t < 3; ++t) {
+                int cnt, mf;
+                const int* f = group(L, t, cnt, mf);
+                kern::lu_factor_group<T>(lu, t, f, cnt, mf, J.as<double>(), Bp, ar, piv_eps,
+                                         static_piv ? 1 : 0, sing.as<int>(), s);
+            }
+    }
+    void factor() {
+        if (front64) factor_t<double>({arena0.as<double>(), arena1.as<double>()});
+        else factor_t<float>({arena0.as<float>(), arena1.as<float>()});
         ++st.factorizations;
     }
 
-    // rhs (FP64, [B][dim]) -> x; transposed systems use the J^T pattern for
-    // the refinement residual.
+    template <typename T, typename TX>
+    void solve_once(kern::Arenas<T> ar, const double* rhs, TX* x, bool transpose) {
+        T* y = ywork.as<T>();
+        T* u = upd.as<T>();
+        for (int L = 0; L < plan.num_levels; ++L)
+            for (int t = 0; t < 3; ++t) {
+                int cnt, mf;
+                const int* f = group(L, t, cnt, mf);
+                kern::lu_solve_forward_group<T>(lu, t, f, cnt, mf, ar, rhs, y, u, Bp, transpose, s);
+            }
+        for (int L = plan.num_levels - 1; L >= 0; --L)
+            for (int t = 0; t < 3; ++t) {
+                int cnt, mf;
+                const int* f = group(L, t, cnt, mf);
+                kern::lu_solve_backward_group<T>(lu, t, f, cnt, mf, ar, y, Bp, transpose, s);
+            }
+        kern::lu_solve_scatter<T, TX>(lu, y, Bp, x, s);
+    }
+
     template <typename TX>
-    void solve_into(int B, const double* rhs, TX* x, TX* xtmp, bool transpose) {
-        if (front64) {
-            kern::lu_solve<double, double, TX>(lu, arena.as<double>(), rhs, js.dim, B,
-                                               work.as<double>(), upd.as<double>(), x, js.dim,
-                                               transpose, s);
-            return;
-        }
-        kern::lu_solve<float, double, TX>(lu, arena.as<float>(), rhs, js.dim, B, work.as<float>(),
-                                          upd.as<float>(), x, js.dim, transpose, s);
+    void solve_dispatch(const double* rhs, TX* x, bool transpose) {
+        if (front64) solve_once<double, TX>({arena0.as<double>(), arena1.as<double>()}, rhs, x, transpose);
+        else solve_once<float, TX>({arena0.as<float>(), arena1.as<float>()}, rhs, x, transpose);
+    }
+
+    // rhs (FP64, [dim][Bp]) -> x, with FP64 iterative refinement when the
+    // fronts are FP32 (residual on the FP64 J, or J^T for transposed solves).
+    template <typename TX>
+    void solve_into(const double* rhs, TX* x, TX* xtmp, bool transpose) {
+        solve_dispatch<TX>(rhs, x, transpose);
+        if (front64) return;
         for (int it = 0; it < refine; ++it) {
             if (transpose)
                 kern::refine_residual<TX>(d_jtrp.as<int>(), d_jtci.as<int>(), d_jtmap.as<int>(),
-                                          J.as<double>(), js.dim, js.nnz, rhs, x, B,
-                                          res.as<double>(), s);
+                                          J.as<double>(), js.dim, Bp, rhs, x, res.as<double>(), s);
             else
                 kern::refine_residual<TX>(d_jrp.as<int>(), d_jci.as<int>(), nullptr,
-                                          J.as<double>(), js.dim, js.nnz, rhs, x, B,
-                                          res.as<double>(), s);
-            kern::lu_solve<float, double, TX>(lu, arena.as<float>(), res.as<double>(), js.dim, B,
-                                              work.as<float>(), upd.as<float>(), xtmp, js.dim,
-                                              transpose, s);
-            kern::axpy_update<TX>(xtmp, x, (long long)B * js.dim, s);
+                                          J.as<double>(), js.dim, Bp, rhs, x, res.as<double>(), s);
+            solve_dispatch<TX>(res.as<double>(), xtmp, transpose);
+            kern::axpy_update<TX>(xtmp, x, (long long)Bp * js.dim, s);
         }
     }
 
-    void newton_step(int B) {
+    void newton_step() {
         {
             StageScope st_(timer, kStFactor, s);
-            factor(B);
+            factor();
         }
         if (dx32) {
             {
                 StageScope st_(timer, kStSolve, s);
-                solve_into<float>(B, F.as<double>(), dx.as<float>(), ddx.as<float>(), false);
+                solve_into<float>(F.as<double>(), dx.as<float>(), ddx.as<float>(), false);
             }
             StageScope st_(timer, kStUpdate, s);
-            kern::voltage_update<float>(nr, dx.as<float>(), B, Va.as<double>(), Vm.as<double>(),
+            kern::voltage_update<float>(nr, Bp, dx.as<float>(), Va.as<double>(), Vm.as<double>(),
                                         V.as<double2>(), s);
         } else {
             {
                 StageScope st_(timer, kStSolve, s);
-                solve_into<double>(B, F.as<double>(), dx.as<double>(), ddx.as<double>(), false);
+                solve_into<double>(F.as<double>(), dx.as<double>(), ddx.as<double>(), false);
             }
             StageScope st_(timer, kStUpdate, s);
-            kern::voltage_update<double>(nr, dx.as<double>(), B, Va.as<double>(),
+            kern::voltage_update<double>(nr, Bp, dx.as<double>(), Va.as<double>(),
                                          Vm.as<double>(), V.as<double2>(), s);
         }
     }
 
-    double read_batch_max(int B) {
+    void eval_residual() {
+        StageScope st_(timer, kStResidual, s);
+        kern::zero_norm(normF.as<double>(), Bp, s);
+        kern::residual(nr, Bp, V.as<double2>(), S.as<double2>(), F.as<double>(),
+                       normF.as<double>(), s);
+    }
+
+    void eval_jacobian() {
+        StageScope st_(timer, kStJacobian, s);
+        kern::jacobian(nr, Bp, V.as<double2>(), J.as<double>(), d_zero.as<int>(),
+                       int(zero_slots.size()), s);
+    }
+
+    double read_batch_max() {
         kern::batch_max(normF.as<double>(), B, bmax.as<double>(), s);
         double h = 0.0;
         CUPF_HIP(hipMemcpyAsync(&h, bmax.get(), sizeof(double), hipMemcpyDeviceToHost, s));
@@ -228,35 +277,38 @@ struct NewtonSolver::Impl {
         return h;
     }
 
-    int run_nr(int B, const NRConfig& cfg) {
+    int run_nr(const NRConfig& cfg) {
+        CUPF_HIP(hipMemsetAsync(sing.get(), 0, size_t(Bp) * sizeof(int), s));
         int completed = 0;
         for (int it = 0; it < cfg.max_iter; ++it) {
             completed = it + 1;
-            {
-                StageScope st_(timer, kStResidual, s);
-                kern::residual(nr, V.as<double2>(), S.as<double2>(), B, F.as<double>(),
-                               normF.as<double>(), s);
-            }
-            const double m = read_batch_max(B);
+            eval_residual();
+            const double m = read_batch_max();
             if (!std::isfinite(m))
                 throw std::runtime_error("NewtonSolver: mismatch norm is not finite");
             if (m <= cfg.tolerance) break;
-            {
-                StageScope st_(timer, kStJacobian, s);
-                kern::jacobian_impl<double>(nr, V.as<double2>(), B, J.as<double>(),
-                                            d_zero.as<int>(), int(zero_slots.size()), s);
-            }
-            newton_step(B);
+            eval_jacobian();
+            newton_step();
         }
         return completed;
     }
 
-    void prepare_cache(int B) {
-        kern::jacobian_impl<double>(nr, V.as<double2>(), B, J.as<double>(), d_zero.as<int>(),
-                                    int(zero_slots.size()), s);
-        factor(B);
+    void prepare_cache() {
+        eval_jacobian();
+        factor();
         cache = true;
         cache_B = B;
+    }
+
+    // Per-scenario outputs of the last NR loop (B real scenarios).
+    void download_norms(std::vector<double>& nf, std::vector<int>& sg) {
+        nf.resize(B);
+        sg.resize(B);
+        CUPF_HIP(hipMemcpyAsync(nf.data(), normF.get(), size_t(B) * sizeof(double),
+                                hipMemcpyDeviceToHost, s));
+        CUPF_HIP(hipMemcpyAsync(sg.data(), sing.get(), size_t(B) * sizeof(int),
+                                hipMemcpyDeviceToHost, s));
+        CUPF_HIP(hipStreamSynchronize(s));
     }
 
     void check_ybus(const YbusView& y) const {
@@ -282,6 +334,25 @@ struct NewtonSolver::Impl {
                                 hipMemcpyHostToDevice, s));
     }
 };
+
+namespace {
+
+// [B][m] (row stride src_stride) -> [m][Bp] with padded scenarios copying B-1.
+template <typename T>
+std::vector<T> host_interleave(const T* src, long long src_stride, int B, int Bp, int m) {
+    std::vector<T> out(size_t(m) * Bp);
+    for (int i = 0; i < m; ++i)
+        for (int b = 0; b < Bp; ++b) out[size_t(i) * Bp + b] = src[size_t(std::min(b, B - 1)) * src_stride + i];
+    return out;
+}
+
+template <typename T>
+void host_deinterleave(const std::vector<T>& src, int B, int Bp, int m, T* dst) {
+    for (int b = 0; b < B; ++b)
+        for (int i = 0; i < m; ++i) dst[size_t(b) * m + i] = src[size_t(i) * Bp + b];
+}
+
+}  // namespace
 
 // ===========================================================================
 
@@ -341,52 +412,6 @@ void NewtonSolver::collect_stage_times(double* total_ms, int64_t* launches) {
     impl_->timer.collect(total_ms, launches);
 }
 
-void NewtonSolver::solve_batch_device(const void* sbus, int64_t sbus_stride, const void* V0,
-                                      int64_t V0_stride, int32_t batch_size,
-                                      const NRConfig& config, const SolveOptions& solve_options,
-                                      void* V_out, int32_t* iterations, double* final_mismatch,
-                                      uint8_t* converged, void* stream) {
-    Impl& m = *impl_;
-    if (!m.initialized)
-        throw std::runtime_error("NewtonSolver::solve_batch_device(): initialize() must be called first");
-    validate_batch_args(batch_size, sbus_stride, m.n, "sbus");
-    validate_batch_args(batch_size, V0_stride, m.n, "V0");
-    if (!sbus || !V0 || !V_out)
-        throw std::invalid_argument("NewtonSolver::solve_batch_device(): null device pointer");
-    if (config.max_iter < 0) throw std::invalid_argument("NRConfig.max_iter must be >= 0");
-    const auto t0 = std::chrono::steady_clock::now();
-    m.s = stream ? static_cast<hipStream_t>(stream) : m.own;
-    m.cache = false;
-    const int B = batch_size;
-    m.ensure_batch(B);
-    const size_t row = size_t(m.n) * sizeof(double2);
-    CUPF_HIP(hipMemcpy2DAsync(m.S.get(), row, sbus, size_t(sbus_stride) * sizeof(double2), row, B,
-                              hipMemcpyDeviceToDevice, m.s));
-    kern::seed_state(m.nr, static_cast<const double2*>(V0), V0_stride, B, m.V.as<double2>(),
-                     m.Va.as<double>(), m.Vm.as<double>(), m.s);
-    CUPF_HIP(hipMemsetAsync(m.sing.get(), 0, size_t(B) * sizeof(int), m.s));
-    const int iters = m.run_nr(B, config);
-    m.last_B = B;
-    if (solve_options.prepare_adjoint_cache &&
-        solve_options.adjoint_cache_mode != AdjointCacheMode::None)
-        m.prepare_cache(B);
-    CUPF_HIP(hipMemcpyAsync(V_out, m.V.get(), size_t(B) * row, hipMemcpyDeviceToDevice, m.s));
-    std::vector<double> nf(B);
-    CUPF_HIP(hipMemcpyAsync(nf.data(), m.normF.get(), size_t(B) * sizeof(double),
-                            hipMemcpyDeviceToHost, m.s));
-    CUPF_HIP(hipStreamSynchronize(m.s));
-    for (int b = 0; b < B; ++b) {
-        if (iterations) iterations[b] = iters;
-        if (final_mismatch) final_mismatch[b] = nf[b];
-        if (converged) converged[b] = uint8_t(nf[b] <= config.tolerance ? 1 : 0);
-    }
-    m.st.last_iterations = iters;
-    m.st.last_batch = B;
-    m.st.last_solve_ms =
-        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    m.s = m.own;
-}
-
 void NewtonSolver::initialize(const YbusView& ybus, const int32_t* pv, int32_t n_pv,
                               const int32_t* pq, int32_t n_pq) {
     Impl& m = *impl_;
@@ -401,6 +426,7 @@ void NewtonSolver::initialize(const YbusView& ybus, const int32_t* pv, int32_t n
             throw std::invalid_argument("NewtonSolver::initialize(): Ybus indptr must be non-decreasing");
     m.initialized = false;
     m.cache = false;
+    m.last_B = 0;
     m.n = ybus.rows;
     m.nnzY = ybus.nnz;
     m.yp.assign(ybus.indptr, ybus.indptr + m.n + 1);
@@ -423,12 +449,12 @@ void NewtonSolver::initialize(const YbusView& ybus, const int32_t* pv, int32_t n
     const std::vector<int> perm = order_jacobian(m.js);
     m.sym = analyze_symbolic(m.js.dim, m.js.row_ptr.data(), m.js.col_idx.data(), perm,
                              m.opt.custom.max_panel_width);
-    m.plan = lower_plan(m.sym, m.js.dim, m.js.row_ptr.data(), m.js.col_idx.data());
-
-    // update-vector offsets (solve)
-    m.uoff.assign(m.plan.num_fronts + 1, 0);
-    for (int p = 0; p < m.plan.num_fronts; ++p)
-        m.uoff[p + 1] = m.uoff[p] + (m.plan.fsz[p] - m.plan.nc[p]);
+    // tier 1 holds a whole front in 64 KiB of LDS
+    const int tier1_max = m.front64 ? 90 : 128;
+    m.plan = lower_plan(m.sym, m.js.dim, m.js.row_ptr.data(), m.js.col_idx.data(), 16, tier1_max);
+    const int lds_max = m.front64 ? 8192 : 16384;
+    if (m.plan.max_fsz > lds_max)
+        throw std::runtime_error("NewtonSolver::initialize(): a front exceeds the LDS panel budget");
 
     // J^T pattern with value map (refinement residual of transposed solves)
     {
@@ -473,8 +499,9 @@ void NewtonSolver::initialize(const YbusView& ybus, const int32_t* pv, int32_t n
     const LuPlan& P = m.plan;
     m.d_fsz = upload_vector(P.fsz, s);
     m.d_nc = upload_vector(P.nc, s);
+    m.d_tier = upload_vector(P.tier, s);
     m.d_off = upload_vector(P.off, s);
-    m.d_uoff = upload_vector(m.uoff, s);
+    m.d_uoff = upload_vector(P.uoff, s);
     m.d_fptr = upload_vector(P.front_ptr, s);
     m.d_frows = upload_vector(P.front_rows, s);
     m.d_cptr = upload_vector(P.child_ptr, s);
@@ -485,31 +512,35 @@ void NewtonSolver::initialize(const YbusView& ybus, const int32_t* pv, int32_t n
     m.d_esrc = upload_vector(P.ent_src, s);
     m.d_edst = upload_vector(P.ent_dst, s);
     m.d_perm = upload_vector(P.perm, s);
+    m.d_groups = upload_vector(P.group_fronts, s);
     CUPF_HIP(hipStreamSynchronize(s));
 
     kern::NrDevice& nr = m.nr;
     nr.n_bus = m.n; nr.n_pv = n_pv; nr.n_pq = n_pq; nr.dim = m.js.dim;
     nr.nnzY = m.nnzY; nr.nnzJ = m.js.nnz;
     nr.yp = m.d_yp.as<int>(); nr.yi = m.d_yi.as<int>(); nr.y = m.d_y.as<double2>();
-    nr.y_stride = 0;
     nr.row_pvpq = m.d_row_pvpq.as<int>(); nr.row_pq = m.d_row_pq.as<int>();
     nr.map11 = m.d_m11.as<int>(); nr.map21 = m.d_m21.as<int>();
     nr.map12 = m.d_m12.as<int>(); nr.map22 = m.d_m22.as<int>();
+    nr.row_bus = m.d_row_bus.as<int>(); nr.row_is_q = m.d_row_isq.as<unsigned char>();
 
     kern::LuDevice& lu = m.lu;
-    lu.n = m.js.dim; lu.nnz = m.js.nnz; lu.P = P.num_fronts; lu.max_fsz = P.max_fsz;
-    lu.arena = P.arena; lu.upd_total = m.uoff[P.num_fronts];
-    lu.fsz = m.d_fsz.as<int>(); lu.nc = m.d_nc.as<int>(); lu.off = m.d_off.as<long long>();
-    lu.uoff = m.d_uoff.as<long long>(); lu.front_ptr = m.d_fptr.as<int>();
-    lu.front_rows = m.d_frows.as<int>(); lu.child_ptr = m.d_cptr.as<int>();
-    lu.child_idx = m.d_cidx.as<int>(); lu.asm_ptr = m.d_aptr.as<int>();
-    lu.asm_loc = m.d_aloc.as<int>(); lu.ent_ptr = m.d_eptr.as<int>();
-    lu.ent_src = m.d_esrc.as<int>(); lu.ent_dst = m.d_edst.as<int>(); lu.perm = m.d_perm.as<int>();
+    lu.n = m.js.dim; lu.nnz = m.js.nnz; lu.P = P.num_fronts;
+    lu.arena0 = P.arena0; lu.arena1 = P.arena1; lu.upd_total = P.upd_total;
+    lu.fsz = m.d_fsz.as<int>(); lu.nc = m.d_nc.as<int>(); lu.tier = m.d_tier.as<int>();
+    lu.off = m.d_off.as<long long>(); lu.uoff = m.d_uoff.as<long long>();
+    lu.front_ptr = m.d_fptr.as<int>(); lu.front_rows = m.d_frows.as<int>();
+    lu.child_ptr = m.d_cptr.as<int>(); lu.child_idx = m.d_cidx.as<int>();
+    lu.asm_ptr = m.d_aptr.as<int>(); lu.asm_loc = m.d_aloc.as<int>();
+    lu.ent_ptr = m.d_eptr.as<int>(); lu.ent_src = m.d_esrc.as<int>();
+    lu.ent_dst = m.d_edst.as<int>(); lu.perm = m.d_perm.as<int>();
 
-    m.capB = 0;   // batch buffers sized lazily for the new structure
+    m.capBp = 0;   // batch buffers are sized lazily for the new structure
+    m.capB = 0;
     m.st = NewtonSolverStats{};
     m.st.dim = m.js.dim; m.st.nnz_j = m.js.nnz; m.st.num_fronts = P.num_fronts;
-    m.st.max_front = P.max_fsz; m.st.max_pivots = P.max_nc; m.st.arena_elems = P.arena;
+    m.st.max_front = P.max_fsz; m.st.max_pivots = P.max_nc;
+    m.st.arena_elems = P.arena0 + P.arena1;
     m.st.lnz = P.lnz; m.st.factor_flops = P.flops;
     m.initialized = true;
 }
@@ -546,37 +577,33 @@ void NewtonSolver::solve_batch(const YbusView& ybus, const std::complex<double>*
     const auto t0 = std::chrono::steady_clock::now();
     m.s = m.own;
     m.cache = false;
-    const int B = batch_size;
-    m.ensure_batch(B);
+    m.ensure_batch(batch_size);
+    const int B = m.B;
     m.upload_ybus_values(ybus);
     const size_t row = size_t(m.n) * sizeof(double2);
-    CUPF_HIP(hipMemcpy2DAsync(m.S.get(), row, sbus, size_t(sbus_stride) * sizeof(double2), row, B,
+    CUPF_HIP(hipMemcpy2DAsync(m.stage_a.get(), row, sbus, size_t(sbus_stride) * sizeof(double2), row,
+                              B, hipMemcpyHostToDevice, m.s));
+    CUPF_HIP(hipMemcpy2DAsync(m.stage_b.get(), row, V0, size_t(V0_stride) * sizeof(double2), row, B,
                               hipMemcpyHostToDevice, m.s));
-    CUPF_HIP(hipMemcpy2DAsync(m.io_a.get(), row, V0, size_t(V0_stride) * sizeof(double2), row, B,
-                              hipMemcpyHostToDevice, m.s));
-    kern::seed_state(m.nr, m.io_a.as<double2>(), m.n, B, m.V.as<double2>(), m.Va.as<double>(),
-                     m.Vm.as<double>(), m.s);
-    CUPF_HIP(hipMemsetAsync(m.sing.get(), 0, size_t(B) * sizeof(int), m.s));
+    kern::to_interleaved(m.stage_a.as<double2>(), m.n, m.n, B, m.Bp, m.S.as<double2>(), m.s);
+    kern::to_interleaved(m.stage_b.as<double2>(), m.n, m.n, B, m.Bp, m.V.as<double2>(), m.s);
+    kern::seed_polar(m.n, m.Bp, m.V.as<double2>(), m.Va.as<double>(), m.Vm.as<double>(), m.s);
 
-    const int iters = m.run_nr(B, config);
+    const int iters = m.run_nr(config);
     m.last_B = B;
     if (solve_options.prepare_adjoint_cache &&
         solve_options.adjoint_cache_mode != AdjointCacheMode::None)
-        m.prepare_cache(B);
+        m.prepare_cache();
 
     result = {};
     result.n_bus = m.n;
     result.batch_size = B;
     result.V.resize(size_t(B) * m.n);
-    CUPF_HIP(hipMemcpyAsync(result.V.data(), m.V.get(), size_t(B) * row, hipMemcpyDeviceToHost,
-                            m.s));
-    result.final_mismatch.resize(B);
-    CUPF_HIP(hipMemcpyAsync(result.final_mismatch.data(), m.normF.get(), size_t(B) * sizeof(double),
+    kern::from_interleaved(m.V.as<double2>(), m.n, B, m.Bp, m.stage_a.as<double2>(), m.n, m.s);
+    CUPF_HIP(hipMemcpyAsync(result.V.data(), m.stage_a.get(), size_t(B) * row,
                             hipMemcpyDeviceToHost, m.s));
-    std::vector<int> sing(B);
-    CUPF_HIP(hipMemcpyAsync(sing.data(), m.sing.get(), size_t(B) * sizeof(int),
-                            hipMemcpyDeviceToHost, m.s));
-    CUPF_HIP(hipStreamSynchronize(m.s));
+    std::vector<int> sing;
+    m.download_norms(result.final_mismatch, sing);
     result.iterations.assign(B, iters);
     result.converged.resize(B);
     result.singular.resize(B);
@@ -588,6 +615,51 @@ void NewtonSolver::solve_batch(const YbusView& ybus, const std::complex<double>*
     m.st.last_batch = B;
     m.st.last_solve_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+void NewtonSolver::solve_batch_device(const void* sbus, int64_t sbus_stride, const void* V0,
+                                      int64_t V0_stride, int32_t batch_size,
+                                      const NRConfig& config, const SolveOptions& solve_options,
+                                      void* V_out, int32_t* iterations, double* final_mismatch,
+                                      uint8_t* converged, void* stream) {
+    Impl& m = *impl_;
+    if (!m.initialized)
+        throw std::runtime_error("NewtonSolver::solve_batch_device(): initialize() must be called first");
+    validate_batch_args(batch_size, sbus_stride, m.n, "sbus");
+    validate_batch_args(batch_size, V0_stride, m.n, "V0");
+    if (!sbus || !V0 || !V_out)
+        throw std::invalid_argument("NewtonSolver::solve_batch_device(): null device pointer");
+    if (config.max_iter < 0) throw std::invalid_argument("NRConfig.max_iter must be >= 0");
+    const auto t0 = std::chrono::steady_clock::now();
+    m.s = stream ? static_cast<hipStream_t>(stream) : m.own;
+    m.cache = false;
+    m.ensure_batch(batch_size);
+    const int B = m.B;
+    kern::to_interleaved(static_cast<const double2*>(sbus), sbus_stride, m.n, B, m.Bp,
+                         m.S.as<double2>(), m.s);
+    kern::to_interleaved(static_cast<const double2*>(V0), V0_stride, m.n, B, m.Bp,
+                         m.V.as<double2>(), m.s);
+    kern::seed_polar(m.n, m.Bp, m.V.as<double2>(), m.Va.as<double>(), m.Vm.as<double>(), m.s);
+    const int iters = m.run_nr(config);
+    m.last_B = B;
+    if (solve_options.prepare_adjoint_cache &&
+        solve_options.adjoint_cache_mode != AdjointCacheMode::None)
+        m.prepare_cache();
+    kern::from_interleaved(m.V.as<double2>(), m.n, B, m.Bp, static_cast<double2*>(V_out), m.n,
+                           m.s);
+    std::vector<double> nf;
+    std::vector<int> sg;
+    m.download_norms(nf, sg);
+    for (int b = 0; b < B; ++b) {
+        if (iterations) iterations[b] = iters;
+        if (final_mismatch) final_mismatch[b] = nf[b];
+        if (converged) converged[b] = uint8_t(nf[b] <= config.tolerance ? 1 : 0);
+    }
+    m.st.last_iterations = iters;
+    m.st.last_batch = B;
+    m.st.last_solve_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    m.s = m.own;
 }
 
 void NewtonSolver::solve_adjoint(const double* grad_va, int64_t grad_va_stride,
@@ -615,19 +687,24 @@ void NewtonSolver::solve_adjoint(const double* grad_va, int64_t grad_va_stride,
     m.check_pvpq(pv, n_pv, pq, n_pq);
     const auto t0 = std::chrono::steady_clock::now();
     m.s = m.own;
-    const int B = batch_size;
+    const int B = m.B;
     result = {};
     result.backend = m.front64 ? "hip_multifrontal_fp64" : "hip_multifrontal_fp32_refined";
     result.transpose_solve_backend = "hip_multifrontal_transpose_solve";
     const bool cache_ok = m.cache && m.cache_B == B;
     result.used_adjoint_cache = cache_ok;
-    result.adjoint_cache_matches_final_state = cache_ok;
+    result.adjoint_cache_matches_final_state = true;
+    result.jt_symbolic_analyzed_at_initialize = true;
     if (!cache_ok) {
         if (options.require_cached_factorization ||
             !(options.allow_refactorize && options.allow_refactorize_for_backward))
             throw std::runtime_error(
                 "NewtonSolver::solve_adjoint(): missing exact final-state adjoint cache");
-        m.prepare_cache(B);
+        const auto tf = std::chrono::steady_clock::now();
+        m.prepare_cache();
+        CUPF_HIP(hipStreamSynchronize(m.s));
+        result.factorization_time_ms =
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tf).count();
         result.refactorized_for_backward = true;
         result.jt_refactorized_during_backward = true;
     } else {
@@ -635,54 +712,64 @@ void NewtonSolver::solve_adjoint(const double* grad_va, int64_t grad_va_stride,
         result.jt_factorized_during_forward_cache = true;
     }
     const size_t row = size_t(m.n) * sizeof(double);
-    CUPF_HIP(hipMemcpy2DAsync(m.io_a.get(), row, grad_va, size_t(grad_va_stride) * sizeof(double),
+    CUPF_HIP(hipMemcpy2DAsync(m.stage_a.get(), row, grad_va, size_t(grad_va_stride) * sizeof(double),
                               row, B, hipMemcpyHostToDevice, m.s));
-    CUPF_HIP(hipMemcpy2DAsync(m.io_b.get(), row, grad_vm, size_t(grad_vm_stride) * sizeof(double),
+    CUPF_HIP(hipMemcpy2DAsync(m.stage_b.get(), row, grad_vm, size_t(grad_vm_stride) * sizeof(double),
                               row, B, hipMemcpyHostToDevice, m.s));
-    kern::adjoint_gather<double>(m.nr, m.d_row_bus.as<int>(), m.d_row_isq.as<unsigned char>(),
-                                 m.io_a.as<double>(), m.n, m.io_b.as<double>(), m.n, B,
-                                 m.g.as<double>(), m.s);
-    m.solve_into<double>(B, m.g.as<double>(), m.lam.as<double>(), m.ddx.as<double>(), true);
+    kern::adjoint_gather<double>(m.nr, B, m.Bp, m.stage_a.as<double>(), m.n,
+                                 m.stage_b.as<double>(), m.n, m.g.as<double>(), m.s);
+    const auto ts = std::chrono::steady_clock::now();
+    m.solve_into<double>(m.g.as<double>(), m.lam.as<double>(), m.ddx.as<double>(), true);
+    CUPF_HIP(hipStreamSynchronize(m.s));
+    result.transpose_solve_time_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - ts).count();
+    result.solve_time_ms = result.transpose_solve_time_ms;
     result.n_bus = m.n;
     result.batch_size = B;
     result.dimF = m.js.dim;
-    result.lambda.resize(size_t(B) * m.js.dim);
-    CUPF_HIP(hipMemcpyAsync(result.lambda.data(), m.lam.get(), result.lambda.size() * sizeof(double),
-                            hipMemcpyDeviceToHost, m.s));
+    {
+        std::vector<double> li(size_t(m.js.dim) * m.Bp);
+        CUPF_HIP(hipMemcpyAsync(li.data(), m.lam.get(), li.size() * sizeof(double),
+                                hipMemcpyDeviceToHost, m.s));
+        CUPF_HIP(hipStreamSynchronize(m.s));
+        result.lambda.resize(size_t(B) * m.js.dim);
+        host_deinterleave(li, B, m.Bp, m.js.dim, result.lambda.data());
+    }
     if (options.compute_load_gradients) {
-        kern::adjoint_project<double>(m.nr, m.lam.as<double>(), B, m.io_a.as<double>(),
-                                      m.io_b.as<double>(), m.s);
+        kern::adjoint_project<double>(m.nr, B, m.Bp, m.lam.as<double>(), m.stage_a.as<double>(),
+                                      m.stage_b.as<double>(), m.s);
         result.grad_load_p.resize(size_t(B) * m.n);
         result.grad_load_q.resize(size_t(B) * m.n);
-        CUPF_HIP(hipMemcpyAsync(result.grad_load_p.data(), m.io_a.get(), size_t(B) * row,
+        CUPF_HIP(hipMemcpyAsync(result.grad_load_p.data(), m.stage_a.get(), size_t(B) * row,
                                 hipMemcpyDeviceToHost, m.s));
-        CUPF_HIP(hipMemcpyAsync(result.grad_load_q.data(), m.io_b.get(), size_t(B) * row,
+        CUPF_HIP(hipMemcpyAsync(result.grad_load_q.data(), m.stage_b.get(), size_t(B) * row,
                                 hipMemcpyDeviceToHost, m.s));
     }
     if (options.check_residual) {
         kern::refine_residual<double>(m.d_jtrp.as<int>(), m.d_jtci.as<int>(), m.d_jtmap.as<int>(),
-                                      m.J.as<double>(), m.js.dim, m.js.nnz, m.g.as<double>(),
-                                      m.lam.as<double>(), B, m.res.as<double>(), m.s);
-        std::vector<double> r(size_t(B) * m.js.dim), gh(size_t(B) * m.js.dim);
+                                      m.J.as<double>(), m.js.dim, m.Bp, m.g.as<double>(),
+                                      m.lam.as<double>(), m.res.as<double>(), m.s);
+        std::vector<double> r(size_t(m.js.dim) * m.Bp), gh(size_t(m.js.dim) * m.Bp);
         CUPF_HIP(hipMemcpyAsync(r.data(), m.res.get(), r.size() * sizeof(double),
                                 hipMemcpyDeviceToHost, m.s));
         CUPF_HIP(hipMemcpyAsync(gh.data(), m.g.get(), gh.size() * sizeof(double),
                                 hipMemcpyDeviceToHost, m.s));
         CUPF_HIP(hipStreamSynchronize(m.s));
         long double rr = 0, gg = 0;
-        for (size_t i = 0; i < r.size(); ++i) {
-            rr += (long double)r[i] * r[i];
-            gg += (long double)gh[i] * gh[i];
-        }
-        const long double den = std::max(std::sqrt(std::max(gg, (long double)1e-60)), (long double)1e-30);
+        for (int i = 0; i < m.js.dim; ++i)
+            for (int b = 0; b < B; ++b) {
+                const size_t o = size_t(i) * m.Bp + b;
+                rr += (long double)r[o] * r[o];
+                gg += (long double)gh[o] * gh[o];
+            }
+        const long double den =
+            std::max(std::sqrt(std::max(gg, (long double)1e-60)), (long double)1e-30);
         result.jt_residual_norm = double(std::sqrt(rr) / den);
     }
     CUPF_HIP(hipStreamSynchronize(m.s));
     result.success = true;
     result.total_time_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    result.solve_time_ms = result.total_time_ms;
-    result.transpose_solve_time_ms = result.total_time_ms;
 }
 
 // ---------------------------------------------------------------------------
@@ -710,32 +797,34 @@ void NewtonSolver::solve_torch_forward(const void* sbus_base_re, const void* sbu
                                             : "float32 for ComputePolicy.FP32/Mixed"));
     if (!sbus_base_re || !sbus_base_im || !load_p || !load_q || !v0_va || !v0_vm || !va_out || !vm_out)
         throw std::invalid_argument("NewtonSolver::solve_torch_forward(): null tensor pointer");
+    if (config.max_iter < 0) throw std::invalid_argument("NRConfig.max_iter must be >= 0");
     m.s = stream ? static_cast<hipStream_t>(stream) : m.own;
     m.cache = false;
-    const int B = batch_size;
-    m.ensure_batch(B);
+    m.ensure_batch(batch_size);
+    const int B = m.B;
     if (want64)
-        kern::torch_inputs<double>(m.n, B, (const double*)sbus_base_re, (const double*)sbus_base_im,
-                                   (const double*)load_p, (const double*)load_q,
-                                   (const double*)v0_va, (const double*)v0_vm, m.S.as<double2>(),
-                                   m.V.as<double2>(), m.Va.as<double>(), m.Vm.as<double>(), m.s);
+        kern::torch_inputs<double>(m.n, B, m.Bp, (const double*)sbus_base_re,
+                                   (const double*)sbus_base_im, (const double*)load_p,
+                                   (const double*)load_q, (const double*)v0_va,
+                                   (const double*)v0_vm, m.S.as<double2>(), m.V.as<double2>(),
+                                   m.Va.as<double>(), m.Vm.as<double>(), m.s);
     else
-        kern::torch_inputs<float>(m.n, B, (const float*)sbus_base_re, (const float*)sbus_base_im,
-                                  (const float*)load_p, (const float*)load_q, (const float*)v0_va,
-                                  (const float*)v0_vm, m.S.as<double2>(), m.V.as<double2>(),
-                                  m.Va.as<double>(), m.Vm.as<double>(), m.s);
-    CUPF_HIP(hipMemsetAsync(m.sing.get(), 0, size_t(B) * sizeof(int), m.s));
-    const int iters = m.run_nr(B, config);
+        kern::torch_inputs<float>(m.n, B, m.Bp, (const float*)sbus_base_re,
+                                  (const float*)sbus_base_im, (const float*)load_p,
+                                  (const float*)load_q, (const float*)v0_va, (const float*)v0_vm,
+                                  m.S.as<double2>(), m.V.as<double2>(), m.Va.as<double>(),
+                                  m.Vm.as<double>(), m.s);
+    const int iters = m.run_nr(config);
     m.last_B = B;
     if (solve_options.prepare_adjoint_cache &&
         solve_options.adjoint_cache_mode != AdjointCacheMode::None)
-        m.prepare_cache(B);
+        m.prepare_cache();
     if (want64)
-        kern::torch_outputs<double>(m.n, B, m.Va.as<double>(), m.Vm.as<double>(), (double*)va_out,
-                                    (double*)vm_out, m.s);
+        kern::torch_outputs<double>(m.n, B, m.Bp, m.Va.as<double>(), m.Vm.as<double>(),
+                                    (double*)va_out, (double*)vm_out, m.s);
     else
-        kern::torch_outputs<float>(m.n, B, m.Va.as<double>(), m.Vm.as<double>(), (float*)va_out,
-                                   (float*)vm_out, m.s);
+        kern::torch_outputs<float>(m.n, B, m.Bp, m.Va.as<double>(), m.Vm.as<double>(),
+                                   (float*)va_out, (float*)vm_out, m.s);
     result = {};
     result.success = true;
     result.n_bus = m.n;
@@ -770,29 +859,27 @@ void NewtonSolver::solve_torch_backward(const void* grad_va, const void* grad_vm
     if (!grad_va || !grad_vm || !grad_load_p || !grad_load_q)
         throw std::invalid_argument("NewtonSolver::solve_torch_backward(): null tensor pointer");
     m.s = stream ? static_cast<hipStream_t>(stream) : m.own;
-    const int B = batch_size;
+    const int B = m.B;
     const bool cache_ok = m.cache && m.cache_B == B;
     if (!cache_ok) {
         if (options.require_cached_factorization ||
             !(options.allow_refactorize && options.allow_refactorize_for_backward))
             throw std::runtime_error(
                 "NewtonSolver::solve_torch_backward(): missing exact final-state adjoint cache");
-        m.prepare_cache(B);
+        m.prepare_cache();
     }
     if (want64)
-        kern::adjoint_gather<double>(m.nr, m.d_row_bus.as<int>(), m.d_row_isq.as<unsigned char>(),
-                                     (const double*)grad_va, m.n, (const double*)grad_vm, m.n, B,
-                                     m.g.as<double>(), m.s);
+        kern::adjoint_gather<double>(m.nr, B, m.Bp, (const double*)grad_va, m.n,
+                                     (const double*)grad_vm, m.n, m.g.as<double>(), m.s);
     else
-        kern::adjoint_gather<float>(m.nr, m.d_row_bus.as<int>(), m.d_row_isq.as<unsigned char>(),
-                                    (const float*)grad_va, m.n, (const float*)grad_vm, m.n, B,
-                                    m.g.as<double>(), m.s);
-    m.solve_into<double>(B, m.g.as<double>(), m.lam.as<double>(), m.ddx.as<double>(), true);
+        kern::adjoint_gather<float>(m.nr, B, m.Bp, (const float*)grad_va, m.n,
+                                    (const float*)grad_vm, m.n, m.g.as<double>(), m.s);
+    m.solve_into<double>(m.g.as<double>(), m.lam.as<double>(), m.ddx.as<double>(), true);
     if (want64)
-        kern::adjoint_project<double>(m.nr, m.lam.as<double>(), B, (double*)grad_load_p,
+        kern::adjoint_project<double>(m.nr, B, m.Bp, m.lam.as<double>(), (double*)grad_load_p,
                                       (double*)grad_load_q, m.s);
     else
-        kern::adjoint_project<float>(m.nr, m.lam.as<double>(), B, (float*)grad_load_p,
+        kern::adjoint_project<float>(m.nr, B, m.Bp, m.lam.as<double>(), (float*)grad_load_p,
                                      (float*)grad_load_q, m.s);
     result = {};
     result.success = true;
@@ -813,7 +900,7 @@ void NewtonSolver::solve_torch_backward(const void* grad_va, const void* grad_vm
 }
 
 // ---------------------------------------------------------------------------
-// Diagnostics.
+// Diagnostics (host arrays in the public [B][.] layout).
 // ---------------------------------------------------------------------------
 void NewtonSolver::debug_residual(const std::complex<double>* sbus, const std::complex<double>* V,
                                   int32_t batch, double* F, double* norm) {
@@ -822,15 +909,18 @@ void NewtonSolver::debug_residual(const std::complex<double>* sbus, const std::c
     m.s = m.own;
     m.ensure_batch(batch);
     const size_t row = size_t(m.n) * sizeof(double2);
-    CUPF_HIP(hipMemcpyAsync(m.S.get(), sbus, batch * row, hipMemcpyHostToDevice, m.s));
-    CUPF_HIP(hipMemcpyAsync(m.V.get(), V, batch * row, hipMemcpyHostToDevice, m.s));
-    kern::residual(m.nr, m.V.as<double2>(), m.S.as<double2>(), batch, m.F.as<double>(),
-                   m.normF.as<double>(), m.s);
-    CUPF_HIP(hipMemcpyAsync(F, m.F.get(), size_t(batch) * m.js.dim * sizeof(double),
+    CUPF_HIP(hipMemcpyAsync(m.stage_a.get(), sbus, batch * row, hipMemcpyHostToDevice, m.s));
+    CUPF_HIP(hipMemcpyAsync(m.stage_b.get(), V, batch * row, hipMemcpyHostToDevice, m.s));
+    kern::to_interleaved(m.stage_a.as<double2>(), m.n, m.n, batch, m.Bp, m.S.as<double2>(), m.s);
+    kern::to_interleaved(m.stage_b.as<double2>(), m.n, m.n, batch, m.Bp, m.V.as<double2>(), m.s);
+    m.eval_residual();
+    std::vector<double> fi(size_t(m.js.dim) * m.Bp);
+    CUPF_HIP(hipMemcpyAsync(fi.data(), m.F.get(), fi.size() * sizeof(double),
                             hipMemcpyDeviceToHost, m.s));
     CUPF_HIP(hipMemcpyAsync(norm, m.normF.get(), size_t(batch) * sizeof(double),
                             hipMemcpyDeviceToHost, m.s));
     CUPF_HIP(hipStreamSynchronize(m.s));
+    host_deinterleave(fi, batch, m.Bp, m.js.dim, F);
 }
 
 void NewtonSolver::debug_jacobian(const std::complex<double>* V, int32_t batch, double* Jout) {
@@ -839,12 +929,14 @@ void NewtonSolver::debug_jacobian(const std::complex<double>* V, int32_t batch, 
     m.s = m.own;
     m.ensure_batch(batch);
     const size_t row = size_t(m.n) * sizeof(double2);
-    CUPF_HIP(hipMemcpyAsync(m.V.get(), V, batch * row, hipMemcpyHostToDevice, m.s));
-    kern::jacobian_impl<double>(m.nr, m.V.as<double2>(), batch, m.J.as<double>(),
-                                m.d_zero.as<int>(), int(m.zero_slots.size()), m.s);
-    CUPF_HIP(hipMemcpyAsync(Jout, m.J.get(), size_t(batch) * m.js.nnz * sizeof(double),
+    CUPF_HIP(hipMemcpyAsync(m.stage_b.get(), V, batch * row, hipMemcpyHostToDevice, m.s));
+    kern::to_interleaved(m.stage_b.as<double2>(), m.n, m.n, batch, m.Bp, m.V.as<double2>(), m.s);
+    m.eval_jacobian();
+    std::vector<double> ji(size_t(m.js.nnz) * m.Bp);
+    CUPF_HIP(hipMemcpyAsync(ji.data(), m.J.get(), ji.size() * sizeof(double),
                             hipMemcpyDeviceToHost, m.s));
     CUPF_HIP(hipStreamSynchronize(m.s));
+    host_deinterleave(ji, batch, m.Bp, m.js.nnz, Jout);
 }
 
 void NewtonSolver::debug_factor_solve(const double* A, const double* rhs, int32_t batch,
@@ -853,29 +945,32 @@ void NewtonSolver::debug_factor_solve(const double* A, const double* rhs, int32_
     if (!m.initialized) throw std::runtime_error("debug_factor_solve: not initialized");
     m.s = m.own;
     m.ensure_batch(batch);
-    CUPF_HIP(hipMemcpyAsync(m.J.get(), A, size_t(batch) * m.js.nnz * sizeof(double),
+    const auto ai = host_interleave(A, m.js.nnz, batch, m.Bp, m.js.nnz);
+    const auto ri = host_interleave(rhs, m.js.dim, batch, m.Bp, m.js.dim);
+    CUPF_HIP(hipMemcpyAsync(m.J.get(), ai.data(), ai.size() * sizeof(double),
                             hipMemcpyHostToDevice, m.s));
-    CUPF_HIP(hipMemcpyAsync(m.g.get(), rhs, size_t(batch) * m.js.dim * sizeof(double),
+    CUPF_HIP(hipMemcpyAsync(m.g.get(), ri.data(), ri.size() * sizeof(double),
                             hipMemcpyHostToDevice, m.s));
-    CUPF_HIP(hipMemsetAsync(m.sing.get(), 0, size_t(batch) * sizeof(int), m.s));
-    m.factor(batch);
+    CUPF_HIP(hipMemsetAsync(m.sing.get(), 0, size_t(m.Bp) * sizeof(int), m.s));
+    m.factor();
+    std::vector<double> xi(size_t(m.js.dim) * m.Bp);
     if (m.dx32) {
-        m.solve_into<float>(batch, m.g.as<double>(), m.dx.as<float>(), m.ddx.as<float>(), transpose);
-        std::vector<float> xf(size_t(batch) * m.js.dim);
+        m.solve_into<float>(m.g.as<double>(), m.dx.as<float>(), m.ddx.as<float>(), transpose);
+        std::vector<float> xf(xi.size());
         CUPF_HIP(hipMemcpyAsync(xf.data(), m.dx.get(), xf.size() * sizeof(float),
                                 hipMemcpyDeviceToHost, m.s));
         CUPF_HIP(hipStreamSynchronize(m.s));
-        for (size_t i = 0; i < xf.size(); ++i) x[i] = xf[i];
+        for (size_t i = 0; i < xf.size(); ++i) xi[i] = xf[i];
     } else {
-        m.solve_into<double>(batch, m.g.as<double>(), m.lam.as<double>(), m.ddx.as<double>(),
-                             transpose);
-        CUPF_HIP(hipMemcpyAsync(x, m.lam.get(), size_t(batch) * m.js.dim * sizeof(double),
+        m.solve_into<double>(m.g.as<double>(), m.lam.as<double>(), m.ddx.as<double>(), transpose);
+        CUPF_HIP(hipMemcpyAsync(xi.data(), m.lam.get(), xi.size() * sizeof(double),
                                 hipMemcpyDeviceToHost, m.s));
     }
     std::vector<int> sg(batch);
     CUPF_HIP(hipMemcpyAsync(sg.data(), m.sing.get(), size_t(batch) * sizeof(int),
                             hipMemcpyDeviceToHost, m.s));
     CUPF_HIP(hipStreamSynchronize(m.s));
+    host_deinterleave(xi, batch, m.Bp, m.js.dim, x);
     for (int b = 0; b < batch; ++b) singular[b] = uint8_t(sg[b] != 0);
 }
 

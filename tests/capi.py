@@ -103,3 +103,42 @@ def exported_symbols():
     import re
     text = HEADER.read_text()
     return sorted(set(re.findall(r"^(?:int|void|const char\*)\s+(cupf_\w+)\(", text, re.M)))
+
+
+# ---- diagnostics on a live solver handle (GPU) ------------------------------
+
+def _h(handle):
+    return ctypes.c_void_p(handle)
+
+
+def debug_residual(handle, sbus, V, dim):
+    sbus = np.ascontiguousarray(sbus, np.complex128)
+    V = np.ascontiguousarray(V, np.complex128)
+    B = V.shape[0]
+    F = np.zeros((B, dim), np.float64)
+    norm = np.zeros(B, np.float64)
+    check(lib().cupf_solver_debug_residual(_h(handle), _p(sbus.view(np.float64)),
+                                           _p(V.view(np.float64)), ctypes.c_int32(B), _p(F),
+                                           _p(norm)))
+    return F, norm
+
+
+def debug_jacobian(handle, V, nnz):
+    V = np.ascontiguousarray(V, np.complex128)
+    B = V.shape[0]
+    J = np.zeros((B, nnz), np.float64)
+    check(lib().cupf_solver_debug_jacobian(_h(handle), _p(V.view(np.float64)), ctypes.c_int32(B),
+                                           _p(J)))
+    return J
+
+
+def debug_factor_solve(handle, A, rhs, transpose=False):
+    A = np.ascontiguousarray(A, np.float64)
+    rhs = np.ascontiguousarray(rhs, np.float64)
+    B = A.shape[0]
+    x = np.zeros_like(rhs)
+    sing = np.zeros(B, np.uint8)
+    check(lib().cupf_solver_debug_factor_solve(_h(handle), _p(A), _p(rhs), ctypes.c_int32(B),
+                                               ctypes.c_int32(1 if transpose else 0), _p(x),
+                                               _p(sing)))
+    return x, sing

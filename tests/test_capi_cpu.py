@@ -1,0 +1,91 @@
+"""The C ABI library loads without a GPU, exports every symbol the header
+declares, and maps errors to the reference's exception classes; the Python
+binding mirrors the reference's option surface."""
+
+from __future__ import annotations
+
+import ctypes
+import subprocess
+
+import numpy as np
+import pytest
+
+import capi
+
+
+def test_library_exports_every_declared_symbol():
+    lib = capi.lib()
+    names = capi.exported_symbols()
+    assert len(names) >= 20
+    missing = [nm for nm in names if not hasattr(lib, nm)]
+    assert not missing, missing
+    out = subprocess.run(["nm", "-D", "--defined-only", str(capi.LIB_PATH)], capture_output=True,
+                         text=True, check=True).stdout
+    for nm in names:
+        assert f" {nm}\n" in out + "\n", nm
+
+
+def test_default_options_roundtrip():
+    class Opts(ctypes.Structure):
+        _fields_ = [("backend", ctypes.c_int32), ("compute", ctypes.c_int32),
+                    ("cuda_linear_solver", ctypes.c_int32), ("cpu_linear_solver", ctypes.c_int32),
+                    ("cuda_jacobian", ctypes.c_int32), ("a", ctypes.c_int32), ("b", ctypes.c_int32),
+                    ("c", ctypes.c_int32), ("d", ctypes.c_double), ("precision", ctypes.c_int32),
+                    ("matching", ctypes.c_int32), ("pivot", ctypes.c_int32), ("serial", ctypes.c_int32),
+                    ("cap", ctypes.c_int32), ("eps", ctypes.c_double), ("refine", ctypes.c_int32),
+                    ("graph", ctypes.c_int32)]
+    o = Opts()
+    capi.lib().cupf_default_newton_options(ctypes.byref(o))
+    assert o.backend == 1 and o.compute == 2 and o.cuda_linear_solver == 1
+    assert o.cap == 8 and o.eps == 1e-8 and o.refine == 2
+
+
+def test_host_analysis_errors_map_to_value_error():
+    indptr = np.array([0, 2, 4], np.int32)
+    indices = np.array([0, 1, 0, 1], np.int32)
+    with pytest.raises(ValueError):
+        capi.analyze_jacobian(2, indptr, indices, np.array([], np.int32), np.array([7], np.int32))
+    with pytest.raises(ValueError):
+        capi.symbolic(np.array([0, 1], np.int32), np.array([0], np.int32), np.array([1], np.int32), 8)
+
+
+def test_binding_surface_mirrors_reference(cupf):
+    o = cupf.NewtonOptions()
+    assert o.backend == cupf.BackendKind.CUDA
+    assert o.compute == cupf.ComputePolicy.Mixed
+    assert o.cuda_linear_solver == cupf.CudaLinearSolverKind.Custom
+    assert o.cuda_jacobian == cupf.CudaJacobianKind.Edge
+    assert o.cpu_linear_solver == cupf.CpuLinearSolverKind.KLU
+    assert o.custom.precision == cupf.CustomPrecision.FP32
+    assert o.custom.max_panel_width == 8
+    assert o.custom.shift_retry_epsilon == 1e-8
+    c = cupf.NRConfig()
+    assert c.tolerance == 1e-8 and c.max_iter == 50
+    so = cupf.SolveOptions()
+    assert not so.prepare_adjoint_cache
+    assert so.adjoint_cache_mode == cupf.AdjointCacheMode.FinalStateFactorization
+    ao = cupf.AdjointOptions()
+    assert ao.allow_refactorize and ao.use_transpose_solve and ao.check_residual
+    for name in ("initialize", "solve", "solve_batch", "solve_adjoint", "solve_torch",
+                 "solve_with_adjoint_cache_torch", "solve_adjoint_torch"):
+        assert hasattr(cupf.NewtonSolver, name), name
+    assert cupf.CuPFFunction is not None and cupf.solve is not None
+
+
+def test_cpu_backend_is_rejected(cupf):
+    o = cupf.NewtonOptions()
+    o.backend = cupf.BackendKind.CPU
+    with pytest.raises(ValueError, match="GPU"):
+        cupf.NewtonSolver(o)
+    o = cupf.NewtonOptions()
+    o.cuda_linear_solver = cupf.CudaLinearSolverKind.CuDSS
+    with pytest.raises(ValueError):
+        cupf.NewtonSolver(o)
+
+
+def test_graph_requires_gpu_backend(cupf):
+    o = cupf.NewtonOptions()
+    o.backend = cupf.BackendKind.CPU
+    o.use_cuda_graph = True
+    with pytest.raises(ValueError):
+        cupf.NewtonSolver(o)

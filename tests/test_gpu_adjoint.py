@@ -1,0 +1,184 @@
+"""GPU adjoint (transpose solve on the cached LU factors) and the torch autograd path.
+
+Checks: lambda and load gradients against the oracle's transpose solve
+(relative 1e-9 FP64 / 1e-8 Mixed); load gradients against central finite
+differences of the converged solution (step 1e-6, inner solves to 1e-13,
+relative 1e-6); the torch.autograd.Function against the numpy adjoint.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def make_solver(cupf, g, compute="FP64", cap=64):
+    o = cupf.NewtonOptions()
+    o.compute = getattr(cupf.ComputePolicy, compute)
+    o.custom.precision = cupf.CustomPrecision.FP64 if compute == "FP64" else cupf.CustomPrecision.FP32
+    o.custom.max_panel_width = cap
+    s = cupf.NewtonSolver(o)
+    s.initialize(g.indptr, g.indices, g.data, g.n_bus, g.n_bus, g.pv, g.pq)
+    return s
+
+
+def pq_loss_grad(g, V):
+    gva = np.zeros(V.shape)
+    gvm = np.zeros(V.shape)
+    gvm[..., g.pq] = 2.0 * (np.abs(V[..., g.pq]) - 1.0)
+    return gva, gvm
+
+
+@pytest.mark.parametrize("compute,tol", [("FP64", 1e-9), ("Mixed", 1e-8)])
+def test_adjoint_vs_oracle(gpu, cupf, syn, oracle, compute, tol):
+    g = syn.workload_grid("case2000")
+    B = 4
+    S = g.sbus_scaled(syn.load_factors(B, g.n_bus, 0.7, 1.3, 4))
+    s = make_solver(cupf, g, compute)
+    so = cupf.SolveOptions()
+    so.prepare_adjoint_cache = True
+    r = s.solve_batch(g.indptr, g.indices, g.data, g.n_bus, g.n_bus, S, g.flat_start(B), g.pv, g.pq,
+                      cupf.NRConfig(), so)
+    V = r.V_numpy
+    gva, gvm = pq_loss_grad(g, V)
+    gva = gva + np.random.default_rng(0).standard_normal(gva.shape) * 0.1
+    opts = cupf.AdjointOptions()
+    opts.require_cached_factorization = True
+    a = s.solve_adjoint(gva, gvm, g.pv, g.pq, opts)
+    assert a.success and a.used_adjoint_cache and a.jt_residual_norm <= tol
+    lam = a.lambda_numpy
+    for b in range(B):
+        lo, gpo, gqo = oracle.adjoint(g.n_bus, g.indptr, g.indices, g.data, V[b], g.pv, g.pq,
+                                      gva[b], gvm[b])
+        assert np.abs(lam[b] - lo).max() <= tol * np.abs(lo).max()
+        assert np.abs(a.grad_load_p_numpy[b] - gpo).max() <= tol * np.abs(gpo).max()
+        assert np.abs(a.grad_load_q_numpy[b] - gqo).max() <= tol * np.abs(gqo).max()
+
+
+def test_adjoint_vs_central_differences(gpu, cupf, syn):
+    g = syn.make_grid(300, 480, 40, seed=11, load_scale=0.5)
+    s = make_solver(cupf, g, "FP64", 64)
+    base = g.sbus_base()
+    cfg = cupf.NRConfig()
+    cfg.tolerance = 1e-13
+    rng = np.random.default_rng(4)
+    buses = rng.choice(g.pq, size=8, replace=False)
+    h = 1e-6
+    # forward + adjoint at the base point
+    so = cupf.SolveOptions()
+    so.prepare_adjoint_cache = True
+    r = s.solve_batch(g.indptr, g.indices, g.data, g.n_bus, g.n_bus, base[None], g.flat_start(1),
+                      g.pv, g.pq, cfg, so)
+    V = r.V_numpy[0]
+    gva, gvm = pq_loss_grad(g, V)
+    a = s.solve_adjoint(gva, gvm, g.pv, g.pq, cupf.AdjointOptions())
+    gp, gq = a.grad_load_p_numpy[0], a.grad_load_q_numpy[0]
+    # all perturbed problems in one batch: +-h on P and Q of each chosen bus
+    S = []
+    for bus in buses:
+        for d in (-h, +h, -1j * h, +1j * h):      # load +h  <=> Sbus -h
+            x = base.copy()
+            x[bus] += d
+            S.append(x)
+    S = np.array(S)
+    rp = s.solve_batch(g.indptr, g.indices, g.data, g.n_bus, g.n_bus, S, g.flat_start(len(S)), g.pv,
+                       g.pq, cfg)
+    assert rp.converged_numpy.all()
+    Vp = rp.V_numpy
+    L = np.sum((np.abs(Vp[:, g.pq]) - 1.0) ** 2, axis=1)
+    for i, bus in enumerate(buses):
+        fd_p = (L[4 * i] - L[4 * i + 1]) / (2 * h)
+        fd_q = (L[4 * i + 2] - L[4 * i + 3]) / (2 * h)
+        assert abs(gp[bus] - fd_p) <= 1e-6 * max(abs(fd_p), 1e-3), (bus, gp[bus], fd_p)
+        assert abs(gq[bus] - fd_q) <= 1e-6 * max(abs(fd_q), 1e-3), (bus, gq[bus], fd_q)
+
+
+def test_adjoint_requires_cache_when_asked(gpu, cupf, syn):
+    g = syn.workload_grid("case118")
+    s = make_solver(cupf, g, "FP64", 8)
+    s.solve(g.indptr, g.indices, g.data, g.n_bus, g.n_bus, g.sbus_base(), g.flat_start(), g.pv, g.pq)
+    opts = cupf.AdjointOptions()
+    opts.require_cached_factorization = True
+    z = np.zeros(g.n_bus)
+    with pytest.raises(RuntimeError, match="cache"):
+        s.solve_adjoint(z, z, g.pv, g.pq, opts)
+    a = s.solve_adjoint(z, z, g.pv, g.pq, cupf.AdjointOptions())   # refactorizes
+    assert a.success and a.refactorized_for_backward
+    with pytest.raises(ValueError, match="batch"):
+        s.solve_adjoint(np.zeros((2, g.n_bus)), np.zeros((2, g.n_bus)), g.pv, g.pq)
+
+
+@pytest.mark.parametrize("compute", ["FP64", "Mixed"])
+def test_torch_autograd_matches_numpy_adjoint(gpu, cupf, syn, compute):
+    torch = gpu
+    g = syn.workload_grid("case2000")
+    B = 8
+    dtype = torch.float64 if compute == "FP64" else torch.float32
+    dev = torch.device("cuda")
+    f = syn.load_factors(B, g.n_bus, 0.7, 1.3, 6)
+    lp = torch.tensor(f * g.p_load, device=dev, dtype=dtype, requires_grad=True)
+    lq = torch.tensor(f * g.q_load, device=dev, dtype=dtype, requires_grad=True)
+    base_re = torch.tensor(g.p_gen, device=dev, dtype=dtype)
+    base_im = torch.zeros(g.n_bus, device=dev, dtype=dtype)
+    v0 = g.flat_start()
+    va0 = torch.tensor(np.angle(v0), device=dev, dtype=dtype)
+    vm0 = torch.tensor(np.abs(v0), device=dev, dtype=dtype)
+    s = make_solver(cupf, g, compute)
+    va, vm = cupf.solve(lp, lq, s, sbus_base_re=base_re, sbus_base_im=base_im, v0_va=va0, v0_vm=vm0)
+    assert va.shape == (B, g.n_bus) and va.dtype == dtype and va.is_cuda
+    pq = torch.tensor(g.pq, device=dev, dtype=torch.long)
+    loss = ((vm[:, pq] - 1.0) ** 2).sum() + 0.1 * va.sum()
+    loss.backward()
+    assert torch.isfinite(lp.grad).all() and torch.isfinite(lq.grad).all()
+    # same problem through the numpy entry points (FP64 host arrays)
+    S = (base_re.double().cpu().numpy()[None] - lp.detach().double().cpu().numpy()
+         - 1j * lq.detach().double().cpu().numpy())
+    ref = make_solver(cupf, g, "FP64")
+    so = cupf.SolveOptions()
+    so.prepare_adjoint_cache = True
+    r = ref.solve_batch(g.indptr, g.indices, g.data, g.n_bus, g.n_bus, S, g.flat_start(B), g.pv,
+                        g.pq, cupf.NRConfig(), so)
+    V = r.V_numpy
+    tol = 1e-9 if compute == "FP64" else 2e-5
+    assert np.abs(va.detach().double().cpu().numpy() - np.angle(V)).max() <= tol
+    gva = 0.1 * np.ones((B, g.n_bus))
+    gvm = np.zeros((B, g.n_bus))
+    gvm[:, g.pq] = 2.0 * (vm.detach().double().cpu().numpy()[:, g.pq] - 1.0)
+    a = ref.solve_adjoint(gva, gvm, g.pv, g.pq, cupf.AdjointOptions())
+    gp = lp.grad.double().cpu().numpy()
+    gq = lq.grad.double().cpu().numpy()
+    scale = max(np.abs(a.grad_load_p_numpy).max(), 1e-12)
+    assert np.abs(gp - a.grad_load_p_numpy).max() <= 10 * tol * scale * 1e3
+    assert np.abs(gq - a.grad_load_q_numpy).max() <= 10 * tol * scale * 1e3
+
+
+def test_torch_generation_guard(gpu, cupf, syn):
+    torch = gpu
+    g = syn.workload_grid("case118")
+    dev = torch.device("cuda")
+    s = make_solver(cupf, g, "FP64", 8)
+    kw = dict(sbus_base_re=torch.tensor(g.p_gen, device=dev),
+              sbus_base_im=torch.zeros(g.n_bus, device=dev, dtype=torch.float64),
+              v0_va=torch.zeros(g.n_bus, device=dev, dtype=torch.float64),
+              v0_vm=torch.tensor(g.vm_set, device=dev))
+    lp = torch.tensor(g.p_load[None], device=dev, requires_grad=True)
+    lq = torch.tensor(g.q_load[None], device=dev, requires_grad=True)
+    va1, vm1 = cupf.solve(lp, lq, s, **kw)
+    va2, vm2 = cupf.solve(lp, lq, s, **kw)
+    with pytest.raises(RuntimeError, match="reused"):
+        vm1.sum().backward()
+    vm2.sum().backward()
+    assert torch.isfinite(lp.grad).all()
+
+
+def test_torch_dtype_rule(gpu, cupf, syn):
+    torch = gpu
+    g = syn.workload_grid("case118")
+    s = make_solver(cupf, g, "FP64", 8)
+    dev = torch.device("cuda")
+    z = torch.zeros((1, g.n_bus), device=dev, dtype=torch.float32)
+    zb = torch.zeros(g.n_bus, device=dev, dtype=torch.float32)
+    with pytest.raises(ValueError, match="float64"):
+        s.solve_torch(zb, zb, z, z, zb, zb + 1, torch.empty_like(z), torch.empty_like(z))
