@@ -112,10 +112,11 @@ struct Arenas {
 
 // Factor one (level, tier) group of `count` fronts (ids at `fronts`, device)
 // for all Bp scenarios.  A = J values [nnz][Bp] (interleaved, FP64).
+// dense_mode for FP32 tier-2 fronts: 0 FMA tiles, 1 MFMA FP32, 2 MFMA 3xBF16.
 template <typename T>
 void lu_factor_group(const LuDevice& d, int tier, const int* fronts, int count, int max_fsz,
                      const double* A, int Bp, Arenas<T> ar, double pivot_eps, int static_piv,
-                     int* sing, hipStream_t s);
+                     int* sing, int dense_mode, hipStream_t s);
 
 // Solve sweeps.  rhs (FP64) and x are [n][Bp]; y [n][Bp] and upd
 // [upd_total][Bp] are work vectors.  gather: y <- P rhs is folded into the

@@ -119,13 +119,13 @@ __global__ __launch_bounds__(kLanes) void k_factor_t0(LuDevice d, const int* __r
 // ===========================================================================
 // Factor, tier 1 (front in LDS) and tier 2 (blocked, in the arena)
 // ===========================================================================
-template <typename T>
+template <typename T, int NT = kFactorBlock>
 __device__ inline void assemble_front(const LuDevice& d, const Arenas<T>& ar, const double* A,
                                       int Bp, int p, int b, int fs, T* F) {
     const int tid = threadIdx.x;
-    for (int e = tid; e < fs * fs; e += kFactorBlock) F[e] = T(0);
+    for (int e = tid; e < fs * fs; e += NT) F[e] = T(0);
     __syncthreads();
-    for (int e = d.ent_ptr[p] + tid; e < d.ent_ptr[p + 1]; e += kFactorBlock)
+    for (int e = d.ent_ptr[p] + tid; e < d.ent_ptr[p + 1]; e += NT)
         F[d.ent_dst[e]] = T(A[(long long)d.ent_src[e] * Bp + b]);
     __syncthreads();
     for (int q = d.child_ptr[p]; q < d.child_ptr[p + 1]; ++q) {
@@ -134,7 +134,7 @@ __device__ inline void assemble_front(const LuDevice& d, const Arenas<T>& ar, co
         long long cs;
         const T* C = front_ptr(d, ar, c, b, Bp, cs);
         const int* loc = d.asm_loc + d.asm_ptr[c];
-        for (int e = tid; e < cuc * cuc; e += kFactorBlock) {
+        for (int e = tid; e < cuc * cuc; e += NT) {
             const int a = e / cuc, bb = e - a * cuc;
             F[loc[a] * fs + loc[bb]] += C[(long long)((cnc + a) * cfs + cnc + bb) * cs];
         }
@@ -295,6 +295,204 @@ __global__ __launch_bounds__(kFactorBlock) void k_factor_t2(LuDevice d,
             }
         }
         __syncthreads();
+    }
+    if (flag) atomicOr(sing + b, 1);
+}
+
+// ===========================================================================
+// Factor, tier 2 on MFMA (FP32 fronts, fsz <= 512)
+//
+// 512 lanes.  Panel width 64: the rows of the column panel are held one per
+// lane in registers during the panel LU (the pivot row is broadcast through a
+// double-buffered LDS row), then stored to LDS as the A operand of the
+// trailing update.  U12 is solved one column per lane.  The trailing update
+// C -= L21 U12 runs on MFMA in 32 x 64 strips per wave:
+//   kMfmaF32   v_mfma_f32_32x32x2_f32;
+//   kMfmaBf16x3 the 3xBF16 split a*b ~ ah*bh + ah*bl + al*bh with
+//               v_mfma_f32_32x32x16_bf16 and FP32 accumulation.
+// Operand layouts (verified on gfx950): A lane l = row l%32, k = l/32 (f32) or
+// k = 8*(l/32) + t, t < 8 (bf16); B lane l = col l%32, same k; accumulator
+// register v of lane l holds row (v/4)*8 + (l/32)*4 + v%4, column l%32.
+// ===========================================================================
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+constexpr int kMfmaF32 = 1;
+constexpr int kMfmaBf16x3 = 2;
+constexpr int kDenseThreads = 512;
+constexpr int kDenseKb = 64;
+constexpr int kDenseMaxFsz = 512;
+constexpr int kPld = kDenseKb + 2;      // panel row stride: A reads conflict-free
+constexpr int kSld = 96;                // stage row stride: B reads conflict-free
+constexpr size_t kDenseLds =
+    size_t(kDenseMaxFsz) * kPld * 4 + size_t(kDenseKb) * kSld * 4 + 2 * kDenseKb * 4;
+
+__device__ inline void split_bf16(float x, __bf16& hi, __bf16& lo) {
+    hi = (__bf16)x;
+    lo = (__bf16)(x - (float)hi);
+}
+
+// Column j of U12 = L11^{-1} A12 (unit lower L11 from the LDS panel), held in
+// registers.  FULL: a 64-wide panel, all straight-line code.  Otherwise rows
+// past kbb are loaded from a clamped (valid) row, carried as don't-care values
+// and written back unchanged; stores run from the last row down so that a
+// clamped row aliasing a real one is overwritten by the real result.
+template <bool FULL>
+__device__ inline void dense_u12(float* G, const float* P, int fs, int k0, int kbb, int j) {
+    float u[kDenseKb];
+#pragma unroll
+    for (int c = 0; c < kDenseKb; ++c)
+        u[c] = G[(long long)(FULL ? k0 + c : min(k0 + c, fs - 1)) * fs + j];
+#pragma unroll
+    for (int c = 1; c < kDenseKb; ++c) {
+        float acc = u[c];
+#pragma unroll
+        for (int q = 0; q < c; ++q) acc -= P[c * kPld + q] * u[q];
+        u[c] = acc;
+    }
+    if constexpr (FULL) {
+#pragma unroll
+        for (int c = 0; c < kDenseKb; ++c) G[(long long)(k0 + c) * fs + j] = u[c];
+    } else {
+#pragma unroll
+        for (int c = kDenseKb - 1; c >= 0; --c) {
+            float* g = G + (long long)min(k0 + c, fs - 1) * fs + j;
+            *g = c < kbb ? u[c] : *g;
+        }
+    }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(kDenseThreads) void k_factor_t2_mfma(LuDevice d,
+                                                                  const int* __restrict__ fronts,
+                                                                  const double* __restrict__ A,
+                                                                  int Bp, Arenas<float> ar,
+                                                                  double eps, int static_piv,
+                                                                  int* __restrict__ sing) {
+    extern __shared__ unsigned char smem_raw[];
+    float* P = reinterpret_cast<float*>(smem_raw);                 // [fsz][kPld]
+    float* stage = P + size_t(kDenseMaxFsz) * kPld;               // [kDenseKb][kSld]
+    float* prow = stage + size_t(kDenseKb) * kSld;                // [2][kDenseKb]
+    int fi, b;
+    decode_block(Bp, fi, b);
+    const int p = fronts[fi];
+    const int fs = d.fsz[p], ncp = d.nc[p];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    float* G = ar.a1 + (long long)b * d.arena1 + d.off[p];
+    assemble_front<float, kDenseThreads>(d, ar, A, Bp, p, b, fs, G);
+    int flag = 0;
+    for (int k0 = 0; k0 < ncp; k0 += kDenseKb) {
+        const int kbb = min(kDenseKb, ncp - k0);
+        const int rows = fs - k0;
+        // ---- panel LU, one row per lane in registers -----------------------
+        const int r = tid;                   // panel row (fs <= 512 rows)
+        float row[kDenseKb];
+#pragma unroll
+        for (int c = 0; c < kDenseKb; ++c)
+            row[c] = (r < rows && c < kbb) ? G[(long long)(k0 + r) * fs + k0 + c] : 0.f;
+#pragma unroll
+        for (int c = 0; c < kDenseKb; ++c) {
+            float* pr = prow + (c & 1) * kDenseKb;
+            if (r == c && c < kbb) {
+                const float g = guard_pivot(row[c], eps, static_piv, flag);
+                row[c] = g;
+#pragma unroll
+                for (int cc = c; cc < kDenseKb; ++cc) pr[cc] = row[cc];
+            }
+            __syncthreads();
+            if (c < kbb && r > c && r < rows) {
+                const float l = row[c] / pr[c];
+                row[c] = l;
+#pragma unroll
+                for (int cc = c + 1; cc < kDenseKb; ++cc) row[cc] -= l * pr[cc];
+            }
+        }
+        if (r < rows) {
+#pragma unroll
+            for (int c = 0; c < kDenseKb; ++c) {
+                if (c < kbb) {
+                    P[r * kPld + c] = row[c];
+                    G[(long long)(k0 + r) * fs + k0 + c] = row[c];
+                } else {
+                    P[r * kPld + c] = 0.f;
+                }
+            }
+        }
+        __syncthreads();
+        // ---- U12 = L11^{-1} A12, one column per lane -------------------------
+        // fs - t0 < kDenseThreads: at most one column per lane.
+        const int t0 = k0 + kbb;
+        if (const int j = t0 + tid; j < fs) {
+            if (kbb == kDenseKb) dense_u12<true>(G, P, fs, k0, kbb, j);
+            else dense_u12<false>(G, P, fs, k0, kbb, j);
+        }
+        __syncthreads();
+        // ---- trailing update C[t0:, t0:] -= L21 U12 on MFMA ----------------
+        const int tn = fs - t0;
+        for (int j0 = t0; j0 < fs; j0 += 64) {
+            for (int e = tid; e < kDenseKb * 64; e += kDenseThreads) {
+                const int c = e >> 6, jj = e & 63;
+                stage[c * kSld + jj] =
+                    (c < kbb && j0 + jj < fs) ? G[(long long)(k0 + c) * fs + j0 + jj] : 0.f;
+            }
+            __syncthreads();
+            const int nstrips = (tn + 31) / 32;
+            for (int st = wave; st < nstrips; st += kDenseThreads / 64) {
+                const int i0 = t0 + st * 32;
+                floatx16 acc0, acc1;
+                // load C (rows i0.., cols j0.. and j0+32..)
+#pragma unroll
+                for (int v = 0; v < 16; ++v) {
+                    const int i = i0 + (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
+                    const int ja = j0 + (lane & 31), jb = ja + 32;
+                    acc0[v] = (i < fs && ja < fs) ? G[(long long)i * fs + ja] : 0.f;
+                    acc1[v] = (i < fs && jb < fs) ? G[(long long)i * fs + jb] : 0.f;
+                }
+                const int pr_row = i0 - k0 + (lane & 31);   // panel row of this lane's A rows
+                const bool arow_ok = (i0 + (lane & 31)) < fs;
+                if (MODE == kMfmaF32) {
+                    for (int kk = 0; kk < kbb; kk += 2) {
+                        const int k = kk + (lane >> 5);
+                        const float a = arow_ok ? -P[pr_row * kPld + k] : 0.f;
+                        const float b0 = stage[k * kSld + (lane & 31)];
+                        const float b1 = stage[k * kSld + 32 + (lane & 31)];
+                        acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b0, acc0, 0, 0, 0);
+                        acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b1, acc1, 0, 0, 0);
+                    }
+                } else {
+                    for (int kk = 0; kk < kbb; kk += 16) {
+                        const int kb8 = kk + 8 * (lane >> 5);
+                        bf16x8 ah, al, b0h, b0l, b1h, b1l;
+#pragma unroll
+                        for (int t = 0; t < 8; ++t) {
+                            const float a = arow_ok ? -P[pr_row * kPld + kb8 + t] : 0.f;
+                            __bf16 h, lo;
+                            split_bf16(a, h, lo);
+                            ah[t] = h; al[t] = lo;
+                            split_bf16(stage[(kb8 + t) * kSld + (lane & 31)], h, lo);
+                            b0h[t] = h; b0l[t] = lo;
+                            split_bf16(stage[(kb8 + t) * kSld + 32 + (lane & 31)], h, lo);
+                            b1h[t] = h; b1l[t] = lo;
+                        }
+                        acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, b0h, acc0, 0, 0, 0);
+                        acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, b0l, acc0, 0, 0, 0);
+                        acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, b0h, acc0, 0, 0, 0);
+                        acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, b1h, acc1, 0, 0, 0);
+                        acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, b1l, acc1, 0, 0, 0);
+                        acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, b1h, acc1, 0, 0, 0);
+                    }
+                }
+#pragma unroll
+                for (int v = 0; v < 16; ++v) {
+                    const int i = i0 + (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
+                    const int ja = j0 + (lane & 31), jb = ja + 32;
+                    if (i < fs && ja < fs) G[(long long)i * fs + ja] = acc0[v];
+                    if (i < fs && jb < fs) G[(long long)i * fs + jb] = acc1[v];
+                }
+            }
+            __syncthreads();
+        }
     }
     if (flag) atomicOr(sing + b, 1);
 }
@@ -555,8 +753,30 @@ __global__ __launch_bounds__(256) void k_scatter(const int* __restrict__ perm, i
 template <typename T>
 void lu_factor_group(const LuDevice& d, int tier, const int* fronts, int count, int max_fsz,
                      const double* A, int Bp, Arenas<T> ar, double pivot_eps, int static_piv,
-                     int* sing, hipStream_t s) {
+                     int* sing, int dense_mode, hipStream_t s) {
     if (count <= 0) return;
+    if constexpr (sizeof(T) == 4) {
+    if (tier == 2 && dense_mode != 0 && max_fsz <= kDenseMaxFsz) {
+        static bool attr_set = false;
+        if (!attr_set) {
+            CUPF_HIP(hipFuncSetAttribute((const void*)k_factor_t2_mfma<kMfmaF32>,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, int(kDenseLds)));
+            CUPF_HIP(hipFuncSetAttribute((const void*)k_factor_t2_mfma<kMfmaBf16x3>,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, int(kDenseLds)));
+            attr_set = true;
+        }
+        Arenas<float> af{reinterpret_cast<float*>(ar.a0), reinterpret_cast<float*>(ar.a1)};
+        const dim3 grid(unsigned(count) * unsigned(Bp));
+        if (dense_mode == kMfmaBf16x3)
+            hipLaunchKernelGGL(k_factor_t2_mfma<kMfmaBf16x3>, grid, dim3(kDenseThreads), kDenseLds, s,
+                               d, fronts, A, Bp, af, pivot_eps, static_piv, sing);
+        else
+            hipLaunchKernelGGL(k_factor_t2_mfma<kMfmaF32>, grid, dim3(kDenseThreads), kDenseLds, s, d,
+                               fronts, A, Bp, af, pivot_eps, static_piv, sing);
+        CUPF_HIP(hipGetLastError());
+        return;
+    }
+    }
     if (tier == 0) {
         if (max_fsz > kTier0Max) throw std::runtime_error("lu_factor_group: tier-0 front too large");
         hipLaunchKernelGGL(k_factor_t0<T>, dim3(count, Bp / kLanes), dim3(kLanes), 0, s, d, fronts,
@@ -634,7 +854,7 @@ void lu_solve_scatter(const LuDevice& d, const T* y, int Bp, TX* x, hipStream_t 
 
 #define CUPF_LU_INST(T)                                                                          \
     template void lu_factor_group<T>(const LuDevice&, int, const int*, int, int, const double*,  \
-                                     int, Arenas<T>, double, int, int*, hipStream_t);            \
+                                     int, Arenas<T>, double, int, int*, int, hipStream_t);       \
     template void lu_solve_forward_group<T>(const LuDevice&, int, const int*, int, int,           \
                                             Arenas<T>, const double*, T*, T*, int, bool,         \
                                             hipStream_t);                                         \

@@ -97,6 +97,7 @@ struct NewtonSolver::Impl {
     int refine = 0;        // FP64 refinement steps of the step
     bool static_piv = true;
     double piv_eps = 1e-8;
+    int dense_mode = 0;    // FP32 dense fronts: 1 MFMA FP32, 2 MFMA 3xBF16
 
     // host structure
     int n = 0, nnzY = 0;
@@ -180,7 +181,7 @@ struct NewtonSolver::Impl {
                 int cnt, mf;
                 const int* f = group(L, t, cnt, mf);
                 kern::lu_factor_group<T>(lu, t, f, cnt, mf, J.as<double>(), Bp, ar, piv_eps,
-                                         static_piv ? 1 : 0, sing.as<int>(), s);
+                                         static_piv ? 1 : 0, sing.as<int>(), dense_mode, s);
             }
     }
     void factor() {
@@ -387,6 +388,7 @@ NewtonSolver::NewtonSolver(const NewtonOptions& options) : impl_(new Impl) {
             m.front64 = false; m.dx32 = true; m.refine = 0;
             break;
     }
+    if (!m.front64) m.dense_mode = options.custom.precision == CustomPrecision::TF32 ? 2 : 1;
     m.static_piv = options.custom.pivot_strategy == CustomPivotStrategy::StaticDiagonalShift &&
                    options.custom.shift_retry_epsilon > 0.0;
     m.piv_eps = options.custom.shift_retry_epsilon;

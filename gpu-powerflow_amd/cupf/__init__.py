@@ -12,6 +12,12 @@ from __future__ import annotations
 import importlib
 from pathlib import Path
 
+# torch first: its wheel bundles a HIP runtime with the same soname
+# (libamdhip64.so.7) as the one libcupf_hip.so links.  Loaded in this order
+# the library binds to the runtime torch already holds; the other order puts
+# two HIP/HSA runtimes into one process.
+import torch  # noqa: F401
+
 _PKG_DIR = Path(__file__).resolve().parent
 
 

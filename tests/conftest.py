@@ -7,6 +7,7 @@ from pathlib import Path
 
 import numpy as np
 import pytest
+import torch  # noqa: F401  (before any HIP library: one HIP runtime per process)
 
 ROOT = Path(__file__).resolve().parents[1]
 for p in (ROOT / "gpu-powerflow_amd", ROOT, ROOT / "tests"):

@@ -26,18 +26,26 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_default_options_roundtrip():
+    # mirrors cupf_custom_config / cupf_newton_options of include/cupf_hip.h,
+    # nested exactly as declared (the nested struct is 8-byte aligned)
+    class Custom(ctypes.Structure):
+        _fields_ = [("precision", ctypes.c_int32), ("matching", ctypes.c_int32),
+                    ("pivot", ctypes.c_int32), ("serial", ctypes.c_int32), ("cap", ctypes.c_int32),
+                    ("eps", ctypes.c_double), ("refine", ctypes.c_int32)]
+
     class Opts(ctypes.Structure):
         _fields_ = [("backend", ctypes.c_int32), ("compute", ctypes.c_int32),
                     ("cuda_linear_solver", ctypes.c_int32), ("cpu_linear_solver", ctypes.c_int32),
                     ("cuda_jacobian", ctypes.c_int32), ("a", ctypes.c_int32), ("b", ctypes.c_int32),
-                    ("c", ctypes.c_int32), ("d", ctypes.c_double), ("precision", ctypes.c_int32),
-                    ("matching", ctypes.c_int32), ("pivot", ctypes.c_int32), ("serial", ctypes.c_int32),
-                    ("cap", ctypes.c_int32), ("eps", ctypes.c_double), ("refine", ctypes.c_int32),
+                    ("c", ctypes.c_int32), ("d", ctypes.c_double), ("custom", Custom),
                     ("graph", ctypes.c_int32)]
-    o = Opts()
+    assert ctypes.sizeof(Opts) == 88
+    buf = (ctypes.c_ubyte * 256)(*([0xAB] * 256))
+    o = Opts.from_buffer(buf)
     capi.lib().cupf_default_newton_options(ctypes.byref(o))
+    assert all(v == 0xAB for v in buf[ctypes.sizeof(Opts):]), "wrote past the declared struct"
     assert o.backend == 1 and o.compute == 2 and o.cuda_linear_solver == 1
-    assert o.cap == 8 and o.eps == 1e-8 and o.refine == 2
+    assert o.custom.cap == 8 and o.custom.eps == 1e-8 and o.custom.refine == 2
 
 
 def test_host_analysis_errors_map_to_value_error():

@@ -21,9 +21,15 @@ pytestmark = pytest.mark.gpu
 
 
 def make_solver(cupf, g, compute="FP64", cap=64, refine=2):
+    """compute "MixedTF32": Mixed with the 3xBF16 trailing products."""
     o = cupf.NewtonOptions()
-    o.compute = getattr(cupf.ComputePolicy, compute)
-    o.custom.precision = cupf.CustomPrecision.FP64 if compute == "FP64" else cupf.CustomPrecision.FP32
+    if compute == "MixedTF32":
+        o.compute = cupf.ComputePolicy.Mixed
+        o.custom.precision = cupf.CustomPrecision.TF32
+    else:
+        o.compute = getattr(cupf.ComputePolicy, compute)
+        o.custom.precision = (cupf.CustomPrecision.FP64 if compute == "FP64"
+                              else cupf.CustomPrecision.FP32)
     o.custom.max_panel_width = cap
     o.custom.refine_steps = refine
     s = cupf.NewtonSolver(o)
@@ -86,7 +92,9 @@ def test_jacobian_vs_oracle(gpu, cupf, oracle, request, which):
 
 @pytest.mark.parametrize("compute,cap,tol", [("FP64", 8, 1e-12), ("FP64", 64, 1e-12),
                                              ("FP64", 1 << 20, 1e-12), ("Mixed", 64, 1e-9),
-                                             ("Mixed", 8, 1e-9)])
+                                             ("Mixed", 8, 1e-9), ("Mixed", 1 << 20, 1e-9),
+                                             ("MixedTF32", 64, 1e-9),
+                                             ("MixedTF32", 1 << 20, 1e-9)])
 @pytest.mark.parametrize("transpose", [False, True])
 def test_factor_solve_vs_oracle(gpu, cupf, oracle, g2000, compute, cap, tol, transpose):
     g = g2000
@@ -109,7 +117,8 @@ def test_factor_solve_vs_oracle(gpu, cupf, oracle, g2000, compute, cap, tol, tra
         np.testing.assert_allclose(x[b], xo, rtol=0, atol=1e3 * tol * np.abs(xo).max())
 
 
-@pytest.mark.parametrize("compute,cap", [("FP64", 8), ("FP64", 64), ("Mixed", 64), ("Mixed", 1 << 20)])
+@pytest.mark.parametrize("compute,cap", [("FP64", 8), ("FP64", 64), ("Mixed", 64), ("Mixed", 1 << 20),
+                                         ("MixedTF32", 64)])
 def test_nr_matches_oracle(gpu, cupf, syn, oracle, g2000, compute, cap):
     g = g2000
     B = 6
