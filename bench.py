@@ -2,10 +2,13 @@
 """Headline benchmark: batched Newton-Raphson power flow, 2,000 buses x 1024.
 
 One step = one batched solve of the whole batch to ||F||inf <= 1e-8, inputs
-(Sbus, V0) already resident in HBM.  The plain FP64 flat-start solve (the
-drop-in) and the mixed-precision solve (FP32 fronts + FP64 refinement) run on
-the same inputs; `value` is the solves/s of the mixed form, the FP64 rate and
-the ratio are reported beside it.  The dominant kernel (the batched
+(Sbus, V0) already resident in HBM.  Forms timed on the same inputs: plain
+FP64 flat-start NR (the drop-in), mixed precision (FP32 fronts on MFMA + FP64
+refinement), the 3xBF16 variant, and the learned warm start + mixed precision
+(MLP trained beforehand, untimed, on 65,536 of the solver's own solutions at
+load factor U(0.6, 1.4), seed 2; the timed batch is the U(0.7, 1.3), seed 1
+batch).  `value` is the solves/s of the learned + mixed form, the FP64 rate
+and the ratio are reported beside it.  The dominant kernel (the batched
 multifrontal factorisation) is timed with HIP events on the solver's stream
 for the roofline figure.  The CPU baseline is the FP64 oracle NR (KLU-method
 sparse LU) on a bounded sample of the same scenarios, one scenario per host
@@ -29,6 +32,9 @@ sys.path.insert(0, str(ROOT / "gpu-powerflow_amd"))
 
 METRIC = "solves/sec @ ||F||inf<=1e-8, 2k-bus x1024: learned+mixed vs plain FP64"
 HBM_PEAK_GBS = 8000.0
+FP32_MFMA_PEAK_TFLOPS = 157.3
+FP64_PEAK_TFLOPS = 78.6
+TRAFFIC_FILE = ROOT / "profiles" / "traffic.json"
 
 
 def parse():
@@ -44,7 +50,67 @@ def parse():
                    help="oracle scenarios per host thread in the CPU baseline sample")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-check", action="store_true")
+    p.add_argument("--forms", default="fp64,mixed,bf16x3,learned",
+                   help="comma list of solver forms to time (fp64, mixed, bf16x3, learned)")
+    p.add_argument("--train-samples", type=int, default=65536,
+                   help="warm-start training set: NR solutions at load factor U(0.6, 1.4), seed 2")
+    p.add_argument("--epochs", type=int, default=60)
+    p.add_argument("--warm-iters", type=int, default=4, help="warm-start NR budget before fallback")
+    p.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                   help="only run a plain device copy of MIB MiB three times (counter scale)")
     return p.parse_args()
+
+
+def calibrate_copy(torch, mib):
+    """A plain copy kernel of known size: reads and writes mib MiB per launch."""
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_per_launch": n * 4, "launches": 3}), flush=True)
+
+
+def dense_roofline(stats, stage, B, front_bytes):
+    """Roofline of the dense-front (tier-2) factor launches from live HIP-event
+    times: algorithmic flops = sum over tier-2 fronts of
+    sum_{k<nc} [2 (fsz-k-1)^2 + (fsz-k-1)] per scenario; minimum bytes = each
+    front element written and read once plus the J entries it assembles."""
+    ms, launches = stage
+    groups = max(1, stats["dense_groups"])
+    if launches == 0:
+        return None
+    avg_s = ms / launches / 1e3
+    flops = stats["dense_flops"] * B / groups
+    nbytes = stats["dense_bytes"] * B / groups
+    intensity = stats["dense_flops"] / max(1.0, stats["dense_bytes"])
+    peak_f = FP32_MFMA_PEAK_TFLOPS if front_bytes == 4 else FP64_PEAK_TFLOPS
+    ridge = peak_f * 1e12 / (HBM_PEAK_GBS * 1e9)
+    if intensity >= ridge:
+        r = {"bound": "mfma", "achieved": flops / avg_s / 1e12, "peak": peak_f, "unit": "TFLOP/s"}
+    else:
+        r = {"bound": "hbm", "achieved": nbytes / avg_s / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s"}
+    r["frac"] = r["achieved"] / r["peak"]
+    r.update({"avg_launch_ms": avg_s * 1e3, "launches_timed": launches,
+              "flops_per_launch": flops, "bytes_per_launch": nbytes,
+              "intensity_flop_per_byte": intensity, "ridge_flop_per_byte": ridge,
+              "hbm_gbs": nbytes / avg_s / 1e9})
+    return r
+
+
+def traffic_for(kernel, workload):
+    """HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    summary (scaled by the plain-copy factor), when one exists for this kernel
+    and workload."""
+    try:
+        t = json.loads(TRAFFIC_FILE.read_text())
+        e = t[kernel]
+        if e.get("workload") != workload:
+            return None
+        return float(e["traffic_bytes_per_launch"])
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def make_solver(cupf, grid, compute, precision, cap, refine):
@@ -114,6 +180,9 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(1)
+    if args.calibrate_copy:
+        calibrate_copy(torch, args.calibrate_copy)
+        return
     import cupf
     from cupf import synthetic as syn
 
@@ -129,77 +198,103 @@ def main():
     Vout = torch.empty_like(V0)
     cfg = cupf.NRConfig()
 
-    plain = make_solver(cupf, grid, cupf.ComputePolicy.FP64, cupf.CustomPrecision.FP64,
-                        args.cap, 0)
-    ms_plain, it_plain, st_plain = timed_solves(torch, plain, S, V0, Vout, B, n, args.steps,
-                                                args.warmup, cfg, stage_timing=True)
-    V_plain = Vout.cpu().numpy()
-
-    mixed = make_solver(cupf, grid, cupf.ComputePolicy.Mixed, cupf.CustomPrecision.FP32,
-                        args.cap, args.refine)
-    ms_mixed, it_mixed, st_mixed = timed_solves(torch, mixed, S, V0, Vout, B, n, args.steps,
-                                                args.warmup, cfg, stage_timing=True)
-    V_mixed = Vout.cpu().numpy()
-    stats = mixed.stats()
-
-    # roofline of the factorisation kernel (mixed form: FP32 fronts, FP64 J)
-    fac_ms, fac_n = st_mixed["factor"]
-    fac_avg_s = fac_ms / max(1, fac_n) / 1e3
-    sum_fsz2 = None
-    try:
-        sys.path.insert(0, str(ROOT / "tests"))
-        import capi
-        a = capi.analyze_jacobian(n, grid.indptr, grid.indices, grid.pv, grid.pq)
-        perm = capi.order_jacobian(n, grid.indptr, grid.indices, grid.pv, grid.pq, a["dim"])
-        sum_fsz2 = capi.plan_figures(a["row_ptr"], a["col_idx"], perm, args.cap)["sum_fsz2"]
-    except Exception:
-        sum_fsz2 = float(stats["arena_elems"])
-    bytes_per_scen = 8.0 * stats["nnz_j"] + 2.0 * 4.0 * sum_fsz2
-    achieved = bytes_per_scen * B / fac_avg_s / 1e9
+    forms = [f.strip() for f in args.forms.split(",") if f.strip()]
+    spec = {"fp64": (cupf.ComputePolicy.FP64, cupf.CustomPrecision.FP64, 0),
+            "mixed": (cupf.ComputePolicy.Mixed, cupf.CustomPrecision.FP32, args.refine),
+            "bf16x3": (cupf.ComputePolicy.Mixed, cupf.CustomPrecision.TF32, args.refine),
+            "learned": (cupf.ComputePolicy.Mixed, cupf.CustomPrecision.FP32, args.refine)}
+    res = {}
+    training = None
+    for f in forms:
+        comp, prec, ref = spec[f]
+        sv = make_solver(cupf, grid, comp, prec, args.cap, ref)
+        if f == "learned":
+            # training data: the solver's own NR solutions (flat start) of
+            # load factor U(0.6, 1.4), seed 2 -- disjoint from the timed batch
+            Nt = args.train_samples
+            St = torch.from_numpy(grid.sbus_scaled(syn.load_factors(Nt, n, 0.6, 1.4, seed=2))).to(dev)
+            V0t = torch.from_numpy(grid.flat_start(Nt)).to(dev)
+            mlp, info = cupf.train_warm_start(sv, St, V0t, grid.pv, grid.pq, hidden=512, depth=3,
+                                              epochs=args.epochs, batch=1024, lr=1e-3,
+                                              weight_decay=1e-4, seed=0)
+            del St, V0t
+            sv.set_warm_start(mlp, args.warm_iters)
+            training = {"samples": info["samples"], "load_factor": [0.6, 1.4], "seed": 2,
+                        "epochs": args.epochs, "batch": 1024, "hidden": [512, 512, 512],
+                        "activation": "gelu", "weights": "bf16", "accumulate": "fp32",
+                        "solve_s": round(info["solve_s"], 2), "train_s": round(info["train_s"], 2),
+                        "loss_first": float(info["epoch_loss"][0]),
+                        "loss_last": float(info["epoch_loss"][-1])}
+        ms, it, st = timed_solves(torch, sv, S, V0, Vout, B, n, args.steps, args.warmup, cfg,
+                                  stage_timing=True)
+        res[f] = {"s_per_step": ms, "iterations": it[-1], "stages": st, "stats": sv.stats(),
+                  "V": Vout.cpu().numpy()}
+        if f == "learned":
+            res[f]["warm_iterations"] = res[f]["stats"]["last_warm_iterations"]
+            res[f]["fallbacks"] = res[f]["stats"]["last_warm_fallbacks"]
+        del sv
+    head = "learned" if "learned" in res else ("mixed" if "mixed" in res else forms[0])
+    h = res[head]
+    workload = f"{args.workload}x{B}_cap{args.cap}"
+    kernel = {"fp64": "k_factor_t2<double>", "mixed": "k_factor_t2_mfma<1>",
+              "bf16x3": "k_factor_t2_mfma<2>", "learned": "k_factor_t2_mfma<1>"}[head]
+    roof = dense_roofline(h["stats"], h["stages"]["factor_dense"], B, 8 if head == "fp64" else 4)
+    if roof is not None:
+        roof["kernel"] = kernel
+        roof["traffic"] = traffic_for(kernel, workload)
 
     out = {
         "metric": METRIC,
-        "value": B / ms_mixed,
+        "value": B / h["s_per_step"],
         "unit": "solves/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_mixed * 1e3,
+        "ms_per_step": h["s_per_step"] * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64",
+        "dtype": "f32+f64" if head != "fp64" else "f64",
         "data": "synthetic",
         "config": {
             "workload": f"{args.workload}: {n}-bus/{grid.meta['n_branch']}-branch synthetic "
                         f"meshed grid x{B} scenarios, load factor U({lo},{hi}), flat start",
-            "batch": B, "n_bus": n, "dim": stats["dim"], "tolerance": cfg.tolerance,
-            "panel_cap": args.cap, "form": f"mixed: FP32 fronts + {args.refine} FP64 refinement steps",
-            "learned_warm_start": False,
+            "batch": B, "n_bus": n, "dim": h["stats"]["dim"], "tolerance": cfg.tolerance,
+            "panel_cap": args.cap,
+            "form": {"learned": f"learned MLP warm start (<= {args.warm_iters} NR iterations, "
+                                f"flat-start fallback) + FP32 fronts, FP32-MFMA dense trailing "
+                                f"updates, {args.refine} FP64 refinement steps",
+                     "mixed": f"FP32 fronts, FP32-MFMA dense trailing updates, "
+                              f"{args.refine} FP64 refinement steps",
+                     "bf16x3": f"FP32 fronts, 3xBF16 MFMA dense trailing updates, "
+                               f"{args.refine} FP64 refinement steps",
+                     "fp64": "FP64 fronts"}[head],
+            "learned_warm_start": head == "learned",
         },
-        "plain_fp64": {"solves_per_s": B / ms_plain, "ms_per_step": ms_plain * 1e3,
-                       "iterations": it_plain[-1]},
-        "mixed": {"iterations": it_mixed[-1]},
-        "speedup_vs_plain_fp64": ms_plain / ms_mixed,
-        "stage_ms_per_step": {k: v[0] / args.steps for k, v in st_mixed.items()},
-        "stage_ms_per_step_fp64": {k: v[0] / args.steps for k, v in st_plain.items()},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_factor<double,float>",
-                     "avg_launch_ms": fac_avg_s * 1e3,
-                     "bytes_per_launch": bytes_per_scen * B,
-                     "flops_per_launch": stats["factor_flops"] * B},
+        "forms": {f: {"solves_per_s": B / r["s_per_step"], "ms_per_step": r["s_per_step"] * 1e3,
+                      "iterations": r["iterations"],
+                      **({"warm_iterations": r["warm_iterations"], "fallbacks": r["fallbacks"]}
+                         if "fallbacks" in r else {}),
+                      "stage_ms_per_step": {k: v[0] / args.steps for k, v in r["stages"].items()}}
+                  for f, r in res.items()},
+        "warm_start_training": training,
+        "roofline": roof,
     }
+    if "fp64" in res and head != "fp64":
+        out["speedup_vs_plain_fp64"] = res["fp64"]["s_per_step"] / h["s_per_step"]
+    if "learned" in res and "mixed" in res:
+        out["speedup_learned_vs_mixed_flat"] = res["mixed"]["s_per_step"] / res["learned"]["s_per_step"]
     if not args.no_check:
         from oracle import pyoracle as po
         k = min(4, B)
         Vo, io, no, so = po.nr_solve_batch(n, grid.indptr, grid.indices, grid.data, S_host[:k],
                                            V0_host[:k], grid.pv, grid.pq)
+
         def err(V):
             return {"va": float(np.abs(np.angle(V[:k]) - np.angle(Vo)).max()),
                     "vm": float(np.abs(np.abs(V[:k]) - np.abs(Vo)).max())}
         out["accuracy_vs_oracle"] = {"scenarios": k, "oracle_iterations": io.tolist(),
-                                     "plain_fp64": err(V_plain), "mixed": err(V_mixed)}
+                                     **{f: err(r["V"]) for f, r in res.items()}}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(grid, S_host, V0_host, args.cpu_per_core)
     print(json.dumps(out), flush=True)

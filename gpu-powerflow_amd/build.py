@@ -41,6 +41,8 @@ SOURCES = {
     "newton_solver.cpp": [],
     "analyze.cpp": [],
     "capi.cpp": [],
+    "mlp_kernels.hip": [],
+    "warm_start.cpp": [],
 }
 
 

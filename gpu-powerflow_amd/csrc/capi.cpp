@@ -5,6 +5,7 @@
 
 #include "analyze.hpp"
 #include "newton_solver.hpp"
+#include "warm_start.hpp"
 
 #include <complex>
 #include <cstring>
@@ -13,6 +14,10 @@
 
 struct cupf_solver {
     NewtonSolver* impl;
+};
+
+struct cupf_warm_start {
+    cupf::WarmStartMLP* impl;
 };
 
 namespace {
@@ -41,6 +46,13 @@ NewtonSolver& solver_of(cupf_solver* s) {
     if (!s || !s->impl) throw std::invalid_argument("null solver handle");
     return *s->impl;
 }
+
+cupf::WarmStartMLP& mlp_of(cupf_warm_start* m) {
+    if (!m || !m->impl) throw std::invalid_argument("null warm-start handle");
+    return *m->impl;
+}
+
+hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
 
 NRConfig to_cfg(const cupf_nr_config* c) {
     NRConfig r;
@@ -303,6 +315,10 @@ int cupf_solver_get_stats(cupf_solver* solver, cupf_solver_stats* out) {
         out->dim = s.dim; out->nnz_j = s.nnz_j; out->num_fronts = s.num_fronts;
         out->max_front = s.max_front; out->max_pivots = s.max_pivots;
         out->arena_elems = s.arena_elems; out->lnz = s.lnz; out->factor_flops = s.factor_flops;
+        out->dense_flops = s.dense_flops; out->dense_bytes = s.dense_bytes;
+        out->dense_groups = s.dense_groups;
+        out->last_warm_iterations = s.last_warm_iterations;
+        out->last_warm_fallbacks = s.last_warm_fallbacks;
     });
 }
 
@@ -424,6 +440,116 @@ int cupf_plan_figures(int32_t n, const int32_t* row_ptr, const int32_t* col_idx,
         out5[2] = p.max_fsz;
         out5[3] = p.num_fronts;
         out5[4] = double(p.lnz);
+    });
+}
+
+// ---- learned warm start ------------------------------------------------
+int cupf_mlp_create(int32_t n_bus, const int32_t* pv, int32_t n_pv, const int32_t* pq,
+                    int32_t n_pq, int32_t hidden, int32_t depth, uint64_t seed, int32_t max_batch,
+                    cupf_warm_start** out) {
+    return guarded([&] {
+        if (!out) throw std::invalid_argument("null output handle");
+        *out = nullptr;
+        auto* m = new cupf::WarmStartMLP(n_bus, pv, n_pv, pq, n_pq, hidden, depth, seed, max_batch);
+        *out = new cupf_warm_start{m};
+    });
+}
+
+int cupf_mlp_destroy(cupf_warm_start* mlp) {
+    return guarded([&] {
+        if (!mlp) return;
+        delete mlp->impl;
+        delete mlp;
+    });
+}
+
+int cupf_mlp_sizes(cupf_warm_start* mlp, int64_t* n_params, int32_t* width_in, int32_t* width_out,
+                   int64_t* n_samples) {
+    return guarded([&] {
+        auto& m = mlp_of(mlp);
+        if (n_params) *n_params = m.num_params();
+        if (width_in) *width_in = m.width_in();
+        if (width_out) *width_out = m.width_out();
+        if (n_samples) *n_samples = m.num_samples();
+    });
+}
+
+int cupf_mlp_add_samples(cupf_warm_start* mlp, const void* sbus, int64_t sbus_stride,
+                         const void* V, int64_t V_stride, int32_t count, void* stream) {
+    return guarded([&] {
+        mlp_of(mlp).add_samples(static_cast<const double2*>(sbus), sbus_stride,
+                                static_cast<const double2*>(V), V_stride, count, as_stream(stream));
+    });
+}
+
+int cupf_mlp_finalize(cupf_warm_start* mlp, void* stream) {
+    return guarded([&] { mlp_of(mlp).finalize_dataset(as_stream(stream)); });
+}
+
+int cupf_mlp_train(cupf_warm_start* mlp, int32_t epochs, int32_t batch, float lr,
+                   float weight_decay, uint64_t seed, float* epoch_loss, void* stream) {
+    return guarded([&] {
+        mlp_of(mlp).train(epochs, batch, lr, weight_decay, seed, epoch_loss, as_stream(stream));
+    });
+}
+
+int cupf_mlp_predict_v0(cupf_warm_start* mlp, const void* sbus, int64_t sbus_stride,
+                        const void* V_ref, int64_t V_ref_stride, int32_t count, void* V0_out,
+                        int64_t V0_stride, void* stream) {
+    return guarded([&] {
+        mlp_of(mlp).predict_v0(static_cast<const double2*>(sbus), sbus_stride,
+                               static_cast<const double2*>(V_ref), V_ref_stride, count,
+                               static_cast<double2*>(V0_out), V0_stride, as_stream(stream));
+        CUPF_HIP(hipStreamSynchronize(as_stream(stream)));
+    });
+}
+
+int cupf_mlp_forward_host(cupf_warm_start* mlp, const float* x, int32_t count, float* y) {
+    return guarded([&] {
+        if (!x || !y) throw std::invalid_argument("null array");
+        mlp_of(mlp).forward_host(x, count, y);
+    });
+}
+
+int cupf_mlp_loss_and_grads_host(cupf_warm_start* mlp, const float* x, const float* t,
+                                 int32_t count, float* loss, float* grads) {
+    return guarded([&] {
+        if (!x || !t || !grads) throw std::invalid_argument("null array");
+        const float l = mlp_of(mlp).loss_and_grads_host(x, t, count, grads);
+        if (loss) *loss = l;
+    });
+}
+
+int cupf_mlp_get_params(cupf_warm_start* mlp, float* out) {
+    return guarded([&] {
+        if (!out) throw std::invalid_argument("null array");
+        mlp_of(mlp).get_params(out);
+    });
+}
+
+int cupf_mlp_set_params(cupf_warm_start* mlp, const float* in) {
+    return guarded([&] {
+        if (!in) throw std::invalid_argument("null array");
+        mlp_of(mlp).set_params(in);
+    });
+}
+
+int cupf_mlp_get_normalization(cupf_warm_start* mlp, float* x_mean, float* x_std, float* y_mean,
+                               float* y_std) {
+    return guarded([&] { mlp_of(mlp).get_normalization(x_mean, x_std, y_mean, y_std); });
+}
+
+int cupf_mlp_set_normalization(cupf_warm_start* mlp, const float* x_mean, const float* x_std,
+                               const float* y_mean, const float* y_std) {
+    return guarded([&] {
+        if (!x_mean || !x_std || !y_mean || !y_std) throw std::invalid_argument("null array");
+        mlp_of(mlp).set_normalization(x_mean, x_std, y_mean, y_std);
+    });
+}
+
+int cupf_solver_set_warm_start(cupf_solver* solver, cupf_warm_start* mlp, int32_t max_warm_iter) {
+    return guarded([&] {
+        solver_of(solver).set_warm_start(mlp ? &mlp_of(mlp) : nullptr, max_warm_iter);
     });
 }
 

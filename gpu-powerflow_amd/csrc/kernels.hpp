@@ -63,6 +63,8 @@ template <typename TX>
 void axpy_update(const TX* ddx, TX* dx, long long count, hipStream_t s);
 // max over scenarios b < B of normF[b] -> out[0]
 void batch_max(const double* normF, int B, double* out, hipStream_t s);
+// the same over the finite entries only; -1 when none is finite
+void batch_max_finite(const double* normF, int B, double* out, hipStream_t s);
 
 // Torch bridge (cuPF torch_bridge_kernels.cu:116-168): S = base - load and V0
 // from a batch-shared (Va, Vm); outputs Va/Vm cast to the tensor dtype.

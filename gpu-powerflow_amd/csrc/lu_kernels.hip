@@ -29,6 +29,8 @@
 #include "kernels.hpp"
 #include "hip_common.hpp"
 
+#include <algorithm>
+
 namespace cupf::kern {
 namespace {
 
@@ -193,8 +195,7 @@ __global__ __launch_bounds__(kFactorBlock) void k_factor_t2(LuDevice d,
     const int p = fronts[fi];
     const int fs = d.fsz[p], ncp = d.nc[p];
     const int tid = threadIdx.x;
-    T* G = ar.a1 + (long long)b * d.arena1 + d.off[p];
-    assemble_front(d, ar, A, Bp, p, b, fs, G);
+    T* G = ar.a1 + (long long)b * d.arena1 + d.off[p];   // assembled by k_assemble_rows
     int flag = 0;
     int kb = kLdsT / fs;
     if (kb > kKb) kb = kKb;
@@ -300,6 +301,78 @@ __global__ __launch_bounds__(kFactorBlock) void k_factor_t2(LuDevice d,
 }
 
 // ===========================================================================
+// Assembly of tier-2 fronts by row blocks (before the factor kernel)
+//
+// One workgroup per (front, row block of kAsmRows rows, scenario).  The
+// block's rows are built in LDS -- zeroed, original entries stored (entries
+// are sorted by destination, so a row range is a contiguous entry range),
+// every child contribution row that lands in the range added (children in a
+// fixed order) -- and written to the arena once, coalesced.  Each front
+// element is written exactly once and each child contribution element read
+// once; there is no read-modify-write on global memory, and the small LDS
+// footprint keeps many workgroups per CU to hide memory latency.
+// ===========================================================================
+constexpr int kAsmThreads = 256;
+constexpr int kAsmRows = 8;
+
+template <typename T>
+__global__ __launch_bounds__(kAsmThreads) void k_assemble_rows(LuDevice d,
+                                                                const int* __restrict__ fronts,
+                                                                const double* __restrict__ A,
+                                                                int Bp, Arenas<T> ar, int nrb,
+                                                                int rb_rows) {
+    extern __shared__ unsigned char smem_raw[];
+    T* F = reinterpret_cast<T*>(smem_raw);   // [rb_rows][fs]
+    // lin -> (front, row block, scenario), scenario fastest inside one XCD
+    const long long lin = blockIdx.x;
+    const long long per = (long long)gridDim.x / kXcd;
+    const long long w = (lin % kXcd) * per + lin / kXcd;
+    const int b = int(w % Bp);
+    const long long rest = w / Bp;
+    const int rb = int(rest % nrb);
+    const int fi = int(rest / nrb);
+    const int p = fronts[fi];
+    const int fs = d.fsz[p];
+    const int i0 = rb * rb_rows;
+    if (i0 >= fs) return;                       // whole block exits together
+    const int i1 = min(fs, i0 + rb_rows);
+    const int nr = i1 - i0;
+    const int tid = threadIdx.x;
+    for (int e = tid; e < nr * fs; e += kAsmThreads) F[e] = T(0);
+    // entry range with destination in [i0*fs, i1*fs): binary search (sorted)
+    int lo = d.ent_ptr[p], hi = d.ent_ptr[p + 1];
+    {
+        int a = lo, z = hi;
+        while (a < z) { const int m = (a + z) >> 1; if (d.ent_dst[m] < i0 * fs) a = m + 1; else z = m; }
+        lo = a;
+        z = hi;
+        while (a < z) { const int m = (a + z) >> 1; if (d.ent_dst[m] < i1 * fs) a = m + 1; else z = m; }
+        hi = a;
+    }
+    __syncthreads();
+    for (int e = lo + tid; e < hi; e += kAsmThreads)
+        F[d.ent_dst[e] - i0 * fs] = T(A[(long long)d.ent_src[e] * Bp + b]);
+    __syncthreads();
+    for (int q = d.child_ptr[p]; q < d.child_ptr[p + 1]; ++q) {
+        const int c = d.child_idx[q];
+        const int cfs = d.fsz[c], cnc = d.nc[c], cuc = cfs - cnc;
+        long long cs;
+        const T* C = front_ptr(d, ar, c, b, Bp, cs);
+        const int* loc = d.asm_loc + d.asm_ptr[c];
+        for (int a = 0; a < cuc; ++a) {
+            const int r = loc[a] - i0;
+            if (r < 0 || r >= nr) continue;      // uniform across the block
+            const T* crow = C + (long long)((cnc + a) * cfs + cnc) * cs;
+            T* frow = F + r * fs;
+            for (int bb = tid; bb < cuc; bb += kAsmThreads) frow[loc[bb]] += crow[bb * cs];
+        }
+        __syncthreads();
+    }
+    T* G = ar.a1 + (long long)b * d.arena1 + d.off[p] + (long long)i0 * fs;
+    for (int e = tid; e < nr * fs; e += kAsmThreads) G[e] = F[e];
+}
+
+// ===========================================================================
 // Factor, tier 2 on MFMA (FP32 fronts, fsz <= 512)
 //
 // 512 lanes.  Panel width 64: the rows of the column panel are held one per
@@ -326,38 +399,113 @@ constexpr int kPld = kDenseKb + 2;      // panel row stride: A reads conflict-fr
 constexpr int kSld = 96;                // stage row stride: B reads conflict-free
 constexpr size_t kDenseLds =
     size_t(kDenseMaxFsz) * kPld * 4 + size_t(kDenseKb) * kSld * 4 + 2 * kDenseKb * 4;
+static_assert(64 * 66 + 1024 <= kDenseKb * kSld, "Linv + temporary must fit the stage buffer");
+
+// Raw buffer view of one scenario's front (fs*fs floats): 32-bit offsets keep
+// address arithmetic in one VGPR plus a uniform SGPR offset, and accesses past
+// the front read 0 / are dropped by the hardware range check.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t front_rsrc(float* G, int fs) {
+    return __builtin_amdgcn_make_buffer_rsrc(G, (short)0, fs * fs * 4, 0x00020000);
+}
+__device__ __forceinline__ float bld(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+    return __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
+}
+__device__ __forceinline__ void bst(float x, __amdgpu_buffer_rsrc_t r, int voff, int soff) {
+    __builtin_amdgcn_raw_buffer_store_b32(x, r, voff, soff, 0);
+}
 
 __device__ inline void split_bf16(float x, __bf16& hi, __bf16& lo) {
     hi = (__bf16)x;
     lo = (__bf16)(x - (float)hi);
 }
 
-// Column j of U12 = L11^{-1} A12 (unit lower L11 from the LDS panel), held in
-// registers.  FULL: a 64-wide panel, all straight-line code.  Otherwise rows
-// past kbb are loaded from a clamped (valid) row, carried as don't-care values
-// and written back unchanged; stores run from the last row down so that a
-// clamped row aliasing a real one is overwritten by the real result.
-template <bool FULL>
-__device__ inline void dense_u12(float* G, const float* P, int fs, int k0, int kbb, int j) {
-    float u[kDenseKb];
-#pragma unroll
-    for (int c = 0; c < kDenseKb; ++c)
-        u[c] = G[(long long)(FULL ? k0 + c : min(k0 + c, fs - 1)) * fs + j];
-#pragma unroll
-    for (int c = 1; c < kDenseKb; ++c) {
-        float acc = u[c];
-#pragma unroll
-        for (int q = 0; q < c; ++q) acc -= P[c * kPld + q] * u[q];
-        u[c] = acc;
+constexpr int kLinvLd = 66;   // Linv row stride: MFMA A-operand reads conflict-free
+
+// Unit lower L11 of the current panel (rows/cols past kbb: identity).
+__device__ inline float panel_l11(const float* P, int i, int k, int kbb) {
+    return i == k ? 1.f : ((k < i && i < kbb) ? P[i * kPld + k] : 0.f);
+}
+
+// Li = L11^{-1} (64 x 64, row stride kLinvLd) by inverting the four 16x16
+// diagonal blocks (one column per lane, forward substitution) and two
+// doubling steps [A 0; C D]^{-1} = [A^-1 0; -D^-1 C A^-1  D^-1] at h = 16, 32.
+// Tmp holds 1024 floats.  Called by all threads; ends synchronised.
+__device__ __forceinline__ void dense_linv(const float* P, float* Li, float* Tmp, int kbb) {
+    const int tid = threadIdx.x;
+    for (int e = tid; e < 64 * 64; e += kDenseThreads) Li[(e >> 6) * kLinvLd + (e & 63)] = 0.f;
+    __syncthreads();
+    // diagonal 16x16 blocks: row i of block column j, rows in sequence (LDS)
+    for (int i = 0; i < 16; ++i) {
+        if (tid < 64) {
+            const int o = (tid >> 4) * 16, j = tid & 15;
+            float acc = i == j ? 1.f : 0.f;
+            for (int k = j; k < i; ++k)
+                acc -= panel_l11(P, o + i, o + k, kbb) * Li[(o + k) * kLinvLd + o + j];
+            Li[(o + i) * kLinvLd + o + j] = acc;
+        }
+        __syncthreads();
     }
-    if constexpr (FULL) {
+    for (int h = 16; h <= 32; h *= 2) {
+        const int total = 32 * h;              // (64 / 2h) pairs of h x h blocks
+        for (int e = tid; e < total; e += kDenseThreads) {
+            const int q = e / (h * h), i = (e / h) % h, k = e % h, o = 2 * h * q;
+            float acc = 0.f;
+            for (int m = k; m < h; ++m)            // T = C * A^-1 (A^-1 lower)
+                acc += panel_l11(P, o + h + i, o + m, kbb) * Li[(o + m) * kLinvLd + o + k];
+            Tmp[e] = acc;
+        }
+        __syncthreads();
+        for (int e = tid; e < total; e += kDenseThreads) {
+            const int q = e / (h * h), i = (e / h) % h, k = e % h, o = 2 * h * q;
+            float acc = 0.f;
+            for (int m = 0; m <= i; ++m)           // -D^-1 T (D^-1 lower)
+                acc -= Li[(o + h + i) * kLinvLd + o + h + m] * Tmp[q * h * h + m * h + k];
+            Li[(o + h + i) * kLinvLd + o + k] = acc;
+        }
+        __syncthreads();
+    }
+}
+
+// U12 = Li A12 on MFMA for the columns [t0, fs) of the panel rows k0..k0+kbb-1,
+// in place.  One wave per 32-column block; it reads its block whole before it
+// writes, and no other wave touches those columns.
+__device__ __forceinline__ void dense_u12(float* G, const float* Li, int fs, int k0, int kbb, int t0) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const __amdgpu_buffer_rsrc_t rG = front_rsrc(G, fs);
+    const int ncb = (fs - t0 + 31) / 32;
+    for (int cb = wave; cb < ncb; cb += kDenseThreads / 64) {
+        const int jj = t0 + cb * 32 + (lane & 31);
+        const bool jok = jj < fs;
+        const int vo = ((k0 + (lane >> 5)) * fs + jj) * 4;   // row k0 + lane/32
+        floatx16 top, bot;
 #pragma unroll
-        for (int c = 0; c < kDenseKb; ++c) G[(long long)(k0 + c) * fs + j] = u[c];
-    } else {
+        for (int v = 0; v < 16; ++v) { top[v] = 0.f; bot[v] = 0.f; }
+#pragma unroll 1
+        for (int s0 = 0; s0 < 32; s0 += 8) {
+            float bv[8];
 #pragma unroll
-        for (int c = kDenseKb - 1; c >= 0; --c) {
-            float* g = G + (long long)min(k0 + c, fs - 1) * fs + j;
-            *g = c < kbb ? u[c] : *g;
+            for (int t = 0; t < 8; ++t) {
+                const int kr = 2 * (s0 + t) + (lane >> 5);
+                bv[t] = (jok && kr < kbb) ? bld(rG, vo, 2 * (s0 + t) * fs * 4) : 0.f;
+            }
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                const int kk = 2 * (s0 + t) + (lane >> 5);
+                const float ab = Li[(32 + (lane & 31)) * kLinvLd + kk];
+                if (s0 < 16) {                     // Li[0:32][32:64] == 0
+                    const float at = Li[(lane & 31) * kLinvLd + kk];
+                    top = __builtin_amdgcn_mfma_f32_32x32x2f32(at, bv[t], top, 0, 0, 0);
+                }
+                bot = __builtin_amdgcn_mfma_f32_32x32x2f32(ab, bv[t], bot, 0, 0, 0);
+            }
+        }
+        const int vs = ((k0 + (lane >> 5) * 4) * fs + jj) * 4;
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+            const int i = (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
+            const int so = ((v >> 2) * 8 + (v & 3)) * fs * 4;
+            if (jok && i < kbb) bst(top[v], rG, vs, so);
+            if (jok && 32 + i < kbb) bst(bot[v], rG, vs, so + 32 * fs * 4);
         }
     }
 }
@@ -379,18 +527,23 @@ __global__ __launch_bounds__(kDenseThreads) void k_factor_t2_mfma(LuDevice d,
     const int fs = d.fsz[p], ncp = d.nc[p];
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
-    float* G = ar.a1 + (long long)b * d.arena1 + d.off[p];
-    assemble_front<float, kDenseThreads>(d, ar, A, Bp, p, b, fs, G);
+    float* G = ar.a1 + (long long)b * d.arena1 + d.off[p];   // assembled by k_assemble_rows
+    const __amdgpu_buffer_rsrc_t rG = front_rsrc(G, fs);
     int flag = 0;
     for (int k0 = 0; k0 < ncp; k0 += kDenseKb) {
         const int kbb = min(kDenseKb, ncp - k0);
         const int rows = fs - k0;
         // ---- panel LU, one row per lane in registers -----------------------
         const int r = tid;                   // panel row (fs <= 512 rows)
+        // coalesced panel load into P (a wave reads one row segment per step)
+        for (int e = tid; e < rows * kDenseKb; e += kDenseThreads) {
+            const int rr = e >> 6, c = e & (kDenseKb - 1);
+            P[rr * kPld + c] = c < kbb ? G[(long long)(k0 + rr) * fs + k0 + c] : 0.f;
+        }
+        __syncthreads();
         float row[kDenseKb];
 #pragma unroll
-        for (int c = 0; c < kDenseKb; ++c)
-            row[c] = (r < rows && c < kbb) ? G[(long long)(k0 + r) * fs + k0 + c] : 0.f;
+        for (int c = 0; c < kDenseKb; ++c) row[c] = r < rows ? P[r * kPld + c] : 0.f;
 #pragma unroll
         for (int c = 0; c < kDenseKb; ++c) {
             float* pr = prow + (c & 1) * kDenseKb;
@@ -401,35 +554,40 @@ __global__ __launch_bounds__(kDenseThreads) void k_factor_t2_mfma(LuDevice d,
                 for (int cc = c; cc < kDenseKb; ++cc) pr[cc] = row[cc];
             }
             __syncthreads();
-            if (c < kbb && r > c && r < rows) {
-                const float l = row[c] / pr[c];
-                row[c] = l;
+            if (c < kbb) {                              // uniform
+                // the pivot row, one entry per lane, broadcast by v_readlane
+                // (branch-free below, so every lane has loaded its entry)
+                const int pv = __float_as_int(pr[lane]);
+                const float piv = __int_as_float(__builtin_amdgcn_readlane(pv, c));
+                const bool act = r > c && r < rows;
+                const float l = row[c] / piv;
+                const float lm = act ? l : 0.f;
+                if (act) row[c] = l;
 #pragma unroll
-                for (int cc = c + 1; cc < kDenseKb; ++cc) row[cc] -= l * pr[cc];
+                for (int cc = c + 1; cc < kDenseKb; ++cc)
+                    row[cc] -= lm * __int_as_float(__builtin_amdgcn_readlane(pv, cc));
             }
         }
         if (r < rows) {
 #pragma unroll
-            for (int c = 0; c < kDenseKb; ++c) {
-                if (c < kbb) {
-                    P[r * kPld + c] = row[c];
-                    G[(long long)(k0 + r) * fs + k0 + c] = row[c];
-                } else {
-                    P[r * kPld + c] = 0.f;
-                }
-            }
+            for (int c = 0; c < kDenseKb; ++c) P[r * kPld + c] = row[c];   // zero past kbb
         }
         __syncthreads();
-        // ---- U12 = L11^{-1} A12, one column per lane -------------------------
-        // fs - t0 < kDenseThreads: at most one column per lane.
-        const int t0 = k0 + kbb;
-        if (const int j = t0 + tid; j < fs) {
-            if (kbb == kDenseKb) dense_u12<true>(G, P, fs, k0, kbb, j);
-            else dense_u12<false>(G, P, fs, k0, kbb, j);
+        for (int e = tid; e < rows * kDenseKb; e += kDenseThreads) {
+            const int rr = e >> 6, c = e & (kDenseKb - 1);
+            if (c < kbb) G[(long long)(k0 + rr) * fs + k0 + c] = P[rr * kPld + c];
         }
+        // ---- U12 = L11^{-1} A12: explicit inverse in the stage buffer, MFMA --
+        const int t0 = k0 + kbb;
+        dense_linv(P, stage, stage + 64 * kLinvLd, kbb);
+        dense_u12(G, stage, fs, k0, kbb, t0);
         __syncthreads();
         // ---- trailing update C[t0:, t0:] -= L21 U12 on MFMA ----------------
+        // Column blocks of 64 (U12 block staged in LDS), row strips of 32 per
+        // wave; the next strip's C tile is loaded before the current one's
+        // MFMA chain.
         const int tn = fs - t0;
+        const int nstrips = (tn + 31) / 32;
         for (int j0 = t0; j0 < fs; j0 += 64) {
             for (int e = tid; e < kDenseKb * 64; e += kDenseThreads) {
                 const int c = e >> 6, jj = e & 63;
@@ -437,18 +595,25 @@ __global__ __launch_bounds__(kDenseThreads) void k_factor_t2_mfma(LuDevice d,
                     (c < kbb && j0 + jj < fs) ? G[(long long)(k0 + c) * fs + j0 + jj] : 0.f;
             }
             __syncthreads();
-            const int nstrips = (tn + 31) / 32;
-            for (int st = wave; st < nstrips; st += kDenseThreads / 64) {
-                const int i0 = t0 + st * 32;
-                floatx16 acc0, acc1;
-                // load C (rows i0.., cols j0.. and j0+32..)
+            const int ja = j0 + (lane & 31);
+            const bool oka = ja < fs, okb = ja + 32 < fs;
+            auto load_c = [&](int st, floatx16& x0, floatx16& x1) {
+                const int ib = t0 + st * 32 + (lane >> 5) * 4;
+                const int vo = (ib * fs + ja) * 4;
 #pragma unroll
                 for (int v = 0; v < 16; ++v) {
-                    const int i = i0 + (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
-                    const int ja = j0 + (lane & 31), jb = ja + 32;
-                    acc0[v] = (i < fs && ja < fs) ? G[(long long)i * fs + ja] : 0.f;
-                    acc1[v] = (i < fs && jb < fs) ? G[(long long)i * fs + jb] : 0.f;
+                    const int ro = (v >> 2) * 8 + (v & 3);
+                    const bool rok = ib + ro < fs;
+                    x0[v] = (rok && oka) ? bld(rG, vo, ro * fs * 4) : 0.f;
+                    x1[v] = (rok && okb) ? bld(rG, vo + 128, ro * fs * 4) : 0.f;
                 }
+            };
+            floatx16 acc0, acc1;
+            if (wave < nstrips) load_c(wave, acc0, acc1);
+            for (int st = wave; st < nstrips; st += kDenseThreads / 64) {
+                floatx16 nx0, nx1;
+                if (st + kDenseThreads / 64 < nstrips) load_c(st + kDenseThreads / 64, nx0, nx1);
+                const int i0 = t0 + st * 32;
                 const int pr_row = i0 - k0 + (lane & 31);   // panel row of this lane's A rows
                 const bool arow_ok = (i0 + (lane & 31)) < fs;
                 if (MODE == kMfmaF32) {
@@ -483,13 +648,17 @@ __global__ __launch_bounds__(kDenseThreads) void k_factor_t2_mfma(LuDevice d,
                         acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, b1h, acc1, 0, 0, 0);
                     }
                 }
+                const int ib = i0 + (lane >> 5) * 4;
+                const int vo = (ib * fs + ja) * 4;
 #pragma unroll
                 for (int v = 0; v < 16; ++v) {
-                    const int i = i0 + (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
-                    const int ja = j0 + (lane & 31), jb = ja + 32;
-                    if (i < fs && ja < fs) G[(long long)i * fs + ja] = acc0[v];
-                    if (i < fs && jb < fs) G[(long long)i * fs + jb] = acc1[v];
+                    const int ro = (v >> 2) * 8 + (v & 3);
+                    const bool rok = ib + ro < fs;
+                    if (rok && oka) bst(acc0[v], rG, vo, ro * fs * 4);
+                    if (rok && okb) bst(acc1[v], rG, vo + 128, ro * fs * 4);
                 }
+                acc0 = nx0;
+                acc1 = nx1;
             }
             __syncthreads();
         }
@@ -751,10 +920,25 @@ __global__ __launch_bounds__(256) void k_scatter(const int* __restrict__ perm, i
 }  // namespace
 
 template <typename T>
+void assemble_rows(const LuDevice& d, const int* fronts, int count, int max_fsz, const double* A,
+                   int Bp, Arenas<T> ar, hipStream_t s) {
+    const int rb_rows = std::max(1, std::min(kAsmRows, (32 * 1024) / int(max_fsz * sizeof(T))));
+    const size_t shmem = size_t(rb_rows) * max_fsz * sizeof(T);
+    if (shmem > size_t(kLdsBytes)) throw std::runtime_error("assemble_rows: front row exceeds LDS");
+    const int nrb = (max_fsz + rb_rows - 1) / rb_rows;
+    const long long blocks = (long long)count * nrb * Bp;
+    if (blocks >= (1LL << 31)) throw std::runtime_error("assemble_rows: grid too large");
+    hipLaunchKernelGGL(k_assemble_rows<T>, dim3(unsigned(blocks)), dim3(kAsmThreads), shmem, s, d,
+                       fronts, A, Bp, ar, nrb, rb_rows);
+    CUPF_HIP(hipGetLastError());
+}
+
+template <typename T>
 void lu_factor_group(const LuDevice& d, int tier, const int* fronts, int count, int max_fsz,
                      const double* A, int Bp, Arenas<T> ar, double pivot_eps, int static_piv,
                      int* sing, int dense_mode, hipStream_t s) {
     if (count <= 0) return;
+    if (tier == 2) assemble_rows<T>(d, fronts, count, max_fsz, A, Bp, ar, s);
     if constexpr (sizeof(T) == 4) {
     if (tier == 2 && dense_mode != 0 && max_fsz <= kDenseMaxFsz) {
         static bool attr_set = false;

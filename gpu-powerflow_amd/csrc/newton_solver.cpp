@@ -6,6 +6,7 @@
 // counts residual evaluations and is reported batch-wide; a non-finite norm
 // raises std::runtime_error; converged[b] = final_mismatch[b] <= tolerance.
 #include "newton_solver.hpp"
+#include "warm_start.hpp"
 
 #include "analyze.hpp"
 #include "hip_common.hpp"
@@ -32,7 +33,7 @@ void validate_batch_args(int32_t batch_size, int64_t stride, int32_t n_bus, cons
 
 // Pairs of HIP events recorded around each stage on the stream it runs on.
 struct StageTimer {
-    static constexpr int kStages = 5;
+    static constexpr int kStages = 7;
     bool on = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pool;
     std::vector<int> stage_of;
@@ -44,7 +45,7 @@ struct StageTimer {
         }
     }
     int begin(int stage, hipStream_t s) {
-        if (!on) return -1;
+        if (!on || stage < 0) return -1;
         if (used == pool.size()) {
             std::pair<hipEvent_t, hipEvent_t> e;
             CUPF_HIP(hipEventCreate(&e.first));
@@ -81,7 +82,11 @@ struct StageScope {
     ~StageScope() { t.end(idx, s); }
 };
 
-enum { kStResidual = 0, kStJacobian = 1, kStFactor = 2, kStSolve = 3, kStUpdate = 4 };
+enum {
+    kStResidual = 0, kStJacobian = 1, kStFactor = 2, kStSolve = 3, kStUpdate = 4,
+    kStFactorDense = 5,   // tier-2 (dense front) launches, nested inside kStFactor
+    kStWarmStart = 6
+};
 
 }  // namespace
 
@@ -121,7 +126,13 @@ struct NewtonSolver::Impl {
     int capBp = 0, capB = 0;
     int B = 0, Bp = 0;
     DeviceBuffer V, Va, Vm, S, F, normF, bmax, J, arena0, arena1, ywork, upd, dx, ddx, res, sing,
-        lam, g, stage_a, stage_b;
+        lam, g, stage_a, stage_b, stage_c;
+
+    // learned warm start (not owned)
+    WarmStartMLP* warm = nullptr;
+    int warm_budget = 0;
+    DeviceBuffer warm_v0, warm_s, warm_r, warm_o;
+    int warm_cap = 0;
     hipStream_t own = nullptr;
     hipStream_t s = nullptr;   // stream of the current call
 
@@ -162,6 +173,7 @@ struct NewtonSolver::Impl {
             const size_t big = size_t(batch) * std::max(n, js.dim) * sizeof(double2);
             stage_a.allocate(big);
             stage_b.allocate(big);
+            stage_c.allocate(big);
             capB = batch;
         }
     }
@@ -180,6 +192,8 @@ struct NewtonSolver::Impl {
             for (int t = 0; t < 3; ++t) {
                 int cnt, mf;
                 const int* f = group(L, t, cnt, mf);
+                if (cnt == 0) continue;
+                StageScope sc(timer, t == 2 ? kStFactorDense : -1, s);
                 kern::lu_factor_group<T>(lu, t, f, cnt, mf, J.as<double>(), Bp, ar, piv_eps,
                                          static_piv ? 1 : 0, sing.as<int>(), dense_mode, s);
             }
@@ -278,20 +292,123 @@ struct NewtonSolver::Impl {
         return h;
     }
 
-    int run_nr(const NRConfig& cfg) {
+    // NR loop (reference newton_solver.cpp:497-558 order: residual and norm,
+    // then Jacobian, factor, solve, update).  tolerant: the warm-start pass --
+    // the test runs over the finite norms only, non-finite scenarios are left
+    // to the caller, and at most `budget` iterations run.
+    int run_nr(const NRConfig& cfg, bool tolerant = false, int budget = -1) {
         CUPF_HIP(hipMemsetAsync(sing.get(), 0, size_t(Bp) * sizeof(int), s));
+        const int limit = budget >= 0 ? std::min(budget, cfg.max_iter) : cfg.max_iter;
         int completed = 0;
-        for (int it = 0; it < cfg.max_iter; ++it) {
+        for (int it = 0; it < limit; ++it) {
             completed = it + 1;
             eval_residual();
-            const double m = read_batch_max();
-            if (!std::isfinite(m))
-                throw std::runtime_error("NewtonSolver: mismatch norm is not finite");
+            double m;
+            if (tolerant) {
+                kern::batch_max_finite(normF.as<double>(), B, bmax.as<double>(), s);
+                CUPF_HIP(hipMemcpyAsync(&m, bmax.get(), sizeof(double), hipMemcpyDeviceToHost, s));
+                CUPF_HIP(hipStreamSynchronize(s));
+                if (m < 0.0) break;                 // nothing finite left
+            } else {
+                m = read_batch_max();
+                if (!std::isfinite(m))
+                    throw std::runtime_error("NewtonSolver: mismatch norm is not finite");
+            }
             if (m <= cfg.tolerance) break;
             eval_jacobian();
             newton_step();
         }
         return completed;
+    }
+
+    void ensure_warm(int batch) {
+        if (batch <= warm_cap) return;
+        const size_t rows = size_t(batch) * n * sizeof(double2);
+        warm_v0.allocate(rows);
+        warm_s.allocate(rows);
+        warm_r.allocate(rows);
+        warm_o.allocate(rows);
+        warm_cap = batch;
+    }
+
+    // Core of solve_batch / solve_batch_device.  Sr, V0r: device rows
+    // ([B][stride] complex128); Vo: device rows [B][n].  Per-scenario
+    // iterations, final mismatch and singular flags on return.
+    //
+    // With a warm start: V0 <- MLP(Sbus) (slack and PV magnitudes from V0r),
+    // a tolerant NR pass of at most warm_budget iterations, then every
+    // scenario that is not converged (or not finite) is solved again from
+    // its own V0r row with the plain NR loop; its iteration count is the sum.
+    void solve_rows(const double2* Sr, int64_t ss, const double2* V0r, int64_t vs, int batch,
+                    const NRConfig& cfg, bool cache_req, double2* Vo, std::vector<int>& iters,
+                    std::vector<double>& nf, std::vector<int>& sg) {
+        ensure_batch(batch);
+        const int Ball = B;
+        kern::to_interleaved(Sr, ss, n, B, Bp, S.as<double2>(), s);
+        if (!warm) {
+            kern::to_interleaved(V0r, vs, n, B, Bp, V.as<double2>(), s);
+            kern::seed_polar(n, Bp, V.as<double2>(), Va.as<double>(), Vm.as<double>(), s);
+            const int it = run_nr(cfg);
+            if (cache_req) prepare_cache();
+            kern::from_interleaved(V.as<double2>(), n, B, Bp, Vo, n, s);
+            download_norms(nf, sg);
+            iters.assign(B, it);
+            st.last_warm_fallbacks = 0;
+            return;
+        }
+        ensure_warm(Ball);
+        {
+            StageScope st_(timer, kStWarmStart, s);
+            warm->predict_v0(Sr, ss, V0r, vs, B, warm_v0.as<double2>(), n, s);
+        }
+        kern::to_interleaved(warm_v0.as<double2>(), n, n, B, Bp, V.as<double2>(), s);
+        kern::seed_polar(n, Bp, V.as<double2>(), Va.as<double>(), Vm.as<double>(), s);
+        const int itw = run_nr(cfg, true, warm_budget);
+        kern::from_interleaved(V.as<double2>(), n, B, Bp, Vo, n, s);
+        download_norms(nf, sg);
+        iters.assign(B, itw);
+        std::vector<int> fails;
+        for (int b = 0; b < Ball; ++b)
+            if (!(std::isfinite(nf[b]) && nf[b] <= cfg.tolerance)) fails.push_back(b);
+        st.last_warm_fallbacks = int(fails.size());
+        st.last_warm_iterations = itw;
+        if (!fails.empty()) {
+            const int Bf = int(fails.size());
+            const size_t row = size_t(n) * sizeof(double2);
+            for (int i = 0; i < Bf; ++i) {
+                CUPF_HIP(hipMemcpyAsync(warm_s.as<double2>() + size_t(i) * n, Sr + fails[i] * ss, row,
+                                        hipMemcpyDeviceToDevice, s));
+                CUPF_HIP(hipMemcpyAsync(warm_r.as<double2>() + size_t(i) * n, V0r + fails[i] * vs, row,
+                                        hipMemcpyDeviceToDevice, s));
+            }
+            ensure_batch(Bf);
+            kern::to_interleaved(warm_s.as<double2>(), n, n, B, Bp, S.as<double2>(), s);
+            kern::to_interleaved(warm_r.as<double2>(), n, n, B, Bp, V.as<double2>(), s);
+            kern::seed_polar(n, Bp, V.as<double2>(), Va.as<double>(), Vm.as<double>(), s);
+            const int itf = run_nr(cfg);
+            kern::from_interleaved(V.as<double2>(), n, B, Bp, warm_o.as<double2>(), n, s);
+            std::vector<double> nff;
+            std::vector<int> sgf;
+            download_norms(nff, sgf);
+            for (int i = 0; i < Bf; ++i) {
+                CUPF_HIP(hipMemcpyAsync(Vo + size_t(fails[i]) * n, warm_o.as<double2>() + size_t(i) * n,
+                                        row, hipMemcpyDeviceToDevice, s));
+                nf[fails[i]] = nff[i];
+                sg[fails[i]] = sgf[i];
+                iters[fails[i]] = itw + itf;
+            }
+            ensure_batch(Ball);
+            if (!cache_req) {           // leave the full batch state behind
+                kern::to_interleaved(Sr, ss, n, B, Bp, S.as<double2>(), s);
+                kern::to_interleaved(Vo, n, n, B, Bp, V.as<double2>(), s);
+            }
+        }
+        if (cache_req) {
+            kern::to_interleaved(Sr, ss, n, B, Bp, S.as<double2>(), s);
+            kern::to_interleaved(Vo, n, n, B, Bp, V.as<double2>(), s);
+            prepare_cache();
+        }
+        CUPF_HIP(hipStreamSynchronize(s));
     }
 
     void prepare_cache() {
@@ -408,6 +525,22 @@ const NewtonSolverStats& NewtonSolver::stats() const { return impl_->st; }
 void NewtonSolver::set_stage_timing(bool on) {
     impl_->timer.on = on;
     impl_->timer.used = 0;
+}
+
+void NewtonSolver::set_warm_start(WarmStartMLP* mlp, int32_t max_warm_iter) {
+    Impl& m = *impl_;
+    if (mlp) {
+        if (!m.initialized)
+            throw std::runtime_error("NewtonSolver::set_warm_start(): initialize() must be called first");
+        if (mlp->n_bus() != m.n)
+            throw std::invalid_argument("NewtonSolver::set_warm_start(): the network was built for another bus count");
+        if (!mlp->normalized())
+            throw std::runtime_error("NewtonSolver::set_warm_start(): the network has no normalisation (train it first)");
+        if (max_warm_iter < 1)
+            throw std::invalid_argument("NewtonSolver::set_warm_start(): max_warm_iter must be >= 1");
+    }
+    m.warm = mlp;
+    m.warm_budget = max_warm_iter;
 }
 
 void NewtonSolver::collect_stage_times(double* total_ms, int64_t* launches) {
@@ -544,6 +677,20 @@ void NewtonSolver::initialize(const YbusView& ybus, const int32_t* pv, int32_t n
     m.st.max_front = P.max_fsz; m.st.max_pivots = P.max_nc;
     m.st.arena_elems = P.arena0 + P.arena1;
     m.st.lnz = P.lnz; m.st.factor_flops = P.flops;
+    for (int p = 0; p < P.num_fronts; ++p) {
+        if (P.tier[p] != 2) continue;
+        const double f = P.fsz[p];
+        double fl = 0.0;
+        for (int k = 0; k < P.nc[p]; ++k) {
+            const double r = f - k - 1;
+            fl += 2.0 * r * r + r;
+        }
+        m.st.dense_flops += fl;
+        m.st.dense_bytes += 2.0 * double(m.tsize()) * f * f +
+                            8.0 * double(P.ent_ptr[p + 1] - P.ent_ptr[p]);
+    }
+    for (int g = 0; g < P.num_levels; ++g)
+        if (P.group_ptr[g * 3 + 3] > P.group_ptr[g * 3 + 2]) ++m.st.dense_groups;
     m.initialized = true;
 }
 
@@ -587,26 +734,21 @@ void NewtonSolver::solve_batch(const YbusView& ybus, const std::complex<double>*
                               B, hipMemcpyHostToDevice, m.s));
     CUPF_HIP(hipMemcpy2DAsync(m.stage_b.get(), row, V0, size_t(V0_stride) * sizeof(double2), row, B,
                               hipMemcpyHostToDevice, m.s));
-    kern::to_interleaved(m.stage_a.as<double2>(), m.n, m.n, B, m.Bp, m.S.as<double2>(), m.s);
-    kern::to_interleaved(m.stage_b.as<double2>(), m.n, m.n, B, m.Bp, m.V.as<double2>(), m.s);
-    kern::seed_polar(m.n, m.Bp, m.V.as<double2>(), m.Va.as<double>(), m.Vm.as<double>(), m.s);
-
-    const int iters = m.run_nr(config);
-    m.last_B = B;
-    if (solve_options.prepare_adjoint_cache &&
-        solve_options.adjoint_cache_mode != AdjointCacheMode::None)
-        m.prepare_cache();
-
+    const bool cache_req = solve_options.prepare_adjoint_cache &&
+                           solve_options.adjoint_cache_mode != AdjointCacheMode::None;
+    std::vector<int> its, sing;
     result = {};
+    m.solve_rows(m.stage_a.as<double2>(), m.n, m.stage_b.as<double2>(), m.n, B, config, cache_req,
+                 m.stage_c.as<double2>(), its, result.final_mismatch, sing);
+    m.last_B = B;
     result.n_bus = m.n;
     result.batch_size = B;
     result.V.resize(size_t(B) * m.n);
-    kern::from_interleaved(m.V.as<double2>(), m.n, B, m.Bp, m.stage_a.as<double2>(), m.n, m.s);
-    CUPF_HIP(hipMemcpyAsync(result.V.data(), m.stage_a.get(), size_t(B) * row,
+    CUPF_HIP(hipMemcpyAsync(result.V.data(), m.stage_c.get(), size_t(B) * row,
                             hipMemcpyDeviceToHost, m.s));
-    std::vector<int> sing;
-    m.download_norms(result.final_mismatch, sing);
-    result.iterations.assign(B, iters);
+    CUPF_HIP(hipStreamSynchronize(m.s));
+    const int iters = *std::max_element(its.begin(), its.end());
+    result.iterations = its;
     result.converged.resize(B);
     result.singular.resize(B);
     for (int b = 0; b < B; ++b) {
@@ -637,23 +779,16 @@ void NewtonSolver::solve_batch_device(const void* sbus, int64_t sbus_stride, con
     m.cache = false;
     m.ensure_batch(batch_size);
     const int B = m.B;
-    kern::to_interleaved(static_cast<const double2*>(sbus), sbus_stride, m.n, B, m.Bp,
-                         m.S.as<double2>(), m.s);
-    kern::to_interleaved(static_cast<const double2*>(V0), V0_stride, m.n, B, m.Bp,
-                         m.V.as<double2>(), m.s);
-    kern::seed_polar(m.n, m.Bp, m.V.as<double2>(), m.Va.as<double>(), m.Vm.as<double>(), m.s);
-    const int iters = m.run_nr(config);
-    m.last_B = B;
-    if (solve_options.prepare_adjoint_cache &&
-        solve_options.adjoint_cache_mode != AdjointCacheMode::None)
-        m.prepare_cache();
-    kern::from_interleaved(m.V.as<double2>(), m.n, B, m.Bp, static_cast<double2*>(V_out), m.n,
-                           m.s);
+    const bool cache_req = solve_options.prepare_adjoint_cache &&
+                           solve_options.adjoint_cache_mode != AdjointCacheMode::None;
+    std::vector<int> its, sg;
     std::vector<double> nf;
-    std::vector<int> sg;
-    m.download_norms(nf, sg);
+    m.solve_rows(static_cast<const double2*>(sbus), sbus_stride, static_cast<const double2*>(V0),
+                 V0_stride, B, config, cache_req, static_cast<double2*>(V_out), its, nf, sg);
+    m.last_B = B;
+    const int iters = *std::max_element(its.begin(), its.end());
     for (int b = 0; b < B; ++b) {
-        if (iterations) iterations[b] = iters;
+        if (iterations) iterations[b] = its[b];
         if (final_mismatch) final_mismatch[b] = nf[b];
         if (converged) converged[b] = uint8_t(nf[b] <= config.tolerance ? 1 : 0);
     }

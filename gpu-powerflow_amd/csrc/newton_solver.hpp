@@ -18,7 +18,16 @@ struct NewtonSolverStats {
     int32_t dim = 0, nnz_j = 0, num_fronts = 0, max_front = 0, max_pivots = 0;
     int64_t arena_elems = 0, lnz = 0;
     double factor_flops = 0.0;
+    // tier-2 (dense) fronts: flops and minimum bytes per scenario, and the
+    // number of tier-2 launches per factorisation
+    double dense_flops = 0.0, dense_bytes = 0.0;
+    int32_t dense_groups = 0;
+    // learned warm start: iterations of the warm pass and scenarios that fell
+    // back to their own V0 in the last solve
+    int32_t last_warm_iterations = 0, last_warm_fallbacks = 0;
 };
+
+namespace cupf { class WarmStartMLP; }
 
 class NewtonSolver {
 public:
@@ -70,7 +79,14 @@ public:
 
     // HIP-event timing of the stages on the stream they run on.
     // Stage order: residual, jacobian, factor, solve (with refinement),
-    // voltage update.  collect_stage_times() synchronises and resets.
+    // voltage update, dense-front factor launches (nested in factor), warm
+    // start.  collect_stage_times() synchronises and resets.
+    // Learned warm start (no reference counterpart).  While set, solve_batch
+    // and solve_batch_device start NR from the network's prediction (at most
+    // max_warm_iter iterations) and re-solve every scenario that did not
+    // converge from its own V0.  nullptr switches it off.  Not owned.
+    void set_warm_start(cupf::WarmStartMLP* mlp, int32_t max_warm_iter);
+
     void set_stage_timing(bool on);
     void collect_stage_times(double* total_ms, int64_t* launches);
 

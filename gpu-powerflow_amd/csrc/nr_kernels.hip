@@ -269,6 +269,33 @@ __global__ __launch_bounds__(kBlock) void k_batch_max(const double* __restrict__
     if (threadIdx.x == 0) out[0] = __longlong_as_double((long long)red[0]);
 }
 
+// max over the finite entries of v[0..B); -1 when none is finite
+__global__ __launch_bounds__(kBlock) void k_batch_max_finite(const double* __restrict__ v, int B,
+                                                             double* __restrict__ out) {
+    unsigned long long m = 0;
+    int any = 0;
+    for (int i = threadIdx.x; i < B; i += kBlock) {
+        const unsigned long long bits = (unsigned long long)__double_as_longlong(v[i]);
+        if ((bits & 0x7ff0000000000000ull) != 0x7ff0000000000000ull) {
+            m = max(m, bits & 0x7fffffffffffffffull);
+            any = 1;
+        }
+    }
+    __shared__ unsigned long long red[kBlock];
+    __shared__ int anyr[kBlock];
+    red[threadIdx.x] = m;
+    anyr[threadIdx.x] = any;
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            red[threadIdx.x] = max(red[threadIdx.x], red[threadIdx.x + s]);
+            anyr[threadIdx.x] |= anyr[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = anyr[0] ? __longlong_as_double((long long)red[0]) : -1.0;
+}
+
 template <typename TI>
 __global__ __launch_bounds__(kBlock) void k_torch_inputs(int n, int B, int Bp,
                                                          const TI* __restrict__ bre,
@@ -411,6 +438,11 @@ void axpy_update(const TX* ddx, TX* dx, long long count, hipStream_t s) {
 
 void batch_max(const double* normF, int B, double* out, hipStream_t s) {
     hipLaunchKernelGGL(k_batch_max, dim3(1), dim3(kBlock), 0, s, normF, B, out);
+    CUPF_HIP(hipGetLastError());
+}
+
+void batch_max_finite(const double* normF, int B, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_batch_max_finite, dim3(1), dim3(kBlock), 0, s, normF, B, out);
     CUPF_HIP(hipGetLastError());
 }
 

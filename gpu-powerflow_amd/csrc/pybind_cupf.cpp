@@ -12,6 +12,7 @@
 #include "../../include/cupf_hip.h"
 #include "newton_solver_types.hpp"
 
+#include <algorithm>
 #include <complex>
 #include <cstring>
 #include <stdexcept>
@@ -132,9 +133,32 @@ public:
     int32_t n_bus = 0;
     int32_t dim = 0;
     int32_t last_batch = 0;
+    py::object warm;   // keeps an attached warm-start network alive
 
 private:
     cupf_solver* h_ = nullptr;
+};
+
+using F32 = py::array_t<float, py::array::c_style | py::array::forcecast>;
+
+// Learned warm start (no reference counterpart): thin owner of the C handle.
+class PyWarmStart {
+public:
+    PyWarmStart(int32_t n_bus, const py::array_t<int32_t, py::array::c_style | py::array::forcecast>& pv,
+                const py::array_t<int32_t, py::array::c_style | py::array::forcecast>& pq,
+                int32_t hidden, int32_t depth, uint64_t seed, int32_t max_batch) {
+        check(cupf_mlp_create(n_bus, pv.data(), int32_t(pv.size()), pq.data(), int32_t(pq.size()),
+                              hidden, depth, seed, max_batch, &h_));
+        check(cupf_mlp_sizes(h_, &n_params, &width_in, &width_out, nullptr));
+        this->n_bus = n_bus;
+    }
+    ~PyWarmStart() { if (h_) cupf_mlp_destroy(h_); }
+    cupf_warm_start* h() const { return h_; }
+    int64_t n_params = 0;
+    int32_t width_in = 0, width_out = 0, n_bus = 0;
+
+private:
+    cupf_warm_start* h_ = nullptr;
 };
 
 NRBatchResult run_batch(PySolver& self, const I32& indptr, const I32& indices, const C128& data,
@@ -360,6 +384,106 @@ PYBIND11_MODULE(_cupf, m) {
         .def_readonly("transpose_solve_backend", &AdjointResult::transpose_solve_backend)
         .def_readonly("sign_convention", &AdjointResult::sign_convention);
 
+    py::class_<PyWarmStart>(m, "WarmStartMLP")
+        .def(py::init<int32_t, const py::array_t<int32_t, py::array::c_style | py::array::forcecast>&,
+                      const py::array_t<int32_t, py::array::c_style | py::array::forcecast>&,
+                      int32_t, int32_t, uint64_t, int32_t>(),
+             py::arg("n_bus"), py::arg("pv"), py::arg("pq"), py::arg("hidden") = 512,
+             py::arg("depth") = 3, py::arg("seed") = 0, py::arg("max_batch") = 1024)
+        .def_readonly("n_params", &PyWarmStart::n_params)
+        .def_readonly("width_in", &PyWarmStart::width_in)
+        .def_readonly("width_out", &PyWarmStart::width_out)
+        .def_readonly("n_bus", &PyWarmStart::n_bus)
+        .def_property_readonly("n_samples",
+                               [](const PyWarmStart& w) {
+                                   int64_t ns = 0;
+                                   check(cupf_mlp_sizes(w.h(), nullptr, nullptr, nullptr, &ns));
+                                   return ns;
+                               })
+        .def("add_samples",
+             [](PyWarmStart& w, uintptr_t sbus, int64_t s_stride, uintptr_t V, int64_t v_stride,
+                int32_t count, uintptr_t stream) {
+                 check(cupf_mlp_add_samples(w.h(), reinterpret_cast<const void*>(sbus), s_stride,
+                                            reinterpret_cast<const void*>(V), v_stride, count,
+                                            reinterpret_cast<void*>(stream)));
+             },
+             py::arg("sbus_ptr"), py::arg("sbus_stride"), py::arg("v_ptr"), py::arg("v_stride"),
+             py::arg("count"), py::arg("stream") = 0)
+        .def("finalize",
+             [](PyWarmStart& w, uintptr_t stream) {
+                 check(cupf_mlp_finalize(w.h(), reinterpret_cast<void*>(stream)));
+             },
+             py::arg("stream") = 0)
+        .def("train",
+             [](PyWarmStart& w, int32_t epochs, int32_t batch, float lr, float wd, uint64_t seed,
+                uintptr_t stream) {
+                 py::array_t<float> loss(std::max(epochs, 0));
+                 float* lp = loss.mutable_data();
+                 {
+                     py::gil_scoped_release nogil;
+                     check(cupf_mlp_train(w.h(), epochs, batch, lr, wd, seed, lp,
+                                          reinterpret_cast<void*>(stream)));
+                 }
+                 return loss;
+             },
+             py::arg("epochs"), py::arg("batch") = 1024, py::arg("lr") = 1e-3f,
+             py::arg("weight_decay") = 1e-4f, py::arg("seed") = 0, py::arg("stream") = 0)
+        .def("predict_v0",
+             [](PyWarmStart& w, uintptr_t sbus, int64_t s_stride, uintptr_t vref, int64_t r_stride,
+                int32_t count, uintptr_t out, int64_t o_stride, uintptr_t stream) {
+                 check(cupf_mlp_predict_v0(w.h(), reinterpret_cast<const void*>(sbus), s_stride,
+                                           reinterpret_cast<const void*>(vref), r_stride, count,
+                                           reinterpret_cast<void*>(out), o_stride,
+                                           reinterpret_cast<void*>(stream)));
+             },
+             py::arg("sbus_ptr"), py::arg("sbus_stride"), py::arg("vref_ptr"), py::arg("vref_stride"),
+             py::arg("count"), py::arg("out_ptr"), py::arg("out_stride"), py::arg("stream") = 0)
+        .def("forward",
+             [](PyWarmStart& w, const F32& x) {
+                 if (x.ndim() != 2 || x.shape(1) != w.width_in)
+                     throw std::invalid_argument("forward: x must be [count, width_in]");
+                 const int32_t count = int32_t(x.shape(0));
+                 py::array_t<float> y({py::ssize_t(count), py::ssize_t(w.width_out)});
+                 check(cupf_mlp_forward_host(w.h(), x.data(), count, y.mutable_data()));
+                 return y;
+             })
+        .def("loss_and_grads",
+             [](PyWarmStart& w, const F32& x, const F32& t) {
+                 if (x.ndim() != 2 || x.shape(1) != w.width_in || t.ndim() != 2 ||
+                     t.shape(1) != w.width_out || t.shape(0) != x.shape(0))
+                     throw std::invalid_argument("loss_and_grads: shapes [count, width_in] / [count, width_out]");
+                 py::array_t<float> g(w.n_params);
+                 float loss = 0.f;
+                 check(cupf_mlp_loss_and_grads_host(w.h(), x.data(), t.data(), int32_t(x.shape(0)),
+                                                    &loss, g.mutable_data()));
+                 return py::make_tuple(loss, g);
+             })
+        .def("get_params",
+             [](PyWarmStart& w) {
+                 py::array_t<float> p(w.n_params);
+                 check(cupf_mlp_get_params(w.h(), p.mutable_data()));
+                 return p;
+             })
+        .def("set_params",
+             [](PyWarmStart& w, const F32& p) {
+                 if (p.size() != w.n_params) throw std::invalid_argument("set_params: wrong size");
+                 check(cupf_mlp_set_params(w.h(), p.data()));
+             })
+        .def("get_normalization",
+             [](PyWarmStart& w) {
+                 py::array_t<float> xm(w.width_in), xs(w.width_in), ym(w.width_out), ys(w.width_out);
+                 check(cupf_mlp_get_normalization(w.h(), xm.mutable_data(), xs.mutable_data(),
+                                                  ym.mutable_data(), ys.mutable_data()));
+                 return py::make_tuple(xm, xs, ym, ys);
+             })
+        .def("set_normalization",
+             [](PyWarmStart& w, const F32& xm, const F32& xs, const F32& ym, const F32& ys) {
+                 if (xm.size() != w.width_in || xs.size() != w.width_in || ym.size() != w.width_out ||
+                     ys.size() != w.width_out)
+                     throw std::invalid_argument("set_normalization: wrong sizes");
+                 check(cupf_mlp_set_normalization(w.h(), xm.data(), xs.data(), ym.data(), ys.data()));
+             });
+
     py::class_<PySolver>(m, "NewtonSolver")
         .def(py::init<const NewtonOptions&>(), py::arg("options") = NewtonOptions{})
         .def(
@@ -526,16 +650,29 @@ PYBIND11_MODULE(_cupf, m) {
             py::arg("batch"), py::arg("config") = NRConfig{},
             py::arg("solve_options") = SolveOptions{}, py::arg("v_out_ptr") = 0,
             py::arg("stream") = 0)
+        .def("set_warm_start",
+             [](PySolver& self, py::object mlp, int32_t max_warm_iter) {
+                 if (mlp.is_none()) {
+                     check(cupf_solver_set_warm_start(self.h(), nullptr, max_warm_iter));
+                     self.warm = py::none();
+                     return;
+                 }
+                 PyWarmStart& w = mlp.cast<PyWarmStart&>();
+                 check(cupf_solver_set_warm_start(self.h(), w.h(), max_warm_iter));
+                 self.warm = mlp;
+             },
+             py::arg("mlp"), py::arg("max_warm_iter") = 4)
         .def("set_stage_timing",
              [](PySolver& self, bool on) { check(cupf_solver_set_stage_timing(self.h(), on)); })
         .def("collect_stage_times",
              [](PySolver& self) {
-                 double ms[5];
-                 int64_t cnt[5];
+                 double ms[7];
+                 int64_t cnt[7];
                  check(cupf_solver_collect_stage_times(self.h(), ms, cnt));
                  py::dict d;
-                 const char* names[5] = {"residual", "jacobian", "factor", "solve", "voltage_update"};
-                 for (int k = 0; k < 5; ++k) d[names[k]] = py::make_tuple(ms[k], cnt[k]);
+                 const char* names[7] = {"residual", "jacobian", "factor", "solve",
+                                         "voltage_update", "factor_dense", "warm_start"};
+                 for (int k = 0; k < 7; ++k) d[names[k]] = py::make_tuple(ms[k], cnt[k]);
                  return d;
              })
         .def("stats",
@@ -555,6 +692,11 @@ PYBIND11_MODULE(_cupf, m) {
                  d["arena_elems"] = s.arena_elems;
                  d["lnz"] = s.lnz;
                  d["factor_flops"] = s.factor_flops;
+                 d["dense_flops"] = s.dense_flops;
+                 d["dense_bytes"] = s.dense_bytes;
+                 d["dense_groups"] = s.dense_groups;
+                 d["last_warm_iterations"] = s.last_warm_iterations;
+                 d["last_warm_fallbacks"] = s.last_warm_fallbacks;
                  return d;
              })
         .def_property_readonly("n_bus", [](const PySolver& s) { return s.n_bus; })

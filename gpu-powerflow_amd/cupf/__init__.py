@@ -51,6 +51,7 @@ NewtonOptions = _cupf.NewtonOptions
 NRResult = _cupf.NRResult
 NRBatchResult = _cupf.NRBatchResult
 NewtonSolver = _cupf.NewtonSolver
+WarmStartMLP = _cupf.WarmStartMLP
 
 LIBRARY_PATH = _PKG_DIR / "libcupf_hip.so"
 
@@ -59,7 +60,7 @@ __all__ = [
     "CudaLinearSolverKind", "CustomPrecision", "CustomMatchingMode", "CustomPivotStrategy",
     "CustomSolverConfig", "AdjointCacheMode", "CuDSSAlgorithm", "CuDSSOptions",
     "AdjointOptions", "AdjointResult", "NRConfig", "SolveOptions", "NewtonOptions",
-    "NRResult", "NRBatchResult", "NewtonSolver", "LIBRARY_PATH",
+    "NRResult", "NRBatchResult", "NewtonSolver", "WarmStartMLP", "LIBRARY_PATH",
 ]
 
 try:
@@ -71,4 +72,5 @@ else:
     _ta.install_torch_methods(NewtonSolver)
     CuPFFunction = _ta.CuPFFunction
     solve = _ta.solve
-    __all__ += ["CuPFFunction", "solve"]
+    from .warm_start import train_warm_start
+    __all__ += ["CuPFFunction", "solve", "train_warm_start"]

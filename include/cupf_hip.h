@@ -110,6 +110,10 @@ typedef struct {
     int32_t dim, nnz_j, num_fronts, max_front, max_pivots;
     int64_t arena_elems, lnz;
     double factor_flops;
+    double dense_flops, dense_bytes;   /* tier-2 fronts, per scenario */
+    int32_t dense_groups;              /* tier-2 launches per factorisation */
+    int32_t last_warm_iterations;      /* warm-start pass of the last solve */
+    int32_t last_warm_fallbacks;       /* scenarios re-solved from their V0 */
 } cupf_solver_stats;
 
 const char* cupf_last_error(void);
@@ -185,8 +189,9 @@ int cupf_solver_solve_batch_device(cupf_solver* solver, const void* sbus, int64_
 
 /* HIP-event stage timing (no reference counterpart; the reference's
  * ENABLE_TIMING ScopedTimer, utils/timer.hpp:52-283, syncs per stage).
- * Stages: 0 residual, 1 jacobian, 2 factor, 3 solve+refine, 4 voltage update.
- * collect synchronises, writes total_ms[5] and launches[5], and resets. */
+ * Stages: 0 residual, 1 jacobian, 2 factor, 3 solve+refine, 4 voltage update,
+ * 5 dense-front (tier-2) factor launches, nested inside 2, 6 warm start.
+ * collect synchronises, writes total_ms[7] and launches[7], and resets. */
 int cupf_solver_set_stage_timing(cupf_solver* solver, int32_t on);
 int cupf_solver_collect_stage_times(cupf_solver* solver, double* total_ms, int64_t* launches);
 
@@ -232,6 +237,43 @@ int cupf_symbolic(int32_t n, const int32_t* row_ptr, const int32_t* col_idx,
  * out = {flops per scenario, sum fsz^2, max fsz, number of fronts, nnz(L)}. */
 int cupf_plan_figures(int32_t n, const int32_t* row_ptr, const int32_t* col_idx,
                       const int32_t* perm, int32_t cap, double* out5);
+
+/* ---- Learned warm start (no reference counterpart) ---------------------
+ * A dense MLP (BF16 weights, FP32 accumulate, GELU) maps per-bus Sbus
+ * injections [P | Q] to [Va | Vm]; trained with AdamW on NR solutions the
+ * solver produced.  The closest reference analogue is the PINN
+ * StatePredictor (benchmark/pinn/model.py:13-37).  Device pointers are
+ * complex128 rows [count][stride]; host arrays are row-major float32. */
+typedef struct cupf_warm_start cupf_warm_start;
+
+int cupf_mlp_create(int32_t n_bus, const int32_t* pv, int32_t n_pv, const int32_t* pq,
+                    int32_t n_pq, int32_t hidden, int32_t depth, uint64_t seed, int32_t max_batch,
+                    cupf_warm_start** out);
+int cupf_mlp_destroy(cupf_warm_start* mlp);
+int cupf_mlp_sizes(cupf_warm_start* mlp, int64_t* n_params, int32_t* width_in, int32_t* width_out,
+                   int64_t* n_samples);
+int cupf_mlp_add_samples(cupf_warm_start* mlp, const void* sbus, int64_t sbus_stride,
+                         const void* V, int64_t V_stride, int32_t count, void* stream);
+int cupf_mlp_finalize(cupf_warm_start* mlp, void* stream);
+int cupf_mlp_train(cupf_warm_start* mlp, int32_t epochs, int32_t batch, float lr,
+                   float weight_decay, uint64_t seed, float* epoch_loss, void* stream);
+int cupf_mlp_predict_v0(cupf_warm_start* mlp, const void* sbus, int64_t sbus_stride,
+                        const void* V_ref, int64_t V_ref_stride, int32_t count, void* V0_out,
+                        int64_t V0_stride, void* stream);
+int cupf_mlp_forward_host(cupf_warm_start* mlp, const float* x, int32_t count, float* y);
+int cupf_mlp_loss_and_grads_host(cupf_warm_start* mlp, const float* x, const float* t,
+                                 int32_t count, float* loss, float* grads);
+int cupf_mlp_get_params(cupf_warm_start* mlp, float* out);
+int cupf_mlp_set_params(cupf_warm_start* mlp, const float* in);
+int cupf_mlp_get_normalization(cupf_warm_start* mlp, float* x_mean, float* x_std, float* y_mean,
+                               float* y_std);
+int cupf_mlp_set_normalization(cupf_warm_start* mlp, const float* x_mean, const float* x_std,
+                               const float* y_mean, const float* y_std);
+/* Attach (mlp != NULL) or detach the warm start of a solver: solve_batch and
+ * solve_batch_device then run at most max_warm_iter NR iterations from the
+ * prediction and re-solve the scenarios that did not converge from V0.  The
+ * network must outlive the attachment. */
+int cupf_solver_set_warm_start(cupf_solver* solver, cupf_warm_start* mlp, int32_t max_warm_iter);
 
 #ifdef __cplusplus
 }
