@@ -490,8 +490,12 @@ LuPlan lower_plan(const Symbolic& s, int n, const int* rp, const int* ci, int ti
     L.asm_loc.resize(s.asm_idx.size());
     for (int p = 0; p < P; ++p) {
         const int par = s.panel_parent[p];
-        for (int k = s.asm_ptr[p]; k < s.asm_ptr[p + 1]; ++k)
+        for (int k = s.asm_ptr[p]; k < s.asm_ptr[p + 1]; ++k) {
             L.asm_loc[k] = s.asm_idx[k] - s.front_ptr[par];
+            // the row-block assembly relies on increasing positions
+            if (k > s.asm_ptr[p] && L.asm_loc[k] <= L.asm_loc[k - 1])
+                throw std::runtime_error("lower_plan: contribution rows are not in front order");
+        }
     }
     // Original entries: A(r, c) lands in the front of the panel owning
     // min(r', c') at (local(r'), local(c')).

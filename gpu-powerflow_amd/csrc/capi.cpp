@@ -335,6 +335,22 @@ int cupf_solver_solve_batch_device(cupf_solver* solver, const void* sbus, int64_
     });
 }
 
+int cupf_solver_solve_batch_device_ybus(cupf_solver* solver, const void* sbus,
+                                        int64_t sbus_stride, const void* V0, int64_t V0_stride,
+                                        const void* ybus, int64_t ybus_stride, int32_t batch,
+                                        const cupf_nr_config* config,
+                                        const cupf_solve_options* options, void* V_out,
+                                        int32_t* iterations, double* final_mismatch,
+                                        uint8_t* converged, void* stream) {
+    return guarded([&] {
+        if (!ybus) throw std::invalid_argument("null ybus values");
+        solver_of(solver).solve_batch_device_ybus(sbus, sbus_stride, V0, V0_stride, ybus,
+                                                  ybus_stride, batch, to_cfg(config),
+                                                  to_sopt(options), V_out, iterations,
+                                                  final_mismatch, converged, stream);
+    });
+}
+
 int cupf_solver_set_stage_timing(cupf_solver* solver, int32_t on) {
     return guarded([&] { solver_of(solver).set_stage_timing(on != 0); });
 }

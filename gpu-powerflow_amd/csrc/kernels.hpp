@@ -24,6 +24,7 @@ struct NrDevice {
     const int* yp = nullptr;          // Ybus CSR row pointer [n+1]
     const int* yi = nullptr;          // Ybus CSR columns [nnzY]
     const double2* y = nullptr;       // Ybus values [nnzY]
+    const double2* yb = nullptr;      // per-scenario Ybus values [nnzY][Bp], or null
     const int* row_pvpq = nullptr;    // bus -> P/angle row (or -1) [n]
     const int* row_pq = nullptr;      // bus -> Q/magnitude row (or -1) [n]
     const int* map11 = nullptr;       // Ybus entry -> J slot, four quadrants [nnzY]

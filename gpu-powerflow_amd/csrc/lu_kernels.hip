@@ -359,12 +359,21 @@ __global__ __launch_bounds__(kAsmThreads) void k_assemble_rows(LuDevice d,
         long long cs;
         const T* C = front_ptr(d, ar, c, b, Bp, cs);
         const int* loc = d.asm_loc + d.asm_ptr[c];
-        for (int a = 0; a < cuc; ++a) {
-            const int r = loc[a] - i0;
-            if (r < 0 || r >= nr) continue;      // uniform across the block
-            const T* crow = C + (long long)((cnc + a) * cfs + cnc) * cs;
-            T* frow = F + r * fs;
-            for (int bb = tid; bb < cuc; bb += kAsmThreads) frow[loc[bb]] += crow[bb * cs];
+        // loc is increasing (contribution rows and front rows are both sorted),
+        // so the child rows landing in [i0, i1) are one range: binary search
+        int alo = 0, ahi = cuc;
+        {
+            int z = cuc;
+            while (alo < z) { const int m = (alo + z) >> 1; if (loc[m] < i0) alo = m + 1; else z = m; }
+            ahi = alo;
+            z = cuc;
+            while (ahi < z) { const int m = (ahi + z) >> 1; if (loc[m] < i1) ahi = m + 1; else z = m; }
+        }
+        // (row, column) pairs of the range spread over all threads
+        const int na = ahi - alo;
+        for (int e = tid; e < na * cuc; e += kAsmThreads) {
+            const int a = alo + e / cuc, bb = e - (e / cuc) * cuc;
+            F[(loc[a] - i0) * fs + loc[bb]] += C[(long long)((cnc + a) * cfs + cnc + bb) * cs];
         }
         __syncthreads();
     }
@@ -401,17 +410,15 @@ constexpr size_t kDenseLds =
     size_t(kDenseMaxFsz) * kPld * 4 + size_t(kDenseKb) * kSld * 4 + 2 * kDenseKb * 4;
 static_assert(64 * 66 + 1024 <= kDenseKb * kSld, "Linv + temporary must fit the stage buffer");
 
-// Raw buffer view of one scenario's front (fs*fs floats): 32-bit offsets keep
-// address arithmetic in one VGPR plus a uniform SGPR offset, and accesses past
-// the front read 0 / are dropped by the hardware range check.
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t front_rsrc(float* G, int fs) {
-    return __builtin_amdgcn_make_buffer_rsrc(G, (short)0, fs * fs * 4, 0x00020000);
+// Front element access by 32-bit byte offsets from the front's (wave-uniform)
+// base: a per-lane offset plus a uniform one, so the address arithmetic stays
+// in one VGPR (global_load with an SGPR base).  Callers keep every offset
+// inside the fs*fs front.
+__device__ __forceinline__ float bld(const float* G, unsigned voff, unsigned soff) {
+    return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(G) + (voff + soff));
 }
-__device__ __forceinline__ float bld(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
-    return __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
-}
-__device__ __forceinline__ void bst(float x, __amdgpu_buffer_rsrc_t r, int voff, int soff) {
-    __builtin_amdgcn_raw_buffer_store_b32(x, r, voff, soff, 0);
+__device__ __forceinline__ void bst(float x, float* G, unsigned voff, unsigned soff) {
+    *reinterpret_cast<float*>(reinterpret_cast<char*>(G) + (voff + soff)) = x;
 }
 
 __device__ inline void split_bf16(float x, __bf16& hi, __bf16& lo) {
@@ -471,7 +478,7 @@ __device__ __forceinline__ void dense_linv(const float* P, float* Li, float* Tmp
 // writes, and no other wave touches those columns.
 __device__ __forceinline__ void dense_u12(float* G, const float* Li, int fs, int k0, int kbb, int t0) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const __amdgpu_buffer_rsrc_t rG = front_rsrc(G, fs);
+    float* const rG = G;
     const int ncb = (fs - t0 + 31) / 32;
     for (int cb = wave; cb < ncb; cb += kDenseThreads / 64) {
         const int jj = t0 + cb * 32 + (lane & 31);
@@ -528,7 +535,7 @@ __global__ __launch_bounds__(kDenseThreads) void k_factor_t2_mfma(LuDevice d,
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     float* G = ar.a1 + (long long)b * d.arena1 + d.off[p];   // assembled by k_assemble_rows
-    const __amdgpu_buffer_rsrc_t rG = front_rsrc(G, fs);
+    float* const rG = G;
     int flag = 0;
     for (int k0 = 0; k0 < ncp; k0 += kDenseKb) {
         const int kbb = min(kDenseKb, ncp - k0);

@@ -131,8 +131,13 @@ struct NewtonSolver::Impl {
     // learned warm start (not owned)
     WarmStartMLP* warm = nullptr;
     int warm_budget = 0;
-    DeviceBuffer warm_v0, warm_s, warm_r, warm_o;
+    DeviceBuffer warm_v0, warm_s, warm_r, warm_o, warm_y;
     int warm_cap = 0;
+
+    // per-scenario Ybus values (interleaved [nnzY][Bp]); nr.yb points here
+    // while the last solve used them
+    DeviceBuffer Yb;
+    int yb_capBp = 0;
     hipStream_t own = nullptr;
     hipStream_t s = nullptr;   // stream of the current call
 
@@ -328,6 +333,7 @@ struct NewtonSolver::Impl {
         warm_s.allocate(rows);
         warm_r.allocate(rows);
         warm_o.allocate(rows);
+        warm_y.release();
         warm_cap = batch;
     }
 
@@ -339,12 +345,27 @@ struct NewtonSolver::Impl {
     // a tolerant NR pass of at most warm_budget iterations, then every
     // scenario that is not converged (or not finite) is solved again from
     // its own V0r row with the plain NR loop; its iteration count is the sum.
+    void load_ybus_rows(const double2* Yr, int64_t ys) {
+        if (!Yr) {
+            nr.yb = nullptr;
+            return;
+        }
+        if (Bp > yb_capBp) {
+            Yb.allocate(size_t(Bp) * nnzY * sizeof(double2));
+            yb_capBp = Bp;
+        }
+        kern::to_interleaved(Yr, ys, nnzY, B, Bp, Yb.as<double2>(), s);
+        nr.yb = Yb.as<double2>();
+    }
+
     void solve_rows(const double2* Sr, int64_t ss, const double2* V0r, int64_t vs, int batch,
                     const NRConfig& cfg, bool cache_req, double2* Vo, std::vector<int>& iters,
-                    std::vector<double>& nf, std::vector<int>& sg) {
+                    std::vector<double>& nf, std::vector<int>& sg, const double2* Yr = nullptr,
+                    int64_t ys = 0) {
         ensure_batch(batch);
         const int Ball = B;
         kern::to_interleaved(Sr, ss, n, B, Bp, S.as<double2>(), s);
+        load_ybus_rows(Yr, ys);
         if (!warm) {
             kern::to_interleaved(V0r, vs, n, B, Bp, V.as<double2>(), s);
             kern::seed_polar(n, Bp, V.as<double2>(), Va.as<double>(), Vm.as<double>(), s);
@@ -381,7 +402,15 @@ struct NewtonSolver::Impl {
                 CUPF_HIP(hipMemcpyAsync(warm_r.as<double2>() + size_t(i) * n, V0r + fails[i] * vs, row,
                                         hipMemcpyDeviceToDevice, s));
             }
+            if (Yr) {
+                const size_t yrow = size_t(nnzY) * sizeof(double2);
+                warm_y.reserve(size_t(Ball) * yrow);
+                for (int i = 0; i < Bf; ++i)
+                    CUPF_HIP(hipMemcpyAsync(warm_y.as<double2>() + size_t(i) * nnzY, Yr + fails[i] * ys,
+                                            yrow, hipMemcpyDeviceToDevice, s));
+            }
             ensure_batch(Bf);
+            if (Yr) load_ybus_rows(warm_y.as<double2>(), nnzY);
             kern::to_interleaved(warm_s.as<double2>(), n, n, B, Bp, S.as<double2>(), s);
             kern::to_interleaved(warm_r.as<double2>(), n, n, B, Bp, V.as<double2>(), s);
             kern::seed_polar(n, Bp, V.as<double2>(), Va.as<double>(), Vm.as<double>(), s);
@@ -398,6 +427,7 @@ struct NewtonSolver::Impl {
                 iters[fails[i]] = itw + itf;
             }
             ensure_batch(Ball);
+            load_ybus_rows(Yr, ys);
             if (!cache_req) {           // leave the full batch state behind
                 kern::to_interleaved(Sr, ss, n, B, Bp, S.as<double2>(), s);
                 kern::to_interleaved(Vo, n, n, B, Bp, V.as<double2>(), s);
@@ -766,6 +796,17 @@ void NewtonSolver::solve_batch_device(const void* sbus, int64_t sbus_stride, con
                                       const NRConfig& config, const SolveOptions& solve_options,
                                       void* V_out, int32_t* iterations, double* final_mismatch,
                                       uint8_t* converged, void* stream) {
+    solve_batch_device_ybus(sbus, sbus_stride, V0, V0_stride, nullptr, 0, batch_size, config,
+                            solve_options, V_out, iterations, final_mismatch, converged, stream);
+}
+
+void NewtonSolver::solve_batch_device_ybus(const void* sbus, int64_t sbus_stride, const void* V0,
+                                           int64_t V0_stride, const void* ybus_values,
+                                           int64_t ybus_stride, int32_t batch_size,
+                                           const NRConfig& config,
+                                           const SolveOptions& solve_options, void* V_out,
+                                           int32_t* iterations, double* final_mismatch,
+                                           uint8_t* converged, void* stream) {
     Impl& m = *impl_;
     if (!m.initialized)
         throw std::runtime_error("NewtonSolver::solve_batch_device(): initialize() must be called first");
@@ -773,6 +814,9 @@ void NewtonSolver::solve_batch_device(const void* sbus, int64_t sbus_stride, con
     validate_batch_args(batch_size, V0_stride, m.n, "V0");
     if (!sbus || !V0 || !V_out)
         throw std::invalid_argument("NewtonSolver::solve_batch_device(): null device pointer");
+    if (ybus_values && ybus_stride < m.nnzY)
+        throw std::invalid_argument(
+            "NewtonSolver::solve_batch_device(): ybus stride must be at least nnz(Ybus)");
     if (config.max_iter < 0) throw std::invalid_argument("NRConfig.max_iter must be >= 0");
     const auto t0 = std::chrono::steady_clock::now();
     m.s = stream ? static_cast<hipStream_t>(stream) : m.own;
@@ -784,7 +828,8 @@ void NewtonSolver::solve_batch_device(const void* sbus, int64_t sbus_stride, con
     std::vector<int> its, sg;
     std::vector<double> nf;
     m.solve_rows(static_cast<const double2*>(sbus), sbus_stride, static_cast<const double2*>(V0),
-                 V0_stride, B, config, cache_req, static_cast<double2*>(V_out), its, nf, sg);
+                 V0_stride, B, config, cache_req, static_cast<double2*>(V_out), its, nf, sg,
+                 static_cast<const double2*>(ybus_values), ybus_stride);
     m.last_B = B;
     const int iters = *std::max_element(its.begin(), its.end());
     for (int b = 0; b < B; ++b) {
@@ -938,6 +983,7 @@ void NewtonSolver::solve_torch_forward(const void* sbus_base_re, const void* sbu
     m.s = stream ? static_cast<hipStream_t>(stream) : m.own;
     m.cache = false;
     m.ensure_batch(batch_size);
+    m.nr.yb = nullptr;   // Ybus values shared by the batch
     const int B = m.B;
     if (want64)
         kern::torch_inputs<double>(m.n, B, m.Bp, (const double*)sbus_base_re,
@@ -1045,6 +1091,7 @@ void NewtonSolver::debug_residual(const std::complex<double>* sbus, const std::c
     if (!m.initialized) throw std::runtime_error("debug_residual: not initialized");
     m.s = m.own;
     m.ensure_batch(batch);
+    m.nr.yb = nullptr;   // Ybus values shared by the batch
     const size_t row = size_t(m.n) * sizeof(double2);
     CUPF_HIP(hipMemcpyAsync(m.stage_a.get(), sbus, batch * row, hipMemcpyHostToDevice, m.s));
     CUPF_HIP(hipMemcpyAsync(m.stage_b.get(), V, batch * row, hipMemcpyHostToDevice, m.s));
@@ -1065,6 +1112,7 @@ void NewtonSolver::debug_jacobian(const std::complex<double>* V, int32_t batch, 
     if (!m.initialized) throw std::runtime_error("debug_jacobian: not initialized");
     m.s = m.own;
     m.ensure_batch(batch);
+    m.nr.yb = nullptr;   // Ybus values shared by the batch
     const size_t row = size_t(m.n) * sizeof(double2);
     CUPF_HIP(hipMemcpyAsync(m.stage_b.get(), V, batch * row, hipMemcpyHostToDevice, m.s));
     kern::to_interleaved(m.stage_b.as<double2>(), m.n, m.n, batch, m.Bp, m.V.as<double2>(), m.s);

@@ -76,6 +76,16 @@ public:
                             int64_t V0_stride, int32_t batch_size, const NRConfig& config,
                             const SolveOptions& solve_options, void* V_out, int32_t* iterations,
                             double* final_mismatch, uint8_t* converged, void* stream);
+    // The same with per-scenario Ybus values (device complex128 rows
+    // [batch][ybus_stride], the CSR pattern given to initialize(); N-1 outage
+    // batches).  The reference keeps a ybus_values_batched storage hook
+    // (cuda_batched_storage.hpp:84-89) that no solve path reaches.
+    void solve_batch_device_ybus(const void* sbus, int64_t sbus_stride, const void* V0,
+                                 int64_t V0_stride, const void* ybus_values, int64_t ybus_stride,
+                                 int32_t batch_size, const NRConfig& config,
+                                 const SolveOptions& solve_options, void* V_out,
+                                 int32_t* iterations, double* final_mismatch, uint8_t* converged,
+                                 void* stream);
 
     // HIP-event timing of the stages on the stream they run on.
     // Stage order: residual, jacobian, factor, solve (with refinement),

@@ -87,7 +87,7 @@ __device__ inline double2 ibus_row(const NrDevice& d, int Bp, const double2* __r
                                    int i, int b) {
     double ir = 0.0, ii = 0.0;
     for (int k = d.yp[i]; k < d.yp[i + 1]; ++k) {
-        const double2 yk = d.y[k];
+        const double2 yk = d.yb ? d.yb[(long long)k * Bp + b] : d.y[k];
         const double2 vk = V[(long long)d.yi[k] * Bp + b];
         const double tr = yk.x * vk.x - yk.y * vk.y;
         const double ti = yk.x * vk.y + yk.y * vk.x;
@@ -174,7 +174,7 @@ __global__ __launch_bounds__(kBlock) void k_jacobian(NrDevice d, int Bp,
         const double ja_i = -0.0 * vi.y + (-1.0) * vi.x;
         for (int k = d.yp[i]; k < d.yp[i + 1]; ++k) {
             const int j = d.yi[k];
-            const double2 yk = d.y[k];
+            const double2 yk = d.yb ? d.yb[(long long)k * Bp + b] : d.y[k];
             const double2 vj = V[(long long)j * Bp + b];
             const double cr = yk.x * vj.x - yk.y * vj.y;
             const double ci = yk.x * vj.y + yk.y * vj.x;

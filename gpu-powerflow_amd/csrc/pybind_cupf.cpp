@@ -650,6 +650,34 @@ PYBIND11_MODULE(_cupf, m) {
             py::arg("batch"), py::arg("config") = NRConfig{},
             py::arg("solve_options") = SolveOptions{}, py::arg("v_out_ptr") = 0,
             py::arg("stream") = 0)
+        .def(
+            "solve_batch_device_ybus",
+            [](PySolver& self, uintptr_t sbus, int64_t sbus_stride, uintptr_t v0,
+               int64_t v0_stride, uintptr_t ybus, int64_t ybus_stride, int32_t batch,
+               const NRConfig& config, const SolveOptions& so, uintptr_t v_out, uintptr_t stream) {
+                NRBatchResult r;
+                r.n_bus = self.n_bus;
+                r.batch_size = batch;
+                r.iterations.resize(batch);
+                r.final_mismatch.resize(batch);
+                r.converged.resize(batch);
+                const cupf_nr_config c = c_cfg(config);
+                const cupf_solve_options s = c_sopt(so);
+                {
+                    py::gil_scoped_release nogil;
+                    check(cupf_solver_solve_batch_device_ybus(
+                        self.h(), (const void*)sbus, sbus_stride, (const void*)v0, v0_stride,
+                        (const void*)ybus, ybus_stride, batch, &c, &s, (void*)v_out,
+                        r.iterations.data(), r.final_mismatch.data(), r.converged.data(),
+                        (void*)stream));
+                }
+                self.last_batch = batch;
+                return r;
+            },
+            py::arg("sbus_ptr"), py::arg("sbus_stride"), py::arg("v0_ptr"), py::arg("v0_stride"),
+            py::arg("ybus_ptr"), py::arg("ybus_stride"), py::arg("batch"),
+            py::arg("config") = NRConfig{}, py::arg("solve_options") = SolveOptions{},
+            py::arg("v_out_ptr") = 0, py::arg("stream") = 0)
         .def("set_warm_start",
              [](PySolver& self, py::object mlp, int32_t max_warm_iter) {
                  if (mlp.is_none()) {

@@ -205,3 +205,79 @@ def two_bus_case() -> dict:
         "pv": np.array([], dtype=np.int32),
         "pq": np.array([1], dtype=np.int32),
     }
+
+
+def bridge_mask(grid: Grid) -> np.ndarray:
+    """True for branches whose removal disconnects the grid (iterative Tarjan lowlink)."""
+    n = grid.n_bus
+    frm, to = grid.branches[:, 0], grid.branches[:, 1]
+    m = len(frm)
+    adj_ptr = np.zeros(n + 1, dtype=np.int64)
+    np.add.at(adj_ptr, frm + 1, 1)
+    np.add.at(adj_ptr, to + 1, 1)
+    adj_ptr = np.cumsum(adj_ptr)
+    fill = adj_ptr[:-1].copy()
+    adj_v = np.empty(2 * m, dtype=np.int64)
+    adj_e = np.empty(2 * m, dtype=np.int64)
+    for e in range(m):
+        a, b = int(frm[e]), int(to[e])
+        adj_v[fill[a]] = b; adj_e[fill[a]] = e; fill[a] += 1
+        adj_v[fill[b]] = a; adj_e[fill[b]] = e; fill[b] += 1
+    disc = np.full(n, -1, dtype=np.int64)
+    low = np.zeros(n, dtype=np.int64)
+    bridge = np.zeros(m, dtype=bool)
+    t = 0
+    for root in range(n):
+        if disc[root] >= 0:
+            continue
+        disc[root] = low[root] = t; t += 1
+        stack = [(root, -1, adj_ptr[root])]
+        while stack:
+            v, pe, it = stack[-1]
+            if it < adj_ptr[v + 1]:
+                stack[-1] = (v, pe, it + 1)
+                w, e = int(adj_v[it]), int(adj_e[it])
+                if e == pe:
+                    continue
+                if disc[w] < 0:
+                    disc[w] = low[w] = t; t += 1
+                    stack.append((w, e, adj_ptr[w]))
+                else:
+                    low[v] = min(low[v], disc[w])
+            else:
+                stack.pop()
+                if stack:
+                    u = stack[-1][0]
+                    low[u] = min(low[u], low[v])
+                    if low[v] > disc[u]:
+                        bridge[pe] = True
+    return bridge
+
+
+def n1_ybus_values(grid: Grid, batch: int, seed: int):
+    """Per-scenario Ybus values for N-1 outages: scenario b removes one random
+    non-bridge branch's pi-model contributions (the CSR pattern is kept, the
+    entries become zero or lose the branch's share).  Returns
+    (values [batch, nnzY] complex128, outage branch index [batch])."""
+    rng = np.random.default_rng(seed)
+    cand = np.flatnonzero(~bridge_mask(grid))
+    if cand.size == 0:
+        raise ValueError("n1_ybus_values: every branch is a bridge")
+    out = rng.choice(cand, size=batch, replace=True)
+    indptr, indices = grid.indptr, grid.indices
+    vals = np.repeat(grid.data[None, :], batch, axis=0)
+
+    def slot(r, c):
+        lo, hi = indptr[r], indptr[r + 1]
+        k = lo + np.searchsorted(indices[lo:hi], c)
+        assert k < hi and indices[k] == c
+        return k
+
+    for b, e in enumerate(out):
+        f, t = int(grid.branches[e, 0]), int(grid.branches[e, 1])
+        y, sh = grid.branch_y[e], 0.5j * grid.branch_b[e]
+        vals[b, slot(f, f)] -= y + sh
+        vals[b, slot(t, t)] -= y + sh
+        vals[b, slot(f, t)] += y
+        vals[b, slot(t, f)] += y
+    return vals, out

@@ -186,6 +186,18 @@ int cupf_solver_solve_batch_device(cupf_solver* solver, const void* sbus, int64_
                                    const cupf_solve_options* options, void* V_out,
                                    int32_t* iterations, double* final_mismatch,
                                    uint8_t* converged, void* stream);
+/* The same with per-scenario Ybus values (N-1 outage batches): ybus is device
+ * complex128 [batch][ybus_stride] with the CSR pattern given to initialize()
+ * (ybus_stride >= nnz).  Replaces the reference's unreached
+ * ybus_values_batched storage hook (cuda_batched_storage.hpp:84-89; its
+ * solve_batch passes false, newton_solver.cpp:294). */
+int cupf_solver_solve_batch_device_ybus(cupf_solver* solver, const void* sbus,
+                                        int64_t sbus_stride, const void* V0, int64_t V0_stride,
+                                        const void* ybus, int64_t ybus_stride, int32_t batch,
+                                        const cupf_nr_config* config,
+                                        const cupf_solve_options* options, void* V_out,
+                                        int32_t* iterations, double* final_mismatch,
+                                        uint8_t* converged, void* stream);
 
 /* HIP-event stage timing (no reference counterpart; the reference's
  * ENABLE_TIMING ScopedTimer, utils/timer.hpp:52-283, syncs per stage).

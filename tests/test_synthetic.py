@@ -70,3 +70,37 @@ def test_workload_2000_converges_in_the_oracle(syn, oracle):
     V, it, nm, st = oracle.nr_solve_batch(g.n_bus, g.indptr, g.indices, g.data, g.sbus_scaled(f),
                                           g.flat_start(2), g.pv, g.pq)
     assert (st == 0).all() and (nm <= 1e-8).all() and (it <= 8).all()
+
+
+def test_bridge_mask_matches_connectivity(syn):
+    import scipy.sparse as sp
+    import scipy.sparse.csgraph as cg
+    g = syn.workload_grid("case118")
+    br = syn.bridge_mask(g)
+
+    def parts(keep):
+        f, t = g.branches[keep, 0], g.branches[keep, 1]
+        A = sp.coo_matrix((np.ones(len(f)), (f, t)), shape=(g.n_bus, g.n_bus))
+        return cg.connected_components(A, directed=False)[0]
+
+    m = len(br)
+    brute = np.array([parts(np.arange(m) != e) > 1 for e in range(m)])
+    np.testing.assert_array_equal(br, brute)
+    assert 0 < br.sum() < m
+
+
+def test_n1_values_equal_rebuilt_ybus(syn):
+    g = syn.workload_grid("case118")
+    vals, out = syn.n1_ybus_values(g, 6, seed=5)
+    assert not syn.bridge_mask(g)[out].any()
+    for b, e in enumerate(out):
+        keep = np.arange(len(g.branches)) != e
+        ip, ix, dv = syn._ybus_csr(g.n_bus, g.branches[keep, 0], g.branches[keep, 1],
+                                   g.branch_y[keep], g.branch_b[keep])
+        dense_ref = np.zeros((g.n_bus, g.n_bus), complex)
+        for r in range(g.n_bus):
+            dense_ref[r, ix[ip[r]:ip[r + 1]]] = dv[ip[r]:ip[r + 1]]
+        dense = np.zeros_like(dense_ref)
+        for r in range(g.n_bus):
+            dense[r, g.indices[g.indptr[r]:g.indptr[r + 1]]] = vals[b, g.indptr[r]:g.indptr[r + 1]]
+        np.testing.assert_allclose(dense, dense_ref, rtol=0, atol=1e-12)
