@@ -56,9 +56,16 @@ def parse():
                    help="warm-start training set: NR solutions at load factor U(0.6, 1.4), seed 2")
     p.add_argument("--epochs", type=int, default=60)
     p.add_argument("--warm-iters", type=int, default=4, help="warm-start NR budget before fallback")
+    p.add_argument("--no-skip", action="store_true",
+                   help="warm-start network without its linear input -> output path")
     p.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                    help="only run a plain device copy of MIB MiB three times (counter scale)")
     return p.parse_args()
+
+
+def log(msg):
+    """Progress line on stderr (the JSON result line stays alone on stdout)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def calibrate_copy(torch, mib):
@@ -207,8 +214,11 @@ def main():
     training = None
     for f in forms:
         comp, prec, ref = spec[f]
+        log(f"form {f}: initialize")
         sv = make_solver(cupf, grid, comp, prec, args.cap, ref)
         if f == "learned":
+            log(f"form {f}: generating {args.train_samples} training solutions, "
+                f"training {args.epochs} epochs")
             # training data: the solver's own NR solutions (flat start) of
             # load factor U(0.6, 1.4), seed 2 -- disjoint from the timed batch
             Nt = args.train_samples
@@ -216,17 +226,21 @@ def main():
             V0t = torch.from_numpy(grid.flat_start(Nt)).to(dev)
             mlp, info = cupf.train_warm_start(sv, St, V0t, grid.pv, grid.pq, hidden=512, depth=3,
                                               epochs=args.epochs, batch=1024, lr=1e-3,
-                                              weight_decay=1e-4, seed=0)
+                                              weight_decay=1e-4, seed=0, skip=not args.no_skip)
             del St, V0t
             sv.set_warm_start(mlp, args.warm_iters)
             training = {"samples": info["samples"], "load_factor": [0.6, 1.4], "seed": 2,
                         "epochs": args.epochs, "batch": 1024, "hidden": [512, 512, 512],
                         "activation": "gelu", "weights": "bf16", "accumulate": "fp32",
+                        "linear_skip": not args.no_skip,
                         "solve_s": round(info["solve_s"], 2), "train_s": round(info["train_s"], 2),
                         "loss_first": float(info["epoch_loss"][0]),
                         "loss_last": float(info["epoch_loss"][-1])}
+            log(f"form {f}: trained in {info['solve_s'] + info['train_s']:.1f}s "
+                f"(loss {info['epoch_loss'][0]:.3g} -> {info['epoch_loss'][-1]:.3g})")
         ms, it, st = timed_solves(torch, sv, S, V0, Vout, B, n, args.steps, args.warmup, cfg,
                                   stage_timing=True)
+        log(f"form {f}: {ms * 1e3:.2f} ms/step, {B / ms:.0f} solves/s, {it[-1]} iterations")
         res[f] = {"s_per_step": ms, "iterations": it[-1], "stages": st, "stats": sv.stats(),
                   "V": Vout.cpu().numpy()}
         if f == "learned":
@@ -296,6 +310,7 @@ def main():
         out["accuracy_vs_oracle"] = {"scenarios": k, "oracle_iterations": io.tolist(),
                                      **{f: err(r["V"]) for f, r in res.items()}}
     if not args.no_cpu:
+        log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(grid, S_host, V0_host, args.cpu_per_core)
     print(json.dumps(out), flush=True)
 

@@ -462,11 +462,12 @@ int cupf_plan_figures(int32_t n, const int32_t* row_ptr, const int32_t* col_idx,
 // ---- learned warm start ------------------------------------------------
 int cupf_mlp_create(int32_t n_bus, const int32_t* pv, int32_t n_pv, const int32_t* pq,
                     int32_t n_pq, int32_t hidden, int32_t depth, uint64_t seed, int32_t max_batch,
-                    cupf_warm_start** out) {
+                    int32_t skip, cupf_warm_start** out) {
     return guarded([&] {
         if (!out) throw std::invalid_argument("null output handle");
         *out = nullptr;
-        auto* m = new cupf::WarmStartMLP(n_bus, pv, n_pv, pq, n_pq, hidden, depth, seed, max_batch);
+        auto* m = new cupf::WarmStartMLP(n_bus, pv, n_pv, pq, n_pq, hidden, depth, seed, max_batch,
+                                         skip != 0);
         *out = new cupf_warm_start{m};
     });
 }

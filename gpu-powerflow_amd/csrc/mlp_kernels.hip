@@ -166,6 +166,9 @@ __global__ __launch_bounds__(kGemmThreads) void k_gemm(int M, int N, int K,
                         if (ep.out_f32) ep.out_f32[o] = x;
                         if (ep.out_bf16) ep.out_bf16[o] = f2bf(x);
                         break;
+                    case kEpAccum:
+                        ep.out_f32[o] += x;
+                        break;
                 }
             }
         }
@@ -365,6 +368,7 @@ void gemm(bool ta, bool tb, int M, int N, int K, const uint16_t* A, int lda, con
         throw std::invalid_argument("mlp::gemm: dimensions and strides must be multiples of 8");
     if (ep.mode == kEpGelu && !ep.out_bf16) throw std::invalid_argument("mlp::gemm: GELU needs a BF16 output");
     if (ep.mode == kEpGeluBwd && !ep.aux) throw std::invalid_argument("mlp::gemm: GELU backward needs aux");
+    if (ep.mode == kEpAccum && !ep.out_f32) throw std::invalid_argument("mlp::gemm: accumulation needs an FP32 output");
     const long long tiles = (long long)((M + kBM - 1) / kBM) * ((N + kBN - 1) / kBN);
     if (tiles >= (1LL << 30)) throw std::invalid_argument("mlp::gemm: problem too large");
     const int nt = int(tiles);

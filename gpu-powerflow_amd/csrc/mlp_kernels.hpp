@@ -8,12 +8,13 @@
 
 namespace cupf::mlp {
 
-enum EpilogueMode { kEpStore = 0, kEpGelu = 1, kEpGeluBwd = 2 };
+enum EpilogueMode { kEpStore = 0, kEpGelu = 1, kEpGeluBwd = 2, kEpAccum = 3 };
 
 // GEMM epilogue: x = alpha * acc + bias[j];
 //   kEpStore  : out_f32 / out_bf16 <- x (either may be null)
 //   kEpGelu   : out_f32 <- x (pre-activation, optional), out_bf16 <- gelu(x)
 //   kEpGeluBwd: x *= gelu'(aux[i][j]); out_f32 / out_bf16 <- x
+//   kEpAccum  : out_f32 += x
 // Outputs and aux are [M][ldc].
 struct Epilogue {
     int mode = kEpStore;

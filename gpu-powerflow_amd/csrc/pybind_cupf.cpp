@@ -146,9 +146,10 @@ class PyWarmStart {
 public:
     PyWarmStart(int32_t n_bus, const py::array_t<int32_t, py::array::c_style | py::array::forcecast>& pv,
                 const py::array_t<int32_t, py::array::c_style | py::array::forcecast>& pq,
-                int32_t hidden, int32_t depth, uint64_t seed, int32_t max_batch) {
+                int32_t hidden, int32_t depth, uint64_t seed, int32_t max_batch, bool skip) {
         check(cupf_mlp_create(n_bus, pv.data(), int32_t(pv.size()), pq.data(), int32_t(pq.size()),
-                              hidden, depth, seed, max_batch, &h_));
+                              hidden, depth, seed, max_batch, skip ? 1 : 0, &h_));
+        this->skip = skip;
         check(cupf_mlp_sizes(h_, &n_params, &width_in, &width_out, nullptr));
         this->n_bus = n_bus;
     }
@@ -156,6 +157,7 @@ public:
     cupf_warm_start* h() const { return h_; }
     int64_t n_params = 0;
     int32_t width_in = 0, width_out = 0, n_bus = 0;
+    bool skip = false;
 
 private:
     cupf_warm_start* h_ = nullptr;
@@ -387,9 +389,11 @@ PYBIND11_MODULE(_cupf, m) {
     py::class_<PyWarmStart>(m, "WarmStartMLP")
         .def(py::init<int32_t, const py::array_t<int32_t, py::array::c_style | py::array::forcecast>&,
                       const py::array_t<int32_t, py::array::c_style | py::array::forcecast>&,
-                      int32_t, int32_t, uint64_t, int32_t>(),
+                      int32_t, int32_t, uint64_t, int32_t, bool>(),
              py::arg("n_bus"), py::arg("pv"), py::arg("pq"), py::arg("hidden") = 512,
-             py::arg("depth") = 3, py::arg("seed") = 0, py::arg("max_batch") = 1024)
+             py::arg("depth") = 3, py::arg("seed") = 0, py::arg("max_batch") = 1024,
+             py::arg("skip") = false)
+        .def_readonly("skip", &PyWarmStart::skip)
         .def_readonly("n_params", &PyWarmStart::n_params)
         .def_readonly("width_in", &PyWarmStart::width_in)
         .def_readonly("width_out", &PyWarmStart::width_out)

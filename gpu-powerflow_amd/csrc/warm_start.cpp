@@ -19,8 +19,9 @@ constexpr double kPi = 3.14159265358979323846;
 }
 
 WarmStartMLP::WarmStartMLP(int n_bus, const int32_t* pv, int32_t n_pv, const int32_t* pq,
-                           int32_t n_pq, int hidden, int depth, uint64_t seed, int max_batch)
-    : n_(n_bus) {
+                           int32_t n_pq, int hidden, int depth, uint64_t seed, int max_batch,
+                           bool skip)
+    : n_(n_bus), skip_(skip) {
     if (n_bus <= 0 || hidden <= 0 || depth < 1 || max_batch <= 0)
         throw std::invalid_argument("WarmStartMLP: sizes must be positive (depth >= 1)");
     if (n_bus < 8 || hidden % 8 || max_batch % 8)
@@ -49,6 +50,12 @@ WarmStartMLP::WarmStartMLP(int n_bus, const int32_t* pv, int32_t n_pv, const int
         wbf_off_.push_back(n_wbf_);
         n_wbf_ += int64_t(dims_[l]) * dims_[l - 1];
     }
+    if (skip_) {
+        skip_off_ = n_params_;
+        skip_wbf_off_ = n_wbf_;
+        n_params_ += int64_t(dims_[L]) * dims_[0];
+        n_wbf_ += int64_t(dims_[L]) * dims_[0];
+    }
     // Glorot-uniform weights (output layer scaled by 0.1), zero biases
     std::vector<float> w(size_t(n_params_), 0.f);
     std::vector<long long> map(size_t(n_params_), -1);
@@ -61,6 +68,10 @@ WarmStartMLP::WarmStartMLP(int n_bus, const int32_t* pv, int32_t n_pv, const int
             w[size_t(w_off_[l - 1] + k)] = float(U(rng));
             map[size_t(w_off_[l - 1] + k)] = wbf_off_[l - 1] + k;
         }
+    }
+    if (skip_) {                             // Ws starts at zero
+        const int64_t nw = int64_t(dims_[L]) * dims_[0];
+        for (int64_t k = 0; k < nw; ++k) map[size_t(skip_off_ + k)] = skip_wbf_off_ + k;
     }
     hipStream_t s = nullptr;
     w_ = upload_vector(w, s);
@@ -159,6 +170,13 @@ void WarmStartMLP::forward(int count, hipStream_t s) {
         if (l < L) {
             ep.mode = mlp::kEpGelu;
             ep.out_bf16 = A_[l].as<uint16_t>();
+        } else if (skip_) {                  // H[L] = A[0] Ws^T, then += last layer
+            mlp::Epilogue es;
+            es.ldc = dims_[L];
+            es.out_f32 = H_[L].as<float>();
+            mlp::gemm(false, true, count, dims_[L], dims_[0], A_[0].as<uint16_t>(), dims_[0],
+                      wbf_.as<uint16_t>() + skip_wbf_off_, dims_[0], es, s);
+            ep.mode = mlp::kEpAccum;
         }
         mlp::gemm(false, true, count, dims_[l], dims_[l - 1], A_[l - 1].as<uint16_t>(), dims_[l - 1],
                   wbf_.as<uint16_t>() + wbf_off_[l - 1], dims_[l - 1], ep, s);
@@ -183,6 +201,13 @@ float WarmStartMLP::loss_grad(int count, const float* T, hipStream_t s, bool rea
 
 void WarmStartMLP::backward(int count, hipStream_t s) {
     const int L = int(dims_.size()) - 1;
+    if (skip_) {                             // dWs = dH[L]^T A[0]
+        mlp::Epilogue es;
+        es.out_f32 = g_.as<float>() + skip_off_;
+        es.ldc = dims_[0];
+        mlp::gemm(true, false, dims_[L], dims_[0], count, dH_[L].as<uint16_t>(), dims_[L],
+                  A_[0].as<uint16_t>(), dims_[0], es, s);
+    }
     for (int l = L; l >= 1; --l) {
         const int in = dims_[l - 1], out = dims_[l];
         mlp::Epilogue ew;                    // dW^T = dH^T A_{l-1}

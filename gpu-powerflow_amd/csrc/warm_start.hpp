@@ -5,7 +5,13 @@
 // y = [Va | Vm] (same width): depth
 // hidden layers of `hidden` units with GELU, BF16 weights and activations,
 // FP32 accumulation on MFMA.  Inputs and outputs are standardised per feature
-// with statistics of the training set.  Training is AdamW on the mean squared
+// with statistics of the training set.  With `skip`, a linear path from the
+// standardised input straight to the output layer (y = Ws x + MLP(x), Ws
+// [width_out][width_in], no bias of its own) runs beside the hidden layers:
+// the load -> voltage response of a grid is close to linear but of full rank
+// (about n), which a `hidden`-wide bottleneck cannot carry; the skip holds
+// that linear response and the hidden layers learn the nonlinear remainder.
+// Training is AdamW on the mean squared
 // error of the standardised outputs, over NR solutions that the solver itself
 // produced.  There is no counterpart in the reference (its closest analogue is
 // the PINN StatePredictor, benchmark/pinn/model.py:13-37, which differs in
@@ -25,7 +31,7 @@ class WarmStartMLP {
 public:
     // bus kinds from the PV / PQ lists (the rest: slack / fixed)
     WarmStartMLP(int n_bus, const int32_t* pv, int32_t n_pv, const int32_t* pq, int32_t n_pq,
-                 int hidden, int depth, uint64_t seed, int max_batch = 1024);
+                 int hidden, int depth, uint64_t seed, int max_batch = 1024, bool skip = false);
     ~WarmStartMLP();
     WarmStartMLP(const WarmStartMLP&) = delete;
     WarmStartMLP& operator=(const WarmStartMLP&) = delete;
@@ -36,6 +42,7 @@ public:
     int64_t num_params() const { return n_params_; }
     int64_t num_samples() const { return n_samples_; }
     bool normalized() const { return normalized_; }
+    bool has_skip() const { return skip_; }
 
     // ---- data set (device pointers; complex128 rows with the given strides)
     void reserve_samples(int64_t capacity);
@@ -58,7 +65,8 @@ public:
     // loss and flat gradient on host data (count multiple of 8)
     float loss_and_grads_host(const float* xn, const float* tn, int count, float* grads);
 
-    // ---- parameters: flat FP32, per layer W^T [out][in] then b [out]
+    // ---- parameters: flat FP32, per layer W^T [out][in] then b [out]; with
+    // the skip path, its Ws [width_out][width_in] follows the last layer
     void get_params(float* out) const;
     void set_params(const float* in);
     // statistics: x mean/std [width_in], y mean/std [width_out]
@@ -75,6 +83,8 @@ private:
     std::vector<int> dims_;                            // d[0] .. d[L]
     std::vector<int64_t> w_off_, b_off_, wbf_off_;
     int64_t n_params_ = 0, n_wbf_ = 0;
+    bool skip_ = false;
+    int64_t skip_off_ = 0, skip_wbf_off_ = 0;   // Ws in the FP32 / BF16 parameter vectors
     DeviceBuffer w_, g_, m_, v_, wbf_, bf_map_;
     DeviceBuffer kind_;
     // activations for cap_batch_ rows

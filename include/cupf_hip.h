@@ -258,9 +258,12 @@ int cupf_plan_figures(int32_t n, const int32_t* row_ptr, const int32_t* col_idx,
  * complex128 rows [count][stride]; host arrays are row-major float32. */
 typedef struct cupf_warm_start cupf_warm_start;
 
+/* skip != 0 adds a linear path from the standardised input to the output
+ * (y = Ws x + MLP(x), Ws [width_out][width_in], zero at creation); its
+ * weights follow the layers in the flat parameter vector. */
 int cupf_mlp_create(int32_t n_bus, const int32_t* pv, int32_t n_pv, const int32_t* pq,
                     int32_t n_pq, int32_t hidden, int32_t depth, uint64_t seed, int32_t max_batch,
-                    cupf_warm_start** out);
+                    int32_t skip, cupf_warm_start** out);
 int cupf_mlp_destroy(cupf_warm_start* mlp);
 int cupf_mlp_sizes(cupf_warm_start* mlp, int64_t* n_params, int32_t* width_in, int32_t* width_out,
                    int64_t* n_samples);

@@ -4,8 +4,10 @@ The warm-start MLP has no counterpart in the reference (its nearest analogue
 is the PINN StatePredictor, benchmark/pinn/model.py:13-37, a different model),
 so this restatement follows the device design itself: a dense network
 x -> [Linear, GELU] * depth -> Linear, parameters flat per layer as W^T
-[out][in] then b [out]; loss = mean squared error over all outputs; AdamW with
-bias correction.  Two modes:
+[out][in] then b [out]; with ``skip`` a linear input -> output path is added
+(y = x Ws^T + MLP(x), Ws [out][in] after the last layer, no bias of its own);
+loss = mean squared error over all outputs; AdamW with bias correction.  Two
+modes:
 
 * ``bf16=True`` rounds exactly where the device rounds (inputs, weights,
   hidden activations, dY and the backpropagated dH to bfloat16 with
@@ -64,27 +66,45 @@ def unpack(params, dims):
     return out
 
 
-def forward(params, dims, x, bf16=True):
+def skip_count(dims):
+    """number of skip-path weights (Ws is [width_out][width_in])"""
+    return dims[-1] * dims[0]
+
+
+def split_skip(params, dims, skip):
+    """flat params -> (layer params, Ws [out][in] or None)"""
+    params = np.asarray(params, dtype=np.float64)
+    if not skip:
+        return params, None
+    k = skip_count(dims)
+    return params[:-k], params[-k:].reshape(dims[-1], dims[0])
+
+
+def forward(params, dims, x, bf16=True, skip=False):
     """returns (Y, cache) for x [count][width_in] (standardised features)"""
     r = bf16_round if bf16 else (lambda v: np.asarray(v, dtype=np.float64))
-    layers = unpack(params, dims)
+    lp, Ws = split_skip(params, dims, skip)
+    layers = unpack(lp, dims)
     A = [r(x)]
     H = [None]
     for l, (Wt, b) in enumerate(layers, start=1):
         h = A[-1] @ r(Wt).T + b
+        if l == len(layers) and Ws is not None:
+            h = h + A[0] @ r(Ws).T
         H.append(h)
         if l < len(layers):
             A.append(r(gelu(h)))
     return H[-1], (layers, A, H)
 
 
-def loss_and_grads(params, dims, x, t, bf16=True):
+def loss_and_grads(params, dims, x, t, bf16=True, skip=False):
     r = bf16_round if bf16 else (lambda v: np.asarray(v, dtype=np.float64))
-    Y, (layers, A, H) = forward(params, dims, x, bf16)
+    Y, (layers, A, H) = forward(params, dims, x, bf16, skip)
     n = Y.size
     d = Y - t
     loss = float((d * d).sum() / n)
     dH = r(2.0 * d / n)
+    gWs = dH.T @ A[0] if skip else None
     grads = []
     for l in range(len(layers), 0, -1):
         Wt, _ = layers[l - 1]
@@ -95,6 +115,8 @@ def loss_and_grads(params, dims, x, t, bf16=True):
             dH = r((dH @ r(Wt)) * gelu_grad(H[l - 1]))
     grads.reverse()
     flat = np.concatenate([np.concatenate([gW.ravel(), gb]) for gW, gb in grads])
+    if skip:
+        flat = np.concatenate([flat, gWs.ravel()])
     return loss, flat
 
 

@@ -42,12 +42,15 @@ def g118(syn):
     return syn.workload_grid("case118")
 
 
-@pytest.mark.parametrize("n_bus,hidden,depth,count", [(12, 16, 2, 16), (2000, 512, 3, 64)])
-def test_forward_matches_oracle(gpu, cupf, n_bus, hidden, depth, count):
+@pytest.mark.parametrize("n_bus,hidden,depth,count,skip",
+                         [(12, 16, 2, 16, False), (2000, 512, 3, 64, False), (12, 16, 2, 16, True),
+                          (2000, 512, 3, 64, True)])
+def test_forward_matches_oracle(gpu, cupf, n_bus, hidden, depth, count, skip):
     rng = np.random.default_rng(n_bus)
     pv = np.arange(1, 1 + n_bus // 10, dtype=np.int32)
     pq = np.arange(1 + n_bus // 10, n_bus, dtype=np.int32)
-    m = cupf.WarmStartMLP(n_bus, pv, pq, hidden=hidden, depth=depth, seed=11)
+    m = cupf.WarmStartMLP(n_bus, pv, pq, hidden=hidden, depth=depth, seed=11, skip=skip)
+    assert m.skip == skip
     p = m.get_params().astype(np.float64)
     # perturb the output layer so the test does not ride on its 0.1 scale
     p = p + 0.05 * rng.standard_normal(p.size)
@@ -55,16 +58,18 @@ def test_forward_matches_oracle(gpu, cupf, n_bus, hidden, depth, count):
     p = m.get_params().astype(np.float64)
     x = rng.standard_normal((count, m.width_in)).astype(np.float32)
     y = m.forward(x)
-    yo, _ = mo.forward(p, dims_of(m, hidden, depth), x.astype(np.float64), bf16=True)
+    yo, _ = mo.forward(p, dims_of(m, hidden, depth), x.astype(np.float64), bf16=True, skip=skip)
     assert rel(y, yo) <= 2e-2, rel(y, yo)
 
 
-@pytest.mark.parametrize("n_bus,hidden,depth,count", [(12, 16, 2, 16), (50, 64, 3, 32)])
-def test_grads_match_oracle(gpu, cupf, n_bus, hidden, depth, count):
+@pytest.mark.parametrize("n_bus,hidden,depth,count,skip",
+                         [(12, 16, 2, 16, False), (50, 64, 3, 32, False), (12, 16, 2, 16, True),
+                          (50, 64, 3, 32, True)])
+def test_grads_match_oracle(gpu, cupf, n_bus, hidden, depth, count, skip):
     rng = np.random.default_rng(100 + n_bus)
     pv = np.arange(1, 3, dtype=np.int32)
     pq = np.arange(3, n_bus, dtype=np.int32)
-    m = cupf.WarmStartMLP(n_bus, pv, pq, hidden=hidden, depth=depth, seed=3)
+    m = cupf.WarmStartMLP(n_bus, pv, pq, hidden=hidden, depth=depth, seed=3, skip=skip)
     p = (m.get_params().astype(np.float64) + 0.1 * rng.standard_normal(m.n_params))
     m.set_params(p.astype(np.float32))
     p = m.get_params().astype(np.float64)
@@ -72,7 +77,9 @@ def test_grads_match_oracle(gpu, cupf, n_bus, hidden, depth, count):
     t = rng.standard_normal((count, m.width_out)).astype(np.float32)
     loss, g = m.loss_and_grads(x, t)
     dims = dims_of(m, hidden, depth)
-    lo, go = mo.loss_and_grads(p, dims, x.astype(np.float64), t.astype(np.float64), bf16=True)
+    lo, go = mo.loss_and_grads(p, dims, x.astype(np.float64), t.astype(np.float64), bf16=True,
+                               skip=skip)
+    assert g.size == go.size
     assert loss == pytest.approx(lo, rel=1e-2)
     k = 0
     for l in range(1, len(dims)):
@@ -81,6 +88,8 @@ def test_grads_match_oracle(gpu, cupf, n_bus, hidden, depth, count):
         k += nw
         assert rel(g[k:k + dims[l]], go[k:k + dims[l]]) <= 2e-2, (l, "b")
         k += dims[l]
+    if skip:
+        assert rel(g[k:], go[k:]) <= 2e-2, ("skip", rel(g[k:], go[k:]))
 
 
 def test_dataset_stats_and_adamw_step(gpu, cupf):

@@ -39,25 +39,37 @@ def test_gelu_closed_forms_and_derivative():
     np.testing.assert_allclose(mo.gelu_grad(x), fd, atol=1e-8)
 
 
-def test_fp64_backward_matches_central_differences():
+@pytest.mark.parametrize("skip", [False, True])
+def test_fp64_backward_matches_central_differences(skip):
     rng = np.random.default_rng(7)
     dims = mo.layer_dims(6, 5, 2, 4)
     n_params = sum(dims[l] * dims[l - 1] + dims[l] for l in range(1, len(dims)))
+    if skip:
+        n_params += mo.skip_count(dims)
     p = rng.standard_normal(n_params) * 0.5
     x = rng.standard_normal((3, 6))
     t = rng.standard_normal((3, 4))
-    loss, g = mo.loss_and_grads(p, dims, x, t, bf16=False)
+    loss, g = mo.loss_and_grads(p, dims, x, t, bf16=False, skip=skip)
+    assert g.size == n_params
     h = 1e-6
     fd = np.empty_like(p)
     for k in range(n_params):
         pp, pm = p.copy(), p.copy()
         pp[k] += h
         pm[k] -= h
-        fd[k] = (mo.loss_and_grads(pp, dims, x, t, bf16=False)[0] -
-                 mo.loss_and_grads(pm, dims, x, t, bf16=False)[0]) / (2 * h)
+        fd[k] = (mo.loss_and_grads(pp, dims, x, t, bf16=False, skip=skip)[0] -
+                 mo.loss_and_grads(pm, dims, x, t, bf16=False, skip=skip)[0]) / (2 * h)
     np.testing.assert_allclose(g, fd, rtol=1e-6, atol=1e-9)
+    if skip:
+        # the skip path is linear in x: with every layer weight and bias zero
+        # the network is exactly x Ws^T (a dropped or transposed Ws fails)
+        q = np.zeros_like(p)
+        Ws = rng.standard_normal((dims[-1], dims[0]))
+        q[-mo.skip_count(dims):] = Ws.ravel()
+        Yq, _ = mo.forward(q, dims, x, bf16=False, skip=True)
+        np.testing.assert_allclose(Yq, x @ Ws.T, rtol=1e-14, atol=1e-14)
     # the loss is the mean squared error of the forward pass
-    Y, _ = mo.forward(p, dims, x, bf16=False)
+    Y, _ = mo.forward(p, dims, x, bf16=False, skip=skip)
     assert loss == pytest.approx(((Y - t) ** 2).mean(), rel=1e-14)
 
 
