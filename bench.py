@@ -45,7 +45,7 @@ def parse():
     p.add_argument("--batch", type=int, default=1024)
     p.add_argument("--workload", default="case2000")
     p.add_argument("--cap", type=int, default=64, help="relaxed-panel cap")
-    p.add_argument("--refine", type=int, default=2)
+    p.add_argument("--refine", type=int, default=1)
     p.add_argument("--cpu-per-core", type=int, default=6,
                    help="oracle scenarios per host thread in the CPU baseline sample")
     p.add_argument("--no-cpu", action="store_true")

@@ -459,10 +459,12 @@ LuPlan lower_plan(const Symbolic& s, int n, const int* rp, const int* ci, int ti
     const int G = L.num_levels * 3;
     L.group_ptr.assign(G + 1, 0);
     L.group_max_fsz.assign(G, 0);
+    L.group_max_nc.assign(G, 0);
     for (int p = 0; p < P; ++p) {
         const int g = L.level[p] * 3 + L.tier[p];
         ++L.group_ptr[g + 1];
         L.group_max_fsz[g] = std::max(L.group_max_fsz[g], L.fsz[p]);
+        L.group_max_nc[g] = std::max(L.group_max_nc[g], L.nc[p]);
     }
     for (int g = 0; g < G; ++g) L.group_ptr[g + 1] += L.group_ptr[g];
     L.group_fronts.assign(P, -1);
@@ -484,6 +486,41 @@ LuPlan lower_plan(const Symbolic& s, int n, const int* rp, const int* ci, int ti
         std::vector<int> nx(L.child_ptr.begin(), L.child_ptr.end() - 1);
         for (int p = 0; p < P; ++p)
             if (s.panel_parent[p] >= 0) L.child_idx[nx[s.panel_parent[p]]++] = p;
+    }
+    // Assembly stages: greedy, children in their original order join the
+    // first stage whose rows they do not meet; the child list is reordered by
+    // stage (order inside a stage kept).
+    L.stage_ptr.assign(P + 1, 0);
+    L.stage.clear();
+    for (int p = 0; p < P; ++p) {
+        const int q0 = L.child_ptr[p], q1 = L.child_ptr[p + 1];
+        std::vector<std::vector<char>> used;   // per stage: front rows taken
+        std::vector<std::vector<int>> st;
+        for (int q = q0; q < q1; ++q) {
+            const int c = L.child_idx[q];
+            const int pb = s.front_ptr[p];
+            int chosen = -1;
+            for (int g = 0; g < int(st.size()) && chosen < 0; ++g) {
+                bool ok = true;
+                for (int k = s.asm_ptr[c]; k < s.asm_ptr[c + 1] && ok; ++k)
+                    ok = !used[size_t(g)][size_t(s.asm_idx[k] - pb)];
+                if (ok) chosen = g;
+            }
+            if (chosen < 0) {
+                chosen = int(st.size());
+                st.emplace_back();
+                used.emplace_back(size_t(L.fsz[p]), char(0));
+            }
+            st[size_t(chosen)].push_back(c);
+            for (int k = s.asm_ptr[c]; k < s.asm_ptr[c + 1]; ++k) used[size_t(chosen)][size_t(s.asm_idx[k] - pb)] = 1;
+        }
+        int q = q0;
+        L.stage.push_back(q0);
+        for (auto& g : st) {
+            for (int c : g) L.child_idx[q++] = c;
+            L.stage.push_back(q);
+        }
+        L.stage_ptr[p + 1] = int(L.stage.size());
     }
     // CB row -> parent-local index
     L.asm_ptr = s.asm_ptr;
@@ -526,7 +563,134 @@ LuPlan lower_plan(const Symbolic& s, int n, const int* rp, const int* ci, int ti
     L.ent_dst.reserve(L.nnz);
     for (auto& v : ent)
         for (auto& e : v) { L.ent_src.push_back(e.first); L.ent_dst.push_back(e.second); }
+    // row bounds of the row-block assembly
+    L.lb_ptr.assign(P + 1, 0);
+    for (int p = 0; p < P; ++p) {
+        const long long sz = (long long)(L.fsz[p] + 1) * (1 + L.child_ptr[p + 1] - L.child_ptr[p]);
+        if (L.lb_ptr[p] + sz > std::numeric_limits<int>::max())
+            throw std::runtime_error("lower_plan: row-bound table too large");
+        L.lb_ptr[p + 1] = L.lb_ptr[p] + int(sz);
+    }
+    L.row_lb.assign(size_t(L.lb_ptr[P]), 0);
+    for (int p = 0; p < P; ++p) {
+        const int f = L.fsz[p];
+        int* rl = L.row_lb.data() + L.lb_ptr[p];
+        int e = L.ent_ptr[p];
+        for (int r = 0; r <= f; ++r) {
+            while (e < L.ent_ptr[p + 1] && L.ent_dst[e] < r * f) ++e;
+            rl[r] = e;
+        }
+        for (int k = 0; k < L.child_ptr[p + 1] - L.child_ptr[p]; ++k) {
+            const int c = L.child_idx[L.child_ptr[p] + k];
+            const int cuc = L.fsz[c] - L.nc[c];
+            const int* loc = L.asm_loc.data() + L.asm_ptr[c];
+            int* cl = rl + (1 + k) * (f + 1);
+            int a = 0;
+            for (int r = 0; r <= f; ++r) {
+                while (a < cuc && loc[a] < r) ++a;
+                cl[r] = a;
+            }
+        }
+    }
     return L;
+}
+
+// ===========================================================================
+// Assembly task lists (see analyze.hpp).
+// ===========================================================================
+AsmLists build_asm_lists(const LuPlan& L, const std::vector<int>& kinds,
+                         const std::vector<int>& rows_per_block) {
+    const int P = L.num_fronts;
+    AsmLists A;
+    A.base.assign(size_t(P) * kAsmKinds, -1);
+    A.task_ptr.push_back(0);
+    A.stage_ptr.push_back(0);
+    struct Piece { int q, a0, a1, b0, b1; };
+    // landing-set intersection of two pieces of (possibly) different children
+    auto meets = [&](const Piece& x, const Piece& y) {
+        const int* lx = L.asm_loc.data() + L.asm_ptr[L.child_idx[x.q]];
+        const int* ly = L.asm_loc.data() + L.asm_ptr[L.child_idx[y.q]];
+        auto inter = [](const int* a, int an, const int* b, int bn) {
+            int i = 0, j = 0;
+            while (i < an && j < bn) {
+                if (a[i] == b[j]) return true;
+                if (a[i] < b[j]) ++i; else ++j;
+            }
+            return false;
+        };
+        return inter(lx + x.a0, x.a1 - x.a0, ly + y.a0, y.a1 - y.a0) &&
+               inter(lx + x.b0, x.b1 - x.b0, ly + y.b0, y.b1 - y.b0);
+    };
+    for (int p = 0; p < P; ++p) {
+        const int fs = L.fsz[p], K = L.nc[p], uc = fs - K;
+        for (int kind = 0; kind < kAsmKinds; ++kind) {
+            if (!(kinds[size_t(p)] >> kind & 1)) continue;
+            int nblk = 0;
+            switch (kind) {
+                case 0: nblk = (uc + kAsmSchurRows - 1) / kAsmSchurRows; break;
+                case 1: nblk = 1; break;
+                case 2: nblk = (uc + kAsmChunkCols - 1) / kAsmChunkCols; break;
+                case 3: nblk = (fs + rows_per_block[size_t(p)] - 1) / rows_per_block[size_t(p)]; break;
+                default: nblk = 1; break;
+            }
+            A.base[size_t(p) * kAsmKinds + kind] = int(A.task_ptr.size()) - 1;
+            for (int blk = 0; blk < nblk; ++blk) {
+                int r0 = 0, r1 = fs, c0 = 0, c1 = fs;
+                switch (kind) {
+                    case 0: r0 = K + blk * kAsmSchurRows; r1 = std::min(fs, r0 + kAsmSchurRows); break;
+                    case 1: r1 = K; c1 = K; break;
+                    case 2: r1 = K; c0 = K + blk * kAsmChunkCols; c1 = std::min(fs, c0 + kAsmChunkCols); break;
+                    case 3: r0 = blk * rows_per_block[size_t(p)];
+                            r1 = std::min(fs, r0 + rows_per_block[size_t(p)]); break;
+                    default: break;
+                }
+                std::vector<Piece> pcs;
+                for (int q = L.child_ptr[p]; q < L.child_ptr[p + 1]; ++q) {
+                    const int c = L.child_idx[q];
+                    const int cuc = L.fsz[c] - L.nc[c];
+                    const int* loc = L.asm_loc.data() + L.asm_ptr[c];
+                    auto lb = [&](int x) { return int(std::lower_bound(loc, loc + cuc, x) - loc); };
+                    const Piece pc{q, lb(r0), lb(r1), lb(c0), lb(c1)};
+                    if (pc.a1 > pc.a0 && pc.b1 > pc.b0) pcs.push_back(pc);
+                }
+                // greedy staging in child order, exact conflict test
+                std::vector<std::vector<int>> st;
+                for (int i = 0; i < int(pcs.size()); ++i) {
+                    int chosen = -1;
+                    for (int g = 0; g < int(st.size()) && chosen < 0; ++g) {
+                        bool ok = true;
+                        for (int j : st[size_t(g)])
+                            if (meets(pcs[size_t(i)], pcs[size_t(j)])) { ok = false; break; }
+                        if (ok) chosen = g;
+                    }
+                    if (chosen < 0) { chosen = int(st.size()); st.emplace_back(); }
+                    st[size_t(chosen)].push_back(i);
+                }
+                A.stage.push_back(int(A.tasks.size()));
+                for (auto& g : st) {
+                    for (int i : g) {
+                        const Piece& pc = pcs[size_t(i)];
+                        const int c = L.child_idx[pc.q];
+                        AsmTask t{};
+                        t.off = L.off[c];
+                        t.tier = L.tier[c];
+                        t.cfs = L.fsz[c];
+                        t.cnc = L.nc[c];
+                        t.loc_ptr = L.asm_ptr[c];
+                        t.a0 = pc.a0; t.na = pc.a1 - pc.a0;
+                        t.b0 = pc.b0; t.nb = pc.b1 - pc.b0;
+                        A.tasks.push_back(t);
+                    }
+                    A.stage.push_back(int(A.tasks.size()));
+                }
+                if (st.size() > 128)
+                    throw std::runtime_error("build_asm_lists: more than 128 assembly stages in a block");
+                A.task_ptr.push_back(int(A.tasks.size()));
+                A.stage_ptr.push_back(int(A.stage.size()));
+            }
+        }
+    }
+    return A;
 }
 
 }  // namespace cupf

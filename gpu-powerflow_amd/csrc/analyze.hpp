@@ -97,11 +97,26 @@ struct LuPlan {
     std::vector<int> child_ptr, child_idx;    // children per front
     std::vector<int> asm_ptr, asm_loc;    // per front: CB row -> parent-local index
     std::vector<int> ent_ptr, ent_src, ent_dst;  // A entries: CSR slot -> local pos
+    // Row bounds for row-block assembly.  For front p, row_lb[lb_ptr[p] + r]
+    // (r = 0..fsz) is the first entry of p (index into ent_*) whose position
+    // is at or after row r; for the k-th child c of p,
+    // row_lb[lb_ptr[p] + (1 + k) * (fsz + 1) + r] is the number of c's
+    // contribution rows landing above row r (asm_loc of c is increasing).
+    // The rows [i0, i1) of a block take the entries [lb(i0), lb(i1)) and the
+    // contribution rows [lb_c(i0), lb_c(i1)) of each child.
+    std::vector<int> lb_ptr, row_lb;
+    // Assembly stages.  The children of each front are ordered so that the
+    // children of one stage have pairwise disjoint contribution rows (so their
+    // extend-adds never touch the same front element and can run together);
+    // stage s of front p holds the children at positions
+    // [stage[stage_ptr[p] + s], stage[stage_ptr[p] + s + 1]) of child_idx.
+    std::vector<int> stage_ptr, stage;
     std::vector<long long> uoff;          // per front offset of its update vector
     long long upd_total = 0;
     std::vector<int> group_ptr;           // [num_levels * 3 + 1] into group_fronts
     std::vector<int> group_fronts;        // fronts of (level, tier) groups
     std::vector<int> group_max_fsz;       // [num_levels * 3]
+    std::vector<int> group_max_nc;        // [num_levels * 3]
     // work figures
     double flops = 0.0;                   // LU flops per scenario
     double factor_bytes_min = 0.0;        // sum fsz^2
@@ -111,5 +126,47 @@ struct LuPlan {
 
 LuPlan lower_plan(const Symbolic& s, int n, const int* rp, const int* ci, int tier0_max = 16,
                   int tier1_max = 128);
+
+// ---------------------------------------------------------------------------
+// Assembly task lists.  A factor kernel assembles a block of a front (a
+// rectangle of front rows x columns) from the children's contribution blocks.
+// For every (front, kind, block) the list holds one task per child whose
+// contribution lands in the block: the child's contribution rows [a0, a0+na)
+// and columns [b0, b0+nb) that land there, with what the kernel needs to
+// address them (arena offset and tier of the child, its size and pivot
+// count, and where its row map starts).  Tasks are ordered in stages: tasks
+// of one stage never add to the same front element (checked exactly, row and
+// column sets), so a stage is walked as one flat sequence; stages follow the
+// child order.
+//   kinds: 0 Schur row blocks (32 rows from the pivot count on, all columns),
+//          1 pivot block (rows and columns below the pivot count),
+//          2 pivot-row chunks (rows below the pivot count, 64 columns each
+//            from the pivot count on),
+//          3 row blocks of rows_per_block[p] rows (all columns),
+//          4 whole front.
+// ---------------------------------------------------------------------------
+constexpr int kAsmKinds = 5;
+constexpr int kAsmSchurRows = 32;
+constexpr int kAsmChunkCols = 64;
+
+struct AsmTask {                     // 48 bytes, device-visible layout
+    long long off;                   // child's offset in its tier's arena
+    int tier, cfs, cnc, loc_ptr;     // child's tier, front size, pivots, asm_ptr
+    int a0, na, b0, nb;              // contribution rows / columns of the task
+    int pad0, pad1;
+};
+
+struct AsmLists {
+    std::vector<int> base;           // [P * kAsmKinds]: first list id of (front, kind), -1 if none
+    std::vector<int> task_ptr;       // [lists + 1] into tasks
+    std::vector<int> stage_ptr;      // [lists + 1] into stage
+    std::vector<int> stage;          // stage boundaries: absolute task indices
+    std::vector<AsmTask> tasks;
+};
+
+// kinds[p] is a bit mask of the kinds built for front p; rows_per_block[p]
+// is the block height of kind 3.
+AsmLists build_asm_lists(const LuPlan& L, const std::vector<int>& kinds,
+                         const std::vector<int>& rows_per_block);
 
 }  // namespace cupf

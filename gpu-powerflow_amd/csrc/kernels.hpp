@@ -86,6 +86,18 @@ void adjoint_project(const NrDevice& d, int B, int Bp, const double* lambda, TG*
                      TG* grad_q, hipStream_t s);
 
 // ---- batched multifrontal LU (see analyze.hpp LuPlan) -----------------------
+// One assembly task (layout of analyze.hpp AsmTask).
+struct AsmTaskDev {
+    long long off;
+    int tier, cfs, cnc, loc_ptr;
+    int a0, na, b0, nb;
+    int pad0, pad1;
+};
+
+// Assembly kinds of the task lists (analyze.hpp build_asm_lists).
+enum { kAsmKSchur = 0, kAsmKPivot = 1, kAsmKChunk = 2, kAsmKRows = 3, kAsmKFront = 4,
+       kAsmKindCount = 5 };
+
 struct LuDevice {
     int n = 0, nnz = 0, P = 0;
     long long arena0 = 0, arena1 = 0, upd_total = 0;
@@ -104,7 +116,17 @@ struct LuDevice {
     const int* ent_src = nullptr;
     const int* ent_dst = nullptr;
     const int* perm = nullptr;
+    const int* lb_ptr = nullptr;      // row bounds of the row-block assembly (LuPlan)
+    const int* row_lb = nullptr;
+    const int* asm_base = nullptr;    // assembly task lists (analyze.hpp AsmLists)
+    const int* asm_tptr = nullptr;
+    const int* asm_sptr = nullptr;
+    const int* asm_stage = nullptr;
+    const AsmTaskDev* asm_tasks = nullptr;
 };
+
+// Block height of the row-block assembly of a tier-2 group (elem = sizeof T).
+int assemble_rows_block(int max_fsz, int elem);
 
 // Arenas: a0 interleaved [arena0][Bp]; a1 scenario-major [Bp][arena1].
 template <typename T>
@@ -120,6 +142,20 @@ template <typename T>
 void lu_factor_group(const LuDevice& d, int tier, const int* fronts, int count, int max_fsz,
                      const double* A, int Bp, Arenas<T> ar, double pivot_eps, int static_piv,
                      int* sing, int dense_mode, hipStream_t s);
+
+// FP32 tier-2 groups whose fronts have at most 64 pivots and 512 rows
+// (cross_schur_path) factor in two launches instead: cross_group (pivot
+// block L11\U11, U12, and U11^-1 into the side buffer uinv of
+// pivot_side_elems(count, Bp) floats) then schur_group (L21 and the
+// contribution block C = A22 - L21 U12, assembled and updated in one pass).
+// dense_mode 1: FP32 MFMA, 2: 3xBF16 MFMA.
+bool cross_schur_path(int max_fsz, int max_nc);
+size_t pivot_side_elems(int count, int Bp);
+void cross_group(const LuDevice& d, const int* fronts, int count, const double* A, int Bp,
+                 Arenas<float> ar, float* uinv, double pivot_eps, int static_piv, int* sing,
+                 hipStream_t s);
+void schur_group(const LuDevice& d, const int* fronts, int count, int max_fsz, const double* A,
+                 int Bp, Arenas<float> ar, const float* uinv, int dense_mode, hipStream_t s);
 
 // Solve sweeps.  rhs (FP64) and x are [n][Bp]; y [n][Bp] and upd
 // [upd_total][Bp] are work vectors.  gather: y <- P rhs is folded into the

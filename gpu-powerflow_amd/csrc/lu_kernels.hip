@@ -76,6 +76,164 @@ __device__ inline const T* front_ptr(const LuDevice& d, const Arenas<T>& ar, int
     return ar.a1 + (long long)b * d.arena1 + d.off[c];
 }
 
+// Original entries: F[map(dst)] = A(src) for the entries e in [e0, e1) whose
+// map(dst) >= 0 (dst: position in the fsz x fsz front, A interleaved
+// [nnz][Bp]); 4 entries in flight per thread.
+template <int NT, typename T, typename Map>
+__device__ inline void put_entries(T* __restrict__ F, const int* __restrict__ dst,
+                                   const int* __restrict__ src, const double* __restrict__ A,
+                                   int Bp, int b, int e0, int e1, Map map) {
+    for (int e = e0 + int(threadIdx.x); e < e1; e += 4 * NT) {
+        int di[4];
+        double v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int ee = e + u * NT;
+            di[u] = ee < e1 ? map(dst[ee]) : -1;
+            v[u] = di[u] >= 0 ? A[(long long)src[ee] * Bp + b] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (di[u] >= 0) F[di[u]] = T(v[u]);
+    }
+}
+
+// Extend-add of a child's contribution block into an LDS buffer:
+//   F[(loc[a] - r_off) * ld + loc[bb] - c_off] += C(a, bb)
+// for rows a in [a0, a1) and columns bb in [b0, b1) of the child's
+// contribution block C(a, bb) = Cf[((cnc + a) * cfs + cnc + bb) * cs] (cs is
+// the element stride of the child's arena).  The (a, bb) pairs are walked
+// flat over the workgroup with D loads in flight per thread; the pairs of
+// one child land on distinct positions, so no two threads add to the same
+// element (callers synchronise between children).
+template <int NT, typename T, int D = 16>
+__device__ inline void extend_add(T* __restrict__ F, int ld, int r_off, int c_off,
+                                  const T* __restrict__ Cf, long long cs, int cfs, int cnc,
+                                  const int* __restrict__ loc, int a0, int a1, int b0, int b1) {
+    const int nb = b1 - b0;
+    if (nb <= 0 || a1 <= a0) return;
+    const int total = (a1 - a0) * nb;
+    const int tid = threadIdx.x;
+    int a = tid / nb, bb = tid - (tid / nb) * nb;
+    const int da = NT / nb, db = NT - (NT / nb) * nb;
+    for (int e0 = tid; e0 < total; e0 += D * NT) {
+        T v[D];
+        int idx[D];
+#pragma unroll
+        for (int u = 0; u < D; ++u) {
+            idx[u] = -1;
+            v[u] = T(0);
+            if (e0 + u * NT < total) {
+                const int ga = a0 + a, gb = b0 + bb;
+                v[u] = Cf[(long long)((cnc + ga) * cfs + cnc + gb) * cs];
+                idx[u] = (loc[ga] - r_off) * ld + loc[gb] - c_off;
+            }
+            a += da;
+            bb += db;
+            if (bb >= nb) { bb -= nb; ++a; }
+        }
+#pragma unroll
+        for (int u = 0; u < D; ++u)
+            if (idx[u] >= 0) F[idx[u]] += v[u];
+    }
+}
+
+// Assembly of one block of a front from its task list (one task per child
+// whose contribution lands in the block; host-built, see LuDevice asm_*).
+// Up to kMaxSeg tasks at a time are loaded one per thread into LDS (with the
+// stage boundaries), then each stage is walked flat over the workgroup with
+// D loads in flight per thread:  F[map(loc[a], loc[bb])] += CB(a, bb).
+// Tasks of one stage land on distinct positions; stages are separated by
+// barriers, so every element receives its contributions in child order.
+constexpr int kMaxSeg = 64;
+constexpr int kMaxStages = 128;            // per list (checked when the lists are built)
+
+template <typename T>
+struct ChildSeg {
+    const T* C;          // element 0 of the child's front for this scenario
+    long long cs;        // element stride of the child's arena
+    const int* loc;      // contribution row -> parent-local row
+    int rowoff;          // cnc * cfs + cnc: CB(a, bb) = C[(rowoff + a * cfs + bb) * cs]
+    int cfs;
+    int a0, b0, nb;
+};
+
+template <int NT, int D, typename T, typename Map>
+__device__ inline void assemble_tasks(const LuDevice& d, const Arenas<T>& ar, int list, int b,
+                                      int Bp, T* __restrict__ F, ChildSeg<T>* seg, int* pref,
+                                      int* stg, Map map) {
+    const int tid = threadIdx.x;
+    const int t_begin = d.asm_tptr[list], t_end = d.asm_tptr[list + 1];
+    const int s_begin = d.asm_sptr[list], s_end = d.asm_sptr[list + 1];
+    for (int cb = t_begin; cb < t_end; cb += kMaxSeg) {      // uniform
+        const int nseg = min(kMaxSeg, t_end - cb);
+        if (tid < 64) {                         // wave 0: load tasks, inclusive count scan
+            int cnt = 0;
+            if (tid < nseg) {
+                const AsmTaskDev t = d.asm_tasks[cb + tid];
+                ChildSeg<T> g;
+                if (t.tier == 0) {
+                    g.C = ar.a0 + t.off * Bp + b;
+                    g.cs = Bp;
+                } else {
+                    g.C = ar.a1 + (long long)b * d.arena1 + t.off;
+                    g.cs = 1;
+                }
+                g.loc = d.asm_loc + t.loc_ptr;
+                g.rowoff = t.cnc * t.cfs + t.cnc;
+                g.cfs = t.cfs;
+                g.a0 = t.a0;
+                g.b0 = t.b0;
+                g.nb = t.nb;
+                cnt = t.na * t.nb;
+                seg[tid] = g;
+            }
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const int y = __shfl_up(cnt, o, 64);
+                if (tid >= o) cnt += y;
+            }
+            pref[tid + 1] = cnt;
+            if (tid == 0) pref[0] = 0;
+        }
+        for (int s = s_begin + tid; s < s_end; s += NT) stg[s - s_begin] = d.asm_stage[s];
+        __syncthreads();
+        for (int s = 0; s + 1 < s_end - s_begin; ++s) {
+            const int lo = max(stg[s], cb) - cb, hi = min(stg[s + 1], cb + nseg) - cb;
+            if (lo >= hi) continue;             // uniform
+            const int e_end = pref[hi];
+            int k = lo, kc = -1;
+            ChildSeg<T> g{};
+            for (int e0 = pref[lo] + tid; e0 < e_end; e0 += D * NT) {
+                T v[D];
+                int idx[D];
+#pragma unroll
+                for (int u = 0; u < D; ++u) {
+                    const int e = e0 + u * NT;
+                    idx[u] = -1;
+                    v[u] = T(0);
+                    if (e < e_end) {
+                        while (pref[k + 1] <= e) ++k;
+                        if (k != kc) {
+                            g = seg[k];
+                            kc = k;
+                        }
+                        const int local = e - pref[k];
+                        const int a = local / g.nb;
+                        const int ga = g.a0 + a, gb = g.b0 + (local - a * g.nb);
+                        v[u] = g.C[(long long)(g.rowoff + ga * g.cfs + gb) * g.cs];
+                        idx[u] = map(g.loc[ga], g.loc[gb]);
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < D; ++u)
+                    if (idx[u] >= 0) F[idx[u]] += v[u];
+            }
+            __syncthreads();
+        }
+    }
+}
+
 // ===========================================================================
 // Factor, tier 0 (lane per scenario)
 // ===========================================================================
@@ -127,23 +285,21 @@ __device__ inline void assemble_front(const LuDevice& d, const Arenas<T>& ar, co
     const int tid = threadIdx.x;
     for (int e = tid; e < fs * fs; e += NT) F[e] = T(0);
     __syncthreads();
-    for (int e = d.ent_ptr[p] + tid; e < d.ent_ptr[p + 1]; e += NT)
-        F[d.ent_dst[e]] = T(A[(long long)d.ent_src[e] * Bp + b]);
+    put_entries<NT>(F, d.ent_dst, d.ent_src, A, Bp, b, d.ent_ptr[p], d.ent_ptr[p + 1],
+                    [](int dst) { return dst; });
     __syncthreads();
-    for (int q = d.child_ptr[p]; q < d.child_ptr[p + 1]; ++q) {
-        const int c = d.child_idx[q];
-        const int cfs = d.fsz[c], cnc = d.nc[c], cuc = cfs - cnc;
-        long long cs;
-        const T* C = front_ptr(d, ar, c, b, Bp, cs);
-        const int* loc = d.asm_loc + d.asm_ptr[c];
-        for (int e = tid; e < cuc * cuc; e += NT) {
-            const int a = e / cuc, bb = e - a * cuc;
-            F[loc[a] * fs + loc[bb]] += C[(long long)((cnc + a) * cfs + cnc + bb) * cs];
-        }
-        __syncthreads();
-    }
+    __shared__ ChildSeg<T> seg[kMaxSeg];
+    __shared__ int pref[kMaxSeg + 1];
+    __shared__ int stg[kMaxStages + 1];
+    assemble_tasks<NT, 16>(d, ar, d.asm_base[p * kAsmKindCount + kAsmKFront], b, Bp, F, seg, pref,
+                           stg, [=](int r, int c) { return r * fs + c; });
 }
 
+// Tier 1: the front is assembled and factored in LDS.  The K pivot columns
+// and rows (the cross) are factored right-looking (column scaled by the
+// guarded pivot, then only the cross updated), and the contribution block
+// gets its rank-K update C -= L21 U12 once at the end, in 4 x 4 register
+// tiles.  The front is then written to the arena once.
 template <typename T>
 __global__ __launch_bounds__(kFactorBlock) void k_factor_t1(LuDevice d,
                                                             const int* __restrict__ fronts,
@@ -159,7 +315,9 @@ __global__ __launch_bounds__(kFactorBlock) void k_factor_t1(LuDevice d,
     const int fs = d.fsz[p], ncp = d.nc[p];
     const int tid = threadIdx.x;
     assemble_front(d, ar, A, Bp, p, b, fs, F);
+    __syncthreads();
     int flag = 0;
+    const int wc = fs - ncp;                    // contribution-block width
     for (int k = 0; k < ncp; ++k) {
         const T piv = F[k * fs + k];
         const T g = guard_pivot(piv, eps, static_piv, flag);
@@ -168,12 +326,53 @@ __global__ __launch_bounds__(kFactorBlock) void k_factor_t1(LuDevice d,
         const int m = fs - k - 1;
         for (int i = tid; i < m; i += kFactorBlock) F[(k + 1 + i) * fs + k] /= g;
         __syncthreads();
-        for (int e = tid; e < m * m; e += kFactorBlock) {
-            const int i = k + 1 + e / m, j = k + 1 + e % m;
+        // the cross: (a) rows > k x pivot columns (k, ncp); (b) pivot rows
+        // (k, ncp) x contribution columns [ncp, fs)
+        const int w = ncp - k - 1;
+        const int na = m * w, nt = na + w * wc;
+        for (int e = tid; e < nt; e += kFactorBlock) {
+            int i, j;
+            if (e < na) {
+                i = k + 1 + e / w;
+                j = k + 1 + (e - (e / w) * w);
+            } else {
+                const int e2 = e - na;
+                i = k + 1 + e2 / wc;
+                j = ncp + (e2 - (e2 / wc) * wc);
+            }
             F[i * fs + j] -= F[i * fs + k] * F[k * fs + j];
         }
         __syncthreads();
     }
+    // contribution block: C -= L21 U12 (rank ncp), 4 x 4 tiles per thread
+    const int tn = (wc + 3) / 4;
+    for (int t = tid; t < tn * tn; t += kFactorBlock) {
+        const int ti = t / tn, tj = t - (t / tn) * tn;
+        const int i0 = ncp + 4 * ti, j0 = ncp + 4 * tj;
+        T acc[4][4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+                acc[r][c] = (i0 + r < fs && j0 + c < fs) ? F[(i0 + r) * fs + j0 + c] : T(0);
+        for (int k = 0; k < ncp; ++k) {
+            T l[4], u[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) l[r] = i0 + r < fs ? F[(i0 + r) * fs + k] : T(0);
+#pragma unroll
+            for (int c = 0; c < 4; ++c) u[c] = j0 + c < fs ? F[k * fs + j0 + c] : T(0);
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) acc[r][c] -= l[r] * u[c];
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+                if (i0 + r < fs && j0 + c < fs) F[(i0 + r) * fs + j0 + c] = acc[r][c];
+    }
+    __syncthreads();
     T* G = ar.a1 + (long long)b * d.arena1 + d.off[p];
     for (int e = tid; e < fs * fs; e += kFactorBlock) G[e] = F[e];
     if (flag) atomicOr(sing + b, 1);
@@ -303,14 +502,15 @@ __global__ __launch_bounds__(kFactorBlock) void k_factor_t2(LuDevice d,
 // ===========================================================================
 // Assembly of tier-2 fronts by row blocks (before the factor kernel)
 //
-// One workgroup per (front, row block of kAsmRows rows, scenario).  The
-// block's rows are built in LDS -- zeroed, original entries stored (entries
-// are sorted by destination, so a row range is a contiguous entry range),
-// every child contribution row that lands in the range added (children in a
-// fixed order) -- and written to the arena once, coalesced.  Each front
-// element is written exactly once and each child contribution element read
-// once; there is no read-modify-write on global memory, and the small LDS
-// footprint keeps many workgroups per CU to hide memory latency.
+// One workgroup per (front, row block of rb_rows rows, scenario).  The
+// block's rows are built in LDS -- zeroed, original entries stored, every
+// child contribution row that lands in the range added (children in a fixed
+// order) -- and written to the arena once, coalesced.  The entry range and
+// each child's contribution-row range of the block come from the plan's row
+// bounds (two loads each, no search).  Each front element is written exactly
+// once and each child contribution element read once; there is no
+// read-modify-write on global memory, and the small LDS footprint keeps many
+// workgroups per CU to hide memory latency.
 // ===========================================================================
 constexpr int kAsmThreads = 256;
 constexpr int kAsmRows = 8;
@@ -338,45 +538,18 @@ __global__ __launch_bounds__(kAsmThreads) void k_assemble_rows(LuDevice d,
     const int i1 = min(fs, i0 + rb_rows);
     const int nr = i1 - i0;
     const int tid = threadIdx.x;
+    const int* rl = d.row_lb + d.lb_ptr[p];
     for (int e = tid; e < nr * fs; e += kAsmThreads) F[e] = T(0);
-    // entry range with destination in [i0*fs, i1*fs): binary search (sorted)
-    int lo = d.ent_ptr[p], hi = d.ent_ptr[p + 1];
-    {
-        int a = lo, z = hi;
-        while (a < z) { const int m = (a + z) >> 1; if (d.ent_dst[m] < i0 * fs) a = m + 1; else z = m; }
-        lo = a;
-        z = hi;
-        while (a < z) { const int m = (a + z) >> 1; if (d.ent_dst[m] < i1 * fs) a = m + 1; else z = m; }
-        hi = a;
-    }
+    const int lo = rl[i0], hi = rl[i1];
     __syncthreads();
-    for (int e = lo + tid; e < hi; e += kAsmThreads)
-        F[d.ent_dst[e] - i0 * fs] = T(A[(long long)d.ent_src[e] * Bp + b]);
+    put_entries<kAsmThreads>(F, d.ent_dst, d.ent_src, A, Bp, b, lo, hi,
+                             [=](int dst) { return dst - i0 * fs; });
     __syncthreads();
-    for (int q = d.child_ptr[p]; q < d.child_ptr[p + 1]; ++q) {
-        const int c = d.child_idx[q];
-        const int cfs = d.fsz[c], cnc = d.nc[c], cuc = cfs - cnc;
-        long long cs;
-        const T* C = front_ptr(d, ar, c, b, Bp, cs);
-        const int* loc = d.asm_loc + d.asm_ptr[c];
-        // loc is increasing (contribution rows and front rows are both sorted),
-        // so the child rows landing in [i0, i1) are one range: binary search
-        int alo = 0, ahi = cuc;
-        {
-            int z = cuc;
-            while (alo < z) { const int m = (alo + z) >> 1; if (loc[m] < i0) alo = m + 1; else z = m; }
-            ahi = alo;
-            z = cuc;
-            while (ahi < z) { const int m = (ahi + z) >> 1; if (loc[m] < i1) ahi = m + 1; else z = m; }
-        }
-        // (row, column) pairs of the range spread over all threads
-        const int na = ahi - alo;
-        for (int e = tid; e < na * cuc; e += kAsmThreads) {
-            const int a = alo + e / cuc, bb = e - (e / cuc) * cuc;
-            F[(loc[a] - i0) * fs + loc[bb]] += C[(long long)((cnc + a) * cfs + cnc + bb) * cs];
-        }
-        __syncthreads();
-    }
+    __shared__ ChildSeg<T> seg[kMaxSeg];
+    __shared__ int pref[kMaxSeg + 1];
+    __shared__ int stg[kMaxStages + 1];
+    assemble_tasks<kAsmThreads, 16>(d, ar, d.asm_base[p * kAsmKindCount + kAsmKRows] + rb, b, Bp, F,
+                                    seg, pref, stg, [=](int r, int c) { return (r - i0) * fs + c; });
     T* G = ar.a1 + (long long)b * d.arena1 + d.off[p] + (long long)i0 * fs;
     for (int e = tid; e < nr * fs; e += kAsmThreads) G[e] = F[e];
 }
@@ -674,6 +847,378 @@ __global__ __launch_bounds__(kDenseThreads) void k_factor_t2_mfma(LuDevice d,
 }
 
 // ===========================================================================
+// Tier 2, FP32 fronts with at most 64 pivots and 512 rows: pivot + Schur.
+//
+// A front of fsz rows and K pivots splits into the pivot block A11 (K x K),
+// the pivot rows A12 (K x uc), the pivot columns A21 (uc x K) and the
+// contribution block CB = A22 - L21 U12 (uc x uc), uc = fsz - K.  Two
+// launches per (level) group:
+//
+//   k_pivot (one workgroup per (front, scenario), 256 lanes): assembles A11
+//     straight from the J values and the children's contribution blocks,
+//     factors it with one row per lane in registers (guarded pivots), forms
+//     L11^-1 and U11^-1 by blocked triangular inversion, then assembles A12
+//     in chunks of 64 columns and writes U12 = L11^-1 A12 (MFMA).  U11^-1 is
+//     left in a per-(front, scenario) side buffer for the Schur kernel.
+//   k_schur (one workgroup per (front, 32-row block, scenario), 256 lanes):
+//     assembles its rows of A21 and A22 in LDS (entries and child
+//     contribution rows, children in a fixed order), forms its rows of
+//     L21 = A21 U11^-1 (MFMA, written to the front), then C = A22 - L21 U12
+//     on MFMA in 32 x 64 tiles per wave (FP32 v_mfma_f32_32x32x2_f32, or the
+//     3xBF16 split on v_mfma_f32_32x32x16_bf16), U12 read through L2, and
+//     writes C once.
+//
+// Every CB element is written once and read once (by the parent's kernels);
+// the front is never staged whole in HBM before it is factored, and the
+// long pivot columns are formed by the many Schur workgroups.
+// ===========================================================================
+constexpr int kPivThreads = 256;
+constexpr int kXMaxK = 64;
+constexpr int kXChunk = 64;                   // A12 columns per U12 step
+constexpr int kXCld = kXChunk + 4;            // chunk row stride (floats)
+constexpr int kUinvElems = kXMaxK * kXMaxK;   // side-buffer floats per (front, scenario)
+constexpr size_t kPivLds = size_t(3 * 64 * kLinvLd + 1024 + 2 * kXMaxK + 64 * kXCld) * 4;
+
+// Li = T^-1 for the 64 x 64 lower triangular T given by t(i, k), k <= i
+// (callers make rows and columns past the pivot count the identity): the
+// four 16 x 16 diagonal blocks by forward substitution (one column per
+// lane), then two doubling steps [A 0; C D]^-1 = [A^-1 0; -D^-1 C A^-1  D^-1]
+// at h = 16, 32.  Tmp holds 1024 floats.  Called by all NT threads; ends
+// synchronised.
+template <int NT, typename Get>
+__device__ inline void lower_inverse(Get t, float* Li, float* Tmp) {
+    const int tid = threadIdx.x;
+    for (int e = tid; e < 64 * 64; e += NT) Li[(e >> 6) * kLinvLd + (e & 63)] = 0.f;
+    __syncthreads();
+    for (int i = 0; i < 16; ++i) {
+        if (tid < 64) {
+            const int o = (tid >> 4) * 16, j = tid & 15;
+            float acc = i == j ? 1.f : 0.f;
+            for (int k = j; k < i; ++k) acc -= t(o + i, o + k) * Li[(o + k) * kLinvLd + o + j];
+            Li[(o + i) * kLinvLd + o + j] = acc / t(o + i, o + i);
+        }
+        __syncthreads();
+    }
+    for (int h = 16; h <= 32; h *= 2) {
+        const int total = 32 * h;              // (64 / 2h) pairs of h x h blocks
+        for (int e = tid; e < total; e += NT) {
+            const int q = e / (h * h), i = (e / h) % h, k = e % h, o = 2 * h * q;
+            float acc = 0.f;
+            for (int m = k; m < h; ++m)            // T = C A^-1 (A^-1 lower)
+                acc += t(o + h + i, o + m) * Li[(o + m) * kLinvLd + o + k];
+            Tmp[e] = acc;
+        }
+        __syncthreads();
+        for (int e = tid; e < total; e += NT) {
+            const int q = e / (h * h), i = (e / h) % h, k = e % h, o = 2 * h * q;
+            float acc = 0.f;
+            for (int m = 0; m <= i; ++m)           // -D^-1 T (D^-1 lower)
+                acc -= Li[(o + h + i) * kLinvLd + o + h + m] * Tmp[q * h * h + m * h + k];
+            Li[(o + h + i) * kLinvLd + o + k] = acc;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kPivThreads) void k_pivot(LuDevice d, const int* __restrict__ fronts,
+                                                        const double* __restrict__ A, int Bp,
+                                                        Arenas<float> ar, float* __restrict__ uinv,
+                                                        double eps, int static_piv,
+                                                        int* __restrict__ sing) {
+    extern __shared__ unsigned char smem_raw[];
+    float* P = reinterpret_cast<float*>(smem_raw);                 // A11: [64][kLinvLd]
+    float* Li = P + 64 * kLinvLd;                                 // L11^-1
+    float* Wt = Li + 64 * kLinvLd;                                // (U11^T)^-1 = (U11^-1)^T
+    float* Tmp = Wt + 64 * kLinvLd;                               // [1024]
+    float* prow = Tmp + 1024;                                     // [2][64]
+    float* Ch = prow + 2 * kXMaxK;                                // [64][kXCld]
+    int fi, b;
+    decode_block(Bp, fi, b);
+    const int p = fronts[fi];
+    const int fs = d.fsz[p], K = d.nc[p];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    float* G = ar.a1 + (long long)b * d.arena1 + d.off[p];
+    const int* rl = d.row_lb + d.lb_ptr[p];
+
+    // ---- A11 -------------------------------------------------------------
+    for (int e = tid; e < 64 * kLinvLd; e += kPivThreads) P[e] = 0.f;
+    __syncthreads();
+    put_entries<kPivThreads>(P, d.ent_dst, d.ent_src, A, Bp, b, rl[0], rl[K], [=](int dst) {
+        const int r = dst / fs, c = dst - r * fs;
+        return c < K ? r * kLinvLd + c : -1;
+    });
+    __syncthreads();
+    __shared__ ChildSeg<float> seg[kMaxSeg];
+    __shared__ int pref[kMaxSeg + 1];
+    __shared__ int stg[kMaxStages + 1];
+    const int* lbase = d.asm_base + p * kAsmKindCount;
+    assemble_tasks<kPivThreads, 16>(d, ar, lbase[kAsmKPivot], b, Bp, P, seg, pref, stg,
+                                    [=](int r, int c) { return r * kLinvLd + c; });
+
+    // ---- LU of A11, one row per lane in registers ---------------------------
+    int flag = 0;
+    const int r = tid;
+    float row[kXMaxK];
+#pragma unroll
+    for (int c = 0; c < kXMaxK; ++c) row[c] = r < K ? P[r * kLinvLd + c] : 0.f;
+#pragma unroll
+    for (int c = 0; c < kXMaxK; ++c) {
+        float* pr = prow + (c & 1) * kXMaxK;
+        if (r == c && c < K) {
+            const float g = guard_pivot(row[c], eps, static_piv, flag);
+            row[c] = g;
+#pragma unroll
+            for (int cc = c; cc < kXMaxK; ++cc) pr[cc] = row[cc];
+        }
+        __syncthreads();
+        if (c < K) {                              // uniform
+            const float piv = pr[c];
+            const bool act = r > c && r < K;
+            const float l = row[c] / piv;
+            const float lm = act ? l : 0.f;
+            if (act) row[c] = l;
+#pragma unroll
+            for (int cc = c + 1; cc < kXMaxK; ++cc) row[cc] -= lm * pr[cc];
+        }
+    }
+    if (r < K) {
+#pragma unroll
+        for (int c = 0; c < kXMaxK; ++c) {
+            P[r * kLinvLd + c] = row[c];
+            if (c < K) G[(long long)r * fs + c] = row[c];
+        }
+    }
+    __syncthreads();
+    // L11^-1 (unit lower) and (U11^T)^-1; rows and columns >= K: identity
+    lower_inverse<kPivThreads>([=](int i, int k) {
+        return i == k ? 1.f : ((k < i && i < K) ? P[i * kLinvLd + k] : 0.f);
+    }, Li, Tmp);
+    lower_inverse<kPivThreads>([=](int i, int k) {
+        return (i < K && k <= i) ? P[k * kLinvLd + i] : (i == k ? 1.f : 0.f);
+    }, Wt, Tmp);
+    {
+        float* Ui = uinv + ((long long)fi * Bp + b) * kUinvElems;   // U11^-1 [k][j]
+        for (int e = tid; e < kUinvElems; e += kPivThreads)
+            Ui[e] = Wt[(e & 63) * kLinvLd + (e >> 6)];
+    }
+
+    // ---- U12 = L11^-1 A12 in chunks of kXChunk columns ----------------------
+    for (int j0 = K; j0 < fs; j0 += kXChunk) {
+        const int jn = min(kXChunk, fs - j0);
+        for (int e = tid; e < 64 * kXCld; e += kPivThreads) Ch[e] = 0.f;
+        __syncthreads();
+        put_entries<kPivThreads>(Ch, d.ent_dst, d.ent_src, A, Bp, b, rl[0], rl[K], [=](int dst) {
+            const int rr = dst / fs, c = dst - rr * fs;
+            return (c >= j0 && c < j0 + jn) ? rr * kXCld + c - j0 : -1;
+        });
+        __syncthreads();
+        assemble_tasks<kPivThreads, 16>(d, ar, lbase[kAsmKChunk] + (j0 - K) / kXChunk, b, Bp, Ch, seg,
+                                        pref, stg, [=](int r, int c) { return r * kXCld + c - j0; });
+        // 64 x 64 output = 2 x 2 tiles of 32 x 32, one per wave; L11^-1 is
+        // lower triangular, so the top row half needs k < 32 only
+        const int rh = wave & 1, ct = wave >> 1;
+        if (32 * ct < jn && 32 * rh < K) {      // wave-uniform
+            floatx16 acc;
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[v] = 0.f;
+            const int kend = rh == 0 ? 32 : 64;
+            for (int kk = 0; kk < kend; kk += 2) {
+                const int k = kk + (lane >> 5);
+                const float av = Li[(32 * rh + (lane & 31)) * kLinvLd + k];
+                const float bv = Ch[k * kXCld + 32 * ct + (lane & 31)];
+                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv, acc, 0, 0, 0);
+            }
+            const int j = j0 + 32 * ct + (lane & 31);
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const int i = 32 * rh + (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
+                if (i < K && j < fs) G[(long long)i * fs + j] = acc[v];
+            }
+        }
+        __syncthreads();                        // before the next chunk reuses Ch
+    }
+    if (flag) atomicOr(sing + b, 1);
+}
+
+constexpr int kSchurThreads = 256;
+constexpr int kSchurRows = 32;
+constexpr int kSchurScen = 16;                 // scenarios interleaved inside an XCD range
+constexpr int kSLd = kXMaxK + 2;               // L21 block row stride (floats)
+
+inline size_t schur_lds(int max_fsz) {
+    return size_t(kSchurRows) * kSLd * 4 + size_t(kSchurRows) * max_fsz * 4;
+}
+
+template <int MODE>
+__global__ __launch_bounds__(kSchurThreads, 2) void k_schur(LuDevice d, const int* __restrict__ fronts,
+                                                         const double* __restrict__ A, int Bp,
+                                                         Arenas<float> ar,
+                                                         const float* __restrict__ uinv, int nrb) {
+    extern __shared__ unsigned char smem_raw[];
+    float* Lb = reinterpret_cast<float*>(smem_raw);               // [32][kSLd]: A21, then L21
+    float* Ab = Lb + kSchurRows * kSLd;                           // [32][uc]
+    // lin -> (front, scenario group, row block, scenario in group): inside an
+    // XCD's contiguous range, kSchurScen neighbouring scenarios run together
+    // (they share the cache lines of the interleaved J values) over the row
+    // blocks of one front (they share that scenario's U12 in L2)
+    const long long lin = blockIdx.x;
+    const long long per = (long long)gridDim.x / kXcd;
+    long long w = (lin % kXcd) * per + lin / kXcd;
+    const int bl = int(w % kSchurScen);
+    w /= kSchurScen;
+    const int rb = int(w % nrb);
+    w /= nrb;
+    const int ngrp = Bp / kSchurScen;
+    const int b = int(w % ngrp) * kSchurScen + bl;
+    const int fi = int(w / ngrp);
+    const int p = fronts[fi];
+    const int fs = d.fsz[p], K = d.nc[p], uc = fs - K;
+    const int i0 = K + rb * kSchurRows;
+    if (i0 >= fs) return;                       // whole block exits together
+    const int i1 = min(fs, i0 + kSchurRows);
+    const int nr = i1 - i0;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    float* G = ar.a1 + (long long)b * d.arena1 + d.off[p];
+    const int* rl = d.row_lb + d.lb_ptr[p];
+
+    // ---- U12 operands (B of the update), per wave and tile.  B-operand k of
+    // slot i:
+    //   FP32 (32x32x2):  k = 2 i + lane/32;
+    //   3xBF16 (32x32x16): k = 16 (i / 8) + 8 (lane/32) + i % 8.
+    const int ntile = (uc + 63) / 64;
+    float b0[kXMaxK / 2], b1[kXMaxK / 2];
+    auto kslot = [&](int i) {
+        return MODE == kMfmaF32 ? 2 * i + (lane >> 5) : 16 * (i >> 3) + 8 * (lane >> 5) + (i & 7);
+    };
+    auto load_u = [&](int t) {
+        const int ja = t * 64 + (lane & 31);
+        const bool oka = ja < uc, okb = ja + 32 < uc;
+        const float* U = G + K + ja;            // U12 row k at U + k * fs
+#pragma unroll
+        for (int i = 0; i < kXMaxK / 2; ++i) {
+            const int k = kslot(i);
+            b0[i] = (k < K && oka) ? U[(long long)k * fs] : 0.f;
+            b1[i] = (k < K && okb) ? U[(long long)k * fs + 32] : 0.f;
+        }
+    };
+    const bool l21w = wave < 2 && 32 * wave < K;          // waves forming L21
+    const int jc = 32 * wave + (lane & 31);              // L21 column of this lane
+
+    // ---- rows [i0, i1) of A21 (-> Lb) and A22 (-> Ab) -----------------------
+    for (int e = tid; e < kSchurRows * (kSLd + uc); e += kSchurThreads) Lb[e] = 0.f;
+    const int lo = rl[i0], hi = rl[i1];
+    __syncthreads();
+    put_entries<kSchurThreads>(Lb, d.ent_dst, d.ent_src, A, Bp, b, lo, hi, [=](int dst) {
+        const int rr = dst / fs - i0, c = dst - (dst / fs) * fs;
+        return c < K ? rr * kSLd + c : kSchurRows * kSLd + rr * uc + c - K;
+    });
+    __syncthreads();
+    __shared__ ChildSeg<float> seg[kMaxSeg];
+    __shared__ int pref[kMaxSeg + 1];
+    __shared__ int stg[kMaxStages + 1];
+    assemble_tasks<kSchurThreads, 16>(d, ar, d.asm_base[p * kAsmKindCount + kAsmKSchur] + rb, b, Bp, Lb,
+                                      seg, pref, stg, [=](int r, int c) {
+                                             return c < K ? (r - i0) * kSLd + c
+                                                          : kSchurRows * kSLd + (r - i0) * uc + c - K;
+                                         });
+
+    // the first tile's U12 operands load while L21 is formed
+    if (wave < ntile) load_u(wave);
+
+    // ---- L21 = A21 U11^-1 (two 32 x 32 tiles, waves 0 and 1) ---------------
+    {
+        floatx16 acc;
+#pragma unroll
+        for (int v = 0; v < 16; ++v) acc[v] = 0.f;
+        if (l21w) {                                       // wave-uniform
+            const float* Ui = uinv + ((long long)fi * Bp + b) * kUinvElems;
+            float ub[kXMaxK / 2];
+#pragma unroll
+            for (int i = 0; i < kXMaxK / 2; ++i) {
+                const int k = 2 * i + (lane >> 5);
+                ub[i] = k < 32 * wave + 32 ? Ui[k * kXMaxK + jc] : 0.f;   // U11^-1 upper
+            }
+#pragma unroll
+            for (int i = 0; i < kXMaxK / 2; ++i) {
+                if (2 * i < 32 * wave + 32) {             // U11^-1 upper: k <= j
+                    const float av = Lb[(lane & 31) * kSLd + 2 * i + (lane >> 5)];
+                    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av, ub[i], acc, 0, 0, 0);
+                }
+            }
+        }
+        __syncthreads();                                  // every read of A21 done
+        if (l21w) {
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const int rr = (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
+                const float x = jc < K ? acc[v] : 0.f;
+                Lb[rr * kSLd + jc] = x;
+                if (rr < nr && jc < K) G[(long long)(i0 + rr) * fs + jc] = x;
+            }
+        }
+        __syncthreads();
+    }
+
+    // ---- C = A22 - L21 U12, 32 x 64 tiles, one wave each --------------------
+    for (int t = wave; t < ntile; t += kSchurThreads / 64) {
+        const int ja = t * 64 + (lane & 31);    // CB column of acc0 (acc1: + 32)
+        const bool oka = ja < uc, okb = ja + 32 < uc;
+        floatx16 acc0, acc1;
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+            const int rr = (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
+            acc0[v] = oka ? Ab[rr * uc + ja] : 0.f;
+            acc1[v] = okb ? Ab[rr * uc + ja + 32] : 0.f;
+        }
+        if (MODE == kMfmaF32) {
+#pragma unroll
+            for (int i = 0; i < kXMaxK / 2; ++i) {
+                if (2 * i < K) {                // uniform
+                    const float a = -Lb[(lane & 31) * kSLd + 2 * i + (lane >> 5)];
+                    acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b0[i], acc0, 0, 0, 0);
+                    acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b1[i], acc1, 0, 0, 0);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int kq = 0; kq < kXMaxK / 16; ++kq) {
+                if (16 * kq < K) {              // uniform
+                    const int kb8 = 16 * kq + 8 * (lane >> 5);
+                    bf16x8 ah, al, b0h, b0l, b1h, b1l;
+#pragma unroll
+                    for (int t8 = 0; t8 < 8; ++t8) {
+                        __bf16 h, lo2;
+                        split_bf16(-Lb[(lane & 31) * kSLd + kb8 + t8], h, lo2);
+                        ah[t8] = h; al[t8] = lo2;
+                        split_bf16(b0[8 * kq + t8], h, lo2);
+                        b0h[t8] = h; b0l[t8] = lo2;
+                        split_bf16(b1[8 * kq + t8], h, lo2);
+                        b1h[t8] = h; b1l[t8] = lo2;
+                    }
+                    acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, b0h, acc0, 0, 0, 0);
+                    acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, b0l, acc0, 0, 0, 0);
+                    acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, b0h, acc0, 0, 0, 0);
+                    acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, b1h, acc1, 0, 0, 0);
+                    acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, b1l, acc1, 0, 0, 0);
+                    acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, b1h, acc1, 0, 0, 0);
+                }
+            }
+        }
+        // next tile's U12 operands in flight during this tile's stores
+        if (t + kSchurThreads / 64 < ntile) load_u(t + kSchurThreads / 64);
+        float* Cw = G + (long long)i0 * fs + K + ja;
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+            const int rr = (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
+            if (rr < nr) {
+                if (oka) Cw[(long long)rr * fs] = acc0[v];
+                if (okb) Cw[(long long)rr * fs + 32] = acc1[v];
+            }
+        }
+    }
+}
+
+// ===========================================================================
 // Solve, tier 0 (lane per scenario): w kept in LDS as w[l][lane]
 // ===========================================================================
 template <typename T, bool TRANS>
@@ -926,10 +1471,14 @@ __global__ __launch_bounds__(256) void k_scatter(const int* __restrict__ perm, i
 
 }  // namespace
 
+int assemble_rows_block(int max_fsz, int elem) {
+    return std::max(1, std::min(kAsmRows, (32 * 1024) / std::max(1, max_fsz * elem)));
+}
+
 template <typename T>
 void assemble_rows(const LuDevice& d, const int* fronts, int count, int max_fsz, const double* A,
                    int Bp, Arenas<T> ar, hipStream_t s) {
-    const int rb_rows = std::max(1, std::min(kAsmRows, (32 * 1024) / int(max_fsz * sizeof(T))));
+    const int rb_rows = assemble_rows_block(max_fsz, int(sizeof(T)));
     const size_t shmem = size_t(rb_rows) * max_fsz * sizeof(T);
     if (shmem > size_t(kLdsBytes)) throw std::runtime_error("assemble_rows: front row exceeds LDS");
     const int nrb = (max_fsz + rb_rows - 1) / rb_rows;
@@ -939,6 +1488,53 @@ void assemble_rows(const LuDevice& d, const int* fronts, int count, int max_fsz,
                        fronts, A, Bp, ar, nrb, rb_rows);
     CUPF_HIP(hipGetLastError());
 }
+
+bool cross_schur_path(int max_fsz, int max_nc) {
+    return max_fsz <= kDenseMaxFsz && max_nc <= kXMaxK;
+}
+
+void cross_group(const LuDevice& d, const int* fronts, int count, const double* A, int Bp,
+                 Arenas<float> ar, float* uinv, double pivot_eps, int static_piv, int* sing,
+                 hipStream_t s) {
+    if (count <= 0) return;
+    static bool attr_set = false;
+    if (!attr_set) {
+        CUPF_HIP(hipFuncSetAttribute((const void*)k_pivot, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     int(kPivLds)));
+        attr_set = true;
+    }
+    hipLaunchKernelGGL(k_pivot, dim3(unsigned(count) * unsigned(Bp)), dim3(kPivThreads), kPivLds, s, d,
+                       fronts, A, Bp, ar, uinv, pivot_eps, static_piv, sing);
+    CUPF_HIP(hipGetLastError());
+}
+
+void schur_group(const LuDevice& d, const int* fronts, int count, int max_fsz, const double* A,
+                 int Bp, Arenas<float> ar, const float* uinv, int dense_mode, hipStream_t s) {
+    if (count <= 0) return;
+    if (max_fsz > kDenseMaxFsz || Bp % kSchurScen)
+        throw std::runtime_error("schur_group: front or batch outside the kernel's limits");
+    const int nrb = (max_fsz + kSchurRows - 1) / kSchurRows;
+    const long long blocks = (long long)count * nrb * Bp;
+    if (blocks >= (1LL << 31)) throw std::runtime_error("schur_group: grid too large");
+    const size_t shmem = schur_lds(max_fsz);
+    static bool attr_set = false;
+    if (!attr_set) {
+        CUPF_HIP(hipFuncSetAttribute((const void*)k_schur<kMfmaF32>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, int(schur_lds(kDenseMaxFsz))));
+        CUPF_HIP(hipFuncSetAttribute((const void*)k_schur<kMfmaBf16x3>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, int(schur_lds(kDenseMaxFsz))));
+        attr_set = true;
+    }
+    if (dense_mode == kMfmaBf16x3)
+        hipLaunchKernelGGL(k_schur<kMfmaBf16x3>, dim3(unsigned(blocks)), dim3(kSchurThreads), shmem, s, d,
+                           fronts, A, Bp, ar, uinv, nrb);
+    else
+        hipLaunchKernelGGL(k_schur<kMfmaF32>, dim3(unsigned(blocks)), dim3(kSchurThreads), shmem, s, d,
+                           fronts, A, Bp, ar, uinv, nrb);
+    CUPF_HIP(hipGetLastError());
+}
+
+size_t pivot_side_elems(int count, int Bp) { return size_t(count) * Bp * kUinvElems; }
 
 template <typename T>
 void lu_factor_group(const LuDevice& d, int tier, const int* fronts, int count, int max_fsz,

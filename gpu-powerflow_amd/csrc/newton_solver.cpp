@@ -118,7 +118,8 @@ struct NewtonSolver::Impl {
     DeviceBuffer d_yp, d_yi, d_y, d_row_pvpq, d_row_pq, d_m11, d_m21, d_m12, d_m22, d_zero,
         d_row_bus, d_row_isq, d_jrp, d_jci, d_jtrp, d_jtci, d_jtmap;
     DeviceBuffer d_fsz, d_nc, d_tier, d_off, d_uoff, d_fptr, d_frows, d_cptr, d_cidx, d_aptr,
-        d_aloc, d_eptr, d_esrc, d_edst, d_perm, d_groups;
+        d_aloc, d_eptr, d_esrc, d_edst, d_perm, d_groups, d_lbptr, d_rowlb, d_abase, d_atptr,
+        d_asptr, d_astage, d_atasks;
     kern::NrDevice nr;
     kern::LuDevice lu;
 
@@ -127,6 +128,7 @@ struct NewtonSolver::Impl {
     int B = 0, Bp = 0;
     DeviceBuffer V, Va, Vm, S, F, normF, bmax, J, arena0, arena1, ywork, upd, dx, ddx, res, sing,
         lam, g, stage_a, stage_b, stage_c;
+    DeviceBuffer uinv;     // U11^-1 side buffer of the pivot + Schur groups
 
     // learned warm start (not owned)
     WarmStartMLP* warm = nullptr;
@@ -172,6 +174,12 @@ struct NewtonSolver::Impl {
             sing.allocate(nb * sizeof(int));
             lam.allocate(nb * js.dim * sizeof(double));
             g.allocate(nb * js.dim * sizeof(double));
+            int cross_count = 0;
+            for (int L = 0; L < plan.num_levels; ++L)
+                if (cross_schur(L))
+                    cross_count = std::max(cross_count, plan.group_ptr[L * 3 + 3] - plan.group_ptr[L * 3 + 2]);
+            if (cross_count > 0)
+                uinv.allocate(kern::pivot_side_elems(cross_count, Bp) * sizeof(float));
             capBp = Bp;
         }
         if (batch > capB) {
@@ -190,7 +198,16 @@ struct NewtonSolver::Impl {
         return d_groups.as<int>() + plan.group_ptr[gi];
     }
 
+    // FP32 tier-2 group factored by the cross + Schur kernels
+    bool cross_schur(int level) const {
+        const int gi = level * 3 + 2;
+        return !front64 && dense_mode != 0 &&
+               kern::cross_schur_path(plan.group_max_fsz[gi], plan.group_max_nc[gi]);
+    }
+
     // ---- factor the current J values ------------------------------------
+    // Stage kStFactorDense times the dense tier-2 work: the Schur launch of a
+    // cross + Schur group, the whole group otherwise.
     template <typename T>
     void factor_t(kern::Arenas<T> ar) {
         for (int L = 0; L < plan.num_levels; ++L)
@@ -198,6 +215,16 @@ struct NewtonSolver::Impl {
                 int cnt, mf;
                 const int* f = group(L, t, cnt, mf);
                 if (cnt == 0) continue;
+                if constexpr (sizeof(T) == 4) {
+                    if (t == 2 && cross_schur(L)) {
+                        kern::cross_group(lu, f, cnt, J.as<double>(), Bp, ar, uinv.as<float>(),
+                                          piv_eps, static_piv ? 1 : 0, sing.as<int>(), s);
+                        StageScope sc(timer, kStFactorDense, s);
+                        kern::schur_group(lu, f, cnt, mf, J.as<double>(), Bp, ar, uinv.as<float>(),
+                                          dense_mode, s);
+                        continue;
+                    }
+                }
                 StageScope sc(timer, t == 2 ? kStFactorDense : -1, s);
                 kern::lu_factor_group<T>(lu, t, f, cnt, mf, J.as<double>(), Bp, ar, piv_eps,
                                          static_piv ? 1 : 0, sing.as<int>(), dense_mode, s);
@@ -678,6 +705,35 @@ void NewtonSolver::initialize(const YbusView& ybus, const int32_t* pv, int32_t n
     m.d_edst = upload_vector(P.ent_dst, s);
     m.d_perm = upload_vector(P.perm, s);
     m.d_groups = upload_vector(P.group_fronts, s);
+    m.d_lbptr = upload_vector(P.lb_ptr, s);
+    m.d_rowlb = upload_vector(P.row_lb, s);
+    {
+        // assembly task lists for the kernels each front's group will run
+        static_assert(sizeof(AsmTask) == sizeof(kern::AsmTaskDev), "assembly task layout");
+        std::vector<int> kinds(size_t(P.num_fronts), 0), rpb(size_t(P.num_fronts), 1);
+        for (int L = 0; L < P.num_levels; ++L)
+            for (int t = 1; t < 3; ++t) {
+                const int gi = L * 3 + t;
+                for (int q = P.group_ptr[gi]; q < P.group_ptr[gi + 1]; ++q) {
+                    const int f = P.group_fronts[q];
+                    if (t == 1) {
+                        kinds[size_t(f)] = 1 << kern::kAsmKFront;
+                    } else if (m.cross_schur(L)) {
+                        kinds[size_t(f)] = (1 << kern::kAsmKSchur) | (1 << kern::kAsmKPivot) |
+                                           (1 << kern::kAsmKChunk);
+                    } else {
+                        kinds[size_t(f)] = 1 << kern::kAsmKRows;
+                        rpb[size_t(f)] = kern::assemble_rows_block(P.group_max_fsz[gi], int(m.tsize()));
+                    }
+                }
+            }
+        const AsmLists al = build_asm_lists(P, kinds, rpb);
+        m.d_abase = upload_vector(al.base, s);
+        m.d_atptr = upload_vector(al.task_ptr, s);
+        m.d_asptr = upload_vector(al.stage_ptr, s);
+        m.d_astage = upload_vector(al.stage, s);
+        m.d_atasks = upload_vector(al.tasks, s);
+    }
     CUPF_HIP(hipStreamSynchronize(s));
 
     kern::NrDevice& nr = m.nr;
@@ -699,6 +755,10 @@ void NewtonSolver::initialize(const YbusView& ybus, const int32_t* pv, int32_t n
     lu.asm_ptr = m.d_aptr.as<int>(); lu.asm_loc = m.d_aloc.as<int>();
     lu.ent_ptr = m.d_eptr.as<int>(); lu.ent_src = m.d_esrc.as<int>();
     lu.ent_dst = m.d_edst.as<int>(); lu.perm = m.d_perm.as<int>();
+    lu.lb_ptr = m.d_lbptr.as<int>(); lu.row_lb = m.d_rowlb.as<int>();
+    lu.asm_base = m.d_abase.as<int>(); lu.asm_tptr = m.d_atptr.as<int>();
+    lu.asm_sptr = m.d_asptr.as<int>(); lu.asm_stage = m.d_astage.as<int>();
+    lu.asm_tasks = m.d_atasks.as<kern::AsmTaskDev>();
 
     m.capBp = 0;   // batch buffers are sized lazily for the new structure
     m.capB = 0;
@@ -707,11 +767,34 @@ void NewtonSolver::initialize(const YbusView& ybus, const int32_t* pv, int32_t n
     m.st.max_front = P.max_fsz; m.st.max_pivots = P.max_nc;
     m.st.arena_elems = P.arena0 + P.arena1;
     m.st.lnz = P.lnz; m.st.factor_flops = P.flops;
+    // Algorithmic work of the timed dense stage, per scenario.
+    //  pivot + Schur groups (the Schur kernel): flops 2 uc^2 K (update) +
+    //    2 uc K^2 (L21); bytes = the rows >= K written once (CB uc^2, L21
+    //    uc K) + the child contribution elements landing in them read once +
+    //    U12 (K uc) read once, at 4 B, + the J entries of those rows at 8 B;
+    //  other tier-2 groups (assembly + dense LU): flops of the front LU;
+    //    bytes = each front element written and read once + its J entries.
     for (int p = 0; p < P.num_fronts; ++p) {
         if (P.tier[p] != 2) continue;
-        const double f = P.fsz[p];
+        const int fsz = P.fsz[p], K = P.nc[p];
+        const double f = fsz;
+        if (m.cross_schur(P.level[p])) {
+            const double uc = f - K;
+            double child = 0.0, ents = 0.0;
+            const int* rl = P.row_lb.data() + P.lb_ptr[p];
+            for (int q = P.child_ptr[p]; q < P.child_ptr[p + 1]; ++q) {
+                const int c = P.child_idx[q];
+                const int* cl = rl + (1 + q - P.child_ptr[p]) * (fsz + 1);
+                const double cuc = P.fsz[c] - P.nc[c];
+                child += (cuc - cl[K]) * cuc;          // contribution rows landing at >= K
+            }
+            ents = rl[fsz] - rl[K];
+            m.st.dense_flops += 2.0 * uc * uc * K + 2.0 * uc * K * K;
+            m.st.dense_bytes += 4.0 * (uc * uc + child + 2.0 * uc * K) + 8.0 * ents;
+            continue;
+        }
         double fl = 0.0;
-        for (int k = 0; k < P.nc[p]; ++k) {
+        for (int k = 0; k < K; ++k) {
             const double r = f - k - 1;
             fl += 2.0 * r * r + r;
         }

@@ -76,7 +76,7 @@ struct CustomSolverConfig {
     double shift_retry_epsilon = 1.0e-8;    // static pivot threshold and shift
     // Additions (not in the reference): FP64 refinement steps of the Newton
     // step for ComputePolicy::Mixed.
-    int refine_steps = 2;
+    int refine_steps = 1;
 };
 
 struct NewtonOptions {

@@ -45,7 +45,7 @@ def test_default_options_roundtrip():
     capi.lib().cupf_default_newton_options(ctypes.byref(o))
     assert all(v == 0xAB for v in buf[ctypes.sizeof(Opts):]), "wrote past the declared struct"
     assert o.backend == 1 and o.compute == 2 and o.cuda_linear_solver == 1
-    assert o.custom.cap == 8 and o.custom.eps == 1e-8 and o.custom.refine == 2
+    assert o.custom.cap == 8 and o.custom.eps == 1e-8 and o.custom.refine == 1
 
 
 def test_host_analysis_errors_map_to_value_error():
