@@ -605,6 +605,7 @@ AsmLists build_asm_lists(const LuPlan& L, const std::vector<int>& kinds,
     A.base.assign(size_t(P) * kAsmKinds, -1);
     A.task_ptr.push_back(0);
     A.stage_ptr.push_back(0);
+    A.item_ptr.push_back(0);
     struct Piece { int q, a0, a1, b0, b1; };
     // landing-set intersection of two pieces of (possibly) different children
     auto meets = [&](const Piece& x, const Piece& y) {
@@ -685,6 +686,25 @@ AsmLists build_asm_lists(const LuPlan& L, const std::vector<int>& kinds,
                 }
                 if (st.size() > 128)
                     throw std::runtime_error("build_asm_lists: more than 128 assembly stages in a block");
+                // row items (Schur blocks): per owner wave, in task order
+                const int tb = A.task_ptr.back(), te = int(A.tasks.size());
+                for (int wv = 0; wv < kAsmItemWaves; ++wv) {
+                    if (kind == 0) {
+                        if (te - tb > kAsmMaxItemTasks)
+                            throw std::runtime_error("build_asm_lists: too many tasks in a Schur block");
+                        for (int t = tb; t < te; ++t) {
+                            const AsmTask& tk = A.tasks[size_t(t)];
+                            const int* loc = L.asm_loc.data() + tk.loc_ptr;
+                            for (int a = tk.a0; a < tk.a0 + tk.na; ++a) {
+                                const int rr = loc[a] - r0;
+                                if (rr % kAsmItemWaves != wv) continue;
+                                if (a >= (1 << 19)) throw std::runtime_error("build_asm_lists: row index too large");
+                                A.items.push_back((t - tb) | (rr << 7) | (a << 13));
+                            }
+                        }
+                    }
+                    A.item_ptr.push_back(int(A.items.size()));
+                }
                 A.task_ptr.push_back(int(A.tasks.size()));
                 A.stage_ptr.push_back(int(A.stage.size()));
             }

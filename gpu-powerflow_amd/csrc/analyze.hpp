@@ -156,12 +156,23 @@ struct AsmTask {                     // 48 bytes, device-visible layout
     int pad0, pad1;
 };
 
+// Row items of the Schur lists (kind 0): the block's rows are owned by
+// kAsmItemWaves waves (row rr by wave rr % kAsmItemWaves); every landing
+// contribution row is one item of its owner, packed as
+// task (bits 0-6, index inside the list) | block row rr (bits 7-12) |
+// contribution row a (bits 13-31).  A wave's items are in task order, so
+// each front element receives its contributions in a fixed order.
+constexpr int kAsmItemWaves = 4;
+constexpr int kAsmMaxItemTasks = 128;
+
 struct AsmLists {
     std::vector<int> base;           // [P * kAsmKinds]: first list id of (front, kind), -1 if none
     std::vector<int> task_ptr;       // [lists + 1] into tasks
     std::vector<int> stage_ptr;      // [lists + 1] into stage
     std::vector<int> stage;          // stage boundaries: absolute task indices
     std::vector<AsmTask> tasks;
+    std::vector<int> item_ptr;       // [lists * kAsmItemWaves + 1] into items
+    std::vector<int> items;
 };
 
 // kinds[p] is a bit mask of the kinds built for front p; rows_per_block[p]

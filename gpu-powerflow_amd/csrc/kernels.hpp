@@ -118,11 +118,15 @@ struct LuDevice {
     const int* perm = nullptr;
     const int* lb_ptr = nullptr;      // row bounds of the row-block assembly (LuPlan)
     const int* row_lb = nullptr;
+    const int* cstage_ptr = nullptr;  // child stages (LuPlan stage_ptr / stage)
+    const int* cstage = nullptr;
     const int* asm_base = nullptr;    // assembly task lists (analyze.hpp AsmLists)
     const int* asm_tptr = nullptr;
     const int* asm_sptr = nullptr;
     const int* asm_stage = nullptr;
     const AsmTaskDev* asm_tasks = nullptr;
+    const int* asm_iptr = nullptr;    // row items of the Schur lists, 4 owner waves per list
+    const int* asm_items = nullptr;
 };
 
 // Block height of the row-block assembly of a tier-2 group (elem = sizeof T).
@@ -162,11 +166,11 @@ void schur_group(const LuDevice& d, const int* fronts, int count, int max_fsz, c
 // forward sweep of each front; scatter writes x[perm[k]] = y[k].
 template <typename T>
 void lu_solve_forward_group(const LuDevice& d, int tier, const int* fronts, int count,
-                            int max_fsz, Arenas<T> ar, const double* rhs, T* y, T* upd, int Bp,
-                            bool transpose, hipStream_t s);
+                            int max_fsz, int max_nc, Arenas<T> ar, const double* rhs, T* y, T* upd,
+                            int Bp, bool transpose, hipStream_t s);
 template <typename T>
 void lu_solve_backward_group(const LuDevice& d, int tier, const int* fronts, int count,
-                             int max_fsz, Arenas<T> ar, T* y, int Bp, bool transpose,
+                             int max_fsz, int max_nc, Arenas<T> ar, T* y, int Bp, bool transpose,
                              hipStream_t s);
 template <typename T, typename TX>
 void lu_solve_scatter(const LuDevice& d, const T* y, int Bp, TX* x, hipStream_t s);

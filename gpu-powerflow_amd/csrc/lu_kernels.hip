@@ -234,6 +234,82 @@ __device__ inline void assemble_tasks(const LuDevice& d, const Arenas<T>& ar, in
     }
 }
 
+// Assembly of one Schur row block from its row items: the tasks of the list
+// are loaded into LDS once, then each wave walks the contribution rows that
+// land in the block rows it owns (rr % 4 == wave), in task order, two rows
+// in flight, the row's columns spread over the lanes:
+//   F[map(rr, loc[bb])] += CB(a, bb).
+// Waves own disjoint rows and one instruction touches one row of one child,
+// so no barrier is needed between children.
+constexpr int kItemTasks = 128;
+constexpr int kItemWaves = 4;
+
+template <int NT, typename T, typename Map>
+__device__ inline void assemble_items(const LuDevice& d, const Arenas<T>& ar, int list, int b,
+                                      int Bp, T* __restrict__ F, ChildSeg<T>* seg, Map map) {
+    static_assert(NT == 64 * kItemWaves, "one owner wave per row residue");
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int t_begin = d.asm_tptr[list];
+    const int ntask = d.asm_tptr[list + 1] - t_begin;
+    for (int t = tid; t < ntask; t += NT) {
+        const AsmTaskDev tk = d.asm_tasks[t_begin + t];
+        ChildSeg<T> g;
+        if (tk.tier == 0) {
+            g.C = ar.a0 + tk.off * Bp + b;
+            g.cs = Bp;
+        } else {
+            g.C = ar.a1 + (long long)b * d.arena1 + tk.off;
+            g.cs = 1;
+        }
+        g.loc = d.asm_loc + tk.loc_ptr;
+        g.rowoff = tk.cnc * tk.cfs + tk.cnc;
+        g.cfs = tk.cfs;
+        g.a0 = tk.a0;
+        g.b0 = tk.b0;
+        g.nb = tk.nb;
+        seg[t] = g;
+    }
+    __syncthreads();
+    const int i_begin = d.asm_iptr[list * kItemWaves + wave];
+    const int i_end = d.asm_iptr[list * kItemWaves + wave + 1];
+    for (int ib = i_begin; ib < i_end; ib += 64) {
+        const int n = min(64, i_end - ib);
+        const int mine = lane < n ? d.asm_items[ib + lane] : 0;
+        for (int j = 0; j < n; j += 2) {
+            const bool two = j + 1 < n;            // uniform
+            const int it0 = __builtin_amdgcn_readlane(mine, j);
+            const int it1 = two ? __builtin_amdgcn_readlane(mine, j + 1) : it0;
+            const ChildSeg<T> g0 = seg[it0 & 127];
+            const ChildSeg<T> g1 = seg[it1 & 127];
+            const int rr0 = (it0 >> 7) & 63, rr1 = (it1 >> 7) & 63;
+            const T* c0 = g0.C + (long long)(g0.rowoff + (it0 >> 13) * g0.cfs + g0.b0) * g0.cs;
+            const T* c1 = g1.C + (long long)(g1.rowoff + (it1 >> 13) * g1.cfs + g1.b0) * g1.cs;
+            const int* l0 = g0.loc + g0.b0;
+            const int* l1 = g1.loc + g1.b0;
+            const int nb0 = g0.nb, nb1 = two ? g1.nb : 0;
+            for (int cb = 0; cb < max(nb0, nb1); cb += 64 * 8) {
+                T v0[8], v1[8];
+                int k0[8], k1[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int bb = cb + lane + 64 * u;
+                    k0[u] = bb < nb0 ? l0[bb] : -1;
+                    v0[u] = bb < nb0 ? c0[(long long)bb * g0.cs] : T(0);
+                    k1[u] = bb < nb1 ? l1[bb] : -1;
+                    v1[u] = bb < nb1 ? c1[(long long)bb * g1.cs] : T(0);
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    if (k0[u] >= 0) F[map(rr0, k0[u])] += v0[u];
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    if (k1[u] >= 0) F[map(rr1, k1[u])] += v1[u];
+            }
+        }
+    }
+    __syncthreads();
+}
+
 // ===========================================================================
 // Factor, tier 0 (lane per scenario)
 // ===========================================================================
@@ -1113,14 +1189,11 @@ __global__ __launch_bounds__(kSchurThreads, 2) void k_schur(LuDevice d, const in
         return c < K ? rr * kSLd + c : kSchurRows * kSLd + rr * uc + c - K;
     });
     __syncthreads();
-    __shared__ ChildSeg<float> seg[kMaxSeg];
-    __shared__ int pref[kMaxSeg + 1];
-    __shared__ int stg[kMaxStages + 1];
-    assemble_tasks<kSchurThreads, 16>(d, ar, d.asm_base[p * kAsmKindCount + kAsmKSchur] + rb, b, Bp, Lb,
-                                      seg, pref, stg, [=](int r, int c) {
-                                             return c < K ? (r - i0) * kSLd + c
-                                                          : kSchurRows * kSLd + (r - i0) * uc + c - K;
-                                         });
+    __shared__ ChildSeg<float> seg[kItemTasks];
+    assemble_items<kSchurThreads>(d, ar, d.asm_base[p * kAsmKindCount + kAsmKSchur] + rb, b, Bp, Lb, seg,
+                                  [=](int rr, int c) {
+                                      return c < K ? rr * kSLd + c : kSchurRows * kSLd + rr * uc + c - K;
+                                  });
 
     // the first tile's U12 operands load while L21 is formed
     if (wave < ntile) load_u(wave);
@@ -1458,6 +1531,181 @@ __global__ __launch_bounds__(kLanes) void k_bwd_t12(LuDevice d, const int* __res
     for (int k = lane; k < ncp; k += kLanes) y[(long long)fr[k] * Bp + b] = w[k];
 }
 
+// ===========================================================================
+// Solve, tiers 1-2 with at most 64 pivots: NT lanes per (front, scenario).
+//
+// Per front with w = the front's slice of the work vector (LDS):
+//   forward   w[0:K] = L11^-1 w[0:K],   w[K:fs] -= L21 w[0:K]
+//   (transposed: U11^-T and U12^T);
+//   backward  w[0:K] -= U12 w[K:fs],   w[0:K] = U11^-1 w[0:K]
+//   (transposed: L21^T and L11^-T).
+// The pivot block is staged in LDS and solved by wave 0 (one row per lane,
+// values broadcast by v_readlane); the rectangular products read rows of the
+// front with one output per lane (L21 w: a row per lane; U12 w and L21^T w:
+// a pivot row per lane, the columns split over the NT / 64 waves and the
+// partial sums combined in LDS; U12^T w: lanes along the outputs).
+// Children's update vectors are gathered one assembly stage at a time
+// (children of a stage have disjoint rows).  Used for FP32 fronts; FP64
+// fronts keep the one-wave sweeps below, which measure faster there.
+// ===========================================================================
+constexpr int kSolveK = 64;
+constexpr int kSolvePld = kSolveK + 1;          // pivot-block row stride in LDS
+
+template <typename T>
+__device__ __forceinline__ T bcast(T x, int lane) {
+    if constexpr (sizeof(T) == 4) {
+        return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), lane));
+    } else {
+        const long long v = __double_as_longlong(x);
+        const int lo = __builtin_amdgcn_readlane(int(v & 0xffffffffLL), lane);
+        const int hi = __builtin_amdgcn_readlane(int(v >> 32), lane);
+        return __longlong_as_double((long long)(unsigned(lo)) | ((long long)hi << 32));
+    }
+}
+
+// Pivot block of G (K x K, row stride fs) into LDS P[K][kSolvePld].
+template <int NT, typename T>
+__device__ inline void stage_pivot(const T* __restrict__ G, int fs, int K, T* P) {
+    for (int e = threadIdx.x; e < K * kSolveK; e += NT) {
+        const int r = e / kSolveK, c = e - (e / kSolveK) * kSolveK;
+        if (c < K) P[r * kSolvePld + c] = G[(long long)r * fs + c];
+    }
+}
+
+// Gather of the children's update vectors into w, one stage at a time.
+template <int NT, typename T>
+__device__ inline void gather_children(const LuDevice& d, int p, int b, int Bp,
+                                       const T* __restrict__ upd, T* w) {
+    const int* st = d.cstage + d.cstage_ptr[p];
+    const int ns = d.cstage_ptr[p + 1] - d.cstage_ptr[p] - 1;
+    for (int sg = 0; sg < ns; ++sg) {
+        for (int q = st[sg]; q < st[sg + 1]; ++q) {
+            const int c = d.child_idx[q];
+            const int cuc = d.fsz[c] - d.nc[c];
+            const int* loc = d.asm_loc + d.asm_ptr[c];
+            const T* u = upd + d.uoff[c] * Bp + b;
+            for (int a = threadIdx.x; a < cuc; a += NT) w[loc[a]] += u[(long long)a * Bp];
+        }
+        __syncthreads();
+    }
+}
+
+template <typename T, bool TRANS, int NT>
+__global__ __launch_bounds__(NT) void k_fwd_w(LuDevice d, const int* __restrict__ fronts,
+                                              Arenas<T> ar, const double* __restrict__ rhs,
+                                              T* __restrict__ y, T* __restrict__ upd, int Bp) {
+    extern __shared__ unsigned char smem_raw[];
+    T* P = reinterpret_cast<T*>(smem_raw);                 // [64][kSolvePld]
+    T* w = P + kSolveK * kSolvePld;                        // [fs]
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    constexpr int NW = NT / 64;
+    int fi, b;
+    decode_block(Bp, fi, b);
+    const int p = fronts[fi];
+    const int fs = d.fsz[p], K = d.nc[p], uc = fs - K;
+    const int* fr = d.front_rows + d.front_ptr[p];
+    const T* G = ar.a1 + (long long)b * d.arena1 + d.off[p];
+    for (int l = tid; l < fs; l += NT)
+        w[l] = (l < K) ? T(rhs[(long long)d.perm[fr[l]] * Bp + b]) : T(0);
+    stage_pivot<NT>(G, fs, K, P);
+    __syncthreads();
+    gather_children<NT>(d, p, b, Bp, upd, w);
+    if (wave == 0) {
+        T wi = lane < K ? w[lane] : T(0);
+        for (int k = 0; k < K; ++k) {
+            if (TRANS) {                                   // U11^T: lower, non-unit
+                if (lane == k) wi /= P[k * kSolvePld + k];
+                const T yk = bcast(wi, k);
+                if (lane > k && lane < K) wi -= P[k * kSolvePld + lane] * yk;
+            } else {                                       // L11: unit lower
+                const T yk = bcast(wi, k);
+                if (lane > k && lane < K) wi -= P[lane * kSolvePld + k] * yk;
+            }
+        }
+        if (lane < K) w[lane] = wi;
+    }
+    __syncthreads();
+    if (TRANS) {
+        // w[K + j] -= sum_k U12[k][j] w[k]: lanes along j
+        for (int j = tid; j < uc; j += NT) {
+            T acc = T(0);
+            for (int k = 0; k < K; ++k) acc += G[(long long)k * fs + K + j] * w[k];
+            w[K + j] -= acc;
+        }
+    } else {
+        // w[r] -= L21[r] . w[0:K]: one row per lane
+        for (int r = K + tid; r < fs; r += NT) {
+            const T* row = G + (long long)r * fs;
+            T acc = T(0);
+#pragma unroll 8
+            for (int k = 0; k < K; ++k) acc += row[k] * w[k];
+            w[r] -= acc;
+        }
+    }
+    __syncthreads();
+    for (int k = tid; k < K; k += NT) y[(long long)fr[k] * Bp + b] = w[k];
+    T* u = upd + d.uoff[p] * Bp + b;
+    for (int a = tid; a < uc; a += NT) u[(long long)a * Bp] = w[K + a];
+}
+
+template <typename T, bool TRANS, int NT>
+__global__ __launch_bounds__(NT) void k_bwd_w(LuDevice d, const int* __restrict__ fronts,
+                                              Arenas<T> ar, T* __restrict__ y, int Bp) {
+    extern __shared__ unsigned char smem_raw[];
+    T* P = reinterpret_cast<T*>(smem_raw);                 // [64][kSolvePld]
+    T* part = P + kSolveK * kSolvePld;                     // [NW][64]
+    T* w = part + (NT / 64) * kSolveK;                     // [fs]
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    constexpr int NW = NT / 64;
+    int fi, b;
+    decode_block(Bp, fi, b);
+    const int p = fronts[fi];
+    const int fs = d.fsz[p], K = d.nc[p], uc = fs - K;
+    const int* fr = d.front_rows + d.front_ptr[p];
+    const T* G = ar.a1 + (long long)b * d.arena1 + d.off[p];
+    for (int l = tid; l < fs; l += NT) w[l] = y[(long long)fr[l] * Bp + b];
+    stage_pivot<NT>(G, fs, K, P);
+    __syncthreads();
+    if (TRANS) {
+        // w[k] -= sum_r L21[r][k] w[r]: lanes along k, rows split over waves
+        T acc = T(0);
+        if (lane < K)
+            for (int r = K + wave; r < fs; r += NW) acc += G[(long long)r * fs + lane] * w[r];
+        part[wave * kSolveK + lane] = acc;
+    } else {
+        // w[k] -= U12[k] . w[K:fs]: one pivot row per lane, columns split over waves
+        T acc = T(0);
+        if (lane < K) {
+            const T* row = G + (long long)lane * fs + K;
+#pragma unroll 8
+            for (int j = wave; j < uc; j += NW) acc += row[j] * w[K + j];
+        }
+        part[wave * kSolveK + lane] = acc;
+    }
+    __syncthreads();
+    if (wave == 0) {
+        T wi = lane < K ? w[lane] : T(0);
+#pragma unroll
+        for (int q = 0; q < NW; ++q) wi -= part[q * kSolveK + lane];
+        for (int k = K - 1; k >= 0; --k) {
+            if (TRANS) {                                   // L11^T: unit upper
+                const T xk = bcast(wi, k);
+                if (lane < k) wi -= P[k * kSolvePld + lane] * xk;
+            } else {                                       // U11: upper, non-unit
+                if (lane == k) wi /= P[k * kSolvePld + k];
+                const T xk = bcast(wi, k);
+                if (lane < k) wi -= P[lane * kSolvePld + k] * xk;
+            }
+        }
+        if (lane < K) y[(long long)fr[lane] * Bp + b] = wi;
+    }
+}
+
+template <int NT, typename T>
+inline size_t solve_w_lds(int max_fsz) {
+    return (size_t(kSolveK) * kSolvePld + size_t(NT / 64) * kSolveK + size_t(max_fsz)) * sizeof(T);
+}
+
 template <typename T, typename TX>
 __global__ __launch_bounds__(256) void k_scatter(const int* __restrict__ perm, int n,
                                                  const T* __restrict__ y, int Bp,
@@ -1584,9 +1832,10 @@ void lu_factor_group(const LuDevice& d, int tier, const int* fronts, int count, 
 
 template <typename T>
 void lu_solve_forward_group(const LuDevice& d, int tier, const int* fronts, int count,
-                            int max_fsz, Arenas<T> ar, const double* rhs, T* y, T* upd, int Bp,
-                            bool transpose, hipStream_t s) {
+                            int max_fsz, int max_nc, Arenas<T> ar, const double* rhs, T* y, T* upd,
+                            int Bp, bool transpose, hipStream_t s) {
     if (count <= 0) return;
+    const dim3 grid(unsigned(count) * unsigned(Bp));
     if (tier == 0) {
         if (transpose)
             hipLaunchKernelGGL((k_fwd_t0<T, true>), dim3(count, Bp / kLanes), dim3(kLanes), 0, s, d,
@@ -1594,9 +1843,26 @@ void lu_solve_forward_group(const LuDevice& d, int tier, const int* fronts, int 
         else
             hipLaunchKernelGGL((k_fwd_t0<T, false>), dim3(count, Bp / kLanes), dim3(kLanes), 0, s,
                                d, fronts, ar, rhs, y, upd, Bp);
+    } else if (sizeof(T) == 4 && max_nc <= kSolveK) {
+        if (tier == 2) {
+            const size_t shmem = solve_w_lds<256, T>(max_fsz);
+            if (transpose)
+                hipLaunchKernelGGL((k_fwd_w<T, true, 256>), grid, dim3(256), shmem, s, d, fronts, ar,
+                                   rhs, y, upd, Bp);
+            else
+                hipLaunchKernelGGL((k_fwd_w<T, false, 256>), grid, dim3(256), shmem, s, d, fronts, ar,
+                                   rhs, y, upd, Bp);
+        } else {
+            const size_t shmem = solve_w_lds<64, T>(max_fsz);
+            if (transpose)
+                hipLaunchKernelGGL((k_fwd_w<T, true, 64>), grid, dim3(64), shmem, s, d, fronts, ar,
+                                   rhs, y, upd, Bp);
+            else
+                hipLaunchKernelGGL((k_fwd_w<T, false, 64>), grid, dim3(64), shmem, s, d, fronts, ar,
+                                   rhs, y, upd, Bp);
+        }
     } else {
         const size_t shmem = size_t(max_fsz) * sizeof(T);
-        const dim3 grid(unsigned(count) * unsigned(Bp));
         if (transpose)
             hipLaunchKernelGGL((k_fwd_t12<T, true>), grid, dim3(kLanes), shmem, s, d, fronts, ar,
                                rhs, y, upd, Bp);
@@ -1609,9 +1875,10 @@ void lu_solve_forward_group(const LuDevice& d, int tier, const int* fronts, int 
 
 template <typename T>
 void lu_solve_backward_group(const LuDevice& d, int tier, const int* fronts, int count,
-                             int max_fsz, Arenas<T> ar, T* y, int Bp, bool transpose,
+                             int max_fsz, int max_nc, Arenas<T> ar, T* y, int Bp, bool transpose,
                              hipStream_t s) {
     if (count <= 0) return;
+    const dim3 grid(unsigned(count) * unsigned(Bp));
     if (tier == 0) {
         if (transpose)
             hipLaunchKernelGGL((k_bwd_t0<T, true>), dim3(count, Bp / kLanes), dim3(kLanes), 0, s, d,
@@ -1619,9 +1886,25 @@ void lu_solve_backward_group(const LuDevice& d, int tier, const int* fronts, int
         else
             hipLaunchKernelGGL((k_bwd_t0<T, false>), dim3(count, Bp / kLanes), dim3(kLanes), 0, s,
                                d, fronts, ar, y, Bp);
+    } else if (sizeof(T) == 4 && max_nc <= kSolveK) {
+        if (tier == 2) {
+            const size_t shmem = solve_w_lds<256, T>(max_fsz);
+            if (transpose)
+                hipLaunchKernelGGL((k_bwd_w<T, true, 256>), grid, dim3(256), shmem, s, d, fronts, ar, y,
+                                   Bp);
+            else
+                hipLaunchKernelGGL((k_bwd_w<T, false, 256>), grid, dim3(256), shmem, s, d, fronts, ar, y,
+                                   Bp);
+        } else {
+            const size_t shmem = solve_w_lds<64, T>(max_fsz);
+            if (transpose)
+                hipLaunchKernelGGL((k_bwd_w<T, true, 64>), grid, dim3(64), shmem, s, d, fronts, ar, y, Bp);
+            else
+                hipLaunchKernelGGL((k_bwd_w<T, false, 64>), grid, dim3(64), shmem, s, d, fronts, ar, y,
+                                   Bp);
+        }
     } else {
         const size_t shmem = size_t(max_fsz) * sizeof(T);
-        const dim3 grid(unsigned(count) * unsigned(Bp));
         if (transpose)
             hipLaunchKernelGGL((k_bwd_t12<T, true>), grid, dim3(kLanes), shmem, s, d, fronts, ar, y,
                                Bp);
@@ -1642,10 +1925,10 @@ void lu_solve_scatter(const LuDevice& d, const T* y, int Bp, TX* x, hipStream_t 
 #define CUPF_LU_INST(T)                                                                          \
     template void lu_factor_group<T>(const LuDevice&, int, const int*, int, int, const double*,  \
                                      int, Arenas<T>, double, int, int*, int, hipStream_t);       \
-    template void lu_solve_forward_group<T>(const LuDevice&, int, const int*, int, int,           \
+    template void lu_solve_forward_group<T>(const LuDevice&, int, const int*, int, int, int,      \
                                             Arenas<T>, const double*, T*, T*, int, bool,         \
                                             hipStream_t);                                         \
-    template void lu_solve_backward_group<T>(const LuDevice&, int, const int*, int, int,          \
+    template void lu_solve_backward_group<T>(const LuDevice&, int, const int*, int, int, int,     \
                                              Arenas<T>, T*, int, bool, hipStream_t);             \
     template void lu_solve_scatter<T, double>(const LuDevice&, const T*, int, double*,            \
                                               hipStream_t);                                       \

@@ -119,7 +119,7 @@ struct NewtonSolver::Impl {
         d_row_bus, d_row_isq, d_jrp, d_jci, d_jtrp, d_jtci, d_jtmap;
     DeviceBuffer d_fsz, d_nc, d_tier, d_off, d_uoff, d_fptr, d_frows, d_cptr, d_cidx, d_aptr,
         d_aloc, d_eptr, d_esrc, d_edst, d_perm, d_groups, d_lbptr, d_rowlb, d_abase, d_atptr,
-        d_asptr, d_astage, d_atasks;
+        d_asptr, d_astage, d_atasks, d_cstptr, d_cstage, d_aiptr, d_aitems;
     kern::NrDevice nr;
     kern::LuDevice lu;
 
@@ -244,13 +244,15 @@ struct NewtonSolver::Impl {
             for (int t = 0; t < 3; ++t) {
                 int cnt, mf;
                 const int* f = group(L, t, cnt, mf);
-                kern::lu_solve_forward_group<T>(lu, t, f, cnt, mf, ar, rhs, y, u, Bp, transpose, s);
+                kern::lu_solve_forward_group<T>(lu, t, f, cnt, mf, plan.group_max_nc[L * 3 + t], ar,
+                                                rhs, y, u, Bp, transpose, s);
             }
         for (int L = plan.num_levels - 1; L >= 0; --L)
             for (int t = 0; t < 3; ++t) {
                 int cnt, mf;
                 const int* f = group(L, t, cnt, mf);
-                kern::lu_solve_backward_group<T>(lu, t, f, cnt, mf, ar, y, Bp, transpose, s);
+                kern::lu_solve_backward_group<T>(lu, t, f, cnt, mf, plan.group_max_nc[L * 3 + t], ar,
+                                                 y, Bp, transpose, s);
             }
         kern::lu_solve_scatter<T, TX>(lu, y, Bp, x, s);
     }
@@ -707,6 +709,8 @@ void NewtonSolver::initialize(const YbusView& ybus, const int32_t* pv, int32_t n
     m.d_groups = upload_vector(P.group_fronts, s);
     m.d_lbptr = upload_vector(P.lb_ptr, s);
     m.d_rowlb = upload_vector(P.row_lb, s);
+    m.d_cstptr = upload_vector(P.stage_ptr, s);
+    m.d_cstage = upload_vector(P.stage, s);
     {
         // assembly task lists for the kernels each front's group will run
         static_assert(sizeof(AsmTask) == sizeof(kern::AsmTaskDev), "assembly task layout");
@@ -733,6 +737,8 @@ void NewtonSolver::initialize(const YbusView& ybus, const int32_t* pv, int32_t n
         m.d_asptr = upload_vector(al.stage_ptr, s);
         m.d_astage = upload_vector(al.stage, s);
         m.d_atasks = upload_vector(al.tasks, s);
+        m.d_aiptr = upload_vector(al.item_ptr, s);
+        m.d_aitems = upload_vector(al.items, s);
     }
     CUPF_HIP(hipStreamSynchronize(s));
 
@@ -756,9 +762,11 @@ void NewtonSolver::initialize(const YbusView& ybus, const int32_t* pv, int32_t n
     lu.ent_ptr = m.d_eptr.as<int>(); lu.ent_src = m.d_esrc.as<int>();
     lu.ent_dst = m.d_edst.as<int>(); lu.perm = m.d_perm.as<int>();
     lu.lb_ptr = m.d_lbptr.as<int>(); lu.row_lb = m.d_rowlb.as<int>();
+    lu.cstage_ptr = m.d_cstptr.as<int>(); lu.cstage = m.d_cstage.as<int>();
     lu.asm_base = m.d_abase.as<int>(); lu.asm_tptr = m.d_atptr.as<int>();
     lu.asm_sptr = m.d_asptr.as<int>(); lu.asm_stage = m.d_astage.as<int>();
     lu.asm_tasks = m.d_atasks.as<kern::AsmTaskDev>();
+    lu.asm_iptr = m.d_aiptr.as<int>(); lu.asm_items = m.d_aitems.as<int>();
 
     m.capBp = 0;   // batch buffers are sized lazily for the new structure
     m.capB = 0;
