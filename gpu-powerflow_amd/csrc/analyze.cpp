@@ -686,21 +686,21 @@ AsmLists build_asm_lists(const LuPlan& L, const std::vector<int>& kinds,
                 }
                 if (st.size() > 128)
                     throw std::runtime_error("build_asm_lists: more than 128 assembly stages in a block");
-                // row items (Schur blocks): per owner wave, in task order
+                // row items: per owner wave, in task order
                 const int tb = A.task_ptr.back(), te = int(A.tasks.size());
+                if (te - tb > kAsmMaxItemTasks)
+                    throw std::runtime_error("build_asm_lists: too many tasks in an assembly block");
+                if (r1 - r0 > (1 << 10))
+                    throw std::runtime_error("build_asm_lists: assembly block too tall");
                 for (int wv = 0; wv < kAsmItemWaves; ++wv) {
-                    if (kind == 0) {
-                        if (te - tb > kAsmMaxItemTasks)
-                            throw std::runtime_error("build_asm_lists: too many tasks in a Schur block");
-                        for (int t = tb; t < te; ++t) {
-                            const AsmTask& tk = A.tasks[size_t(t)];
-                            const int* loc = L.asm_loc.data() + tk.loc_ptr;
-                            for (int a = tk.a0; a < tk.a0 + tk.na; ++a) {
-                                const int rr = loc[a] - r0;
-                                if (rr % kAsmItemWaves != wv) continue;
-                                if (a >= (1 << 19)) throw std::runtime_error("build_asm_lists: row index too large");
-                                A.items.push_back((t - tb) | (rr << 7) | (a << 13));
-                            }
+                    for (int t = tb; t < te; ++t) {
+                        const AsmTask& tk = A.tasks[size_t(t)];
+                        const int* loc = L.asm_loc.data() + tk.loc_ptr;
+                        for (int a = tk.a0; a < tk.a0 + tk.na; ++a) {
+                            const int rr = loc[a] - r0;
+                            if (rr % kAsmItemWaves != wv) continue;
+                            if (a >= (1 << 15)) throw std::runtime_error("build_asm_lists: row index too large");
+                            A.items.push_back((t - tb) | (rr << 7) | (a << 17));
                         }
                     }
                     A.item_ptr.push_back(int(A.items.size()));

@@ -156,11 +156,11 @@ struct AsmTask {                     // 48 bytes, device-visible layout
     int pad0, pad1;
 };
 
-// Row items of the Schur lists (kind 0): the block's rows are owned by
-// kAsmItemWaves waves (row rr by wave rr % kAsmItemWaves); every landing
+// Row items of every list: the block's rows are owned by kAsmItemWaves
+// waves (block row rr by wave rr % kAsmItemWaves); every landing
 // contribution row is one item of its owner, packed as
-// task (bits 0-6, index inside the list) | block row rr (bits 7-12) |
-// contribution row a (bits 13-31).  A wave's items are in task order, so
+// task (bits 0-6, index inside the list) | block row rr (bits 7-16) |
+// contribution row a (bits 17-31).  A wave's items are in task order, so
 // each front element receives its contributions in a fixed order.
 constexpr int kAsmItemWaves = 4;
 constexpr int kAsmMaxItemTasks = 128;

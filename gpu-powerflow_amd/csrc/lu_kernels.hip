@@ -234,7 +234,7 @@ __device__ inline void assemble_tasks(const LuDevice& d, const Arenas<T>& ar, in
     }
 }
 
-// Assembly of one Schur row block from its row items: the tasks of the list
+// Assembly of one block of a front from its row items: the tasks of the list
 // are loaded into LDS once, then each wave walks the contribution rows that
 // land in the block rows it owns (rr % 4 == wave), in task order, two rows
 // in flight, the row's columns spread over the lanes:
@@ -281,9 +281,9 @@ __device__ inline void assemble_items(const LuDevice& d, const Arenas<T>& ar, in
             const int it1 = two ? __builtin_amdgcn_readlane(mine, j + 1) : it0;
             const ChildSeg<T> g0 = seg[it0 & 127];
             const ChildSeg<T> g1 = seg[it1 & 127];
-            const int rr0 = (it0 >> 7) & 63, rr1 = (it1 >> 7) & 63;
-            const T* c0 = g0.C + (long long)(g0.rowoff + (it0 >> 13) * g0.cfs + g0.b0) * g0.cs;
-            const T* c1 = g1.C + (long long)(g1.rowoff + (it1 >> 13) * g1.cfs + g1.b0) * g1.cs;
+            const int rr0 = (it0 >> 7) & 1023, rr1 = (it1 >> 7) & 1023;
+            const T* c0 = g0.C + (long long)(g0.rowoff + (it0 >> 17) * g0.cfs + g0.b0) * g0.cs;
+            const T* c1 = g1.C + (long long)(g1.rowoff + (it1 >> 17) * g1.cfs + g1.b0) * g1.cs;
             const int* l0 = g0.loc + g0.b0;
             const int* l1 = g1.loc + g1.b0;
             const int nb0 = g0.nb, nb1 = two ? g1.nb : 0;
@@ -621,11 +621,9 @@ __global__ __launch_bounds__(kAsmThreads) void k_assemble_rows(LuDevice d,
     put_entries<kAsmThreads>(F, d.ent_dst, d.ent_src, A, Bp, b, lo, hi,
                              [=](int dst) { return dst - i0 * fs; });
     __syncthreads();
-    __shared__ ChildSeg<T> seg[kMaxSeg];
-    __shared__ int pref[kMaxSeg + 1];
-    __shared__ int stg[kMaxStages + 1];
-    assemble_tasks<kAsmThreads, 16>(d, ar, d.asm_base[p * kAsmKindCount + kAsmKRows] + rb, b, Bp, F,
-                                    seg, pref, stg, [=](int r, int c) { return (r - i0) * fs + c; });
+    __shared__ ChildSeg<T> seg[kItemTasks];
+    assemble_items<kAsmThreads>(d, ar, d.asm_base[p * kAsmKindCount + kAsmKRows] + rb, b, Bp, F, seg,
+                                [=](int rr, int c) { return rr * fs + c; });
     T* G = ar.a1 + (long long)b * d.arena1 + d.off[p] + (long long)i0 * fs;
     for (int e = tid; e < nr * fs; e += kAsmThreads) G[e] = F[e];
 }
