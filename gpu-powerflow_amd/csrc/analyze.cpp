@@ -599,7 +599,7 @@ LuPlan lower_plan(const Symbolic& s, int n, const int* rp, const int* ci, int ti
 // Assembly task lists (see analyze.hpp).
 // ===========================================================================
 AsmLists build_asm_lists(const LuPlan& L, const std::vector<int>& kinds,
-                         const std::vector<int>& rows_per_block) {
+                         const std::vector<int>& rows_per_block, int schur_rows) {
     const int P = L.num_fronts;
     AsmLists A;
     A.base.assign(size_t(P) * kAsmKinds, -1);
@@ -628,7 +628,7 @@ AsmLists build_asm_lists(const LuPlan& L, const std::vector<int>& kinds,
             if (!(kinds[size_t(p)] >> kind & 1)) continue;
             int nblk = 0;
             switch (kind) {
-                case 0: nblk = (uc + kAsmSchurRows - 1) / kAsmSchurRows; break;
+                case 0: nblk = (uc + schur_rows - 1) / schur_rows; break;
                 case 1: nblk = 1; break;
                 case 2: nblk = (uc + kAsmChunkCols - 1) / kAsmChunkCols; break;
                 case 3: nblk = (fs + rows_per_block[size_t(p)] - 1) / rows_per_block[size_t(p)]; break;
@@ -638,7 +638,7 @@ AsmLists build_asm_lists(const LuPlan& L, const std::vector<int>& kinds,
             for (int blk = 0; blk < nblk; ++blk) {
                 int r0 = 0, r1 = fs, c0 = 0, c1 = fs;
                 switch (kind) {
-                    case 0: r0 = K + blk * kAsmSchurRows; r1 = std::min(fs, r0 + kAsmSchurRows); break;
+                    case 0: r0 = K + blk * schur_rows; r1 = std::min(fs, r0 + schur_rows); break;
                     case 1: r1 = K; c1 = K; break;
                     case 2: r1 = K; c0 = K + blk * kAsmChunkCols; c1 = std::min(fs, c0 + kAsmChunkCols); break;
                     case 3: r0 = blk * rows_per_block[size_t(p)];

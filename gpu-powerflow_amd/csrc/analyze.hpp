@@ -138,7 +138,8 @@ LuPlan lower_plan(const Symbolic& s, int n, const int* rp, const int* ci, int ti
 // of one stage never add to the same front element (checked exactly, row and
 // column sets), so a stage is walked as one flat sequence; stages follow the
 // child order.
-//   kinds: 0 Schur row blocks (32 rows from the pivot count on, all columns),
+//   kinds: 0 Schur row blocks (schur_rows rows from the pivot count on, all
+//            columns),
 //          1 pivot block (rows and columns below the pivot count),
 //          2 pivot-row chunks (rows below the pivot count, 64 columns each
 //            from the pivot count on),
@@ -146,7 +147,6 @@ LuPlan lower_plan(const Symbolic& s, int n, const int* rp, const int* ci, int ti
 //          4 whole front.
 // ---------------------------------------------------------------------------
 constexpr int kAsmKinds = 5;
-constexpr int kAsmSchurRows = 32;
 constexpr int kAsmChunkCols = 64;
 
 struct AsmTask {                     // 48 bytes, device-visible layout
@@ -176,8 +176,8 @@ struct AsmLists {
 };
 
 // kinds[p] is a bit mask of the kinds built for front p; rows_per_block[p]
-// is the block height of kind 3.
+// is the block height of kind 3, schur_rows the block height of kind 0.
 AsmLists build_asm_lists(const LuPlan& L, const std::vector<int>& kinds,
-                         const std::vector<int>& rows_per_block);
+                         const std::vector<int>& rows_per_block, int schur_rows);
 
 }  // namespace cupf

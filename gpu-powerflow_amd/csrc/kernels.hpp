@@ -147,19 +147,23 @@ void lu_factor_group(const LuDevice& d, int tier, const int* fronts, int count, 
                      const double* A, int Bp, Arenas<T> ar, double pivot_eps, int static_piv,
                      int* sing, int dense_mode, hipStream_t s);
 
-// FP32 tier-2 groups whose fronts have at most 64 pivots and 512 rows
+// Tier-2 groups whose fronts have at most 64 pivots and 512 rows
 // (cross_schur_path) factor in two launches instead: cross_group (pivot
 // block L11\U11, U12, and U11^-1 into the side buffer uinv of
-// pivot_side_elems(count, Bp) floats) then schur_group (L21 and the
-// contribution block C = A22 - L21 U12, assembled and updated in one pass).
-// dense_mode 1: FP32 MFMA, 2: 3xBF16 MFMA.
+// pivot_side_elems(count, Bp) elements) then schur_group (L21 and the
+// contribution block C = A22 - L21 U12, assembled and updated in one pass,
+// schur_block_rows(sizeof(T)) rows per workgroup).  FP32 dense_mode 1: FP32
+// MFMA, 2: 3xBF16 MFMA; FP64 runs on the vector units.
 bool cross_schur_path(int max_fsz, int max_nc);
 size_t pivot_side_elems(int count, int Bp);
+int schur_block_rows(int elem);
+template <typename T>
 void cross_group(const LuDevice& d, const int* fronts, int count, const double* A, int Bp,
-                 Arenas<float> ar, float* uinv, double pivot_eps, int static_piv, int* sing,
+                 Arenas<T> ar, T* uinv, double pivot_eps, int static_piv, int* sing,
                  hipStream_t s);
+template <typename T>
 void schur_group(const LuDevice& d, const int* fronts, int count, int max_fsz, const double* A,
-                 int Bp, Arenas<float> ar, const float* uinv, int dense_mode, hipStream_t s);
+                 int Bp, Arenas<T> ar, const T* uinv, int dense_mode, hipStream_t s);
 
 // Solve sweeps.  rhs (FP64) and x are [n][Bp]; y [n][Bp] and upd
 // [upd_total][Bp] are work vectors.  gather: y <- P rhs is folded into the

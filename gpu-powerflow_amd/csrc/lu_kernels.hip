@@ -949,25 +949,29 @@ __global__ __launch_bounds__(kDenseThreads) void k_factor_t2_mfma(LuDevice d,
 constexpr int kPivThreads = 256;
 constexpr int kXMaxK = 64;
 constexpr int kXChunk = 64;                   // A12 columns per U12 step
-constexpr int kXCld = kXChunk + 4;            // chunk row stride (floats)
-constexpr int kUinvElems = kXMaxK * kXMaxK;   // side-buffer floats per (front, scenario)
-constexpr size_t kPivLds = size_t(3 * 64 * kLinvLd + 1024 + 2 * kXMaxK + 64 * kXCld) * 4;
+constexpr int kXCld = kXChunk + 4;            // chunk row stride (elements)
+constexpr int kUinvElems = kXMaxK * kXMaxK;   // side-buffer elements per (front, scenario)
+
+template <typename T>
+constexpr size_t pivot_lds() {
+    return size_t(3 * 64 * kLinvLd + 1024 + 2 * kXMaxK + 64 * kXCld) * sizeof(T);
+}
 
 // Li = T^-1 for the 64 x 64 lower triangular T given by t(i, k), k <= i
 // (callers make rows and columns past the pivot count the identity): the
 // four 16 x 16 diagonal blocks by forward substitution (one column per
 // lane), then two doubling steps [A 0; C D]^-1 = [A^-1 0; -D^-1 C A^-1  D^-1]
-// at h = 16, 32.  Tmp holds 1024 floats.  Called by all NT threads; ends
+// at h = 16, 32.  Tmp holds 1024 elements.  Called by all NT threads; ends
 // synchronised.
-template <int NT, typename Get>
-__device__ inline void lower_inverse(Get t, float* Li, float* Tmp) {
+template <int NT, typename T, typename Get>
+__device__ inline void lower_inverse(Get t, T* Li, T* Tmp) {
     const int tid = threadIdx.x;
-    for (int e = tid; e < 64 * 64; e += NT) Li[(e >> 6) * kLinvLd + (e & 63)] = 0.f;
+    for (int e = tid; e < 64 * 64; e += NT) Li[(e >> 6) * kLinvLd + (e & 63)] = T(0);
     __syncthreads();
     for (int i = 0; i < 16; ++i) {
         if (tid < 64) {
             const int o = (tid >> 4) * 16, j = tid & 15;
-            float acc = i == j ? 1.f : 0.f;
+            T acc = i == j ? T(1) : T(0);
             for (int k = j; k < i; ++k) acc -= t(o + i, o + k) * Li[(o + k) * kLinvLd + o + j];
             Li[(o + i) * kLinvLd + o + j] = acc / t(o + i, o + i);
         }
@@ -977,7 +981,7 @@ __device__ inline void lower_inverse(Get t, float* Li, float* Tmp) {
         const int total = 32 * h;              // (64 / 2h) pairs of h x h blocks
         for (int e = tid; e < total; e += NT) {
             const int q = e / (h * h), i = (e / h) % h, k = e % h, o = 2 * h * q;
-            float acc = 0.f;
+            T acc = T(0);
             for (int m = k; m < h; ++m)            // T = C A^-1 (A^-1 lower)
                 acc += t(o + h + i, o + m) * Li[(o + m) * kLinvLd + o + k];
             Tmp[e] = acc;
@@ -985,7 +989,7 @@ __device__ inline void lower_inverse(Get t, float* Li, float* Tmp) {
         __syncthreads();
         for (int e = tid; e < total; e += NT) {
             const int q = e / (h * h), i = (e / h) % h, k = e % h, o = 2 * h * q;
-            float acc = 0.f;
+            T acc = T(0);
             for (int m = 0; m <= i; ++m)           // -D^-1 T (D^-1 lower)
                 acc -= Li[(o + h + i) * kLinvLd + o + h + m] * Tmp[q * h * h + m * h + k];
             Li[(o + h + i) * kLinvLd + o + k] = acc;
@@ -994,35 +998,36 @@ __device__ inline void lower_inverse(Get t, float* Li, float* Tmp) {
     }
 }
 
+template <typename T>
 __global__ __launch_bounds__(kPivThreads) void k_pivot(LuDevice d, const int* __restrict__ fronts,
                                                         const double* __restrict__ A, int Bp,
-                                                        Arenas<float> ar, float* __restrict__ uinv,
+                                                        Arenas<T> ar, T* __restrict__ uinv,
                                                         double eps, int static_piv,
                                                         int* __restrict__ sing) {
     extern __shared__ unsigned char smem_raw[];
-    float* P = reinterpret_cast<float*>(smem_raw);                 // A11: [64][kLinvLd]
-    float* Li = P + 64 * kLinvLd;                                 // L11^-1
-    float* Wt = Li + 64 * kLinvLd;                                // (U11^T)^-1 = (U11^-1)^T
-    float* Tmp = Wt + 64 * kLinvLd;                               // [1024]
-    float* prow = Tmp + 1024;                                     // [2][64]
-    float* Ch = prow + 2 * kXMaxK;                                // [64][kXCld]
+    T* P = reinterpret_cast<T*>(smem_raw);                         // A11: [64][kLinvLd]
+    T* Li = P + 64 * kLinvLd;                                     // L11^-1
+    T* Wt = Li + 64 * kLinvLd;                                    // (U11^T)^-1 = (U11^-1)^T
+    T* Tmp = Wt + 64 * kLinvLd;                                   // [1024]
+    T* prow = Tmp + 1024;                                         // [2][64]
+    T* Ch = prow + 2 * kXMaxK;                                    // [64][kXCld]
     int fi, b;
     decode_block(Bp, fi, b);
     const int p = fronts[fi];
     const int fs = d.fsz[p], K = d.nc[p];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    float* G = ar.a1 + (long long)b * d.arena1 + d.off[p];
+    T* G = ar.a1 + (long long)b * d.arena1 + d.off[p];
     const int* rl = d.row_lb + d.lb_ptr[p];
 
     // ---- A11 -------------------------------------------------------------
-    for (int e = tid; e < 64 * kLinvLd; e += kPivThreads) P[e] = 0.f;
+    for (int e = tid; e < 64 * kLinvLd; e += kPivThreads) P[e] = T(0);
     __syncthreads();
     put_entries<kPivThreads>(P, d.ent_dst, d.ent_src, A, Bp, b, rl[0], rl[K], [=](int dst) {
         const int r = dst / fs, c = dst - r * fs;
         return c < K ? r * kLinvLd + c : -1;
     });
     __syncthreads();
-    __shared__ ChildSeg<float> seg[kMaxSeg];
+    __shared__ ChildSeg<T> seg[kMaxSeg];
     __shared__ int pref[kMaxSeg + 1];
     __shared__ int stg[kMaxStages + 1];
     const int* lbase = d.asm_base + p * kAsmKindCount;
@@ -1032,24 +1037,24 @@ __global__ __launch_bounds__(kPivThreads) void k_pivot(LuDevice d, const int* __
     // ---- LU of A11, one row per lane in registers ---------------------------
     int flag = 0;
     const int r = tid;
-    float row[kXMaxK];
+    T row[kXMaxK];
 #pragma unroll
-    for (int c = 0; c < kXMaxK; ++c) row[c] = r < K ? P[r * kLinvLd + c] : 0.f;
+    for (int c = 0; c < kXMaxK; ++c) row[c] = r < K ? P[r * kLinvLd + c] : T(0);
 #pragma unroll
     for (int c = 0; c < kXMaxK; ++c) {
-        float* pr = prow + (c & 1) * kXMaxK;
+        T* pr = prow + (c & 1) * kXMaxK;
         if (r == c && c < K) {
-            const float g = guard_pivot(row[c], eps, static_piv, flag);
+            const T g = guard_pivot(row[c], eps, static_piv, flag);
             row[c] = g;
 #pragma unroll
             for (int cc = c; cc < kXMaxK; ++cc) pr[cc] = row[cc];
         }
         __syncthreads();
         if (c < K) {                              // uniform
-            const float piv = pr[c];
+            const T piv = pr[c];
             const bool act = r > c && r < K;
-            const float l = row[c] / piv;
-            const float lm = act ? l : 0.f;
+            const T l = row[c] / piv;
+            const T lm = act ? l : T(0);
             if (act) row[c] = l;
 #pragma unroll
             for (int cc = c + 1; cc < kXMaxK; ++cc) row[cc] -= lm * pr[cc];
@@ -1064,14 +1069,14 @@ __global__ __launch_bounds__(kPivThreads) void k_pivot(LuDevice d, const int* __
     }
     __syncthreads();
     // L11^-1 (unit lower) and (U11^T)^-1; rows and columns >= K: identity
-    lower_inverse<kPivThreads>([=](int i, int k) {
-        return i == k ? 1.f : ((k < i && i < K) ? P[i * kLinvLd + k] : 0.f);
+    lower_inverse<kPivThreads, T>([=](int i, int k) {
+        return i == k ? T(1) : ((k < i && i < K) ? P[i * kLinvLd + k] : T(0));
     }, Li, Tmp);
-    lower_inverse<kPivThreads>([=](int i, int k) {
-        return (i < K && k <= i) ? P[k * kLinvLd + i] : (i == k ? 1.f : 0.f);
+    lower_inverse<kPivThreads, T>([=](int i, int k) {
+        return (i < K && k <= i) ? P[k * kLinvLd + i] : (i == k ? T(1) : T(0));
     }, Wt, Tmp);
     {
-        float* Ui = uinv + ((long long)fi * Bp + b) * kUinvElems;   // U11^-1 [k][j]
+        T* Ui = uinv + ((long long)fi * Bp + b) * kUinvElems;       // U11^-1 [k][j]
         for (int e = tid; e < kUinvElems; e += kPivThreads)
             Ui[e] = Wt[(e & 63) * kLinvLd + (e >> 6)];
     }
@@ -1079,7 +1084,7 @@ __global__ __launch_bounds__(kPivThreads) void k_pivot(LuDevice d, const int* __
     // ---- U12 = L11^-1 A12 in chunks of kXChunk columns ----------------------
     for (int j0 = K; j0 < fs; j0 += kXChunk) {
         const int jn = min(kXChunk, fs - j0);
-        for (int e = tid; e < 64 * kXCld; e += kPivThreads) Ch[e] = 0.f;
+        for (int e = tid; e < 64 * kXCld; e += kPivThreads) Ch[e] = T(0);
         __syncthreads();
         put_entries<kPivThreads>(Ch, d.ent_dst, d.ent_src, A, Bp, b, rl[0], rl[K], [=](int dst) {
             const int rr = dst / fs, c = dst - rr * fs;
@@ -1088,25 +1093,43 @@ __global__ __launch_bounds__(kPivThreads) void k_pivot(LuDevice d, const int* __
         __syncthreads();
         assemble_tasks<kPivThreads, 16>(d, ar, lbase[kAsmKChunk] + (j0 - K) / kXChunk, b, Bp, Ch, seg,
                                         pref, stg, [=](int r, int c) { return r * kXCld + c - j0; });
-        // 64 x 64 output = 2 x 2 tiles of 32 x 32, one per wave; L11^-1 is
-        // lower triangular, so the top row half needs k < 32 only
-        const int rh = wave & 1, ct = wave >> 1;
-        if (32 * ct < jn && 32 * rh < K) {      // wave-uniform
-            floatx16 acc;
+        if constexpr (sizeof(T) == 4) {
+            // 64 x 64 output = 2 x 2 tiles of 32 x 32, one per wave; L11^-1 is
+            // lower triangular, so the top row half needs k < 32 only
+            const int rh = wave & 1, ct = wave >> 1;
+            if (32 * ct < jn && 32 * rh < K) {  // wave-uniform
+                floatx16 acc;
 #pragma unroll
-            for (int v = 0; v < 16; ++v) acc[v] = 0.f;
-            const int kend = rh == 0 ? 32 : 64;
-            for (int kk = 0; kk < kend; kk += 2) {
-                const int k = kk + (lane >> 5);
-                const float av = Li[(32 * rh + (lane & 31)) * kLinvLd + k];
-                const float bv = Ch[k * kXCld + 32 * ct + (lane & 31)];
-                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv, acc, 0, 0, 0);
+                for (int v = 0; v < 16; ++v) acc[v] = 0.f;
+                const int kend = rh == 0 ? 32 : 64;
+                for (int kk = 0; kk < kend; kk += 2) {
+                    const int k = kk + (lane >> 5);
+                    const float av = Li[(32 * rh + (lane & 31)) * kLinvLd + k];
+                    const float bv = Ch[k * kXCld + 32 * ct + (lane & 31)];
+                    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv, acc, 0, 0, 0);
+                }
+                const int j = j0 + 32 * ct + (lane & 31);
+#pragma unroll
+                for (int v = 0; v < 16; ++v) {
+                    const int i = 32 * rh + (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
+                    if (i < K && j < fs) G[(long long)i * fs + j] = acc[v];
+                }
             }
-            const int j = j0 + 32 * ct + (lane & 31);
+        } else {
+            // FP64 on the vector units: row i = tid / 4, 16 columns per thread
+            const int i = tid >> 2, cb = (tid & 3) * 16;
+            if (i < K && cb < jn) {
+                T acc[16];
 #pragma unroll
-            for (int v = 0; v < 16; ++v) {
-                const int i = 32 * rh + (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
-                if (i < K && j < fs) G[(long long)i * fs + j] = acc[v];
+                for (int c = 0; c < 16; ++c) acc[c] = T(0);
+                for (int k = 0; k <= i; ++k) {  // L11^-1 lower
+                    const T l = Li[i * kLinvLd + k];
+#pragma unroll
+                    for (int c = 0; c < 16; ++c) acc[c] += l * Ch[k * kXCld + cb + c];
+                }
+#pragma unroll
+                for (int c = 0; c < 16; ++c)
+                    if (cb + c < jn) G[(long long)i * fs + j0 + cb + c] = acc[c];
             }
         }
         __syncthreads();                        // before the next chunk reuses Ch
@@ -1115,22 +1138,28 @@ __global__ __launch_bounds__(kPivThreads) void k_pivot(LuDevice d, const int* __
 }
 
 constexpr int kSchurThreads = 256;
-constexpr int kSchurRows = 32;
 constexpr int kSchurScen = 16;                 // scenarios interleaved inside an XCD range
-constexpr int kSLd = kXMaxK + 2;               // L21 block row stride (floats)
+constexpr int kSLd = kXMaxK + 2;               // L21 block row stride (elements)
+constexpr int kMfmaNone = 0;                   // FP64: vector-unit FMA tiles
 
+// rows per Schur block: 32 (FP32 MFMA tiles), 16 (FP64, LDS budget)
+template <typename T>
+constexpr int schur_rows() { return sizeof(T) == 4 ? 32 : 16; }
+
+template <typename T>
 inline size_t schur_lds(int max_fsz) {
-    return size_t(kSchurRows) * kSLd * 4 + size_t(kSchurRows) * max_fsz * 4;
+    return (size_t(schur_rows<T>()) * kSLd + size_t(schur_rows<T>()) * max_fsz) * sizeof(T);
 }
 
-template <int MODE>
+template <typename T, int MODE>
 __global__ __launch_bounds__(kSchurThreads, 2) void k_schur(LuDevice d, const int* __restrict__ fronts,
                                                          const double* __restrict__ A, int Bp,
-                                                         Arenas<float> ar,
-                                                         const float* __restrict__ uinv, int nrb) {
+                                                         Arenas<T> ar,
+                                                         const T* __restrict__ uinv, int nrb) {
+    constexpr int RB = schur_rows<T>();
     extern __shared__ unsigned char smem_raw[];
-    float* Lb = reinterpret_cast<float*>(smem_raw);               // [32][kSLd]: A21, then L21
-    float* Ab = Lb + kSchurRows * kSLd;                           // [32][uc]
+    T* Lb = reinterpret_cast<T*>(smem_raw);                       // [RB][kSLd]: A21, then L21
+    T* Ab = Lb + RB * kSLd;                                       // [RB][uc]
     // lin -> (front, scenario group, row block, scenario in group): inside an
     // XCD's contiguous range, kSchurScen neighbouring scenarios run together
     // (they share the cache lines of the interleaved J values) over the row
@@ -1147,143 +1176,189 @@ __global__ __launch_bounds__(kSchurThreads, 2) void k_schur(LuDevice d, const in
     const int fi = int(w / ngrp);
     const int p = fronts[fi];
     const int fs = d.fsz[p], K = d.nc[p], uc = fs - K;
-    const int i0 = K + rb * kSchurRows;
+    const int i0 = K + rb * RB;
     if (i0 >= fs) return;                       // whole block exits together
-    const int i1 = min(fs, i0 + kSchurRows);
+    const int i1 = min(fs, i0 + RB);
     const int nr = i1 - i0;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    float* G = ar.a1 + (long long)b * d.arena1 + d.off[p];
+    T* G = ar.a1 + (long long)b * d.arena1 + d.off[p];
     const int* rl = d.row_lb + d.lb_ptr[p];
-
-    // ---- U12 operands (B of the update), per wave and tile.  B-operand k of
-    // slot i:
-    //   FP32 (32x32x2):  k = 2 i + lane/32;
-    //   3xBF16 (32x32x16): k = 16 (i / 8) + 8 (lane/32) + i % 8.
-    const int ntile = (uc + 63) / 64;
-    float b0[kXMaxK / 2], b1[kXMaxK / 2];
-    auto kslot = [&](int i) {
-        return MODE == kMfmaF32 ? 2 * i + (lane >> 5) : 16 * (i >> 3) + 8 * (lane >> 5) + (i & 7);
-    };
-    auto load_u = [&](int t) {
-        const int ja = t * 64 + (lane & 31);
-        const bool oka = ja < uc, okb = ja + 32 < uc;
-        const float* U = G + K + ja;            // U12 row k at U + k * fs
-#pragma unroll
-        for (int i = 0; i < kXMaxK / 2; ++i) {
-            const int k = kslot(i);
-            b0[i] = (k < K && oka) ? U[(long long)k * fs] : 0.f;
-            b1[i] = (k < K && okb) ? U[(long long)k * fs + 32] : 0.f;
-        }
-    };
-    const bool l21w = wave < 2 && 32 * wave < K;          // waves forming L21
-    const int jc = 32 * wave + (lane & 31);              // L21 column of this lane
+    const T* Ui = uinv + ((long long)fi * Bp + b) * kUinvElems;
 
     // ---- rows [i0, i1) of A21 (-> Lb) and A22 (-> Ab) -----------------------
-    for (int e = tid; e < kSchurRows * (kSLd + uc); e += kSchurThreads) Lb[e] = 0.f;
+    for (int e = tid; e < RB * (kSLd + uc); e += kSchurThreads) Lb[e] = T(0);
     const int lo = rl[i0], hi = rl[i1];
     __syncthreads();
     put_entries<kSchurThreads>(Lb, d.ent_dst, d.ent_src, A, Bp, b, lo, hi, [=](int dst) {
         const int rr = dst / fs - i0, c = dst - (dst / fs) * fs;
-        return c < K ? rr * kSLd + c : kSchurRows * kSLd + rr * uc + c - K;
+        return c < K ? rr * kSLd + c : RB * kSLd + rr * uc + c - K;
     });
     __syncthreads();
-    __shared__ ChildSeg<float> seg[kItemTasks];
+    __shared__ ChildSeg<T> seg[kItemTasks];
     assemble_items<kSchurThreads>(d, ar, d.asm_base[p * kAsmKindCount + kAsmKSchur] + rb, b, Bp, Lb, seg,
                                   [=](int rr, int c) {
-                                      return c < K ? rr * kSLd + c : kSchurRows * kSLd + rr * uc + c - K;
+                                      return c < K ? rr * kSLd + c : RB * kSLd + rr * uc + c - K;
                                   });
+    const int ntile = (uc + 63) / 64;
 
-    // the first tile's U12 operands load while L21 is formed
-    if (wave < ntile) load_u(wave);
-
-    // ---- L21 = A21 U11^-1 (two 32 x 32 tiles, waves 0 and 1) ---------------
-    {
-        floatx16 acc;
+    if constexpr (MODE == kMfmaNone) {
+        // ---- FP64: L21 = A21 U11^-1, row r = tid / 16, 4 columns per thread ----
+        {
+            const int r = tid >> 4, cb = (tid & 15) * 4;
+            T acc[4] = {T(0), T(0), T(0), T(0)};
+            if (cb < K) {
+#pragma unroll 4
+                for (int k = 0; k < K; ++k) {
+                    const T a = Lb[r * kSLd + k];
 #pragma unroll
-        for (int v = 0; v < 16; ++v) acc[v] = 0.f;
-        if (l21w) {                                       // wave-uniform
-            const float* Ui = uinv + ((long long)fi * Bp + b) * kUinvElems;
-            float ub[kXMaxK / 2];
-#pragma unroll
-            for (int i = 0; i < kXMaxK / 2; ++i) {
-                const int k = 2 * i + (lane >> 5);
-                ub[i] = k < 32 * wave + 32 ? Ui[k * kXMaxK + jc] : 0.f;   // U11^-1 upper
-            }
-#pragma unroll
-            for (int i = 0; i < kXMaxK / 2; ++i) {
-                if (2 * i < 32 * wave + 32) {             // U11^-1 upper: k <= j
-                    const float av = Lb[(lane & 31) * kSLd + 2 * i + (lane >> 5)];
-                    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av, ub[i], acc, 0, 0, 0);
+                    for (int c = 0; c < 4; ++c)
+                        if (k <= cb + c) acc[c] += a * Ui[k * kXMaxK + cb + c];   // U11^-1 upper
                 }
             }
+            __syncthreads();                                  // every read of A21 done
+            if (cb < K) {
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const int j = cb + c;
+                    const T x = j < K ? acc[c] : T(0);
+                    Lb[r * kSLd + j] = x;
+                    if (r < nr && j < K) G[(long long)(i0 + r) * fs + j] = x;
+                }
+            }
+            __syncthreads();
         }
-        __syncthreads();                                  // every read of A21 done
-        if (l21w) {
+        // ---- FP64: C = A22 - L21 U12, a column per lane, RB rows, 64-column tiles
+        for (int t = wave; t < ntile; t += kSchurThreads / 64) {
+            const int j = t * 64 + lane;
+            if (j >= uc) continue;              // lanes only; no barrier below
+            T acc[RB];
+#pragma unroll
+            for (int rr = 0; rr < RB; ++rr) acc[rr] = Ab[rr * uc + j];
+            const T* U = G + K + j;
+#pragma unroll 4
+            for (int k = 0; k < K; ++k) {
+                const T u = U[(long long)k * fs];
+#pragma unroll
+                for (int rr = 0; rr < RB; ++rr) acc[rr] -= Lb[rr * kSLd + k] * u;
+            }
+#pragma unroll
+            for (int rr = 0; rr < RB; ++rr)
+                if (rr < nr) G[(long long)(i0 + rr) * fs + K + j] = acc[rr];
+        }
+    } else {
+        // ---- U12 operands (B of the update), per wave and tile.  B-operand k of
+        // slot i:
+        //   FP32 (32x32x2):  k = 2 i + lane/32;
+        //   3xBF16 (32x32x16): k = 16 (i / 8) + 8 (lane/32) + i % 8.
+        float b0[kXMaxK / 2], b1[kXMaxK / 2];
+        auto kslot = [&](int i) {
+            return MODE == kMfmaF32 ? 2 * i + (lane >> 5) : 16 * (i >> 3) + 8 * (lane >> 5) + (i & 7);
+        };
+        auto load_u = [&](int t) {
+            const int ja = t * 64 + (lane & 31);
+            const bool oka = ja < uc, okb = ja + 32 < uc;
+            const float* U = G + K + ja;        // U12 row k at U + k * fs
+#pragma unroll
+            for (int i = 0; i < kXMaxK / 2; ++i) {
+                const int k = kslot(i);
+                b0[i] = (k < K && oka) ? U[(long long)k * fs] : 0.f;
+                b1[i] = (k < K && okb) ? U[(long long)k * fs + 32] : 0.f;
+            }
+        };
+        const bool l21w = wave < 2 && 32 * wave < K;          // waves forming L21
+        const int jc = 32 * wave + (lane & 31);              // L21 column of this lane
+
+        // the first tile's U12 operands load while L21 is formed
+        if (wave < ntile) load_u(wave);
+
+        // ---- L21 = A21 U11^-1 (two 32 x 32 tiles, waves 0 and 1) -----------
+        {
+            floatx16 acc;
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[v] = 0.f;
+            if (l21w) {                                   // wave-uniform
+                float ub[kXMaxK / 2];
+#pragma unroll
+                for (int i = 0; i < kXMaxK / 2; ++i) {
+                    const int k = 2 * i + (lane >> 5);
+                    ub[i] = k < 32 * wave + 32 ? Ui[k * kXMaxK + jc] : 0.f;   // U11^-1 upper
+                }
+#pragma unroll
+                for (int i = 0; i < kXMaxK / 2; ++i) {
+                    if (2 * i < 32 * wave + 32) {         // U11^-1 upper: k <= j
+                        const float av = Lb[(lane & 31) * kSLd + 2 * i + (lane >> 5)];
+                        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av, ub[i], acc, 0, 0, 0);
+                    }
+                }
+            }
+            __syncthreads();                              // every read of A21 done
+            if (l21w) {
+#pragma unroll
+                for (int v = 0; v < 16; ++v) {
+                    const int rr = (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
+                    const float x = jc < K ? acc[v] : 0.f;
+                    Lb[rr * kSLd + jc] = x;
+                    if (rr < nr && jc < K) G[(long long)(i0 + rr) * fs + jc] = x;
+                }
+            }
+            __syncthreads();
+        }
+
+        // ---- C = A22 - L21 U12, 32 x 64 tiles, one wave each ----------------
+        for (int t = wave; t < ntile; t += kSchurThreads / 64) {
+            const int ja = t * 64 + (lane & 31);    // CB column of acc0 (acc1: + 32)
+            const bool oka = ja < uc, okb = ja + 32 < uc;
+            floatx16 acc0, acc1;
 #pragma unroll
             for (int v = 0; v < 16; ++v) {
                 const int rr = (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
-                const float x = jc < K ? acc[v] : 0.f;
-                Lb[rr * kSLd + jc] = x;
-                if (rr < nr && jc < K) G[(long long)(i0 + rr) * fs + jc] = x;
+                acc0[v] = oka ? Ab[rr * uc + ja] : 0.f;
+                acc1[v] = okb ? Ab[rr * uc + ja + 32] : 0.f;
             }
-        }
-        __syncthreads();
-    }
-
-    // ---- C = A22 - L21 U12, 32 x 64 tiles, one wave each --------------------
-    for (int t = wave; t < ntile; t += kSchurThreads / 64) {
-        const int ja = t * 64 + (lane & 31);    // CB column of acc0 (acc1: + 32)
-        const bool oka = ja < uc, okb = ja + 32 < uc;
-        floatx16 acc0, acc1;
+            if (MODE == kMfmaF32) {
 #pragma unroll
-        for (int v = 0; v < 16; ++v) {
-            const int rr = (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
-            acc0[v] = oka ? Ab[rr * uc + ja] : 0.f;
-            acc1[v] = okb ? Ab[rr * uc + ja + 32] : 0.f;
-        }
-        if (MODE == kMfmaF32) {
-#pragma unroll
-            for (int i = 0; i < kXMaxK / 2; ++i) {
-                if (2 * i < K) {                // uniform
-                    const float a = -Lb[(lane & 31) * kSLd + 2 * i + (lane >> 5)];
-                    acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b0[i], acc0, 0, 0, 0);
-                    acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b1[i], acc1, 0, 0, 0);
-                }
-            }
-        } else {
-#pragma unroll
-            for (int kq = 0; kq < kXMaxK / 16; ++kq) {
-                if (16 * kq < K) {              // uniform
-                    const int kb8 = 16 * kq + 8 * (lane >> 5);
-                    bf16x8 ah, al, b0h, b0l, b1h, b1l;
-#pragma unroll
-                    for (int t8 = 0; t8 < 8; ++t8) {
-                        __bf16 h, lo2;
-                        split_bf16(-Lb[(lane & 31) * kSLd + kb8 + t8], h, lo2);
-                        ah[t8] = h; al[t8] = lo2;
-                        split_bf16(b0[8 * kq + t8], h, lo2);
-                        b0h[t8] = h; b0l[t8] = lo2;
-                        split_bf16(b1[8 * kq + t8], h, lo2);
-                        b1h[t8] = h; b1l[t8] = lo2;
+                for (int i = 0; i < kXMaxK / 2; ++i) {
+                    if (2 * i < K) {                // uniform
+                        const float a = -Lb[(lane & 31) * kSLd + 2 * i + (lane >> 5)];
+                        acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b0[i], acc0, 0, 0, 0);
+                        acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b1[i], acc1, 0, 0, 0);
                     }
-                    acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, b0h, acc0, 0, 0, 0);
-                    acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, b0l, acc0, 0, 0, 0);
-                    acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, b0h, acc0, 0, 0, 0);
-                    acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, b1h, acc1, 0, 0, 0);
-                    acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, b1l, acc1, 0, 0, 0);
-                    acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, b1h, acc1, 0, 0, 0);
+                }
+            } else {
+#pragma unroll
+                for (int kq = 0; kq < kXMaxK / 16; ++kq) {
+                    if (16 * kq < K) {              // uniform
+                        const int kb8 = 16 * kq + 8 * (lane >> 5);
+                        bf16x8 ah, al, b0h, b0l, b1h, b1l;
+#pragma unroll
+                        for (int t8 = 0; t8 < 8; ++t8) {
+                            __bf16 h, lo2;
+                            split_bf16(-Lb[(lane & 31) * kSLd + kb8 + t8], h, lo2);
+                            ah[t8] = h; al[t8] = lo2;
+                            split_bf16(b0[8 * kq + t8], h, lo2);
+                            b0h[t8] = h; b0l[t8] = lo2;
+                            split_bf16(b1[8 * kq + t8], h, lo2);
+                            b1h[t8] = h; b1l[t8] = lo2;
+                        }
+                        acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, b0h, acc0, 0, 0, 0);
+                        acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, b0l, acc0, 0, 0, 0);
+                        acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, b0h, acc0, 0, 0, 0);
+                        acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, b1h, acc1, 0, 0, 0);
+                        acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, b1l, acc1, 0, 0, 0);
+                        acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, b1h, acc1, 0, 0, 0);
+                    }
                 }
             }
-        }
-        // next tile's U12 operands in flight during this tile's stores
-        if (t + kSchurThreads / 64 < ntile) load_u(t + kSchurThreads / 64);
-        float* Cw = G + (long long)i0 * fs + K + ja;
+            // next tile's U12 operands in flight during this tile's stores
+            if (t + kSchurThreads / 64 < ntile) load_u(t + kSchurThreads / 64);
+            float* Cw = G + (long long)i0 * fs + K + ja;
 #pragma unroll
-        for (int v = 0; v < 16; ++v) {
-            const int rr = (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
-            if (rr < nr) {
-                if (oka) Cw[(long long)rr * fs] = acc0[v];
-                if (okb) Cw[(long long)rr * fs + 32] = acc1[v];
+            for (int v = 0; v < 16; ++v) {
+                const int rr = (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
+                if (rr < nr) {
+                    if (oka) Cw[(long long)rr * fs] = acc0[v];
+                    if (okb) Cw[(long long)rr * fs + 32] = acc1[v];
+                }
             }
         }
     }
@@ -1739,48 +1814,66 @@ bool cross_schur_path(int max_fsz, int max_nc) {
     return max_fsz <= kDenseMaxFsz && max_nc <= kXMaxK;
 }
 
+template <typename T>
 void cross_group(const LuDevice& d, const int* fronts, int count, const double* A, int Bp,
-                 Arenas<float> ar, float* uinv, double pivot_eps, int static_piv, int* sing,
+                 Arenas<T> ar, T* uinv, double pivot_eps, int static_piv, int* sing,
                  hipStream_t s) {
     if (count <= 0) return;
     static bool attr_set = false;
     if (!attr_set) {
-        CUPF_HIP(hipFuncSetAttribute((const void*)k_pivot, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     int(kPivLds)));
+        CUPF_HIP(hipFuncSetAttribute((const void*)k_pivot<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     int(pivot_lds<T>())));
         attr_set = true;
     }
-    hipLaunchKernelGGL(k_pivot, dim3(unsigned(count) * unsigned(Bp)), dim3(kPivThreads), kPivLds, s, d,
-                       fronts, A, Bp, ar, uinv, pivot_eps, static_piv, sing);
+    hipLaunchKernelGGL(k_pivot<T>, dim3(unsigned(count) * unsigned(Bp)), dim3(kPivThreads), pivot_lds<T>(),
+                       s, d, fronts, A, Bp, ar, uinv, pivot_eps, static_piv, sing);
     CUPF_HIP(hipGetLastError());
 }
 
+template <typename T>
 void schur_group(const LuDevice& d, const int* fronts, int count, int max_fsz, const double* A,
-                 int Bp, Arenas<float> ar, const float* uinv, int dense_mode, hipStream_t s) {
+                 int Bp, Arenas<T> ar, const T* uinv, int dense_mode, hipStream_t s) {
     if (count <= 0) return;
     if (max_fsz > kDenseMaxFsz || Bp % kSchurScen)
         throw std::runtime_error("schur_group: front or batch outside the kernel's limits");
-    const int nrb = (max_fsz + kSchurRows - 1) / kSchurRows;
+    const int nrb = (max_fsz + schur_rows<T>() - 1) / schur_rows<T>();
     const long long blocks = (long long)count * nrb * Bp;
     if (blocks >= (1LL << 31)) throw std::runtime_error("schur_group: grid too large");
-    const size_t shmem = schur_lds(max_fsz);
-    static bool attr_set = false;
-    if (!attr_set) {
-        CUPF_HIP(hipFuncSetAttribute((const void*)k_schur<kMfmaF32>,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, int(schur_lds(kDenseMaxFsz))));
-        CUPF_HIP(hipFuncSetAttribute((const void*)k_schur<kMfmaBf16x3>,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, int(schur_lds(kDenseMaxFsz))));
-        attr_set = true;
+    const size_t shmem = schur_lds<T>(max_fsz);
+    if constexpr (sizeof(T) == 8) {
+        static bool attr_set = false;
+        if (!attr_set) {
+            CUPF_HIP(hipFuncSetAttribute((const void*)k_schur<double, kMfmaNone>,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         int(schur_lds<double>(kDenseMaxFsz))));
+            attr_set = true;
+        }
+        hipLaunchKernelGGL((k_schur<double, kMfmaNone>), dim3(unsigned(blocks)), dim3(kSchurThreads), shmem,
+                           s, d, fronts, A, Bp, ar, uinv, nrb);
+    } else {
+        static bool attr_set = false;
+        if (!attr_set) {
+            CUPF_HIP(hipFuncSetAttribute((const void*)k_schur<float, kMfmaF32>,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         int(schur_lds<float>(kDenseMaxFsz))));
+            CUPF_HIP(hipFuncSetAttribute((const void*)k_schur<float, kMfmaBf16x3>,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         int(schur_lds<float>(kDenseMaxFsz))));
+            attr_set = true;
+        }
+        if (dense_mode == kMfmaBf16x3)
+            hipLaunchKernelGGL((k_schur<float, kMfmaBf16x3>), dim3(unsigned(blocks)), dim3(kSchurThreads),
+                               shmem, s, d, fronts, A, Bp, ar, uinv, nrb);
+        else
+            hipLaunchKernelGGL((k_schur<float, kMfmaF32>), dim3(unsigned(blocks)), dim3(kSchurThreads),
+                               shmem, s, d, fronts, A, Bp, ar, uinv, nrb);
     }
-    if (dense_mode == kMfmaBf16x3)
-        hipLaunchKernelGGL(k_schur<kMfmaBf16x3>, dim3(unsigned(blocks)), dim3(kSchurThreads), shmem, s, d,
-                           fronts, A, Bp, ar, uinv, nrb);
-    else
-        hipLaunchKernelGGL(k_schur<kMfmaF32>, dim3(unsigned(blocks)), dim3(kSchurThreads), shmem, s, d,
-                           fronts, A, Bp, ar, uinv, nrb);
     CUPF_HIP(hipGetLastError());
 }
 
 size_t pivot_side_elems(int count, int Bp) { return size_t(count) * Bp * kUinvElems; }
+
+int schur_block_rows(int elem) { return elem == 4 ? schur_rows<float>() : schur_rows<double>(); }
 
 template <typename T>
 void lu_factor_group(const LuDevice& d, int tier, const int* fronts, int count, int max_fsz,
@@ -1934,5 +2027,14 @@ void lu_solve_scatter(const LuDevice& d, const T* y, int Bp, TX* x, hipStream_t 
 
 CUPF_LU_INST(double)
 CUPF_LU_INST(float)
+
+#define CUPF_XS_INST(T)                                                                          \
+    template void cross_group<T>(const LuDevice&, const int*, int, const double*, int, Arenas<T>, \
+                                 T*, double, int, int*, hipStream_t);                             \
+    template void schur_group<T>(const LuDevice&, const int*, int, int, const double*, int,       \
+                                 Arenas<T>, const T*, int, hipStream_t);
+
+CUPF_XS_INST(double)
+CUPF_XS_INST(float)
 
 }  // namespace cupf::kern

@@ -179,7 +179,7 @@ struct NewtonSolver::Impl {
                 if (cross_schur(L))
                     cross_count = std::max(cross_count, plan.group_ptr[L * 3 + 3] - plan.group_ptr[L * 3 + 2]);
             if (cross_count > 0)
-                uinv.allocate(kern::pivot_side_elems(cross_count, Bp) * sizeof(float));
+                uinv.allocate(kern::pivot_side_elems(cross_count, Bp) * tsize());
             capBp = Bp;
         }
         if (batch > capB) {
@@ -198,11 +198,10 @@ struct NewtonSolver::Impl {
         return d_groups.as<int>() + plan.group_ptr[gi];
     }
 
-    // FP32 tier-2 group factored by the cross + Schur kernels
+    // tier-2 group factored by the pivot + Schur kernels
     bool cross_schur(int level) const {
         const int gi = level * 3 + 2;
-        return !front64 && dense_mode != 0 &&
-               kern::cross_schur_path(plan.group_max_fsz[gi], plan.group_max_nc[gi]);
+        return kern::cross_schur_path(plan.group_max_fsz[gi], plan.group_max_nc[gi]);
     }
 
     // ---- factor the current J values ------------------------------------
@@ -215,15 +214,13 @@ struct NewtonSolver::Impl {
                 int cnt, mf;
                 const int* f = group(L, t, cnt, mf);
                 if (cnt == 0) continue;
-                if constexpr (sizeof(T) == 4) {
-                    if (t == 2 && cross_schur(L)) {
-                        kern::cross_group(lu, f, cnt, J.as<double>(), Bp, ar, uinv.as<float>(),
-                                          piv_eps, static_piv ? 1 : 0, sing.as<int>(), s);
-                        StageScope sc(timer, kStFactorDense, s);
-                        kern::schur_group(lu, f, cnt, mf, J.as<double>(), Bp, ar, uinv.as<float>(),
-                                          dense_mode, s);
-                        continue;
-                    }
+                if (t == 2 && cross_schur(L)) {
+                    kern::cross_group<T>(lu, f, cnt, J.as<double>(), Bp, ar, uinv.as<T>(), piv_eps,
+                                         static_piv ? 1 : 0, sing.as<int>(), s);
+                    StageScope sc(timer, kStFactorDense, s);
+                    kern::schur_group<T>(lu, f, cnt, mf, J.as<double>(), Bp, ar, uinv.as<T>(),
+                                         dense_mode, s);
+                    continue;
                 }
                 StageScope sc(timer, t == 2 ? kStFactorDense : -1, s);
                 kern::lu_factor_group<T>(lu, t, f, cnt, mf, J.as<double>(), Bp, ar, piv_eps,
@@ -731,7 +728,7 @@ void NewtonSolver::initialize(const YbusView& ybus, const int32_t* pv, int32_t n
                     }
                 }
             }
-        const AsmLists al = build_asm_lists(P, kinds, rpb);
+        const AsmLists al = build_asm_lists(P, kinds, rpb, kern::schur_block_rows(int(m.tsize())));
         m.d_abase = upload_vector(al.base, s);
         m.d_atptr = upload_vector(al.task_ptr, s);
         m.d_asptr = upload_vector(al.stage_ptr, s);
@@ -779,7 +776,8 @@ void NewtonSolver::initialize(const YbusView& ybus, const int32_t* pv, int32_t n
     //  pivot + Schur groups (the Schur kernel): flops 2 uc^2 K (update) +
     //    2 uc K^2 (L21); bytes = the rows >= K written once (CB uc^2, L21
     //    uc K) + the child contribution elements landing in them read once +
-    //    U12 (K uc) read once, at 4 B, + the J entries of those rows at 8 B;
+    //    U12 (K uc) read once, at the front element size, + the J entries of
+    //    those rows at 8 B;
     //  other tier-2 groups (assembly + dense LU): flops of the front LU;
     //    bytes = each front element written and read once + its J entries.
     for (int p = 0; p < P.num_fronts; ++p) {
@@ -798,7 +796,7 @@ void NewtonSolver::initialize(const YbusView& ybus, const int32_t* pv, int32_t n
             }
             ents = rl[fsz] - rl[K];
             m.st.dense_flops += 2.0 * uc * uc * K + 2.0 * uc * K * K;
-            m.st.dense_bytes += 4.0 * (uc * uc + child + 2.0 * uc * K) + 8.0 * ents;
+            m.st.dense_bytes += double(m.tsize()) * (uc * uc + child + 2.0 * uc * K) + 8.0 * ents;
             continue;
         }
         double fl = 0.0;
