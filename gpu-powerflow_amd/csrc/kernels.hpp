@@ -148,9 +148,10 @@ void lu_factor_group(const LuDevice& d, int tier, const int* fronts, int count, 
                      int* sing, int dense_mode, hipStream_t s);
 
 // Tier-2 groups whose fronts have at most 64 pivots and 512 rows
-// (cross_schur_path) factor in two launches instead: cross_group (pivot
-// block L11\U11, U12, and U11^-1 into the side buffer uinv of
-// pivot_side_elems(count, Bp) elements) then schur_group (L21 and the
+// (cross_schur_path) factor in three launches instead: cross_group (the
+// pivot block L11\U11, with U11^-1 and L11^-1 kept in the side buffer uinv of
+// pivot_side_elems(count, Bp) elements, then U12 by 64-column chunks) and
+// schur_group (L21 and the
 // contribution block C = A22 - L21 U12, assembled and updated in one pass,
 // schur_block_rows(sizeof(T)) rows per workgroup).  FP32 dense_mode 1: FP32
 // MFMA, 2: 3xBF16 MFMA; FP64 runs on the vector units.
@@ -158,8 +159,8 @@ bool cross_schur_path(int max_fsz, int max_nc);
 size_t pivot_side_elems(int count, int Bp);
 int schur_block_rows(int elem);
 template <typename T>
-void cross_group(const LuDevice& d, const int* fronts, int count, const double* A, int Bp,
-                 Arenas<T> ar, T* uinv, double pivot_eps, int static_piv, int* sing,
+void cross_group(const LuDevice& d, const int* fronts, int count, int max_fsz, const double* A,
+                 int Bp, Arenas<T> ar, T* uinv, double pivot_eps, int static_piv, int* sing,
                  hipStream_t s);
 template <typename T>
 void schur_group(const LuDevice& d, const int* fronts, int count, int max_fsz, const double* A,

@@ -946,83 +946,62 @@ __global__ __launch_bounds__(kDenseThreads) void k_factor_t2_mfma(LuDevice d,
 // the front is never staged whole in HBM before it is factored, and the
 // long pivot columns are formed by the many Schur workgroups.
 // ===========================================================================
+// Wave broadcast of lane `lane`'s value (v_readlane).
+template <typename T>
+__device__ __forceinline__ T bcast(T x, int lane) {
+    if constexpr (sizeof(T) == 4) {
+        return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), lane));
+    } else {
+        const long long v = __double_as_longlong(x);
+        const int lo = __builtin_amdgcn_readlane(int(v & 0xffffffffLL), lane);
+        const int hi = __builtin_amdgcn_readlane(int(v >> 32), lane);
+        return __longlong_as_double((long long)(unsigned(lo)) | ((long long)hi << 32));
+    }
+}
+
 constexpr int kPivThreads = 256;
 constexpr int kXMaxK = 64;
 constexpr int kXChunk = 64;                   // A12 columns per U12 step
 constexpr int kXCld = kXChunk + 4;            // chunk row stride (elements)
-constexpr int kUinvElems = kXMaxK * kXMaxK;   // side-buffer elements per (front, scenario)
+constexpr int kUinvElems = kXMaxK * kXMaxK;   // U11^-1 (then L11^-1) per (front, scenario)
+constexpr int kSideElems = 2 * kUinvElems;    // side-buffer elements per (front, scenario)
 
 template <typename T>
 constexpr size_t pivot_lds() {
-    return size_t(3 * 64 * kLinvLd + 1024 + 2 * kXMaxK + 64 * kXCld) * sizeof(T);
-}
-
-// Li = T^-1 for the 64 x 64 lower triangular T given by t(i, k), k <= i
-// (callers make rows and columns past the pivot count the identity): the
-// four 16 x 16 diagonal blocks by forward substitution (one column per
-// lane), then two doubling steps [A 0; C D]^-1 = [A^-1 0; -D^-1 C A^-1  D^-1]
-// at h = 16, 32.  Tmp holds 1024 elements.  Called by all NT threads; ends
-// synchronised.
-template <int NT, typename T, typename Get>
-__device__ inline void lower_inverse(Get t, T* Li, T* Tmp) {
-    const int tid = threadIdx.x;
-    for (int e = tid; e < 64 * 64; e += NT) Li[(e >> 6) * kLinvLd + (e & 63)] = T(0);
-    __syncthreads();
-    for (int i = 0; i < 16; ++i) {
-        if (tid < 64) {
-            const int o = (tid >> 4) * 16, j = tid & 15;
-            T acc = i == j ? T(1) : T(0);
-            for (int k = j; k < i; ++k) acc -= t(o + i, o + k) * Li[(o + k) * kLinvLd + o + j];
-            Li[(o + i) * kLinvLd + o + j] = acc / t(o + i, o + i);
-        }
-        __syncthreads();
-    }
-    for (int h = 16; h <= 32; h *= 2) {
-        const int total = 32 * h;              // (64 / 2h) pairs of h x h blocks
-        for (int e = tid; e < total; e += NT) {
-            const int q = e / (h * h), i = (e / h) % h, k = e % h, o = 2 * h * q;
-            T acc = T(0);
-            for (int m = k; m < h; ++m)            // T = C A^-1 (A^-1 lower)
-                acc += t(o + h + i, o + m) * Li[(o + m) * kLinvLd + o + k];
-            Tmp[e] = acc;
-        }
-        __syncthreads();
-        for (int e = tid; e < total; e += NT) {
-            const int q = e / (h * h), i = (e / h) % h, k = e % h, o = 2 * h * q;
-            T acc = T(0);
-            for (int m = 0; m <= i; ++m)           // -D^-1 T (D^-1 lower)
-                acc -= Li[(o + h + i) * kLinvLd + o + h + m] * Tmp[q * h * h + m * h + k];
-            Li[(o + h + i) * kLinvLd + o + k] = acc;
-        }
-        __syncthreads();
-    }
+    return size_t(64 * kLinvLd) * sizeof(T);
 }
 
 template <typename T>
-__global__ __launch_bounds__(kPivThreads) void k_pivot(LuDevice d, const int* __restrict__ fronts,
-                                                        const double* __restrict__ A, int Bp,
-                                                        Arenas<T> ar, T* __restrict__ uinv,
-                                                        double eps, int static_piv,
-                                                        int* __restrict__ sing) {
+constexpr size_t u12_lds() {
+    return size_t(64 * kLinvLd + 64 * kXCld) * sizeof(T);
+}
+
+// Pivot block of one (front, scenario), one wave, no barrier after the
+// assembly: A11 assembled in LDS (entries, then the children's staged walk),
+// LU with one row per lane in registers (the pivot row broadcast by
+// v_readlane, guarded pivots), L11\U11 written to the front and back to LDS,
+// then L11^-1 and U11^-1 with one column per lane (forward / backward
+// substitution against the identity, the factors read from LDS by broadcast)
+// into the per-(front, scenario) side buffer: U11^-1 [k][j], then L11^-1 [i][k].
+template <typename T>
+__global__ __launch_bounds__(64) void k_pivot(LuDevice d, const int* __restrict__ fronts,
+                                               const double* __restrict__ A, int Bp, Arenas<T> ar,
+                                               T* __restrict__ side, double eps, int static_piv,
+                                               int* __restrict__ sing) {
     extern __shared__ unsigned char smem_raw[];
     T* P = reinterpret_cast<T*>(smem_raw);                         // A11: [64][kLinvLd]
-    T* Li = P + 64 * kLinvLd;                                     // L11^-1
-    T* Wt = Li + 64 * kLinvLd;                                    // (U11^T)^-1 = (U11^-1)^T
-    T* Tmp = Wt + 64 * kLinvLd;                                   // [1024]
-    T* prow = Tmp + 1024;                                         // [2][64]
-    T* Ch = prow + 2 * kXMaxK;                                    // [64][kXCld]
     int fi, b;
     decode_block(Bp, fi, b);
     const int p = fronts[fi];
     const int fs = d.fsz[p], K = d.nc[p];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int lane = threadIdx.x;
     T* G = ar.a1 + (long long)b * d.arena1 + d.off[p];
     const int* rl = d.row_lb + d.lb_ptr[p];
 
     // ---- A11 -------------------------------------------------------------
-    for (int e = tid; e < 64 * kLinvLd; e += kPivThreads) P[e] = T(0);
+    for (int e = lane; e < 64 * kLinvLd; e += 64) P[e] = T(0);
     __syncthreads();
-    put_entries<kPivThreads>(P, d.ent_dst, d.ent_src, A, Bp, b, rl[0], rl[K], [=](int dst) {
+    put_entries<64>(P, d.ent_dst, d.ent_src, A, Bp, b, rl[0], rl[K], [=](int dst) {
         const int r = dst / fs, c = dst - r * fs;
         return c < K ? r * kLinvLd + c : -1;
     });
@@ -1030,59 +1009,102 @@ __global__ __launch_bounds__(kPivThreads) void k_pivot(LuDevice d, const int* __
     __shared__ ChildSeg<T> seg[kMaxSeg];
     __shared__ int pref[kMaxSeg + 1];
     __shared__ int stg[kMaxStages + 1];
-    const int* lbase = d.asm_base + p * kAsmKindCount;
-    assemble_tasks<kPivThreads, 16>(d, ar, lbase[kAsmKPivot], b, Bp, P, seg, pref, stg,
-                                    [=](int r, int c) { return r * kLinvLd + c; });
+    assemble_tasks<64, 16>(d, ar, d.asm_base[p * kAsmKindCount + kAsmKPivot], b, Bp, P, seg, pref, stg,
+                           [=](int r, int c) { return r * kLinvLd + c; });
 
-    // ---- LU of A11, one row per lane in registers ---------------------------
+    // ---- LU of A11, lane = row ------------------------------------------------
     int flag = 0;
-    const int r = tid;
     T row[kXMaxK];
 #pragma unroll
-    for (int c = 0; c < kXMaxK; ++c) row[c] = r < K ? P[r * kLinvLd + c] : T(0);
+    for (int c = 0; c < kXMaxK; ++c) row[c] = lane < K ? P[lane * kLinvLd + c] : T(0);
 #pragma unroll
     for (int c = 0; c < kXMaxK; ++c) {
-        T* pr = prow + (c & 1) * kXMaxK;
-        if (r == c && c < K) {
-            const T g = guard_pivot(row[c], eps, static_piv, flag);
-            row[c] = g;
-#pragma unroll
-            for (int cc = c; cc < kXMaxK; ++cc) pr[cc] = row[cc];
-        }
-        __syncthreads();
         if (c < K) {                              // uniform
-            const T piv = pr[c];
-            const bool act = r > c && r < K;
-            const T l = row[c] / piv;
-            const T lm = act ? l : T(0);
+            if (lane == c) row[c] = guard_pivot(row[c], eps, static_piv, flag);
+            const T piv = bcast(row[c], c);
+            const bool act = lane > c && lane < K;
+            const T l = act ? row[c] / piv : T(0);
             if (act) row[c] = l;
 #pragma unroll
-            for (int cc = c + 1; cc < kXMaxK; ++cc) row[cc] -= lm * pr[cc];
+            for (int cc = c + 1; cc < kXMaxK; ++cc) row[cc] -= l * bcast(row[cc], c);
         }
     }
-    if (r < K) {
+    if (lane < K) {
 #pragma unroll
         for (int c = 0; c < kXMaxK; ++c) {
-            P[r * kLinvLd + c] = row[c];
-            if (c < K) G[(long long)r * fs + c] = row[c];
+            P[lane * kLinvLd + c] = row[c];
+            if (c < K) G[(long long)lane * fs + c] = row[c];
         }
     }
     __syncthreads();
-    // L11^-1 (unit lower) and (U11^T)^-1; rows and columns >= K: identity
-    lower_inverse<kPivThreads, T>([=](int i, int k) {
-        return i == k ? T(1) : ((k < i && i < K) ? P[i * kLinvLd + k] : T(0));
-    }, Li, Tmp);
-    lower_inverse<kPivThreads, T>([=](int i, int k) {
-        return (i < K && k <= i) ? P[k * kLinvLd + i] : (i == k ? T(1) : T(0));
-    }, Wt, Tmp);
+    T* Ui = side + ((long long)fi * Bp + b) * kSideElems;         // U11^-1 [k][j]
+    T* Lo = Ui + kUinvElems;                                      // L11^-1 [i][k]
+    // ---- L11^-1, lane = column j: x_i = delta_ij - sum_{k<i} L_ik x_k -------
     {
-        T* Ui = uinv + ((long long)fi * Bp + b) * kUinvElems;       // U11^-1 [k][j]
-        for (int e = tid; e < kUinvElems; e += kPivThreads)
-            Ui[e] = Wt[(e & 63) * kLinvLd + (e >> 6)];
+        T x[kXMaxK];
+#pragma unroll
+        for (int i = 0; i < kXMaxK; ++i) {
+            T acc = lane == i ? T(1) : T(0);
+            if (i < K) {                          // uniform; rows >= K are the identity
+#pragma unroll
+                for (int k = 0; k < i; ++k) acc -= P[i * kLinvLd + k] * x[k];
+            }
+            x[i] = acc;
+            Lo[i * kXMaxK + lane] = acc;
+        }
     }
+    // ---- U11^-1, lane = column j: y_i = (delta_ij - sum_{k>i} U_ik y_k) / U_ii
+    {
+        T y[kXMaxK];
+#pragma unroll
+        for (int i = kXMaxK - 1; i >= 0; --i) {
+            T acc = lane == i ? T(1) : T(0);
+            if (i < K) {                          // uniform; rows >= K are the identity
+#pragma unroll
+                for (int k = i + 1; k < kXMaxK; ++k)
+                    if (k < K) acc -= P[i * kLinvLd + k] * y[k];
+                acc /= P[i * kLinvLd + i];
+            }
+            y[i] = acc;
+            Ui[i * kXMaxK + lane] = acc;
+        }
+    }
+    if (flag) atomicOr(sing + b, 1);
+}
 
-    // ---- U12 = L11^-1 A12 in chunks of kXChunk columns ----------------------
-    for (int j0 = K; j0 < fs; j0 += kXChunk) {
+// U12 = L11^-1 A12 for one chunk of kXChunk columns of one (front, scenario):
+// A12 chunk assembled in LDS (entries, then the children's staged walk),
+// L11^-1 from the side buffer, the product on MFMA (FP32) or vector FMA tiles
+// (FP64), written to the front's pivot rows.
+template <typename T>
+__global__ __launch_bounds__(kPivThreads) void k_u12(LuDevice d, const int* __restrict__ fronts,
+                                                      const double* __restrict__ A, int Bp,
+                                                      Arenas<T> ar, const T* __restrict__ side,
+                                                      int nch) {
+    extern __shared__ unsigned char smem_raw[];
+    T* Li = reinterpret_cast<T*>(smem_raw);                        // [64][kLinvLd]
+    T* Ch = Li + 64 * kLinvLd;                                    // [64][kXCld]
+    // lin -> (front, chunk, scenario), scenario fastest inside one XCD
+    const long long lin = blockIdx.x;
+    const long long per = (long long)gridDim.x / kXcd;
+    long long w = (lin % kXcd) * per + lin / kXcd;
+    const int b = int(w % Bp);
+    w /= Bp;
+    const int ch = int(w % nch);
+    const int fi = int(w / nch);
+    const int p = fronts[fi];
+    const int fs = d.fsz[p], K = d.nc[p];
+    const int j0 = K + ch * kXChunk;
+    if (j0 >= fs) return;                       // whole block exits together
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    T* G = ar.a1 + (long long)b * d.arena1 + d.off[p];
+    const int* rl = d.row_lb + d.lb_ptr[p];
+    const T* Lo = side + ((long long)fi * Bp + b) * kSideElems + kUinvElems;
+    for (int e = tid; e < kUinvElems; e += kPivThreads) Li[(e >> 6) * kLinvLd + (e & 63)] = Lo[e];
+    __shared__ ChildSeg<T> seg[kMaxSeg];
+    __shared__ int pref[kMaxSeg + 1];
+    __shared__ int stg[kMaxStages + 1];
+    {
         const int jn = min(kXChunk, fs - j0);
         for (int e = tid; e < 64 * kXCld; e += kPivThreads) Ch[e] = T(0);
         __syncthreads();
@@ -1091,8 +1113,8 @@ __global__ __launch_bounds__(kPivThreads) void k_pivot(LuDevice d, const int* __
             return (c >= j0 && c < j0 + jn) ? rr * kXCld + c - j0 : -1;
         });
         __syncthreads();
-        assemble_tasks<kPivThreads, 16>(d, ar, lbase[kAsmKChunk] + (j0 - K) / kXChunk, b, Bp, Ch, seg,
-                                        pref, stg, [=](int r, int c) { return r * kXCld + c - j0; });
+        assemble_tasks<kPivThreads, 16>(d, ar, d.asm_base[p * kAsmKindCount + kAsmKChunk] + ch, b, Bp, Ch,
+                                        seg, pref, stg, [=](int r, int c) { return r * kXCld + c - j0; });
         if constexpr (sizeof(T) == 4) {
             // 64 x 64 output = 2 x 2 tiles of 32 x 32, one per wave; L11^-1 is
             // lower triangular, so the top row half needs k < 32 only
@@ -1132,9 +1154,7 @@ __global__ __launch_bounds__(kPivThreads) void k_pivot(LuDevice d, const int* __
                     if (cb + c < jn) G[(long long)i * fs + j0 + cb + c] = acc[c];
             }
         }
-        __syncthreads();                        // before the next chunk reuses Ch
     }
-    if (flag) atomicOr(sing + b, 1);
 }
 
 constexpr int kSchurThreads = 256;
@@ -1183,7 +1203,7 @@ __global__ __launch_bounds__(kSchurThreads, 2) void k_schur(LuDevice d, const in
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     T* G = ar.a1 + (long long)b * d.arena1 + d.off[p];
     const int* rl = d.row_lb + d.lb_ptr[p];
-    const T* Ui = uinv + ((long long)fi * Bp + b) * kUinvElems;
+    const T* Ui = uinv + ((long long)fi * Bp + b) * kSideElems;
 
     // ---- rows [i0, i1) of A21 (-> Lb) and A22 (-> Ab) -----------------------
     for (int e = tid; e < RB * (kSLd + uc); e += kSchurThreads) Lb[e] = T(0);
@@ -1624,18 +1644,6 @@ __global__ __launch_bounds__(kLanes) void k_bwd_t12(LuDevice d, const int* __res
 constexpr int kSolveK = 64;
 constexpr int kSolvePld = kSolveK + 1;          // pivot-block row stride in LDS
 
-template <typename T>
-__device__ __forceinline__ T bcast(T x, int lane) {
-    if constexpr (sizeof(T) == 4) {
-        return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), lane));
-    } else {
-        const long long v = __double_as_longlong(x);
-        const int lo = __builtin_amdgcn_readlane(int(v & 0xffffffffLL), lane);
-        const int hi = __builtin_amdgcn_readlane(int(v >> 32), lane);
-        return __longlong_as_double((long long)(unsigned(lo)) | ((long long)hi << 32));
-    }
-}
-
 // Pivot block of G (K x K, row stride fs) into LDS P[K][kSolvePld].
 template <int NT, typename T>
 __device__ inline void stage_pivot(const T* __restrict__ G, int fs, int K, T* P) {
@@ -1671,7 +1679,6 @@ __global__ __launch_bounds__(NT) void k_fwd_w(LuDevice d, const int* __restrict_
     T* P = reinterpret_cast<T*>(smem_raw);                 // [64][kSolvePld]
     T* w = P + kSolveK * kSolvePld;                        // [fs]
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    constexpr int NW = NT / 64;
     int fi, b;
     decode_block(Bp, fi, b);
     const int p = fronts[fi];
@@ -1815,18 +1822,24 @@ bool cross_schur_path(int max_fsz, int max_nc) {
 }
 
 template <typename T>
-void cross_group(const LuDevice& d, const int* fronts, int count, const double* A, int Bp,
-                 Arenas<T> ar, T* uinv, double pivot_eps, int static_piv, int* sing,
+void cross_group(const LuDevice& d, const int* fronts, int count, int max_fsz, const double* A,
+                 int Bp, Arenas<T> ar, T* uinv, double pivot_eps, int static_piv, int* sing,
                  hipStream_t s) {
     if (count <= 0) return;
     static bool attr_set = false;
     if (!attr_set) {
         CUPF_HIP(hipFuncSetAttribute((const void*)k_pivot<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                      int(pivot_lds<T>())));
+        CUPF_HIP(hipFuncSetAttribute((const void*)k_u12<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     int(u12_lds<T>())));
         attr_set = true;
     }
-    hipLaunchKernelGGL(k_pivot<T>, dim3(unsigned(count) * unsigned(Bp)), dim3(kPivThreads), pivot_lds<T>(),
-                       s, d, fronts, A, Bp, ar, uinv, pivot_eps, static_piv, sing);
+    hipLaunchKernelGGL(k_pivot<T>, dim3(unsigned(count) * unsigned(Bp)), dim3(64), pivot_lds<T>(), s, d,
+                       fronts, A, Bp, ar, uinv, pivot_eps, static_piv, sing);
+    CUPF_HIP(hipGetLastError());
+    const int nch = (max_fsz + kXChunk - 1) / kXChunk;
+    hipLaunchKernelGGL(k_u12<T>, dim3(unsigned(count) * unsigned(nch) * unsigned(Bp)), dim3(kPivThreads),
+                       u12_lds<T>(), s, d, fronts, A, Bp, ar, uinv, nch);
     CUPF_HIP(hipGetLastError());
 }
 
@@ -1871,7 +1884,7 @@ void schur_group(const LuDevice& d, const int* fronts, int count, int max_fsz, c
     CUPF_HIP(hipGetLastError());
 }
 
-size_t pivot_side_elems(int count, int Bp) { return size_t(count) * Bp * kUinvElems; }
+size_t pivot_side_elems(int count, int Bp) { return size_t(count) * Bp * kSideElems; }
 
 int schur_block_rows(int elem) { return elem == 4 ? schur_rows<float>() : schur_rows<double>(); }
 
@@ -2029,8 +2042,8 @@ CUPF_LU_INST(double)
 CUPF_LU_INST(float)
 
 #define CUPF_XS_INST(T)                                                                          \
-    template void cross_group<T>(const LuDevice&, const int*, int, const double*, int, Arenas<T>, \
-                                 T*, double, int, int*, hipStream_t);                             \
+    template void cross_group<T>(const LuDevice&, const int*, int, int, const double*, int,       \
+                                 Arenas<T>, T*, double, int, int*, hipStream_t);                  \
     template void schur_group<T>(const LuDevice&, const int*, int, int, const double*, int,       \
                                  Arenas<T>, const T*, int, hipStream_t);
 

@@ -215,7 +215,7 @@ struct NewtonSolver::Impl {
                 const int* f = group(L, t, cnt, mf);
                 if (cnt == 0) continue;
                 if (t == 2 && cross_schur(L)) {
-                    kern::cross_group<T>(lu, f, cnt, J.as<double>(), Bp, ar, uinv.as<T>(), piv_eps,
+                    kern::cross_group<T>(lu, f, cnt, mf, J.as<double>(), Bp, ar, uinv.as<T>(), piv_eps,
                                          static_piv ? 1 : 0, sing.as<int>(), s);
                     StageScope sc(timer, kStFactorDense, s);
                     kern::schur_group<T>(lu, f, cnt, mf, J.as<double>(), Bp, ar, uinv.as<T>(),
