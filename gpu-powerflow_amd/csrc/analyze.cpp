@@ -606,6 +606,7 @@ AsmLists build_asm_lists(const LuPlan& L, const std::vector<int>& kinds,
     A.task_ptr.push_back(0);
     A.stage_ptr.push_back(0);
     A.item_ptr.push_back(0);
+    A.chunk_ptr.push_back(0);
     struct Piece { int q, a0, a1, b0, b1; };
     // landing-set intersection of two pieces of (possibly) different children
     auto meets = [&](const Piece& x, const Piece& y) {
@@ -704,6 +705,31 @@ AsmLists build_asm_lists(const LuPlan& L, const std::vector<int>& kinds,
                         }
                     }
                     A.item_ptr.push_back(int(A.items.size()));
+                }
+                // chunk maps, per owner wave, in task order: every landing
+                // contribution row split into pieces of kAsmChunkLen columns
+                for (int wv = 0; wv < kAsmItemWaves; ++wv) {
+                    for (int t = tb; t < te; ++t) {
+                        const AsmTask& tk = A.tasks[size_t(t)];
+                        const int* loc = L.asm_loc.data() + tk.loc_ptr;
+                        for (int a = tk.a0; a < tk.a0 + tk.na; ++a) {
+                            const int rr = loc[a] - r0;
+                            if (rr % kAsmItemWaves != wv) continue;
+                            const long long row = tk.off + (long long)(tk.cnc + a) * tk.cfs + tk.cnc;
+                            for (int c0 = 0; c0 < tk.nb; c0 += kAsmChunkLen) {
+                                const long long src = row + tk.b0 + c0;
+                                if (src + kAsmChunkLen >= (1LL << 31) - 1)
+                                    throw std::runtime_error("build_asm_lists: element index too large");
+                                A.chunks.push_back(tk.tier == 0 ? int(-1 - src) : int(src));
+                                A.chunks.push_back(tk.loc_ptr + tk.b0 + c0);
+                                A.chunks.push_back(rr);
+                                A.chunks.push_back(std::min(kAsmChunkLen, tk.nb - c0));
+                            }
+                        }
+                    }
+                    if (A.chunks.size() / 4 >= size_t(1) << 31)
+                        throw std::runtime_error("build_asm_lists: chunk maps too large");
+                    A.chunk_ptr.push_back(int(A.chunks.size() / 4));
                 }
                 A.task_ptr.push_back(int(A.tasks.size()));
                 A.stage_ptr.push_back(int(A.stage.size()));

@@ -165,6 +165,24 @@ struct AsmTask {                     // 48 bytes, device-visible layout
 constexpr int kAsmItemWaves = 4;
 constexpr int kAsmMaxItemTasks = 128;
 
+// Chunk maps: every list also holds its landing contribution rows cut into
+// chunks of up to kAsmChunkLen columns, split by owner wave like the row
+// items (block row rr by wave rr % kAsmItemWaves) and in task order inside
+// each part (child, then contribution row, then column), four ints each:
+//   src   element index of the chunk's first element CB(a, b0 + c0) inside
+//         the child's arena for one scenario (tier 1/2:
+//         off + (cnc + a) * cfs + cnc + b0 + c0, the chunk's elements follow
+//         contiguously; tier 0: -1 - that index, the elements then at a
+//         stride of the padded batch),
+//   loc   index into asm_loc of the chunk's first column (parent-local
+//         column of element j: asm_loc[loc + j]),
+//   rr    block row the chunk lands on, and
+//   len   its number of columns.
+// A wave walks its part with one chunk per instruction (a lane per column)
+// and several chunks in flight; one wave owns every element of a row, so
+// each element receives its contributions in a fixed order.
+constexpr int kAsmChunkLen = 64;
+
 struct AsmLists {
     std::vector<int> base;           // [P * kAsmKinds]: first list id of (front, kind), -1 if none
     std::vector<int> task_ptr;       // [lists + 1] into tasks
@@ -173,6 +191,8 @@ struct AsmLists {
     std::vector<AsmTask> tasks;
     std::vector<int> item_ptr;       // [lists * kAsmItemWaves + 1] into items
     std::vector<int> items;
+    std::vector<int> chunk_ptr;      // [lists * kAsmItemWaves + 1] into chunks (quads)
+    std::vector<int> chunks;         // (src, loc, rr, len) quads
 };
 
 // kinds[p] is a bit mask of the kinds built for front p; rows_per_block[p]

@@ -127,6 +127,8 @@ struct LuDevice {
     const AsmTaskDev* asm_tasks = nullptr;
     const int* asm_iptr = nullptr;    // row items of the Schur lists, 4 owner waves per list
     const int* asm_items = nullptr;
+    const int* asm_cptr = nullptr;    // chunk maps of the lists: (src, loc, rr, len) quads
+    const int4* asm_chunks = nullptr;
 };
 
 // Block height of the row-block assembly of a tier-2 group (elem = sizeof T).

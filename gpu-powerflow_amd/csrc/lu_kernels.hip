@@ -310,6 +310,58 @@ __device__ inline void assemble_items(const LuDevice& d, const Arenas<T>& ar, in
     __syncthreads();
 }
 
+// Assembly of one block of a front from its chunk map (host-built, see
+// LuDevice asm_cptr / asm_chunks): each wave walks the chunks of the block
+// rows it owns (rr % 4 == wave; a 64-lane workgroup walks all four parts),
+// one chunk (up to 64 columns of one contribution row) per instruction with
+// a lane per column, D chunks in flight, and adds them into LDS:
+//   F[map(rr, loc[c])] += CB(a, c).
+// One instruction touches distinct columns of one row (a plain LDS
+// read-add-write, no atomics), and one wave owns every element and issues
+// its adds in map order, so the additions into an element happen in a fixed
+// order.
+
+template <int NT, int D, typename T, typename Map>
+__device__ inline void assemble_chunks(const LuDevice& d, const Arenas<T>& ar, int list, int b,
+                                       int Bp, T* __restrict__ F, Map map) {
+    static_assert(NT == 64 || NT == 64 * kItemWaves, "one wave, or one wave per owner part");
+    const int lane = int(threadIdx.x) & 63;
+    const T* c1 = ar.a1 + (long long)b * d.arena1;   // tier 1/2 children: element s at c1[s]
+    const T* c0 = ar.a0 + b;                         // tier 0 children: element s at c0[s * Bp]
+    for (int part = int(threadIdx.x) >> 6; part < kItemWaves; part += NT / 64) {
+        const int q_begin = d.asm_cptr[list * kItemWaves + part];
+        const int q_end = d.asm_cptr[list * kItemWaves + part + 1];
+        for (int qb = q_begin; qb < q_end; qb += 64) {
+            const int n = __builtin_amdgcn_readfirstlane(min(64, q_end - qb));
+            const int4 mine = lane < n ? d.asm_chunks[qb + lane] : make_int4(0, 0, 0, 0);
+            for (int u0 = 0; u0 < n; u0 += D) {
+                // the D chunks' fields (uniform) first, then every load, then
+                // the adds: no wait between the chunks' loads
+                int src[D], lo[D], len[D], rr[D];
+#pragma unroll
+                for (int u = 0; u < D; ++u) {
+                    const int j = min(u0 + u, n - 1);
+                    src[u] = __builtin_amdgcn_readlane(mine.x, j);
+                    lo[u] = __builtin_amdgcn_readlane(mine.y, j);
+                    rr[u] = __builtin_amdgcn_readlane(mine.z, j);
+                    len[u] = u0 + u < n ? __builtin_amdgcn_readlane(mine.w, j) : 0;
+                }
+                T v[D];
+                int k[D];
+#pragma unroll
+                for (int u = 0; u < D; ++u) {
+                    const int c = min(lane, max(len[u] - 1, 0));      // in-bounds for every lane
+                    k[u] = d.asm_loc[lo[u] + c];
+                    v[u] = src[u] >= 0 ? c1[src[u] + c] : c0[(long long)(-1 - src[u] + c) * Bp];
+                }
+#pragma unroll
+                for (int u = 0; u < D; ++u)
+                    if (lane < len[u]) F[map(rr[u], k[u])] += v[u];
+            }
+        }
+    }
+}
+
 // ===========================================================================
 // Factor, tier 0 (lane per scenario)
 // ===========================================================================
@@ -1214,11 +1266,11 @@ __global__ __launch_bounds__(kSchurThreads, 2) void k_schur(LuDevice d, const in
         return c < K ? rr * kSLd + c : RB * kSLd + rr * uc + c - K;
     });
     __syncthreads();
-    __shared__ ChildSeg<T> seg[kItemTasks];
-    assemble_items<kSchurThreads>(d, ar, d.asm_base[p * kAsmKindCount + kAsmKSchur] + rb, b, Bp, Lb, seg,
-                                  [=](int rr, int c) {
-                                      return c < K ? rr * kSLd + c : RB * kSLd + rr * uc + c - K;
-                                  });
+    assemble_chunks<kSchurThreads, 16>(d, ar, d.asm_base[p * kAsmKindCount + kAsmKSchur] + rb, b, Bp, Lb,
+                                      [=](int rr, int c) {
+                                          return c < K ? rr * kSLd + c : RB * kSLd + rr * uc + c - K;
+                                      });
+    __syncthreads();
     const int ntile = (uc + 63) / 64;
 
     if constexpr (MODE == kMfmaNone) {
