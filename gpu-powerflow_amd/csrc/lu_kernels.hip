@@ -416,11 +416,9 @@ __device__ inline void assemble_front(const LuDevice& d, const Arenas<T>& ar, co
     put_entries<NT>(F, d.ent_dst, d.ent_src, A, Bp, b, d.ent_ptr[p], d.ent_ptr[p + 1],
                     [](int dst) { return dst; });
     __syncthreads();
-    __shared__ ChildSeg<T> seg[kMaxSeg];
-    __shared__ int pref[kMaxSeg + 1];
-    __shared__ int stg[kMaxStages + 1];
-    assemble_tasks<NT, 16>(d, ar, d.asm_base[p * kAsmKindCount + kAsmKFront], b, Bp, F, seg, pref,
-                           stg, [=](int r, int c) { return r * fs + c; });
+    assemble_chunks<NT, 16>(d, ar, d.asm_base[p * kAsmKindCount + kAsmKFront], b, Bp, F,
+                            [=](int r, int c) { return r * fs + c; });
+    __syncthreads();
 }
 
 // Tier 1: the front is assembled and factored in LDS.  The K pivot columns
@@ -673,9 +671,9 @@ __global__ __launch_bounds__(kAsmThreads) void k_assemble_rows(LuDevice d,
     put_entries<kAsmThreads>(F, d.ent_dst, d.ent_src, A, Bp, b, lo, hi,
                              [=](int dst) { return dst - i0 * fs; });
     __syncthreads();
-    __shared__ ChildSeg<T> seg[kItemTasks];
-    assemble_items<kAsmThreads>(d, ar, d.asm_base[p * kAsmKindCount + kAsmKRows] + rb, b, Bp, F, seg,
-                                [=](int rr, int c) { return rr * fs + c; });
+    assemble_chunks<kAsmThreads, 16>(d, ar, d.asm_base[p * kAsmKindCount + kAsmKRows] + rb, b, Bp, F,
+                                     [=](int rr, int c) { return rr * fs + c; });
+    __syncthreads();
     T* G = ar.a1 + (long long)b * d.arena1 + d.off[p] + (long long)i0 * fs;
     for (int e = tid; e < nr * fs; e += kAsmThreads) G[e] = F[e];
 }
@@ -1058,11 +1056,9 @@ __global__ __launch_bounds__(64) void k_pivot(LuDevice d, const int* __restrict_
         return c < K ? r * kLinvLd + c : -1;
     });
     __syncthreads();
-    __shared__ ChildSeg<T> seg[kMaxSeg];
-    __shared__ int pref[kMaxSeg + 1];
-    __shared__ int stg[kMaxStages + 1];
-    assemble_tasks<64, 16>(d, ar, d.asm_base[p * kAsmKindCount + kAsmKPivot], b, Bp, P, seg, pref, stg,
-                           [=](int r, int c) { return r * kLinvLd + c; });
+    assemble_chunks<64, 16>(d, ar, d.asm_base[p * kAsmKindCount + kAsmKPivot], b, Bp, P,
+                            [=](int r, int c) { return r * kLinvLd + c; });
+    __syncthreads();
 
     // ---- LU of A11, lane = row ------------------------------------------------
     int flag = 0;
@@ -1153,9 +1149,6 @@ __global__ __launch_bounds__(kPivThreads) void k_u12(LuDevice d, const int* __re
     const int* rl = d.row_lb + d.lb_ptr[p];
     const T* Lo = side + ((long long)fi * Bp + b) * kSideElems + kUinvElems;
     for (int e = tid; e < kUinvElems; e += kPivThreads) Li[(e >> 6) * kLinvLd + (e & 63)] = Lo[e];
-    __shared__ ChildSeg<T> seg[kMaxSeg];
-    __shared__ int pref[kMaxSeg + 1];
-    __shared__ int stg[kMaxStages + 1];
     {
         const int jn = min(kXChunk, fs - j0);
         for (int e = tid; e < 64 * kXCld; e += kPivThreads) Ch[e] = T(0);
@@ -1165,8 +1158,9 @@ __global__ __launch_bounds__(kPivThreads) void k_u12(LuDevice d, const int* __re
             return (c >= j0 && c < j0 + jn) ? rr * kXCld + c - j0 : -1;
         });
         __syncthreads();
-        assemble_tasks<kPivThreads, 16>(d, ar, d.asm_base[p * kAsmKindCount + kAsmKChunk] + ch, b, Bp, Ch,
-                                        seg, pref, stg, [=](int r, int c) { return r * kXCld + c - j0; });
+        assemble_chunks<kPivThreads, 16>(d, ar, d.asm_base[p * kAsmKindCount + kAsmKChunk] + ch, b, Bp, Ch,
+                                         [=](int r, int c) { return r * kXCld + c - j0; });
+        __syncthreads();
         if constexpr (sizeof(T) == 4) {
             // 64 x 64 output = 2 x 2 tiles of 32 x 32, one per wave; L11^-1 is
             // lower triangular, so the top row half needs k < 32 only
