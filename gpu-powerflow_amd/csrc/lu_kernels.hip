@@ -312,7 +312,8 @@ __device__ inline void assemble_items(const LuDevice& d, const Arenas<T>& ar, in
 
 // Assembly of one block of a front from its chunk map (host-built, see
 // LuDevice asm_cptr / asm_chunks): each wave walks the chunks of the block
-// rows it owns (rr % 4 == wave; a 64-lane workgroup walks all four parts),
+// rows it owns (rr % 4 == wave; with fewer waves, wave w walks the parts
+// w, w + waves, ...),
 // one chunk (up to 64 columns of one contribution row) per instruction with
 // a lane per column, D chunks in flight, and adds them into LDS:
 //   F[map(rr, loc[c])] += CB(a, c).
@@ -324,7 +325,7 @@ __device__ inline void assemble_items(const LuDevice& d, const Arenas<T>& ar, in
 template <int NT, int D, typename T, typename Map>
 __device__ inline void assemble_chunks(const LuDevice& d, const Arenas<T>& ar, int list, int b,
                                        int Bp, T* __restrict__ F, Map map) {
-    static_assert(NT == 64 || NT == 64 * kItemWaves, "one wave, or one wave per owner part");
+    static_assert(NT % 64 == 0 && kItemWaves % (NT / 64) == 0, "whole waves sharing the owner parts");
     const int lane = int(threadIdx.x) & 63;
     const T* c1 = ar.a1 + (long long)b * d.arena1;   // tier 1/2 children: element s at c1[s]
     const T* c0 = ar.a0 + b;                         // tier 0 children: element s at c0[s * Bp]
@@ -975,15 +976,16 @@ __global__ __launch_bounds__(kDenseThreads) void k_factor_t2_mfma(LuDevice d,
 //
 // A front of fsz rows and K pivots splits into the pivot block A11 (K x K),
 // the pivot rows A12 (K x uc), the pivot columns A21 (uc x K) and the
-// contribution block CB = A22 - L21 U12 (uc x uc), uc = fsz - K.  Two
+// contribution block CB = A22 - L21 U12 (uc x uc), uc = fsz - K.  Three
 // launches per (level) group:
 //
 //   k_pivot (one workgroup per (front, scenario), 256 lanes): assembles A11
 //     straight from the J values and the children's contribution blocks,
-//     factors it with one row per lane in registers (guarded pivots), forms
-//     L11^-1 and U11^-1 by blocked triangular inversion, then assembles A12
-//     in chunks of 64 columns and writes U12 = L11^-1 A12 (MFMA).  U11^-1 is
-//     left in a per-(front, scenario) side buffer for the Schur kernel.
+//     factors it with one row per lane in registers (guarded pivots) and
+//     forms L11^-1 and U11^-1 by substitution with one column per lane (two
+//     waves at once), left in a per-(front, scenario) side buffer;
+//   k_u12 (one workgroup per (front, 64-column chunk, scenario)): assembles
+//     its chunk of A12 and writes U12 = L11^-1 A12 (MFMA).
 //   k_schur (one workgroup per (front, 32-row block, scenario), 256 lanes):
 //     assembles its rows of A21 and A22 in LDS (entries and child
 //     contribution rows, children in a fixed order), forms its rows of
@@ -1026,98 +1028,123 @@ constexpr size_t u12_lds() {
     return size_t(64 * kLinvLd + 64 * kXCld) * sizeof(T);
 }
 
-// Pivot block of one (front, scenario), one wave, no barrier after the
-// assembly: A11 assembled in LDS (entries, then the children's staged walk),
-// LU with one row per lane in registers (the pivot row broadcast by
-// v_readlane, guarded pivots), L11\U11 written to the front and back to LDS,
-// then L11^-1 and U11^-1 with one column per lane (forward / backward
-// substitution against the identity, the factors read from LDS by broadcast)
-// into the per-(front, scenario) side buffer: U11^-1 [k][j], then L11^-1 [i][k].
+// Pivot block of one (front, scenario), four waves (FP32) or two (FP64,
+// whose register rows take twice the space): A11 assembled in LDS by all of
+// them (entries, then the chunk map), then
+//   wave 0: LU with one row per lane in registers (the pivot row broadcast
+//           by v_readlane, guarded pivots), L11\U11 back to LDS; then
+//           L11^-1 with one column per lane by forward substitution, the
+//           rows of L11 broadcast by v_readlane from its own registers;
+//   wave 1: U11^-1 with one column per lane by backward substitution, the
+//           rows of U11 loaded from LDS into registers and broadcast the same
+//           way;
+//   waves 2, 3 (FP64: wave 1 first): L11\U11 copied to the front's pivot
+//           block.
+// The inverses go to the per-(front, scenario) side buffer, row-major:
+// U11^-1 [k][j], then L11^-1 [i][k].
 template <typename T>
-__global__ __launch_bounds__(64) void k_pivot(LuDevice d, const int* __restrict__ fronts,
-                                               const double* __restrict__ A, int Bp, Arenas<T> ar,
-                                               T* __restrict__ side, double eps, int static_piv,
-                                               int* __restrict__ sing) {
+constexpr int pivot_threads() { return sizeof(T) == 4 ? 256 : 128; }
+
+template <typename T>
+__global__ __launch_bounds__(pivot_threads<T>()) void k_pivot(LuDevice d, const int* __restrict__ fronts,
+                                                        const double* __restrict__ A, int Bp, Arenas<T> ar,
+                                                        T* __restrict__ side, double eps, int static_piv,
+                                                        int* __restrict__ sing) {
     extern __shared__ unsigned char smem_raw[];
     T* P = reinterpret_cast<T*>(smem_raw);                         // A11: [64][kLinvLd]
     int fi, b;
     decode_block(Bp, fi, b);
     const int p = fronts[fi];
     const int fs = d.fsz[p], K = d.nc[p];
-    const int lane = threadIdx.x;
+    constexpr int NT = pivot_threads<T>();
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     T* G = ar.a1 + (long long)b * d.arena1 + d.off[p];
     const int* rl = d.row_lb + d.lb_ptr[p];
 
     // ---- A11 -------------------------------------------------------------
-    for (int e = lane; e < 64 * kLinvLd; e += 64) P[e] = T(0);
+    for (int e = tid; e < 64 * kLinvLd; e += NT) P[e] = T(0);
     __syncthreads();
-    put_entries<64>(P, d.ent_dst, d.ent_src, A, Bp, b, rl[0], rl[K], [=](int dst) {
+    put_entries<NT>(P, d.ent_dst, d.ent_src, A, Bp, b, rl[0], rl[K], [=](int dst) {
         const int r = dst / fs, c = dst - r * fs;
         return c < K ? r * kLinvLd + c : -1;
     });
     __syncthreads();
-    assemble_chunks<64, 16>(d, ar, d.asm_base[p * kAsmKindCount + kAsmKPivot], b, Bp, P,
+    assemble_chunks<NT, 16>(d, ar, d.asm_base[p * kAsmKindCount + kAsmKPivot], b, Bp, P,
                             [=](int r, int c) { return r * kLinvLd + c; });
     __syncthreads();
 
-    // ---- LU of A11, lane = row ------------------------------------------------
-    int flag = 0;
-    T row[kXMaxK];
-#pragma unroll
-    for (int c = 0; c < kXMaxK; ++c) row[c] = lane < K ? P[lane * kLinvLd + c] : T(0);
-#pragma unroll
-    for (int c = 0; c < kXMaxK; ++c) {
-        if (c < K) {                              // uniform
-            if (lane == c) row[c] = guard_pivot(row[c], eps, static_piv, flag);
-            const T piv = bcast(row[c], c);
-            const bool act = lane > c && lane < K;
-            const T l = act ? row[c] / piv : T(0);
-            if (act) row[c] = l;
-#pragma unroll
-            for (int cc = c + 1; cc < kXMaxK; ++cc) row[cc] -= l * bcast(row[cc], c);
-        }
-    }
-    if (lane < K) {
-#pragma unroll
-        for (int c = 0; c < kXMaxK; ++c) {
-            P[lane * kLinvLd + c] = row[c];
-            if (c < K) G[(long long)lane * fs + c] = row[c];
-        }
-    }
-    __syncthreads();
     T* Ui = side + ((long long)fi * Bp + b) * kSideElems;         // U11^-1 [k][j]
     T* Lo = Ui + kUinvElems;                                      // L11^-1 [i][k]
-    // ---- L11^-1, lane = column j: x_i = delta_ij - sum_{k<i} L_ik x_k -------
-    {
+    T row[kXMaxK];                                                // lane's row of L\U
+    if (wave == 0) {
+        // ---- LU of A11, lane = row ----------------------------------------
+        int flag = 0;
+#pragma unroll
+        for (int c = 0; c < kXMaxK; ++c) row[c] = lane < K ? P[lane * kLinvLd + c] : T(0);
+#pragma unroll
+        for (int c = 0; c < kXMaxK; ++c) {
+            if (c < K) {                              // uniform
+                if (lane == c) row[c] = guard_pivot(row[c], eps, static_piv, flag);
+                const T piv = bcast(row[c], c);
+                const bool act = lane > c && lane < K;
+                const T l = act ? row[c] / piv : T(0);
+                if (act) row[c] = l;
+#pragma unroll
+                for (int cc = c + 1; cc < kXMaxK; ++cc) row[cc] -= l * bcast(row[cc], c);
+            }
+        }
+        if (lane < K) {
+#pragma unroll
+            for (int c = 0; c < kXMaxK; ++c) P[lane * kLinvLd + c] = row[c];
+        }
+        if (flag) atomicOr(sing + b, 1);
+    }
+    __syncthreads();
+    if (wave == 0) {
+        // ---- L11^-1, lane = column j: x_i = delta_ij - sum_{k<i} L_ik x_k -------
         T x[kXMaxK];
 #pragma unroll
         for (int i = 0; i < kXMaxK; ++i) {
             T acc = lane == i ? T(1) : T(0);
-            if (i < K) {                          // uniform; rows >= K are the identity
+            if (i < K) {                              // uniform; rows >= K are the identity
 #pragma unroll
-                for (int k = 0; k < i; ++k) acc -= P[i * kLinvLd + k] * x[k];
+                for (int k = 0; k < i; ++k) acc -= bcast(row[k], i) * x[k];
             }
             x[i] = acc;
             Lo[i * kXMaxK + lane] = acc;
         }
-    }
-    // ---- U11^-1, lane = column j: y_i = (delta_ij - sum_{k>i} U_ik y_k) / U_ii
-    {
+    } else if (wave == 1) {
+        if constexpr (NT == 128) {
+            // ---- L11\U11 to the front's pivot block ----------------------
+            for (int e = lane; e < K * K; e += 64) {
+                const int r = e / K, c = e - r * K;
+                G[(long long)r * fs + c] = P[r * kLinvLd + c];
+            }
+        }
+        // ---- U11^-1, lane = column j: y_i = (delta_ij - sum_{k>i} U_ik y_k) / U_ii
+#pragma unroll
+        for (int c = 0; c < kXMaxK; ++c)
+            row[c] = (lane < K && c >= lane && c < K) ? P[lane * kLinvLd + c] : T(0);
         T y[kXMaxK];
 #pragma unroll
         for (int i = kXMaxK - 1; i >= 0; --i) {
             T acc = lane == i ? T(1) : T(0);
-            if (i < K) {                          // uniform; rows >= K are the identity
+            if (i < K) {                              // uniform; rows >= K are the identity
 #pragma unroll
                 for (int k = i + 1; k < kXMaxK; ++k)
-                    if (k < K) acc -= P[i * kLinvLd + k] * y[k];
-                acc /= P[i * kLinvLd + i];
+                    if (k < K) acc -= bcast(row[k], i) * y[k];
+                acc /= bcast(row[i], i);
             }
             y[i] = acc;
             Ui[i * kXMaxK + lane] = acc;
         }
+    } else {
+        // ---- L11\U11 to the front's pivot block --------------------------
+        for (int e = tid - 128; e < K * K; e += NT - 128) {
+            const int r = e / K, c = e - r * K;
+            G[(long long)r * fs + c] = P[r * kLinvLd + c];
+        }
     }
-    if (flag) atomicOr(sing + b, 1);
 }
 
 // U12 = L11^-1 A12 for one chunk of kXChunk columns of one (front, scenario):
@@ -1880,7 +1907,8 @@ void cross_group(const LuDevice& d, const int* fronts, int count, int max_fsz, c
                                      int(u12_lds<T>())));
         attr_set = true;
     }
-    hipLaunchKernelGGL(k_pivot<T>, dim3(unsigned(count) * unsigned(Bp)), dim3(64), pivot_lds<T>(), s, d,
+    hipLaunchKernelGGL(k_pivot<T>, dim3(unsigned(count) * unsigned(Bp)), dim3(pivot_threads<T>()),
+                       pivot_lds<T>(), s, d,
                        fronts, A, Bp, ar, uinv, pivot_eps, static_piv, sing);
     CUPF_HIP(hipGetLastError());
     const int nch = (max_fsz + kXChunk - 1) / kXChunk;
