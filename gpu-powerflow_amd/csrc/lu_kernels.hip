@@ -1018,9 +1018,11 @@ constexpr int kXCld = kXChunk + 4;            // chunk row stride (elements)
 constexpr int kUinvElems = kXMaxK * kXMaxK;   // U11^-1 (then L11^-1) per (front, scenario)
 constexpr int kSideElems = 2 * kUinvElems;    // side-buffer elements per (front, scenario)
 
+constexpr int kPivLd = 68;    // pivot-block row stride in LDS (16-byte aligned rows)
+
 template <typename T>
 constexpr size_t pivot_lds() {
-    return size_t(64 * kLinvLd) * sizeof(T);
+    return size_t(64 * kPivLd) * sizeof(T);
 }
 
 template <typename T>
@@ -1032,12 +1034,10 @@ constexpr size_t u12_lds() {
 // whose register rows take twice the space): A11 assembled in LDS by all of
 // them (entries, then the chunk map), then
 //   wave 0: LU with one row per lane in registers (the pivot row broadcast
-//           by v_readlane, guarded pivots), L11\U11 back to LDS; then
-//           L11^-1 with one column per lane by forward substitution, the
-//           rows of L11 broadcast by v_readlane from its own registers;
-//   wave 1: U11^-1 with one column per lane by backward substitution, the
-//           rows of U11 loaded from LDS into registers and broadcast the same
-//           way;
+//           by v_readlane, guarded pivots), L11\U11 written back to LDS
+//           transposed; then L11^-1 with one column per lane, right-looking
+//           (x_i -= L_ik x_k), each column of L11 read as a broadcast LDS row;
+//   wave 1: U11^-1 the same way, backward (y_k /= U_kk, y_i -= U_ik y_k);
 //   waves 2, 3 (FP64: wave 1 first): L11\U11 copied to the front's pivot
 //           block.
 // The inverses go to the per-(front, scenario) side buffer, row-major:
@@ -1051,7 +1051,7 @@ __global__ __launch_bounds__(pivot_threads<T>()) void k_pivot(LuDevice d, const 
                                                         T* __restrict__ side, double eps, int static_piv,
                                                         int* __restrict__ sing) {
     extern __shared__ unsigned char smem_raw[];
-    T* P = reinterpret_cast<T*>(smem_raw);                         // A11: [64][kLinvLd]
+    T* P = reinterpret_cast<T*>(smem_raw);                         // A11: [64][kPivLd], then transposed
     int fi, b;
     decode_block(Bp, fi, b);
     const int p = fronts[fi];
@@ -1062,15 +1062,15 @@ __global__ __launch_bounds__(pivot_threads<T>()) void k_pivot(LuDevice d, const 
     const int* rl = d.row_lb + d.lb_ptr[p];
 
     // ---- A11 -------------------------------------------------------------
-    for (int e = tid; e < 64 * kLinvLd; e += NT) P[e] = T(0);
+    for (int e = tid; e < 64 * kPivLd; e += NT) P[e] = T(0);
     __syncthreads();
     put_entries<NT>(P, d.ent_dst, d.ent_src, A, Bp, b, rl[0], rl[K], [=](int dst) {
         const int r = dst / fs, c = dst - r * fs;
-        return c < K ? r * kLinvLd + c : -1;
+        return c < K ? r * kPivLd + c : -1;
     });
     __syncthreads();
     assemble_chunks<NT, 16>(d, ar, d.asm_base[p * kAsmKindCount + kAsmKPivot], b, Bp, P,
-                            [=](int r, int c) { return r * kLinvLd + c; });
+                            [=](int r, int c) { return r * kPivLd + c; });
     __syncthreads();
 
     T* Ui = side + ((long long)fi * Bp + b) * kSideElems;         // U11^-1 [k][j]
@@ -1080,7 +1080,7 @@ __global__ __launch_bounds__(pivot_threads<T>()) void k_pivot(LuDevice d, const 
         // ---- LU of A11, lane = row ----------------------------------------
         int flag = 0;
 #pragma unroll
-        for (int c = 0; c < kXMaxK; ++c) row[c] = lane < K ? P[lane * kLinvLd + c] : T(0);
+        for (int c = 0; c < kXMaxK; ++c) row[c] = lane < K ? P[lane * kPivLd + c] : T(0);
 #pragma unroll
         for (int c = 0; c < kXMaxK; ++c) {
             if (c < K) {                              // uniform
@@ -1093,56 +1093,58 @@ __global__ __launch_bounds__(pivot_threads<T>()) void k_pivot(LuDevice d, const 
                 for (int cc = c + 1; cc < kXMaxK; ++cc) row[cc] -= l * bcast(row[cc], c);
             }
         }
-        if (lane < K) {
+        // transposed into the same LDS block (this wave's reads of it are
+        // done): PT[c][r] = (L\U)[r][c], zero outside K x K
 #pragma unroll
-            for (int c = 0; c < kXMaxK; ++c) P[lane * kLinvLd + c] = row[c];
-        }
+        for (int c = 0; c < kXMaxK; ++c) P[c * kPivLd + lane] = row[c];
         if (flag) atomicOr(sing + b, 1);
     }
     __syncthreads();
     if (wave == 0) {
-        // ---- L11^-1, lane = column j: x_i = delta_ij - sum_{k<i} L_ik x_k -------
+        // ---- L11^-1, lane = column j, right-looking: x_i -= L_ik x_k (i > k),
+        // column k of L read as a broadcast row of PT
         T x[kXMaxK];
 #pragma unroll
-        for (int i = 0; i < kXMaxK; ++i) {
-            T acc = lane == i ? T(1) : T(0);
-            if (i < K) {                              // uniform; rows >= K are the identity
+        for (int i = 0; i < kXMaxK; ++i) x[i] = lane == i ? T(1) : T(0);
 #pragma unroll
-                for (int k = 0; k < i; ++k) acc -= bcast(row[k], i) * x[k];
+        for (int k = 0; k < kXMaxK; ++k) {
+            if (k < K) {                              // uniform
+                const T xk = x[k];
+#pragma unroll
+                for (int i = k + 1; i < kXMaxK; ++i) x[i] -= P[k * kPivLd + i] * xk;
             }
-            x[i] = acc;
-            Lo[i * kXMaxK + lane] = acc;
         }
+#pragma unroll
+        for (int i = 0; i < kXMaxK; ++i) Lo[i * kXMaxK + lane] = x[i];
     } else if (wave == 1) {
         if constexpr (NT == 128) {
             // ---- L11\U11 to the front's pivot block ----------------------
             for (int e = lane; e < K * K; e += 64) {
                 const int r = e / K, c = e - r * K;
-                G[(long long)r * fs + c] = P[r * kLinvLd + c];
+                G[(long long)r * fs + c] = P[c * kPivLd + r];
             }
         }
-        // ---- U11^-1, lane = column j: y_i = (delta_ij - sum_{k>i} U_ik y_k) / U_ii
-#pragma unroll
-        for (int c = 0; c < kXMaxK; ++c)
-            row[c] = (lane < K && c >= lane && c < K) ? P[lane * kLinvLd + c] : T(0);
+        // ---- U11^-1, lane = column j, right-looking backward:
+        // y_k /= U_kk, then y_i -= U_ik y_k (i < k)
         T y[kXMaxK];
 #pragma unroll
-        for (int i = kXMaxK - 1; i >= 0; --i) {
-            T acc = lane == i ? T(1) : T(0);
-            if (i < K) {                              // uniform; rows >= K are the identity
+        for (int i = 0; i < kXMaxK; ++i) y[i] = lane == i ? T(1) : T(0);
 #pragma unroll
-                for (int k = i + 1; k < kXMaxK; ++k)
-                    if (k < K) acc -= bcast(row[k], i) * y[k];
-                acc /= bcast(row[i], i);
+        for (int k = kXMaxK - 1; k >= 0; --k) {
+            if (k < K) {                              // uniform
+                const T yk = y[k] / P[k * kPivLd + k];
+                y[k] = yk;
+#pragma unroll
+                for (int i = 0; i < k; ++i) y[i] -= P[k * kPivLd + i] * yk;
             }
-            y[i] = acc;
-            Ui[i * kXMaxK + lane] = acc;
         }
+#pragma unroll
+        for (int i = 0; i < kXMaxK; ++i) Ui[i * kXMaxK + lane] = y[i];
     } else {
         // ---- L11\U11 to the front's pivot block --------------------------
         for (int e = tid - 128; e < K * K; e += NT - 128) {
             const int r = e / K, c = e - r * K;
-            G[(long long)r * fs + c] = P[r * kLinvLd + c];
+            G[(long long)r * fs + c] = P[c * kPivLd + r];
         }
     }
 }
@@ -1786,13 +1788,35 @@ __global__ __launch_bounds__(NT) void k_fwd_w(LuDevice d, const int* __restrict_
             w[K + j] -= acc;
         }
     } else {
-        // w[r] -= L21[r] . w[0:K]: one row per lane
-        for (int r = K + tid; r < fs; r += NT) {
-            const T* row = G + (long long)r * fs;
-            T acc = T(0);
-#pragma unroll 8
-            for (int k = 0; k < K; ++k) acc += row[k] * w[k];
-            w[r] -= acc;
+        // w[r] -= L21[r] . w[0:K]: 16 lanes per row (4 pivot columns each),
+        // four rows per group in flight, partial sums combined by shuffles
+        const int sub = tid & 15;
+        T yv[4];
+        int kc[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int k = 4 * sub + i;
+            yv[i] = k < K ? w[k] : T(0);
+            kc[i] = min(k, K - 1);                 // clamped: every load in the row
+        }
+        constexpr int RP = NT / 16;                // rows per pass
+        for (int r0 = K + (tid >> 4); r0 < fs; r0 += 4 * RP) {   // uniform per 16 lanes
+            T v[4][4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const T* row = G + (long long)min(r0 + q * RP, fs - 1) * fs;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) v[q][i] = row[kc[i]];
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                T acc = v[q][0] * yv[0] + v[q][1] * yv[1] + v[q][2] * yv[2] + v[q][3] * yv[3];
+                acc += __shfl_xor(acc, 8, 16);
+                acc += __shfl_xor(acc, 4, 16);
+                acc += __shfl_xor(acc, 2, 16);
+                acc += __shfl_xor(acc, 1, 16);
+                if (sub == 0 && r0 + q * RP < fs) w[r0 + q * RP] -= acc;
+            }
         }
     }
     __syncthreads();
@@ -1826,20 +1850,42 @@ __global__ __launch_bounds__(NT) void k_bwd_w(LuDevice d, const int* __restrict_
             for (int r = K + wave; r < fs; r += NW) acc += G[(long long)r * fs + lane] * w[r];
         part[wave * kSolveK + lane] = acc;
     } else {
-        // w[k] -= U12[k] . w[K:fs]: one pivot row per lane, columns split over waves
-        T acc = T(0);
-        if (lane < K) {
-            const T* row = G + (long long)lane * fs + K;
-#pragma unroll 8
-            for (int j = wave; j < uc; j += NW) acc += row[j] * w[K + j];
+        // w[k] -= U12[k] . w[K:fs]: 16 lanes per pivot row, 4 columns per lane
+        // per step, 16 loads in flight, partial sums combined by shuffles
+        const int sub = tid & 15;
+        for (int k = tid >> 4; k < K; k += NT / 16) {          // uniform per 16 lanes
+            const T* row = G + (long long)k * fs + K;
+            T acc = T(0);
+            for (int j0 = 4 * sub; j0 < uc; j0 += 4 * 64) {
+                T v[4][4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) v[q][i] = row[min(j0 + 64 * q + i, uc - 1)];
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const int j = j0 + 64 * q + i;
+                        if (j < uc) acc += v[q][i] * w[K + j];
+                    }
+            }
+            acc += __shfl_xor(acc, 8, 16);
+            acc += __shfl_xor(acc, 4, 16);
+            acc += __shfl_xor(acc, 2, 16);
+            acc += __shfl_xor(acc, 1, 16);
+            if (sub == 0) part[k] = acc;
         }
-        part[wave * kSolveK + lane] = acc;
     }
     __syncthreads();
     if (wave == 0) {
         T wi = lane < K ? w[lane] : T(0);
+        if (TRANS) {
 #pragma unroll
-        for (int q = 0; q < NW; ++q) wi -= part[q * kSolveK + lane];
+            for (int q = 0; q < NW; ++q) wi -= part[q * kSolveK + lane];
+        } else {
+            wi -= lane < K ? part[lane] : T(0);       // part[k] written for k < K
+        }
         for (int k = K - 1; k >= 0; --k) {
             if (TRANS) {                                   // L11^T: unit upper
                 const T xk = bcast(wi, k);
