@@ -1030,20 +1030,19 @@ constexpr size_t u12_lds() {
     return size_t(64 * kLinvLd + 64 * kXCld) * sizeof(T);
 }
 
-// Pivot block of one (front, scenario), four waves (FP32) or two (FP64,
-// whose register rows take twice the space): A11 assembled in LDS by all of
-// them (entries, then the chunk map), then
+// Pivot block of one (front, scenario), four waves (two also work: the
+// block copy then goes to wave 1): A11 assembled in LDS by all of them
+// (entries, then the chunk map), then
 //   wave 0: LU with one row per lane in registers (the pivot row broadcast
-//           by v_readlane, guarded pivots), L11\U11 written back to LDS
+//           through LDS, guarded pivots), L11\U11 written back to LDS
 //           transposed; then L11^-1 with one column per lane, right-looking
 //           (x_i -= L_ik x_k), each column of L11 read as a broadcast LDS row;
 //   wave 1: U11^-1 the same way, backward (y_k /= U_kk, y_i -= U_ik y_k);
-//   waves 2, 3 (FP64: wave 1 first): L11\U11 copied to the front's pivot
-//           block.
+//   waves 2, 3: L11\U11 copied to the front's pivot block.
 // The inverses go to the per-(front, scenario) side buffer, row-major:
 // U11^-1 [k][j], then L11^-1 [i][k].
 template <typename T>
-constexpr int pivot_threads() { return sizeof(T) == 4 ? 256 : 128; }
+constexpr int pivot_threads() { return 256; }
 
 template <typename T>
 __global__ __launch_bounds__(pivot_threads<T>()) void k_pivot(LuDevice d, const int* __restrict__ fronts,
@@ -1081,16 +1080,23 @@ __global__ __launch_bounds__(pivot_threads<T>()) void k_pivot(LuDevice d, const 
         int flag = 0;
 #pragma unroll
         for (int c = 0; c < kXMaxK; ++c) row[c] = lane < K ? P[lane * kPivLd + c] : T(0);
+        // the pivot row of each step is published by its lane in LDS row 0
+        // (free once every row is in registers) and read back as broadcasts
+        T* R = P;
 #pragma unroll
         for (int c = 0; c < kXMaxK; ++c) {
             if (c < K) {                              // uniform
-                if (lane == c) row[c] = guard_pivot(row[c], eps, static_piv, flag);
-                const T piv = bcast(row[c], c);
+                if (lane == c) {
+                    row[c] = guard_pivot(row[c], eps, static_piv, flag);
+#pragma unroll
+                    for (int cc = c; cc < kXMaxK; ++cc) R[cc] = row[cc];
+                }
+                const T piv = R[c];
                 const bool act = lane > c && lane < K;
                 const T l = act ? row[c] / piv : T(0);
                 if (act) row[c] = l;
 #pragma unroll
-                for (int cc = c + 1; cc < kXMaxK; ++cc) row[cc] -= l * bcast(row[cc], c);
+                for (int cc = c + 1; cc < kXMaxK; ++cc) row[cc] -= l * R[cc];
             }
         }
         // transposed into the same LDS block (this wave's reads of it are
