@@ -592,6 +592,30 @@ LuPlan lower_plan(const Symbolic& s, int n, const int* rp, const int* ci, int ti
             }
         }
     }
+    // solve gather maps: per front position, the update-vector elements of
+    // the children that land there (child order), and the right-hand-side row
+    const int rows = L.front_ptr[P];
+    L.rhs_idx.assign(size_t(rows), 0);
+    for (int g = 0; g < rows; ++g) L.rhs_idx[size_t(g)] = L.perm[size_t(L.front_rows[size_t(g)])];
+    if (L.upd_total >= std::numeric_limits<int>::max())
+        throw std::runtime_error("lower_plan: update vectors too large");
+    L.gat_ptr.assign(size_t(rows) + 1, 0);
+    for (int p = 0; p < P; ++p)
+        for (int q = L.child_ptr[p]; q < L.child_ptr[p + 1]; ++q) {
+            const int c = L.child_idx[q];
+            const int* loc = L.asm_loc.data() + L.asm_ptr[c];
+            for (int a = 0; a < L.fsz[c] - L.nc[c]; ++a) ++L.gat_ptr[size_t(L.front_ptr[p] + loc[a]) + 1];
+        }
+    for (int g = 0; g < rows; ++g) L.gat_ptr[size_t(g) + 1] += L.gat_ptr[size_t(g)];
+    L.gat_src.assign(size_t(L.gat_ptr[size_t(rows)]), 0);
+    std::vector<int> cur(L.gat_ptr.begin(), L.gat_ptr.end() - 1);
+    for (int p = 0; p < P; ++p)
+        for (int q = L.child_ptr[p]; q < L.child_ptr[p + 1]; ++q) {
+            const int c = L.child_idx[q];
+            const int* loc = L.asm_loc.data() + L.asm_ptr[c];
+            for (int a = 0; a < L.fsz[c] - L.nc[c]; ++a)
+                L.gat_src[size_t(cur[size_t(L.front_ptr[p] + loc[a])]++)] = int(L.uoff[c] + a);
+        }
     return L;
 }
 

@@ -127,6 +127,9 @@ struct LuDevice {
     const AsmTaskDev* asm_tasks = nullptr;
     const int* asm_iptr = nullptr;    // row items of the Schur lists, 4 owner waves per list
     const int* asm_items = nullptr;
+    const int* gat_ptr = nullptr;     // solve gather maps (LuPlan gat_ptr / gat_src / rhs_idx)
+    const int* gat_src = nullptr;
+    const int* rhs_idx = nullptr;
     const int* asm_cptr = nullptr;    // chunk maps of the lists: (src, loc, rr, len) quads
     const int4* asm_chunks = nullptr;
 };

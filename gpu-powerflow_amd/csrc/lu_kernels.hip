@@ -1654,6 +1654,22 @@ __device__ inline void sweep_bwd_Lt(const T* __restrict__ G, int fs, int nc, T* 
     }
 }
 
+// Work vector of a front for the forward sweep: w[l] = rhs (pivot rows) plus
+// the children's update-vector elements landing on l (gather map, child
+// order); every position's loads are independent.
+template <int NT, typename T>
+__device__ inline void gather_w(const LuDevice& d, int p, int b, int Bp, int fs, int K,
+                                const double* __restrict__ rhs, const T* __restrict__ upd, T* w) {
+    const int g0 = d.front_ptr[p];
+    for (int l = int(threadIdx.x); l < fs; l += NT) {
+        const int g = g0 + l;
+        T acc = l < K ? T(rhs[(long long)d.rhs_idx[g] * Bp + b]) : T(0);
+        const int q1 = d.gat_ptr[g + 1];
+        for (int q = d.gat_ptr[g]; q < q1; ++q) acc += upd[(long long)d.gat_src[q] * Bp + b];
+        w[l] = acc;
+    }
+}
+
 template <typename T, bool TRANS>
 __global__ __launch_bounds__(kLanes) void k_fwd_t12(LuDevice d, const int* __restrict__ fronts,
                                                     Arenas<T> ar, const double* __restrict__ rhs,
@@ -1668,17 +1684,8 @@ __global__ __launch_bounds__(kLanes) void k_fwd_t12(LuDevice d, const int* __res
     const int fs = d.fsz[p], ncp = d.nc[p], uc = fs - ncp;
     const int* fr = d.front_rows + d.front_ptr[p];
     const T* G = ar.a1 + (long long)b * d.arena1 + d.off[p];
-    for (int l = lane; l < fs; l += kLanes)
-        w[l] = (l < ncp) ? T(rhs[(long long)d.perm[fr[l]] * Bp + b]) : T(0);
+    gather_w<kLanes>(d, p, b, Bp, fs, ncp, rhs, upd, w);
     __syncthreads();
-    for (int q = d.child_ptr[p]; q < d.child_ptr[p + 1]; ++q) {
-        const int c = d.child_idx[q];
-        const int cuc = d.fsz[c] - d.nc[c];
-        const int* loc = d.asm_loc + d.asm_ptr[c];
-        const T* u = upd + d.uoff[c] * Bp + b;
-        for (int a = lane; a < cuc; a += kLanes) w[loc[a]] += u[(long long)a * Bp];
-        __syncthreads();
-    }
     if (TRANS) sweep_fwd_Ut(G, fs, ncp, w, lane);
     else sweep_fwd_L(G, fs, ncp, w, lane);
     for (int k = lane; k < ncp; k += kLanes) y[(long long)fr[k] * Bp + b] = w[k];
@@ -1718,37 +1725,33 @@ __global__ __launch_bounds__(kLanes) void k_bwd_t12(LuDevice d, const int* __res
 // front with one output per lane (L21 w: a row per lane; U12 w and L21^T w:
 // a pivot row per lane, the columns split over the NT / 64 waves and the
 // partial sums combined in LDS; U12^T w: lanes along the outputs).
-// Children's update vectors are gathered one assembly stage at a time
-// (children of a stage have disjoint rows).  Used for FP32 fronts; FP64
-// fronts keep the one-wave sweeps below, which measure faster there.
+// The pivot block's loads are issued before the work vector is gathered
+// (gather map: every position's loads independent).  Used for FP32 fronts;
+// FP64 fronts keep the one-wave sweeps above, which measure faster there.
 // ===========================================================================
 constexpr int kSolveK = 64;
 constexpr int kSolvePld = kSolveK + 1;          // pivot-block row stride in LDS
 
-// Pivot block of G (K x K, row stride fs) into LDS P[K][kSolvePld].
+// Pivot block of G (K x K, row stride fs) into registers (element
+// e = tid + u * NT of the 64 x 64 block, clamped loads), then into LDS
+// P[K][kSolvePld].
 template <int NT, typename T>
-__device__ inline void stage_pivot(const T* __restrict__ G, int fs, int K, T* P) {
-    for (int e = threadIdx.x; e < K * kSolveK; e += NT) {
-        const int r = e / kSolveK, c = e - (e / kSolveK) * kSolveK;
-        if (c < K) P[r * kSolvePld + c] = G[(long long)r * fs + c];
+__device__ inline void load_pivot(const T* __restrict__ G, int fs, int K, T* pv) {
+#pragma unroll
+    for (int u = 0; u < kSolveK * kSolveK / NT; ++u) {
+        const int e = int(threadIdx.x) + u * NT;
+        const int r = min(e / kSolveK, K - 1), c = min(e % kSolveK, K - 1);
+        pv[u] = G[(long long)r * fs + c];
     }
 }
 
-// Gather of the children's update vectors into w, one stage at a time.
 template <int NT, typename T>
-__device__ inline void gather_children(const LuDevice& d, int p, int b, int Bp,
-                                       const T* __restrict__ upd, T* w) {
-    const int* st = d.cstage + d.cstage_ptr[p];
-    const int ns = d.cstage_ptr[p + 1] - d.cstage_ptr[p] - 1;
-    for (int sg = 0; sg < ns; ++sg) {
-        for (int q = st[sg]; q < st[sg + 1]; ++q) {
-            const int c = d.child_idx[q];
-            const int cuc = d.fsz[c] - d.nc[c];
-            const int* loc = d.asm_loc + d.asm_ptr[c];
-            const T* u = upd + d.uoff[c] * Bp + b;
-            for (int a = threadIdx.x; a < cuc; a += NT) w[loc[a]] += u[(long long)a * Bp];
-        }
-        __syncthreads();
+__device__ inline void store_pivot(int K, const T* pv, T* P) {
+#pragma unroll
+    for (int u = 0; u < kSolveK * kSolveK / NT; ++u) {
+        const int e = int(threadIdx.x) + u * NT;
+        const int r = e / kSolveK, c = e % kSolveK;
+        if (r < K && c < K) P[r * kSolvePld + c] = pv[u];
     }
 }
 
@@ -1766,11 +1769,11 @@ __global__ __launch_bounds__(NT) void k_fwd_w(LuDevice d, const int* __restrict_
     const int fs = d.fsz[p], K = d.nc[p], uc = fs - K;
     const int* fr = d.front_rows + d.front_ptr[p];
     const T* G = ar.a1 + (long long)b * d.arena1 + d.off[p];
-    for (int l = tid; l < fs; l += NT)
-        w[l] = (l < K) ? T(rhs[(long long)d.perm[fr[l]] * Bp + b]) : T(0);
-    stage_pivot<NT>(G, fs, K, P);
+    T pv[kSolveK * kSolveK / NT];
+    load_pivot<NT>(G, fs, K, pv);                          // in flight during the gather
+    gather_w<NT>(d, p, b, Bp, fs, K, rhs, upd, w);
+    store_pivot<NT>(K, pv, P);
     __syncthreads();
-    gather_children<NT>(d, p, b, Bp, upd, w);
     if (wave == 0) {
         T wi = lane < K ? w[lane] : T(0);
         for (int k = 0; k < K; ++k) {
@@ -1846,8 +1849,10 @@ __global__ __launch_bounds__(NT) void k_bwd_w(LuDevice d, const int* __restrict_
     const int fs = d.fsz[p], K = d.nc[p], uc = fs - K;
     const int* fr = d.front_rows + d.front_ptr[p];
     const T* G = ar.a1 + (long long)b * d.arena1 + d.off[p];
+    T pv[kSolveK * kSolveK / NT];
+    load_pivot<NT>(G, fs, K, pv);
     for (int l = tid; l < fs; l += NT) w[l] = y[(long long)fr[l] * Bp + b];
-    stage_pivot<NT>(G, fs, K, P);
+    store_pivot<NT>(K, pv, P);
     __syncthreads();
     if (TRANS) {
         // w[k] -= sum_r L21[r][k] w[r]: lanes along k, rows split over waves

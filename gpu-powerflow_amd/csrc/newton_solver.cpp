@@ -120,7 +120,7 @@ struct NewtonSolver::Impl {
     DeviceBuffer d_fsz, d_nc, d_tier, d_off, d_uoff, d_fptr, d_frows, d_cptr, d_cidx, d_aptr,
         d_aloc, d_eptr, d_esrc, d_edst, d_perm, d_groups, d_lbptr, d_rowlb, d_abase, d_atptr,
         d_asptr, d_astage, d_atasks, d_cstptr, d_cstage, d_aiptr, d_aitems,
-        d_acptr, d_achunks;
+        d_acptr, d_achunks, d_gptr, d_gsrc, d_ridx;
     kern::NrDevice nr;
     kern::LuDevice lu;
 
@@ -737,6 +737,9 @@ void NewtonSolver::initialize(const YbusView& ybus, const int32_t* pv, int32_t n
         m.d_atasks = upload_vector(al.tasks, s);
         m.d_aiptr = upload_vector(al.item_ptr, s);
         m.d_aitems = upload_vector(al.items, s);
+        m.d_gptr = upload_vector(P.gat_ptr, s);
+        m.d_gsrc = upload_vector(P.gat_src, s);
+        m.d_ridx = upload_vector(P.rhs_idx, s);
         m.d_acptr = upload_vector(al.chunk_ptr, s);
         m.d_achunks = upload_vector(al.chunks, s);
     }
@@ -768,6 +771,7 @@ void NewtonSolver::initialize(const YbusView& ybus, const int32_t* pv, int32_t n
     lu.asm_tasks = m.d_atasks.as<kern::AsmTaskDev>();
     lu.asm_iptr = m.d_aiptr.as<int>(); lu.asm_items = m.d_aitems.as<int>();
     lu.asm_cptr = m.d_acptr.as<int>(); lu.asm_chunks = m.d_achunks.as<int4>();
+    lu.gat_ptr = m.d_gptr.as<int>(); lu.gat_src = m.d_gsrc.as<int>(); lu.rhs_idx = m.d_ridx.as<int>();
 
     m.capBp = 0;   // batch buffers are sized lazily for the new structure
     m.capB = 0;
