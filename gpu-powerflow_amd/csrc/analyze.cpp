@@ -436,7 +436,8 @@ LuPlan lower_plan(const Symbolic& s, int n, const int* rp, const int* ci, int ti
         const int f = s.front_ptr[p + 1] - s.front_ptr[p];
         L.fsz[p] = f;
         L.nc[p] = s.panel_ncols[p];
-        L.tier[p] = f <= tier0_max ? 0 : (f <= tier1_max ? 1 : 2);
+        // fronts with many pivots leave tier 1 (its per-pivot barriers)
+        L.tier[p] = f <= tier0_max ? 0 : (f <= tier1_max && L.nc[p] <= kTier1MaxPivots ? 1 : 2);
         const long long f2 = (long long)f * f;
         if (L.tier[p] == 0) { L.off[p] = L.arena0; L.arena0 += f2; }
         else { L.off[p] = L.arena1; L.arena1 += f2; }

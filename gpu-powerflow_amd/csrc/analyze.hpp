@@ -79,13 +79,15 @@ Symbolic analyze_symbolic(int n, const int* rp, const int* ci, const std::vector
 //   tier 0 (fsz <= tier0_max): one lane per scenario; the front lives in a
 //          scenario-interleaved arena, element e of scenario b at
 //          (off + e) * Bp + b, Bp the batch padded to 64;
-//   tier 1 (fsz <= tier1_max): one workgroup per (front, scenario), the
-//          front assembled and factored in LDS, stored scenario-major at
-//          b * arena1 + off + e;
+//   tier 1 (fsz <= tier1_max, at most kTier1MaxPivots pivots): one
+//          workgroup per (front, scenario), the front assembled and factored
+//          in LDS, stored scenario-major at b * arena1 + off + e;
 //   tier 2 (larger): one workgroup per (front, scenario), blocked dense LU
 //          in the scenario-major arena.
 // Every front is a dense fsz x fsz row-major block.
 // ---------------------------------------------------------------------------
+constexpr int kTier1MaxPivots = 32;
+
 struct LuPlan {
     int n = 0, nnz = 0, num_fronts = 0, num_levels = 0;
     int tier0_max = 16, tier1_max = 128;
