@@ -134,6 +134,9 @@ struct LuDevice {
     const int4* asm_chunks = nullptr;
 };
 
+// Most pivots of a tier-1 front (the tier-1 factor kernel's register rows).
+constexpr int kT1MaxK = 32;
+
 // Block height of the row-block assembly of a tier-2 group (elem = sizeof T).
 int assemble_rows_block(int max_fsz, int elem);
 

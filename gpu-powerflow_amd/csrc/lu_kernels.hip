@@ -422,11 +422,14 @@ __device__ inline void assemble_front(const LuDevice& d, const Arenas<T>& ar, co
     __syncthreads();
 }
 
-// Tier 1: the front is assembled and factored in LDS.  The K pivot columns
-// and rows (the cross) are factored right-looking (column scaled by the
-// guarded pivot, then only the cross updated), and the contribution block
-// gets its rank-K update C -= L21 U12 once at the end, in 4 x 4 register
-// tiles.  The front is then written to the arena once.
+// Tier 1: the front (fsz <= 128, at most kT1MaxK pivots) is assembled and
+// factored in LDS: wave 0 factors the pivot block with one row per lane in
+// registers (the pivot row broadcast through LDS, guarded pivots); then,
+// at once, threads 0..127 form U12 = L11^-1 A12 (a column each) and threads
+// 128..255 L21 = A21 U11^-1 (a row each); the contribution block gets its
+// rank-K update C -= L21 U12 in 4 x 4 register tiles, and the front is
+// written to the arena once.
+
 template <typename T>
 __global__ __launch_bounds__(kFactorBlock) void k_factor_t1(LuDevice d,
                                                             const int* __restrict__ fronts,
@@ -434,44 +437,84 @@ __global__ __launch_bounds__(kFactorBlock) void k_factor_t1(LuDevice d,
                                                             Arenas<T> ar, double eps,
                                                             int static_piv,
                                                             int* __restrict__ sing) {
+    static_assert(kFactorBlock == 256, "two 128-thread halves for the triangular solves");
     extern __shared__ unsigned char smem_raw[];
     T* F = reinterpret_cast<T*>(smem_raw);
+    __shared__ T R[kT1MaxK];                    // published pivot row
     int fi, b;
     decode_block(Bp, fi, b);
     const int p = fronts[fi];
     const int fs = d.fsz[p], ncp = d.nc[p];
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63;
     assemble_front(d, ar, A, Bp, p, b, fs, F);
     __syncthreads();
-    int flag = 0;
-    const int wc = fs - ncp;                    // contribution-block width
-    for (int k = 0; k < ncp; ++k) {
-        const T piv = F[k * fs + k];
-        const T g = guard_pivot(piv, eps, static_piv, flag);
-        __syncthreads();
-        if (tid == 0 && g != piv) F[k * fs + k] = g;
-        const int m = fs - k - 1;
-        for (int i = tid; i < m; i += kFactorBlock) F[(k + 1 + i) * fs + k] /= g;
-        __syncthreads();
-        // the cross: (a) rows > k x pivot columns (k, ncp); (b) pivot rows
-        // (k, ncp) x contribution columns [ncp, fs)
-        const int w = ncp - k - 1;
-        const int na = m * w, nt = na + w * wc;
-        for (int e = tid; e < nt; e += kFactorBlock) {
-            int i, j;
-            if (e < na) {
-                i = k + 1 + e / w;
-                j = k + 1 + (e - (e / w) * w);
-            } else {
-                const int e2 = e - na;
-                i = k + 1 + e2 / wc;
-                j = ncp + (e2 - (e2 / wc) * wc);
+    if (tid < 64) {
+        // ---- LU of the pivot block, lane = row ----------------------------
+        int flag = 0;
+        T row[kT1MaxK];
+#pragma unroll
+        for (int c = 0; c < kT1MaxK; ++c) row[c] = (lane < ncp && c < ncp) ? F[lane * fs + c] : T(0);
+#pragma unroll
+        for (int c = 0; c < kT1MaxK; ++c) {
+            if (c < ncp) {                            // uniform
+                if (lane == c) {
+                    row[c] = guard_pivot(row[c], eps, static_piv, flag);
+#pragma unroll
+                    for (int cc = c; cc < kT1MaxK; ++cc) R[cc] = row[cc];
+                }
+                const T piv = R[c];
+                const bool act = lane > c && lane < ncp;
+                const T l = act ? row[c] / piv : T(0);
+                if (act) row[c] = l;
+#pragma unroll
+                for (int cc = c + 1; cc < kT1MaxK; ++cc) row[cc] -= l * R[cc];
             }
-            F[i * fs + j] -= F[i * fs + k] * F[k * fs + j];
         }
-        __syncthreads();
+        if (lane < ncp) {
+#pragma unroll
+            for (int c = 0; c < kT1MaxK; ++c)
+                if (c < ncp) F[lane * fs + c] = row[c];
+        }
+        if (flag) atomicOr(sing + b, 1);
     }
+    __syncthreads();
+    if (tid < 128) {
+        // ---- U12 column j: x_i = A12[i][j] - sum_{k<i} L_ik x_k ------------
+        const int j = ncp + tid;
+        if (j < fs) {
+            T x[kT1MaxK];
+#pragma unroll
+            for (int i = 0; i < kT1MaxK; ++i) {
+                if (i < ncp) {                        // uniform
+                    T acc = F[i * fs + j];
+#pragma unroll
+                    for (int k = 0; k < i; ++k) acc -= F[i * fs + k] * x[k];
+                    x[i] = acc;
+                    F[i * fs + j] = acc;
+                }
+            }
+        }
+    } else {
+        // ---- L21 row r: x_j = (A21[r][j] - sum_{k<j} x_k U_kj) / U_jj ------
+        const int r = ncp + tid - 128;
+        if (r < fs) {
+            T x[kT1MaxK];
+#pragma unroll
+            for (int jj = 0; jj < kT1MaxK; ++jj) {
+                if (jj < ncp) {                       // uniform
+                    T acc = F[r * fs + jj];
+#pragma unroll
+                    for (int k = 0; k < jj; ++k) acc -= x[k] * F[k * fs + jj];
+                    acc /= F[jj * fs + jj];
+                    x[jj] = acc;
+                    F[r * fs + jj] = acc;
+                }
+            }
+        }
+    }
+    __syncthreads();
     // contribution block: C -= L21 U12 (rank ncp), 4 x 4 tiles per thread
+    const int wc = fs - ncp;
     const int tn = (wc + 3) / 4;
     for (int t = tid; t < tn * tn; t += kFactorBlock) {
         const int ti = t / tn, tj = t - (t / tn) * tn;
@@ -502,7 +545,6 @@ __global__ __launch_bounds__(kFactorBlock) void k_factor_t1(LuDevice d,
     __syncthreads();
     T* G = ar.a1 + (long long)b * d.arena1 + d.off[p];
     for (int e = tid; e < fs * fs; e += kFactorBlock) G[e] = F[e];
-    if (flag) atomicOr(sing + b, 1);
 }
 
 template <typename T>

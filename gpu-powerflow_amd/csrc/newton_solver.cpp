@@ -642,6 +642,7 @@ void NewtonSolver::initialize(const YbusView& ybus, const int32_t* pv, int32_t n
     m.sym = analyze_symbolic(m.js.dim, m.js.row_ptr.data(), m.js.col_idx.data(), perm,
                              m.opt.custom.max_panel_width);
     // tier 1 holds a whole front in 64 KiB of LDS
+    static_assert(kern::kT1MaxK >= kTier1MaxPivots, "tier-1 fronts fit the tier-1 factor kernel");
     const int tier1_max = m.front64 ? 90 : 128;
     m.plan = lower_plan(m.sym, m.js.dim, m.js.row_ptr.data(), m.js.col_idx.data(), 16, tier1_max);
     const int lds_max = m.front64 ? 8192 : 16384;
