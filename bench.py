@@ -82,12 +82,12 @@ def calibrate_copy(torch, mib):
 def dense_roofline(stats, stage, B, front_bytes):
     """Roofline of the timed dense stage from live HIP-event times around each
     of its launches on the solver's stream.  Per scenario (the solver's
-    stats()["dense_flops"/"dense_bytes"]): for the FP32 Schur kernel, flops
-    2 uc^2 K + 2 uc K^2 and bytes = the rows >= K of each tier-2 front written
-    once + the child contribution elements landing there read once + U12 read
-    once (4 B each) + the J entries of those rows (8 B); for the FP64 tier-2
-    groups, the LU flops and each front element written and read once plus
-    its J entries.  Per launch = that x B / the launches per factorisation."""
+    stats()["dense_flops"/"dense_bytes"]): for the Schur kernel of the pivot +
+    Schur groups, flops 2 uc^2 K + 2 uc K^2 and bytes = the rows >= K of each
+    such front written once + the child contribution elements landing there
+    read once + U12 read once (4 B each in FP32, 8 B in FP64) + the J entries
+    of those rows (8 B).  Per launch = that x B / the launches per
+    factorisation."""
     ms, launches = stage
     groups = max(1, stats["dense_groups"])
     if launches == 0:
@@ -254,9 +254,8 @@ def main():
     head = "learned" if "learned" in res else ("mixed" if "mixed" in res else forms[0])
     h = res[head]
     workload = f"{args.workload}x{B}_cap{args.cap}"
-    # the timed dense stage: the Schur kernel of the FP32 pivot + Schur groups,
-    # the whole assembly + dense LU of the FP64 tier-2 groups
-    kernel = {"fp64": "k_assemble_rows<double>+k_factor_t2<double>", "mixed": "k_schur<1>",
+    # the timed dense stage: the Schur kernel of the pivot + Schur groups
+    kernel = {"fp64": "k_schur<double,0>", "mixed": "k_schur<1>",
               "bf16x3": "k_schur<2>", "learned": "k_schur<1>"}[head]
     roof = dense_roofline(h["stats"], h["stages"]["factor_dense"], B, 8 if head == "fp64" else 4)
     if roof is not None:
