@@ -1774,26 +1774,28 @@ __global__ __launch_bounds__(kLanes) void k_bwd_t12(LuDevice d, const int* __res
 constexpr int kSolveK = 64;
 constexpr int kSolvePld = kSolveK + 1;          // pivot-block row stride in LDS
 
-// Pivot block of G (K x K, row stride fs) into registers (element
-// e = tid + u * NT of the 64 x 64 block, clamped loads), then into LDS
-// P[K][kSolvePld].
+// Pivot block of G (K x K, row stride fs) into registers, one row per wave
+// and instruction (row r = wave + waves * u, a lane per column, clamped),
+// then into LDS P[K][kSolvePld].
 template <int NT, typename T>
 __device__ inline void load_pivot(const T* __restrict__ G, int fs, int K, T* pv) {
+    constexpr int NW = NT / 64;
+    const int lane = int(threadIdx.x) & 63, wave = int(threadIdx.x) >> 6;
 #pragma unroll
-    for (int u = 0; u < kSolveK * kSolveK / NT; ++u) {
-        const int e = int(threadIdx.x) + u * NT;
-        const int r = min(e / kSolveK, K - 1), c = min(e % kSolveK, K - 1);
-        pv[u] = G[(long long)r * fs + c];
+    for (int u = 0; u < kSolveK / NW; ++u) {
+        const int r = wave + NW * u;
+        if (r < K) pv[u] = G[(long long)r * fs + min(lane, K - 1)];   // wave-uniform branch
     }
 }
 
 template <int NT, typename T>
 __device__ inline void store_pivot(int K, const T* pv, T* P) {
+    constexpr int NW = NT / 64;
+    const int lane = int(threadIdx.x) & 63, wave = int(threadIdx.x) >> 6;
 #pragma unroll
-    for (int u = 0; u < kSolveK * kSolveK / NT; ++u) {
-        const int e = int(threadIdx.x) + u * NT;
-        const int r = e / kSolveK, c = e % kSolveK;
-        if (r < K && c < K) P[r * kSolvePld + c] = pv[u];
+    for (int u = 0; u < kSolveK / NW; ++u) {
+        const int r = wave + NW * u;
+        if (r < K && lane < K) P[r * kSolvePld + lane] = pv[u];
     }
 }
 
@@ -1811,7 +1813,7 @@ __global__ __launch_bounds__(NT) void k_fwd_w(LuDevice d, const int* __restrict_
     const int fs = d.fsz[p], K = d.nc[p], uc = fs - K;
     const int* fr = d.front_rows + d.front_ptr[p];
     const T* G = ar.a1 + (long long)b * d.arena1 + d.off[p];
-    T pv[kSolveK * kSolveK / NT];
+    T pv[kSolveK * 64 / NT];
     load_pivot<NT>(G, fs, K, pv);                          // in flight during the gather
     gather_w<NT>(d, p, b, Bp, fs, K, rhs, upd, w);
     store_pivot<NT>(K, pv, P);
@@ -1891,7 +1893,7 @@ __global__ __launch_bounds__(NT) void k_bwd_w(LuDevice d, const int* __restrict_
     const int fs = d.fsz[p], K = d.nc[p], uc = fs - K;
     const int* fr = d.front_rows + d.front_ptr[p];
     const T* G = ar.a1 + (long long)b * d.arena1 + d.off[p];
-    T pv[kSolveK * kSolveK / NT];
+    T pv[kSolveK * 64 / NT];
     load_pivot<NT>(G, fs, K, pv);
     for (int l = tid; l < fs; l += NT) w[l] = y[(long long)fr[l] * Bp + b];
     store_pivot<NT>(K, pv, P);
