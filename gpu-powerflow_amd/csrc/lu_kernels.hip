@@ -1076,7 +1076,7 @@ constexpr size_t u12_lds() {
 // block copy then goes to wave 1): A11 assembled in LDS by all of them
 // (entries, then the chunk map), then
 //   wave 0: LU with one row per lane in registers (the pivot row broadcast
-//           through LDS, guarded pivots), L11\U11 written back to LDS
+//           by v_readlane, guarded pivots), L11\U11 written back to LDS
 //           transposed; then L11^-1 with one column per lane, right-looking
 //           (x_i -= L_ik x_k), each column of L11 read as a broadcast LDS row;
 //   wave 1: U11^-1 the same way, backward (y_k /= U_kk, y_i -= U_ik y_k);
@@ -1122,23 +1122,16 @@ __global__ __launch_bounds__(pivot_threads<T>()) void k_pivot(LuDevice d, const 
         int flag = 0;
 #pragma unroll
         for (int c = 0; c < kXMaxK; ++c) row[c] = lane < K ? P[lane * kPivLd + c] : T(0);
-        // the pivot row of each step is published by its lane in LDS row 0
-        // (free once every row is in registers) and read back as broadcasts
-        T* R = P;
 #pragma unroll
         for (int c = 0; c < kXMaxK; ++c) {
             if (c < K) {                              // uniform
-                if (lane == c) {
-                    row[c] = guard_pivot(row[c], eps, static_piv, flag);
-#pragma unroll
-                    for (int cc = c; cc < kXMaxK; ++cc) R[cc] = row[cc];
-                }
-                const T piv = R[c];
+                if (lane == c) row[c] = guard_pivot(row[c], eps, static_piv, flag);
+                const T piv = bcast(row[c], c);
                 const bool act = lane > c && lane < K;
                 const T l = act ? row[c] / piv : T(0);
                 if (act) row[c] = l;
 #pragma unroll
-                for (int cc = c + 1; cc < kXMaxK; ++cc) row[cc] -= l * R[cc];
+                for (int cc = c + 1; cc < kXMaxK; ++cc) row[cc] -= l * bcast(row[cc], c);
             }
         }
         // transposed into the same LDS block (this wave's reads of it are
@@ -1703,12 +1696,30 @@ template <int NT, typename T>
 __device__ inline void gather_w(const LuDevice& d, int p, int b, int Bp, int fs, int K,
                                 const double* __restrict__ rhs, const T* __restrict__ upd, T* w) {
     const int g0 = d.front_ptr[p];
-    for (int l = int(threadIdx.x); l < fs; l += NT) {
-        const int g = g0 + l;
-        T acc = l < K ? T(rhs[(long long)d.rhs_idx[g] * Bp + b]) : T(0);
-        const int q1 = d.gat_ptr[g + 1];
-        for (int q = d.gat_ptr[g]; q < q1; ++q) acc += upd[(long long)d.gat_src[q] * Bp + b];
-        w[l] = acc;
+    // two positions per lane at a time, each with its first two
+    // contributions loaded together (most positions have at most two)
+    for (int l0 = int(threadIdx.x); l0 < fs; l0 += 2 * NT) {
+        T acc[2], v[2][2];
+        int q0[2], q1[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int l = min(l0 + h * NT, fs - 1), g = g0 + l;
+            acc[h] = l < K ? T(rhs[(long long)d.rhs_idx[g] * Bp + b]) : T(0);
+            q0[h] = d.gat_ptr[g];
+            q1[h] = d.gat_ptr[g + 1];
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                const int q = q0[h] + t;
+                v[h][t] = q < q1[h] ? upd[(long long)d.gat_src[q] * Bp + b] : T(0);
+            }
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            T a = acc[h] + v[h][0];
+            a += v[h][1];
+            for (int q = q0[h] + 2; q < q1[h]; ++q) a += upd[(long long)d.gat_src[q] * Bp + b];
+            if (l0 + h * NT < fs) w[l0 + h * NT] = a;
+        }
     }
 }
 
@@ -1799,6 +1810,39 @@ __device__ inline void store_pivot(int K, const T* pv, T* P) {
     }
 }
 
+// w[r] -= L21[r] . w[0:K] for the rows r in [K, fs) (K <= 4 GL): GL lanes
+// per row with 4 pivot columns each, four rows per lane group in flight,
+// partial sums combined by shuffles.
+template <int GL, int NT, typename T>
+__device__ inline void l21_update(const T* __restrict__ G, int fs, int K, T* w) {
+    const int tid = threadIdx.x, sub = tid & (GL - 1);
+    T yv[4];
+    int kc[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int k = 4 * sub + i;
+        yv[i] = k < K ? w[k] : T(0);
+        kc[i] = min(k, K - 1);                     // clamped: every load in the row
+    }
+    constexpr int RP = NT / GL;                    // rows per pass
+    for (int r0 = K + tid / GL; r0 < fs; r0 += 4 * RP) {   // uniform per lane group
+        T v[4][4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const T* row = G + (long long)min(r0 + q * RP, fs - 1) * fs;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) v[q][i] = row[kc[i]];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            T acc = v[q][0] * yv[0] + v[q][1] * yv[1] + v[q][2] * yv[2] + v[q][3] * yv[3];
+#pragma unroll
+            for (int m = GL / 2; m >= 1; m /= 2) acc += __shfl_xor(acc, m, GL);
+            if (sub == 0 && r0 + q * RP < fs) w[r0 + q * RP] -= acc;
+        }
+    }
+}
+
 template <typename T, bool TRANS, int NT>
 __global__ __launch_bounds__(NT) void k_fwd_w(LuDevice d, const int* __restrict__ fronts,
                                               Arenas<T> ar, const double* __restrict__ rhs,
@@ -1841,36 +1885,11 @@ __global__ __launch_bounds__(NT) void k_fwd_w(LuDevice d, const int* __restrict_
             w[K + j] -= acc;
         }
     } else {
-        // w[r] -= L21[r] . w[0:K]: 16 lanes per row (4 pivot columns each),
-        // four rows per group in flight, partial sums combined by shuffles
-        const int sub = tid & 15;
-        T yv[4];
-        int kc[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int k = 4 * sub + i;
-            yv[i] = k < K ? w[k] : T(0);
-            kc[i] = min(k, K - 1);                 // clamped: every load in the row
-        }
-        constexpr int RP = NT / 16;                // rows per pass
-        for (int r0 = K + (tid >> 4); r0 < fs; r0 += 4 * RP) {   // uniform per 16 lanes
-            T v[4][4];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const T* row = G + (long long)min(r0 + q * RP, fs - 1) * fs;
-#pragma unroll
-                for (int i = 0; i < 4; ++i) v[q][i] = row[kc[i]];
-            }
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                T acc = v[q][0] * yv[0] + v[q][1] * yv[1] + v[q][2] * yv[2] + v[q][3] * yv[3];
-                acc += __shfl_xor(acc, 8, 16);
-                acc += __shfl_xor(acc, 4, 16);
-                acc += __shfl_xor(acc, 2, 16);
-                acc += __shfl_xor(acc, 1, 16);
-                if (sub == 0 && r0 + q * RP < fs) w[r0 + q * RP] -= acc;
-            }
-        }
+        // w[r] -= L21[r] . w[0:K]: 4, 8 or 16 lanes per row (4 pivot columns
+        // each, as K needs)
+        if (K <= 16) l21_update<4, NT>(G, fs, K, w);
+        else if (K <= 32) l21_update<8, NT>(G, fs, K, w);
+        else l21_update<16, NT>(G, fs, K, w);
     }
     __syncthreads();
     for (int k = tid; k < K; k += NT) y[(long long)fr[k] * Bp + b] = w[k];
