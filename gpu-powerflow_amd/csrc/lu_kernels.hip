@@ -1321,6 +1321,21 @@ __global__ __launch_bounds__(kSchurThreads, 2) void k_schur(LuDevice d, const in
     const int* rl = d.row_lb + d.lb_ptr[p];
     const T* Ui = uinv + ((long long)fi * Bp + b) * kSideElems;
 
+    // U11^-1 operands of the L21 tiles (waves 0 and 1, MFMA modes), in flight
+    // during the assembly: B-operand k = 2 i + lane / 32, column jc
+    float ub[kXMaxK / 2];
+    const int jc = 32 * wave + (lane & 31);              // L21 column of this lane
+    const bool l21w = MODE != kMfmaNone && wave < 2 && 32 * wave < K;   // waves forming L21
+    if constexpr (MODE != kMfmaNone) {
+        if (l21w) {
+#pragma unroll
+            for (int i = 0; i < kXMaxK / 2; ++i) {
+                const int k = 2 * i + (lane >> 5);
+                ub[i] = k < 32 * wave + 32 ? float(Ui[k * kXMaxK + jc]) : 0.f;   // U11^-1 upper
+            }
+        }
+    }
+
     // ---- rows [i0, i1) of A21 (-> Lb) and A22 (-> Ab) -----------------------
     for (int e = tid; e < RB * (kSLd + uc); e += kSchurThreads) Lb[e] = T(0);
     const int lo = rl[i0], hi = rl[i1];
@@ -1401,8 +1416,6 @@ __global__ __launch_bounds__(kSchurThreads, 2) void k_schur(LuDevice d, const in
                 b1[i] = (k < K && okb) ? U[(long long)k * fs + 32] : 0.f;
             }
         };
-        const bool l21w = wave < 2 && 32 * wave < K;          // waves forming L21
-        const int jc = 32 * wave + (lane & 31);              // L21 column of this lane
 
         // the first tile's U12 operands load while L21 is formed
         if (wave < ntile) load_u(wave);
@@ -1413,12 +1426,6 @@ __global__ __launch_bounds__(kSchurThreads, 2) void k_schur(LuDevice d, const in
 #pragma unroll
             for (int v = 0; v < 16; ++v) acc[v] = 0.f;
             if (l21w) {                                   // wave-uniform
-                float ub[kXMaxK / 2];
-#pragma unroll
-                for (int i = 0; i < kXMaxK / 2; ++i) {
-                    const int k = 2 * i + (lane >> 5);
-                    ub[i] = k < 32 * wave + 32 ? Ui[k * kXMaxK + jc] : 0.f;   // U11^-1 upper
-                }
 #pragma unroll
                 for (int i = 0; i < kXMaxK / 2; ++i) {
                     if (2 * i < 32 * wave + 32) {         // U11^-1 upper: k <= j
