@@ -459,6 +459,122 @@ int cupf_plan_figures(int32_t n, const int32_t* row_ptr, const int32_t* col_idx,
     });
 }
 
+int cupf_plan_maps_check(int32_t n, const int32_t* row_ptr, const int32_t* col_idx,
+                         const int32_t* perm, int32_t cap, int64_t* out8) {
+    return guarded([&] {
+        if (!row_ptr || !col_idx || !perm || !out8) throw std::invalid_argument("null argument");
+        const std::vector<int> pv(perm, perm + n);
+        const cupf::Symbolic s = cupf::analyze_symbolic(n, row_ptr, col_idx, pv, cap);
+        const cupf::LuPlan L = cupf::lower_plan(s, n, row_ptr, col_idx);
+        const int P = L.num_fronts;
+        // every kind the solver uses: tier-1 fronts whole, tier-2 pivot + Schur
+        // fronts by pivot block / pivot-row chunks / Schur row blocks, other
+        // tier-2 fronts by row blocks
+        std::vector<int> kinds(size_t(P), 0), rpb(size_t(P), 8);
+        for (int p = 0; p < P; ++p) {
+            if (L.tier[p] == 1) kinds[size_t(p)] = 1 << 4;
+            else if (L.tier[p] == 2)
+                kinds[size_t(p)] = (L.fsz[p] <= 512 && L.nc[p] <= 64) ? 7 : (1 << 3);
+        }
+        const cupf::AsmLists A = cupf::build_asm_lists(L, kinds, rpb, 32);
+        // covered count per arena element (tier 0, tiers 1/2)
+        std::vector<int> c0(size_t(L.arena0), 0), c1(size_t(L.arena1), 0);
+        long long wrong_dst = 0;
+        for (int p = 0; p < P; ++p)
+            for (int kind = 0; kind < cupf::kAsmKinds; ++kind) {
+                const int base = A.base[size_t(p) * cupf::kAsmKinds + kind];
+                if (base < 0) continue;
+                const int fs = L.fsz[p], K = L.nc[p], uc = fs - K;
+                int nblk = 1;
+                if (kind == 0) nblk = (uc + 31) / 32;
+                else if (kind == 2) nblk = (uc + cupf::kAsmChunkCols - 1) / cupf::kAsmChunkCols;
+                else if (kind == 3) nblk = (fs + 7) / 8;
+                for (int blk = 0; blk < nblk; ++blk) {
+                    const int list = base + blk;
+                    const int r0 = kind == 0 ? K + blk * 32 : (kind == 3 ? blk * 8 : 0);
+                    for (int q = A.chunk_ptr[size_t(list) * cupf::kAsmItemWaves];
+                         q < A.chunk_ptr[size_t(list + 1) * cupf::kAsmItemWaves]; ++q) {
+                        const int src = A.chunks[size_t(q) * 4], lo = A.chunks[size_t(q) * 4 + 1];
+                        const int rr = A.chunks[size_t(q) * 4 + 2], len = A.chunks[size_t(q) * 4 + 3];
+                        // the child and contribution (row, column) of the chunk's elements
+                        for (int j = 0; j < len; ++j) {
+                            const long long e = src >= 0 ? (long long)src + j : (long long)(-1 - src) + j;
+                            std::vector<int>& cov = src >= 0 ? c1 : c0;
+                            if (e < 0 || e >= (long long)cov.size()) { ++wrong_dst; continue; }
+                            ++cov[size_t(e)];
+                            // find the child owning e and check the landing position
+                            int child = -1;
+                            for (int qq = L.child_ptr[p]; qq < L.child_ptr[p + 1]; ++qq) {
+                                const int c = L.child_idx[qq];
+                                if ((L.tier[c] == 0) != (src < 0)) continue;
+                                if (e >= L.off[c] && e < L.off[c] + (long long)L.fsz[c] * L.fsz[c]) child = c;
+                            }
+                            if (child < 0) { ++wrong_dst; continue; }
+                            const long long le = e - L.off[child];
+                            const int ca = int(le / L.fsz[child]) - L.nc[child];
+                            const int cb = int(le % L.fsz[child]) - L.nc[child];
+                            const int* loc = L.asm_loc.data() + L.asm_ptr[child];
+                            if (ca < 0 || cb < 0 || loc[ca] != r0 + rr || L.asm_loc[size_t(lo + j)] != loc[cb])
+                                ++wrong_dst;
+                        }
+                    }
+                }
+            }
+        long long cb_total = 0, once = 0, multi = 0, stray = 0;
+        std::vector<char> is_cb0(c0.size(), 0), is_cb1(c1.size(), 0);
+        for (int c = 0; c < P; ++c) {
+            const int f = L.fsz[c], k = L.nc[c];
+            bool mapped = false;            // a child of a tier-1/2 front (tier-0
+            for (int p = 0; p < P && !mapped; ++p)  // fronts assemble lane per scenario)
+                for (int qq = L.child_ptr[p]; qq < L.child_ptr[p + 1]; ++qq)
+                    if (L.child_idx[qq] == c && L.tier[p] != 0) mapped = true;
+            if (!mapped) continue;
+            for (int a = k; a < f; ++a)
+                for (int bb = k; bb < f; ++bb) {
+                    const size_t e = size_t(L.off[c] + (long long)a * f + bb);
+                    (L.tier[c] == 0 ? is_cb0 : is_cb1)[e] = 1;
+                    ++cb_total;
+                }
+        }
+        for (int t = 0; t < 2; ++t) {
+            const std::vector<int>& cov = t ? c1 : c0;
+            const std::vector<char>& is = t ? is_cb1 : is_cb0;
+            for (size_t e = 0; e < cov.size(); ++e) {
+                if (is[e] && cov[e] == 1) ++once;
+                if (cov[e] > 1) ++multi;
+                if (!is[e] && cov[e] > 0) ++stray;
+            }
+        }
+        // gather maps
+        std::vector<int> gcount(size_t(L.upd_total), 0);
+        std::vector<int> gpos(size_t(L.upd_total), -1);
+        for (int p = 0; p < P; ++p)
+            for (int l = 0; l < L.fsz[p]; ++l) {
+                const int g = L.front_ptr[p] + l;
+                for (int q = L.gat_ptr[size_t(g)]; q < L.gat_ptr[size_t(g) + 1]; ++q) {
+                    const int u = L.gat_src[size_t(q)];
+                    ++gcount[size_t(u)];
+                    gpos[size_t(u)] = g;
+                }
+            }
+        long long g_ok = 0;
+        for (int p = 0; p < P; ++p)
+            for (int qq = L.child_ptr[p]; qq < L.child_ptr[p + 1]; ++qq) {
+                const int c = L.child_idx[qq];
+                const int* loc = L.asm_loc.data() + L.asm_ptr[c];
+                for (int a = 0; a < L.fsz[c] - L.nc[c]; ++a) {
+                    const size_t u = size_t(L.uoff[c] + a);
+                    if (gcount[u] == 1 && gpos[u] == L.front_ptr[p] + loc[a]) ++g_ok;
+                }
+            }
+        int t1max = 0;
+        for (int p = 0; p < P; ++p)
+            if (L.tier[p] == 1) t1max = std::max(t1max, L.nc[p]);
+        out8[0] = cb_total; out8[1] = once; out8[2] = multi; out8[3] = stray;
+        out8[4] = wrong_dst; out8[5] = L.upd_total; out8[6] = g_ok; out8[7] = t1max;
+    });
+}
+
 // ---- learned warm start ------------------------------------------------
 int cupf_mlp_create(int32_t n_bus, const int32_t* pv, int32_t n_pv, const int32_t* pq,
                     int32_t n_pq, int32_t hidden, int32_t depth, uint64_t seed, int32_t max_batch,

@@ -250,6 +250,19 @@ int cupf_symbolic(int32_t n, const int32_t* row_ptr, const int32_t* col_idx,
 int cupf_plan_figures(int32_t n, const int32_t* row_ptr, const int32_t* col_idx,
                       const int32_t* perm, int32_t cap, double* out5);
 
+/* Coverage of the lowered plan's assembly chunk maps and solve gather maps
+ * for (pattern, perm, cap) (host analysis only; no reference counterpart:
+ * the reference assembles through its extend-add map directly):
+ * out = {contribution-block elements of the children of tier-1/2 fronts
+ *        (tier-0 fronts assemble lane per scenario), elements the chunk maps
+ *        cover exactly once, elements covered twice or more, elements covered
+ *        that are not contribution-block elements, chunk map entries landing
+ *        on a wrong (row, column) of the parent, update-vector elements,
+ *        elements the gather maps list exactly once under the right parent
+ *        position, most pivots of a tier-1 front}. */
+int cupf_plan_maps_check(int32_t n, const int32_t* row_ptr, const int32_t* col_idx,
+                         const int32_t* perm, int32_t cap, int64_t* out8);
+
 /* ---- Learned warm start (no reference counterpart) ---------------------
  * A dense MLP (BF16 weights, FP32 accumulate, GELU) maps per-bus Sbus
  * injections [P | Q] to [Va | Vm]; trained with AdamW on NR solutions the

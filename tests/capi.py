@@ -89,6 +89,16 @@ def symbolic(row_ptr, col_idx, perm, cap):
     return o
 
 
+def plan_maps_check(row_ptr, col_idx, perm, cap):
+    n = len(row_ptr) - 1
+    out = np.zeros(8, np.int64)
+    check(lib().cupf_plan_maps_check(ctypes.c_int32(n), _p(_i32(row_ptr)), _p(_i32(col_idx)),
+                                     _p(_i32(perm)), ctypes.c_int32(cap), _p(out)))
+    keys = ("cb_elements", "covered_once", "covered_twice", "stray", "wrong_dst",
+            "update_elements", "gathered_once", "tier1_max_pivots")
+    return dict(zip(keys, (int(x) for x in out)))
+
+
 def plan_figures(row_ptr, col_idx, perm, cap):
     n = len(row_ptr) - 1
     out = np.zeros(5, np.float64)

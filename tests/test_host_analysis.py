@@ -132,6 +132,21 @@ def test_multifrontal_invariants(jac, cap):
             assert par > p
 
 
+@pytest.mark.parametrize("cap", [8, 64])
+def test_assembly_and_gather_maps_cover_every_contribution_once(jac, cap):
+    """Each contribution-block element of every child of a tier-1/2 front is
+    assembled by exactly one chunk-map entry, onto the parent position its
+    extend-add map names; each update-vector element is gathered exactly once, under the
+    right parent position; tier-1 fronts stay within the tier-1 pivot bound."""
+    _, a, perm = jac
+    c = capi.plan_maps_check(a["row_ptr"], a["col_idx"], perm, cap)
+    assert c["cb_elements"] > 0
+    assert c["covered_once"] == c["cb_elements"]
+    assert c["covered_twice"] == 0 and c["stray"] == 0 and c["wrong_dst"] == 0
+    assert c["gathered_once"] == c["update_elements"] > 0
+    assert c["tier1_max_pivots"] <= 32
+
+
 def test_plan_figures_are_consistent(jac):
     _, a, perm = jac
     f8 = capi.plan_figures(a["row_ptr"], a["col_idx"], perm, 8)
