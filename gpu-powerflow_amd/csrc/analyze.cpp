@@ -629,25 +629,7 @@ AsmLists build_asm_lists(const LuPlan& L, const std::vector<int>& kinds,
     AsmLists A;
     A.base.assign(size_t(P) * kAsmKinds, -1);
     A.task_ptr.push_back(0);
-    A.stage_ptr.push_back(0);
-    A.item_ptr.push_back(0);
     A.chunk_ptr.push_back(0);
-    struct Piece { int q, a0, a1, b0, b1; };
-    // landing-set intersection of two pieces of (possibly) different children
-    auto meets = [&](const Piece& x, const Piece& y) {
-        const int* lx = L.asm_loc.data() + L.asm_ptr[L.child_idx[x.q]];
-        const int* ly = L.asm_loc.data() + L.asm_ptr[L.child_idx[y.q]];
-        auto inter = [](const int* a, int an, const int* b, int bn) {
-            int i = 0, j = 0;
-            while (i < an && j < bn) {
-                if (a[i] == b[j]) return true;
-                if (a[i] < b[j]) ++i; else ++j;
-            }
-            return false;
-        };
-        return inter(lx + x.a0, x.a1 - x.a0, ly + y.a0, y.a1 - y.a0) &&
-               inter(lx + x.b0, x.b1 - x.b0, ly + y.b0, y.b1 - y.b0);
-    };
     for (int p = 0; p < P; ++p) {
         const int fs = L.fsz[p], K = L.nc[p], uc = fs - K;
         for (int kind = 0; kind < kAsmKinds; ++kind) {
@@ -671,66 +653,26 @@ AsmLists build_asm_lists(const LuPlan& L, const std::vector<int>& kinds,
                             r1 = std::min(fs, r0 + rows_per_block[size_t(p)]); break;
                     default: break;
                 }
-                std::vector<Piece> pcs;
+                // one task per child whose contribution lands in the block, child order
+                const int tb = int(A.tasks.size());
                 for (int q = L.child_ptr[p]; q < L.child_ptr[p + 1]; ++q) {
                     const int c = L.child_idx[q];
                     const int cuc = L.fsz[c] - L.nc[c];
                     const int* loc = L.asm_loc.data() + L.asm_ptr[c];
                     auto lb = [&](int x) { return int(std::lower_bound(loc, loc + cuc, x) - loc); };
-                    const Piece pc{q, lb(r0), lb(r1), lb(c0), lb(c1)};
-                    if (pc.a1 > pc.a0 && pc.b1 > pc.b0) pcs.push_back(pc);
+                    const int a0 = lb(r0), a1 = lb(r1), b0 = lb(c0), b1 = lb(c1);
+                    if (a1 <= a0 || b1 <= b0) continue;
+                    AsmTask t{};
+                    t.off = L.off[c];
+                    t.tier = L.tier[c];
+                    t.cfs = L.fsz[c];
+                    t.cnc = L.nc[c];
+                    t.loc_ptr = L.asm_ptr[c];
+                    t.a0 = a0; t.na = a1 - a0;
+                    t.b0 = b0; t.nb = b1 - b0;
+                    A.tasks.push_back(t);
                 }
-                // greedy staging in child order, exact conflict test
-                std::vector<std::vector<int>> st;
-                for (int i = 0; i < int(pcs.size()); ++i) {
-                    int chosen = -1;
-                    for (int g = 0; g < int(st.size()) && chosen < 0; ++g) {
-                        bool ok = true;
-                        for (int j : st[size_t(g)])
-                            if (meets(pcs[size_t(i)], pcs[size_t(j)])) { ok = false; break; }
-                        if (ok) chosen = g;
-                    }
-                    if (chosen < 0) { chosen = int(st.size()); st.emplace_back(); }
-                    st[size_t(chosen)].push_back(i);
-                }
-                A.stage.push_back(int(A.tasks.size()));
-                for (auto& g : st) {
-                    for (int i : g) {
-                        const Piece& pc = pcs[size_t(i)];
-                        const int c = L.child_idx[pc.q];
-                        AsmTask t{};
-                        t.off = L.off[c];
-                        t.tier = L.tier[c];
-                        t.cfs = L.fsz[c];
-                        t.cnc = L.nc[c];
-                        t.loc_ptr = L.asm_ptr[c];
-                        t.a0 = pc.a0; t.na = pc.a1 - pc.a0;
-                        t.b0 = pc.b0; t.nb = pc.b1 - pc.b0;
-                        A.tasks.push_back(t);
-                    }
-                    A.stage.push_back(int(A.tasks.size()));
-                }
-                if (st.size() > 128)
-                    throw std::runtime_error("build_asm_lists: more than 128 assembly stages in a block");
-                // row items: per owner wave, in task order
-                const int tb = A.task_ptr.back(), te = int(A.tasks.size());
-                if (te - tb > kAsmMaxItemTasks)
-                    throw std::runtime_error("build_asm_lists: too many tasks in an assembly block");
-                if (r1 - r0 > (1 << 10))
-                    throw std::runtime_error("build_asm_lists: assembly block too tall");
-                for (int wv = 0; wv < kAsmItemWaves; ++wv) {
-                    for (int t = tb; t < te; ++t) {
-                        const AsmTask& tk = A.tasks[size_t(t)];
-                        const int* loc = L.asm_loc.data() + tk.loc_ptr;
-                        for (int a = tk.a0; a < tk.a0 + tk.na; ++a) {
-                            const int rr = loc[a] - r0;
-                            if (rr % kAsmItemWaves != wv) continue;
-                            if (a >= (1 << 15)) throw std::runtime_error("build_asm_lists: row index too large");
-                            A.items.push_back((t - tb) | (rr << 7) | (a << 17));
-                        }
-                    }
-                    A.item_ptr.push_back(int(A.items.size()));
-                }
+                const int te = int(A.tasks.size());
                 // chunk maps, per owner wave, in task order: every landing
                 // contribution row split into pieces of kAsmChunkLen columns
                 for (int wv = 0; wv < kAsmItemWaves; ++wv) {
@@ -741,14 +683,14 @@ AsmLists build_asm_lists(const LuPlan& L, const std::vector<int>& kinds,
                             const int rr = loc[a] - r0;
                             if (rr % kAsmItemWaves != wv) continue;
                             const long long row = tk.off + (long long)(tk.cnc + a) * tk.cfs + tk.cnc;
-                            for (int c0 = 0; c0 < tk.nb; c0 += kAsmChunkLen) {
-                                const long long src = row + tk.b0 + c0;
+                            for (int j0 = 0; j0 < tk.nb; j0 += kAsmChunkLen) {
+                                const long long src = row + tk.b0 + j0;
                                 if (src + kAsmChunkLen >= (1LL << 31) - 1)
                                     throw std::runtime_error("build_asm_lists: element index too large");
                                 A.chunks.push_back(tk.tier == 0 ? int(-1 - src) : int(src));
-                                A.chunks.push_back(tk.loc_ptr + tk.b0 + c0);
+                                A.chunks.push_back(tk.loc_ptr + tk.b0 + j0);
                                 A.chunks.push_back(rr);
-                                A.chunks.push_back(std::min(kAsmChunkLen, tk.nb - c0));
+                                A.chunks.push_back(std::min(kAsmChunkLen, tk.nb - j0));
                             }
                         }
                     }
@@ -757,7 +699,6 @@ AsmLists build_asm_lists(const LuPlan& L, const std::vector<int>& kinds,
                     A.chunk_ptr.push_back(int(A.chunks.size() / 4));
                 }
                 A.task_ptr.push_back(int(A.tasks.size()));
-                A.stage_ptr.push_back(int(A.stage.size()));
             }
         }
     }

@@ -135,16 +135,13 @@ LuPlan lower_plan(const Symbolic& s, int n, const int* rp, const int* ci, int ti
                   int tier1_max = 128);
 
 // ---------------------------------------------------------------------------
-// Assembly task lists.  A factor kernel assembles a block of a front (a
+// Assembly lists.  A factor kernel assembles a block of a front (a
 // rectangle of front rows x columns) from the children's contribution blocks.
 // For every (front, kind, block) the list holds one task per child whose
-// contribution lands in the block: the child's contribution rows [a0, a0+na)
-// and columns [b0, b0+nb) that land there, with what the kernel needs to
-// address them (arena offset and tier of the child, its size and pivot
-// count, and where its row map starts).  Tasks are ordered in stages: tasks
-// of one stage never add to the same front element (checked exactly, row and
-// column sets), so a stage is walked as one flat sequence; stages follow the
-// child order.
+// contribution lands in the block (child order): the child's contribution
+// rows [a0, a0+na) and columns [b0, b0+nb) that land there, with what is
+// needed to address them (arena offset and tier of the child, its size and
+// pivot count, and where its row map starts).
 //   kinds: 0 Schur row blocks (schur_rows rows from the pivot count on, all
 //            columns),
 //          1 pivot block (rows and columns below the pivot count),
@@ -156,26 +153,17 @@ LuPlan lower_plan(const Symbolic& s, int n, const int* rp, const int* ci, int ti
 constexpr int kAsmKinds = 5;
 constexpr int kAsmChunkCols = 64;
 
-struct AsmTask {                     // 48 bytes, device-visible layout
+struct AsmTask {
     long long off;                   // child's offset in its tier's arena
     int tier, cfs, cnc, loc_ptr;     // child's tier, front size, pivots, asm_ptr
     int a0, na, b0, nb;              // contribution rows / columns of the task
     int pad0, pad1;
 };
 
-// Row items of every list: the block's rows are owned by kAsmItemWaves
-// waves (block row rr by wave rr % kAsmItemWaves); every landing
-// contribution row is one item of its owner, packed as
-// task (bits 0-6, index inside the list) | block row rr (bits 7-16) |
-// contribution row a (bits 17-31).  A wave's items are in task order, so
-// each front element receives its contributions in a fixed order.
-constexpr int kAsmItemWaves = 4;
-constexpr int kAsmMaxItemTasks = 128;
-
-// Chunk maps: every list also holds its landing contribution rows cut into
-// chunks of up to kAsmChunkLen columns, split by owner wave like the row
-// items (block row rr by wave rr % kAsmItemWaves) and in task order inside
-// each part (child, then contribution row, then column), four ints each:
+// Chunk maps: the tasks of every list, cut into chunks of up to kAsmChunkLen
+// columns of one contribution row, split by owner wave (block row rr by wave
+// rr % kAsmItemWaves) and in task order inside each part (child, then
+// contribution row, then column), four ints each:
 //   src   element index of the chunk's first element CB(a, b0 + c0) inside
 //         the child's arena for one scenario (tier 1/2:
 //         off + (cnc + a) * cfs + cnc + b0 + c0, the chunk's elements follow
@@ -189,15 +177,12 @@ constexpr int kAsmMaxItemTasks = 128;
 // and several chunks in flight; one wave owns every element of a row, so
 // each element receives its contributions in a fixed order.
 constexpr int kAsmChunkLen = 64;
+constexpr int kAsmItemWaves = 4;
 
 struct AsmLists {
     std::vector<int> base;           // [P * kAsmKinds]: first list id of (front, kind), -1 if none
     std::vector<int> task_ptr;       // [lists + 1] into tasks
-    std::vector<int> stage_ptr;      // [lists + 1] into stage
-    std::vector<int> stage;          // stage boundaries: absolute task indices
     std::vector<AsmTask> tasks;
-    std::vector<int> item_ptr;       // [lists * kAsmItemWaves + 1] into items
-    std::vector<int> items;
     std::vector<int> chunk_ptr;      // [lists * kAsmItemWaves + 1] into chunks (quads)
     std::vector<int> chunks;         // (src, loc, rr, len) quads
 };

@@ -86,15 +86,7 @@ void adjoint_project(const NrDevice& d, int B, int Bp, const double* lambda, TG*
                      TG* grad_q, hipStream_t s);
 
 // ---- batched multifrontal LU (see analyze.hpp LuPlan) -----------------------
-// One assembly task (layout of analyze.hpp AsmTask).
-struct AsmTaskDev {
-    long long off;
-    int tier, cfs, cnc, loc_ptr;
-    int a0, na, b0, nb;
-    int pad0, pad1;
-};
-
-// Assembly kinds of the task lists (analyze.hpp build_asm_lists).
+// Assembly kinds of the lists (analyze.hpp build_asm_lists).
 enum { kAsmKSchur = 0, kAsmKPivot = 1, kAsmKChunk = 2, kAsmKRows = 3, kAsmKFront = 4,
        kAsmKindCount = 5 };
 
@@ -118,15 +110,7 @@ struct LuDevice {
     const int* perm = nullptr;
     const int* lb_ptr = nullptr;      // row bounds of the row-block assembly (LuPlan)
     const int* row_lb = nullptr;
-    const int* cstage_ptr = nullptr;  // child stages (LuPlan stage_ptr / stage)
-    const int* cstage = nullptr;
-    const int* asm_base = nullptr;    // assembly task lists (analyze.hpp AsmLists)
-    const int* asm_tptr = nullptr;
-    const int* asm_sptr = nullptr;
-    const int* asm_stage = nullptr;
-    const AsmTaskDev* asm_tasks = nullptr;
-    const int* asm_iptr = nullptr;    // row items of the Schur lists, 4 owner waves per list
-    const int* asm_items = nullptr;
+    const int* asm_base = nullptr;    // first assembly list of (front, kind) (analyze.hpp AsmLists)
     const int* gat_ptr = nullptr;     // solve gather maps (LuPlan gat_ptr / gat_src / rhs_idx)
     const int* gat_src = nullptr;
     const int* rhs_idx = nullptr;

@@ -98,217 +98,7 @@ __device__ inline void put_entries(T* __restrict__ F, const int* __restrict__ ds
     }
 }
 
-// Extend-add of a child's contribution block into an LDS buffer:
-//   F[(loc[a] - r_off) * ld + loc[bb] - c_off] += C(a, bb)
-// for rows a in [a0, a1) and columns bb in [b0, b1) of the child's
-// contribution block C(a, bb) = Cf[((cnc + a) * cfs + cnc + bb) * cs] (cs is
-// the element stride of the child's arena).  The (a, bb) pairs are walked
-// flat over the workgroup with D loads in flight per thread; the pairs of
-// one child land on distinct positions, so no two threads add to the same
-// element (callers synchronise between children).
-template <int NT, typename T, int D = 16>
-__device__ inline void extend_add(T* __restrict__ F, int ld, int r_off, int c_off,
-                                  const T* __restrict__ Cf, long long cs, int cfs, int cnc,
-                                  const int* __restrict__ loc, int a0, int a1, int b0, int b1) {
-    const int nb = b1 - b0;
-    if (nb <= 0 || a1 <= a0) return;
-    const int total = (a1 - a0) * nb;
-    const int tid = threadIdx.x;
-    int a = tid / nb, bb = tid - (tid / nb) * nb;
-    const int da = NT / nb, db = NT - (NT / nb) * nb;
-    for (int e0 = tid; e0 < total; e0 += D * NT) {
-        T v[D];
-        int idx[D];
-#pragma unroll
-        for (int u = 0; u < D; ++u) {
-            idx[u] = -1;
-            v[u] = T(0);
-            if (e0 + u * NT < total) {
-                const int ga = a0 + a, gb = b0 + bb;
-                v[u] = Cf[(long long)((cnc + ga) * cfs + cnc + gb) * cs];
-                idx[u] = (loc[ga] - r_off) * ld + loc[gb] - c_off;
-            }
-            a += da;
-            bb += db;
-            if (bb >= nb) { bb -= nb; ++a; }
-        }
-#pragma unroll
-        for (int u = 0; u < D; ++u)
-            if (idx[u] >= 0) F[idx[u]] += v[u];
-    }
-}
-
-// Assembly of one block of a front from its task list (one task per child
-// whose contribution lands in the block; host-built, see LuDevice asm_*).
-// Up to kMaxSeg tasks at a time are loaded one per thread into LDS (with the
-// stage boundaries), then each stage is walked flat over the workgroup with
-// D loads in flight per thread:  F[map(loc[a], loc[bb])] += CB(a, bb).
-// Tasks of one stage land on distinct positions; stages are separated by
-// barriers, so every element receives its contributions in child order.
-constexpr int kMaxSeg = 64;
-constexpr int kMaxStages = 128;            // per list (checked when the lists are built)
-
-template <typename T>
-struct ChildSeg {
-    const T* C;          // element 0 of the child's front for this scenario
-    long long cs;        // element stride of the child's arena
-    const int* loc;      // contribution row -> parent-local row
-    int rowoff;          // cnc * cfs + cnc: CB(a, bb) = C[(rowoff + a * cfs + bb) * cs]
-    int cfs;
-    int a0, b0, nb;
-};
-
-template <int NT, int D, typename T, typename Map>
-__device__ inline void assemble_tasks(const LuDevice& d, const Arenas<T>& ar, int list, int b,
-                                      int Bp, T* __restrict__ F, ChildSeg<T>* seg, int* pref,
-                                      int* stg, Map map) {
-    const int tid = threadIdx.x;
-    const int t_begin = d.asm_tptr[list], t_end = d.asm_tptr[list + 1];
-    const int s_begin = d.asm_sptr[list], s_end = d.asm_sptr[list + 1];
-    for (int cb = t_begin; cb < t_end; cb += kMaxSeg) {      // uniform
-        const int nseg = min(kMaxSeg, t_end - cb);
-        if (tid < 64) {                         // wave 0: load tasks, inclusive count scan
-            int cnt = 0;
-            if (tid < nseg) {
-                const AsmTaskDev t = d.asm_tasks[cb + tid];
-                ChildSeg<T> g;
-                if (t.tier == 0) {
-                    g.C = ar.a0 + t.off * Bp + b;
-                    g.cs = Bp;
-                } else {
-                    g.C = ar.a1 + (long long)b * d.arena1 + t.off;
-                    g.cs = 1;
-                }
-                g.loc = d.asm_loc + t.loc_ptr;
-                g.rowoff = t.cnc * t.cfs + t.cnc;
-                g.cfs = t.cfs;
-                g.a0 = t.a0;
-                g.b0 = t.b0;
-                g.nb = t.nb;
-                cnt = t.na * t.nb;
-                seg[tid] = g;
-            }
-#pragma unroll
-            for (int o = 1; o < 64; o <<= 1) {
-                const int y = __shfl_up(cnt, o, 64);
-                if (tid >= o) cnt += y;
-            }
-            pref[tid + 1] = cnt;
-            if (tid == 0) pref[0] = 0;
-        }
-        for (int s = s_begin + tid; s < s_end; s += NT) stg[s - s_begin] = d.asm_stage[s];
-        __syncthreads();
-        for (int s = 0; s + 1 < s_end - s_begin; ++s) {
-            const int lo = max(stg[s], cb) - cb, hi = min(stg[s + 1], cb + nseg) - cb;
-            if (lo >= hi) continue;             // uniform
-            const int e_end = pref[hi];
-            int k = lo, kc = -1;
-            ChildSeg<T> g{};
-            for (int e0 = pref[lo] + tid; e0 < e_end; e0 += D * NT) {
-                T v[D];
-                int idx[D];
-#pragma unroll
-                for (int u = 0; u < D; ++u) {
-                    const int e = e0 + u * NT;
-                    idx[u] = -1;
-                    v[u] = T(0);
-                    if (e < e_end) {
-                        while (pref[k + 1] <= e) ++k;
-                        if (k != kc) {
-                            g = seg[k];
-                            kc = k;
-                        }
-                        const int local = e - pref[k];
-                        const int a = local / g.nb;
-                        const int ga = g.a0 + a, gb = g.b0 + (local - a * g.nb);
-                        v[u] = g.C[(long long)(g.rowoff + ga * g.cfs + gb) * g.cs];
-                        idx[u] = map(g.loc[ga], g.loc[gb]);
-                    }
-                }
-#pragma unroll
-                for (int u = 0; u < D; ++u)
-                    if (idx[u] >= 0) F[idx[u]] += v[u];
-            }
-            __syncthreads();
-        }
-    }
-}
-
-// Assembly of one block of a front from its row items: the tasks of the list
-// are loaded into LDS once, then each wave walks the contribution rows that
-// land in the block rows it owns (rr % 4 == wave), in task order, two rows
-// in flight, the row's columns spread over the lanes:
-//   F[map(rr, loc[bb])] += CB(a, bb).
-// Waves own disjoint rows and one instruction touches one row of one child,
-// so no barrier is needed between children.
-constexpr int kItemTasks = 128;
-constexpr int kItemWaves = 4;
-
-template <int NT, typename T, typename Map>
-__device__ inline void assemble_items(const LuDevice& d, const Arenas<T>& ar, int list, int b,
-                                      int Bp, T* __restrict__ F, ChildSeg<T>* seg, Map map) {
-    static_assert(NT == 64 * kItemWaves, "one owner wave per row residue");
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int t_begin = d.asm_tptr[list];
-    const int ntask = d.asm_tptr[list + 1] - t_begin;
-    for (int t = tid; t < ntask; t += NT) {
-        const AsmTaskDev tk = d.asm_tasks[t_begin + t];
-        ChildSeg<T> g;
-        if (tk.tier == 0) {
-            g.C = ar.a0 + tk.off * Bp + b;
-            g.cs = Bp;
-        } else {
-            g.C = ar.a1 + (long long)b * d.arena1 + tk.off;
-            g.cs = 1;
-        }
-        g.loc = d.asm_loc + tk.loc_ptr;
-        g.rowoff = tk.cnc * tk.cfs + tk.cnc;
-        g.cfs = tk.cfs;
-        g.a0 = tk.a0;
-        g.b0 = tk.b0;
-        g.nb = tk.nb;
-        seg[t] = g;
-    }
-    __syncthreads();
-    const int i_begin = d.asm_iptr[list * kItemWaves + wave];
-    const int i_end = d.asm_iptr[list * kItemWaves + wave + 1];
-    for (int ib = i_begin; ib < i_end; ib += 64) {
-        const int n = min(64, i_end - ib);
-        const int mine = lane < n ? d.asm_items[ib + lane] : 0;
-        for (int j = 0; j < n; j += 2) {
-            const bool two = j + 1 < n;            // uniform
-            const int it0 = __builtin_amdgcn_readlane(mine, j);
-            const int it1 = two ? __builtin_amdgcn_readlane(mine, j + 1) : it0;
-            const ChildSeg<T> g0 = seg[it0 & 127];
-            const ChildSeg<T> g1 = seg[it1 & 127];
-            const int rr0 = (it0 >> 7) & 1023, rr1 = (it1 >> 7) & 1023;
-            const T* c0 = g0.C + (long long)(g0.rowoff + (it0 >> 17) * g0.cfs + g0.b0) * g0.cs;
-            const T* c1 = g1.C + (long long)(g1.rowoff + (it1 >> 17) * g1.cfs + g1.b0) * g1.cs;
-            const int* l0 = g0.loc + g0.b0;
-            const int* l1 = g1.loc + g1.b0;
-            const int nb0 = g0.nb, nb1 = two ? g1.nb : 0;
-            for (int cb = 0; cb < max(nb0, nb1); cb += 64 * 8) {
-                T v0[8], v1[8];
-                int k0[8], k1[8];
-#pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    const int bb = cb + lane + 64 * u;
-                    k0[u] = bb < nb0 ? l0[bb] : -1;
-                    v0[u] = bb < nb0 ? c0[(long long)bb * g0.cs] : T(0);
-                    k1[u] = bb < nb1 ? l1[bb] : -1;
-                    v1[u] = bb < nb1 ? c1[(long long)bb * g1.cs] : T(0);
-                }
-#pragma unroll
-                for (int u = 0; u < 8; ++u)
-                    if (k0[u] >= 0) F[map(rr0, k0[u])] += v0[u];
-#pragma unroll
-                for (int u = 0; u < 8; ++u)
-                    if (k1[u] >= 0) F[map(rr1, k1[u])] += v1[u];
-            }
-        }
-    }
-    __syncthreads();
-}
+constexpr int kItemWaves = 4;              // owner waves of a block's rows (rr % 4)
 
 // Assembly of one block of a front from its chunk map (host-built, see
 // LuDevice asm_cptr / asm_chunks): each wave walks the chunks of the block

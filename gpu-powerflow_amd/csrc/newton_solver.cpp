@@ -118,8 +118,7 @@ struct NewtonSolver::Impl {
     DeviceBuffer d_yp, d_yi, d_y, d_row_pvpq, d_row_pq, d_m11, d_m21, d_m12, d_m22, d_zero,
         d_row_bus, d_row_isq, d_jrp, d_jci, d_jtrp, d_jtci, d_jtmap;
     DeviceBuffer d_fsz, d_nc, d_tier, d_off, d_uoff, d_fptr, d_frows, d_cptr, d_cidx, d_aptr,
-        d_aloc, d_eptr, d_esrc, d_edst, d_perm, d_groups, d_lbptr, d_rowlb, d_abase, d_atptr,
-        d_asptr, d_astage, d_atasks, d_cstptr, d_cstage, d_aiptr, d_aitems,
+        d_aloc, d_eptr, d_esrc, d_edst, d_perm, d_groups, d_lbptr, d_rowlb, d_abase,
         d_acptr, d_achunks, d_gptr, d_gsrc, d_ridx;
     kern::NrDevice nr;
     kern::LuDevice lu;
@@ -708,11 +707,8 @@ void NewtonSolver::initialize(const YbusView& ybus, const int32_t* pv, int32_t n
     m.d_groups = upload_vector(P.group_fronts, s);
     m.d_lbptr = upload_vector(P.lb_ptr, s);
     m.d_rowlb = upload_vector(P.row_lb, s);
-    m.d_cstptr = upload_vector(P.stage_ptr, s);
-    m.d_cstage = upload_vector(P.stage, s);
     {
-        // assembly task lists for the kernels each front's group will run
-        static_assert(sizeof(AsmTask) == sizeof(kern::AsmTaskDev), "assembly task layout");
+        // assembly lists for the kernels each front's group will run
         std::vector<int> kinds(size_t(P.num_fronts), 0), rpb(size_t(P.num_fronts), 1);
         for (int L = 0; L < P.num_levels; ++L)
             for (int t = 1; t < 3; ++t) {
@@ -732,12 +728,6 @@ void NewtonSolver::initialize(const YbusView& ybus, const int32_t* pv, int32_t n
             }
         const AsmLists al = build_asm_lists(P, kinds, rpb, kern::schur_block_rows(int(m.tsize())));
         m.d_abase = upload_vector(al.base, s);
-        m.d_atptr = upload_vector(al.task_ptr, s);
-        m.d_asptr = upload_vector(al.stage_ptr, s);
-        m.d_astage = upload_vector(al.stage, s);
-        m.d_atasks = upload_vector(al.tasks, s);
-        m.d_aiptr = upload_vector(al.item_ptr, s);
-        m.d_aitems = upload_vector(al.items, s);
         m.d_gptr = upload_vector(P.gat_ptr, s);
         m.d_gsrc = upload_vector(P.gat_src, s);
         m.d_ridx = upload_vector(P.rhs_idx, s);
@@ -766,11 +756,7 @@ void NewtonSolver::initialize(const YbusView& ybus, const int32_t* pv, int32_t n
     lu.ent_ptr = m.d_eptr.as<int>(); lu.ent_src = m.d_esrc.as<int>();
     lu.ent_dst = m.d_edst.as<int>(); lu.perm = m.d_perm.as<int>();
     lu.lb_ptr = m.d_lbptr.as<int>(); lu.row_lb = m.d_rowlb.as<int>();
-    lu.cstage_ptr = m.d_cstptr.as<int>(); lu.cstage = m.d_cstage.as<int>();
-    lu.asm_base = m.d_abase.as<int>(); lu.asm_tptr = m.d_atptr.as<int>();
-    lu.asm_sptr = m.d_asptr.as<int>(); lu.asm_stage = m.d_astage.as<int>();
-    lu.asm_tasks = m.d_atasks.as<kern::AsmTaskDev>();
-    lu.asm_iptr = m.d_aiptr.as<int>(); lu.asm_items = m.d_aitems.as<int>();
+    lu.asm_base = m.d_abase.as<int>();
     lu.asm_cptr = m.d_acptr.as<int>(); lu.asm_chunks = m.d_achunks.as<int4>();
     lu.gat_ptr = m.d_gptr.as<int>(); lu.gat_src = m.d_gsrc.as<int>(); lu.rhs_idx = m.d_ridx.as<int>();
 
