@@ -615,7 +615,8 @@ LuPlan lower_plan(const Symbolic& s, int n, const int* rp, const int* ci, int ti
             const int c = L.child_idx[q];
             const int* loc = L.asm_loc.data() + L.asm_ptr[c];
             for (int a = 0; a < L.fsz[c] - L.nc[c]; ++a)
-                L.gat_src[size_t(cur[size_t(L.front_ptr[p] + loc[a])]++)] = int(L.uoff[c] + a);
+                L.gat_src[size_t(cur[size_t(L.front_ptr[p] + loc[a])]++)] =
+                    L.tier[c] == 0 ? int(L.uoff[c] + a) : int(-1 - (L.uoff[c] + a));
         }
     return L;
 }

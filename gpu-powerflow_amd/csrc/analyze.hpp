@@ -117,8 +117,11 @@ struct LuPlan {
     long long upd_total = 0;
     // Solve gather maps over the front positions g = front_ptr[p] + l: the
     // children's update-vector elements landing on position l of front p are
-    // gat_src[gat_ptr[g] .. gat_ptr[g + 1]) (element uoff[c] + a, children in
-    // child order); rhs_idx[g] = perm[front_rows[g]] is its right-hand-side row.
+    // gat_src[gat_ptr[g] .. gat_ptr[g + 1]) (element u = uoff[c] + a, stored as
+    // u for a tier-0 child, whose update vector is scenario-interleaved, and
+    // as -1 - u for a tier-1/2 child, whose update vector is scenario-major;
+    // children in child order); rhs_idx[g] = perm[front_rows[g]] is its
+    // right-hand-side row.
     std::vector<int> gat_ptr, gat_src, rhs_idx;
     std::vector<int> group_ptr;           // [num_levels * 3 + 1] into group_fronts
     std::vector<int> group_fronts;        // fronts of (level, tier) groups

@@ -552,7 +552,15 @@ int cupf_plan_maps_check(int32_t n, const int32_t* row_ptr, const int32_t* col_i
             for (int l = 0; l < L.fsz[p]; ++l) {
                 const int g = L.front_ptr[p] + l;
                 for (int q = L.gat_ptr[size_t(g)]; q < L.gat_ptr[size_t(g) + 1]; ++q) {
-                    const int u = L.gat_src[size_t(q)];
+                    const int s = L.gat_src[size_t(q)];
+                    const int u = s >= 0 ? s : -1 - s;
+                    // the layout flag must match the child's tier
+                    bool tier_ok = false;
+                    for (int qq = L.child_ptr[p]; qq < L.child_ptr[p + 1]; ++qq) {
+                        const int c = L.child_idx[qq];
+                        if (u >= L.uoff[c] && u < L.uoff[c + 1]) tier_ok = (L.tier[c] == 0) == (s >= 0);
+                    }
+                    if (!tier_ok) continue;
                     ++gcount[size_t(u)];
                     gpos[size_t(u)] = g;
                 }

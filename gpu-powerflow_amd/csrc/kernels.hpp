@@ -158,9 +158,11 @@ template <typename T>
 void schur_group(const LuDevice& d, const int* fronts, int count, int max_fsz, const double* A,
                  int Bp, Arenas<T> ar, const T* uinv, int dense_mode, hipStream_t s);
 
-// Solve sweeps.  rhs (FP64) and x are [n][Bp]; y [n][Bp] and upd
-// [upd_total][Bp] are work vectors.  gather: y <- P rhs is folded into the
-// forward sweep of each front; scatter writes x[perm[k]] = y[k].
+// Solve sweeps.  rhs (FP64) and x are [n][Bp]; y [n][Bp] and upd are work
+// vectors, upd holding the tier-0 fronts' update vectors as [upd_total][Bp]
+// followed by the tier-1/2 fronts' as [Bp][upd_total].  gather: y <- P rhs
+// is folded into the forward sweep of each front; scatter writes
+// x[perm[k]] = y[k].
 template <typename T>
 void lu_solve_forward_group(const LuDevice& d, int tier, const int* fronts, int count,
                             int max_fsz, int max_nc, Arenas<T> ar, const double* rhs, T* y, T* upd,

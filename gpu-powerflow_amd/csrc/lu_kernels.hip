@@ -1298,6 +1298,25 @@ __global__ __launch_bounds__(kSchurThreads, 2) void k_schur(LuDevice d, const in
 }
 
 // ===========================================================================
+// Update vectors of the solve: tier-0 fronts (lane per scenario) write theirs
+// scenario-interleaved, element e of scenario b at upd[e * Bp + b]; tier-1/2
+// fronts (a workgroup per scenario) scenario-major after that block, at
+// upd[upd_total * Bp + b * upd_total + e], so each workgroup writes one
+// contiguous run.  Gather-map entries name the layout: s >= 0 interleaved,
+// s < 0 scenario-major element -1 - s.
+// ===========================================================================
+template <typename T>
+__device__ __forceinline__ T* upd1_of(const LuDevice& d, T* upd, int Bp, int b) {
+    return upd + d.upd_total * Bp + (long long)b * d.upd_total;
+}
+
+template <typename T>
+__device__ __forceinline__ T upd_elem(const LuDevice& d, const T* upd, int Bp, int b, int s) {
+    return s >= 0 ? upd[(long long)s * Bp + b]
+                  : upd[d.upd_total * Bp + (long long)b * d.upd_total + (-1 - s)];
+}
+
+// ===========================================================================
 // Solve, tier 0 (lane per scenario): w kept in LDS as w[l][lane]
 // ===========================================================================
 template <typename T, bool TRANS>
@@ -1320,8 +1339,13 @@ __global__ __launch_bounds__(kLanes) void k_fwd_t0(LuDevice d, const int* __rest
         const int c = d.child_idx[q];
         const int cuc = d.fsz[c] - d.nc[c];
         const int* loc = d.asm_loc + d.asm_ptr[c];
-        const T* u = upd + d.uoff[c] * Bp + b;
-        for (int a = 0; a < cuc; ++a) w[loc[a] * kLanes] += u[a * s];
+        if (d.tier[c] == 0) {                     // interleaved update vector
+            const T* u = upd + d.uoff[c] * Bp + b;
+            for (int a = 0; a < cuc; ++a) w[loc[a] * kLanes] += u[a * s];
+        } else {                                  // scenario-major update vector
+            const T* u = upd1_of(d, upd, Bp, b) + d.uoff[c];
+            for (int a = 0; a < cuc; ++a) w[loc[a] * kLanes] += u[a];
+        }
     }
     for (int k = 0; k < ncp; ++k) {
         T yk = w[k * kLanes];
@@ -1507,14 +1531,14 @@ __device__ inline void gather_w(const LuDevice& d, int p, int b, int Bp, int fs,
 #pragma unroll
             for (int t = 0; t < 2; ++t) {
                 const int q = q0[h] + t;
-                v[h][t] = q < q1[h] ? upd[(long long)d.gat_src[q] * Bp + b] : T(0);
+                v[h][t] = q < q1[h] ? upd_elem(d, upd, Bp, b, d.gat_src[q]) : T(0);
             }
         }
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             T a = acc[h] + v[h][0];
             a += v[h][1];
-            for (int q = q0[h] + 2; q < q1[h]; ++q) a += upd[(long long)d.gat_src[q] * Bp + b];
+            for (int q = q0[h] + 2; q < q1[h]; ++q) a += upd_elem(d, upd, Bp, b, d.gat_src[q]);
             if (l0 + h * NT < fs) w[l0 + h * NT] = a;
         }
     }
@@ -1539,8 +1563,8 @@ __global__ __launch_bounds__(kLanes) void k_fwd_t12(LuDevice d, const int* __res
     if (TRANS) sweep_fwd_Ut(G, fs, ncp, w, lane);
     else sweep_fwd_L(G, fs, ncp, w, lane);
     for (int k = lane; k < ncp; k += kLanes) y[(long long)fr[k] * Bp + b] = w[k];
-    T* u = upd + d.uoff[p] * Bp + b;
-    for (int a = lane; a < uc; a += kLanes) u[(long long)a * Bp] = w[ncp + a];
+    T* u = upd1_of(d, upd, Bp, b) + d.uoff[p];
+    for (int a = lane; a < uc; a += kLanes) u[a] = w[ncp + a];
 }
 
 template <typename T, bool TRANS>
@@ -1690,8 +1714,8 @@ __global__ __launch_bounds__(NT) void k_fwd_w(LuDevice d, const int* __restrict_
     }
     __syncthreads();
     for (int k = tid; k < K; k += NT) y[(long long)fr[k] * Bp + b] = w[k];
-    T* u = upd + d.uoff[p] * Bp + b;
-    for (int a = tid; a < uc; a += NT) u[(long long)a * Bp] = w[K + a];
+    T* u = upd1_of(d, upd, Bp, b) + d.uoff[p];
+    for (int a = tid; a < uc; a += NT) u[a] = w[K + a];
 }
 
 template <typename T, bool TRANS, int NT>
@@ -1939,23 +1963,13 @@ void lu_solve_forward_group(const LuDevice& d, int tier, const int* fronts, int 
             hipLaunchKernelGGL((k_fwd_t0<T, false>), dim3(count, Bp / kLanes), dim3(kLanes), 0, s,
                                d, fronts, ar, rhs, y, upd, Bp);
     } else if (sizeof(T) == 4 && max_nc <= kSolveK) {
-        if (tier == 2) {
-            const size_t shmem = solve_w_lds<256, T>(max_fsz);
-            if (transpose)
-                hipLaunchKernelGGL((k_fwd_w<T, true, 256>), grid, dim3(256), shmem, s, d, fronts, ar,
-                                   rhs, y, upd, Bp);
-            else
-                hipLaunchKernelGGL((k_fwd_w<T, false, 256>), grid, dim3(256), shmem, s, d, fronts, ar,
-                                   rhs, y, upd, Bp);
-        } else {
-            const size_t shmem = solve_w_lds<64, T>(max_fsz);
-            if (transpose)
-                hipLaunchKernelGGL((k_fwd_w<T, true, 64>), grid, dim3(64), shmem, s, d, fronts, ar,
-                                   rhs, y, upd, Bp);
-            else
-                hipLaunchKernelGGL((k_fwd_w<T, false, 64>), grid, dim3(64), shmem, s, d, fronts, ar,
-                                   rhs, y, upd, Bp);
-        }
+        const size_t shmem = solve_w_lds<256, T>(max_fsz);
+        if (transpose)
+            hipLaunchKernelGGL((k_fwd_w<T, true, 256>), grid, dim3(256), shmem, s, d, fronts, ar,
+                               rhs, y, upd, Bp);
+        else
+            hipLaunchKernelGGL((k_fwd_w<T, false, 256>), grid, dim3(256), shmem, s, d, fronts, ar,
+                               rhs, y, upd, Bp);
     } else {
         const size_t shmem = size_t(max_fsz) * sizeof(T);
         if (transpose)
@@ -1982,22 +1996,13 @@ void lu_solve_backward_group(const LuDevice& d, int tier, const int* fronts, int
             hipLaunchKernelGGL((k_bwd_t0<T, false>), dim3(count, Bp / kLanes), dim3(kLanes), 0, s,
                                d, fronts, ar, y, Bp);
     } else if (sizeof(T) == 4 && max_nc <= kSolveK) {
-        if (tier == 2) {
-            const size_t shmem = solve_w_lds<256, T>(max_fsz);
-            if (transpose)
-                hipLaunchKernelGGL((k_bwd_w<T, true, 256>), grid, dim3(256), shmem, s, d, fronts, ar, y,
-                                   Bp);
-            else
-                hipLaunchKernelGGL((k_bwd_w<T, false, 256>), grid, dim3(256), shmem, s, d, fronts, ar, y,
-                                   Bp);
-        } else {
-            const size_t shmem = solve_w_lds<64, T>(max_fsz);
-            if (transpose)
-                hipLaunchKernelGGL((k_bwd_w<T, true, 64>), grid, dim3(64), shmem, s, d, fronts, ar, y, Bp);
-            else
-                hipLaunchKernelGGL((k_bwd_w<T, false, 64>), grid, dim3(64), shmem, s, d, fronts, ar, y,
-                                   Bp);
-        }
+        const size_t shmem = solve_w_lds<256, T>(max_fsz);
+        if (transpose)
+            hipLaunchKernelGGL((k_bwd_w<T, true, 256>), grid, dim3(256), shmem, s, d, fronts, ar, y,
+                               Bp);
+        else
+            hipLaunchKernelGGL((k_bwd_w<T, false, 256>), grid, dim3(256), shmem, s, d, fronts, ar, y,
+                               Bp);
     } else {
         const size_t shmem = size_t(max_fsz) * sizeof(T);
         if (transpose)

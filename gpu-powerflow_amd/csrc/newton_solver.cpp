@@ -167,7 +167,8 @@ struct NewtonSolver::Impl {
             arena0.allocate(nb * std::max<size_t>(1, size_t(plan.arena0)) * tsize());
             arena1.allocate(nb * std::max<size_t>(1, size_t(plan.arena1)) * tsize());
             ywork.allocate(nb * js.dim * tsize());
-            upd.allocate(nb * std::max<size_t>(1, size_t(plan.upd_total)) * tsize());
+            // interleaved (tier-0 fronts) then scenario-major (tier-1/2 fronts)
+            upd.allocate(2 * nb * std::max<size_t>(1, size_t(plan.upd_total)) * tsize());
             dx.allocate(nb * js.dim * xsize());
             ddx.allocate(nb * js.dim * xsize());
             res.allocate(nb * js.dim * sizeof(double));
