@@ -20,8 +20,12 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kWaves = kBlock / kLanes;
-constexpr int kBusPerWave = 16;
-constexpr int kBusPerBlock = kBusPerWave * kWaves;
+// buses per wave: the residual keeps 4 (one max-reduction and atomic per
+// block and scenario), the Jacobian fill 1 (more waves in flight)
+constexpr int kResBusPerWave = 4;
+constexpr int kResBusPerBlock = kResBusPerWave * kWaves;
+constexpr int kJacBusPerWave = 1;
+constexpr int kJacBusPerBlock = kJacBusPerWave * kWaves;
 constexpr unsigned long long kNanBits = 0x7FF8000000000000ULL;
 
 // ---- layout transforms (tiled through LDS, coalesced on both sides) ----
@@ -104,10 +108,10 @@ __global__ __launch_bounds__(kBlock) void k_residual(NrDevice d, int Bp,
                                                      double* __restrict__ normF) {
     const int lane = threadIdx.x & (kLanes - 1), w = threadIdx.x / kLanes;
     const int b = blockIdx.y * kLanes + lane;
-    const int i0 = blockIdx.x * kBusPerBlock + w * kBusPerWave;
+    const int i0 = blockIdx.x * kResBusPerBlock + w * kResBusPerWave;
     double m = 0.0;
     bool nan = false;
-    for (int t = 0; t < kBusPerWave; ++t) {
+    for (int t = 0; t < kResBusPerWave; ++t) {
         const int i = i0 + t;
         if (i >= d.n_bus) break;
         const int r1 = d.row_pvpq[i];
@@ -152,8 +156,8 @@ __global__ __launch_bounds__(kBlock) void k_jacobian(NrDevice d, int Bp,
     const int b = blockIdx.y * kLanes + lane;
     if (blockIdx.x == 0)
         for (int z = w; z < n_zero; z += kWaves) J[(long long)zero_slots[z] * Bp + b] = 0.0;
-    const int i0 = blockIdx.x * kBusPerBlock + w * kBusPerWave;
-    for (int t = 0; t < kBusPerWave; ++t) {
+    const int i0 = blockIdx.x * kJacBusPerBlock + w * kJacBusPerWave;
+    for (int t = 0; t < kJacBusPerWave; ++t) {
         const int i = i0 + t;
         if (i >= d.n_bus) break;
         if (d.row_pvpq[i] < 0) continue;
@@ -401,14 +405,14 @@ void zero_norm(double* normF, int Bp, hipStream_t s) {
 
 void residual(const NrDevice& d, int Bp, const double2* V, const double2* S, double* F,
               double* normF, hipStream_t s) {
-    dim3 grid((d.n_bus + kBusPerBlock - 1) / kBusPerBlock, Bp / kLanes);
+    dim3 grid((d.n_bus + kResBusPerBlock - 1) / kResBusPerBlock, Bp / kLanes);
     hipLaunchKernelGGL(k_residual, grid, dim3(kBlock), 0, s, d, Bp, V, S, F, normF);
     CUPF_HIP(hipGetLastError());
 }
 
 void jacobian(const NrDevice& d, int Bp, const double2* V, double* J, const int* zero_slots,
               int n_zero, hipStream_t s) {
-    dim3 grid((d.n_bus + kBusPerBlock - 1) / kBusPerBlock, Bp / kLanes);
+    dim3 grid((d.n_bus + kJacBusPerBlock - 1) / kJacBusPerBlock, Bp / kLanes);
     hipLaunchKernelGGL(k_jacobian, grid, dim3(kBlock), 0, s, d, Bp, V, J, zero_slots, n_zero);
     CUPF_HIP(hipGetLastError());
 }
