@@ -143,6 +143,28 @@ struct NewtonSolver::Impl {
     hipStream_t own = nullptr;
     hipStream_t s = nullptr;   // stream of the current call
 
+    // captured Newton step (use_cuda_graph): Jacobian, factor, solve with
+    // refinement, voltage update, then the next residual and its batch max.
+    // Captured on the private stream, launched on the call's stream; the key
+    // names everything the captured launches bake in.
+    struct GraphKey {
+        int B = -1, Bp = -1, tolerant = -1;
+        const void* yb = nullptr;
+        unsigned long long gen = 0;
+        bool operator==(const GraphKey& o) const {
+            return B == o.B && Bp == o.Bp && tolerant == o.tolerant && yb == o.yb && gen == o.gen;
+        }
+    };
+    hipGraphExec_t step_graph = nullptr;
+    GraphKey graph_key;
+    unsigned long long buf_gen = 1;   // bumped whenever a device buffer the step uses moves
+    bool graph_warm = false;          // one eager step first (first-use kernel attributes)
+    bool capturing = false;
+    void drop_graph() {
+        if (step_graph) (void)hipGraphExecDestroy(step_graph);
+        step_graph = nullptr;
+    }
+
     // adjoint cache
     bool cache = false;
     int cache_B = 0;
@@ -155,6 +177,7 @@ struct NewtonSolver::Impl {
         B = batch;
         Bp = kern::pad_batch(batch);
         if (Bp > capBp) {
+            ++buf_gen;
             const size_t nb = size_t(Bp);
             V.allocate(nb * n * sizeof(double2));
             Va.allocate(nb * n * sizeof(double));
@@ -231,7 +254,7 @@ struct NewtonSolver::Impl {
     void factor() {
         if (front64) factor_t<double>({arena0.as<double>(), arena1.as<double>()});
         else factor_t<float>({arena0.as<float>(), arena1.as<float>()});
-        ++st.factorizations;
+        if (!capturing) ++st.factorizations;   // a captured step counts at each launch
     }
 
     template <typename T, typename TX>
@@ -331,6 +354,7 @@ struct NewtonSolver::Impl {
     int run_nr(const NRConfig& cfg, bool tolerant = false, int budget = -1) {
         CUPF_HIP(hipMemsetAsync(sing.get(), 0, size_t(Bp) * sizeof(int), s));
         const int limit = budget >= 0 ? std::min(budget, cfg.max_iter) : cfg.max_iter;
+        if (opt.use_cuda_graph && !timer.on && limit > 0) return run_nr_graph(cfg, tolerant, limit);
         int completed = 0;
         for (int it = 0; it < limit; ++it) {
             completed = it + 1;
@@ -349,6 +373,77 @@ struct NewtonSolver::Impl {
             if (m <= cfg.tolerance) break;
             eval_jacobian();
             newton_step();
+        }
+        return completed;
+    }
+
+    // Stop test on a batch max: true when the loop ends here.
+    bool nr_stop(double m, const NRConfig& cfg, bool tolerant) const {
+        if (tolerant) {
+            if (m < 0.0) return true;               // nothing finite left
+        } else if (!std::isfinite(m)) {
+            throw std::runtime_error("NewtonSolver: mismatch norm is not finite");
+        }
+        return m <= cfg.tolerance;
+    }
+
+    void enqueue_batch_max(bool tolerant) {
+        if (tolerant) kern::batch_max_finite(normF.as<double>(), B, bmax.as<double>(), s);
+        else kern::batch_max(normF.as<double>(), B, bmax.as<double>(), s);
+    }
+
+    double read_bmax() {
+        double h = 0.0;
+        CUPF_HIP(hipMemcpyAsync(&h, bmax.get(), sizeof(double), hipMemcpyDeviceToHost, s));
+        CUPF_HIP(hipStreamSynchronize(s));
+        return h;
+    }
+
+    // One Newton step and the residual it leads to (the captured body).
+    void step_and_residual(bool tolerant) {
+        eval_jacobian();
+        newton_step();
+        eval_residual();
+        enqueue_batch_max(tolerant);
+    }
+
+    // NR loop on a captured step (reference graph_iteration.hpp:85-146
+    // semantics): the residual at the initial V eagerly, then one graph
+    // launch per Newton step (step + next residual + device batch max) and a
+    // readback of the max; at most limit - 1 steps, completed = 1 + steps.
+    int run_nr_graph(const NRConfig& cfg, bool tolerant, int limit) {
+        eval_residual();
+        enqueue_batch_max(tolerant);
+        if (nr_stop(read_bmax(), cfg, tolerant)) return 1;
+        int completed = 1;
+        for (int step = 0; step + 1 < limit; ++step) {
+            if (!graph_warm) {
+                step_and_residual(tolerant);        // eager: first-use kernel set-up
+                graph_warm = true;
+            } else {
+                const GraphKey key{B, Bp, tolerant ? 1 : 0, nr.yb, buf_gen};
+                if (!step_graph || !(key == graph_key)) {
+                    drop_graph();
+                    const hipStream_t call = s;
+                    s = own;                        // capture on the private stream
+                    capturing = true;
+                    hipGraph_t g = nullptr;
+                    CUPF_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                    step_and_residual(tolerant);
+                    const hipError_t e = hipStreamEndCapture(s, &g);
+                    capturing = false;
+                    s = call;
+                    CUPF_HIP(e);
+                    const hipError_t ei = hipGraphInstantiate(&step_graph, g, nullptr, nullptr, 0);
+                    (void)hipGraphDestroy(g);
+                    CUPF_HIP(ei);
+                    graph_key = key;
+                }
+                CUPF_HIP(hipGraphLaunch(step_graph, s));
+                ++st.factorizations;
+            }
+            completed = step + 2;
+            if (nr_stop(read_bmax(), cfg, tolerant)) break;
         }
         return completed;
     }
@@ -571,6 +666,7 @@ NewtonSolver::NewtonSolver(const NewtonOptions& options) : impl_(new Impl) {
 }
 
 NewtonSolver::~NewtonSolver() {
+    if (impl_) impl_->drop_graph();
     if (impl_ && impl_->own) {
         (void)hipStreamSynchronize(impl_->own);
         (void)hipStreamDestroy(impl_->own);
@@ -762,6 +858,8 @@ void NewtonSolver::initialize(const YbusView& ybus, const int32_t* pv, int32_t n
     lu.gat_ptr = m.d_gptr.as<int>(); lu.gat_src = m.d_gsrc.as<int>(); lu.rhs_idx = m.d_ridx.as<int>();
 
     m.capBp = 0;   // batch buffers are sized lazily for the new structure
+    m.drop_graph();
+    ++m.buf_gen;
     m.capB = 0;
     m.st = NewtonSolverStats{};
     m.st.dim = m.js.dim; m.st.nnz_j = m.js.nnz; m.st.num_fronts = P.num_fronts;
