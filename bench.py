@@ -49,6 +49,8 @@ def parse():
     p.add_argument("--cpu-per-core", type=int, default=6,
                    help="oracle scenarios per host thread in the CPU baseline sample")
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--no-graph", action="store_true",
+                   help="skip the extra run of each form on the captured Newton step")
     p.add_argument("--no-check", action="store_true")
     p.add_argument("--forms", default="fp64,mixed,bf16x3,learned",
                    help="comma list of solver forms to time (fp64, mixed, bf16x3, learned)")
@@ -124,12 +126,13 @@ def traffic_for(kernel, workload):
         return None
 
 
-def make_solver(cupf, grid, compute, precision, cap, refine):
+def make_solver(cupf, grid, compute, precision, cap, refine, graph=False):
     o = cupf.NewtonOptions()
     o.compute = compute
     o.custom.precision = precision
     o.custom.max_panel_width = cap
     o.custom.refine_steps = refine
+    o.use_cuda_graph = graph
     s = cupf.NewtonSolver(o)
     s.initialize(grid.indptr, grid.indices, grid.data, grid.n_bus, grid.n_bus, grid.pv, grid.pq)
     return s
@@ -251,6 +254,19 @@ def main():
             res[f]["warm_iterations"] = res[f]["stats"]["last_warm_iterations"]
             res[f]["fallbacks"] = res[f]["stats"]["last_warm_fallbacks"]
         del sv
+        if not args.no_graph:
+            # the same form again on the captured Newton step (use_cuda_graph),
+            # without the per-launch events (they stay out of captured work)
+            sg = make_solver(cupf, grid, comp, prec, args.cap, ref, graph=True)
+            if f == "learned":
+                sg.set_warm_start(mlp, args.warm_iters)
+            msg, itg, _ = timed_solves(torch, sg, S, V0, Vout, B, n, args.steps, args.warmup, cfg)
+            res[f]["graph"] = {"solves_per_s": B / msg, "ms_per_step": msg * 1e3,
+                               "iterations": itg[-1],
+                               "bit_identical_to_eager": bool(np.array_equal(Vout.cpu().numpy(),
+                                                                             res[f]["V"]))}
+            log(f"form {f}: captured step: {msg * 1e3:.2f} ms/step, {B / msg:.0f} solves/s")
+            del sg
     head = "learned" if "learned" in res else ("mixed" if "mixed" in res else forms[0])
     h = res[head]
     workload = f"{args.workload}x{B}_cap{args.cap}"
@@ -294,6 +310,7 @@ def main():
                       "iterations": r["iterations"],
                       **({"warm_iterations": r["warm_iterations"], "fallbacks": r["fallbacks"]}
                          if "fallbacks" in r else {}),
+                      **({"use_cuda_graph": r["graph"]} if "graph" in r else {}),
                       "stage_ms_per_step": {k: v[0] / args.steps for k, v in r["stages"].items()}}
                   for f, r in res.items()},
         "warm_start_training": training,
