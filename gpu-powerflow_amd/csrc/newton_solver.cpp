@@ -99,7 +99,8 @@ struct NewtonSolver::Impl {
     // precision profile
     bool front64 = true;   // fronts in FP64
     bool dx32 = false;     // Newton step stored in FP32 (ComputePolicy::FP32)
-    int refine = 0;        // FP64 refinement steps of the step
+    int refine = 0;        // FP64 refinement steps of a step taken at a small mismatch
+    int refine_now = 0;    // refinement steps of the current step
     bool static_piv = true;
     double piv_eps = 1e-8;
     int dense_mode = 0;    // FP32 dense fronts: 1 MFMA FP32, 2 MFMA 3xBF16
@@ -148,22 +149,32 @@ struct NewtonSolver::Impl {
     // Captured on the private stream, launched on the call's stream; the key
     // names everything the captured launches bake in.
     struct GraphKey {
-        int B = -1, Bp = -1, tolerant = -1;
+        int B = -1, Bp = -1, tolerant = -1, refine = -1;
         const void* yb = nullptr;
         unsigned long long gen = 0;
         bool operator==(const GraphKey& o) const {
-            return B == o.B && Bp == o.Bp && tolerant == o.tolerant && yb == o.yb && gen == o.gen;
+            return B == o.B && Bp == o.Bp && tolerant == o.tolerant && refine == o.refine &&
+                   yb == o.yb && gen == o.gen;
         }
     };
-    hipGraphExec_t step_graph = nullptr;
-    GraphKey graph_key;
+    hipGraphExec_t step_graphs[2] = {nullptr, nullptr};   // without / with refinement
+    GraphKey graph_keys[2];
     unsigned long long buf_gen = 1;   // bumped whenever a device buffer the step uses moves
     bool graph_warm = false;          // one eager step first (first-use kernel attributes)
     bool capturing = false;
     void drop_graph() {
-        if (step_graph) (void)hipGraphExecDestroy(step_graph);
-        step_graph = nullptr;
+        for (auto& g : step_graphs) {
+            if (g) (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
     }
+
+    // Inexact Newton: the FP64 refinement of the FP32 step pays off only
+    // once the mismatch is small; above kRefineBelow the step's own
+    // quadratic error dwarfs the FP32 solve's (relative error far below
+    // kRefineBelow), so the step is taken unrefined.
+    static constexpr double kRefineBelow = 1e-3;
+    int refine_for(double m) const { return m <= kRefineBelow ? refine : 0; }
 
     // adjoint cache
     bool cache = false;
@@ -287,10 +298,10 @@ struct NewtonSolver::Impl {
     // rhs (FP64, [dim][Bp]) -> x, with FP64 iterative refinement when the
     // fronts are FP32 (residual on the FP64 J, or J^T for transposed solves).
     template <typename TX>
-    void solve_into(const double* rhs, TX* x, TX* xtmp, bool transpose) {
+    void solve_into(const double* rhs, TX* x, TX* xtmp, bool transpose, int steps) {
         solve_dispatch<TX>(rhs, x, transpose);
         if (front64) return;
-        for (int it = 0; it < refine; ++it) {
+        for (int it = 0; it < steps; ++it) {
             if (transpose)
                 kern::refine_residual<TX>(d_jtrp.as<int>(), d_jtci.as<int>(), d_jtmap.as<int>(),
                                           J.as<double>(), js.dim, Bp, rhs, x, res.as<double>(), s);
@@ -310,7 +321,7 @@ struct NewtonSolver::Impl {
         if (dx32) {
             {
                 StageScope st_(timer, kStSolve, s);
-                solve_into<float>(F.as<double>(), dx.as<float>(), ddx.as<float>(), false);
+                solve_into<float>(F.as<double>(), dx.as<float>(), ddx.as<float>(), false, refine_now);
             }
             StageScope st_(timer, kStUpdate, s);
             kern::voltage_update<float>(nr, Bp, dx.as<float>(), Va.as<double>(), Vm.as<double>(),
@@ -318,7 +329,8 @@ struct NewtonSolver::Impl {
         } else {
             {
                 StageScope st_(timer, kStSolve, s);
-                solve_into<double>(F.as<double>(), dx.as<double>(), ddx.as<double>(), false);
+                solve_into<double>(F.as<double>(), dx.as<double>(), ddx.as<double>(), false,
+                                   refine_now);
             }
             StageScope st_(timer, kStUpdate, s);
             kern::voltage_update<double>(nr, Bp, dx.as<double>(), Va.as<double>(),
@@ -371,6 +383,7 @@ struct NewtonSolver::Impl {
                     throw std::runtime_error("NewtonSolver: mismatch norm is not finite");
             }
             if (m <= cfg.tolerance) break;
+            refine_now = refine_for(m);
             eval_jacobian();
             newton_step();
         }
@@ -414,16 +427,21 @@ struct NewtonSolver::Impl {
     int run_nr_graph(const NRConfig& cfg, bool tolerant, int limit) {
         eval_residual();
         enqueue_batch_max(tolerant);
-        if (nr_stop(read_bmax(), cfg, tolerant)) return 1;
+        double m = read_bmax();
+        if (nr_stop(m, cfg, tolerant)) return 1;
         int completed = 1;
         for (int step = 0; step + 1 < limit; ++step) {
+            refine_now = refine_for(m);
             if (!graph_warm) {
                 step_and_residual(tolerant);        // eager: first-use kernel set-up
                 graph_warm = true;
             } else {
-                const GraphKey key{B, Bp, tolerant ? 1 : 0, nr.yb, buf_gen};
+                const GraphKey key{B, Bp, tolerant ? 1 : 0, refine_now, nr.yb, buf_gen};
+                hipGraphExec_t& step_graph = step_graphs[refine_now > 0 ? 1 : 0];
+                GraphKey& graph_key = graph_keys[refine_now > 0 ? 1 : 0];
                 if (!step_graph || !(key == graph_key)) {
-                    drop_graph();
+                    if (step_graph) (void)hipGraphExecDestroy(step_graph);
+                    step_graph = nullptr;
                     const hipStream_t call = s;
                     s = own;                        // capture on the private stream
                     capturing = true;
@@ -443,7 +461,8 @@ struct NewtonSolver::Impl {
                 ++st.factorizations;
             }
             completed = step + 2;
-            if (nr_stop(read_bmax(), cfg, tolerant)) break;
+            m = read_bmax();
+            if (nr_stop(m, cfg, tolerant)) break;
         }
         return completed;
     }
@@ -1084,7 +1103,7 @@ void NewtonSolver::solve_adjoint(const double* grad_va, int64_t grad_va_stride,
     kern::adjoint_gather<double>(m.nr, B, m.Bp, m.stage_a.as<double>(), m.n,
                                  m.stage_b.as<double>(), m.n, m.g.as<double>(), m.s);
     const auto ts = std::chrono::steady_clock::now();
-    m.solve_into<double>(m.g.as<double>(), m.lam.as<double>(), m.ddx.as<double>(), true);
+    m.solve_into<double>(m.g.as<double>(), m.lam.as<double>(), m.ddx.as<double>(), true, m.refine);
     CUPF_HIP(hipStreamSynchronize(m.s));
     result.transpose_solve_time_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - ts).count();
@@ -1240,7 +1259,7 @@ void NewtonSolver::solve_torch_backward(const void* grad_va, const void* grad_vm
     else
         kern::adjoint_gather<float>(m.nr, B, m.Bp, (const float*)grad_va, m.n,
                                     (const float*)grad_vm, m.n, m.g.as<double>(), m.s);
-    m.solve_into<double>(m.g.as<double>(), m.lam.as<double>(), m.ddx.as<double>(), true);
+    m.solve_into<double>(m.g.as<double>(), m.lam.as<double>(), m.ddx.as<double>(), true, m.refine);
     if (want64)
         kern::adjoint_project<double>(m.nr, B, m.Bp, m.lam.as<double>(), (double*)grad_load_p,
                                       (double*)grad_load_q, m.s);
@@ -1323,14 +1342,15 @@ void NewtonSolver::debug_factor_solve(const double* A, const double* rhs, int32_
     m.factor();
     std::vector<double> xi(size_t(m.js.dim) * m.Bp);
     if (m.dx32) {
-        m.solve_into<float>(m.g.as<double>(), m.dx.as<float>(), m.ddx.as<float>(), transpose);
+        m.solve_into<float>(m.g.as<double>(), m.dx.as<float>(), m.ddx.as<float>(), transpose, m.refine);
         std::vector<float> xf(xi.size());
         CUPF_HIP(hipMemcpyAsync(xf.data(), m.dx.get(), xf.size() * sizeof(float),
                                 hipMemcpyDeviceToHost, m.s));
         CUPF_HIP(hipStreamSynchronize(m.s));
         for (size_t i = 0; i < xf.size(); ++i) xi[i] = xf[i];
     } else {
-        m.solve_into<double>(m.g.as<double>(), m.lam.as<double>(), m.ddx.as<double>(), transpose);
+        m.solve_into<double>(m.g.as<double>(), m.lam.as<double>(), m.ddx.as<double>(), transpose,
+                             m.refine);
         CUPF_HIP(hipMemcpyAsync(xi.data(), m.lam.get(), xi.size() * sizeof(double),
                                 hipMemcpyDeviceToHost, m.s));
     }

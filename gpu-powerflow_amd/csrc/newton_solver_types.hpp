@@ -74,8 +74,9 @@ struct CustomSolverConfig {
     bool serial_nd = false;                 // ordering is always deterministic here
     int max_panel_width = 8;                // relaxed-panel cap
     double shift_retry_epsilon = 1.0e-8;    // static pivot threshold and shift
-    // Additions (not in the reference): FP64 refinement steps of the Newton
-    // step for ComputePolicy::Mixed.
+    // Additions (not in the reference): FP64 refinement steps of a Newton
+    // step for ComputePolicy::Mixed, applied to the steps taken once the
+    // batch mismatch is at most 1e-3 (the adjoint solves always refine).
     int refine_steps = 1;
 };
 
