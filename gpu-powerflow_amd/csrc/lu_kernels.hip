@@ -1008,7 +1008,9 @@ __global__ __launch_bounds__(kPivThreads) void k_u12(LuDevice d, const int* __re
     T* G = ar.a1 + (long long)b * d.arena1 + d.off[p];
     const int* rl = d.row_lb + d.lb_ptr[p];
     const T* Lo = side + ((long long)fi * Bp + b) * kSideElems + kUinvElems;
-    for (int e = tid; e < kUinvElems; e += kPivThreads) Li[(e >> 6) * kLinvLd + (e & 63)] = Lo[e];
+    T lv[kUinvElems / kPivThreads];             // L11^-1, in flight during the assembly
+#pragma unroll
+    for (int u = 0; u < kUinvElems / kPivThreads; ++u) lv[u] = Lo[tid + u * kPivThreads];
     {
         const int jn = min(kXChunk, fs - j0);
         for (int e = tid; e < 64 * kXCld; e += kPivThreads) Ch[e] = T(0);
@@ -1020,6 +1022,11 @@ __global__ __launch_bounds__(kPivThreads) void k_u12(LuDevice d, const int* __re
         __syncthreads();
         assemble_chunks<kPivThreads, 16>(d, ar, d.asm_base[p * kAsmKindCount + kAsmKChunk] + ch, b, Bp, Ch,
                                          [=](int r, int c) { return r * kXCld + c - j0; });
+#pragma unroll
+        for (int u = 0; u < kUinvElems / kPivThreads; ++u) {
+            const int e = tid + u * kPivThreads;
+            Li[(e >> 6) * kLinvLd + (e & 63)] = lv[u];
+        }
         __syncthreads();
         if constexpr (sizeof(T) == 4) {
             // 64 x 64 output = 2 x 2 tiles of 32 x 32, one per wave; L11^-1 is
