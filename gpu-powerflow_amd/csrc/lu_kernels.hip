@@ -212,13 +212,26 @@ __device__ inline void assemble_front(const LuDevice& d, const Arenas<T>& ar, co
     __syncthreads();
 }
 
+// Wave broadcast of lane `lane`'s value (v_readlane).
+template <typename T>
+__device__ __forceinline__ T bcast(T x, int lane) {
+    if constexpr (sizeof(T) == 4) {
+        return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), lane));
+    } else {
+        const long long v = __double_as_longlong(x);
+        const int lo = __builtin_amdgcn_readlane(int(v & 0xffffffffLL), lane);
+        const int hi = __builtin_amdgcn_readlane(int(v >> 32), lane);
+        return __longlong_as_double((long long)(unsigned(lo)) | ((long long)hi << 32));
+    }
+}
+
 // Tier 1: the front (fsz <= 128, at most kT1MaxK pivots) is assembled and
 // factored in LDS: wave 0 factors the pivot block with one row per lane in
-// registers (the pivot row broadcast through LDS, guarded pivots); then,
+// registers (the pivot row broadcast by v_readlane, guarded pivots); then,
 // at once, threads 0..127 form U12 = L11^-1 A12 (a column each) and threads
 // 128..255 L21 = A21 U11^-1 (a row each); the contribution block gets its
-// rank-K update C -= L21 U12 in 4 x 4 register tiles, and the front is
-// written to the arena once.
+// rank-K update C -= L21 U12 in 4 x 4 register tiles written straight to
+// the arena, then the pivot rows and columns follow from LDS.
 
 template <typename T>
 __global__ __launch_bounds__(kFactorBlock) void k_factor_t1(LuDevice d,
@@ -230,7 +243,6 @@ __global__ __launch_bounds__(kFactorBlock) void k_factor_t1(LuDevice d,
     static_assert(kFactorBlock == 256, "two 128-thread halves for the triangular solves");
     extern __shared__ unsigned char smem_raw[];
     T* F = reinterpret_cast<T*>(smem_raw);
-    __shared__ T R[kT1MaxK];                    // published pivot row
     int fi, b;
     decode_block(Bp, fi, b);
     const int p = fronts[fi];
@@ -247,17 +259,13 @@ __global__ __launch_bounds__(kFactorBlock) void k_factor_t1(LuDevice d,
 #pragma unroll
         for (int c = 0; c < kT1MaxK; ++c) {
             if (c < ncp) {                            // uniform
-                if (lane == c) {
-                    row[c] = guard_pivot(row[c], eps, static_piv, flag);
-#pragma unroll
-                    for (int cc = c; cc < kT1MaxK; ++cc) R[cc] = row[cc];
-                }
-                const T piv = R[c];
+                if (lane == c) row[c] = guard_pivot(row[c], eps, static_piv, flag);
+                const T piv = bcast(row[c], c);
                 const bool act = lane > c && lane < ncp;
                 const T l = act ? row[c] / piv : T(0);
                 if (act) row[c] = l;
 #pragma unroll
-                for (int cc = c + 1; cc < kT1MaxK; ++cc) row[cc] -= l * R[cc];
+                for (int cc = c + 1; cc < kT1MaxK; ++cc) row[cc] -= l * bcast(row[cc], c);
             }
         }
         if (lane < ncp) {
@@ -303,7 +311,9 @@ __global__ __launch_bounds__(kFactorBlock) void k_factor_t1(LuDevice d,
         }
     }
     __syncthreads();
-    // contribution block: C -= L21 U12 (rank ncp), 4 x 4 tiles per thread
+    // contribution block: C -= L21 U12 (rank ncp), 4 x 4 tiles per thread,
+    // written straight to the arena
+    T* G = ar.a1 + (long long)b * d.arena1 + d.off[p];
     const int wc = fs - ncp;
     const int tn = (wc + 3) / 4;
     for (int t = tid; t < tn * tn; t += kFactorBlock) {
@@ -330,11 +340,14 @@ __global__ __launch_bounds__(kFactorBlock) void k_factor_t1(LuDevice d,
         for (int r = 0; r < 4; ++r)
 #pragma unroll
             for (int c = 0; c < 4; ++c)
-                if (i0 + r < fs && j0 + c < fs) F[(i0 + r) * fs + j0 + c] = acc[r][c];
+                if (i0 + r < fs && j0 + c < fs) G[(long long)(i0 + r) * fs + j0 + c] = acc[r][c];
     }
-    __syncthreads();
-    T* G = ar.a1 + (long long)b * d.arena1 + d.off[p];
-    for (int e = tid; e < fs * fs; e += kFactorBlock) G[e] = F[e];
+    // the pivot rows (L11\U11, U12) and pivot columns (L21) from LDS
+    for (int e = tid; e < ncp * fs; e += kFactorBlock) G[e] = F[e];
+    for (int e = tid; e < wc * ncp; e += kFactorBlock) {
+        const int r = ncp + e / ncp, c = e - (e / ncp) * ncp;
+        G[(long long)r * fs + c] = F[r * fs + c];
+    }
 }
 
 template <typename T>
@@ -830,18 +843,6 @@ __global__ __launch_bounds__(kDenseThreads) void k_factor_t2_mfma(LuDevice d,
 // the front is never staged whole in HBM before it is factored, and the
 // long pivot columns are formed by the many Schur workgroups.
 // ===========================================================================
-// Wave broadcast of lane `lane`'s value (v_readlane).
-template <typename T>
-__device__ __forceinline__ T bcast(T x, int lane) {
-    if constexpr (sizeof(T) == 4) {
-        return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), lane));
-    } else {
-        const long long v = __double_as_longlong(x);
-        const int lo = __builtin_amdgcn_readlane(int(v & 0xffffffffLL), lane);
-        const int hi = __builtin_amdgcn_readlane(int(v >> 32), lane);
-        return __longlong_as_double((long long)(unsigned(lo)) | ((long long)hi << 32));
-    }
-}
 
 constexpr int kPivThreads = 256;
 constexpr int kXMaxK = 64;
