@@ -173,7 +173,7 @@ struct NewtonSolver::Impl {
     // once the mismatch is small; above kRefineBelow the step's own
     // quadratic error dwarfs the FP32 solve's (relative error far below
     // kRefineBelow), so the step is taken unrefined.
-    static constexpr double kRefineBelow = 1e-3;
+    static constexpr double kRefineBelow = 1e-5;
     int refine_for(double m) const { return m <= kRefineBelow ? refine : 0; }
 
     // adjoint cache

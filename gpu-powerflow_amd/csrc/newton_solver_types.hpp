@@ -76,7 +76,7 @@ struct CustomSolverConfig {
     double shift_retry_epsilon = 1.0e-8;    // static pivot threshold and shift
     // Additions (not in the reference): FP64 refinement steps of a Newton
     // step for ComputePolicy::Mixed, applied to the steps taken once the
-    // batch mismatch is at most 1e-3 (the adjoint solves always refine).
+    // batch mismatch is at most 1e-5 (the adjoint solves always refine).
     int refine_steps = 1;
 };
 

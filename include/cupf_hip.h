@@ -48,7 +48,7 @@ typedef struct {
 } cupf_solve_options;
 
 /* CustomSolverConfig (newton_solver_types.hpp:152-159) + refine_steps (FP64 refinement steps of a
- * Mixed Newton step taken once the batch mismatch is at most 1e-3) */
+ * Mixed Newton step taken once the batch mismatch is at most 1e-5) */
 typedef struct {
     int32_t precision;
     int32_t matching;
