@@ -446,11 +446,22 @@ struct NewtonSolver::Impl {
                     s = own;                        // capture on the private stream
                     capturing = true;
                     hipGraph_t g = nullptr;
-                    CUPF_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-                    step_and_residual(tolerant);
-                    const hipError_t e = hipStreamEndCapture(s, &g);
+                    hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+                    if (e == hipSuccess) {
+                        try {
+                            step_and_residual(tolerant);
+                        } catch (...) {             // end the capture, restore, rethrow
+                            (void)hipStreamEndCapture(s, &g);
+                            if (g) (void)hipGraphDestroy(g);
+                            capturing = false;
+                            s = call;
+                            throw;
+                        }
+                        e = hipStreamEndCapture(s, &g);
+                    }
                     capturing = false;
                     s = call;
+                    if (e != hipSuccess && g) (void)hipGraphDestroy(g);
                     CUPF_HIP(e);
                     const hipError_t ei = hipGraphInstantiate(&step_graph, g, nullptr, nullptr, 0);
                     (void)hipGraphDestroy(g);
