@@ -143,6 +143,29 @@ struct NewtonSolver::Impl {
     int yb_capBp = 0;
     hipStream_t own = nullptr;
     hipStream_t s = nullptr;   // stream of the current call
+    // a level's tier-0 group runs on `aux` beside its tier-1/2 groups on `s`
+    // (different fronts, inputs only from lower levels): forked from and
+    // joined back into `s` through the two events
+    hipStream_t aux = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+
+    template <class F0, class F12>
+    void level_split(bool has0, bool has12, F0 tier0, F12 tiers12) {
+        if (has0 && has12) {
+            CUPF_HIP(hipEventRecord(ev_fork, s));
+            CUPF_HIP(hipStreamWaitEvent(aux, ev_fork, 0));
+            tier0(aux);
+            CUPF_HIP(hipEventRecord(ev_join, aux));
+            tiers12();
+            CUPF_HIP(hipStreamWaitEvent(s, ev_join, 0));
+        } else {
+            if (has0) tier0(s);
+            tiers12();
+        }
+    }
+    bool has_group(int level, int tier) const {
+        return plan.group_ptr[level * 3 + tier + 1] > plan.group_ptr[level * 3 + tier];
+    }
 
     // captured Newton step (use_cuda_graph): Jacobian, factor, solve with
     // refinement, voltage update, then the next residual and its batch max.
@@ -244,23 +267,33 @@ struct NewtonSolver::Impl {
     // cross + Schur group, the whole group otherwise.
     template <typename T>
     void factor_t(kern::Arenas<T> ar) {
-        for (int L = 0; L < plan.num_levels; ++L)
-            for (int t = 0; t < 3; ++t) {
+        for (int L = 0; L < plan.num_levels; ++L) {
+            auto tier0 = [&](hipStream_t st0) {
                 int cnt, mf;
-                const int* f = group(L, t, cnt, mf);
-                if (cnt == 0) continue;
-                if (t == 2 && cross_schur(L)) {
-                    kern::cross_group<T>(lu, f, cnt, mf, J.as<double>(), Bp, ar, uinv.as<T>(), piv_eps,
-                                         static_piv ? 1 : 0, sing.as<int>(), s);
-                    StageScope sc(timer, kStFactorDense, s);
-                    kern::schur_group<T>(lu, f, cnt, mf, J.as<double>(), Bp, ar, uinv.as<T>(),
-                                         dense_mode, s);
-                    continue;
+                const int* f = group(L, 0, cnt, mf);
+                kern::lu_factor_group<T>(lu, 0, f, cnt, mf, J.as<double>(), Bp, ar, piv_eps,
+                                         static_piv ? 1 : 0, sing.as<int>(), dense_mode, st0);
+            };
+            auto tiers12 = [&]() {
+                for (int t = 1; t < 3; ++t) {
+                    int cnt, mf;
+                    const int* f = group(L, t, cnt, mf);
+                    if (cnt == 0) continue;
+                    if (t == 2 && cross_schur(L)) {
+                        kern::cross_group<T>(lu, f, cnt, mf, J.as<double>(), Bp, ar, uinv.as<T>(),
+                                             piv_eps, static_piv ? 1 : 0, sing.as<int>(), s);
+                        StageScope sc(timer, kStFactorDense, s);
+                        kern::schur_group<T>(lu, f, cnt, mf, J.as<double>(), Bp, ar, uinv.as<T>(),
+                                             dense_mode, s);
+                        continue;
+                    }
+                    StageScope sc(timer, t == 2 ? kStFactorDense : -1, s);
+                    kern::lu_factor_group<T>(lu, t, f, cnt, mf, J.as<double>(), Bp, ar, piv_eps,
+                                             static_piv ? 1 : 0, sing.as<int>(), dense_mode, s);
                 }
-                StageScope sc(timer, t == 2 ? kStFactorDense : -1, s);
-                kern::lu_factor_group<T>(lu, t, f, cnt, mf, J.as<double>(), Bp, ar, piv_eps,
-                                         static_piv ? 1 : 0, sing.as<int>(), dense_mode, s);
-            }
+            };
+            level_split(has_group(L, 0), has_group(L, 1) || has_group(L, 2), tier0, tiers12);
+        }
     }
     void factor() {
         if (front64) factor_t<double>({arena0.as<double>(), arena1.as<double>()});
@@ -272,20 +305,26 @@ struct NewtonSolver::Impl {
     void solve_once(kern::Arenas<T> ar, const double* rhs, TX* x, bool transpose) {
         T* y = ywork.as<T>();
         T* u = upd.as<T>();
+        auto fwd = [&](int L, int t, hipStream_t st) {
+            int cnt, mf;
+            const int* f = group(L, t, cnt, mf);
+            kern::lu_solve_forward_group<T>(lu, t, f, cnt, mf, plan.group_max_nc[L * 3 + t], ar,
+                                            rhs, y, u, Bp, transpose, st);
+        };
+        auto bwd = [&](int L, int t, hipStream_t st) {
+            int cnt, mf;
+            const int* f = group(L, t, cnt, mf);
+            kern::lu_solve_backward_group<T>(lu, t, f, cnt, mf, plan.group_max_nc[L * 3 + t], ar,
+                                             y, Bp, transpose, st);
+        };
         for (int L = 0; L < plan.num_levels; ++L)
-            for (int t = 0; t < 3; ++t) {
-                int cnt, mf;
-                const int* f = group(L, t, cnt, mf);
-                kern::lu_solve_forward_group<T>(lu, t, f, cnt, mf, plan.group_max_nc[L * 3 + t], ar,
-                                                rhs, y, u, Bp, transpose, s);
-            }
+            level_split(has_group(L, 0), has_group(L, 1) || has_group(L, 2),
+                        [&](hipStream_t st0) { fwd(L, 0, st0); },
+                        [&]() { fwd(L, 1, s); fwd(L, 2, s); });
         for (int L = plan.num_levels - 1; L >= 0; --L)
-            for (int t = 0; t < 3; ++t) {
-                int cnt, mf;
-                const int* f = group(L, t, cnt, mf);
-                kern::lu_solve_backward_group<T>(lu, t, f, cnt, mf, plan.group_max_nc[L * 3 + t], ar,
-                                                 y, Bp, transpose, s);
-            }
+            level_split(has_group(L, 0), has_group(L, 1) || has_group(L, 2),
+                        [&](hipStream_t st0) { bwd(L, 0, st0); },
+                        [&]() { bwd(L, 1, s); bwd(L, 2, s); });
         kern::lu_solve_scatter<T, TX>(lu, y, Bp, x, s);
     }
 
@@ -692,6 +731,9 @@ NewtonSolver::NewtonSolver(const NewtonOptions& options) : impl_(new Impl) {
                    options.custom.shift_retry_epsilon > 0.0;
     m.piv_eps = options.custom.shift_retry_epsilon;
     CUPF_HIP(hipStreamCreate(&m.own));
+    CUPF_HIP(hipStreamCreate(&m.aux));
+    CUPF_HIP(hipEventCreateWithFlags(&m.ev_fork, hipEventDisableTiming));
+    CUPF_HIP(hipEventCreateWithFlags(&m.ev_join, hipEventDisableTiming));
     m.s = m.own;
 }
 
@@ -701,6 +743,12 @@ NewtonSolver::~NewtonSolver() {
         (void)hipStreamSynchronize(impl_->own);
         (void)hipStreamDestroy(impl_->own);
     }
+    if (impl_ && impl_->aux) {
+        (void)hipStreamSynchronize(impl_->aux);
+        (void)hipStreamDestroy(impl_->aux);
+    }
+    if (impl_ && impl_->ev_fork) (void)hipEventDestroy(impl_->ev_fork);
+    if (impl_ && impl_->ev_join) (void)hipEventDestroy(impl_->ev_join);
 }
 
 const NewtonSolverStats& NewtonSolver::stats() const { return impl_->st; }
