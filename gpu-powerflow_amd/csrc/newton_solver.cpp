@@ -143,9 +143,9 @@ struct NewtonSolver::Impl {
     int yb_capBp = 0;
     hipStream_t own = nullptr;
     hipStream_t s = nullptr;   // stream of the current call
-    // a level's tier-0 group runs on `aux` beside its tier-1/2 groups on `s`
-    // (different fronts, inputs only from lower levels): forked from and
-    // joined back into `s` through the two events
+    // the groups of one level touch different fronts and read only lower
+    // levels: the lowest tier present runs on `aux` beside the others on `s`,
+    // forked from and joined back into `s` through the two events
     hipStream_t aux = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 
@@ -165,6 +165,17 @@ struct NewtonSolver::Impl {
     }
     bool has_group(int level, int tier) const {
         return plan.group_ptr[level * 3 + tier + 1] > plan.group_ptr[level * 3 + tier];
+    }
+    // lowest tier present at a level (3 when none)
+    int first_tier(int level) const {
+        int t = 0;
+        while (t < 3 && !has_group(level, t)) ++t;
+        return t;
+    }
+    bool has_after(int level, int tier) const {
+        for (int t = tier + 1; t < 3; ++t)
+            if (has_group(level, t)) return true;
+        return false;
     }
 
     // captured Newton step (use_cuda_graph): Jacobian, factor, solve with
@@ -267,32 +278,27 @@ struct NewtonSolver::Impl {
     // cross + Schur group, the whole group otherwise.
     template <typename T>
     void factor_t(kern::Arenas<T> ar) {
+        auto run = [&](int L, int t, hipStream_t st) {
+            int cnt, mf;
+            const int* f = group(L, t, cnt, mf);
+            if (cnt == 0) return;
+            if (t == 2 && cross_schur(L)) {
+                kern::cross_group<T>(lu, f, cnt, mf, J.as<double>(), Bp, ar, uinv.as<T>(), piv_eps,
+                                     static_piv ? 1 : 0, sing.as<int>(), st);
+                StageScope sc(timer, kStFactorDense, st);
+                kern::schur_group<T>(lu, f, cnt, mf, J.as<double>(), Bp, ar, uinv.as<T>(),
+                                     dense_mode, st);
+                return;
+            }
+            StageScope sc(timer, t == 2 ? kStFactorDense : -1, st);
+            kern::lu_factor_group<T>(lu, t, f, cnt, mf, J.as<double>(), Bp, ar, piv_eps,
+                                     static_piv ? 1 : 0, sing.as<int>(), dense_mode, st);
+        };
         for (int L = 0; L < plan.num_levels; ++L) {
-            auto tier0 = [&](hipStream_t st0) {
-                int cnt, mf;
-                const int* f = group(L, 0, cnt, mf);
-                kern::lu_factor_group<T>(lu, 0, f, cnt, mf, J.as<double>(), Bp, ar, piv_eps,
-                                         static_piv ? 1 : 0, sing.as<int>(), dense_mode, st0);
-            };
-            auto tiers12 = [&]() {
-                for (int t = 1; t < 3; ++t) {
-                    int cnt, mf;
-                    const int* f = group(L, t, cnt, mf);
-                    if (cnt == 0) continue;
-                    if (t == 2 && cross_schur(L)) {
-                        kern::cross_group<T>(lu, f, cnt, mf, J.as<double>(), Bp, ar, uinv.as<T>(),
-                                             piv_eps, static_piv ? 1 : 0, sing.as<int>(), s);
-                        StageScope sc(timer, kStFactorDense, s);
-                        kern::schur_group<T>(lu, f, cnt, mf, J.as<double>(), Bp, ar, uinv.as<T>(),
-                                             dense_mode, s);
-                        continue;
-                    }
-                    StageScope sc(timer, t == 2 ? kStFactorDense : -1, s);
-                    kern::lu_factor_group<T>(lu, t, f, cnt, mf, J.as<double>(), Bp, ar, piv_eps,
-                                             static_piv ? 1 : 0, sing.as<int>(), dense_mode, s);
-                }
-            };
-            level_split(has_group(L, 0), has_group(L, 1) || has_group(L, 2), tier0, tiers12);
+            const int t0 = first_tier(L);
+            if (t0 > 2) continue;
+            level_split(true, has_after(L, t0), [&](hipStream_t st) { run(L, t0, st); },
+                        [&]() { for (int t = t0 + 1; t < 3; ++t) run(L, t, s); });
         }
     }
     void factor() {
@@ -317,14 +323,18 @@ struct NewtonSolver::Impl {
             kern::lu_solve_backward_group<T>(lu, t, f, cnt, mf, plan.group_max_nc[L * 3 + t], ar,
                                              y, Bp, transpose, st);
         };
-        for (int L = 0; L < plan.num_levels; ++L)
-            level_split(has_group(L, 0), has_group(L, 1) || has_group(L, 2),
-                        [&](hipStream_t st0) { fwd(L, 0, st0); },
-                        [&]() { fwd(L, 1, s); fwd(L, 2, s); });
-        for (int L = plan.num_levels - 1; L >= 0; --L)
-            level_split(has_group(L, 0), has_group(L, 1) || has_group(L, 2),
-                        [&](hipStream_t st0) { bwd(L, 0, st0); },
-                        [&]() { bwd(L, 1, s); bwd(L, 2, s); });
+        for (int L = 0; L < plan.num_levels; ++L) {
+            const int t0 = first_tier(L);
+            if (t0 > 2) continue;
+            level_split(true, has_after(L, t0), [&](hipStream_t st) { fwd(L, t0, st); },
+                        [&]() { for (int t = t0 + 1; t < 3; ++t) fwd(L, t, s); });
+        }
+        for (int L = plan.num_levels - 1; L >= 0; --L) {
+            const int t0 = first_tier(L);
+            if (t0 > 2) continue;
+            level_split(true, has_after(L, t0), [&](hipStream_t st) { bwd(L, t0, st); },
+                        [&]() { for (int t = t0 + 1; t < 3; ++t) bwd(L, t, s); });
+        }
         kern::lu_solve_scatter<T, TX>(lu, y, Bp, x, s);
     }
 
