@@ -298,11 +298,11 @@ def main():
             "panel_cap": args.cap,
             "form": {"learned": f"learned MLP warm start (<= {args.warm_iters} NR iterations, "
                                 f"flat-start fallback) + FP32 fronts, FP32-MFMA dense trailing "
-                                f"updates, {args.refine} FP64 refinement steps",
+                                f"updates, {args.refine} FP64 refinement step(s) per Newton step at ||F||inf <= 1e-5",
                      "mixed": f"FP32 fronts, FP32-MFMA dense trailing updates, "
-                              f"{args.refine} FP64 refinement steps",
+                              f"{args.refine} FP64 refinement step(s) per Newton step at ||F||inf <= 1e-5",
                      "bf16x3": f"FP32 fronts, 3xBF16 MFMA dense trailing updates, "
-                               f"{args.refine} FP64 refinement steps",
+                               f"{args.refine} FP64 refinement step(s) per Newton step at ||F||inf <= 1e-5",
                      "fp64": "FP64 fronts"}[head],
             "learned_warm_start": head == "learned",
         },
