@@ -622,7 +622,7 @@ LuPlan lower_plan(const Symbolic& s, int n, const int* rp, const int* ci, int ti
 }
 
 // ===========================================================================
-// Assembly task lists (see analyze.hpp).
+// Assembly lists: tasks and chunk maps (see analyze.hpp).
 // ===========================================================================
 AsmLists build_asm_lists(const LuPlan& L, const std::vector<int>& kinds,
                          const std::vector<int>& rows_per_block, int schur_rows) {

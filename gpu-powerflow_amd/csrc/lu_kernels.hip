@@ -8,15 +8,19 @@
 //           arena (element e of scenario b at (off + e) * Bp + b), so every
 //           access of the wave is one contiguous segment and all control flow
 //           and index loads are wave-uniform.  No synchronisation at all.
-//   tier 1  fsz <= tier1_max: one workgroup per (front, scenario); the front
-//           is assembled and factored in LDS and stored scenario-major.
-//   tier 2  larger: one workgroup per (front, scenario); blocked right-looking
+//   tier 1  fsz <= tier1_max, at most 32 pivots: one workgroup per (front,
+//           scenario); the front is assembled and factored in LDS and stored
+//           scenario-major.
+//   tier 2  larger: fronts with at most 64 pivots and fsz <= 512 take the
+//           pivot + Schur path (k_pivot, k_u12, k_schur: many workgroups per
+//           front and scenario, MFMA products); others a blocked right-looking
 //           LU in the scenario-major arena with the column panel staged and
-//           factored in LDS, U12 solved column-per-lane and the trailing block
-//           updated in 64x64 tiles of 4x4 register micro-tiles.
+//           factored in LDS and the trailing block updated in 64x64 tiles of
+//           4x4 register micro-tiles.
 //
-// Assembly is owner-computes: a front zeroes itself, stores its original
-// entries and gathers each child's contribution block in a fixed order
+// Assembly is owner-computes: a block of a front is zeroed, gets its
+// original entries and then the children's contributions from the host-built
+// chunk maps, each element owned by one wave that adds in a fixed order
 // (deterministic; no atomics).  No pivoting: |pivot| <= eps becomes
 // sign(pivot)*eps and the scenario's singular flag is raised (reference
 // GuardedPivot, custom_linear_solver/src/factorize/front_ops.cuh:146-160; the
@@ -982,7 +986,7 @@ __global__ __launch_bounds__(pivot_threads<T>()) void k_pivot(LuDevice d, const 
 }
 
 // U12 = L11^-1 A12 for one chunk of kXChunk columns of one (front, scenario):
-// A12 chunk assembled in LDS (entries, then the children's staged walk),
+// A12 chunk assembled in LDS (entries, then the chunk map),
 // L11^-1 from the side buffer, the product on MFMA (FP32) or vector FMA tiles
 // (FP64), written to the front's pivot rows.
 template <typename T>
