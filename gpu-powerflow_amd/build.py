@@ -39,6 +39,7 @@ SOURCES = {
     "nr_kernels.hip": ["-ffp-contract=off"],
     "lu_kernels.hip": [],
     "newton_solver.cpp": [],
+    "batched_lu.cpp": [],
     "analyze.cpp": [],
     "capi.cpp": [],
     "mlp_kernels.hip": [],

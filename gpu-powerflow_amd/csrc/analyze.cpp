@@ -229,28 +229,18 @@ std::vector<int> column_counts(int n, const int* cp, const int* ri, const std::v
 // minimum degree on J with its natural supervariables.  The elimination graph
 // is one bitset per bus; ties go to the lowest bus.
 // ===========================================================================
-std::vector<int> order_jacobian(const JacobianStructure& js) {
-    const int m = int(js.pvpq.size());   // vertices = pvpq positions
+namespace {
+
+// Minimum (weighted) degree elimination order of an m-vertex graph given as
+// dense adjacency bit rows (adj[a * W + w], W = ceil(m / 64) words per row,
+// no self loops; consumed).  A vertex's degree counts its live neighbours,
+// those marked heavy twice; ties go to the smallest key.  Elimination joins
+// the eliminated vertex's live neighbourhood into a clique.
+std::vector<int> min_degree(int m, std::vector<uint64_t>& adj, const std::vector<uint64_t>& heavy,
+                            const std::vector<int>& key) {
     const int W = (m + 63) / 64;
-    std::vector<int> pos_of_bus(js.n_bus, -1);
-    for (int i = 0; i < m; ++i) pos_of_bus[js.pvpq[i]] = i;
-    std::vector<uint64_t> adj(size_t(m) * W, 0), alive(W, 0), heavy(W, 0);
-    auto bit = [](std::vector<uint64_t>& v, size_t base, int b) {
-        v[base + (b >> 6)] |= uint64_t(1) << (b & 63);
-    };
-    // J is structurally symmetric and its row pattern lists the neighbouring
-    // J columns; collapse them to bus positions.
-    for (int r = 0; r < js.dim; ++r) {
-        const int a = pos_of_bus[js.row_bus[r]];
-        for (int k = js.row_ptr[r]; k < js.row_ptr[r + 1]; ++k) {
-            const int b = pos_of_bus[js.row_bus[js.col_idx[k]]];
-            if (a != b) { bit(adj, size_t(a) * W, b); bit(adj, size_t(b) * W, a); }
-        }
-    }
-    for (int i = 0; i < m; ++i) {
-        alive[i >> 6] |= uint64_t(1) << (i & 63);
-        if (i >= js.n_pv) heavy[i >> 6] |= uint64_t(1) << (i & 63);
-    }
+    std::vector<uint64_t> alive(W, 0);
+    for (int i = 0; i < m; ++i) alive[i >> 6] |= uint64_t(1) << (i & 63);
     auto degree = [&](int a) {
         const uint64_t* row = &adj[size_t(a) * W];
         int d = 0;
@@ -263,19 +253,18 @@ std::vector<int> order_jacobian(const JacobianStructure& js) {
     std::vector<int> deg(m);
     for (int i = 0; i < m; ++i) deg[i] = degree(i);
     std::vector<char> gone(m, 0);
-    std::vector<int> bus_order;
-    bus_order.reserve(m);
+    std::vector<int> order;
+    order.reserve(m);
     std::vector<int> nbr;
     for (int step = 0; step < m; ++step) {
-        int v = -1, best = std::numeric_limits<int>::max(), vbus = 0;
+        int v = -1, best = std::numeric_limits<int>::max(), vkey = 0;
         for (int i = 0; i < m; ++i) {
             if (gone[i]) continue;
-            const int b = js.pvpq[i];
-            if (deg[i] < best || (deg[i] == best && b < vbus)) { best = deg[i]; v = i; vbus = b; }
+            if (deg[i] < best || (deg[i] == best && key[i] < vkey)) { best = deg[i]; v = i; vkey = key[i]; }
         }
         gone[v] = 1;
         alive[v >> 6] &= ~(uint64_t(1) << (v & 63));
-        bus_order.push_back(v);
+        order.push_back(v);
         nbr.clear();
         const uint64_t* rv = &adj[size_t(v) * W];
         for (int w = 0; w < W; ++w) {
@@ -289,6 +278,57 @@ std::vector<int> order_jacobian(const JacobianStructure& js) {
         }
         for (int u : nbr) deg[u] = degree(u);
     }
+    return order;
+}
+
+// order[new] = old relabelled by the elimination-tree postorder of the
+// pattern (rp, ci) under that order.
+std::vector<int> postorder_relabel(int n, const int* rp, const int* ci,
+                                   const std::vector<int>& order) {
+    std::vector<int> iord(n);
+    for (int k = 0; k < n; ++k) iord[order[k]] = k;
+    std::vector<int> cp, ri;
+    symmetric_pattern(n, rp, ci, cp, ri);
+    std::vector<int> pcp(n + 1, 0), pri(ri.size());
+    for (int newc = 0; newc < n; ++newc) {
+        const int oldc = order[newc];
+        pcp[newc + 1] = pcp[newc] + (cp[oldc + 1] - cp[oldc]);
+    }
+    for (int newc = 0; newc < n; ++newc) {
+        const int oldc = order[newc];
+        int w = pcp[newc];
+        for (int p = cp[oldc]; p < cp[oldc + 1]; ++p) pri[w++] = iord[ri[p]];
+    }
+    const auto par = etree(n, pcp.data(), pri.data());
+    const auto post = postorder(par);
+    std::vector<int> perm(n);
+    for (int k = 0; k < n; ++k) perm[k] = order[post[k]];
+    return perm;
+}
+
+}  // namespace
+
+std::vector<int> order_jacobian(const JacobianStructure& js) {
+    const int m = int(js.pvpq.size());   // vertices = pvpq positions
+    const int W = (m + 63) / 64;
+    std::vector<int> pos_of_bus(js.n_bus, -1);
+    for (int i = 0; i < m; ++i) pos_of_bus[js.pvpq[i]] = i;
+    std::vector<uint64_t> adj(size_t(m) * W, 0), heavy(W, 0);
+    auto bit = [](std::vector<uint64_t>& v, size_t base, int b) {
+        v[base + (b >> 6)] |= uint64_t(1) << (b & 63);
+    };
+    // J is structurally symmetric and its row pattern lists the neighbouring
+    // J columns; collapse them to bus positions.
+    for (int r = 0; r < js.dim; ++r) {
+        const int a = pos_of_bus[js.row_bus[r]];
+        for (int k = js.row_ptr[r]; k < js.row_ptr[r + 1]; ++k) {
+            const int b = pos_of_bus[js.row_bus[js.col_idx[k]]];
+            if (a != b) { bit(adj, size_t(a) * W, b); bit(adj, size_t(b) * W, a); }
+        }
+    }
+    // a PQ bus carries two J columns: its neighbours count twice
+    for (int i = js.n_pv; i < m; ++i) heavy[i >> 6] |= uint64_t(1) << (i & 63);
+    const std::vector<int> bus_order = min_degree(m, adj, heavy, js.pvpq);
     // Expand to J indices (angle row, then magnitude row of a PQ bus).
     std::vector<int> order;
     order.reserve(js.dim);
@@ -297,26 +337,23 @@ std::vector<int> order_jacobian(const JacobianStructure& js) {
         order.push_back(js.row_pvpq[b]);
         if (js.row_pq[b] >= 0) order.push_back(js.row_pq[b]);
     }
-    // Relabel by the etree postorder of the ordered pattern.
-    std::vector<int> iord(js.dim);
-    for (int k = 0; k < js.dim; ++k) iord[order[k]] = k;
-    std::vector<int> cp, ri;
-    symmetric_pattern(js.dim, js.row_ptr.data(), js.col_idx.data(), cp, ri);
-    std::vector<int> pcp(js.dim + 1, 0), pri(ri.size());
-    for (int newc = 0; newc < js.dim; ++newc) {
-        const int oldc = order[newc];
-        pcp[newc + 1] = pcp[newc] + (cp[oldc + 1] - cp[oldc]);
-    }
-    for (int newc = 0; newc < js.dim; ++newc) {
-        const int oldc = order[newc];
-        int w = pcp[newc];
-        for (int p = cp[oldc]; p < cp[oldc + 1]; ++p) pri[w++] = iord[ri[p]];
-    }
-    const auto par = etree(js.dim, pcp.data(), pri.data());
-    const auto post = postorder(par);
-    std::vector<int> perm(js.dim);
-    for (int k = 0; k < js.dim; ++k) perm[k] = order[post[k]];
-    return perm;
+    return postorder_relabel(js.dim, js.row_ptr.data(), js.col_idx.data(), order);
+}
+
+std::vector<int> order_matrix(int n, const int* rp, const int* ci) {
+    const int W = (n + 63) / 64;
+    std::vector<uint64_t> adj(size_t(n) * W, 0), heavy(W, 0);
+    for (int r = 0; r < n; ++r)
+        for (int k = rp[r]; k < rp[r + 1]; ++k) {
+            const int c = ci[k];
+            if (c < 0 || c >= n) throw std::invalid_argument("order_matrix: column index out of range");
+            if (c == r) continue;
+            adj[size_t(r) * W + (c >> 6)] |= uint64_t(1) << (c & 63);
+            adj[size_t(c) * W + (r >> 6)] |= uint64_t(1) << (r & 63);
+        }
+    std::vector<int> key(n);
+    for (int i = 0; i < n; ++i) key[i] = i;
+    return postorder_relabel(n, rp, ci, min_degree(n, adj, heavy, key));
 }
 
 // ===========================================================================

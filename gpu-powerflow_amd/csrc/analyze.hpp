@@ -67,6 +67,11 @@ void fill_pattern(int n, const int* cp, const int* ri, const std::vector<int>& p
 // relabelled by the elimination-tree postorder so panels are contiguous.
 std::vector<int> order_jacobian(const JacobianStructure& js);
 
+// Fill-reducing, postordered column order (perm[new] = old) of any square
+// CSR pattern: minimum degree on the graph of A + A^T (ties to the smaller
+// index), relabelled by the elimination-tree postorder.
+std::vector<int> order_matrix(int n, const int* rp, const int* ci);
+
 // Full symbolic pipeline for a given order (no reordering is done here).
 Symbolic analyze_symbolic(int n, const int* rp, const int* ci, const std::vector<int>& perm,
                           int cap);

@@ -1,0 +1,374 @@
+// Batched multifrontal sparse LU: host driver of the factor and solve
+// launches (see batched_lu.hpp).
+#include "batched_lu.hpp"
+
+#include <algorithm>
+#include <stdexcept>
+
+namespace cupf {
+
+// The second stream and its events are made by the first analyze(), so an
+// object that is never analysed touches no GPU.
+BatchedLU::BatchedLU() = default;
+
+BatchedLU::~BatchedLU() {
+    if (aux_) {
+        (void)hipStreamSynchronize(aux_);
+        (void)hipStreamDestroy(aux_);
+    }
+    if (ev_fork_) (void)hipEventDestroy(ev_fork_);
+    if (ev_join_) (void)hipEventDestroy(ev_join_);
+}
+
+void BatchedLU::analyze(int n, const int* rp, const int* ci, const std::vector<int>& perm,
+                        const LuOptions& opt, hipStream_t s) {
+    if (n <= 0 || !rp || (rp[n] > 0 && !ci))
+        throw std::invalid_argument("BatchedLU::analyze(): empty or null pattern");
+    if (opt.cap < 1) throw std::invalid_argument("BatchedLU::analyze(): panel cap must be >= 1");
+    n_ = 0;
+    opt_ = opt;
+    if (!aux_) {
+        CUPF_HIP(hipStreamCreate(&aux_));
+        CUPF_HIP(hipEventCreateWithFlags(&ev_fork_, hipEventDisableTiming));
+        CUPF_HIP(hipEventCreateWithFlags(&ev_join_, hipEventDisableTiming));
+    }
+    const int nnz = rp[n];
+    sym_ = analyze_symbolic(n, rp, ci, perm, opt.cap);
+    // tier 1 holds a whole front in 64 KiB of LDS
+    static_assert(kern::kT1MaxK >= kTier1MaxPivots, "tier-1 fronts fit the tier-1 factor kernel");
+    const int tier1_max = opt.front64 ? 90 : 128;
+    plan_ = lower_plan(sym_, n, rp, ci, 16, tier1_max);
+    const int lds_max = opt.front64 ? 8192 : 16384;
+    if (plan_.max_fsz > lds_max)
+        throw std::runtime_error("BatchedLU::analyze(): a front exceeds the LDS panel budget");
+
+    // transpose pattern with value map (refinement residual of A^T solves)
+    std::vector<int> trp(n + 1, 0), tci(nnz), tmap(nnz);
+    for (int k = 0; k < nnz; ++k) ++trp[ci[k] + 1];
+    for (int c = 0; c < n; ++c) trp[c + 1] += trp[c];
+    {
+        std::vector<int> nx(trp.begin(), trp.end() - 1);
+        for (int r = 0; r < n; ++r)
+            for (int k = rp[r]; k < rp[r + 1]; ++k) {
+                const int pos = nx[ci[k]]++;
+                tci[pos] = r;
+                tmap[pos] = k;
+            }
+    }
+    d_rp = upload_vector(std::vector<int>(rp, rp + n + 1), s);
+    d_ci = upload_vector(std::vector<int>(ci, ci + nnz), s);
+    d_trp = upload_vector(trp, s);
+    d_tci = upload_vector(tci, s);
+    d_tmap = upload_vector(tmap, s);
+
+    const LuPlan& P = plan_;
+    d_fsz = upload_vector(P.fsz, s);
+    d_nc = upload_vector(P.nc, s);
+    d_tier = upload_vector(P.tier, s);
+    d_off = upload_vector(P.off, s);
+    d_uoff = upload_vector(P.uoff, s);
+    d_fptr = upload_vector(P.front_ptr, s);
+    d_frows = upload_vector(P.front_rows, s);
+    d_cptr = upload_vector(P.child_ptr, s);
+    d_cidx = upload_vector(P.child_idx, s);
+    d_aptr = upload_vector(P.asm_ptr, s);
+    d_aloc = upload_vector(P.asm_loc, s);
+    d_eptr = upload_vector(P.ent_ptr, s);
+    d_esrc = upload_vector(P.ent_src, s);
+    d_edst = upload_vector(P.ent_dst, s);
+    d_perm = upload_vector(P.perm, s);
+    d_groups = upload_vector(P.group_fronts, s);
+    d_lbptr = upload_vector(P.lb_ptr, s);
+    d_rowlb = upload_vector(P.row_lb, s);
+    {
+        // assembly lists for the kernels each front's group will run
+        std::vector<int> kinds(size_t(P.num_fronts), 0), rpb(size_t(P.num_fronts), 1);
+        for (int L = 0; L < P.num_levels; ++L)
+            for (int t = 1; t < 3; ++t) {
+                const int gi = L * 3 + t;
+                for (int q = P.group_ptr[gi]; q < P.group_ptr[gi + 1]; ++q) {
+                    const int f = P.group_fronts[q];
+                    if (t == 1) {
+                        kinds[size_t(f)] = 1 << kern::kAsmKFront;
+                    } else if (cross_schur(L)) {
+                        kinds[size_t(f)] = (1 << kern::kAsmKSchur) | (1 << kern::kAsmKPivot) |
+                                           (1 << kern::kAsmKChunk);
+                    } else {
+                        kinds[size_t(f)] = 1 << kern::kAsmKRows;
+                        rpb[size_t(f)] =
+                            kern::assemble_rows_block(P.group_max_fsz[gi], int(front_bytes()));
+                    }
+                }
+            }
+        const AsmLists al =
+            build_asm_lists(P, kinds, rpb, kern::schur_block_rows(int(front_bytes())));
+        d_abase = upload_vector(al.base, s);
+        d_gptr = upload_vector(P.gat_ptr, s);
+        d_gsrc = upload_vector(P.gat_src, s);
+        d_ridx = upload_vector(P.rhs_idx, s);
+        d_acptr = upload_vector(al.chunk_ptr, s);
+        d_achunks = upload_vector(al.chunks, s);
+    }
+    CUPF_HIP(hipStreamSynchronize(s));
+
+    kern::LuDevice& lu = lu_;
+    lu = kern::LuDevice{};
+    lu.n = n; lu.nnz = nnz; lu.P = P.num_fronts;
+    lu.arena0 = P.arena0; lu.arena1 = P.arena1; lu.upd_total = P.upd_total;
+    lu.fsz = d_fsz.as<int>(); lu.nc = d_nc.as<int>(); lu.tier = d_tier.as<int>();
+    lu.off = d_off.as<long long>(); lu.uoff = d_uoff.as<long long>();
+    lu.front_ptr = d_fptr.as<int>(); lu.front_rows = d_frows.as<int>();
+    lu.child_ptr = d_cptr.as<int>(); lu.child_idx = d_cidx.as<int>();
+    lu.asm_ptr = d_aptr.as<int>(); lu.asm_loc = d_aloc.as<int>();
+    lu.ent_ptr = d_eptr.as<int>(); lu.ent_src = d_esrc.as<int>();
+    lu.ent_dst = d_edst.as<int>(); lu.perm = d_perm.as<int>();
+    lu.lb_ptr = d_lbptr.as<int>(); lu.row_lb = d_rowlb.as<int>();
+    lu.asm_base = d_abase.as<int>();
+    lu.asm_cptr = d_acptr.as<int>(); lu.asm_chunks = d_achunks.as<int4>();
+    lu.gat_ptr = d_gptr.as<int>(); lu.gat_src = d_gsrc.as<int>(); lu.rhs_idx = d_ridx.as<int>();
+
+    // Algorithmic work of the timed dense stage, per scenario.
+    //  pivot + Schur groups (the Schur kernel): flops 2 uc^2 K (update) +
+    //    2 uc K^2 (L21); bytes = the rows >= K written once (CB uc^2, L21
+    //    uc K) + the child contribution elements landing in them read once +
+    //    U12 (K uc) read once, at the front element size, + the values of
+    //    those rows at 8 B;
+    //  other tier-2 groups (assembly + dense LU): flops of the front LU;
+    //    bytes = each front element written and read once + its values.
+    dense_flops_ = dense_bytes_ = 0.0;
+    dense_groups_ = 0;
+    for (int p = 0; p < P.num_fronts; ++p) {
+        if (P.tier[p] != 2) continue;
+        const int fsz = P.fsz[p], K = P.nc[p];
+        const double f = fsz;
+        if (cross_schur(P.level[p])) {
+            const double uc = f - K;
+            double child = 0.0;
+            const int* rl = P.row_lb.data() + P.lb_ptr[p];
+            for (int q = P.child_ptr[p]; q < P.child_ptr[p + 1]; ++q) {
+                const int c = P.child_idx[q];
+                const int* cl = rl + (1 + q - P.child_ptr[p]) * (fsz + 1);
+                const double cuc = P.fsz[c] - P.nc[c];
+                child += (cuc - cl[K]) * cuc;          // contribution rows landing at >= K
+            }
+            const double ents = rl[fsz] - rl[K];
+            dense_flops_ += 2.0 * uc * uc * K + 2.0 * uc * K * K;
+            dense_bytes_ += double(front_bytes()) * (uc * uc + child + 2.0 * uc * K) + 8.0 * ents;
+            continue;
+        }
+        double fl = 0.0;
+        for (int k = 0; k < K; ++k) {
+            const double r = f - k - 1;
+            fl += 2.0 * r * r + r;
+        }
+        dense_flops_ += fl;
+        dense_bytes_ += 2.0 * double(front_bytes()) * f * f +
+                        8.0 * double(P.ent_ptr[p + 1] - P.ent_ptr[p]);
+    }
+    for (int g = 0; g < P.num_levels; ++g)
+        if (P.group_ptr[g * 3 + 3] > P.group_ptr[g * 3 + 2]) ++dense_groups_;
+
+    // batch buffers are sized lazily for the new structure
+    capBp_ = 0;
+    for (DeviceBuffer* b : {&arena0_, &arena1_, &ywork_, &upd_, &res_, &sing_, &uinv_}) b->release();
+    n_ = n;
+    nnz_ = nnz;
+}
+
+bool BatchedLU::reserve(int Bp) {
+    if (!analyzed()) throw std::runtime_error("BatchedLU: analyze() must be called first");
+    if (Bp % kern::kLanes) throw std::invalid_argument("BatchedLU::reserve(): Bp must be padded to 64");
+    if (Bp <= capBp_) return false;
+    const size_t nb = size_t(Bp);
+    const size_t ts = front_bytes();
+    arena0_.allocate(nb * std::max<size_t>(1, size_t(plan_.arena0)) * ts);
+    arena1_.allocate(nb * std::max<size_t>(1, size_t(plan_.arena1)) * ts);
+    ywork_.allocate(nb * n_ * ts);
+    // interleaved (tier-0 fronts) then scenario-major (tier-1/2 fronts)
+    upd_.allocate(2 * nb * std::max<size_t>(1, size_t(plan_.upd_total)) * ts);
+    res_.allocate(nb * n_ * sizeof(double));
+    sing_.allocate(nb * sizeof(int));
+    int cross_count = 0;
+    for (int L = 0; L < plan_.num_levels; ++L)
+        if (cross_schur(L))
+            cross_count = std::max(cross_count, plan_.group_ptr[L * 3 + 3] - plan_.group_ptr[L * 3 + 2]);
+    if (cross_count > 0) uinv_.allocate(kern::pivot_side_elems(cross_count, Bp) * ts);
+    capBp_ = Bp;
+    return true;
+}
+
+void BatchedLU::clear_singular(int Bp, hipStream_t s) {
+    CUPF_HIP(hipMemsetAsync(sing_.get(), 0, size_t(Bp) * sizeof(int), s));
+}
+
+// Joins aux back into s when a launch between the fork and the join throws,
+// so a stream under capture never ends the capture with an unjoined fork.
+struct BatchedLU::AuxJoin {
+    BatchedLU& lu;
+    hipStream_t s;
+    bool done = false;
+    void join() {
+        CUPF_HIP(hipEventRecord(lu.ev_join_, lu.aux_));
+        CUPF_HIP(hipStreamWaitEvent(s, lu.ev_join_, 0));
+        done = true;
+    }
+    ~AuxJoin() {
+        if (done) return;
+        (void)hipEventRecord(lu.ev_join_, lu.aux_);
+        (void)hipStreamWaitEvent(s, lu.ev_join_, 0);
+    }
+};
+
+// The groups of one level touch different fronts and read only lower
+// levels: the lowest tier present runs on aux beside the others on s,
+// forked from and joined back into s through two events.
+template <class F0, class F12>
+void BatchedLU::level_split(bool has0, bool has12, F0 tier0, F12 tiers12, hipStream_t s) {
+    if (has0 && has12) {
+        CUPF_HIP(hipEventRecord(ev_fork_, s));
+        CUPF_HIP(hipStreamWaitEvent(aux_, ev_fork_, 0));
+        AuxJoin j{*this, s};
+        tier0(aux_);
+        tiers12();
+        j.join();
+    } else {
+        if (has0) tier0(s);
+        tiers12();
+    }
+}
+
+bool BatchedLU::has_group(int level, int tier) const {
+    return plan_.group_ptr[level * 3 + tier + 1] > plan_.group_ptr[level * 3 + tier];
+}
+
+int BatchedLU::first_tier(int level) const {
+    int t = 0;
+    while (t < 3 && !has_group(level, t)) ++t;
+    return t;
+}
+
+bool BatchedLU::has_after(int level, int tier) const {
+    for (int t = tier + 1; t < 3; ++t)
+        if (has_group(level, t)) return true;
+    return false;
+}
+
+const int* BatchedLU::group(int level, int tier, int& count, int& max_fsz) const {
+    const int gi = level * 3 + tier;
+    count = plan_.group_ptr[gi + 1] - plan_.group_ptr[gi];
+    max_fsz = plan_.group_max_fsz[gi];
+    return d_groups.as<int>() + plan_.group_ptr[gi];
+}
+
+// tier-2 group factored by the pivot + Schur kernels
+bool BatchedLU::cross_schur(int level) const {
+    const int gi = level * 3 + 2;
+    return kern::cross_schur_path(plan_.group_max_fsz[gi], plan_.group_max_nc[gi]);
+}
+
+// Stage `stage` times the dense tier-2 work: the Schur launch of a pivot +
+// Schur group, the whole group otherwise.
+template <typename T>
+void BatchedLU::factor_t(const double* A, int Bp, hipStream_t s, StageTimer* timer, int stage) {
+    const kern::Arenas<T> ar{arena0_.as<T>(), arena1_.as<T>()};
+    const int sp = opt_.static_pivot ? 1 : 0;
+    auto run = [&](int L, int t, hipStream_t st) {
+        int cnt, mf;
+        const int* f = group(L, t, cnt, mf);
+        if (cnt == 0) return;
+        if (t == 2 && cross_schur(L)) {
+            kern::cross_group<T>(lu_, f, cnt, mf, A, Bp, ar, uinv_.as<T>(), opt_.pivot_eps, sp,
+                                 sing_.as<int>(), st);
+            StageScope sc(timer, stage, st);
+            kern::schur_group<T>(lu_, f, cnt, mf, A, Bp, ar, uinv_.as<T>(), opt_.dense_mode, st);
+            return;
+        }
+        StageScope sc(timer, t == 2 ? stage : -1, st);
+        kern::lu_factor_group<T>(lu_, t, f, cnt, mf, A, Bp, ar, opt_.pivot_eps, sp,
+                                 sing_.as<int>(), opt_.dense_mode, st);
+    };
+    for (int L = 0; L < plan_.num_levels; ++L) {
+        const int t0 = first_tier(L);
+        if (t0 > 2) continue;
+        level_split(true, has_after(L, t0), [&](hipStream_t st) { run(L, t0, st); },
+                    [&]() { for (int t = t0 + 1; t < 3; ++t) run(L, t, s); }, s);
+    }
+}
+
+void BatchedLU::factor(const double* A, int Bp, hipStream_t s, StageTimer* timer, int stage) {
+    if (Bp > capBp_) throw std::runtime_error("BatchedLU::factor(): batch exceeds the reserved size");
+    if (opt_.front64) factor_t<double>(A, Bp, s, timer, stage);
+    else factor_t<float>(A, Bp, s, timer, stage);
+}
+
+template <typename T, typename TX>
+void BatchedLU::solve_once(const double* rhs, TX* x, bool transpose, int Bp, hipStream_t s) {
+    const kern::Arenas<T> ar{arena0_.as<T>(), arena1_.as<T>()};
+    T* y = ywork_.as<T>();
+    T* u = upd_.as<T>();
+    auto fwd = [&](int L, int t, hipStream_t st) {
+        int cnt, mf;
+        const int* f = group(L, t, cnt, mf);
+        kern::lu_solve_forward_group<T>(lu_, t, f, cnt, mf, plan_.group_max_nc[L * 3 + t], ar, rhs,
+                                        y, u, Bp, transpose, st);
+    };
+    auto bwd = [&](int L, int t, hipStream_t st) {
+        int cnt, mf;
+        const int* f = group(L, t, cnt, mf);
+        kern::lu_solve_backward_group<T>(lu_, t, f, cnt, mf, plan_.group_max_nc[L * 3 + t], ar, y,
+                                         Bp, transpose, st);
+    };
+    for (int L = 0; L < plan_.num_levels; ++L) {
+        const int t0 = first_tier(L);
+        if (t0 > 2) continue;
+        level_split(true, has_after(L, t0), [&](hipStream_t st) { fwd(L, t0, st); },
+                    [&]() { for (int t = t0 + 1; t < 3; ++t) fwd(L, t, s); }, s);
+    }
+    for (int L = plan_.num_levels - 1; L >= 0; --L) {
+        const int t0 = first_tier(L);
+        if (t0 > 2) continue;
+        level_split(true, has_after(L, t0), [&](hipStream_t st) { bwd(L, t0, st); },
+                    [&]() { for (int t = t0 + 1; t < 3; ++t) bwd(L, t, s); }, s);
+    }
+    kern::lu_solve_scatter<T, TX>(lu_, y, Bp, x, s);
+}
+
+template <typename TX>
+void BatchedLU::solve_dispatch(const double* rhs, TX* x, bool transpose, int Bp, hipStream_t s) {
+    if (opt_.front64) solve_once<double, TX>(rhs, x, transpose, Bp, s);
+    else solve_once<float, TX>(rhs, x, transpose, Bp, s);
+}
+
+template <typename TX>
+void BatchedLU::residual(const double* A, const double* rhs, const TX* x, double* r,
+                         bool transpose, int Bp, hipStream_t s) const {
+    if (transpose)
+        kern::refine_residual<TX>(d_trp.as<int>(), d_tci.as<int>(), d_tmap.as<int>(), A, n_, Bp,
+                                  rhs, x, r, s);
+    else
+        kern::refine_residual<TX>(d_rp.as<int>(), d_ci.as<int>(), nullptr, A, n_, Bp, rhs, x, r, s);
+}
+
+template <typename TX>
+void BatchedLU::solve(const double* A, const double* rhs, TX* x, TX* xtmp, bool transpose,
+                      int steps, int Bp, hipStream_t s) {
+    if (Bp > capBp_) throw std::runtime_error("BatchedLU::solve(): batch exceeds the reserved size");
+    solve_dispatch<TX>(rhs, x, transpose, Bp, s);
+    if (opt_.front64) return;
+    for (int it = 0; it < steps; ++it) {
+        residual<TX>(A, rhs, x, res_.as<double>(), transpose, Bp, s);
+        solve_dispatch<TX>(res_.as<double>(), xtmp, transpose, Bp, s);
+        kern::axpy_update<TX>(xtmp, x, (long long)Bp * n_, s);
+    }
+}
+
+template void BatchedLU::solve<double>(const double*, const double*, double*, double*, bool, int,
+                                       int, hipStream_t);
+template void BatchedLU::solve<float>(const double*, const double*, float*, float*, bool, int, int,
+                                      hipStream_t);
+template void BatchedLU::residual<double>(const double*, const double*, const double*, double*,
+                                          bool, int, hipStream_t) const;
+template void BatchedLU::residual<float>(const double*, const double*, const float*, double*, bool,
+                                         int, hipStream_t) const;
+
+}  // namespace cupf

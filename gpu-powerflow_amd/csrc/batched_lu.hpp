@@ -1,0 +1,125 @@
+// Batched multifrontal sparse LU of one square CSR pattern for many value
+// sets at once.
+//
+// One object holds the host analysis of the pattern (fill-reducing order,
+// symbolic factorisation, lowered plan, assembly chunk maps and solve gather
+// maps), its device copy, the per-batch front arenas and work vectors, and
+// the launch sequences of the factorisation and of the (transpose) solve with
+// FP64 iterative refinement.  NewtonSolver factors its Jacobian with it; the
+// inner-solver C ABI (cupf_lu_*) exposes it for any square CSR matrix.
+//
+// Values, right-hand sides and solutions are scenario-interleaved here
+// (x[i * Bp + b], Bp = batch padded to 64); callers convert at the boundary.
+#pragma once
+
+#include "analyze.hpp"
+#include "hip_common.hpp"
+#include "kernels.hpp"
+#include "stage_timer.hpp"
+
+#include <vector>
+
+namespace cupf {
+
+struct LuOptions {
+    bool front64 = true;        // FP64 fronts; otherwise FP32 fronts
+    int dense_mode = 0;         // FP32 dense fronts: 1 MFMA FP32, 2 MFMA 3xBF16
+    int cap = 8;                // relaxed-panel cap
+    double pivot_eps = 1e-8;    // static pivot threshold and replacement magnitude
+    bool static_pivot = true;   // replace small pivots (StaticDiagonalShift)
+};
+
+class BatchedLU {
+public:
+    BatchedLU();
+    ~BatchedLU();
+    BatchedLU(const BatchedLU&) = delete;
+    BatchedLU& operator=(const BatchedLU&) = delete;
+
+    // Host analysis of the n x n pattern (rp, ci) under perm (perm[new] = old)
+    // and upload of the plan on stream s (synchronised before returning).
+    // Drops every batch buffer.
+    void analyze(int n, const int* rp, const int* ci, const std::vector<int>& perm,
+                 const LuOptions& opt, hipStream_t s);
+    bool analyzed() const { return n_ > 0; }
+
+    // Size the batch buffers for Bp (a multiple of 64) scenarios; true when
+    // any buffer moved (a captured graph that used them is stale).
+    bool reserve(int Bp);
+    int capacity() const { return capBp_; }
+
+    // Clear the per-scenario singular-pivot flags.
+    void clear_singular(int Bp, hipStream_t s);
+    const int* singular() const { return sing_.as<int>(); }
+
+    // Factor the values A [nnz][Bp] (interleaved FP64).  dense_timer, when
+    // set, times the dense tier-2 launches under stage dense_stage.
+    void factor(const double* A, int Bp, hipStream_t s, StageTimer* dense_timer = nullptr,
+                int dense_stage = -1);
+
+    // x = A^-1 rhs (or A^-T rhs) on the factors of the last factor(); rhs
+    // [n][Bp] FP64; x and xtmp [n][Bp] of TX.  With FP32 fronts, `steps`
+    // FP64 refinement steps follow (residual against the FP64 values A).
+    template <typename TX>
+    void solve(const double* A, const double* rhs, TX* x, TX* xtmp, bool transpose, int steps,
+               int Bp, hipStream_t s);
+
+    // r = rhs - op(A) x with op(A) = A or A^T (FP64 residual).
+    template <typename TX>
+    void residual(const double* A, const double* rhs, const TX* x, double* r, bool transpose,
+                  int Bp, hipStream_t s) const;
+
+    const LuPlan& plan() const { return plan_; }
+    const Symbolic& symbolic() const { return sym_; }
+    const LuOptions& options() const { return opt_; }
+    int n() const { return n_; }
+    int nnz() const { return nnz_; }
+    size_t front_bytes() const { return opt_.front64 ? sizeof(double) : sizeof(float); }
+    // Work of the timed dense stage per scenario (see analyze()).
+    double dense_flops() const { return dense_flops_; }
+    double dense_bytes() const { return dense_bytes_; }
+    int dense_groups() const { return dense_groups_; }
+    // residual work buffer [n][Bp] FP64 (free between calls)
+    double* res() const { return res_.as<double>(); }
+
+private:
+    struct AuxJoin;
+    template <class F0, class F12>
+    void level_split(bool has0, bool has12, F0 tier0, F12 tiers12, hipStream_t s);
+    bool has_group(int level, int tier) const;
+    int first_tier(int level) const;
+    bool has_after(int level, int tier) const;
+    const int* group(int level, int tier, int& count, int& max_fsz) const;
+    bool cross_schur(int level) const;
+    template <typename T>
+    void factor_t(const double* A, int Bp, hipStream_t s, StageTimer* timer, int stage);
+    template <typename T, typename TX>
+    void solve_once(const double* rhs, TX* x, bool transpose, int Bp, hipStream_t s);
+    template <typename TX>
+    void solve_dispatch(const double* rhs, TX* x, bool transpose, int Bp, hipStream_t s);
+
+    LuOptions opt_;
+    int n_ = 0, nnz_ = 0;
+    Symbolic sym_;
+    LuPlan plan_;
+    double dense_flops_ = 0.0, dense_bytes_ = 0.0;
+    int dense_groups_ = 0;
+
+    // device plan
+    DeviceBuffer d_fsz, d_nc, d_tier, d_off, d_uoff, d_fptr, d_frows, d_cptr, d_cidx, d_aptr,
+        d_aloc, d_eptr, d_esrc, d_edst, d_perm, d_groups, d_lbptr, d_rowlb, d_abase, d_acptr,
+        d_achunks, d_gptr, d_gsrc, d_ridx;
+    // pattern and its transpose with value map (refinement residuals)
+    DeviceBuffer d_rp, d_ci, d_trp, d_tci, d_tmap;
+    kern::LuDevice lu_;
+
+    // batch buffers
+    int capBp_ = 0;
+    DeviceBuffer arena0_, arena1_, ywork_, upd_, res_, sing_, uinv_;
+
+    // the lowest tier present at a level runs on aux beside the others
+    hipStream_t aux_ = nullptr;
+    hipEvent_t ev_fork_ = nullptr, ev_join_ = nullptr;
+};
+
+}  // namespace cupf

@@ -4,13 +4,19 @@
 #include "../../include/cupf_hip.h"
 
 #include "analyze.hpp"
+#include "batched_lu.hpp"
+#include "hip_common.hpp"
+#include "kernels.hpp"
 #include "newton_solver.hpp"
 #include "warm_start.hpp"
 
+#include <algorithm>
 #include <complex>
 #include <cstring>
+#include <memory>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 struct cupf_solver {
     NewtonSolver* impl;
@@ -18,6 +24,25 @@ struct cupf_solver {
 
 struct cupf_warm_start {
     cupf::WarmStartMLP* impl;
+};
+
+// Inner solver behind the phase C ABI: one batched LU, the batch it is set up
+// for and its interleaved staging buffers.
+struct cupf_lu {
+    cupf::BatchedLU lu;
+    cupf::LuOptions opt;
+    int refine = 1;
+    int n = 0, nnz = 0;
+    int B = 0, Bp = 0;
+    bool factored = false;
+    hipStream_t own = nullptr;
+    cupf::DeviceBuffer A, rhs, x, xtmp;
+    ~cupf_lu() {
+        if (own) {
+            (void)hipStreamSynchronize(own);
+            (void)hipStreamDestroy(own);
+        }
+    }
 };
 
 namespace {
@@ -145,6 +170,43 @@ YbusView make_view(int32_t rows, int32_t cols, const int32_t* indptr, const int3
                     cols, nnz};
 }
 
+// Status codes of the inner-solver entry points (custom_linear_solver
+// Status, solver.hpp:21-29): the phase that failed names the code.
+struct LuFailure : std::runtime_error {
+    int status;
+    LuFailure(int st, const std::string& m) : std::runtime_error(m), status(st) {}
+};
+
+template <typename F>
+int lu_guarded(int phase_status, F&& f) {
+    try {
+        f();
+        g_last_error.clear();
+        return CUPF_LU_SUCCESS;
+    } catch (const LuFailure& e) {
+        g_last_error = e.what();
+        return e.status;
+    } catch (const std::invalid_argument& e) {
+        g_last_error = e.what();
+        return CUPF_LU_INVALID_VALUE;
+    } catch (const std::bad_alloc& e) {
+        g_last_error = e.what();
+        return CUPF_LU_ALLOCATION_FAILED;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        const std::string m = e.what();
+        return m.find("hipMalloc") != std::string::npos ? CUPF_LU_ALLOCATION_FAILED : phase_status;
+    } catch (...) {
+        g_last_error = "unknown error";
+        return phase_status;
+    }
+}
+
+cupf_lu& lu_of(cupf_lu* h) {
+    if (!h) throw std::invalid_argument("null inner-solver handle");
+    return *h;
+}
+
 }  // namespace
 
 extern "C" {
@@ -258,6 +320,25 @@ int cupf_solver_solve_batch(cupf_solver* solver, int32_t rows, int32_t cols,
     });
 }
 
+int cupf_solver_solve(cupf_solver* solver, int32_t rows, int32_t cols, const int32_t* indptr,
+                      const int32_t* indices, const double* data, int32_t nnz, const double* sbus,
+                      const double* V0, const int32_t* pv, int32_t n_pv, const int32_t* pq,
+                      int32_t n_pq, const cupf_nr_config* config,
+                      const cupf_solve_options* options, double* V_out, int32_t* iterations,
+                      double* final_mismatch, uint8_t* converged) {
+    return guarded([&] {
+        NRResult r;
+        solver_of(solver).solve(make_view(rows, cols, indptr, indices, data, nnz),
+                                reinterpret_cast<const std::complex<double>*>(sbus),
+                                reinterpret_cast<const std::complex<double>*>(V0), pv, n_pv, pq,
+                                n_pq, to_cfg(config), to_sopt(options), r);
+        if (V_out) std::memcpy(V_out, r.V.data(), r.V.size() * sizeof(std::complex<double>));
+        if (iterations) *iterations = r.iterations;
+        if (final_mismatch) *final_mismatch = r.final_mismatch;
+        if (converged) *converged = r.converged ? 1 : 0;
+    });
+}
+
 int cupf_solver_solve_adjoint(cupf_solver* solver, const double* grad_va, int64_t grad_va_stride,
                               const double* grad_vm, int64_t grad_vm_stride, int32_t batch,
                               const int32_t* pv, int32_t n_pv, const int32_t* pq, int32_t n_pq,
@@ -319,6 +400,7 @@ int cupf_solver_get_stats(cupf_solver* solver, cupf_solver_stats* out) {
         out->dense_groups = s.dense_groups;
         out->last_warm_iterations = s.last_warm_iterations;
         out->last_warm_fallbacks = s.last_warm_fallbacks;
+        out->state_version = s.state_version;
     });
 }
 
@@ -691,6 +773,180 @@ int cupf_mlp_set_normalization(cupf_warm_start* mlp, const float* x_mean, const 
 int cupf_solver_set_warm_start(cupf_solver* solver, cupf_warm_start* mlp, int32_t max_warm_iter) {
     return guarded([&] {
         solver_of(solver).set_warm_start(mlp ? &mlp_of(mlp) : nullptr, max_warm_iter);
+    });
+}
+
+// ---- inner solver phase API (custom_linear_solver::Solver) ---------------
+
+void cupf_lu_default_config(cupf_lu_config* o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->precision = CUPF_PRECISION_FP64;        // SolverConfig::precision default
+    o->max_panel_width = 8;
+    o->pivot_strategy = 1;                     // StaticDiagonalShift
+    o->shift_retry_epsilon = 1.0e-8;
+    o->refine_steps = 1;
+}
+
+const char* cupf_lu_status_string(int status) {
+    switch (status) {
+        case CUPF_LU_SUCCESS: return "success";
+        case CUPF_LU_INVALID_VALUE: return "invalid value";
+        case CUPF_LU_INVALID_STATE: return "invalid state";
+        case CUPF_LU_ALLOCATION_FAILED: return "allocation failed";
+        case CUPF_LU_ANALYSIS_FAILED: return "analysis failed";
+        case CUPF_LU_FACTORIZATION_FAILED: return "factorization failed";
+        case CUPF_LU_SOLVE_FAILED: return "solve failed";
+        default: return "unknown status";
+    }
+}
+
+int cupf_lu_create(const cupf_lu_config* config, cupf_lu** out) {
+    return lu_guarded(CUPF_LU_INVALID_VALUE, [&] {
+        if (!out) throw std::invalid_argument("cupf_lu_create: out must not be null");
+        *out = nullptr;
+        cupf_lu_config c;
+        cupf_lu_default_config(&c);
+        if (config) c = *config;
+        if (c.precision < CUPF_PRECISION_FP64 || c.precision > CUPF_PRECISION_TF32)
+            throw std::invalid_argument("cupf_lu_create: unknown precision");
+        if (c.max_panel_width < 1)
+            throw std::invalid_argument("cupf_lu_create: max_panel_width must be >= 1");
+        if (c.refine_steps < 0) throw std::invalid_argument("cupf_lu_create: refine_steps must be >= 0");
+        auto h = std::make_unique<cupf_lu>();
+        h->opt.front64 = c.precision == CUPF_PRECISION_FP64;
+        h->opt.dense_mode = h->opt.front64 ? 0 : (c.precision == CUPF_PRECISION_TF32 ? 2 : 1);
+        h->opt.cap = c.max_panel_width;
+        h->opt.pivot_eps = c.shift_retry_epsilon;
+        h->opt.static_pivot = c.pivot_strategy == 1 && c.shift_retry_epsilon > 0.0;
+        h->refine = c.refine_steps;
+        CUPF_HIP(hipStreamCreate(&h->own));
+        *out = h.release();
+    });
+}
+
+int cupf_lu_destroy(cupf_lu* h) {
+    return lu_guarded(CUPF_LU_INVALID_STATE, [&] { delete h; });
+}
+
+int cupf_lu_analyze(cupf_lu* h, int32_t n, int32_t nnz, const int32_t* row_ptr,
+                    const int32_t* col_idx, int32_t location) {
+    return lu_guarded(CUPF_LU_ANALYSIS_FAILED, [&] {
+        cupf_lu& m = lu_of(h);
+        if (n <= 0 || nnz <= 0 || !row_ptr || !col_idx)
+            throw std::invalid_argument("cupf_lu_analyze: empty or null pattern");
+        if (location != CUPF_LU_HOST && location != CUPF_LU_DEVICE)
+            throw std::invalid_argument("cupf_lu_analyze: location must be host or device");
+        std::vector<int> rp(size_t(n) + 1), ci(nnz);
+        if (location == CUPF_LU_DEVICE) {
+            CUPF_HIP(hipMemcpy(rp.data(), row_ptr, rp.size() * sizeof(int), hipMemcpyDeviceToHost));
+            CUPF_HIP(hipMemcpy(ci.data(), col_idx, ci.size() * sizeof(int), hipMemcpyDeviceToHost));
+        } else {
+            std::copy(row_ptr, row_ptr + n + 1, rp.begin());
+            std::copy(col_idx, col_idx + nnz, ci.begin());
+        }
+        if (rp[0] != 0 || rp[n] != nnz)
+            throw std::invalid_argument("cupf_lu_analyze: row_ptr is inconsistent with nnz");
+        for (int r = 0; r < n; ++r) {
+            if (rp[r + 1] < rp[r]) throw std::invalid_argument("cupf_lu_analyze: row_ptr must be non-decreasing");
+            for (int k = rp[r]; k < rp[r + 1]; ++k) {
+                if (ci[k] < 0 || ci[k] >= n)
+                    throw std::invalid_argument("cupf_lu_analyze: column index out of range");
+                if (k > rp[r] && ci[k] <= ci[k - 1])
+                    throw std::invalid_argument("cupf_lu_analyze: columns must be sorted and unique in each row");
+            }
+        }
+        m.factored = false;
+        m.B = m.Bp = 0;
+        for (auto* b : {&m.A, &m.rhs, &m.x, &m.xtmp}) b->release();
+        m.n = 0;
+        m.lu.analyze(n, rp.data(), ci.data(), cupf::order_matrix(n, rp.data(), ci.data()), m.opt,
+                     m.own);
+        m.n = n;
+        m.nnz = nnz;
+    });
+}
+
+int cupf_lu_setup(cupf_lu* h, int32_t batch) {
+    return lu_guarded(CUPF_LU_ALLOCATION_FAILED, [&] {
+        cupf_lu& m = lu_of(h);
+        if (m.n == 0) throw LuFailure(CUPF_LU_INVALID_STATE, "cupf_lu_setup: analyze first");
+        if (batch <= 0) throw std::invalid_argument("cupf_lu_setup: batch must be positive");
+        m.factored = false;
+        const int Bp = cupf::kern::pad_batch(batch);
+        m.lu.reserve(Bp);
+        if (Bp > m.Bp) {
+            m.A.allocate(size_t(Bp) * m.nnz * sizeof(double));
+            m.rhs.allocate(size_t(Bp) * m.n * sizeof(double));
+            m.x.allocate(size_t(Bp) * m.n * sizeof(double));
+            m.xtmp.allocate(size_t(Bp) * m.n * sizeof(double));
+        }
+        m.B = batch;
+        m.Bp = std::max(m.Bp, Bp);
+    });
+}
+
+int cupf_lu_factor(cupf_lu* h, const double* values, void* stream) {
+    return lu_guarded(CUPF_LU_FACTORIZATION_FAILED, [&] {
+        cupf_lu& m = lu_of(h);
+        if (m.B == 0) throw LuFailure(CUPF_LU_INVALID_STATE, "cupf_lu_factor: setup first");
+        if (!values) throw std::invalid_argument("cupf_lu_factor: values must not be null");
+        const hipStream_t s = stream ? static_cast<hipStream_t>(stream) : m.own;
+        const int Bp = cupf::kern::pad_batch(m.B);
+        cupf::kern::to_interleaved(values, m.nnz, m.nnz, m.B, Bp, m.A.as<double>(), s);
+        m.lu.clear_singular(Bp, s);
+        m.lu.factor(m.A.as<double>(), Bp, s);
+        m.factored = true;
+        if (!stream) CUPF_HIP(hipStreamSynchronize(s));
+    });
+}
+
+static int lu_solve_common(cupf_lu* h, const double* rhs, double* x, void* stream, bool transpose) {
+    return lu_guarded(CUPF_LU_SOLVE_FAILED, [&] {
+        cupf_lu& m = lu_of(h);
+        if (!m.factored) throw LuFailure(CUPF_LU_INVALID_STATE, "cupf_lu_solve: factor first");
+        if (!rhs || !x) throw std::invalid_argument("cupf_lu_solve: rhs and x must not be null");
+        const hipStream_t s = stream ? static_cast<hipStream_t>(stream) : m.own;
+        const int Bp = cupf::kern::pad_batch(m.B);
+        cupf::kern::to_interleaved(rhs, m.n, m.n, m.B, Bp, m.rhs.as<double>(), s);
+        m.lu.solve<double>(m.A.as<double>(), m.rhs.as<double>(), m.x.as<double>(),
+                           m.xtmp.as<double>(), transpose, m.refine, Bp, s);
+        cupf::kern::from_interleaved(m.x.as<double>(), m.n, m.B, Bp, x, m.n, s);
+        if (!stream) CUPF_HIP(hipStreamSynchronize(s));
+    });
+}
+
+int cupf_lu_solve(cupf_lu* h, const double* rhs, double* x, void* stream) {
+    return lu_solve_common(h, rhs, x, stream, false);
+}
+
+int cupf_lu_solve_transpose(cupf_lu* h, const double* rhs, double* x, void* stream) {
+    return lu_solve_common(h, rhs, x, stream, true);
+}
+
+int cupf_lu_singular(cupf_lu* h, uint8_t* flags) {
+    return lu_guarded(CUPF_LU_INVALID_STATE, [&] {
+        cupf_lu& m = lu_of(h);
+        if (!m.factored) throw LuFailure(CUPF_LU_INVALID_STATE, "cupf_lu_singular: factor first");
+        if (!flags) throw std::invalid_argument("cupf_lu_singular: flags must not be null");
+        std::vector<int> sg(m.B);
+        CUPF_HIP(hipMemcpy(sg.data(), m.lu.singular(), sg.size() * sizeof(int), hipMemcpyDeviceToHost));
+        for (int b = 0; b < m.B; ++b) flags[b] = uint8_t(sg[b] != 0);
+    });
+}
+
+int cupf_lu_info(cupf_lu* h, int64_t* out6) {
+    return lu_guarded(CUPF_LU_INVALID_STATE, [&] {
+        cupf_lu& m = lu_of(h);
+        if (m.n == 0) throw LuFailure(CUPF_LU_INVALID_STATE, "cupf_lu_info: analyze first");
+        if (!out6) throw std::invalid_argument("cupf_lu_info: out must not be null");
+        const cupf::LuPlan& P = m.lu.plan();
+        out6[0] = m.n;
+        out6[1] = m.nnz;
+        out6[2] = P.num_fronts;
+        out6[3] = P.max_fsz;
+        out6[4] = P.lnz;
+        out6[5] = P.arena0 + P.arena1;
     });
 }
 

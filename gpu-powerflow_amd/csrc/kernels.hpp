@@ -40,6 +40,10 @@ void to_interleaved(const double2* src, long long src_stride, int rows, int B, i
                     double2* dst, hipStream_t s);
 void from_interleaved(const double2* src, int rows, int B, int Bp, double2* dst,
                       long long dst_stride, hipStream_t s);
+void to_interleaved(const double* src, long long src_stride, int rows, int B, int Bp, double* dst,
+                    hipStream_t s);
+void from_interleaved(const double* src, int rows, int B, int Bp, double* dst,
+                      long long dst_stride, hipStream_t s);
 
 // ---- NR stages (all arrays interleaved) ------------------------------------
 // Va = arg V, Vm = |V| (cuPF storage_convert.cu:22-38 semantics).

@@ -9,8 +9,10 @@
 #include "warm_start.hpp"
 
 #include "analyze.hpp"
+#include "batched_lu.hpp"
 #include "hip_common.hpp"
 #include "kernels.hpp"
+#include "stage_timer.hpp"
 
 #include <algorithm>
 #include <chrono>
@@ -30,57 +32,6 @@ void validate_batch_args(int32_t batch_size, int64_t stride, int32_t n_bus, cons
         throw std::invalid_argument(std::string("NewtonSolver::solve_batch(): ") + name +
                                     " stride must be at least n_bus");
 }
-
-// Pairs of HIP events recorded around each stage on the stream it runs on.
-struct StageTimer {
-    static constexpr int kStages = 7;
-    bool on = false;
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> pool;
-    std::vector<int> stage_of;
-    size_t used = 0;
-    ~StageTimer() {
-        for (auto& e : pool) {
-            (void)hipEventDestroy(e.first);
-            (void)hipEventDestroy(e.second);
-        }
-    }
-    int begin(int stage, hipStream_t s) {
-        if (!on || stage < 0) return -1;
-        if (used == pool.size()) {
-            std::pair<hipEvent_t, hipEvent_t> e;
-            CUPF_HIP(hipEventCreate(&e.first));
-            CUPF_HIP(hipEventCreate(&e.second));
-            pool.push_back(e);
-            stage_of.push_back(stage);
-        }
-        stage_of[used] = stage;
-        CUPF_HIP(hipEventRecord(pool[used].first, s));
-        return int(used++);
-    }
-    void end(int idx, hipStream_t s) {
-        if (idx >= 0) CUPF_HIP(hipEventRecord(pool[idx].second, s));
-    }
-    void collect(double* ms, int64_t* count) {
-        for (int k = 0; k < kStages; ++k) { ms[k] = 0.0; count[k] = 0; }
-        for (size_t i = 0; i < used; ++i) {
-            CUPF_HIP(hipEventSynchronize(pool[i].second));
-            float t = 0.f;
-            CUPF_HIP(hipEventElapsedTime(&t, pool[i].first, pool[i].second));
-            ms[stage_of[i]] += t;
-            count[stage_of[i]] += 1;
-        }
-        used = 0;
-    }
-};
-
-struct StageScope {
-    StageTimer& t;
-    hipStream_t s;
-    int idx;
-    StageScope(StageTimer& timer, int stage, hipStream_t stream)
-        : t(timer), s(stream), idx(timer.begin(stage, stream)) {}
-    ~StageScope() { t.end(idx, s); }
-};
 
 enum {
     kStResidual = 0, kStJacobian = 1, kStFactor = 2, kStSolve = 3, kStUpdate = 4,
@@ -110,26 +61,18 @@ struct NewtonSolver::Impl {
     std::vector<int> yp, yi;
     std::vector<int> pv, pq;
     JacobianStructure js;
-    Symbolic sym;
-    LuPlan plan;
     std::vector<int> zero_slots;
-    std::vector<int> jt_rp, jt_ci, jt_map;
 
     // device structure
     DeviceBuffer d_yp, d_yi, d_y, d_row_pvpq, d_row_pq, d_m11, d_m21, d_m12, d_m22, d_zero,
-        d_row_bus, d_row_isq, d_jrp, d_jci, d_jtrp, d_jtci, d_jtmap;
-    DeviceBuffer d_fsz, d_nc, d_tier, d_off, d_uoff, d_fptr, d_frows, d_cptr, d_cidx, d_aptr,
-        d_aloc, d_eptr, d_esrc, d_edst, d_perm, d_groups, d_lbptr, d_rowlb, d_abase,
-        d_acptr, d_achunks, d_gptr, d_gsrc, d_ridx;
+        d_row_bus, d_row_isq;
     kern::NrDevice nr;
-    kern::LuDevice lu;
+    BatchedLU lu;          // the Jacobian's batched multifrontal LU
 
     // batch buffers (scenario-interleaved, Bp = batch padded to 64)
     int capBp = 0, capB = 0;
     int B = 0, Bp = 0;
-    DeviceBuffer V, Va, Vm, S, F, normF, bmax, J, arena0, arena1, ywork, upd, dx, ddx, res, sing,
-        lam, g, stage_a, stage_b, stage_c;
-    DeviceBuffer uinv;     // U11^-1 side buffer of the pivot + Schur groups
+    DeviceBuffer V, Va, Vm, S, F, normF, bmax, J, dx, ddx, lam, g, stage_a, stage_b, stage_c;
 
     // learned warm start (not owned)
     WarmStartMLP* warm = nullptr;
@@ -143,40 +86,15 @@ struct NewtonSolver::Impl {
     int yb_capBp = 0;
     hipStream_t own = nullptr;
     hipStream_t s = nullptr;   // stream of the current call
-    // the groups of one level touch different fronts and read only lower
-    // levels: the lowest tier present runs on `aux` beside the others on `s`,
-    // forked from and joined back into `s` through the two events
-    hipStream_t aux = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-
-    template <class F0, class F12>
-    void level_split(bool has0, bool has12, F0 tier0, F12 tiers12) {
-        if (has0 && has12) {
-            CUPF_HIP(hipEventRecord(ev_fork, s));
-            CUPF_HIP(hipStreamWaitEvent(aux, ev_fork, 0));
-            tier0(aux);
-            CUPF_HIP(hipEventRecord(ev_join, aux));
-            tiers12();
-            CUPF_HIP(hipStreamWaitEvent(s, ev_join, 0));
-        } else {
-            if (has0) tier0(s);
-            tiers12();
+    // Sets the call's stream for the scope of an entry point and restores the
+    // private stream on every exit, exceptions included.
+    struct CallStream {
+        Impl& m;
+        CallStream(Impl& impl, void* stream) : m(impl) {
+            m.s = stream ? static_cast<hipStream_t>(stream) : m.own;
         }
-    }
-    bool has_group(int level, int tier) const {
-        return plan.group_ptr[level * 3 + tier + 1] > plan.group_ptr[level * 3 + tier];
-    }
-    // lowest tier present at a level (3 when none)
-    int first_tier(int level) const {
-        int t = 0;
-        while (t < 3 && !has_group(level, t)) ++t;
-        return t;
-    }
-    bool has_after(int level, int tier) const {
-        for (int t = tier + 1; t < 3; ++t)
-            if (has_group(level, t)) return true;
-        return false;
-    }
+        ~CallStream() { m.s = m.own; }
+    };
 
     // captured Newton step (use_cuda_graph): Jacobian, factor, solve with
     // refinement, voltage update, then the next residual and its batch max.
@@ -197,9 +115,9 @@ struct NewtonSolver::Impl {
     bool graph_warm = false;          // one eager step first (first-use kernel attributes)
     bool capturing = false;
     void drop_graph() {
-        for (auto& g : step_graphs) {
-            if (g) (void)hipGraphExecDestroy(g);
-            g = nullptr;
+        for (auto& gr : step_graphs) {
+            if (gr) (void)hipGraphExecDestroy(gr);
+            gr = nullptr;
         }
     }
 
@@ -210,17 +128,21 @@ struct NewtonSolver::Impl {
     static constexpr double kRefineBelow = 1e-5;
     int refine_for(double m) const { return m <= kRefineBelow ? refine : 0; }
 
+    // state version (see NewtonSolverStats::state_version)
+    int64_t version = 0;
+    void bump() { st.state_version = ++version; }
+
     // adjoint cache
     bool cache = false;
     int cache_B = 0;
     int last_B = 0;
 
-    size_t tsize() const { return front64 ? sizeof(double) : sizeof(float); }
     size_t xsize() const { return dx32 ? sizeof(float) : sizeof(double); }
 
     void ensure_batch(int batch) {
         B = batch;
         Bp = kern::pad_batch(batch);
+        if (lu.reserve(Bp)) ++buf_gen;
         if (Bp > capBp) {
             ++buf_gen;
             const size_t nb = size_t(Bp);
@@ -232,23 +154,10 @@ struct NewtonSolver::Impl {
             normF.allocate(nb * sizeof(double));
             bmax.allocate(sizeof(double));
             J.allocate(nb * js.nnz * sizeof(double));
-            arena0.allocate(nb * std::max<size_t>(1, size_t(plan.arena0)) * tsize());
-            arena1.allocate(nb * std::max<size_t>(1, size_t(plan.arena1)) * tsize());
-            ywork.allocate(nb * js.dim * tsize());
-            // interleaved (tier-0 fronts) then scenario-major (tier-1/2 fronts)
-            upd.allocate(2 * nb * std::max<size_t>(1, size_t(plan.upd_total)) * tsize());
             dx.allocate(nb * js.dim * xsize());
             ddx.allocate(nb * js.dim * xsize());
-            res.allocate(nb * js.dim * sizeof(double));
-            sing.allocate(nb * sizeof(int));
             lam.allocate(nb * js.dim * sizeof(double));
             g.allocate(nb * js.dim * sizeof(double));
-            int cross_count = 0;
-            for (int L = 0; L < plan.num_levels; ++L)
-                if (cross_schur(L))
-                    cross_count = std::max(cross_count, plan.group_ptr[L * 3 + 3] - plan.group_ptr[L * 3 + 2]);
-            if (cross_count > 0)
-                uinv.allocate(kern::pivot_side_elems(cross_count, Bp) * tsize());
             capBp = Bp;
         }
         if (batch > capB) {
@@ -260,106 +169,17 @@ struct NewtonSolver::Impl {
         }
     }
 
-    const int* group(int level, int tier, int& count, int& max_fsz) const {
-        const int gi = level * 3 + tier;
-        count = plan.group_ptr[gi + 1] - plan.group_ptr[gi];
-        max_fsz = plan.group_max_fsz[gi];
-        return d_groups.as<int>() + plan.group_ptr[gi];
-    }
-
-    // tier-2 group factored by the pivot + Schur kernels
-    bool cross_schur(int level) const {
-        const int gi = level * 3 + 2;
-        return kern::cross_schur_path(plan.group_max_fsz[gi], plan.group_max_nc[gi]);
-    }
-
     // ---- factor the current J values ------------------------------------
-    // Stage kStFactorDense times the dense tier-2 work: the Schur launch of a
-    // cross + Schur group, the whole group otherwise.
-    template <typename T>
-    void factor_t(kern::Arenas<T> ar) {
-        auto run = [&](int L, int t, hipStream_t st) {
-            int cnt, mf;
-            const int* f = group(L, t, cnt, mf);
-            if (cnt == 0) return;
-            if (t == 2 && cross_schur(L)) {
-                kern::cross_group<T>(lu, f, cnt, mf, J.as<double>(), Bp, ar, uinv.as<T>(), piv_eps,
-                                     static_piv ? 1 : 0, sing.as<int>(), st);
-                StageScope sc(timer, kStFactorDense, st);
-                kern::schur_group<T>(lu, f, cnt, mf, J.as<double>(), Bp, ar, uinv.as<T>(),
-                                     dense_mode, st);
-                return;
-            }
-            StageScope sc(timer, t == 2 ? kStFactorDense : -1, st);
-            kern::lu_factor_group<T>(lu, t, f, cnt, mf, J.as<double>(), Bp, ar, piv_eps,
-                                     static_piv ? 1 : 0, sing.as<int>(), dense_mode, st);
-        };
-        for (int L = 0; L < plan.num_levels; ++L) {
-            const int t0 = first_tier(L);
-            if (t0 > 2) continue;
-            level_split(true, has_after(L, t0), [&](hipStream_t st) { run(L, t0, st); },
-                        [&]() { for (int t = t0 + 1; t < 3; ++t) run(L, t, s); });
-        }
-    }
     void factor() {
-        if (front64) factor_t<double>({arena0.as<double>(), arena1.as<double>()});
-        else factor_t<float>({arena0.as<float>(), arena1.as<float>()});
+        lu.factor(J.as<double>(), Bp, s, &timer, kStFactorDense);
         if (!capturing) ++st.factorizations;   // a captured step counts at each launch
-    }
-
-    template <typename T, typename TX>
-    void solve_once(kern::Arenas<T> ar, const double* rhs, TX* x, bool transpose) {
-        T* y = ywork.as<T>();
-        T* u = upd.as<T>();
-        auto fwd = [&](int L, int t, hipStream_t st) {
-            int cnt, mf;
-            const int* f = group(L, t, cnt, mf);
-            kern::lu_solve_forward_group<T>(lu, t, f, cnt, mf, plan.group_max_nc[L * 3 + t], ar,
-                                            rhs, y, u, Bp, transpose, st);
-        };
-        auto bwd = [&](int L, int t, hipStream_t st) {
-            int cnt, mf;
-            const int* f = group(L, t, cnt, mf);
-            kern::lu_solve_backward_group<T>(lu, t, f, cnt, mf, plan.group_max_nc[L * 3 + t], ar,
-                                             y, Bp, transpose, st);
-        };
-        for (int L = 0; L < plan.num_levels; ++L) {
-            const int t0 = first_tier(L);
-            if (t0 > 2) continue;
-            level_split(true, has_after(L, t0), [&](hipStream_t st) { fwd(L, t0, st); },
-                        [&]() { for (int t = t0 + 1; t < 3; ++t) fwd(L, t, s); });
-        }
-        for (int L = plan.num_levels - 1; L >= 0; --L) {
-            const int t0 = first_tier(L);
-            if (t0 > 2) continue;
-            level_split(true, has_after(L, t0), [&](hipStream_t st) { bwd(L, t0, st); },
-                        [&]() { for (int t = t0 + 1; t < 3; ++t) bwd(L, t, s); });
-        }
-        kern::lu_solve_scatter<T, TX>(lu, y, Bp, x, s);
-    }
-
-    template <typename TX>
-    void solve_dispatch(const double* rhs, TX* x, bool transpose) {
-        if (front64) solve_once<double, TX>({arena0.as<double>(), arena1.as<double>()}, rhs, x, transpose);
-        else solve_once<float, TX>({arena0.as<float>(), arena1.as<float>()}, rhs, x, transpose);
     }
 
     // rhs (FP64, [dim][Bp]) -> x, with FP64 iterative refinement when the
     // fronts are FP32 (residual on the FP64 J, or J^T for transposed solves).
     template <typename TX>
     void solve_into(const double* rhs, TX* x, TX* xtmp, bool transpose, int steps) {
-        solve_dispatch<TX>(rhs, x, transpose);
-        if (front64) return;
-        for (int it = 0; it < steps; ++it) {
-            if (transpose)
-                kern::refine_residual<TX>(d_jtrp.as<int>(), d_jtci.as<int>(), d_jtmap.as<int>(),
-                                          J.as<double>(), js.dim, Bp, rhs, x, res.as<double>(), s);
-            else
-                kern::refine_residual<TX>(d_jrp.as<int>(), d_jci.as<int>(), nullptr,
-                                          J.as<double>(), js.dim, Bp, rhs, x, res.as<double>(), s);
-            solve_dispatch<TX>(res.as<double>(), xtmp, transpose);
-            kern::axpy_update<TX>(xtmp, x, (long long)Bp * js.dim, s);
-        }
+        lu.solve<TX>(J.as<double>(), rhs, x, xtmp, transpose, steps, Bp, s);
     }
 
     void newton_step() {
@@ -413,7 +233,7 @@ struct NewtonSolver::Impl {
     // the test runs over the finite norms only, non-finite scenarios are left
     // to the caller, and at most `budget` iterations run.
     int run_nr(const NRConfig& cfg, bool tolerant = false, int budget = -1) {
-        CUPF_HIP(hipMemsetAsync(sing.get(), 0, size_t(Bp) * sizeof(int), s));
+        lu.clear_singular(Bp, s);
         const int limit = budget >= 0 ? std::min(budget, cfg.max_iter) : cfg.max_iter;
         if (opt.use_cuda_graph && !timer.on && limit > 0) return run_nr_graph(cfg, tolerant, limit);
         int completed = 0;
@@ -655,7 +475,7 @@ struct NewtonSolver::Impl {
         sg.resize(B);
         CUPF_HIP(hipMemcpyAsync(nf.data(), normF.get(), size_t(B) * sizeof(double),
                                 hipMemcpyDeviceToHost, s));
-        CUPF_HIP(hipMemcpyAsync(sg.data(), sing.get(), size_t(B) * sizeof(int),
+        CUPF_HIP(hipMemcpyAsync(sg.data(), lu.singular(), size_t(B) * sizeof(int),
                                 hipMemcpyDeviceToHost, s));
         CUPF_HIP(hipStreamSynchronize(s));
     }
@@ -741,9 +561,6 @@ NewtonSolver::NewtonSolver(const NewtonOptions& options) : impl_(new Impl) {
                    options.custom.shift_retry_epsilon > 0.0;
     m.piv_eps = options.custom.shift_retry_epsilon;
     CUPF_HIP(hipStreamCreate(&m.own));
-    CUPF_HIP(hipStreamCreate(&m.aux));
-    CUPF_HIP(hipEventCreateWithFlags(&m.ev_fork, hipEventDisableTiming));
-    CUPF_HIP(hipEventCreateWithFlags(&m.ev_join, hipEventDisableTiming));
     m.s = m.own;
 }
 
@@ -753,12 +570,6 @@ NewtonSolver::~NewtonSolver() {
         (void)hipStreamSynchronize(impl_->own);
         (void)hipStreamDestroy(impl_->own);
     }
-    if (impl_ && impl_->aux) {
-        (void)hipStreamSynchronize(impl_->aux);
-        (void)hipStreamDestroy(impl_->aux);
-    }
-    if (impl_ && impl_->ev_fork) (void)hipEventDestroy(impl_->ev_fork);
-    if (impl_ && impl_->ev_join) (void)hipEventDestroy(impl_->ev_join);
 }
 
 const NewtonSolverStats& NewtonSolver::stats() const { return impl_->st; }
@@ -823,32 +634,13 @@ void NewtonSolver::initialize(const YbusView& ybus, const int32_t* pv, int32_t n
     }
 
     const std::vector<int> perm = order_jacobian(m.js);
-    m.sym = analyze_symbolic(m.js.dim, m.js.row_ptr.data(), m.js.col_idx.data(), perm,
-                             m.opt.custom.max_panel_width);
-    // tier 1 holds a whole front in 64 KiB of LDS
-    static_assert(kern::kT1MaxK >= kTier1MaxPivots, "tier-1 fronts fit the tier-1 factor kernel");
-    const int tier1_max = m.front64 ? 90 : 128;
-    m.plan = lower_plan(m.sym, m.js.dim, m.js.row_ptr.data(), m.js.col_idx.data(), 16, tier1_max);
-    const int lds_max = m.front64 ? 8192 : 16384;
-    if (m.plan.max_fsz > lds_max)
-        throw std::runtime_error("NewtonSolver::initialize(): a front exceeds the LDS panel budget");
-
-    // J^T pattern with value map (refinement residual of transposed solves)
-    {
-        const int d = m.js.dim;
-        m.jt_rp.assign(d + 1, 0);
-        for (int k = 0; k < m.js.nnz; ++k) ++m.jt_rp[m.js.col_idx[k] + 1];
-        for (int c = 0; c < d; ++c) m.jt_rp[c + 1] += m.jt_rp[c];
-        m.jt_ci.assign(m.js.nnz, 0);
-        m.jt_map.assign(m.js.nnz, 0);
-        std::vector<int> nx(m.jt_rp.begin(), m.jt_rp.end() - 1);
-        for (int r = 0; r < d; ++r)
-            for (int k = m.js.row_ptr[r]; k < m.js.row_ptr[r + 1]; ++k) {
-                const int pos = nx[m.js.col_idx[k]]++;
-                m.jt_ci[pos] = r;
-                m.jt_map[pos] = k;
-            }
-    }
+    LuOptions lo;
+    lo.front64 = m.front64;
+    lo.dense_mode = m.dense_mode;
+    lo.cap = m.opt.custom.max_panel_width;
+    lo.pivot_eps = m.piv_eps;
+    lo.static_pivot = m.static_piv;
+    m.lu.analyze(m.js.dim, m.js.row_ptr.data(), m.js.col_idx.data(), perm, lo, m.own);
 
     hipStream_t s = m.own;
     m.s = s;
@@ -867,58 +659,6 @@ void NewtonSolver::initialize(const YbusView& ybus, const int32_t* pv, int32_t n
     m.d_zero = upload_vector(m.zero_slots, s);
     m.d_row_bus = upload_vector(m.js.row_bus, s);
     m.d_row_isq = upload_vector(m.js.row_is_q, s);
-    m.d_jrp = upload_vector(m.js.row_ptr, s);
-    m.d_jci = upload_vector(m.js.col_idx, s);
-    m.d_jtrp = upload_vector(m.jt_rp, s);
-    m.d_jtci = upload_vector(m.jt_ci, s);
-    m.d_jtmap = upload_vector(m.jt_map, s);
-
-    const LuPlan& P = m.plan;
-    m.d_fsz = upload_vector(P.fsz, s);
-    m.d_nc = upload_vector(P.nc, s);
-    m.d_tier = upload_vector(P.tier, s);
-    m.d_off = upload_vector(P.off, s);
-    m.d_uoff = upload_vector(P.uoff, s);
-    m.d_fptr = upload_vector(P.front_ptr, s);
-    m.d_frows = upload_vector(P.front_rows, s);
-    m.d_cptr = upload_vector(P.child_ptr, s);
-    m.d_cidx = upload_vector(P.child_idx, s);
-    m.d_aptr = upload_vector(P.asm_ptr, s);
-    m.d_aloc = upload_vector(P.asm_loc, s);
-    m.d_eptr = upload_vector(P.ent_ptr, s);
-    m.d_esrc = upload_vector(P.ent_src, s);
-    m.d_edst = upload_vector(P.ent_dst, s);
-    m.d_perm = upload_vector(P.perm, s);
-    m.d_groups = upload_vector(P.group_fronts, s);
-    m.d_lbptr = upload_vector(P.lb_ptr, s);
-    m.d_rowlb = upload_vector(P.row_lb, s);
-    {
-        // assembly lists for the kernels each front's group will run
-        std::vector<int> kinds(size_t(P.num_fronts), 0), rpb(size_t(P.num_fronts), 1);
-        for (int L = 0; L < P.num_levels; ++L)
-            for (int t = 1; t < 3; ++t) {
-                const int gi = L * 3 + t;
-                for (int q = P.group_ptr[gi]; q < P.group_ptr[gi + 1]; ++q) {
-                    const int f = P.group_fronts[q];
-                    if (t == 1) {
-                        kinds[size_t(f)] = 1 << kern::kAsmKFront;
-                    } else if (m.cross_schur(L)) {
-                        kinds[size_t(f)] = (1 << kern::kAsmKSchur) | (1 << kern::kAsmKPivot) |
-                                           (1 << kern::kAsmKChunk);
-                    } else {
-                        kinds[size_t(f)] = 1 << kern::kAsmKRows;
-                        rpb[size_t(f)] = kern::assemble_rows_block(P.group_max_fsz[gi], int(m.tsize()));
-                    }
-                }
-            }
-        const AsmLists al = build_asm_lists(P, kinds, rpb, kern::schur_block_rows(int(m.tsize())));
-        m.d_abase = upload_vector(al.base, s);
-        m.d_gptr = upload_vector(P.gat_ptr, s);
-        m.d_gsrc = upload_vector(P.gat_src, s);
-        m.d_ridx = upload_vector(P.rhs_idx, s);
-        m.d_acptr = upload_vector(al.chunk_ptr, s);
-        m.d_achunks = upload_vector(al.chunks, s);
-    }
     CUPF_HIP(hipStreamSynchronize(s));
 
     kern::NrDevice& nr = m.nr;
@@ -930,68 +670,32 @@ void NewtonSolver::initialize(const YbusView& ybus, const int32_t* pv, int32_t n
     nr.map12 = m.d_m12.as<int>(); nr.map22 = m.d_m22.as<int>();
     nr.row_bus = m.d_row_bus.as<int>(); nr.row_is_q = m.d_row_isq.as<unsigned char>();
 
-    kern::LuDevice& lu = m.lu;
-    lu.n = m.js.dim; lu.nnz = m.js.nnz; lu.P = P.num_fronts;
-    lu.arena0 = P.arena0; lu.arena1 = P.arena1; lu.upd_total = P.upd_total;
-    lu.fsz = m.d_fsz.as<int>(); lu.nc = m.d_nc.as<int>(); lu.tier = m.d_tier.as<int>();
-    lu.off = m.d_off.as<long long>(); lu.uoff = m.d_uoff.as<long long>();
-    lu.front_ptr = m.d_fptr.as<int>(); lu.front_rows = m.d_frows.as<int>();
-    lu.child_ptr = m.d_cptr.as<int>(); lu.child_idx = m.d_cidx.as<int>();
-    lu.asm_ptr = m.d_aptr.as<int>(); lu.asm_loc = m.d_aloc.as<int>();
-    lu.ent_ptr = m.d_eptr.as<int>(); lu.ent_src = m.d_esrc.as<int>();
-    lu.ent_dst = m.d_edst.as<int>(); lu.perm = m.d_perm.as<int>();
-    lu.lb_ptr = m.d_lbptr.as<int>(); lu.row_lb = m.d_rowlb.as<int>();
-    lu.asm_base = m.d_abase.as<int>();
-    lu.asm_cptr = m.d_acptr.as<int>(); lu.asm_chunks = m.d_achunks.as<int4>();
-    lu.gat_ptr = m.d_gptr.as<int>(); lu.gat_src = m.d_gsrc.as<int>(); lu.rhs_idx = m.d_ridx.as<int>();
-
-    m.capBp = 0;   // batch buffers are sized lazily for the new structure
-    m.drop_graph();
-    ++m.buf_gen;
+    // Every buffer sized by the previous structure goes: batch buffers are
+    // sized lazily for the new one, the per-scenario Ybus values and the
+    // warm-start buffers are re-made on use, and a warm-start network (built
+    // for the previous grid) is detached.
+    m.capBp = 0;
     m.capB = 0;
+    m.Yb.release();
+    m.yb_capBp = 0;
+    m.nr.yb = nullptr;
+    m.warm = nullptr;
+    m.warm_budget = 0;
+    for (DeviceBuffer* b : {&m.warm_v0, &m.warm_s, &m.warm_r, &m.warm_o, &m.warm_y}) b->release();
+    m.warm_cap = 0;
+    m.drop_graph();
+    m.graph_warm = false;
+    ++m.buf_gen;
     m.st = NewtonSolverStats{};
+    m.bump();
+    const LuPlan& P = m.lu.plan();
     m.st.dim = m.js.dim; m.st.nnz_j = m.js.nnz; m.st.num_fronts = P.num_fronts;
     m.st.max_front = P.max_fsz; m.st.max_pivots = P.max_nc;
     m.st.arena_elems = P.arena0 + P.arena1;
     m.st.lnz = P.lnz; m.st.factor_flops = P.flops;
-    // Algorithmic work of the timed dense stage, per scenario.
-    //  pivot + Schur groups (the Schur kernel): flops 2 uc^2 K (update) +
-    //    2 uc K^2 (L21); bytes = the rows >= K written once (CB uc^2, L21
-    //    uc K) + the child contribution elements landing in them read once +
-    //    U12 (K uc) read once, at the front element size, + the J entries of
-    //    those rows at 8 B;
-    //  other tier-2 groups (assembly + dense LU): flops of the front LU;
-    //    bytes = each front element written and read once + its J entries.
-    for (int p = 0; p < P.num_fronts; ++p) {
-        if (P.tier[p] != 2) continue;
-        const int fsz = P.fsz[p], K = P.nc[p];
-        const double f = fsz;
-        if (m.cross_schur(P.level[p])) {
-            const double uc = f - K;
-            double child = 0.0, ents = 0.0;
-            const int* rl = P.row_lb.data() + P.lb_ptr[p];
-            for (int q = P.child_ptr[p]; q < P.child_ptr[p + 1]; ++q) {
-                const int c = P.child_idx[q];
-                const int* cl = rl + (1 + q - P.child_ptr[p]) * (fsz + 1);
-                const double cuc = P.fsz[c] - P.nc[c];
-                child += (cuc - cl[K]) * cuc;          // contribution rows landing at >= K
-            }
-            ents = rl[fsz] - rl[K];
-            m.st.dense_flops += 2.0 * uc * uc * K + 2.0 * uc * K * K;
-            m.st.dense_bytes += double(m.tsize()) * (uc * uc + child + 2.0 * uc * K) + 8.0 * ents;
-            continue;
-        }
-        double fl = 0.0;
-        for (int k = 0; k < K; ++k) {
-            const double r = f - k - 1;
-            fl += 2.0 * r * r + r;
-        }
-        m.st.dense_flops += fl;
-        m.st.dense_bytes += 2.0 * double(m.tsize()) * f * f +
-                            8.0 * double(P.ent_ptr[p + 1] - P.ent_ptr[p]);
-    }
-    for (int g = 0; g < P.num_levels; ++g)
-        if (P.group_ptr[g * 3 + 3] > P.group_ptr[g * 3 + 2]) ++m.st.dense_groups;
+    m.st.dense_flops = m.lu.dense_flops();
+    m.st.dense_bytes = m.lu.dense_bytes();
+    m.st.dense_groups = m.lu.dense_groups();
     m.initialized = true;
 }
 
@@ -1027,6 +731,7 @@ void NewtonSolver::solve_batch(const YbusView& ybus, const std::complex<double>*
     const auto t0 = std::chrono::steady_clock::now();
     m.s = m.own;
     m.cache = false;
+    m.bump();
     m.ensure_batch(batch_size);
     const int B = m.B;
     m.upload_ybus_values(ybus);
@@ -1090,8 +795,9 @@ void NewtonSolver::solve_batch_device_ybus(const void* sbus, int64_t sbus_stride
             "NewtonSolver::solve_batch_device(): ybus stride must be at least nnz(Ybus)");
     if (config.max_iter < 0) throw std::invalid_argument("NRConfig.max_iter must be >= 0");
     const auto t0 = std::chrono::steady_clock::now();
-    m.s = stream ? static_cast<hipStream_t>(stream) : m.own;
+    Impl::CallStream call(m, stream);
     m.cache = false;
+    m.bump();
     m.ensure_batch(batch_size);
     const int B = m.B;
     const bool cache_req = solve_options.prepare_adjoint_cache &&
@@ -1112,7 +818,6 @@ void NewtonSolver::solve_batch_device_ybus(const void* sbus, int64_t sbus_stride
     m.st.last_batch = B;
     m.st.last_solve_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    m.s = m.own;
 }
 
 void NewtonSolver::solve_adjoint(const double* grad_va, int64_t grad_va_stride,
@@ -1199,11 +904,10 @@ void NewtonSolver::solve_adjoint(const double* grad_va, int64_t grad_va_stride,
                                 hipMemcpyDeviceToHost, m.s));
     }
     if (options.check_residual) {
-        kern::refine_residual<double>(m.d_jtrp.as<int>(), m.d_jtci.as<int>(), m.d_jtmap.as<int>(),
-                                      m.J.as<double>(), m.js.dim, m.Bp, m.g.as<double>(),
-                                      m.lam.as<double>(), m.res.as<double>(), m.s);
+        m.lu.residual<double>(m.J.as<double>(), m.g.as<double>(), m.lam.as<double>(), m.lu.res(),
+                              true, m.Bp, m.s);
         std::vector<double> r(size_t(m.js.dim) * m.Bp), gh(size_t(m.js.dim) * m.Bp);
-        CUPF_HIP(hipMemcpyAsync(r.data(), m.res.get(), r.size() * sizeof(double),
+        CUPF_HIP(hipMemcpyAsync(r.data(), m.lu.res(), r.size() * sizeof(double),
                                 hipMemcpyDeviceToHost, m.s));
         CUPF_HIP(hipMemcpyAsync(gh.data(), m.g.get(), gh.size() * sizeof(double),
                                 hipMemcpyDeviceToHost, m.s));
@@ -1251,8 +955,9 @@ void NewtonSolver::solve_torch_forward(const void* sbus_base_re, const void* sbu
     if (!sbus_base_re || !sbus_base_im || !load_p || !load_q || !v0_va || !v0_vm || !va_out || !vm_out)
         throw std::invalid_argument("NewtonSolver::solve_torch_forward(): null tensor pointer");
     if (config.max_iter < 0) throw std::invalid_argument("NRConfig.max_iter must be >= 0");
-    m.s = stream ? static_cast<hipStream_t>(stream) : m.own;
+    Impl::CallStream call(m, stream);
     m.cache = false;
+    m.bump();
     m.ensure_batch(batch_size);
     m.nr.yb = nullptr;   // Ybus values shared by the batch
     const int B = m.B;
@@ -1293,7 +998,6 @@ void NewtonSolver::solve_torch_forward(const void* sbus_base_re, const void* sbu
     result.backend = m.front64 ? "hip_multifrontal_fp64" : "hip_multifrontal_fp32_refined";
     m.st.last_iterations = iters;
     m.st.last_batch = B;
-    m.s = m.own;
 }
 
 void NewtonSolver::solve_torch_backward(const void* grad_va, const void* grad_vm,
@@ -1312,7 +1016,7 @@ void NewtonSolver::solve_torch_backward(const void* grad_va, const void* grad_vm
         throw std::invalid_argument("NewtonSolver::solve_torch_backward(): dtype does not match the forward");
     if (!grad_va || !grad_vm || !grad_load_p || !grad_load_q)
         throw std::invalid_argument("NewtonSolver::solve_torch_backward(): null tensor pointer");
-    m.s = stream ? static_cast<hipStream_t>(stream) : m.own;
+    Impl::CallStream call(m, stream);
     const int B = m.B;
     const bool cache_ok = m.cache && m.cache_B == B;
     if (!cache_ok) {
@@ -1350,7 +1054,6 @@ void NewtonSolver::solve_torch_backward(const void* grad_va, const void* grad_vm
     result.current_stream_integrated = stream != nullptr;
     result.backend = m.front64 ? "hip_multifrontal_fp64" : "hip_multifrontal_fp32_refined";
     result.transpose_solve_backend = "hip_multifrontal_transpose_solve";
-    m.s = m.own;
 }
 
 // ---------------------------------------------------------------------------
@@ -1361,6 +1064,8 @@ void NewtonSolver::debug_residual(const std::complex<double>* sbus, const std::c
     Impl& m = *impl_;
     if (!m.initialized) throw std::runtime_error("debug_residual: not initialized");
     m.s = m.own;
+    m.cache = false;
+    m.bump();
     m.ensure_batch(batch);
     m.nr.yb = nullptr;   // Ybus values shared by the batch
     const size_t row = size_t(m.n) * sizeof(double2);
@@ -1382,6 +1087,8 @@ void NewtonSolver::debug_jacobian(const std::complex<double>* V, int32_t batch, 
     Impl& m = *impl_;
     if (!m.initialized) throw std::runtime_error("debug_jacobian: not initialized");
     m.s = m.own;
+    m.cache = false;
+    m.bump();
     m.ensure_batch(batch);
     m.nr.yb = nullptr;   // Ybus values shared by the batch
     const size_t row = size_t(m.n) * sizeof(double2);
@@ -1400,6 +1107,8 @@ void NewtonSolver::debug_factor_solve(const double* A, const double* rhs, int32_
     Impl& m = *impl_;
     if (!m.initialized) throw std::runtime_error("debug_factor_solve: not initialized");
     m.s = m.own;
+    m.cache = false;
+    m.bump();
     m.ensure_batch(batch);
     const auto ai = host_interleave(A, m.js.nnz, batch, m.Bp, m.js.nnz);
     const auto ri = host_interleave(rhs, m.js.dim, batch, m.Bp, m.js.dim);
@@ -1407,7 +1116,7 @@ void NewtonSolver::debug_factor_solve(const double* A, const double* rhs, int32_
                             hipMemcpyHostToDevice, m.s));
     CUPF_HIP(hipMemcpyAsync(m.g.get(), ri.data(), ri.size() * sizeof(double),
                             hipMemcpyHostToDevice, m.s));
-    CUPF_HIP(hipMemsetAsync(m.sing.get(), 0, size_t(m.Bp) * sizeof(int), m.s));
+    m.lu.clear_singular(m.Bp, m.s);
     m.factor();
     std::vector<double> xi(size_t(m.js.dim) * m.Bp);
     if (m.dx32) {
@@ -1424,7 +1133,7 @@ void NewtonSolver::debug_factor_solve(const double* A, const double* rhs, int32_
                                 hipMemcpyDeviceToHost, m.s));
     }
     std::vector<int> sg(batch);
-    CUPF_HIP(hipMemcpyAsync(sg.data(), m.sing.get(), size_t(batch) * sizeof(int),
+    CUPF_HIP(hipMemcpyAsync(sg.data(), m.lu.singular(), size_t(batch) * sizeof(int),
                             hipMemcpyDeviceToHost, m.s));
     CUPF_HIP(hipStreamSynchronize(m.s));
     host_deinterleave(xi, batch, m.Bp, m.js.dim, x);
@@ -1436,5 +1145,5 @@ void NewtonSolver::debug_pattern(int32_t* row_ptr, int32_t* col_idx, int32_t* pe
     if (!m.initialized) throw std::runtime_error("debug_pattern: not initialized");
     std::copy(m.js.row_ptr.begin(), m.js.row_ptr.end(), row_ptr);
     std::copy(m.js.col_idx.begin(), m.js.col_idx.end(), col_idx);
-    std::copy(m.plan.perm.begin(), m.plan.perm.end(), perm);
+    std::copy(m.lu.plan().perm.begin(), m.lu.plan().perm.end(), perm);
 }

@@ -25,6 +25,11 @@ struct NewtonSolverStats {
     // learned warm start: iterations of the warm pass and scenarios that fell
     // back to their own V0 in the last solve
     int32_t last_warm_iterations = 0, last_warm_fallbacks = 0;
+    // bumped by every call that replaces the solver's batch state (initialize,
+    // each forward solve, the diagnostic stage runs); never reset, so a
+    // caller that recorded it after a forward can tell whether the cached
+    // final-state factors are still that forward's
+    int64_t state_version = 0;
 };
 
 namespace cupf { class WarmStartMLP; }
@@ -87,16 +92,17 @@ public:
                                  int32_t* iterations, double* final_mismatch, uint8_t* converged,
                                  void* stream);
 
+    // Learned warm start (no reference counterpart).  While set, solve_batch
+    // and solve_batch_device start NR from the network's prediction (at most
+    // max_warm_iter iterations) and re-solve every scenario that did not
+    // converge from its own V0.  nullptr switches it off.  Not owned.  A
+    // later initialize() detaches it (the network belongs to the old grid).
+    void set_warm_start(cupf::WarmStartMLP* mlp, int32_t max_warm_iter);
+
     // HIP-event timing of the stages on the stream they run on.
     // Stage order: residual, jacobian, factor, solve (with refinement),
     // voltage update, dense-front factor launches (nested in factor), warm
     // start.  collect_stage_times() synchronises and resets.
-    // Learned warm start (no reference counterpart).  While set, solve_batch
-    // and solve_batch_device start NR from the network's prediction (at most
-    // max_warm_iter iterations) and re-solve every scenario that did not
-    // converge from its own V0.  nullptr switches it off.  Not owned.
-    void set_warm_start(cupf::WarmStartMLP* mlp, int32_t max_warm_iter);
-
     void set_stage_timing(bool on);
     void collect_stage_times(double* total_ms, int64_t* launches);
 

@@ -31,17 +31,25 @@ constexpr unsigned long long kNanBits = 0x7FF8000000000000ULL;
 // ---- layout transforms (tiled through LDS, coalesced on both sides) ----
 constexpr int kTr = 32;   // rows per tile; a tile covers 64 scenarios
 
-__global__ __launch_bounds__(kBlock) void k_to_inter(const double2* __restrict__ src,
-                                                     long long stride, int rows, int B, int Bp,
-                                                     double2* __restrict__ dst) {
-    __shared__ double2 tile[kTr][kLanes + 1];
+template <typename E>
+__device__ inline E zero_of();
+template <>
+__device__ inline double2 zero_of<double2>() { return make_double2(0.0, 0.0); }
+template <>
+__device__ inline double zero_of<double>() { return 0.0; }
+
+template <typename E>
+__global__ __launch_bounds__(kBlock) void k_to_inter(const E* __restrict__ src, long long stride,
+                                                     int rows, int B, int Bp,
+                                                     E* __restrict__ dst) {
+    __shared__ E tile[kTr][kLanes + 1];
     const int i0 = blockIdx.x * kTr;
     const int b0 = blockIdx.y * kLanes;
     for (int t = threadIdx.x; t < kTr * kLanes; t += kBlock) {
         const int bb = t / kTr, ii = t % kTr;
         const int b = min(b0 + bb, B - 1);
         const int i = i0 + ii;
-        tile[ii][bb] = (i < rows) ? src[(long long)b * stride + i] : make_double2(0.0, 0.0);
+        tile[ii][bb] = (i < rows) ? src[(long long)b * stride + i] : zero_of<E>();
     }
     __syncthreads();
     for (int t = threadIdx.x; t < kTr * kLanes; t += kBlock) {
@@ -51,10 +59,11 @@ __global__ __launch_bounds__(kBlock) void k_to_inter(const double2* __restrict__
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_from_inter(const double2* __restrict__ src, int rows,
-                                                       int B, int Bp, double2* __restrict__ dst,
+template <typename E>
+__global__ __launch_bounds__(kBlock) void k_from_inter(const E* __restrict__ src, int rows, int B,
+                                                       int Bp, E* __restrict__ dst,
                                                        long long stride) {
-    __shared__ double2 tile[kTr][kLanes + 1];
+    __shared__ E tile[kTr][kLanes + 1];
     const int i0 = blockIdx.x * kTr;
     const int b0 = blockIdx.y * kLanes;
     for (int t = threadIdx.x; t < kTr * kLanes; t += kBlock) {
@@ -377,18 +386,43 @@ inline dim3 rows_grid(int rows, int Bp) { return dim3((rows + kWaves - 1) / kWav
 
 }  // namespace
 
+template <typename E>
+void to_interleaved_t(const E* src, long long src_stride, int rows, int B, int Bp, E* dst,
+                      hipStream_t s) {
+    if (rows <= 0) return;
+    dim3 grid((rows + kTr - 1) / kTr, Bp / kLanes);
+    hipLaunchKernelGGL(k_to_inter<E>, grid, dim3(kBlock), 0, s, src, src_stride, rows, B, Bp, dst);
+    CUPF_HIP(hipGetLastError());
+}
+
+template <typename E>
+void from_interleaved_t(const E* src, int rows, int B, int Bp, E* dst, long long dst_stride,
+                        hipStream_t s) {
+    if (rows <= 0) return;
+    dim3 grid((rows + kTr - 1) / kTr, Bp / kLanes);
+    hipLaunchKernelGGL(k_from_inter<E>, grid, dim3(kBlock), 0, s, src, rows, B, Bp, dst,
+                       dst_stride);
+    CUPF_HIP(hipGetLastError());
+}
+
 void to_interleaved(const double2* src, long long src_stride, int rows, int B, int Bp,
                     double2* dst, hipStream_t s) {
-    dim3 grid((rows + kTr - 1) / kTr, Bp / kLanes);
-    hipLaunchKernelGGL(k_to_inter, grid, dim3(kBlock), 0, s, src, src_stride, rows, B, Bp, dst);
-    CUPF_HIP(hipGetLastError());
+    to_interleaved_t(src, src_stride, rows, B, Bp, dst, s);
 }
 
 void from_interleaved(const double2* src, int rows, int B, int Bp, double2* dst,
                       long long dst_stride, hipStream_t s) {
-    dim3 grid((rows + kTr - 1) / kTr, Bp / kLanes);
-    hipLaunchKernelGGL(k_from_inter, grid, dim3(kBlock), 0, s, src, rows, B, Bp, dst, dst_stride);
-    CUPF_HIP(hipGetLastError());
+    from_interleaved_t(src, rows, B, Bp, dst, dst_stride, s);
+}
+
+void to_interleaved(const double* src, long long src_stride, int rows, int B, int Bp, double* dst,
+                    hipStream_t s) {
+    to_interleaved_t(src, src_stride, rows, B, Bp, dst, s);
+}
+
+void from_interleaved(const double* src, int rows, int B, int Bp, double* dst,
+                      long long dst_stride, hipStream_t s) {
+    from_interleaved_t(src, rows, B, Bp, dst, dst_stride, s);
 }
 
 void seed_polar(int n, int Bp, const double2* V, double* Va, double* Vm, hipStream_t s) {

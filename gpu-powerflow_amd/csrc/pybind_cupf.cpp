@@ -495,12 +495,16 @@ PYBIND11_MODULE(_cupf, m) {
             [](PySolver& self, I32 indptr, I32 indices, C128 data, int32_t rows, int32_t cols,
                I32 pv, I32 pq) {
                 check_ybus(indptr, indices, data, rows, cols);
-                py::gil_scoped_release nogil;
-                check(cupf_solver_initialize(self.h(), rows, cols, indptr.data(), indices.data(),
-                                             cplx(data), int32_t(data.size()), pv.data(),
-                                             int32_t(pv.size()), pq.data(), int32_t(pq.size())));
+                {
+                    py::gil_scoped_release nogil;
+                    check(cupf_solver_initialize(self.h(), rows, cols, indptr.data(),
+                                                 indices.data(), cplx(data), int32_t(data.size()),
+                                                 pv.data(), int32_t(pv.size()), pq.data(),
+                                                 int32_t(pq.size())));
+                }
                 self.n_bus = rows;
                 self.dim = int32_t(pv.size() + 2 * pq.size());
+                self.warm = py::none();   // initialize() detaches the warm start
             },
             py::arg("indptr"), py::arg("indices"), py::arg("data"), py::arg("rows"),
             py::arg("cols"), py::arg("pv"), py::arg("pq"))
@@ -729,8 +733,15 @@ PYBIND11_MODULE(_cupf, m) {
                  d["dense_groups"] = s.dense_groups;
                  d["last_warm_iterations"] = s.last_warm_iterations;
                  d["last_warm_fallbacks"] = s.last_warm_fallbacks;
+                 d["state_version"] = s.state_version;
                  return d;
              })
+        .def_property_readonly("state_version",
+                               [](PySolver& self) {
+                                   cupf_solver_stats s;
+                                   check(cupf_solver_get_stats(self.h(), &s));
+                                   return s.state_version;
+                               })
         .def_property_readonly("n_bus", [](const PySolver& s) { return s.n_bus; })
         .def_property_readonly("dimF", [](const PySolver& s) { return s.dim; })
         .def_property_readonly("last_batch", [](const PySolver& s) { return s.last_batch; })

@@ -11,13 +11,11 @@ float32 for FP32/Mixed.
 
 from __future__ import annotations
 
-from typing import Any
-
 import torch
 
-from ._cupf import AdjointOptions, NRConfig, SolveOptions
+from ._cupf import AdjointCacheMode, AdjointOptions, NRConfig, SolveOptions
 
-_SOLVER_GENERATIONS: dict[int, int] = {}
+_NO_CACHE = getattr(AdjointCacheMode, "None")   # "None" is a Python keyword
 
 
 def _check(t: torch.Tensor, name: str, dtype: torch.dtype, shape: tuple, device) -> None:
@@ -91,67 +89,79 @@ def install_torch_methods(cls) -> None:
     cls.solve_adjoint_torch = _solve_adjoint_torch
 
 
-def _prepare_backward_solve_options(solve_options: Any | None) -> Any:
-    if solve_options is None:
-        solve_options = SolveOptions()
-    solve_options.prepare_adjoint_cache = True
-    solve_options.allow_explicit_transpose_fallback = True
-    return solve_options
+def _cached_forward_options(solve_options):
+    """A fresh SolveOptions for a differentiable forward (the caller's object
+    is left as it is): the final-state factorization is prepared and kept for
+    the transpose solve of the backward pass."""
+    so = SolveOptions()
+    if solve_options is not None:
+        so.adjoint_cache_mode = solve_options.adjoint_cache_mode
+    if so.adjoint_cache_mode == _NO_CACHE:
+        so.adjoint_cache_mode = AdjointCacheMode.FinalStateFactorization
+    so.prepare_adjoint_cache = True
+    so.allow_explicit_transpose_fallback = True
+    return so
 
 
-def _next_solver_generation(solver: Any) -> tuple[int, int]:
-    solver_id = id(solver)
-    generation = _SOLVER_GENERATIONS.get(solver_id, 0) + 1
-    _SOLVER_GENERATIONS[solver_id] = generation
-    return solver_id, generation
+def _backward_options():
+    """AdjointOptions of the backward pass: only the factors cached by this
+    graph's forward may be used; nothing is refactorized and the residual
+    check (a host readback) is skipped."""
+    o = AdjointOptions()
+    o.require_cached_factorization = True
+    o.allow_refactorize = False
+    o.allow_refactorize_for_backward = False
+    o.check_residual = False
+    return o
 
 
 class CuPFFunction(torch.autograd.Function):
-    """Forward: batched NR with the final-state factorization cached.
-    Backward: transpose solve J^T lambda = dL/dx on the cached factors;
-    dL/dload = -lambda on the P/Q rows (zero elsewhere)."""
+    """Differentiable batched power flow: (load_p, load_q) [B, n] -> (Va, Vm).
+
+    The forward solves to the configured tolerance and leaves the factors of
+    the Jacobian at the final voltages cached in the solver.  The backward
+    solves J^T lambda = [dL/dVa@pv, dL/dVa@pq, dL/dVm@pq] on those factors
+    and returns dL/dload = -lambda on the P and Q rows (zero elsewhere); see
+    cuPF/python/cupf/torch_autograd.py:28-103 for the interface it keeps.
+
+    The solver records a state version that every later solve bumps; the
+    backward refuses to run when the version differs from the one its forward
+    left, because the cached factors then belong to another solve.
+    """
 
     @staticmethod
-    def forward(ctx: Any, load_p: torch.Tensor, load_q: torch.Tensor, solver: Any,
-                sbus_base_re: torch.Tensor, sbus_base_im: torch.Tensor, v0_va: torch.Tensor,
-                v0_vm: torch.Tensor, config: Any | None = None,
-                solve_options: Any | None = None) -> tuple[torch.Tensor, torch.Tensor]:
-        if config is None:
-            config = NRConfig()
-        solve_options = _prepare_backward_solve_options(solve_options)
+    def forward(ctx, load_p, load_q, solver, sbus_base_re, sbus_base_im, v0_va, v0_vm,
+                config=None, solve_options=None):
+        load_p = load_p.contiguous()
+        load_q = load_q.contiguous()
         va = torch.empty_like(load_p)
         vm = torch.empty_like(load_q)
-        solver_id, solver_generation = _next_solver_generation(solver)
-        solver.solve_with_adjoint_cache_torch(sbus_base_re, sbus_base_im, load_p.contiguous(),
-                                              load_q.contiguous(), v0_va, v0_vm, va, vm, config,
-                                              solve_options)
+        solver.solve_with_adjoint_cache_torch(
+            sbus_base_re, sbus_base_im, load_p, load_q, v0_va, v0_vm, va, vm,
+            NRConfig() if config is None else config, _cached_forward_options(solve_options))
         ctx.solver = solver
-        ctx.solver_id = solver_id
-        ctx.solver_generation = solver_generation
+        ctx.version = solver.state_version
         return va, vm
 
     @staticmethod
-    def backward(ctx: Any, grad_va: torch.Tensor, grad_vm: torch.Tensor) -> tuple[Any, ...]:
-        if _SOLVER_GENERATIONS.get(ctx.solver_id) != ctx.solver_generation:
+    def backward(ctx, grad_va, grad_vm):
+        solver = ctx.solver
+        if solver.state_version != ctx.version:
             raise RuntimeError(
-                "cuPF autograd detected that the same NewtonSolver was reused "
-                "before this graph's backward pass. Use one solver per active "
-                "autograd graph or run backward before issuing another forward.")
-        grad_load_p = torch.empty_like(grad_va)
-        grad_load_q = torch.empty_like(grad_vm)
-        opts = AdjointOptions()
-        opts.require_cached_factorization = True
-        opts.allow_refactorize = False
-        opts.allow_refactorize_for_backward = False
-        opts.allow_explicit_transpose_fallback = True
-        opts.check_residual = False
-        ctx.solver.solve_adjoint_torch(grad_va.contiguous(), grad_vm.contiguous(), grad_load_p,
-                                       grad_load_q, opts)
-        return grad_load_p, grad_load_q, None, None, None, None, None, None, None
+                "cupf: this NewtonSolver has run another solve since the forward pass being "
+                "differentiated, so its cached factorization no longer matches; run backward "
+                "first or give each live autograd graph its own solver")
+        grad_p = torch.empty_like(grad_va)
+        grad_q = torch.empty_like(grad_vm)
+        solver.solve_adjoint_torch(grad_va.contiguous(), grad_vm.contiguous(), grad_p, grad_q,
+                                   _backward_options())
+        # no gradients for the solver, the base injections, V0, config, options
+        return (grad_p, grad_q) + (None,) * 7
 
 
-def solve(load_p: torch.Tensor, load_q: torch.Tensor, solver: Any,
-          **kwargs: Any) -> tuple[torch.Tensor, torch.Tensor]:
-    return CuPFFunction.apply(load_p, load_q, solver, kwargs["sbus_base_re"],
-                              kwargs["sbus_base_im"], kwargs["v0_va"], kwargs["v0_vm"],
-                              kwargs.get("config"), kwargs.get("solve_options"))
+def solve(load_p, load_q, solver, *, sbus_base_re, sbus_base_im, v0_va, v0_vm, config=None,
+          solve_options=None):
+    """(Va, Vm) of the batched power flow with Sbus = base - load, differentiable
+    with respect to load_p and load_q (cuPF/python/cupf/torch_autograd.py:92-103)."""
+    return CuPFFunction.apply(load_p, load_q, solver, sbus_base_re, sbus_base_im, v0_va,
+                              v0_vm, config, solve_options)

@@ -115,6 +115,9 @@ typedef struct {
     int32_t dense_groups;              /* tier-2 launches per factorisation */
     int32_t last_warm_iterations;      /* warm-start pass of the last solve */
     int32_t last_warm_fallbacks;       /* scenarios re-solved from their V0 */
+    int64_t state_version;             /* bumped by every call that replaces the
+                                          batch state (initialize, forward solves);
+                                          never reset */
 } cupf_solver_stats;
 
 const char* cupf_last_error(void);
@@ -124,18 +127,18 @@ void cupf_default_solve_options(cupf_solve_options* out);
 void cupf_default_newton_options(cupf_newton_options* out);
 void cupf_default_adjoint_options(cupf_adjoint_options* out);
 
-/* NewtonSolver(const NewtonOptions&)  (newton_solver.hpp:318) */
+/* NewtonSolver(const NewtonOptions&)  (newton_solver.hpp:28) */
 int cupf_solver_create(const cupf_newton_options* options, cupf_solver** out);
-/* ~NewtonSolver()  (newton_solver.hpp:319) */
+/* ~NewtonSolver()  (newton_solver.hpp:29) */
 int cupf_solver_destroy(cupf_solver* solver);
 
-/* initialize(ybus, pv, n_pv, pq, n_pq)  (newton_solver.hpp:321-323) */
+/* initialize(ybus, pv, n_pv, pq, n_pq)  (newton_solver.hpp:31-33) */
 int cupf_solver_initialize(cupf_solver* solver, int32_t rows, int32_t cols,
                            const int32_t* indptr, const int32_t* indices, const double* data,
                            int32_t nnz, const int32_t* pv, int32_t n_pv, const int32_t* pq,
                            int32_t n_pq);
 
-/* solve_batch(...)  (newton_solver.hpp:334-344).  sbus/V0: [batch][stride]
+/* solve_batch(...)  (newton_solver.hpp:44-54).  sbus/V0: [batch][stride]
  * complex; V_out: [batch][rows] complex; iterations/final_mismatch/
  * converged/singular: [batch] (singular may be NULL). */
 int cupf_solver_solve_batch(cupf_solver* solver, int32_t rows, int32_t cols,
@@ -147,7 +150,17 @@ int cupf_solver_solve_batch(cupf_solver* solver, int32_t rows, int32_t cols,
                             double* V_out, int32_t* iterations, double* final_mismatch,
                             uint8_t* converged, uint8_t* singular);
 
-/* solve_adjoint(...)  (newton_solver.hpp:346-354).  lambda: [batch][dimF];
+/* solve(...)  (newton_solver.hpp:35-42): one scenario.  sbus, V0, V_out:
+ * [rows] complex; iterations, final_mismatch, converged: one value each (may
+ * be NULL). */
+int cupf_solver_solve(cupf_solver* solver, int32_t rows, int32_t cols, const int32_t* indptr,
+                      const int32_t* indices, const double* data, int32_t nnz, const double* sbus,
+                      const double* V0, const int32_t* pv, int32_t n_pv, const int32_t* pq,
+                      int32_t n_pq, const cupf_nr_config* config,
+                      const cupf_solve_options* options, double* V_out, int32_t* iterations,
+                      double* final_mismatch, uint8_t* converged);
+
+/* solve_adjoint(...)  (newton_solver.hpp:56-65).  lambda: [batch][dimF];
  * grad_load_p/q: [batch][n_bus] (may be NULL when not computed). */
 int cupf_solver_solve_adjoint(cupf_solver* solver, const double* grad_va, int64_t grad_va_stride,
                               const double* grad_vm, int64_t grad_vm_stride, int32_t batch,
@@ -155,7 +168,7 @@ int cupf_solver_solve_adjoint(cupf_solver* solver, const double* grad_va, int64_
                               const cupf_adjoint_options* options, double* lambda,
                               double* grad_load_p, double* grad_load_q, cupf_adjoint_info* info);
 
-/* solve_torch_forward(...)  (newton_solver.hpp:367-380; torch_bridge.cpp:309-482).
+/* solve_torch_forward(...)  (newton_solver.hpp:77-92; torch_bridge.cpp:309-482).
  * Device pointers; dtype "float64" (FP64) or "float32" (FP32/Mixed); stream is
  * a hipStream_t (NULL = the solver's own stream). */
 int cupf_solver_solve_torch_forward(cupf_solver* solver, const void* sbus_base_re,
@@ -166,7 +179,7 @@ int cupf_solver_solve_torch_forward(cupf_solver* solver, const void* sbus_base_r
                                     const cupf_solve_options* options, void* stream,
                                     cupf_adjoint_info* info);
 
-/* solve_torch_backward(...)  (newton_solver.hpp:357-365; torch_bridge.cpp:209-303) */
+/* solve_torch_backward(...)  (newton_solver.hpp:67-75; torch_bridge.cpp:209-303) */
 int cupf_solver_solve_torch_backward(cupf_solver* solver, const void* grad_va, const void* grad_vm,
                                      void* grad_load_p, void* grad_load_q, int32_t batch,
                                      int32_t n_bus, const char* dtype,
@@ -263,6 +276,67 @@ int cupf_plan_figures(int32_t n, const int32_t* row_ptr, const int32_t* col_idx,
  *        position, most pivots of a tier-1 front}. */
 int cupf_plan_maps_check(int32_t n, const int32_t* row_ptr, const int32_t* col_idx,
                          const int32_t* perm, int32_t cap, int64_t* out8);
+
+/* ---- Inner sparse solver: phase API ------------------------------------
+ * The batched multifrontal LU of one square CSR matrix for `batch` value sets
+ * at once, on device pointers and a hipStream_t.  Replaces the phase API of
+ * custom_linear_solver::Solver (custom_linear_solver/src/solver.hpp:80-114):
+ * SetData + Analyze -> cupf_lu_analyze, Setup(B) -> cupf_lu_setup,
+ * SetValues + Factorize -> cupf_lu_factor, SetRhs/SetSolution + Solve ->
+ * cupf_lu_solve; cupf_lu_solve_transpose solves A^T x = b on the same factors
+ * (no reference counterpart: cuPF's custom backend has no transpose solve).
+ * Batch-strided device buffers as in the reference: values[b * nnz + k],
+ * rhs[b * n + i], x[b * n + i], all FP64.  FP32 / TF32 precision keeps FP32
+ * fronts (TF32 = 3xBF16 split products on MFMA) and follows each solve with
+ * refine_steps FP64 iterative-refinement steps.  stream NULL = the solver's
+ * own stream, synchronised before returning; otherwise the call only
+ * enqueues.  Return values are the reference's Status codes. */
+typedef struct cupf_lu cupf_lu;
+
+/* custom_linear_solver::Status (solver.hpp:21-29) */
+enum {
+    CUPF_LU_SUCCESS = 0,
+    CUPF_LU_INVALID_VALUE = 1,
+    CUPF_LU_INVALID_STATE = 2,
+    CUPF_LU_ALLOCATION_FAILED = 3,
+    CUPF_LU_ANALYSIS_FAILED = 4,
+    CUPF_LU_FACTORIZATION_FAILED = 5,
+    CUPF_LU_SOLVE_FAILED = 6
+};
+/* DataLocation (internal/matrix_view.hpp:15-18) */
+enum { CUPF_LU_HOST = 0, CUPF_LU_DEVICE = 1 };
+
+/* SolverConfig (solver.hpp:33-67), its numeric fields; refine_steps is an
+ * addition (FP64 refinement steps after an FP32-front solve). */
+typedef struct {
+    int32_t precision;             /* CUPF_PRECISION_FP64 (default) / FP32 / TF32 */
+    int32_t max_panel_width;       /* relaxed-panel cap, default 8 */
+    int32_t pivot_strategy;        /* 0 None, 1 StaticDiagonalShift (default) */
+    double shift_retry_epsilon;    /* pivot threshold and replacement, default 1e-8 */
+    int32_t refine_steps;          /* default 1 */
+} cupf_lu_config;
+
+void cupf_lu_default_config(cupf_lu_config* out);
+const char* cupf_lu_status_string(int status);           /* StatusString (solver.hpp:116) */
+/* Solver(config) / ~Solver() */
+int cupf_lu_create(const cupf_lu_config* config, cupf_lu** out);
+int cupf_lu_destroy(cupf_lu* lu);
+/* SetData + Analyze(): CSR pattern (int32, sorted unique columns per row) on
+ * the host or the device; ordering, symbolic analysis and plan, once. */
+int cupf_lu_analyze(cupf_lu* lu, int32_t n, int32_t nnz, const int32_t* row_ptr,
+                    const int32_t* col_idx, int32_t location);
+/* Setup(batch): per-batch arenas (grown, never shrunk). */
+int cupf_lu_setup(cupf_lu* lu, int32_t batch);
+/* SetValues + Factorize(): values device FP64 [batch][nnz]. */
+int cupf_lu_factor(cupf_lu* lu, const double* values, void* stream);
+/* SetRhs + SetSolution + Solve(): x = A^-1 rhs, device FP64 [batch][n]. */
+int cupf_lu_solve(cupf_lu* lu, const double* rhs, double* x, void* stream);
+/* x = A^-T rhs on the factors of the last cupf_lu_factor. */
+int cupf_lu_solve_transpose(cupf_lu* lu, const double* rhs, double* x, void* stream);
+/* Per system: 1 where a static pivot shift was applied in the last factor. */
+int cupf_lu_singular(cupf_lu* lu, uint8_t* flags);
+/* {n, nnz, fronts, largest front, nnz(L), front arena elements per system} */
+int cupf_lu_info(cupf_lu* lu, int64_t* out6);
 
 /* ---- Learned warm start (no reference counterpart) ---------------------
  * A dense MLP (BF16 weights, FP32 accumulate, GELU) maps per-bus Sbus

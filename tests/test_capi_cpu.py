@@ -97,3 +97,51 @@ def test_graph_requires_gpu_backend(cupf):
     o.use_cuda_graph = True
     with pytest.raises(ValueError):
         cupf.NewtonSolver(o)
+
+
+def _top_level_args(text):
+    """Comma-separated arguments at bracket depth 0 of a call or declaration."""
+    depth, count, seen = 0, 0, False
+    for ch in text:
+        if ch in "([{":
+            depth += 1
+        elif ch in ")]}":
+            depth -= 1
+        elif ch == "," and depth == 0:
+            count += 1
+        elif not ch.isspace():
+            seen = True
+    return count + 1 if seen else 0
+
+
+def _declared_arity():
+    import re
+    src = capi.HEADER.read_text()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    arity = {}
+    for m in re.finditer(r"\b(cupf_\w+)\s*\(([^;{]*?)\)\s*;", src, flags=re.S):
+        params = m.group(2).strip()
+        arity[m.group(1)] = 0 if params in ("", "void") else _top_level_args(params)
+    return arity
+
+
+def test_integration_ctypes_snippets_match_the_header():
+    """Every lib.cupf_*(...) call in INTEGRATION.md passes as many arguments
+    as the header declares."""
+    import re
+    doc = (capi.ROOT / "INTEGRATION.md").read_text()
+    arity = _declared_arity()
+    calls = 0
+    for m in re.finditer(r"lib\.(cupf_\w+)\(", doc):
+        name = m.group(1)
+        depth, i = 1, m.end()
+        while depth:
+            depth += {"(": 1, ")": -1}.get(doc[i], 0)
+            i += 1
+        args = doc[m.end():i - 1]
+        if name.endswith("restype"):
+            continue
+        assert name in arity, f"{name} is not declared in the header"
+        assert _top_level_args(args) == arity[name], (name, args, arity[name])
+        calls += 1
+    assert calls >= 5

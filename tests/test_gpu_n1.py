@@ -58,3 +58,45 @@ def test_n1_batch_matches_oracle(gpu, cupf, syn, oracle, workload, compute, B, t
                                            tol=1e-13)
     assert r2.converged_numpy.all()
     assert np.abs(Vout.cpu().numpy() - Vb).max() <= tol
+
+
+def test_reinitialize_larger_grid_same_batch(gpu, cupf, syn, oracle):
+    """initialize() again with a larger grid: the per-scenario Ybus buffer and
+    every batch buffer are re-made for the new structure (a 118-bus N-1 solve,
+    then a 2,000-bus N-1 solve at the same batch on the same solver, both
+    against the oracle), and an attached warm start is detached."""
+    torch = gpu
+    B = 4
+    o = cupf.NewtonOptions()
+    o.custom.max_panel_width = 64
+    s = cupf.NewtonSolver(o)
+    for workload in ("case118", "case2000"):
+        print(f"[reinit] {workload} B={B}", flush=True)
+        g = syn.workload_grid(workload)
+        s.initialize(g.indptr, g.indices, g.data, g.n_bus, g.n_bus, g.pv, g.pq)
+        assert s.stats()["dim"] == g.dim
+        vals, _ = syn.n1_ybus_values(g, B, seed=9)
+        S = g.sbus_scaled(syn.load_factors(B, g.n_bus, 0.9, 1.1, seed=9))
+        V0 = g.flat_start(B)
+        dS, dV0, dY = (torch.from_numpy(a).cuda() for a in (S, V0, vals))
+        Vout = torch.empty_like(dV0)
+        r = s.solve_batch_device_ybus(dS.data_ptr(), g.n_bus, dV0.data_ptr(), g.n_bus,
+                                      dY.data_ptr(), vals.shape[1], B, cupf.NRConfig(),
+                                      v_out_ptr=Vout.data_ptr())
+        assert r.converged_numpy.all()
+        V = Vout.cpu().numpy()
+        for b in range(B):
+            Vo, _, _, _ = oracle.nr_solve_batch(g.n_bus, g.indptr, g.indices, vals[b],
+                                                S[b:b + 1], V0[b:b + 1], g.pv, g.pq, tol=1e-13)
+            assert np.abs(V[b] - Vo[0]).max() <= 1e-9, (workload, b)
+        if workload == "case118":
+            # a warm start attached to the 118-bus grid is dropped by the next initialize()
+            mlp = cupf.WarmStartMLP(g.n_bus, g.pv, g.pq, hidden=16, depth=1, seed=0, max_batch=64,
+                                    skip=False)
+            Sd = torch.from_numpy(S).cuda()
+            mlp.add_samples(Sd.data_ptr(), g.n_bus, Vout.data_ptr(), g.n_bus, B, 0)
+            mlp.finalize(0)
+            s.set_warm_start(mlp, 4)
+    # the 2,000-bus solve above ran without the 118-bus network (or it would have thrown on
+    # the bus count); the plain path is what ran
+    assert s.stats()["last_warm_fallbacks"] == 0 and s.stats()["last_warm_iterations"] == 0
