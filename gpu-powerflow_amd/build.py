@@ -44,6 +44,7 @@ SOURCES = {
     "capi.cpp": [],
     "mlp_kernels.hip": [],
     "warm_start.cpp": [],
+    "static_workspace.hip": [],
 }
 
 

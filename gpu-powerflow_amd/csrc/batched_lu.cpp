@@ -171,34 +171,94 @@ void BatchedLU::analyze(int n, const int* rp, const int* ci, const std::vector<i
     // batch buffers are sized lazily for the new structure
     capBp_ = 0;
     for (DeviceBuffer* b : {&arena0_, &arena1_, &ywork_, &upd_, &res_, &sing_, &uinv_}) b->release();
+    use_reserved();
     n_ = n;
     nnz_ = nnz;
 }
 
-bool BatchedLU::reserve(int Bp) {
-    if (!analyzed()) throw std::runtime_error("BatchedLU: analyze() must be called first");
-    if (Bp % kern::kLanes) throw std::invalid_argument("BatchedLU::reserve(): Bp must be padded to 64");
-    if (Bp <= capBp_) return false;
-    const size_t nb = size_t(Bp);
-    const size_t ts = front_bytes();
-    arena0_.allocate(nb * std::max<size_t>(1, size_t(plan_.arena0)) * ts);
-    arena1_.allocate(nb * std::max<size_t>(1, size_t(plan_.arena1)) * ts);
-    ywork_.allocate(nb * n_ * ts);
-    // interleaved (tier-0 fronts) then scenario-major (tier-1/2 fronts)
-    upd_.allocate(2 * nb * std::max<size_t>(1, size_t(plan_.upd_total)) * ts);
-    res_.allocate(nb * n_ * sizeof(double));
-    sing_.allocate(nb * sizeof(int));
+// Pivot side buffer: U11^-1 and L11^-1 of every front of the widest pivot +
+// Schur group.
+size_t BatchedLU::uinv_elems(int Bp) const {
     int cross_count = 0;
     for (int L = 0; L < plan_.num_levels; ++L)
         if (cross_schur(L))
             cross_count = std::max(cross_count, plan_.group_ptr[L * 3 + 3] - plan_.group_ptr[L * 3 + 2]);
-    if (cross_count > 0) uinv_.allocate(kern::pivot_side_elems(cross_count, Bp) * ts);
+    return cross_count > 0 ? kern::pivot_side_elems(cross_count, Bp) : 0;
+}
+
+namespace {
+// sizes of the batch buffers in LuBatchMem order
+void batch_parts(const LuPlan& P, int n, size_t ts, size_t uinv, int Bp, size_t out[7]) {
+    const size_t nb = size_t(Bp);
+    out[0] = nb * std::max<size_t>(1, size_t(P.arena0)) * ts;
+    out[1] = nb * std::max<size_t>(1, size_t(P.arena1)) * ts;
+    out[2] = nb * size_t(n) * ts;
+    // interleaved (tier-0 fronts) then scenario-major (tier-1/2 fronts)
+    out[3] = 2 * nb * std::max<size_t>(1, size_t(P.upd_total)) * ts;
+    out[4] = nb * size_t(n) * sizeof(double);
+    out[5] = nb * sizeof(int);
+    out[6] = uinv * ts;
+}
+constexpr size_t kPartAlign = 256;
+size_t align_up(size_t x) { return (x + kPartAlign - 1) / kPartAlign * kPartAlign; }
+}  // namespace
+
+bool BatchedLU::reserve(int Bp) {
+    if (!analyzed()) throw std::runtime_error("BatchedLU: analyze() must be called first");
+    if (Bp % kern::kLanes) throw std::invalid_argument("BatchedLU::reserve(): Bp must be padded to 64");
+    if (Bp <= capBp_) {
+        use_reserved();
+        return false;
+    }
+    size_t sz[7];
+    batch_parts(plan_, n_, front_bytes(), uinv_elems(Bp), Bp, sz);
+    DeviceBuffer* own[7] = {&arena0_, &arena1_, &ywork_, &upd_, &res_, &sing_, &uinv_};
+    for (int i = 0; i < 7; ++i) {
+        if (sz[i]) own[i]->allocate(sz[i]);
+        else own[i]->release();
+    }
     capBp_ = Bp;
+    use_reserved();
     return true;
 }
 
+size_t BatchedLU::batch_bytes(int Bp) const {
+    size_t sz[7], total = 0;
+    batch_parts(plan_, n_, front_bytes(), uinv_elems(Bp), Bp, sz);
+    for (size_t v : sz) total += align_up(v);
+    return total;
+}
+
+LuBatchMem BatchedLU::carve(void* base, int Bp) const {
+    size_t sz[7];
+    batch_parts(plan_, n_, front_bytes(), uinv_elems(Bp), Bp, sz);
+    char* p = static_cast<char*>(base);
+    void* part[7];
+    for (int i = 0; i < 7; ++i) {
+        part[i] = sz[i] ? p : nullptr;
+        p += align_up(sz[i]);
+    }
+    LuBatchMem m;
+    m.arena0 = part[0]; m.arena1 = part[1]; m.ywork = part[2]; m.upd = part[3];
+    m.res = static_cast<double*>(part[4]); m.sing = static_cast<int*>(part[5]); m.uinv = part[6];
+    return m;
+}
+
+void BatchedLU::use_memory(const LuBatchMem& mem, int Bp) {
+    if (Bp % kern::kLanes) throw std::invalid_argument("BatchedLU::use_memory(): Bp must be padded to 64");
+    mem_ = mem;
+    memBp_ = Bp;
+}
+
+void BatchedLU::use_reserved() {
+    mem_.arena0 = arena0_.get(); mem_.arena1 = arena1_.get(); mem_.ywork = ywork_.get();
+    mem_.upd = upd_.get(); mem_.res = res_.as<double>(); mem_.sing = sing_.as<int>();
+    mem_.uinv = uinv_.get();
+    memBp_ = capBp_;
+}
+
 void BatchedLU::clear_singular(int Bp, hipStream_t s) {
-    CUPF_HIP(hipMemsetAsync(sing_.get(), 0, size_t(Bp) * sizeof(int), s));
+    kern::zero_flags(mem_.sing, Bp, s);
 }
 
 // Joins aux back into s when a launch between the fork and the join throws,
@@ -270,22 +330,23 @@ bool BatchedLU::cross_schur(int level) const {
 // Schur group, the whole group otherwise.
 template <typename T>
 void BatchedLU::factor_t(const double* A, int Bp, hipStream_t s, StageTimer* timer, int stage) {
-    const kern::Arenas<T> ar{arena0_.as<T>(), arena1_.as<T>()};
+    const kern::Arenas<T> ar{static_cast<T*>(mem_.arena0), static_cast<T*>(mem_.arena1)};
+    T* const uinv = static_cast<T*>(mem_.uinv);
     const int sp = opt_.static_pivot ? 1 : 0;
     auto run = [&](int L, int t, hipStream_t st) {
         int cnt, mf;
         const int* f = group(L, t, cnt, mf);
         if (cnt == 0) return;
         if (t == 2 && cross_schur(L)) {
-            kern::cross_group<T>(lu_, f, cnt, mf, A, Bp, ar, uinv_.as<T>(), opt_.pivot_eps, sp,
-                                 sing_.as<int>(), st);
+            kern::cross_group<T>(lu_, f, cnt, mf, A, Bp, ar, uinv, opt_.pivot_eps, sp, mem_.sing,
+                                 st);
             StageScope sc(timer, stage, st);
-            kern::schur_group<T>(lu_, f, cnt, mf, A, Bp, ar, uinv_.as<T>(), opt_.dense_mode, st);
+            kern::schur_group<T>(lu_, f, cnt, mf, A, Bp, ar, uinv, opt_.dense_mode, st);
             return;
         }
         StageScope sc(timer, t == 2 ? stage : -1, st);
-        kern::lu_factor_group<T>(lu_, t, f, cnt, mf, A, Bp, ar, opt_.pivot_eps, sp,
-                                 sing_.as<int>(), opt_.dense_mode, st);
+        kern::lu_factor_group<T>(lu_, t, f, cnt, mf, A, Bp, ar, opt_.pivot_eps, sp, mem_.sing,
+                                 opt_.dense_mode, st);
     };
     for (int L = 0; L < plan_.num_levels; ++L) {
         const int t0 = first_tier(L);
@@ -296,16 +357,16 @@ void BatchedLU::factor_t(const double* A, int Bp, hipStream_t s, StageTimer* tim
 }
 
 void BatchedLU::factor(const double* A, int Bp, hipStream_t s, StageTimer* timer, int stage) {
-    if (Bp > capBp_) throw std::runtime_error("BatchedLU::factor(): batch exceeds the reserved size");
+    if (Bp > memBp_) throw std::runtime_error("BatchedLU::factor(): batch exceeds the reserved size");
     if (opt_.front64) factor_t<double>(A, Bp, s, timer, stage);
     else factor_t<float>(A, Bp, s, timer, stage);
 }
 
 template <typename T, typename TX>
 void BatchedLU::solve_once(const double* rhs, TX* x, bool transpose, int Bp, hipStream_t s) {
-    const kern::Arenas<T> ar{arena0_.as<T>(), arena1_.as<T>()};
-    T* y = ywork_.as<T>();
-    T* u = upd_.as<T>();
+    const kern::Arenas<T> ar{static_cast<T*>(mem_.arena0), static_cast<T*>(mem_.arena1)};
+    T* y = static_cast<T*>(mem_.ywork);
+    T* u = static_cast<T*>(mem_.upd);
     auto fwd = [&](int L, int t, hipStream_t st) {
         int cnt, mf;
         const int* f = group(L, t, cnt, mf);
@@ -352,12 +413,12 @@ void BatchedLU::residual(const double* A, const double* rhs, const TX* x, double
 template <typename TX>
 void BatchedLU::solve(const double* A, const double* rhs, TX* x, TX* xtmp, bool transpose,
                       int steps, int Bp, hipStream_t s) {
-    if (Bp > capBp_) throw std::runtime_error("BatchedLU::solve(): batch exceeds the reserved size");
+    if (Bp > memBp_) throw std::runtime_error("BatchedLU::solve(): batch exceeds the reserved size");
     solve_dispatch<TX>(rhs, x, transpose, Bp, s);
     if (opt_.front64) return;
     for (int it = 0; it < steps; ++it) {
-        residual<TX>(A, rhs, x, res_.as<double>(), transpose, Bp, s);
-        solve_dispatch<TX>(res_.as<double>(), xtmp, transpose, Bp, s);
+        residual<TX>(A, rhs, x, mem_.res, transpose, Bp, s);
+        solve_dispatch<TX>(mem_.res, xtmp, transpose, Bp, s);
         kern::axpy_update<TX>(xtmp, x, (long long)Bp * n_, s);
     }
 }

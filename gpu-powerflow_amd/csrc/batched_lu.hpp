@@ -21,6 +21,18 @@
 
 namespace cupf {
 
+// Device memory of one batch size: front arenas, solve work vectors,
+// refinement residual, singular-pivot flags and the pivot side buffer.
+struct LuBatchMem {
+    void* arena0 = nullptr;
+    void* arena1 = nullptr;
+    void* ywork = nullptr;
+    void* upd = nullptr;
+    double* res = nullptr;
+    int* sing = nullptr;
+    void* uinv = nullptr;
+};
+
 struct LuOptions {
     bool front64 = true;        // FP64 fronts; otherwise FP32 fronts
     int dense_mode = 0;         // FP32 dense fronts: 1 MFMA FP32, 2 MFMA 3xBF16
@@ -46,11 +58,20 @@ public:
     // Size the batch buffers for Bp (a multiple of 64) scenarios; true when
     // any buffer moved (a captured graph that used them is stale).
     bool reserve(int Bp);
-    int capacity() const { return capBp_; }
+    int capacity() const { return memBp_; }
 
-    // Clear the per-scenario singular-pivot flags.
+    // Bytes of one block holding every batch buffer for Bp scenarios (parts
+    // 256-byte aligned), and those buffers laid out in the block at base.
+    size_t batch_bytes(int Bp) const;
+    LuBatchMem carve(void* base, int Bp) const;
+    // Run the following calls on mem (laid out for Bp scenarios, not owned)
+    // instead of the reserved buffers; use_reserved() switches back.
+    void use_memory(const LuBatchMem& mem, int Bp);
+    void use_reserved();
+
+    // Clear the per-scenario singular-pivot flags (a fill kernel).
     void clear_singular(int Bp, hipStream_t s);
-    const int* singular() const { return sing_.as<int>(); }
+    const int* singular() const { return mem_.sing; }
 
     // Factor the values A [nnz][Bp] (interleaved FP64).  dense_timer, when
     // set, times the dense tier-2 launches under stage dense_stage.
@@ -80,7 +101,7 @@ public:
     double dense_bytes() const { return dense_bytes_; }
     int dense_groups() const { return dense_groups_; }
     // residual work buffer [n][Bp] FP64 (free between calls)
-    double* res() const { return res_.as<double>(); }
+    double* res() const { return mem_.res; }
 
 private:
     struct AuxJoin;
@@ -91,6 +112,7 @@ private:
     bool has_after(int level, int tier) const;
     const int* group(int level, int tier, int& count, int& max_fsz) const;
     bool cross_schur(int level) const;
+    size_t uinv_elems(int Bp) const;
     template <typename T>
     void factor_t(const double* A, int Bp, hipStream_t s, StageTimer* timer, int stage);
     template <typename T, typename TX>
@@ -113,9 +135,11 @@ private:
     DeviceBuffer d_rp, d_ci, d_trp, d_tci, d_tmap;
     kern::LuDevice lu_;
 
-    // batch buffers
+    // batch buffers: the reserved ones (capBp_) and the ones in use (memBp_)
     int capBp_ = 0;
     DeviceBuffer arena0_, arena1_, ywork_, upd_, res_, sing_, uinv_;
+    LuBatchMem mem_;
+    int memBp_ = 0;
 
     // the lowest tier present at a level runs on aux beside the others
     hipStream_t aux_ = nullptr;

@@ -8,6 +8,7 @@
 #include "hip_common.hpp"
 #include "kernels.hpp"
 #include "newton_solver.hpp"
+#include "static_workspace.hpp"
 #include "warm_start.hpp"
 
 #include <algorithm>
@@ -821,6 +822,9 @@ int cupf_lu_create(const cupf_lu_config* config, cupf_lu** out) {
         h->opt.static_pivot = c.pivot_strategy == 1 && c.shift_retry_epsilon > 0.0;
         h->refine = c.refine_steps;
         CUPF_HIP(hipStreamCreate(&h->own));
+        // the static workspace's address on this device, for the
+        // allocation-free entry point
+        (void)cupf::static_workspace();
         *out = h.release();
     });
 }
@@ -922,6 +926,47 @@ int cupf_lu_solve(cupf_lu* h, const double* rhs, double* x, void* stream) {
 
 int cupf_lu_solve_transpose(cupf_lu* h, const double* rhs, double* x, void* stream) {
     return lu_solve_common(h, rhs, x, stream, true);
+}
+
+int cupf_lu_factor_solve_static(cupf_lu* h, int32_t batch, const double* values,
+                                const double* rhs, double* x, int32_t transpose, void* stream) {
+    return lu_guarded(CUPF_LU_FACTORIZATION_FAILED, [&] {
+        cupf_lu& m = lu_of(h);
+        if (m.n == 0) throw LuFailure(CUPF_LU_INVALID_STATE, "cupf_lu_factor_solve_static: analyze first");
+        if (batch <= 0) throw std::invalid_argument("cupf_lu_factor_solve_static: batch must be positive");
+        if (!values || !rhs || !x)
+            throw std::invalid_argument("cupf_lu_factor_solve_static: values, rhs and x must not be null");
+        const int Bp = cupf::kern::pad_batch(batch);
+        // staging (interleaved values, right-hand side, solution and
+        // correction), then the LU's batch buffers, each 256-byte aligned
+        auto up = [](size_t v) { return (v + 255) / 256 * 256; };
+        const size_t vals = up(size_t(Bp) * m.nnz * sizeof(double));
+        const size_t vec = up(size_t(Bp) * m.n * sizeof(double));
+        const size_t need = vals + 3 * vec + m.lu.batch_bytes(Bp);
+        if (need > cupf::kStaticWorkspaceBytes)
+            throw std::invalid_argument(
+                "cupf_lu_factor_solve_static: " + std::to_string(batch) + " systems of dimension " +
+                std::to_string(m.n) + " need " + std::to_string(need) + " bytes of device workspace; " +
+                "the static workspace holds " + std::to_string(cupf::kStaticWorkspaceBytes));
+        const hipStream_t s = stream ? static_cast<hipStream_t>(stream) : m.own;
+        char* base = static_cast<char*>(cupf::static_workspace());
+        double* A = reinterpret_cast<double*>(base);
+        double* b = reinterpret_cast<double*>(base + vals);
+        double* xi = reinterpret_cast<double*>(base + vals + vec);
+        double* xt = reinterpret_cast<double*>(base + vals + 2 * vec);
+        struct Restore {
+            cupf::BatchedLU& lu;
+            ~Restore() { lu.use_reserved(); }
+        } restore{m.lu};
+        m.lu.use_memory(m.lu.carve(base + vals + 3 * vec, Bp), Bp);
+        cupf::kern::to_interleaved(values, m.nnz, m.nnz, batch, Bp, A, s);
+        cupf::kern::to_interleaved(rhs, m.n, m.n, batch, Bp, b, s);
+        m.lu.clear_singular(Bp, s);
+        m.lu.factor(A, Bp, s);
+        m.lu.solve<double>(A, b, xi, xt, transpose != 0, m.refine, Bp, s);
+        cupf::kern::from_interleaved(xi, m.n, batch, Bp, x, m.n, s);
+        if (!stream) CUPF_HIP(hipStreamSynchronize(s));
+    });
 }
 
 int cupf_lu_singular(cupf_lu* h, uint8_t* flags) {

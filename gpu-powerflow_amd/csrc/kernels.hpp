@@ -50,6 +50,8 @@ void from_interleaved(const double* src, int rows, int B, int Bp, double* dst,
 void seed_polar(int n, int Bp, const double2* V, double* Va, double* Vm, hipStream_t s);
 // normF must be zeroed first (zero_norm); per-scenario max |F| via atomics.
 void zero_norm(double* normF, int Bp, hipStream_t s);
+// flags[0..n) = 0 by a fill kernel (capturable; no memset node)
+void zero_flags(int* flags, int n, hipStream_t s);
 void residual(const NrDevice& d, int Bp, const double2* V, const double2* S, double* F,
               double* normF, hipStream_t s);
 // zero_slots lists J slots no Ybus entry reaches (empty for a structurally

@@ -95,6 +95,11 @@ __global__ __launch_bounds__(kBlock) void k_zero(double* __restrict__ p, int n) 
     if (i < n) p[i] = 0.0;
 }
 
+__global__ __launch_bounds__(kBlock) void k_zero_i32(int* __restrict__ p, int n) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
 // Ibus_i for this lane's scenario, accumulated in increasing column order.
 __device__ inline double2 ibus_row(const NrDevice& d, int Bp, const double2* __restrict__ V,
                                    int i, int b) {
@@ -434,6 +439,12 @@ void seed_polar(int n, int Bp, const double2* V, double* Va, double* Vm, hipStre
 
 void zero_norm(double* normF, int Bp, hipStream_t s) {
     hipLaunchKernelGGL(k_zero, dim3((Bp + kBlock - 1) / kBlock), dim3(kBlock), 0, s, normF, Bp);
+    CUPF_HIP(hipGetLastError());
+}
+
+void zero_flags(int* flags, int n, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_zero_i32, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, flags, n);
     CUPF_HIP(hipGetLastError());
 }
 

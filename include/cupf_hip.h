@@ -333,6 +333,21 @@ int cupf_lu_factor(cupf_lu* lu, const double* values, void* stream);
 int cupf_lu_solve(cupf_lu* lu, const double* rhs, double* x, void* stream);
 /* x = A^-T rhs on the factors of the last cupf_lu_factor. */
 int cupf_lu_solve_transpose(cupf_lu* lu, const double* rhs, double* x, void* stream);
+/* Factor values and solve op(A) x = rhs (op = A^T when transpose != 0) in one
+ * call that allocates no device memory: no hipMalloc, hipFree or allocating
+ * library call.  No counterpart in the reference; it is for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid allocation).  Needs cupf_lu_analyze only (no setup);
+ * values [batch][nnz], rhs and x [batch][n], device FP64, as above.  The
+ * factors and every work buffer live in one static device workspace of
+ * 128 MiB per GPU (held from the first cupf_lu_create on that GPU until the
+ * process ends); a batch whose buffers need more is rejected with
+ * CUPF_LU_INVALID_VALUE.  All calls on one GPU share that workspace and it
+ * keeps the last call's contents: calls that overlap on one GPU race, and
+ * the factors do not outlive the call (cupf_lu_solve and cupf_lu_singular
+ * still refer to the last cupf_lu_factor). */
+int cupf_lu_factor_solve_static(cupf_lu* lu, int32_t batch, const double* values,
+                                const double* rhs, double* x, int32_t transpose, void* stream);
 /* Per system: 1 where a static pivot shift was applied in the last factor. */
 int cupf_lu_singular(cupf_lu* lu, uint8_t* flags);
 /* {n, nnz, fronts, largest front, nnz(L), front arena elements per system} */
