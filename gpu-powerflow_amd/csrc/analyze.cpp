@@ -662,7 +662,8 @@ LuPlan lower_plan(const Symbolic& s, int n, const int* rp, const int* ci, int ti
 // Assembly lists: tasks and chunk maps (see analyze.hpp).
 // ===========================================================================
 AsmLists build_asm_lists(const LuPlan& L, const std::vector<int>& kinds,
-                         const std::vector<int>& rows_per_block, int schur_rows) {
+                         const std::vector<int>& rows_per_block, const AsmShape& shape) {
+    const int schur_rows = shape.schur_rows;
     const int P = L.num_fronts;
     AsmLists A;
     A.base.assign(size_t(P) * kAsmKinds, -1);
@@ -673,11 +674,14 @@ AsmLists build_asm_lists(const LuPlan& L, const std::vector<int>& kinds,
         for (int kind = 0; kind < kAsmKinds; ++kind) {
             if (!(kinds[size_t(p)] >> kind & 1)) continue;
             int nblk = 0;
+            const int nct = (uc + shape.tile_cols - 1) / shape.tile_cols;
             switch (kind) {
                 case 0: nblk = (uc + schur_rows - 1) / schur_rows; break;
                 case 1: nblk = 1; break;
                 case 2: nblk = (uc + kAsmChunkCols - 1) / kAsmChunkCols; break;
                 case 3: nblk = (fs + rows_per_block[size_t(p)] - 1) / rows_per_block[size_t(p)]; break;
+                case 5: nblk = (uc + shape.l21_rows - 1) / shape.l21_rows; break;
+                case 6: nblk = (uc + shape.tile_rows - 1) / shape.tile_rows * nct; break;
                 default: nblk = 1; break;
             }
             A.base[size_t(p) * kAsmKinds + kind] = int(A.task_ptr.size()) - 1;
@@ -689,6 +693,12 @@ AsmLists build_asm_lists(const LuPlan& L, const std::vector<int>& kinds,
                     case 2: r1 = K; c0 = K + blk * kAsmChunkCols; c1 = std::min(fs, c0 + kAsmChunkCols); break;
                     case 3: r0 = blk * rows_per_block[size_t(p)];
                             r1 = std::min(fs, r0 + rows_per_block[size_t(p)]); break;
+                    case 5: r0 = K + blk * shape.l21_rows; r1 = std::min(fs, r0 + shape.l21_rows);
+                            c1 = K; break;
+                    case 6: r0 = K + blk / nct * shape.tile_rows;
+                            r1 = std::min(fs, r0 + shape.tile_rows);
+                            c0 = K + blk % nct * shape.tile_cols;
+                            c1 = std::min(fs, c0 + shape.tile_cols); break;
                     default: break;
                 }
                 // one task per child whose contribution lands in the block, child order

@@ -156,10 +156,21 @@ LuPlan lower_plan(const Symbolic& s, int n, const int* rp, const int* ci, int ti
 //          2 pivot-row chunks (rows below the pivot count, 64 columns each
 //            from the pivot count on),
 //          3 row blocks of rows_per_block[p] rows (all columns),
-//          4 whole front.
+//          4 whole front,
+//          5 pivot-column blocks (l21_rows rows each from the pivot count on,
+//            columns below the pivot count),
+//          6 contribution-block tiles (tile_rows x tile_cols from the pivot
+//            count on; tile (rb, ct) is block rb * ceil(uc / tile_cols) + ct).
 // ---------------------------------------------------------------------------
-constexpr int kAsmKinds = 5;
+constexpr int kAsmKinds = 7;
 constexpr int kAsmChunkCols = 64;
+
+struct AsmShape {
+    int schur_rows = 32;     // kind 0 block height
+    int l21_rows = 64;       // kind 5 block height
+    int tile_rows = 64;      // kind 6 tile height
+    int tile_cols = 64;      // kind 6 tile width
+};
 
 struct AsmTask {
     long long off;                   // child's offset in its tier's arena
@@ -196,8 +207,8 @@ struct AsmLists {
 };
 
 // kinds[p] is a bit mask of the kinds built for front p; rows_per_block[p]
-// is the block height of kind 3, schur_rows the block height of kind 0.
+// is the block height of kind 3, shape the block sizes of kinds 0, 5 and 6.
 AsmLists build_asm_lists(const LuPlan& L, const std::vector<int>& kinds,
-                         const std::vector<int>& rows_per_block, int schur_rows);
+                         const std::vector<int>& rows_per_block, const AsmShape& shape);
 
 }  // namespace cupf

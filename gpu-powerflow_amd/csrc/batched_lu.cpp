@@ -7,6 +7,10 @@
 
 namespace cupf {
 
+// neighbouring scenarios that run each contribution-block tile together
+// (1, 4 and 16 measured alike on the 2,000-bus batch)
+constexpr int kTileScenGroup = 4;
+
 // The second stream and its events are made by the first analyze(), so an
 // object that is never analysed touches no GPU.
 BatchedLU::BatchedLU() = default;
@@ -90,6 +94,9 @@ void BatchedLU::analyze(int n, const int* rp, const int* ci, const std::vector<i
                     const int f = P.group_fronts[q];
                     if (t == 1) {
                         kinds[size_t(f)] = 1 << kern::kAsmKFront;
+                    } else if (tile_schur(L)) {
+                        kinds[size_t(f)] = (1 << kern::kAsmKPivot) | (1 << kern::kAsmKChunk) |
+                                           (1 << kern::kAsmKL21) | (1 << kern::kAsmKTile);
                     } else if (cross_schur(L)) {
                         kinds[size_t(f)] = (1 << kern::kAsmKSchur) | (1 << kern::kAsmKPivot) |
                                            (1 << kern::kAsmKChunk);
@@ -100,14 +107,40 @@ void BatchedLU::analyze(int n, const int* rp, const int* ci, const std::vector<i
                     }
                 }
             }
-        const AsmLists al =
-            build_asm_lists(P, kinds, rpb, kern::schur_block_rows(int(front_bytes())));
+        AsmShape shape;
+        shape.schur_rows = kern::schur_block_rows(int(front_bytes()));
+        shape.l21_rows = kern::kL21Rows;
+        shape.tile_rows = kern::kSchurTileRows;
+        shape.tile_cols = kern::kSchurTileCols;
+        const AsmLists al = build_asm_lists(P, kinds, rpb, shape);
         d_abase = upload_vector(al.base, s);
         d_gptr = upload_vector(P.gat_ptr, s);
         d_gsrc = upload_vector(P.gat_src, s);
         d_ridx = upload_vector(P.rhs_idx, s);
         d_acptr = upload_vector(al.chunk_ptr, s);
         d_achunks = upload_vector(al.chunks, s);
+        // tiles of the tile-path levels: (front index in the group,
+        // row block << 16 | column tile), row-major per front
+        std::vector<int2> tiles;
+        tile_ptr_.assign(size_t(P.num_levels) + 1, 0);
+        for (int L = 0; L < P.num_levels; ++L) {
+            tile_ptr_[size_t(L)] = int(tiles.size());
+            if (!tile_schur(L)) continue;
+            const int gi = L * 3 + 2;
+            for (int q = P.group_ptr[gi]; q < P.group_ptr[gi + 1]; ++q) {
+                const int f = P.group_fronts[q];
+                const int uc = P.fsz[f] - P.nc[f];
+                const int nrb = (uc + kern::kSchurTileRows - 1) / kern::kSchurTileRows;
+                const int nct = (uc + kern::kSchurTileCols - 1) / kern::kSchurTileCols;
+                if (nrb >= (1 << 15) || nct >= (1 << 16))
+                    throw std::runtime_error("BatchedLU::analyze(): front too large for the tile list");
+                for (int rb = 0; rb < nrb; ++rb)
+                    for (int ct = 0; ct < nct; ++ct)
+                        tiles.push_back(make_int2(q - P.group_ptr[gi], (rb << 16) | ct));
+            }
+        }
+        tile_ptr_[size_t(P.num_levels)] = int(tiles.size());
+        d_tiles = upload_vector(tiles, s);
     }
     CUPF_HIP(hipStreamSynchronize(s));
 
@@ -141,6 +174,26 @@ void BatchedLU::analyze(int n, const int* rp, const int* ci, const std::vector<i
         if (P.tier[p] != 2) continue;
         const int fsz = P.fsz[p], K = P.nc[p];
         const double f = fsz;
+        if (tile_schur(P.level[p])) {
+            // the tile kernel: C written once, the children's contribution
+            // elements landing in rows and columns >= K read once, L21 and
+            // U12 read once, the A22 entries
+            const double uc = f - K;
+            double child = 0.0;
+            const int* rl = P.row_lb.data() + P.lb_ptr[p];
+            for (int q = P.child_ptr[p]; q < P.child_ptr[p + 1]; ++q) {
+                const int c = P.child_idx[q];
+                const int* cl = rl + (1 + q - P.child_ptr[p]) * (fsz + 1);
+                const double cuc = P.fsz[c] - P.nc[c];
+                child += (cuc - cl[K]) * (cuc - cl[K]);   // contribution rows and columns at >= K
+            }
+            double ents = 0.0;
+            for (int e = rl[K]; e < rl[fsz]; ++e)
+                if (P.ent_dst[e] % fsz >= K) ents += 1.0;
+            dense_flops_ += 2.0 * uc * uc * K + 2.0 * uc * K * K;
+            dense_bytes_ += double(front_bytes()) * (uc * uc + child + 2.0 * uc * K) + 8.0 * ents;
+            continue;
+        }
         if (cross_schur(P.level[p])) {
             const double uc = f - K;
             double child = 0.0;
@@ -326,6 +379,17 @@ bool BatchedLU::cross_schur(int level) const {
     return kern::cross_schur_path(plan_.group_max_fsz[gi], plan_.group_max_nc[gi]);
 }
 
+// Pivot + Schur group whose contribution block is formed by tiles: FP64
+// fronts (FP64 MFMA tiles; 1.35x the row-block kernel's vector FMA on the
+// 2,000-bus batch) and fronts beyond the row-block kernel's 512 rows.  FP32
+// fronts up to 512 rows keep the row-block kernel, which assembles and
+// forms L21 and the contribution rows in one launch (7-13 % faster there
+// than the L21 launch + tiles).
+bool BatchedLU::tile_schur(int level) const {
+    return cross_schur(level) &&
+           (opt_.front64 || !kern::row_schur_fits(plan_.group_max_fsz[level * 3 + 2]));
+}
+
 // Stage `stage` times the dense tier-2 work: the Schur launch of a pivot +
 // Schur group, the whole group otherwise.
 template <typename T>
@@ -338,8 +402,17 @@ void BatchedLU::factor_t(const double* A, int Bp, hipStream_t s, StageTimer* tim
         const int* f = group(L, t, cnt, mf);
         if (cnt == 0) return;
         if (t == 2 && cross_schur(L)) {
+            const bool tiles = tile_schur(L);
             kern::cross_group<T>(lu_, f, cnt, mf, A, Bp, ar, uinv, opt_.pivot_eps, sp, mem_.sing,
-                                 st);
+                                 tiles, st);
+            if (tiles) {
+                StageScope sc(timer, stage, st);
+                const int t0 = tile_ptr_[size_t(L)];
+                kern::schur_tile_group<T>(lu_, f, d_tiles.as<int2>() + t0,
+                                          tile_ptr_[size_t(L) + 1] - t0, kTileScenGroup, A, Bp, ar,
+                                          opt_.dense_mode, st);
+                return;
+            }
             StageScope sc(timer, stage, st);
             kern::schur_group<T>(lu_, f, cnt, mf, A, Bp, ar, uinv, opt_.dense_mode, st);
             return;

@@ -112,6 +112,7 @@ private:
     bool has_after(int level, int tier) const;
     const int* group(int level, int tier, int& count, int& max_fsz) const;
     bool cross_schur(int level) const;
+    bool tile_schur(int level) const;
     size_t uinv_elems(int Bp) const;
     template <typename T>
     void factor_t(const double* A, int Bp, hipStream_t s, StageTimer* timer, int stage);
@@ -130,7 +131,10 @@ private:
     // device plan
     DeviceBuffer d_fsz, d_nc, d_tier, d_off, d_uoff, d_fptr, d_frows, d_cptr, d_cidx, d_aptr,
         d_aloc, d_eptr, d_esrc, d_edst, d_perm, d_groups, d_lbptr, d_rowlb, d_abase, d_acptr,
-        d_achunks, d_gptr, d_gsrc, d_ridx;
+        d_achunks, d_gptr, d_gsrc, d_ridx, d_tiles;
+    // contribution-block tiles of the tile-path levels: level L's at
+    // tile_ptr_[L] .. tile_ptr_[L + 1] of d_tiles
+    std::vector<int> tile_ptr_;
     // pattern and its transpose with value map (refinement residuals)
     DeviceBuffer d_rp, d_ci, d_trp, d_tci, d_tmap;
     kern::LuDevice lu_;

@@ -550,16 +550,20 @@ int cupf_plan_maps_check(int32_t n, const int32_t* row_ptr, const int32_t* col_i
         const cupf::Symbolic s = cupf::analyze_symbolic(n, row_ptr, col_idx, pv, cap);
         const cupf::LuPlan L = cupf::lower_plan(s, n, row_ptr, col_idx);
         const int P = L.num_fronts;
-        // every kind the solver uses: tier-1 fronts whole, tier-2 pivot + Schur
-        // fronts by pivot block / pivot-row chunks / Schur row blocks, other
-        // tier-2 fronts by row blocks
+        // every kind the solver uses: tier-1 fronts whole, tier-2 fronts with
+        // at most 64 pivots by pivot block / pivot-row chunks / pivot-column
+        // blocks / contribution-block tiles, other tier-2 fronts by row blocks
         std::vector<int> kinds(size_t(P), 0), rpb(size_t(P), 8);
         for (int p = 0; p < P; ++p) {
             if (L.tier[p] == 1) kinds[size_t(p)] = 1 << 4;
             else if (L.tier[p] == 2)
-                kinds[size_t(p)] = (L.fsz[p] <= 512 && L.nc[p] <= 64) ? 7 : (1 << 3);
+                kinds[size_t(p)] = L.nc[p] <= 64 ? (1 << 1) | (1 << 2) | (1 << 5) | (1 << 6) : (1 << 3);
         }
-        const cupf::AsmLists A = cupf::build_asm_lists(L, kinds, rpb, 32);
+        cupf::AsmShape shape;
+        shape.l21_rows = 64;
+        shape.tile_rows = 64;
+        shape.tile_cols = 64;
+        const cupf::AsmLists A = cupf::build_asm_lists(L, kinds, rpb, shape);
         // covered count per arena element (tier 0, tiers 1/2)
         std::vector<int> c0(size_t(L.arena0), 0), c1(size_t(L.arena1), 0);
         long long wrong_dst = 0;
@@ -568,13 +572,26 @@ int cupf_plan_maps_check(int32_t n, const int32_t* row_ptr, const int32_t* col_i
                 const int base = A.base[size_t(p) * cupf::kAsmKinds + kind];
                 if (base < 0) continue;
                 const int fs = L.fsz[p], K = L.nc[p], uc = fs - K;
+                const int nct = (uc + shape.tile_cols - 1) / shape.tile_cols;
                 int nblk = 1;
                 if (kind == 0) nblk = (uc + 31) / 32;
                 else if (kind == 2) nblk = (uc + cupf::kAsmChunkCols - 1) / cupf::kAsmChunkCols;
                 else if (kind == 3) nblk = (fs + 7) / 8;
+                else if (kind == 5) nblk = (uc + shape.l21_rows - 1) / shape.l21_rows;
+                else if (kind == 6) nblk = (uc + shape.tile_rows - 1) / shape.tile_rows * nct;
                 for (int blk = 0; blk < nblk; ++blk) {
                     const int list = base + blk;
-                    const int r0 = kind == 0 ? K + blk * 32 : (kind == 3 ? blk * 8 : 0);
+                    int r0 = 0, r1 = fs, q0 = 0, q1 = fs;   // block rows / columns
+                    if (kind == 0) { r0 = K + blk * 32; r1 = std::min(fs, r0 + 32); }
+                    else if (kind == 1) { r1 = K; q1 = K; }
+                    else if (kind == 2) { r1 = K; q0 = K + blk * 64; q1 = std::min(fs, q0 + 64); }
+                    else if (kind == 3) { r0 = blk * 8; r1 = std::min(fs, r0 + 8); }
+                    else if (kind == 5) { r0 = K + blk * shape.l21_rows;
+                                          r1 = std::min(fs, r0 + shape.l21_rows); q1 = K; }
+                    else if (kind == 6) { r0 = K + blk / nct * shape.tile_rows;
+                                          r1 = std::min(fs, r0 + shape.tile_rows);
+                                          q0 = K + blk % nct * shape.tile_cols;
+                                          q1 = std::min(fs, q0 + shape.tile_cols); }
                     for (int q = A.chunk_ptr[size_t(list) * cupf::kAsmItemWaves];
                          q < A.chunk_ptr[size_t(list + 1) * cupf::kAsmItemWaves]; ++q) {
                         const int src = A.chunks[size_t(q) * 4], lo = A.chunks[size_t(q) * 4 + 1];
@@ -597,7 +614,8 @@ int cupf_plan_maps_check(int32_t n, const int32_t* row_ptr, const int32_t* col_i
                             const int ca = int(le / L.fsz[child]) - L.nc[child];
                             const int cb = int(le % L.fsz[child]) - L.nc[child];
                             const int* loc = L.asm_loc.data() + L.asm_ptr[child];
-                            if (ca < 0 || cb < 0 || loc[ca] != r0 + rr || L.asm_loc[size_t(lo + j)] != loc[cb])
+                            if (ca < 0 || cb < 0 || loc[ca] != r0 + rr || L.asm_loc[size_t(lo + j)] != loc[cb] ||
+                                r0 + rr >= r1 || loc[cb] < q0 || loc[cb] >= q1)
                                 ++wrong_dst;
                         }
                     }

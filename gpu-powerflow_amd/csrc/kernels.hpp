@@ -94,7 +94,13 @@ void adjoint_project(const NrDevice& d, int B, int Bp, const double* lambda, TG*
 // ---- batched multifrontal LU (see analyze.hpp LuPlan) -----------------------
 // Assembly kinds of the lists (analyze.hpp build_asm_lists).
 enum { kAsmKSchur = 0, kAsmKPivot = 1, kAsmKChunk = 2, kAsmKRows = 3, kAsmKFront = 4,
-       kAsmKindCount = 5 };
+       kAsmKL21 = 5, kAsmKTile = 6, kAsmKindCount = 7 };
+
+// Block sizes of the pivot-column blocks (kind 5) and contribution-block
+// tiles (kind 6).
+constexpr int kL21Rows = 64;
+constexpr int kSchurTileRows = 64;
+constexpr int kSchurTileCols = 64;
 
 struct LuDevice {
     int n = 0, nnz = 0, P = 0;
@@ -154,15 +160,30 @@ void lu_factor_group(const LuDevice& d, int tier, const int* fronts, int count, 
 // schur_block_rows(sizeof(T)) rows per workgroup).  FP32 dense_mode 1: FP32
 // MFMA, 2: 3xBF16 MFMA; FP64 runs on the vector units.
 bool cross_schur_path(int max_fsz, int max_nc);
+// schur_group (row blocks) handles fronts of at most 512 rows
+bool row_schur_fits(int max_fsz);
 size_t pivot_side_elems(int count, int Bp);
 int schur_block_rows(int elem);
 template <typename T>
 void cross_group(const LuDevice& d, const int* fronts, int count, int max_fsz, const double* A,
                  int Bp, Arenas<T> ar, T* uinv, double pivot_eps, int static_piv, int* sing,
-                 hipStream_t s);
+                 bool with_l21, hipStream_t s);
 template <typename T>
 void schur_group(const LuDevice& d, const int* fronts, int count, int max_fsz, const double* A,
                  int Bp, Arenas<T> ar, const T* uinv, int dense_mode, hipStream_t s);
+
+// Tile path of a group whose fronts have at most 64 pivots (any size):
+// cross_group with with_l21 also writes L21 = A21 U11^-1 by blocks of
+// kL21Rows rows (FP32 MFMA for FP32 fronts, FP64 MFMA for FP64 fronts) in
+// the launch that forms U12; schur_tile_group then forms C = A22 - L21 U12 by
+// kSchurTileRows x kSchurTileCols tiles, listed at tiles[0..ntiles) as (front
+// index in the group, row block << 16 | column tile); scen_group
+// neighbouring scenarios run each tile together.  FP32 dense_mode 1: FP32
+// MFMA, 2: 3xBF16 MFMA; FP64 fronts: FP64 MFMA.
+template <typename T>
+void schur_tile_group(const LuDevice& d, const int* fronts, const int2* tiles, int ntiles,
+                      int scen_group, const double* A, int Bp, Arenas<T> ar, int dense_mode,
+                      hipStream_t s);
 
 // Solve sweeps.  rhs (FP64) and x are [n][Bp]; y [n][Bp] and upd are work
 // vectors, upd holding the tier-0 fronts' update vectors as [upd_total][Bp]

@@ -83,26 +83,52 @@ __device__ inline const T* front_ptr(const LuDevice& d, const Arenas<T>& ar, int
 // Original entries: F[map(dst)] = A(src) for the entries e in [e0, e1) whose
 // map(dst) >= 0 (dst: position in the fsz x fsz front, A interleaved
 // [nnz][Bp]); 4 entries in flight per thread.
+// Every load is issued unconditionally from an index clamped into [e0, e1)
+// (the loop runs only while e < e1), so the four entries' index loads and
+// then their value loads are in flight together.
 template <int NT, typename T, typename Map>
 __device__ inline void put_entries(T* __restrict__ F, const int* __restrict__ dst,
                                    const int* __restrict__ src, const double* __restrict__ A,
                                    int Bp, int b, int e0, int e1, Map map) {
     for (int e = e0 + int(threadIdx.x); e < e1; e += 4 * NT) {
-        int di[4];
-        double v[4];
+        int dd[4], ss[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            const int ee = e + u * NT;
-            di[u] = ee < e1 ? map(dst[ee]) : -1;
-            v[u] = di[u] >= 0 ? A[(long long)src[ee] * Bp + b] : 0.0;
+            const int ee = min(e + u * NT, e1 - 1);
+            dd[u] = dst[ee];
+            ss[u] = src[ee];
         }
+        double v[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
-            if (di[u] >= 0) F[di[u]] = T(v[u]);
+        for (int u = 0; u < 4; ++u) v[u] = A[(long long)ss[u] * Bp + b];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int di = e + u * NT < e1 ? map(dd[u]) : -1;
+            if (di >= 0) F[di] = T(v[u]);
+        }
     }
 }
 
 constexpr int kItemWaves = 4;              // owner waves of a block's rows (rr % 4)
+
+// Workgroup barrier that orders LDS accesses only: global loads in flight
+// stay in flight across it (__syncthreads() also waits for those).  For
+// barriers between LDS writes and LDS reads of other waves.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Zero bytes [0, 16 n16) of LDS at p (16-byte stores), all threads of a
+// workgroup of NT.
+template <int NT>
+__device__ __forceinline__ void lds_zero16(void* p, int n16) {
+    int4* q = static_cast<int4*>(p);
+    for (int e = int(threadIdx.x); e < n16; e += NT) q[e] = make_int4(0, 0, 0, 0);
+}
+
+
 
 // Assembly of one block of a front from its chunk map (host-built, see
 // LuDevice asm_cptr / asm_chunks): each wave walks the chunks of the block
@@ -852,6 +878,7 @@ constexpr int kPivThreads = 256;
 constexpr int kXMaxK = 64;
 constexpr int kXChunk = 64;                   // A12 columns per U12 step
 constexpr int kXCld = kXChunk + 4;            // chunk row stride (elements)
+constexpr int kL21Ld = kXMaxK + 2;            // A21 block row stride (elements)
 constexpr int kUinvElems = kXMaxK * kXMaxK;   // U11^-1 (then L11^-1) per (front, scenario)
 constexpr int kSideElems = 2 * kUinvElems;    // side-buffer elements per (front, scenario)
 
@@ -990,21 +1017,13 @@ __global__ __launch_bounds__(pivot_threads<T>()) void k_pivot(LuDevice d, const 
 // L11^-1 from the side buffer, the product on MFMA (FP32) or vector FMA tiles
 // (FP64), written to the front's pivot rows.
 template <typename T>
-__global__ __launch_bounds__(kPivThreads) void k_u12(LuDevice d, const int* __restrict__ fronts,
-                                                      const double* __restrict__ A, int Bp,
-                                                      Arenas<T> ar, const T* __restrict__ side,
-                                                      int nch) {
+__device__ __forceinline__ void u12_chunk(const LuDevice& d, const int* __restrict__ fronts,
+                                          const double* __restrict__ A, int Bp,
+                                          const Arenas<T>& ar, const T* __restrict__ side, int fi,
+                                          int ch, int b) {
     extern __shared__ unsigned char smem_raw[];
     T* Li = reinterpret_cast<T*>(smem_raw);                        // [64][kLinvLd]
     T* Ch = Li + 64 * kLinvLd;                                    // [64][kXCld]
-    // lin -> (front, chunk, scenario), scenario fastest inside one XCD
-    const long long lin = blockIdx.x;
-    const long long per = (long long)gridDim.x / kXcd;
-    long long w = (lin % kXcd) * per + lin / kXcd;
-    const int b = int(w % Bp);
-    w /= Bp;
-    const int ch = int(w % nch);
-    const int fi = int(w / nch);
     const int p = fronts[fi];
     const int fs = d.fsz[p], K = d.nc[p];
     const int j0 = K + ch * kXChunk;
@@ -1019,12 +1038,12 @@ __global__ __launch_bounds__(kPivThreads) void k_u12(LuDevice d, const int* __re
     {
         const int jn = min(kXChunk, fs - j0);
         for (int e = tid; e < 64 * kXCld; e += kPivThreads) Ch[e] = T(0);
-        __syncthreads();
+        lds_barrier();
         put_entries<kPivThreads>(Ch, d.ent_dst, d.ent_src, A, Bp, b, rl[0], rl[K], [=](int dst) {
             const int rr = dst / fs, c = dst - rr * fs;
             return (c >= j0 && c < j0 + jn) ? rr * kXCld + c - j0 : -1;
         });
-        __syncthreads();
+        lds_barrier();
         assemble_chunks<kPivThreads, 16>(d, ar, d.asm_base[p * kAsmKindCount + kAsmKChunk] + ch, b, Bp, Ch,
                                          [=](int r, int c) { return r * kXCld + c - j0; });
 #pragma unroll
@@ -1032,7 +1051,7 @@ __global__ __launch_bounds__(kPivThreads) void k_u12(LuDevice d, const int* __re
             const int e = tid + u * kPivThreads;
             Li[(e >> 6) * kLinvLd + (e & 63)] = lv[u];
         }
-        __syncthreads();
+        lds_barrier();
         if constexpr (sizeof(T) == 4) {
             // 64 x 64 output = 2 x 2 tiles of 32 x 32, one per wave; L11^-1 is
             // lower triangular, so the top row half needs k < 32 only
@@ -1306,6 +1325,320 @@ __global__ __launch_bounds__(kSchurThreads, 2) void k_schur(LuDevice d, const in
                 }
             }
         }
+    }
+}
+
+// ===========================================================================
+// Tier 2, fronts with at most 64 pivots: pivot columns and contribution-block
+// tiles (after k_pivot and k_u12 of the same group).
+//
+//   k_l21 (one workgroup per (front, 64-row block, scenario), 256 lanes):
+//     assembles its rows of A21 in LDS (entries and child contribution
+//     rows) and writes L21 = A21 U11^-1 to the front (FP32 fronts: FP32
+//     MFMA 32x32x2, one 32 x 32 tile per wave; FP64 fronts: FP64 MFMA
+//     16x16x4, a 16-row strip per wave).
+//   k_schur_tile (one workgroup per (tile, scenario), 256 lanes): one
+//     64 x 64 tile of the contribution block C = A22 - L21 U12.  The tile of
+//     A22 is assembled in LDS (entries, then the tile's chunk map) while each
+//     wave's operands -- 32 rows of L21 and 32 columns of U12, read from the
+//     front -- are in flight; each wave then forms its 32 x 32 quarter on
+//     MFMA (FP32 32x32x2, 3xBF16 32x32x16, or FP64 16x16x4 as 2 x 2 tiles)
+//     starting from the assembled values, and writes it once.
+//
+// The operands' k index is split by lane group so every lane reads a run of
+// contiguous L21 elements: in an FP32 32x32x2 instruction i, lane half h
+// carries k = 32 h + i; in a 3xBF16 32x32x16 instruction j, k = 32 h + 8 j +
+// t (t < 8); in an FP64 16x16x4 instruction i, lane quarter q carries
+// k = 16 q + i.  FP64 16x16x4 layouts (checked on gfx950): A lane l = row
+// l % 16, k = l / 16; B lane l = column l % 16, same k; accumulator register
+// v of lane l holds row l / 16 + 4 v, column l % 16.  The MFMA sums over every k of an instruction, so any
+// assignment shared by both operands gives the same product.
+//
+// Tiles are listed per group on the host (front index in the group, row
+// block << 16 | column tile), row-major per front; blocks are decoded so that
+// an XCD runs its scenarios' tiles in list order, a few neighbouring
+// scenarios interleaved (they share the J values' cache lines), so a
+// front's L21 rows and U12 columns are re-read from that XCD's L2.
+// ===========================================================================
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+
+constexpr int kTileRows = kSchurTileRows;
+constexpr int kTileCols = kSchurTileCols;
+
+template <typename T>
+constexpr int tile_ld() { return sizeof(T) == 4 ? kTileCols + 8 : kTileCols + 4; }
+
+template <typename T>
+constexpr size_t tile_lds() { return size_t(kTileRows) * tile_ld<T>() * sizeof(T); }
+
+constexpr size_t l21_lds_elems() { return size_t(kL21Rows) * kL21Ld; }
+
+template <typename T>
+__device__ __forceinline__ void l21_block(const LuDevice& d, const int* __restrict__ fronts,
+                                          const double* __restrict__ A, int Bp,
+                                          const Arenas<T>& ar, const T* __restrict__ uinv, int fi,
+                                          int blk, int b) {
+    extern __shared__ unsigned char smem_raw[];
+    T* Ab = reinterpret_cast<T*>(smem_raw);                        // [kL21Rows][kL21Ld]
+    const int p = fronts[fi];
+    const int fs = d.fsz[p], K = d.nc[p];
+    const int i0 = K + blk * kL21Rows;
+    if (i0 >= fs) return;                       // whole block exits together
+    const int i1 = min(fs, i0 + kL21Rows), nr = i1 - i0;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    T* G = ar.a1 + (long long)b * d.arena1 + d.off[p];
+    const int* rl = d.row_lb + d.lb_ptr[p];
+    const T* Ui = uinv + ((long long)fi * Bp + b) * kSideElems;   // U11^-1 [k][j], upper
+
+    static_assert(l21_lds_elems() * sizeof(T) % 16 == 0, "16-byte zeroing");
+    lds_zero16<256>(Ab, int(l21_lds_elems() * sizeof(T) / 16));
+    lds_barrier();
+    put_entries<256>(Ab, d.ent_dst, d.ent_src, A, Bp, b, rl[i0], rl[i1], [=](int dst) {
+        const int r = dst / fs, c = dst - r * fs;
+        return c < K ? (r - i0) * kL21Ld + c : -1;
+    });
+    lds_barrier();
+    assemble_chunks<256, 16>(d, ar, d.asm_base[p * kAsmKindCount + kAsmKL21] + blk, b, Bp, Ab,
+                             [=](int rr, int c) { return rr * kL21Ld + c; });
+    lds_barrier();
+    if constexpr (sizeof(T) == 4) {
+        // 64 x 64 = 2 x 2 tiles of 32 x 32, one per wave
+        const int rt = wave & 1, ct = wave >> 1;
+        if (32 * rt < nr && 32 * ct < K) {                  // wave-uniform
+            floatx16 acc;
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[v] = 0.f;
+            const int j = 32 * ct + (lane & 31);
+            const int kend = min(K, 32 * ct + 32);           // U11^-1 upper: k <= j
+            for (int kk = 0; kk < kend; kk += 2) {
+                const int k = kk + (lane >> 5);
+                const float av = Ab[(32 * rt + (lane & 31)) * kL21Ld + k];
+                const float bv = (k < K && j < K) ? float(Ui[k * kXMaxK + j]) : 0.f;
+                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv, acc, 0, 0, 0);
+            }
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const int r = 32 * rt + (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
+                if (r < nr && j < K) G[(long long)(i0 + r) * fs + j] = acc[v];
+            }
+        }
+    } else {
+        // 64 x 64 = 4 x 4 tiles of 16 x 16; wave w: the 16-row strip w
+        if (16 * wave < nr) {                               // wave-uniform
+            doublex4 acc[4];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) acc[c] = doublex4{0.0, 0.0, 0.0, 0.0};
+            const int r = 16 * wave + (lane & 15), q = lane >> 4;
+            for (int kk = 0; kk < K; kk += 4) {
+                const int k = kk + q;
+                const double av = k < K ? double(Ab[r * kL21Ld + k]) : 0.0;
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    if (16 * c < K) {                       // uniform
+                        const int j = 16 * c + (lane & 15);
+                        const double bv = (k < K && j < K) ? double(Ui[k * kXMaxK + j]) : 0.0;
+                        acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[c], 0, 0, 0);
+                    }
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const int j = 16 * c + (lane & 15);
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int rr = 16 * wave + q + 4 * v;
+                    if (rr < nr && j < K) G[(long long)(i0 + rr) * fs + j] = T(acc[c][v]);
+                }
+            }
+        }
+    }
+}
+
+// Block decode of the pivot-row chunk and pivot-column block launches:
+// lin -> (front, part, scenario), an XCD owning a contiguous range of
+// scenarios (the block count is a multiple of 8).
+__device__ inline void decode_part(int Bp, int parts, int& fi, int& part, int& b) {
+    const long long lin = blockIdx.x;
+    const long long per = (long long)gridDim.x / kXcd;
+    long long w = (lin % kXcd) * per + lin / kXcd;
+    b = int(w % Bp);
+    w /= Bp;
+    part = int(w % parts);
+    fi = int(w / parts);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kPivThreads) void k_u12(LuDevice d, const int* __restrict__ fronts,
+                                                      const double* __restrict__ A, int Bp,
+                                                      Arenas<T> ar, const T* __restrict__ side,
+                                                      int nch) {
+    int fi, ch, b;
+    decode_part(Bp, nch, fi, ch, b);
+    u12_chunk<T>(d, fronts, A, Bp, ar, side, fi, ch, b);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_l21(LuDevice d, const int* __restrict__ fronts,
+                                             const double* __restrict__ A, int Bp, Arenas<T> ar,
+                                             const T* __restrict__ uinv, int nblk) {
+    int fi, blk, b;
+    decode_part(Bp, nblk, fi, blk, b);
+    l21_block<T>(d, fronts, A, Bp, ar, uinv, fi, blk, b);
+}
+
+template <typename T, int MODE>
+__global__ __launch_bounds__(256) void k_schur_tile(LuDevice d, const int* __restrict__ fronts,
+                                                    const int2* __restrict__ tiles, int ntiles,
+                                                    int scen_group, const double* __restrict__ A,
+                                                    int Bp, Arenas<T> ar) {
+    constexpr int LD = tile_ld<T>();
+    extern __shared__ unsigned char smem_raw[];
+    T* At = reinterpret_cast<T*>(smem_raw);                        // [kTileRows][LD]
+    const long long lin = blockIdx.x;
+    const long long per = (long long)gridDim.x / kXcd;
+    long long w = (lin % kXcd) * per + lin / kXcd;
+    const int bl = int(w % scen_group);
+    w /= scen_group;
+    const int t = int(w % ntiles);
+    const int b = int(w / ntiles) * scen_group + bl;
+    const int2 tl = tiles[t];
+    const int p = fronts[tl.x];
+    const int rb = tl.y >> 16, ct = tl.y & 0xffff;
+    const int fs = d.fsz[p], K = d.nc[p], uc = fs - K;
+    const int nct = (uc + kTileCols - 1) / kTileCols;
+    const int r0 = K + rb * kTileRows, c0 = K + ct * kTileCols;
+    const int nr = min(kTileRows, fs - r0), ncl = min(kTileCols, fs - c0);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wr = wave & 1, wc = wave >> 1;                     // this wave's 32 x 32 quarter
+    const bool busy = 32 * wr < nr && 32 * wc < ncl;             // wave-uniform
+    T* G = ar.a1 + (long long)b * d.arena1 + d.off[p];
+    const int* rl = d.row_lb + d.lb_ptr[p];
+
+    // ---- operands, in flight during the assembly ----------------------------
+    constexpr int NOP = sizeof(T) == 4 ? 32 : 16;
+    constexpr int NT = sizeof(T) == 4 ? 1 : 2;                  // 16 x 16 sub-tiles per side (FP64)
+    T a[NT][NOP], u[NT][NOP];
+    // Rows and columns past the front read its last row / column (the
+    // results there are not stored); pivot indices k at or past K read pivot
+    // column / row K - 1 and are zeroed in the L21 operand after the
+    // assembly.  Every index is clamped into the front (r, c in [0, fs),
+    // k in [0, K)), so all loads are unconditional; the element offsets
+    // r fs + k and k fs + c are below fs^2, and their byte offsets below
+    // 2^32 (fronts are far smaller than 2^29 elements).
+    if (busy) {
+        constexpr int KW = sizeof(T) == 4 ? 32 : 16;             // k run per lane group
+        const int g = sizeof(T) == 4 ? lane >> 5 : lane >> 4;
+        const int li = sizeof(T) == 4 ? lane & 31 : lane & 15;
+        const bool full = K == kXMaxK;                           // uniform
+        const char* Gb = reinterpret_cast<const char*>(G);
+        constexpr unsigned ES = unsigned(sizeof(T));
+#pragma unroll
+        for (int s = 0; s < NT; ++s) {
+            const int r = min(r0 + 32 * wr + 16 * s + li, fs - 1);
+            const int c = min(c0 + 32 * wc + 16 * s + li, fs - 1);
+#pragma unroll
+            for (int i = 0; i < NOP; ++i) {
+                const int k = full ? KW * g + i : min(KW * g + i, K - 1);   // in [0, K)
+                a[s][i] = *reinterpret_cast<const T*>(Gb + unsigned(r * fs + k) * ES);
+                u[s][i] = *reinterpret_cast<const T*>(Gb + unsigned(k * fs + c) * ES);
+            }
+        }
+    }
+
+    // ---- the tile of A22 ------------------------------------------------------
+    static_assert(tile_lds<T>() % 16 == 0, "16-byte zeroing");
+    lds_zero16<256>(At, int(tile_lds<T>() / 16));
+    lds_barrier();
+    put_entries<256>(At, d.ent_dst, d.ent_src, A, Bp, b, rl[r0], rl[r0 + nr], [=](int dst) {
+        const int r = dst / fs, c = dst - r * fs;
+        return (c >= c0 && c < c0 + ncl) ? (r - r0) * LD + c - c0 : -1;
+    });
+    lds_barrier();
+    assemble_chunks<256, 16>(d, ar, d.asm_base[p * kAsmKindCount + kAsmKTile] + rb * nct + ct, b,
+                             Bp, At, [=](int rr, int c) { return rr * LD + c - c0; });
+    lds_barrier();
+    if (!busy) return;                           // no barrier below
+    if (K < kXMaxK) {                            // uniform: k >= K terms of the L21 operand are 0
+        constexpr int KW = sizeof(T) == 4 ? 32 : 16;
+        const int g = sizeof(T) == 4 ? lane >> 5 : lane >> 4;
+#pragma unroll
+        for (int s = 0; s < NT; ++s)
+#pragma unroll
+            for (int i = 0; i < NOP; ++i)
+                if (KW * g + i >= K) a[s][i] = T(0);
+    }
+
+    if constexpr (sizeof(T) == 4) {
+        const int h = lane >> 5, li = lane & 31;
+        floatx16 acc;
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+            const int rr = 32 * wr + (v >> 2) * 8 + h * 4 + (v & 3);
+            acc[v] = At[rr * LD + 32 * wc + li];
+        }
+        if (MODE == kMfmaF32) {
+#pragma unroll
+            for (int i = 0; i < NOP; ++i)
+                if (i < K)                               // uniform
+                    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(-a[0][i], u[0][i], acc, 0, 0, 0);
+        } else {
+#pragma unroll
+            for (int j = 0; j < NOP / 8; ++j) {
+                if (8 * j < K) {                         // uniform
+                    bf16x8 ah, al, bh, blo;
+#pragma unroll
+                    for (int t8 = 0; t8 < 8; ++t8) {
+                        __bf16 hi, lo;
+                        split_bf16(-a[0][8 * j + t8], hi, lo);
+                        ah[t8] = hi; al[t8] = lo;
+                        split_bf16(u[0][8 * j + t8], hi, lo);
+                        bh[t8] = hi; blo[t8] = lo;
+                    }
+                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc, 0, 0, 0);
+                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, blo, acc, 0, 0, 0);
+                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc, 0, 0, 0);
+                }
+            }
+        }
+        const int cl = 32 * wc + li;
+        T* Cw = G + (long long)r0 * fs + c0 + cl;
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+            const int rr = 32 * wr + (v >> 2) * 8 + h * 4 + (v & 3);
+            if (rr < nr && cl < ncl) Cw[(long long)rr * fs] = acc[v];
+        }
+    } else {
+        const int q = lane >> 4, li = lane & 15;
+        doublex4 acc[NT][NT];
+#pragma unroll
+        for (int s = 0; s < NT; ++s)
+#pragma unroll
+            for (int c = 0; c < NT; ++c)
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                    acc[s][c][v] = At[(32 * wr + 16 * s + q + 4 * v) * LD + 32 * wc + 16 * c + li];
+#pragma unroll
+        for (int i = 0; i < NOP; ++i) {
+            if (i < K) {                                 // uniform
+#pragma unroll
+                for (int s = 0; s < NT; ++s)
+#pragma unroll
+                    for (int c = 0; c < NT; ++c)
+                        acc[s][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a[s][i], u[c][i], acc[s][c],
+                                                                         0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < NT; ++s)
+#pragma unroll
+            for (int c = 0; c < NT; ++c) {
+                const int cl = 32 * wc + 16 * c + li;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int rr = 32 * wr + 16 * s + q + 4 * v;
+                    if (rr < nr && cl < ncl) G[(long long)(r0 + rr) * fs + c0 + cl] = acc[s][c][v];
+                }
+            }
     }
 }
 
@@ -1844,13 +2177,16 @@ void assemble_rows(const LuDevice& d, const int* fronts, int count, int max_fsz,
 }
 
 bool cross_schur_path(int max_fsz, int max_nc) {
-    return max_fsz <= kDenseMaxFsz && max_nc <= kXMaxK;
+    (void)max_fsz;
+    return max_nc <= kXMaxK;
 }
+
+bool row_schur_fits(int max_fsz) { return max_fsz <= kDenseMaxFsz; }
 
 template <typename T>
 void cross_group(const LuDevice& d, const int* fronts, int count, int max_fsz, const double* A,
                  int Bp, Arenas<T> ar, T* uinv, double pivot_eps, int static_piv, int* sing,
-                 hipStream_t s) {
+                 bool with_l21, hipStream_t s) {
     if (count <= 0) return;
     static bool attr_set = false;
     if (!attr_set) {
@@ -1865,9 +2201,19 @@ void cross_group(const LuDevice& d, const int* fronts, int count, int max_fsz, c
                        fronts, A, Bp, ar, uinv, pivot_eps, static_piv, sing);
     CUPF_HIP(hipGetLastError());
     const int nch = (max_fsz + kXChunk - 1) / kXChunk;
-    hipLaunchKernelGGL(k_u12<T>, dim3(unsigned(count) * unsigned(nch) * unsigned(Bp)), dim3(kPivThreads),
-                       u12_lds<T>(), s, d, fronts, A, Bp, ar, uinv, nch);
+    const long long blocks = (long long)count * nch * Bp;
+    if (blocks >= (1LL << 31)) throw std::runtime_error("cross_group: grid too large");
+    hipLaunchKernelGGL(k_u12<T>, dim3(static_cast<unsigned>(blocks)), dim3(kPivThreads), u12_lds<T>(),
+                       s, d, fronts, A, Bp, ar, uinv, nch);
     CUPF_HIP(hipGetLastError());
+    if (with_l21) {
+        const int nblk = (max_fsz + kL21Rows - 1) / kL21Rows;
+        const long long lb = (long long)count * nblk * Bp;
+        if (lb >= (1LL << 31)) throw std::runtime_error("cross_group: grid too large");
+        hipLaunchKernelGGL(k_l21<T>, dim3(static_cast<unsigned>(lb)), dim3(256),
+                           l21_lds_elems() * sizeof(T), s, d, fronts, A, Bp, ar, uinv, nblk);
+        CUPF_HIP(hipGetLastError());
+    }
 }
 
 template <typename T>
@@ -1907,6 +2253,29 @@ void schur_group(const LuDevice& d, const int* fronts, int count, int max_fsz, c
         else
             hipLaunchKernelGGL((k_schur<float, kMfmaF32>), dim3(unsigned(blocks)), dim3(kSchurThreads),
                                shmem, s, d, fronts, A, Bp, ar, uinv, nrb);
+    }
+    CUPF_HIP(hipGetLastError());
+}
+
+template <typename T>
+void schur_tile_group(const LuDevice& d, const int* fronts, const int2* tiles, int ntiles,
+                      int scen_group, const double* A, int Bp, Arenas<T> ar, int dense_mode,
+                      hipStream_t s) {
+    if (ntiles <= 0) return;
+    if (scen_group < 1 || Bp % scen_group || kLanes % scen_group)
+        throw std::runtime_error("schur_tile_group: scenario group must divide 64");
+    const long long blocks = (long long)ntiles * Bp;
+    if (blocks >= (1LL << 31)) throw std::runtime_error("schur_tile_group: grid too large");
+    const dim3 grid(static_cast<unsigned>(blocks));
+    if constexpr (sizeof(T) == 8) {
+        hipLaunchKernelGGL((k_schur_tile<double, kMfmaNone>), grid, dim3(256), tile_lds<double>(), s,
+                           d, fronts, tiles, ntiles, scen_group, A, Bp, ar);
+    } else if (dense_mode == kMfmaBf16x3) {
+        hipLaunchKernelGGL((k_schur_tile<float, kMfmaBf16x3>), grid, dim3(256), tile_lds<float>(), s,
+                           d, fronts, tiles, ntiles, scen_group, A, Bp, ar);
+    } else {
+        hipLaunchKernelGGL((k_schur_tile<float, kMfmaF32>), grid, dim3(256), tile_lds<float>(), s, d,
+                           fronts, tiles, ntiles, scen_group, A, Bp, ar);
     }
     CUPF_HIP(hipGetLastError());
 }
@@ -2051,11 +2420,18 @@ CUPF_LU_INST(float)
 
 #define CUPF_XS_INST(T)                                                                          \
     template void cross_group<T>(const LuDevice&, const int*, int, int, const double*, int,       \
-                                 Arenas<T>, T*, double, int, int*, hipStream_t);                  \
+                                 Arenas<T>, T*, double, int, int*, bool, hipStream_t);            \
     template void schur_group<T>(const LuDevice&, const int*, int, int, const double*, int,       \
                                  Arenas<T>, const T*, int, hipStream_t);
 
 CUPF_XS_INST(double)
 CUPF_XS_INST(float)
+
+#define CUPF_TILE_INST(T)                                                                        \
+    template void schur_tile_group<T>(const LuDevice&, const int*, const int2*, int, int,         \
+                                      const double*, int, Arenas<T>, int, hipStream_t);
+
+CUPF_TILE_INST(double)
+CUPF_TILE_INST(float)
 
 }  // namespace cupf::kern
