@@ -42,6 +42,7 @@
 // ===========================================================================
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <complex>
 #include <cstdint>
@@ -583,6 +584,43 @@ int orc_nr_solve_batch(int n, const int* yp, const int* yi, const double* y, con
         worst = std::max(worst, status[b]);
         std::memcpy(V_out + size_t(b) * 2 * n, s.V.data(), sizeof(cplx) * n);
     }
+    return worst;
+}
+
+// Steady-state timing of the same solves (the cupf_bench.cpp:186-216
+// methodology): the analysis once and one warm-up solve of scenario 0, both
+// untimed, then every scenario solved in turn under steady_clock; seconds
+// gets the timed wall time.
+int orc_nr_time_batch(int n, const int* yp, const int* yi, const double* y, const double* sbus,
+                      const double* V0, int batch, const int* pv, int n_pv, const int* pq,
+                      int n_pq, double tol, int max_iter, double* V_out, int* iters,
+                      double* norm, int* status, double* seconds) {
+    NRState s;
+    nr_setup(s, n, yp, yi, y, pv, n_pv, pq, n_pq);
+    auto seed = [&](int b) {
+        const cplx* v0 = reinterpret_cast<const cplx*>(V0) + size_t(b) * n;
+        for (int i = 0; i < n; ++i) {
+            s.V[i] = v0[i];
+            s.Va[i] = std::arg(v0[i]);
+            s.Vm[i] = std::abs(v0[i]);
+        }
+    };
+    if (batch > 0) {                                  // warm-up
+        int it = 0;
+        double nm = 0.0;
+        seed(0);
+        (void)nr_run(s, reinterpret_cast<const cplx*>(sbus), tol, max_iter, &it, &nm);
+    }
+    int worst = 0;
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int b = 0; b < batch; ++b) {
+        seed(b);
+        const cplx* S = reinterpret_cast<const cplx*>(sbus) + size_t(b) * n;
+        status[b] = nr_run(s, S, tol, max_iter, &iters[b], &norm[b]);
+        worst = std::max(worst, status[b]);
+        std::memcpy(V_out + size_t(b) * 2 * n, s.V.data(), sizeof(cplx) * n);
+    }
+    *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return worst;
 }
 
