@@ -3,6 +3,7 @@
 * libcupf_hip.so  -- HIP kernels for gfx950 + C++ host driver + C ABI
                      (hipcc, one object per source, compiled in parallel)
 * _cupf*.so       -- pybind11 module over the C ABI (g++, no HIP/torch headers)
+* cupf_bench      -- dump-case command-line benchmark over the C ABI (g++)
 
 Both land in gpu-powerflow_amd/cupf/.  Run:  python gpu-powerflow_amd/build.py
 """
@@ -45,6 +46,7 @@ SOURCES = {
     "mlp_kernels.hip": [],
     "warm_start.cpp": [],
     "static_workspace.hip": [],
+    "case_io.cpp": [],
 }
 
 
@@ -94,9 +96,23 @@ def build_pybind_module() -> Path:
     return out
 
 
+def build_cli() -> Path:
+    """cupf_bench: the dump-case command-line benchmark, over the C ABI."""
+    out = PKG / "cupf_bench"
+    tmp = PKG / "cupf_bench.tmp"
+    cxx = shutil.which("g++") or "g++"
+    rocm_lib = Path(HIPCC).resolve().parent.parent / "lib"
+    _run([cxx, "-O2", "-std=c++17", "-Wall", f"-I{ROOT / 'include'}",
+          str(CSRC / "cupf_bench.cpp"), "-o", str(tmp),
+          f"-L{PKG}", "-lcupf_hip", "-Wl,-rpath,$ORIGIN", f"-Wl,-rpath-link,{rocm_lib}"])
+    os.replace(tmp, out)
+    return out
+
+
 def build() -> None:
     build_hip_library()
     build_pybind_module()
+    build_cli()
 
 
 if __name__ == "__main__":

@@ -5,6 +5,7 @@
 
 #include "analyze.hpp"
 #include "batched_lu.hpp"
+#include "case_io.hpp"
 #include "hip_common.hpp"
 #include "kernels.hpp"
 #include "newton_solver.hpp"
@@ -25,6 +26,14 @@ struct cupf_solver {
 
 struct cupf_warm_start {
     cupf::WarmStartMLP* impl;
+};
+
+struct cupf_case {
+    cupf::DumpCase c;
+};
+
+struct cupf_mm {
+    cupf::MatrixMarket m;
 };
 
 // Inner solver behind the phase C ABI: one batched LU, the batch it is set up
@@ -984,6 +993,140 @@ int cupf_lu_factor_solve_static(cupf_lu* h, int32_t batch, const double* values,
         m.lu.solve<double>(A, b, xi, xt, transpose != 0, m.refine, Bp, s);
         cupf::kern::from_interleaved(xi, m.n, batch, Bp, x, m.n, s);
         if (!stream) CUPF_HIP(hipStreamSynchronize(s));
+    });
+}
+
+int cupf_case_load(const char* dir, cupf_case** out) {
+    return guarded([&] {
+        if (!dir || !out) throw std::invalid_argument("cupf_case_load: null argument");
+        *out = nullptr;
+        auto h = std::make_unique<cupf_case>();
+        h->c = cupf::load_dump_case(dir);
+        *out = h.release();
+    });
+}
+
+int cupf_case_sizes(const cupf_case* h, int32_t* out5) {
+    return guarded([&] {
+        if (!h || !out5) throw std::invalid_argument("cupf_case_sizes: null argument");
+        out5[0] = h->c.rows;
+        out5[1] = int32_t(h->c.indices.size());
+        out5[2] = int32_t(h->c.pv.size());
+        out5[3] = int32_t(h->c.pq.size());
+        out5[4] = h->c.vref.empty() ? 0 : 1;
+    });
+}
+
+const char* cupf_case_name(const cupf_case* h) { return h ? h->c.name.c_str() : ""; }
+
+int cupf_case_arrays(const cupf_case* h, int32_t* indptr, int32_t* indices, double* data,
+                     double* sbus, double* v0, int32_t* pv, int32_t* pq, double* vref) {
+    return guarded([&] {
+        if (!h || !indptr || !indices || !data || !sbus || !v0 || (!pv && !h->c.pv.empty()) ||
+            (!pq && !h->c.pq.empty()))
+            throw std::invalid_argument("cupf_case_arrays: null argument");
+        const cupf::DumpCase& c = h->c;
+        std::copy(c.indptr.begin(), c.indptr.end(), indptr);
+        std::copy(c.indices.begin(), c.indices.end(), indices);
+        std::memcpy(data, c.ybus.data(), c.ybus.size() * sizeof(std::complex<double>));
+        std::memcpy(sbus, c.sbus.data(), c.sbus.size() * sizeof(std::complex<double>));
+        std::memcpy(v0, c.v0.data(), c.v0.size() * sizeof(std::complex<double>));
+        std::copy(c.pv.begin(), c.pv.end(), pv);
+        std::copy(c.pq.begin(), c.pq.end(), pq);
+        if (vref) {
+            if (c.vref.empty()) throw std::invalid_argument("cupf_case_arrays: the case has no Vref");
+            std::memcpy(vref, c.vref.data(), c.vref.size() * sizeof(std::complex<double>));
+        }
+    });
+}
+
+int cupf_case_destroy(cupf_case* h) {
+    delete h;
+    return CUPF_OK;
+}
+
+int cupf_case_write(const char* dir, int32_t n_bus, const int32_t* indptr, const int32_t* indices,
+                    const double* data, const double* sbus, const double* v0, const int32_t* pv,
+                    int32_t n_pv, const int32_t* pq, int32_t n_pq, const double* vref) {
+    return guarded([&] {
+        if (!dir || n_bus <= 0 || !indptr || !sbus || !v0 || n_pv < 0 || n_pq < 0 ||
+            (n_pv > 0 && !pv) || (n_pq > 0 && !pq))
+            throw std::invalid_argument("cupf_case_write: bad arguments");
+        const int nnz = indptr[n_bus];
+        if (nnz < 0 || (nnz > 0 && (!indices || !data)))
+            throw std::invalid_argument("cupf_case_write: bad Ybus");
+        cupf::DumpCase c;
+        c.rows = c.cols = n_bus;
+        c.indptr.assign(indptr, indptr + n_bus + 1);
+        c.indices.assign(indices, indices + nnz);
+        const auto* yz = reinterpret_cast<const std::complex<double>*>(data);
+        const auto* sz = reinterpret_cast<const std::complex<double>*>(sbus);
+        const auto* vz = reinterpret_cast<const std::complex<double>*>(v0);
+        c.ybus.assign(yz, yz + nnz);
+        c.sbus.assign(sz, sz + n_bus);
+        c.v0.assign(vz, vz + n_bus);
+        if (vref) {
+            const auto* rz = reinterpret_cast<const std::complex<double>*>(vref);
+            c.vref.assign(rz, rz + n_bus);
+        }
+        c.pv.assign(pv, pv + n_pv);
+        c.pq.assign(pq, pq + n_pq);
+        cupf::write_dump_case(dir, c);
+    });
+}
+
+int cupf_mm_read(const char* path, cupf_mm** out) {
+    return guarded([&] {
+        if (!path || !out) throw std::invalid_argument("cupf_mm_read: null argument");
+        *out = nullptr;
+        auto h = std::make_unique<cupf_mm>();
+        h->m = cupf::read_matrix_market(path);
+        *out = h.release();
+    });
+}
+
+int cupf_mm_info(const cupf_mm* h, int64_t* out5) {
+    return guarded([&] {
+        if (!h || !out5) throw std::invalid_argument("cupf_mm_info: null argument");
+        const cupf::MatrixMarket& m = h->m;
+        out5[0] = m.rows;
+        out5[1] = m.cols;
+        out5[2] = m.is_dense ? int64_t(m.rows) * m.cols : int64_t(m.indices.size());
+        out5[3] = m.is_complex ? 1 : 0;
+        out5[4] = m.is_dense ? 1 : 0;
+    });
+}
+
+int cupf_mm_arrays(const cupf_mm* h, int32_t* indptr, int32_t* indices, double* values) {
+    return guarded([&] {
+        if (!h || !values || (!h->m.is_dense && (!indptr || !indices)))
+            throw std::invalid_argument("cupf_mm_arrays: null argument");
+        const cupf::MatrixMarket& m = h->m;
+        if (!m.is_dense) {
+            std::copy(m.indptr.begin(), m.indptr.end(), indptr);
+            std::copy(m.indices.begin(), m.indices.end(), indices);
+        }
+        std::copy(m.values.begin(), m.values.end(), values);
+    });
+}
+
+int cupf_mm_destroy(cupf_mm* h) {
+    delete h;
+    return CUPF_OK;
+}
+
+int cupf_mm_write_csr(const char* path, int32_t rows, int32_t cols, const int32_t* indptr,
+                      const int32_t* indices, const double* values, int32_t is_complex) {
+    return guarded([&] {
+        if (!path) throw std::invalid_argument("cupf_mm_write_csr: null path");
+        cupf::write_matrix_market_csr(path, rows, cols, indptr, indices, values, is_complex != 0);
+    });
+}
+
+int cupf_mm_write_dense(const char* path, int32_t rows, int32_t cols, const double* values) {
+    return guarded([&] {
+        if (!path) throw std::invalid_argument("cupf_mm_write_dense: null path");
+        cupf::write_matrix_market_dense(path, rows, cols, values);
     });
 }
 

@@ -15,8 +15,10 @@
 #include <algorithm>
 #include <complex>
 #include <cstring>
+#include <memory>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 namespace py = pybind11;
 
@@ -747,4 +749,161 @@ PYBIND11_MODULE(_cupf, m) {
         .def_property_readonly("last_batch", [](const PySolver& s) { return s.last_batch; })
         .def_property_readonly("handle",
                                [](const PySolver& s) { return reinterpret_cast<uintptr_t>(s.h()); });
+
+    // ---- case and matrix files (cuPF/tests/dump_case_loader.cpp:92-231,
+    // benchmark/common/matpower_data.py:350-358) -----------------------------
+    m.def(
+        "load_dump_case",
+        [](const std::string& dir) {
+            cupf_case* c = nullptr;
+            check(cupf_case_load(dir.c_str(), &c));
+            std::unique_ptr<cupf_case, int (*)(cupf_case*)> guard(c, cupf_case_destroy);
+            int32_t sz[5];
+            check(cupf_case_sizes(c, sz));
+            const int n = sz[0], nnz = sz[1];
+            py::array_t<int32_t> indptr(n + 1), indices(nnz), pv(sz[2]), pq(sz[3]);
+            py::array_t<std::complex<double>> data(nnz), sbus(n), v0(n);
+            py::object vref = py::none();
+            py::array_t<std::complex<double>> vr;
+            if (sz[4]) vr = py::array_t<std::complex<double>>(n);
+            check(cupf_case_arrays(c, indptr.mutable_data(), indices.mutable_data(),
+                                   reinterpret_cast<double*>(data.mutable_data()),
+                                   reinterpret_cast<double*>(sbus.mutable_data()),
+                                   reinterpret_cast<double*>(v0.mutable_data()), pv.mutable_data(),
+                                   pq.mutable_data(),
+                                   sz[4] ? reinterpret_cast<double*>(vr.mutable_data()) : nullptr));
+            if (sz[4]) vref = vr;
+            py::dict d;
+            d["case_name"] = std::string(cupf_case_name(c));
+            d["rows"] = n;
+            d["cols"] = n;
+            d["indptr"] = indptr;
+            d["indices"] = indices;
+            d["data"] = data;
+            d["sbus"] = sbus;
+            d["v0"] = v0;
+            d["pv"] = pv;
+            d["pq"] = pq;
+            d["vref"] = vref;
+            return d;
+        },
+        py::arg("case_dir"),
+        "Load a dump-case directory (dump_Ybus.mtx, dump_Sbus.txt, dump_V.txt, dump_pv.txt, "
+        "dump_pq.txt, optional dump_Vref.txt).");
+    m.def(
+        "save_dump_case",
+        [](const std::string& dir, py::array_t<int32_t, py::array::c_style | py::array::forcecast> indptr,
+           py::array_t<int32_t, py::array::c_style | py::array::forcecast> indices,
+           py::array_t<std::complex<double>, py::array::c_style | py::array::forcecast> data,
+           py::array_t<std::complex<double>, py::array::c_style | py::array::forcecast> sbus,
+           py::array_t<std::complex<double>, py::array::c_style | py::array::forcecast> v0,
+           py::array_t<int32_t, py::array::c_style | py::array::forcecast> pv,
+           py::array_t<int32_t, py::array::c_style | py::array::forcecast> pq, py::object vref) {
+            const int n = int(indptr.size()) - 1;
+            if (n <= 0 || sbus.size() != n || v0.size() != n || indices.size() != data.size() ||
+                (n >= 0 && indptr.data()[n] != indices.size()))
+                throw std::invalid_argument("save_dump_case: inconsistent arrays");
+            py::array_t<std::complex<double>, py::array::c_style | py::array::forcecast> vr;
+            const double* vp = nullptr;
+            if (!vref.is_none()) {
+                vr = vref.cast<py::array_t<std::complex<double>, py::array::c_style | py::array::forcecast>>();
+                if (vr.size() != n) throw std::invalid_argument("save_dump_case: vref size");
+                vp = reinterpret_cast<const double*>(vr.data());
+            }
+            check(cupf_case_write(dir.c_str(), n, indptr.data(), indices.data(),
+                                  reinterpret_cast<const double*>(data.data()),
+                                  reinterpret_cast<const double*>(sbus.data()),
+                                  reinterpret_cast<const double*>(v0.data()), pv.data(),
+                                  int32_t(pv.size()), pq.data(), int32_t(pq.size()), vp));
+        },
+        py::arg("case_dir"), py::arg("indptr"), py::arg("indices"), py::arg("data"),
+        py::arg("sbus"), py::arg("v0"), py::arg("pv"), py::arg("pq"), py::arg("vref") = py::none(),
+        "Write a dump-case directory in the format load_dump_case reads.");
+    m.def(
+        "read_matrix_market",
+        [](const std::string& path) {
+            cupf_mm* h = nullptr;
+            check(cupf_mm_read(path.c_str(), &h));
+            std::unique_ptr<cupf_mm, int (*)(cupf_mm*)> guard(h, cupf_mm_destroy);
+            int64_t info[5];
+            check(cupf_mm_info(h, info));
+            const int64_t rows = info[0], cols = info[1], nnz = info[2];
+            const bool cplx = info[3] != 0, dense = info[4] != 0;
+            py::dict d;
+            d["rows"] = rows;
+            d["cols"] = cols;
+            if (dense) {
+                // column-major file order -> [rows, cols] array
+                std::vector<double> v(size_t(nnz) * (cplx ? 2 : 1));
+                check(cupf_mm_arrays(h, nullptr, nullptr, v.data()));
+                if (cplx) {
+                    py::array_t<std::complex<double>> a({rows, cols});
+                    auto* o = a.mutable_data();
+                    for (int64_t c = 0; c < cols; ++c)
+                        for (int64_t r = 0; r < rows; ++r)
+                            o[r * cols + c] = {v[2 * (c * rows + r)], v[2 * (c * rows + r) + 1]};
+                    d["values"] = a;
+                } else {
+                    py::array_t<double> a({rows, cols});
+                    auto* o = a.mutable_data();
+                    for (int64_t c = 0; c < cols; ++c)
+                        for (int64_t r = 0; r < rows; ++r) o[r * cols + c] = v[c * rows + r];
+                    d["values"] = a;
+                }
+                d["dense"] = true;
+                return d;
+            }
+            py::array_t<int32_t> indptr(rows + 1), indices(nnz);
+            if (cplx) {
+                py::array_t<std::complex<double>> vals(nnz);
+                check(cupf_mm_arrays(h, indptr.mutable_data(), indices.mutable_data(),
+                                     reinterpret_cast<double*>(vals.mutable_data())));
+                d["values"] = vals;
+            } else {
+                py::array_t<double> vals(nnz);
+                check(cupf_mm_arrays(h, indptr.mutable_data(), indices.mutable_data(),
+                                     vals.mutable_data()));
+                d["values"] = vals;
+            }
+            d["indptr"] = indptr;
+            d["indices"] = indices;
+            d["dense"] = false;
+            return d;
+        },
+        py::arg("path"),
+        "Read a MatrixMarket file: coordinate -> dict(rows, cols, indptr, indices, values, "
+        "dense=False) in CSR; array -> dict(rows, cols, values [rows, cols], dense=True).");
+    m.def(
+        "write_matrix_market",
+        [](const std::string& path, py::object a, py::object indptr, py::object indices,
+           py::object shape) {
+            if (indptr.is_none()) {
+                auto d = a.cast<py::array_t<double, py::array::f_style | py::array::forcecast>>();
+                if (d.ndim() != 1 && d.ndim() != 2)
+                    throw std::invalid_argument("write_matrix_market: dense values must be 1-D or 2-D");
+                const int rows = int(d.shape(0)), cols = d.ndim() == 2 ? int(d.shape(1)) : 1;
+                check(cupf_mm_write_dense(path.c_str(), rows, cols, d.data()));
+                return;
+            }
+            auto ip = indptr.cast<py::array_t<int32_t, py::array::c_style | py::array::forcecast>>();
+            auto ix = indices.cast<py::array_t<int32_t, py::array::c_style | py::array::forcecast>>();
+            auto sh = shape.cast<std::pair<int, int>>();
+            if (int(ip.size()) != sh.first + 1) throw std::invalid_argument("write_matrix_market: indptr size");
+            py::array arr = a.cast<py::array>();
+            if (arr.dtype().kind() == 'c') {
+                auto v = a.cast<py::array_t<std::complex<double>, py::array::c_style | py::array::forcecast>>();
+                if (v.size() != ix.size()) throw std::invalid_argument("write_matrix_market: values size");
+                check(cupf_mm_write_csr(path.c_str(), sh.first, sh.second, ip.data(), ix.data(),
+                                        reinterpret_cast<const double*>(v.data()), 1));
+            } else {
+                auto v = a.cast<py::array_t<double, py::array::c_style | py::array::forcecast>>();
+                if (v.size() != ix.size()) throw std::invalid_argument("write_matrix_market: values size");
+                check(cupf_mm_write_csr(path.c_str(), sh.first, sh.second, ip.data(), ix.data(),
+                                        v.data(), 0));
+            }
+        },
+        py::arg("path"), py::arg("values"), py::arg("indptr") = py::none(),
+        py::arg("indices") = py::none(), py::arg("shape") = py::none(),
+        "Write CSR (values, indptr, indices, shape) as coordinate general, or a dense "
+        "1-D / 2-D array as array real general.");
 }

@@ -52,6 +52,10 @@ NRResult = _cupf.NRResult
 NRBatchResult = _cupf.NRBatchResult
 NewtonSolver = _cupf.NewtonSolver
 WarmStartMLP = _cupf.WarmStartMLP
+load_dump_case = _cupf.load_dump_case
+save_dump_case = _cupf.save_dump_case
+read_matrix_market = _cupf.read_matrix_market
+write_matrix_market = _cupf.write_matrix_market
 
 LIBRARY_PATH = _PKG_DIR / "libcupf_hip.so"
 
@@ -61,6 +65,7 @@ __all__ = [
     "CustomSolverConfig", "AdjointCacheMode", "CuDSSAlgorithm", "CuDSSOptions",
     "AdjointOptions", "AdjointResult", "NRConfig", "SolveOptions", "NewtonOptions",
     "NRResult", "NRBatchResult", "NewtonSolver", "WarmStartMLP", "LIBRARY_PATH",
+    "load_dump_case", "save_dump_case", "read_matrix_market", "write_matrix_market",
 ]
 
 try:

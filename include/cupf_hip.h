@@ -277,6 +277,46 @@ int cupf_plan_figures(int32_t n, const int32_t* row_ptr, const int32_t* col_idx,
 int cupf_plan_maps_check(int32_t n, const int32_t* row_ptr, const int32_t* col_idx,
                          const int32_t* perm, int32_t cap, int64_t* out8);
 
+/* ---- Case and matrix files (no GPU needed) -----------------------------
+ * Dump-case directories as the reference's C++ bench reads them
+ * (cuPF/tests/dump_case_loader.cpp:92-231: dump_Ybus.mtx MatrixMarket
+ * coordinate complex, dump_Sbus.txt / dump_V.txt "re im" lines, dump_pv.txt /
+ * dump_pq.txt 0-based bus indices, optional dump_Vref.txt) and as its Python
+ * tooling writes them (benchmark/common/matpower_data.py:350-358), and the
+ * MatrixMarket files of its linear-system datasets (J.mtx coordinate real,
+ * F.mtx array real; custom_linear_solver/tests/datasets/README.md:1-20).
+ * Complex arrays are interleaved (re, im) pairs. */
+typedef struct cupf_case cupf_case;
+int cupf_case_load(const char* dir, cupf_case** out);
+/* out5 = {n_bus, Ybus nnz, n_pv, n_pq, has_vref} */
+int cupf_case_sizes(const cupf_case* c, int32_t* out5);
+/* The directory's name (valid until cupf_case_destroy). */
+const char* cupf_case_name(const cupf_case* c);
+/* indptr [n+1], indices [nnz], data [nnz] complex, sbus / v0 [n] complex,
+ * pv [n_pv], pq [n_pq], vref [n] complex (NULL to skip). */
+int cupf_case_arrays(const cupf_case* c, int32_t* indptr, int32_t* indices, double* data,
+                     double* sbus, double* v0, int32_t* pv, int32_t* pq, double* vref);
+int cupf_case_destroy(cupf_case* c);
+/* Writes a dump-case directory (created if missing); vref may be NULL. */
+int cupf_case_write(const char* dir, int32_t n_bus, const int32_t* indptr, const int32_t* indices,
+                    const double* data, const double* sbus, const double* v0, const int32_t* pv,
+                    int32_t n_pv, const int32_t* pq, int32_t n_pq, const double* vref);
+
+typedef struct cupf_mm cupf_mm;
+/* Reads a MatrixMarket matrix: coordinate (real, integer or complex;
+ * general or symmetric) into CSR with sorted columns and summed duplicates,
+ * or array (dense, column-major). */
+int cupf_mm_read(const char* path, cupf_mm** out);
+/* out5 = {rows, cols, stored entries (rows*cols for an array), is_complex, is_dense} */
+int cupf_mm_info(const cupf_mm* m, int64_t* out5);
+/* coordinate: indptr [rows+1], indices [nnz], values [nnz] (x2 complex);
+ * array: values only (indptr, indices may be NULL). */
+int cupf_mm_arrays(const cupf_mm* m, int32_t* indptr, int32_t* indices, double* values);
+int cupf_mm_destroy(cupf_mm* m);
+int cupf_mm_write_csr(const char* path, int32_t rows, int32_t cols, const int32_t* indptr,
+                      const int32_t* indices, const double* values, int32_t is_complex);
+int cupf_mm_write_dense(const char* path, int32_t rows, int32_t cols, const double* values);
+
 /* ---- Inner sparse solver: phase API ------------------------------------
  * The batched multifrontal LU of one square CSR matrix for `batch` value sets
  * at once, on device pointers and a hipStream_t.  Replaces the phase API of

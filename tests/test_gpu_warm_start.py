@@ -1,6 +1,7 @@
 """GPU tests of the learned warm start, called through the native module.
 
 * the BF16 MFMA forward pass and the backward pass against the CPU oracle
+  (gradients also at the full 4,000 -> 3 x 512 -> 4,000 widths)
   (oracle/mlp_oracle.py, bfloat16 rounding at the same points; the remaining
   difference is FP32-vs-FP64 accumulation order): relative error <= 2e-2 in
   norm per output / per layer gradient;
@@ -11,10 +12,17 @@
   scenario converges, and the solutions agree with a tightly converged FP64
   oracle to 1e-9;
 * a deliberately useless network: every scenario falls back to its own V0
-  and the result equals the plain solve.
+  and the result equals the plain solve;
+* the learned-warm-start workload on the 2,000-bus grid: networks (plain MLP
+  4,000 -> 3 x 512 -> 4,000, and with the linear skip path) trained on 65,536
+  of the solver's solutions, evaluated on 4,096 held-out scenarios (fewer
+  NR iterations than the flat start, <= 1 % fallbacks, Va / Vm within 1e-9 of
+  the oracle at 1e-13).
 """
 
 from __future__ import annotations
+
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 import pytest
@@ -64,7 +72,8 @@ def test_forward_matches_oracle(gpu, cupf, n_bus, hidden, depth, count, skip):
 
 @pytest.mark.parametrize("n_bus,hidden,depth,count,skip",
                          [(12, 16, 2, 16, False), (50, 64, 3, 32, False), (12, 16, 2, 16, True),
-                          (50, 64, 3, 32, True)])
+                          (50, 64, 3, 32, True), (2000, 512, 3, 16, False),
+                          (2000, 512, 3, 16, True)])
 def test_grads_match_oracle(gpu, cupf, n_bus, hidden, depth, count, skip):
     rng = np.random.default_rng(100 + n_bus)
     pv = np.arange(1, 3, dtype=np.int32)
@@ -199,3 +208,77 @@ def test_fallback_with_useless_network(gpu, cupf, syn, g118):
     s.set_warm_start(None)
     r2 = s.solve_batch(g.indptr, g.indices, g.data, g.n_bus, g.n_bus, Se, V0e, g.pv, g.pq)
     np.testing.assert_array_equal(r2.iterations_numpy, plain.iterations_numpy)
+
+
+@pytest.fixture(scope="module")
+def case2000_networks(gpu, cupf, syn, g2000):
+    """The learned warm start of the 2,000-bus workload: 65,536 of the
+    solver's own solutions at load factor U(0.6, 1.4), seed 2, and two
+    networks trained on them (60 epochs), the plain MLP and the one with the
+    linear skip path."""
+    torch = gpu
+    g = g2000
+    s = mixed_solver(cupf, g)
+    print("[warm-start case2000] solving 65536 training scenarios", flush=True)
+    _, _, St, V0t = make_data(syn, g, 65536, 0.6, 1.4, 2, torch)
+    nets = {}
+    V = None
+    for skip in (False, True):
+        mlp, info = cupf.train_warm_start(s, St, V0t, g.pv, g.pq, hidden=512, depth=3, epochs=60,
+                                          batch=1024, lr=1e-3, weight_decay=1e-4, seed=0,
+                                          skip=skip, solutions=V)
+        assert info["samples"] == 65536
+        V = info["V"]
+        nets[skip] = (mlp, info)
+        print(f"[warm-start case2000] skip={skip}: loss {info['epoch_loss'][0]:.3g} -> "
+              f"{info['epoch_loss'][-1]:.3g}", flush=True)
+    return nets
+
+
+def mixed_solver(cupf, g):
+    o = cupf.NewtonOptions()
+    o.custom.max_panel_width = 64
+    s = cupf.NewtonSolver(o)
+    s.initialize(g.indptr, g.indices, g.data, g.n_bus, g.n_bus, g.pv, g.pq)
+    return s
+
+
+@pytest.mark.parametrize("skip", [False, True])
+def test_learned_warm_start_case2000_heldout(gpu, cupf, syn, oracle, g2000, case2000_networks,
+                                             skip):
+    """The learned-warm-start workload: the trained network on 4,096 held-out
+    scenarios (U(0.6, 1.4), seed 3), NR to 1e-8 with flat-start fallback:
+    every scenario converges, the mean iteration count is below the flat
+    start's, at most 1 % of the scenarios fall back, and Va / Vm agree with
+    the oracle converged to 1e-13 within 1e-9 on 8 scenarios."""
+    torch = gpu
+    g = g2000
+    mlp, info = case2000_networks[skip]
+    assert info["epoch_loss"][-1] < 0.5 * info["epoch_loss"][0]
+    B = 4096
+    print(f"[warm-start case2000] held-out x{B} skip={skip}", flush=True)
+    Se, V0e, Set, V0et = make_data(syn, g, B, 0.6, 1.4, 3, torch)
+    s = mixed_solver(cupf, g)
+    Vf = torch.empty_like(V0et)
+    flat = s.solve_batch_device(Set.data_ptr(), g.n_bus, V0et.data_ptr(), g.n_bus, B,
+                                cupf.NRConfig(), v_out_ptr=Vf.data_ptr())
+    s.set_warm_start(mlp, 4)
+    Vw = torch.empty_like(V0et)
+    warm = s.solve_batch_device(Set.data_ptr(), g.n_bus, V0et.data_ptr(), g.n_bus, B,
+                                cupf.NRConfig(), v_out_ptr=Vw.data_ptr())
+    st = s.stats()
+    assert flat.converged_numpy.all() and warm.converged_numpy.all()
+    assert warm.iterations_numpy.mean() < flat.iterations_numpy.mean(), (
+        warm.iterations_numpy.mean(), flat.iterations_numpy.mean())
+    assert st["last_warm_fallbacks"] <= B // 100
+    k = 8
+
+    def one(i):
+        return oracle.nr_solve_batch(g.n_bus, g.indptr, g.indices, g.data, Se[i:i + 1],
+                                     V0e[i:i + 1], g.pv, g.pq, tol=1e-13)[0][0]
+
+    with ThreadPoolExecutor(max_workers=k) as ex:
+        Vo = np.array(list(ex.map(one, range(k))))
+    V = Vw[:k].cpu().numpy()
+    assert np.abs(np.angle(V) - np.angle(Vo)).max() <= 1e-9
+    assert np.abs(np.abs(V) - np.abs(Vo)).max() <= 1e-9
