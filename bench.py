@@ -276,7 +276,9 @@ def cpu_baseline(po, grid, S, V0, per_core, one_core_n):
 def case118(torch, cupf, syn, po, args):
     """One scenario at a time on the 118-bus grid (GPU solve through the
     reference's single-scenario solve() with host arrays, and the device-
-    resident batch entry at batch 1) against the oracle on one core."""
+    resident batch entry at batch 1, eager and on the captured Newton step;
+    batch 1 runs the single-system LU kernels) against the oracle on one
+    core."""
     g = syn.workload_grid("case118")
     n = g.n_bus
     N = 64
@@ -311,10 +313,23 @@ def case118(torch, cupf, syn, po, args):
                                   v_out_ptr=Vd[i].data_ptr())
         torch.cuda.synchronize()
         td = (time.perf_counter() - t0) / N
-        out[f] = {"ms_per_solve_host_arrays": th * 1e3, "ms_per_solve_device": td * 1e3,
-                  "solves_per_s_device": 1.0 / td,
-                  "error_vs_oracle_1e-13": va_vm_err(np.array(Vh[:8]), Vo)}
         del sv
+        # the same on the captured Newton step (use_cuda_graph)
+        sg = make_solver(cupf, g, comp, prec, 8, ref, graph=True)
+        for i in range(3):
+            sg.solve_batch_device(dS[i].data_ptr(), n, dV0[i].data_ptr(), n, 1, cfg,
+                                  v_out_ptr=Vd[i].data_ptr())
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(N):
+            sg.solve_batch_device(dS[i].data_ptr(), n, dV0[i].data_ptr(), n, 1, cfg,
+                                  v_out_ptr=Vd[i].data_ptr())
+        torch.cuda.synchronize()
+        tg = (time.perf_counter() - t0) / N
+        del sg
+        out[f] = {"ms_per_solve_host_arrays": th * 1e3, "ms_per_solve_device": td * 1e3,
+                  "solves_per_s_device": 1.0 / td, "ms_per_solve_device_graph": tg * 1e3,
+                  "error_vs_oracle_1e-13": va_vm_err(np.array(Vh[:8]), Vo)}
     sec, it, nm, st = po.nr_time_batch(n, g.indptr, g.indices, g.data, S, V0, g.pv, g.pq)
     out["cpu_oracle_one_core"] = {"ms_per_solve": sec / N * 1e3, "solves_per_s": N / sec,
                                   "iterations": int(it.max())}

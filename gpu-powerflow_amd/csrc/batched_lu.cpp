@@ -141,6 +141,23 @@ void BatchedLU::analyze(int n, const int* rp, const int* ci, const std::vector<i
         }
         tile_ptr_[size_t(P.num_levels)] = int(tiles.size());
         d_tiles = upload_vector(tiles, s);
+        // single-system arena (fronts in plan order: children first)
+        std::vector<long long> soff(size_t(P.num_fronts), 0);
+        long long tot = 0;
+        for (int p = 0; p < P.num_fronts; ++p) {
+            soff[size_t(p)] = tot;
+            tot += (long long)P.fsz[p] * P.fsz[p];
+        }
+        single_total_ = tot;
+        single_fits_ = kern::single_lds_bytes(tot, n, int(front_bytes())) <= kern::kSingleLdsMax;
+        single_ = false;
+        if (single_fits_) {
+            d_soff = upload_vector(soff, s);
+            single_store_.allocate(size_t(tot) * front_bytes());
+        } else {
+            d_soff.release();
+            single_store_.release();
+        }
     }
     CUPF_HIP(hipStreamSynchronize(s));
 
@@ -431,6 +448,16 @@ void BatchedLU::factor_t(const double* A, int Bp, hipStream_t s, StageTimer* tim
 
 void BatchedLU::factor(const double* A, int Bp, hipStream_t s, StageTimer* timer, int stage) {
     if (Bp > memBp_) throw std::runtime_error("BatchedLU::factor(): batch exceeds the reserved size");
+    if (single_ && Bp == kern::kLanes) {
+        const int sp = opt_.static_pivot ? 1 : 0;
+        if (opt_.front64)
+            kern::single_factor<double>(lu_, d_soff.as<long long>(), single_total_, A, Bp,
+                                        single_store_.as<double>(), opt_.pivot_eps, sp, mem_.sing, s);
+        else
+            kern::single_factor<float>(lu_, d_soff.as<long long>(), single_total_, A, Bp,
+                                       single_store_.as<float>(), opt_.pivot_eps, sp, mem_.sing, s);
+        return;
+    }
     if (opt_.front64) factor_t<double>(A, Bp, s, timer, stage);
     else factor_t<float>(A, Bp, s, timer, stage);
 }
@@ -469,6 +496,15 @@ void BatchedLU::solve_once(const double* rhs, TX* x, bool transpose, int Bp, hip
 
 template <typename TX>
 void BatchedLU::solve_dispatch(const double* rhs, TX* x, bool transpose, int Bp, hipStream_t s) {
+    if (single_ && Bp == kern::kLanes) {
+        if (opt_.front64)
+            kern::single_solve<double, TX>(lu_, d_soff.as<long long>(), single_total_,
+                                           single_store_.as<double>(), rhs, Bp, x, transpose, s);
+        else
+            kern::single_solve<float, TX>(lu_, d_soff.as<long long>(), single_total_,
+                                          single_store_.as<float>(), rhs, Bp, x, transpose, s);
+        return;
+    }
     if (opt_.front64) solve_once<double, TX>(rhs, x, transpose, Bp, s);
     else solve_once<float, TX>(rhs, x, transpose, Bp, s);
 }

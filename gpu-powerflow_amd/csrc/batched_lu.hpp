@@ -69,6 +69,12 @@ public:
     void use_memory(const LuBatchMem& mem, int Bp);
     void use_reserved();
 
+    // Batch-1 mode: with on and the whole factorisation fitting one wave's
+    // LDS, factor() and solve() at Bp == 64 run the single-system kernels on
+    // scenario 0 (solutions written to every lane).
+    void set_single(bool on) { single_ = on && single_fits_; }
+    bool single() const { return single_; }
+
     // Clear the per-scenario singular-pivot flags (a fill kernel).
     void clear_singular(int Bp, hipStream_t s);
     const int* singular() const { return mem_.sing; }
@@ -135,6 +141,10 @@ private:
     // contribution-block tiles of the tile-path levels: level L's at
     // tile_ptr_[L] .. tile_ptr_[L + 1] of d_tiles
     std::vector<int> tile_ptr_;
+    // single-system arena: front p at d_soff[p], single_total_ elements
+    DeviceBuffer d_soff, single_store_;
+    long long single_total_ = 0;
+    bool single_fits_ = false, single_ = false;
     // pattern and its transpose with value map (refinement residuals)
     DeviceBuffer d_rp, d_ci, d_trp, d_tci, d_tmap;
     kern::LuDevice lu_;

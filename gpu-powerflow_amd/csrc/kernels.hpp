@@ -185,6 +185,20 @@ void schur_tile_group(const LuDevice& d, const int* fronts, const int2* tiles, i
                       int scen_group, const double* A, int Bp, Arenas<T> ar, int dense_mode,
                       hipStream_t s);
 
+// One system (the padded batch is one wave and scenario 0 is the system):
+// the whole factorisation by one wave with every front in LDS (fronts at
+// soff[p], total elements), the factored fronts copied to store; the (plain
+// or transposed) solve from store, x written to every lane of the batch.
+// LDS need: single_lds_bytes(total, n, sizeof T) <= kSingleLdsMax.
+constexpr size_t kSingleLdsMax = 159 * 1024;
+size_t single_lds_bytes(long long total, int n, int elem);
+template <typename T>
+void single_factor(const LuDevice& d, const long long* soff, long long total, const double* A,
+                   int Bp, T* store, double pivot_eps, int static_piv, int* sing, hipStream_t s);
+template <typename T, typename TX>
+void single_solve(const LuDevice& d, const long long* soff, long long total, const T* store,
+                  const double* rhs, int Bp, TX* x, bool transpose, hipStream_t s);
+
 // Solve sweeps.  rhs (FP64) and x are [n][Bp]; y [n][Bp] and upd are work
 // vectors, upd holding the tier-0 fronts' update vectors as [upd_total][Bp]
 // followed by the tier-1/2 fronts' as [Bp][upd_total].  gather: y <- P rhs

@@ -2156,6 +2156,163 @@ __global__ __launch_bounds__(256) void k_scatter(const int* __restrict__ perm, i
     x[(long long)perm[k] * Bp + b] = TX(y[(long long)k * Bp + b]);
 }
 
+// ===========================================================================
+// One system (batch 1): the whole factorisation, and each solve, by one wave
+// with every front resident in LDS at once (fronts at soff[p] in a compact
+// single-system arena, children before parents).  Launch latency, not
+// arithmetic, bounds a single small system: the batched sweep issues a
+// launch per (level, tier) group, these one launch each.
+//
+//   k_single_factor: per front in postorder: zero, original entries, the
+//     children's contribution blocks (extend-add), then the partial LU of
+//     its K pivots (guarded pivots, unit L below, U on and above), one front
+//     row per lane; the factored fronts are copied to `store` for the solves.
+//   k_single_solve: fronts from `store` into LDS, y = P rhs, then
+//     plain:      L forward (fronts in postorder, each pivot's column
+//                 scattered to the rows below), U backward (reverse
+//                 postorder, each pivot row's dot product reduced over the
+//                 wave);
+//     transposed: U^T forward (each pivot row scattered), L^T backward (each
+//                 pivot column's dot product reduced);
+//     x[perm[k]] = y[k], written to every lane of the padded batch.
+// Lanes that read what other lanes of the wave wrote do so after
+// wave_sync() (a wavefront fence: the wave's LDS accesses are performed in
+// order, and the fence keeps the compiler from reordering across it).
+// ===========================================================================
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kLanes) void k_single_factor(LuDevice d,
+                                                          const long long* __restrict__ soff,
+                                                          long long total,
+                                                          const double* __restrict__ A, int Bp,
+                                                          T* __restrict__ store, double eps,
+                                                          int static_piv, int* __restrict__ sing) {
+    extern __shared__ unsigned char smem_raw[];
+    T* F = reinterpret_cast<T*>(smem_raw);                        // [total]
+    const int lane = threadIdx.x;
+    int flag = 0;
+    for (int p = 0; p < d.P; ++p) {
+        const int fs = d.fsz[p], K = d.nc[p];
+        T* G = F + soff[p];
+        for (int e = lane; e < fs * fs; e += kLanes) G[e] = T(0);
+        wave_sync();
+        for (int e = d.ent_ptr[p] + lane; e < d.ent_ptr[p + 1]; e += kLanes)
+            G[d.ent_dst[e]] = T(A[(long long)d.ent_src[e] * Bp]);      // scenario 0
+        wave_sync();
+        for (int q = d.child_ptr[p]; q < d.child_ptr[p + 1]; ++q) {
+            const int c = d.child_idx[q];
+            const int cfs = d.fsz[c], cnc = d.nc[c], cuc = cfs - cnc;
+            const int* loc = d.asm_loc + d.asm_ptr[c];
+            const T* C = F + soff[c] + (long long)cnc * cfs + cnc;
+            // one contribution row per lane: distinct landing rows
+            for (int a = lane; a < cuc; a += kLanes) {
+                T* Gr = G + loc[a] * fs;
+                const T* Cr = C + a * cfs;
+                for (int bb = 0; bb < cuc; ++bb) Gr[loc[bb]] += Cr[bb];
+            }
+            wave_sync();                                          // next child may share elements
+        }
+        for (int k = 0; k < K; ++k) {
+            const T piv = G[k * fs + k];
+            const T g = guard_pivot(piv, eps, static_piv, flag);
+            const int m = fs - k - 1;
+            const T* U = G + k * fs + k + 1;                      // pivot row, columns > k
+            for (int i = lane; i < m; i += kLanes) {              // one row below the pivot per lane
+                T* Gr = G + (k + 1 + i) * fs;
+                const T l = Gr[k] / g;
+                Gr[k] = l;
+                for (int j = 0; j < m; ++j) Gr[k + 1 + j] -= l * U[j];
+            }
+            if (lane == 0 && g != piv) G[k * fs + k] = g;
+            wave_sync();
+        }
+    }
+    for (long long e = lane; e < total; e += kLanes) store[e] = F[e];
+    if (flag) atomicOr(sing, 1);
+}
+
+template <typename T, bool TRANS, typename TX>
+__global__ __launch_bounds__(kLanes) void k_single_solve(LuDevice d,
+                                                         const long long* __restrict__ soff,
+                                                         long long total,
+                                                         const T* __restrict__ store,
+                                                         const double* __restrict__ rhs, int Bp,
+                                                         TX* __restrict__ x) {
+    extern __shared__ unsigned char smem_raw[];
+    T* F = reinterpret_cast<T*>(smem_raw);                        // [total]
+    T* y = F + total;                                             // [n]
+    const int lane = threadIdx.x;
+    for (long long e = lane; e < total; e += kLanes) F[e] = store[e];
+    for (int k = lane; k < d.n; k += kLanes) y[k] = T(rhs[(long long)d.perm[k] * Bp]);
+    wave_sync();
+    if constexpr (!TRANS) {
+        for (int p = 0; p < d.P; ++p) {                           // L y = P rhs
+            const int fs = d.fsz[p], K = d.nc[p];
+            const T* G = F + soff[p];
+            const int* rows = d.front_rows + d.front_ptr[p];
+            for (int k = 0; k < K; ++k) {
+                const T yk = y[rows[k]];
+                for (int i = k + 1 + lane; i < fs; i += kLanes) y[rows[i]] -= G[i * fs + k] * yk;
+                wave_sync();
+            }
+        }
+        for (int p = d.P - 1; p >= 0; --p) {                      // U x = y
+            const int fs = d.fsz[p], K = d.nc[p];
+            const T* G = F + soff[p];
+            const int* rows = d.front_rows + d.front_ptr[p];
+            for (int k = K - 1; k >= 0; --k) {
+                T sum = T(0);
+                for (int j = k + 1 + lane; j < fs; j += kLanes) sum += G[k * fs + j] * y[rows[j]];
+                sum = wave_sum(sum);
+                const T v = (y[rows[k]] - sum) / G[k * fs + k];
+                wave_sync();
+                if (lane == 0) y[rows[k]] = v;
+                wave_sync();
+            }
+        }
+    } else {
+        for (int p = 0; p < d.P; ++p) {                           // U^T z = P rhs
+            const int fs = d.fsz[p], K = d.nc[p];
+            const T* G = F + soff[p];
+            const int* rows = d.front_rows + d.front_ptr[p];
+            for (int k = 0; k < K; ++k) {
+                const T zk = y[rows[k]] / G[k * fs + k];
+                wave_sync();
+                if (lane == 0) y[rows[k]] = zk;
+                for (int j = k + 1 + lane; j < fs; j += kLanes) y[rows[j]] -= G[k * fs + j] * zk;
+                wave_sync();
+            }
+        }
+        for (int p = d.P - 1; p >= 0; --p) {                      // L^T x = z
+            const int fs = d.fsz[p], K = d.nc[p];
+            const T* G = F + soff[p];
+            const int* rows = d.front_rows + d.front_ptr[p];
+            for (int k = K - 1; k >= 0; --k) {
+                T sum = T(0);
+                for (int i = k + 1 + lane; i < fs; i += kLanes) sum += G[i * fs + k] * y[rows[i]];
+                sum = wave_sum(sum);
+                const T v = y[rows[k]] - sum;
+                wave_sync();
+                if (lane == 0) y[rows[k]] = v;
+                wave_sync();
+            }
+        }
+    }
+    for (int k = 0; k < d.n; ++k) x[(long long)d.perm[k] * Bp + lane] = TX(y[k]);
+}
+
 }  // namespace
 
 int assemble_rows_block(int max_fsz, int elem) {
@@ -2402,6 +2559,60 @@ void lu_solve_scatter(const LuDevice& d, const T* y, int Bp, TX* x, hipStream_t 
     hipLaunchKernelGGL((k_scatter<T, TX>), grid, dim3(256), 0, s, d.perm, d.n, y, Bp, x);
     CUPF_HIP(hipGetLastError());
 }
+
+size_t single_lds_bytes(long long total, int n, int elem) {
+    return size_t(total + n) * size_t(elem);
+}
+
+template <typename T>
+void single_factor(const LuDevice& d, const long long* soff, long long total, const double* A,
+                   int Bp, T* store, double pivot_eps, int static_piv, int* sing, hipStream_t s) {
+    const size_t lds = single_lds_bytes(total, 0, int(sizeof(T)));
+    if (lds > kSingleLdsMax) throw std::runtime_error("single_factor: fronts exceed the LDS");
+    static bool attr_set = false;
+    if (!attr_set) {
+        CUPF_HIP(hipFuncSetAttribute((const void*)k_single_factor<T>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, int(kSingleLdsMax)));
+        attr_set = true;
+    }
+    hipLaunchKernelGGL(k_single_factor<T>, dim3(1), dim3(kLanes), lds, s, d, soff, total, A, Bp,
+                       store, pivot_eps, static_piv, sing);
+    CUPF_HIP(hipGetLastError());
+}
+
+template <typename T, typename TX>
+void single_solve(const LuDevice& d, const long long* soff, long long total, const T* store,
+                  const double* rhs, int Bp, TX* x, bool transpose, hipStream_t s) {
+    if (Bp != kLanes) throw std::runtime_error("single_solve: the padded batch must be one wave");
+    const size_t lds = single_lds_bytes(total, d.n, int(sizeof(T)));
+    if (lds > kSingleLdsMax) throw std::runtime_error("single_solve: fronts exceed the LDS");
+    static bool attr_set = false;
+    if (!attr_set) {
+        CUPF_HIP(hipFuncSetAttribute((const void*)k_single_solve<T, false, TX>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, int(kSingleLdsMax)));
+        CUPF_HIP(hipFuncSetAttribute((const void*)k_single_solve<T, true, TX>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, int(kSingleLdsMax)));
+        attr_set = true;
+    }
+    if (transpose)
+        hipLaunchKernelGGL((k_single_solve<T, true, TX>), dim3(1), dim3(kLanes), lds, s, d, soff,
+                           total, store, rhs, Bp, x);
+    else
+        hipLaunchKernelGGL((k_single_solve<T, false, TX>), dim3(1), dim3(kLanes), lds, s, d, soff,
+                           total, store, rhs, Bp, x);
+    CUPF_HIP(hipGetLastError());
+}
+
+#define CUPF_SINGLE_INST(T)                                                                      \
+    template void single_factor<T>(const LuDevice&, const long long*, long long, const double*,   \
+                                   int, T*, double, int, int*, hipStream_t);                      \
+    template void single_solve<T, double>(const LuDevice&, const long long*, long long, const T*, \
+                                          const double*, int, double*, bool, hipStream_t);        \
+    template void single_solve<T, float>(const LuDevice&, const long long*, long long, const T*,  \
+                                         const double*, int, float*, bool, hipStream_t);
+
+CUPF_SINGLE_INST(double)
+CUPF_SINGLE_INST(float)
 
 #define CUPF_LU_INST(T)                                                                          \
     template void lu_factor_group<T>(const LuDevice&, int, const int*, int, int, const double*,  \

@@ -143,6 +143,7 @@ struct NewtonSolver::Impl {
         B = batch;
         Bp = kern::pad_batch(batch);
         if (lu.reserve(Bp)) ++buf_gen;
+        lu.set_single(batch == 1);
         if (Bp > capBp) {
             ++buf_gen;
             const size_t nb = size_t(Bp);

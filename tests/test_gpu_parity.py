@@ -145,6 +145,51 @@ def test_case118_single_scenario(gpu, cupf, oracle, g118):
     np.testing.assert_allclose(r.V_numpy, Vo[0], rtol=0, atol=1e-12)
 
 
+@pytest.mark.parametrize("compute,tol", [("FP64", 1e-12), ("Mixed", 1e-9), ("MixedTF32", 1e-9)])
+@pytest.mark.parametrize("transpose", [False, True])
+def test_single_system_factor_solve(gpu, cupf, oracle, g118, compute, tol, transpose):
+    """Batch 1 takes the single-system kernels (one wave, every front in
+    LDS): factor + solve, plain and transposed, against the oracle's sparse
+    LU; at batch 2 the same system goes through the batched kernels."""
+    g = g118
+    s = make_solver(cupf, g, compute, 8)
+    ja = oracle.jacobian_analysis(g.n_bus, g.indptr, g.indices, g.pv, g.pq)
+    V = random_states(g, 1, seed=3)
+    A = oracle.jacobian_values(g.n_bus, g.indptr, g.indices, g.data, V[0], g.pv, g.pq,
+                               ja["nnz"])[None]
+    rhs = np.random.default_rng(2).standard_normal((1, ja["dim"]))
+    x1, sing = capi.debug_factor_solve(s.handle, A, rhs, transpose=transpose)
+    assert not sing.any()
+    M = sp.csr_matrix((A[0], ja["col_idx"], ja["row_ptr"]), shape=(ja["dim"],) * 2)
+    if transpose:
+        M = M.T.tocsr()
+    r = np.linalg.norm(M @ x1[0] - rhs[0]) / np.linalg.norm(rhs[0])
+    assert r <= tol, r
+    xo = oracle.sparse_solve(M.indptr, M.indices, M.data, rhs[0])
+    np.testing.assert_allclose(x1[0], xo, rtol=0, atol=1e3 * tol * np.abs(xo).max())
+    x2, _ = capi.debug_factor_solve(s.handle, np.repeat(A, 2, axis=0), np.repeat(rhs, 2, axis=0),
+                                    transpose=transpose)
+    np.testing.assert_allclose(x1[0], x2[0], rtol=0, atol=1e3 * tol * np.abs(xo).max())
+
+
+@pytest.mark.parametrize("compute", ["FP64", "Mixed"])
+def test_single_system_nr_matches_batched(gpu, cupf, syn, oracle, g118, compute):
+    """NR at batch 1 (single-system LU) against the same scenario inside a
+    batch of 2 (batched LU) and the oracle: same iteration count, V within
+    1e-10 of the oracle."""
+    g = g118
+    S = g.sbus_scaled(syn.load_factors(2, g.n_bus, 0.7, 1.3, seed=8))
+    V0 = g.flat_start(2)
+    s = make_solver(cupf, g, compute, 8, refine=1)
+    r1 = s.solve(g.indptr, g.indices, g.data, g.n_bus, g.n_bus, S[0], V0[0], g.pv, g.pq)
+    r2 = s.solve_batch(g.indptr, g.indices, g.data, g.n_bus, g.n_bus, S, V0, g.pv, g.pq)
+    Vo, io, no, so = oracle.nr_solve_batch(g.n_bus, g.indptr, g.indices, g.data, S[:1], V0[:1],
+                                           g.pv, g.pq)
+    assert r1.converged and r1.iterations == io[0]
+    np.testing.assert_allclose(r1.V_numpy, Vo[0], rtol=0, atol=1e-10)
+    np.testing.assert_allclose(r1.V_numpy, r2.V_numpy[0], rtol=0, atol=1e-10)
+
+
 def test_batch_consistency_and_padding(gpu, cupf, syn, g118):
     g = g118
     s = make_solver(cupf, g, "Mixed", 64)
