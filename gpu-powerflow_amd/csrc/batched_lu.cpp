@@ -142,17 +142,27 @@ void BatchedLU::analyze(int n, const int* rp, const int* ci, const std::vector<i
         tile_ptr_[size_t(P.num_levels)] = int(tiles.size());
         d_tiles = upload_vector(tiles, s);
         // single-system arena (fronts in plan order: children first)
-        std::vector<long long> soff(size_t(P.num_fronts), 0);
+        std::vector<int> soff(size_t(P.num_fronts), 0);
         long long tot = 0;
         for (int p = 0; p < P.num_fronts; ++p) {
-            soff[size_t(p)] = tot;
+            soff[size_t(p)] = int(std::min<long long>(tot, 1LL << 30));
             tot += (long long)P.fsz[p] * P.fsz[p];
         }
-        single_total_ = tot;
-        single_fits_ = kern::single_lds_bytes(tot, n, int(front_bytes())) <= kern::kSingleLdsMax;
+        sdims_ = kern::SingleDims{};
+        sdims_.total = tot;
+        sdims_.P = P.num_fronts;
+        sdims_.nnz = P.ent_ptr[size_t(P.num_fronts)];
+        sdims_.nch = P.child_ptr[size_t(P.num_fronts)];
+        sdims_.nasm = P.asm_ptr[size_t(P.num_fronts)];
+        sdims_.nrows = P.front_ptr[size_t(P.num_fronts)];
+        sdims_.n = n;
+        const size_t need = std::max(kern::single_lds_bytes(sdims_, int(front_bytes()), false),
+                                     kern::single_lds_bytes(sdims_, int(front_bytes()), true));
+        single_fits_ = tot < (1LL << 30) && need <= kern::kSingleLdsMax;
         single_ = false;
         if (single_fits_) {
             d_soff = upload_vector(soff, s);
+            sdims_.soff = d_soff.as<int>();
             single_store_.allocate(size_t(tot) * front_bytes());
         } else {
             d_soff.release();
@@ -451,11 +461,11 @@ void BatchedLU::factor(const double* A, int Bp, hipStream_t s, StageTimer* timer
     if (single_ && Bp == kern::kLanes) {
         const int sp = opt_.static_pivot ? 1 : 0;
         if (opt_.front64)
-            kern::single_factor<double>(lu_, d_soff.as<long long>(), single_total_, A, Bp,
-                                        single_store_.as<double>(), opt_.pivot_eps, sp, mem_.sing, s);
+            kern::single_factor<double>(lu_, sdims_, A, Bp, single_store_.as<double>(),
+                                        opt_.pivot_eps, sp, mem_.sing, s);
         else
-            kern::single_factor<float>(lu_, d_soff.as<long long>(), single_total_, A, Bp,
-                                       single_store_.as<float>(), opt_.pivot_eps, sp, mem_.sing, s);
+            kern::single_factor<float>(lu_, sdims_, A, Bp, single_store_.as<float>(),
+                                       opt_.pivot_eps, sp, mem_.sing, s);
         return;
     }
     if (opt_.front64) factor_t<double>(A, Bp, s, timer, stage);
@@ -498,11 +508,11 @@ template <typename TX>
 void BatchedLU::solve_dispatch(const double* rhs, TX* x, bool transpose, int Bp, hipStream_t s) {
     if (single_ && Bp == kern::kLanes) {
         if (opt_.front64)
-            kern::single_solve<double, TX>(lu_, d_soff.as<long long>(), single_total_,
-                                           single_store_.as<double>(), rhs, Bp, x, transpose, s);
+            kern::single_solve<double, TX>(lu_, sdims_, single_store_.as<double>(), rhs, Bp, x,
+                                           transpose, s);
         else
-            kern::single_solve<float, TX>(lu_, d_soff.as<long long>(), single_total_,
-                                          single_store_.as<float>(), rhs, Bp, x, transpose, s);
+            kern::single_solve<float, TX>(lu_, sdims_, single_store_.as<float>(), rhs, Bp, x,
+                                          transpose, s);
         return;
     }
     if (opt_.front64) solve_once<double, TX>(rhs, x, transpose, Bp, s);

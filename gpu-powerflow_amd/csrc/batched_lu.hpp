@@ -141,9 +141,9 @@ private:
     // contribution-block tiles of the tile-path levels: level L's at
     // tile_ptr_[L] .. tile_ptr_[L + 1] of d_tiles
     std::vector<int> tile_ptr_;
-    // single-system arena: front p at d_soff[p], single_total_ elements
+    // single-system arena: front p at d_soff[p]
     DeviceBuffer d_soff, single_store_;
-    long long single_total_ = 0;
+    kern::SingleDims sdims_;
     bool single_fits_ = false, single_ = false;
     // pattern and its transpose with value map (refinement residuals)
     DeviceBuffer d_rp, d_ci, d_trp, d_tci, d_tmap;

@@ -186,18 +186,25 @@ void schur_tile_group(const LuDevice& d, const int* fronts, const int2* tiles, i
                       hipStream_t s);
 
 // One system (the padded batch is one wave and scenario 0 is the system):
-// the whole factorisation by one wave with every front in LDS (fronts at
-// soff[p], total elements), the factored fronts copied to store; the (plain
-// or transposed) solve from store, x written to every lane of the batch.
-// LDS need: single_lds_bytes(total, n, sizeof T) <= kSingleLdsMax.
+// the whole factorisation by one workgroup with every front in LDS (fronts
+// at soff[p], total elements), the factored fronts copied to store; the
+// (plain or transposed) solve from store, x written to every lane of the
+// batch.  The plan arrays and entry values are staged in LDS too; LDS need
+// single_lds_bytes(dims, sizeof T, solve) <= kSingleLdsMax.
 constexpr size_t kSingleLdsMax = 159 * 1024;
-size_t single_lds_bytes(long long total, int n, int elem);
+struct SingleDims {
+    const int* soff = nullptr;        // [P] front offsets in the single-system arena
+    long long total = 0;              // arena elements
+    int P = 0, nnz = 0, nch = 0, nasm = 0, nrows = 0, n = 0;   // fronts, entries,
+                                      // child list, asm_loc, front_rows, dim
+};
+size_t single_lds_bytes(const SingleDims& sd, int elem, bool solve);
 template <typename T>
-void single_factor(const LuDevice& d, const long long* soff, long long total, const double* A,
-                   int Bp, T* store, double pivot_eps, int static_piv, int* sing, hipStream_t s);
+void single_factor(const LuDevice& d, const SingleDims& sd, const double* A, int Bp, T* store,
+                   double pivot_eps, int static_piv, int* sing, hipStream_t s);
 template <typename T, typename TX>
-void single_solve(const LuDevice& d, const long long* soff, long long total, const T* store,
-                  const double* rhs, int Bp, TX* x, bool transpose, hipStream_t s);
+void single_solve(const LuDevice& d, const SingleDims& sd, const T* store, const double* rhs,
+                  int Bp, TX* x, bool transpose, hipStream_t s);
 
 // Solve sweeps.  rhs (FP64) and x are [n][Bp]; y [n][Bp] and upd are work
 // vectors, upd holding the tier-0 fronts' update vectors as [upd_total][Bp]

@@ -2157,160 +2157,254 @@ __global__ __launch_bounds__(256) void k_scatter(const int* __restrict__ perm, i
 }
 
 // ===========================================================================
-// One system (batch 1): the whole factorisation, and each solve, by one wave
-// with every front resident in LDS at once (fronts at soff[p] in a compact
-// single-system arena, children before parents).  Launch latency, not
-// arithmetic, bounds a single small system: the batched sweep issues a
-// launch per (level, tier) group, these one launch each.
+// One system (batch 1): the whole factorisation, and each solve, by one
+// workgroup with every front resident in LDS at once (fronts at soff[p] in a
+// compact single-system arena, children before parents).  Launch latency,
+// not arithmetic, bounds a single small system: the batched sweep issues a
+// launch per (level, tier) group, these one launch each.  The plan arrays
+// the sweep walks (front sizes, offsets, entry and child lists, row maps)
+// and the system's entry values are staged into LDS first with batched
+// loads, so the per-front work touches LDS only.  Each front is processed in
+// blocks with a few barriers, whatever its pivot count:
 //
-//   k_single_factor: per front in postorder: zero, original entries, the
-//     children's contribution blocks (extend-add), then the partial LU of
-//     its K pivots (guarded pivots, unit L below, U on and above), one front
-//     row per lane; the factored fronts are copied to `store` for the solves.
-//   k_single_solve: fronts from `store` into LDS, y = P rhs, then
-//     plain:      L forward (fronts in postorder, each pivot's column
-//                 scattered to the rows below), U backward (reverse
-//                 postorder, each pivot row's dot product reduced over the
-//                 wave);
-//     transposed: U^T forward (each pivot row scattered), L^T backward (each
-//                 pivot column's dot product reduced);
-//     x[perm[k]] = y[k], written to every lane of the padded batch.
-// Lanes that read what other lanes of the wave wrote do so after
+//   k_single_factor: zero, original entries, the children's contribution
+//     blocks (one child at a time); wave 0 factors the K x K pivot block
+//     (guarded pivots, a pivot step per wave-synchronous round); then U12 =
+//     L11^-1 A12 (a column per thread) beside L21 = A21 U11^-1 (a row per
+//     thread); then C = A22 - L21 U12 (an element per thread).  The factored
+//     fronts are copied to `store` for the solves.
+//   k_single_solve: fronts from `store` into LDS, y = P rhs, then per front
+//     the pivot block's triangular solve on wave 0 and the contribution rows'
+//     update by all threads (plain: L forward then U backward; transposed:
+//     U^T forward then L^T backward), x[perm[k]] = y[k] on every lane of the
+//     padded batch.
+// Inside wave 0's rounds, lanes read what other lanes wrote after
 // wave_sync() (a wavefront fence: the wave's LDS accesses are performed in
-// order, and the fence keeps the compiler from reordering across it).
+// order and the compiler does not move them across it).
 // ===========================================================================
+constexpr int kSingleThreads = 256;
+
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <typename T>
-__device__ __forceinline__ T wave_sum(T v) {
+// dst[0..n) = src[0..n) (converted), four loads in flight per thread, every
+// load from an index clamped into [0, n).
+template <typename TD, typename TS>
+__device__ inline void stage(TD* __restrict__ dst, const TS* __restrict__ src, long long n) {
+    for (long long e0 = threadIdx.x; e0 < n; e0 += 4 * kSingleThreads) {
+        TS v[4];
 #pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
+        for (int u = 0; u < 4; ++u) v[u] = src[min(e0 + u * kSingleThreads, n - 1)];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (e0 + u * kSingleThreads < n) dst[e0 + u * kSingleThreads] = TD(v[u]);
+    }
 }
 
+// The LDS layout of the single-system kernels: T arrays first, then ints.
+struct SingleLds {
+    int* fsz;
+    int* nc;
+    int* soff;
+    int* ptr;       // factor: ent_ptr; solve: front_ptr        [P + 1]
+    int* chp;       // factor: child_ptr [P + 1]
+    int* chi;       // factor: child_idx [nch]
+    int* asmp;      // factor: asm_ptr [P + 1]
+    int* list;      // factor: ent_dst [nnz]; solve: front_rows [nrows]
+    int* aloc;      // factor: asm_loc [nasm]; solve: perm [n]
+};
+
 template <typename T>
-__global__ __launch_bounds__(kLanes) void k_single_factor(LuDevice d,
-                                                          const long long* __restrict__ soff,
-                                                          long long total,
-                                                          const double* __restrict__ A, int Bp,
-                                                          T* __restrict__ store, double eps,
-                                                          int static_piv, int* __restrict__ sing) {
+__global__ __launch_bounds__(kSingleThreads) void k_single_factor(
+    LuDevice d, SingleDims sd, const double* __restrict__ A, int Bp, T* __restrict__ store,
+    double eps, int static_piv, int* __restrict__ sing) {
     extern __shared__ unsigned char smem_raw[];
     T* F = reinterpret_cast<T*>(smem_raw);                        // [total]
-    const int lane = threadIdx.x;
-    int flag = 0;
-    for (int p = 0; p < d.P; ++p) {
-        const int fs = d.fsz[p], K = d.nc[p];
-        T* G = F + soff[p];
-        for (int e = lane; e < fs * fs; e += kLanes) G[e] = T(0);
-        wave_sync();
-        for (int e = d.ent_ptr[p] + lane; e < d.ent_ptr[p + 1]; e += kLanes)
-            G[d.ent_dst[e]] = T(A[(long long)d.ent_src[e] * Bp]);      // scenario 0
-        wave_sync();
-        for (int q = d.child_ptr[p]; q < d.child_ptr[p + 1]; ++q) {
-            const int c = d.child_idx[q];
-            const int cfs = d.fsz[c], cnc = d.nc[c], cuc = cfs - cnc;
-            const int* loc = d.asm_loc + d.asm_ptr[c];
-            const T* C = F + soff[c] + (long long)cnc * cfs + cnc;
-            // one contribution row per lane: distinct landing rows
-            for (int a = lane; a < cuc; a += kLanes) {
-                T* Gr = G + loc[a] * fs;
-                const T* Cr = C + a * cfs;
-                for (int bb = 0; bb < cuc; ++bb) Gr[loc[bb]] += Cr[bb];
-            }
-            wave_sync();                                          // next child may share elements
-        }
-        for (int k = 0; k < K; ++k) {
-            const T piv = G[k * fs + k];
-            const T g = guard_pivot(piv, eps, static_piv, flag);
-            const int m = fs - k - 1;
-            const T* U = G + k * fs + k + 1;                      // pivot row, columns > k
-            for (int i = lane; i < m; i += kLanes) {              // one row below the pivot per lane
-                T* Gr = G + (k + 1 + i) * fs;
-                const T l = Gr[k] / g;
-                Gr[k] = l;
-                for (int j = 0; j < m; ++j) Gr[k + 1 + j] -= l * U[j];
-            }
-            if (lane == 0 && g != piv) G[k * fs + k] = g;
-            wave_sync();
-        }
+    T* av = F + sd.total;                                         // entry values [nnz]
+    SingleLds L;
+    L.fsz = reinterpret_cast<int*>(av + sd.nnz);
+    L.nc = L.fsz + sd.P;
+    L.soff = L.nc + sd.P;
+    L.ptr = L.soff + sd.P;
+    L.chp = L.ptr + sd.P + 1;
+    L.chi = L.chp + sd.P + 1;
+    L.asmp = L.chi + sd.nch;
+    L.list = L.asmp + sd.P + 1;
+    L.aloc = L.list + sd.nnz;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    stage(L.fsz, d.fsz, sd.P);
+    stage(L.nc, d.nc, sd.P);
+    stage(L.soff, sd.soff, sd.P);
+    stage(L.ptr, d.ent_ptr, sd.P + 1);
+    stage(L.chp, d.child_ptr, sd.P + 1);
+    stage(L.chi, d.child_idx, sd.nch);
+    stage(L.asmp, d.asm_ptr, sd.P + 1);
+    stage(L.list, d.ent_dst, sd.nnz);
+    stage(L.aloc, d.asm_loc, sd.nasm);
+    for (int e0 = tid; e0 < sd.nnz; e0 += 4 * kSingleThreads) {  // entry values, scenario 0
+        int src[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) src[u] = d.ent_src[min(e0 + u * kSingleThreads, sd.nnz - 1)];
+        double v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = A[(long long)src[u] * Bp];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (e0 + u * kSingleThreads < sd.nnz) av[e0 + u * kSingleThreads] = T(v[u]);
     }
-    for (long long e = lane; e < total; e += kLanes) store[e] = F[e];
-    if (flag) atomicOr(sing, 1);
+    __syncthreads();
+    int flag = 0;
+    for (int p = 0; p < sd.P; ++p) {
+        const int fs = L.fsz[p], K = L.nc[p], uc = fs - K;
+        T* G = F + L.soff[p];
+        for (int e = tid; e < fs * fs; e += kSingleThreads) G[e] = T(0);
+        __syncthreads();
+        for (int e = L.ptr[p] + tid; e < L.ptr[p + 1]; e += kSingleThreads) G[L.list[e]] = av[e];
+        __syncthreads();
+        for (int q = L.chp[p]; q < L.chp[p + 1]; ++q) {
+            const int c = L.chi[q];
+            const int cfs = L.fsz[c], cnc = L.nc[c], cuc = cfs - cnc;
+            const int* loc = L.aloc + L.asmp[c];
+            const T* C = F + L.soff[c] + cnc * cfs + cnc;
+            // wave per contribution row, lane per column: distinct elements
+            for (int a = wave; a < cuc; a += kSingleThreads / 64) {
+                T* Gr = G + loc[a] * fs;
+                for (int b = lane; b < cuc; b += 64) Gr[loc[b]] += C[a * cfs + b];
+            }
+            __syncthreads();                                      // the next child may share elements
+        }
+        if (wave == 0) {                                          // pivot block L11 \ U11
+            for (int k = 0; k < K; ++k) {
+                const T piv = G[k * fs + k];
+                const T g = guard_pivot(piv, eps, static_piv, flag);
+                for (int i = k + 1 + lane; i < K; i += 64) G[i * fs + k] /= g;
+                wave_sync();
+                if (lane == 0 && g != piv) G[k * fs + k] = g;
+                const int m = K - k - 1;
+                for (int e = lane; e < m * m; e += 64) {
+                    const int i = k + 1 + e / m, j = k + 1 + e % m;
+                    G[i * fs + j] -= G[i * fs + k] * G[k * fs + j];
+                }
+                wave_sync();
+            }
+        }
+        __syncthreads();
+        // U12 = L11^-1 A12 (a column per thread) and L21 = A21 U11^-1 (a row
+        // per thread): disjoint elements, both read only the pivot block
+        for (int t = tid; t < 2 * uc; t += kSingleThreads) {
+            if (t < uc) {
+                const int j = K + t;
+                for (int k = 0; k < K; ++k) {
+                    const T u = G[k * fs + j];
+                    for (int i = k + 1; i < K; ++i) G[i * fs + j] -= G[i * fs + k] * u;
+                }
+            } else {
+                T* Gr = G + (K + t - uc) * fs;
+                for (int k = 0; k < K; ++k) {
+                    T v = Gr[k];
+                    for (int j = 0; j < k; ++j) v -= Gr[j] * G[j * fs + k];
+                    Gr[k] = v / G[k * fs + k];
+                }
+            }
+        }
+        __syncthreads();
+        for (int e = tid; e < uc * uc; e += kSingleThreads) {    // C = A22 - L21 U12
+            const int i = K + e / uc, j = K + e % uc;
+            T v = G[i * fs + j];
+            for (int k = 0; k < K; ++k) v -= G[i * fs + k] * G[k * fs + j];
+            G[i * fs + j] = v;
+        }
+        __syncthreads();
+    }
+    for (long long e = tid; e < sd.total; e += kSingleThreads) store[e] = F[e];
+    if (flag && tid == 0) atomicOr(sing, 1);                  // flag is wave 0's (pivot block)
 }
 
 template <typename T, bool TRANS, typename TX>
-__global__ __launch_bounds__(kLanes) void k_single_solve(LuDevice d,
-                                                         const long long* __restrict__ soff,
-                                                         long long total,
-                                                         const T* __restrict__ store,
-                                                         const double* __restrict__ rhs, int Bp,
-                                                         TX* __restrict__ x) {
+__global__ __launch_bounds__(kSingleThreads) void k_single_solve(
+    LuDevice d, SingleDims sd, const T* __restrict__ store, const double* __restrict__ rhs, int Bp,
+    TX* __restrict__ x) {
     extern __shared__ unsigned char smem_raw[];
     T* F = reinterpret_cast<T*>(smem_raw);                        // [total]
-    T* y = F + total;                                             // [n]
-    const int lane = threadIdx.x;
-    for (long long e = lane; e < total; e += kLanes) F[e] = store[e];
-    for (int k = lane; k < d.n; k += kLanes) y[k] = T(rhs[(long long)d.perm[k] * Bp]);
-    wave_sync();
-    if constexpr (!TRANS) {
-        for (int p = 0; p < d.P; ++p) {                           // L y = P rhs
-            const int fs = d.fsz[p], K = d.nc[p];
-            const T* G = F + soff[p];
-            const int* rows = d.front_rows + d.front_ptr[p];
-            for (int k = 0; k < K; ++k) {
-                const T yk = y[rows[k]];
-                for (int i = k + 1 + lane; i < fs; i += kLanes) y[rows[i]] -= G[i * fs + k] * yk;
-                wave_sync();
-            }
-        }
-        for (int p = d.P - 1; p >= 0; --p) {                      // U x = y
-            const int fs = d.fsz[p], K = d.nc[p];
-            const T* G = F + soff[p];
-            const int* rows = d.front_rows + d.front_ptr[p];
-            for (int k = K - 1; k >= 0; --k) {
-                T sum = T(0);
-                for (int j = k + 1 + lane; j < fs; j += kLanes) sum += G[k * fs + j] * y[rows[j]];
-                sum = wave_sum(sum);
-                const T v = (y[rows[k]] - sum) / G[k * fs + k];
-                wave_sync();
-                if (lane == 0) y[rows[k]] = v;
-                wave_sync();
-            }
-        }
-    } else {
-        for (int p = 0; p < d.P; ++p) {                           // U^T z = P rhs
-            const int fs = d.fsz[p], K = d.nc[p];
-            const T* G = F + soff[p];
-            const int* rows = d.front_rows + d.front_ptr[p];
-            for (int k = 0; k < K; ++k) {
-                const T zk = y[rows[k]] / G[k * fs + k];
-                wave_sync();
-                if (lane == 0) y[rows[k]] = zk;
-                for (int j = k + 1 + lane; j < fs; j += kLanes) y[rows[j]] -= G[k * fs + j] * zk;
-                wave_sync();
-            }
-        }
-        for (int p = d.P - 1; p >= 0; --p) {                      // L^T x = z
-            const int fs = d.fsz[p], K = d.nc[p];
-            const T* G = F + soff[p];
-            const int* rows = d.front_rows + d.front_ptr[p];
-            for (int k = K - 1; k >= 0; --k) {
-                T sum = T(0);
-                for (int i = k + 1 + lane; i < fs; i += kLanes) sum += G[i * fs + k] * y[rows[i]];
-                sum = wave_sum(sum);
-                const T v = y[rows[k]] - sum;
-                wave_sync();
-                if (lane == 0) y[rows[k]] = v;
-                wave_sync();
-            }
-        }
+    T* y = F + sd.total;                                          // [n]
+    SingleLds L;
+    L.fsz = reinterpret_cast<int*>(y + sd.n);
+    L.nc = L.fsz + sd.P;
+    L.soff = L.nc + sd.P;
+    L.ptr = L.soff + sd.P;
+    L.list = L.ptr + sd.P + 1;
+    L.aloc = L.list + sd.nrows;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    stage(F, store, sd.total);
+    stage(L.fsz, d.fsz, sd.P);
+    stage(L.nc, d.nc, sd.P);
+    stage(L.soff, sd.soff, sd.P);
+    stage(L.ptr, d.front_ptr, sd.P + 1);
+    stage(L.list, d.front_rows, sd.nrows);
+    stage(L.aloc, d.perm, sd.n);
+    __syncthreads();
+    for (int e0 = tid; e0 < sd.n; e0 += 4 * kSingleThreads) {     // y = P rhs (scenario 0)
+        double v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            v[u] = rhs[(long long)L.aloc[min(e0 + u * kSingleThreads, sd.n - 1)] * Bp];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (e0 + u * kSingleThreads < sd.n) y[e0 + u * kSingleThreads] = T(v[u]);
     }
-    for (int k = 0; k < d.n; ++k) x[(long long)d.perm[k] * Bp + lane] = TX(y[k]);
+    __syncthreads();
+    for (int p = 0; p < sd.P; ++p) {                              // forward, postorder
+        const int fs = L.fsz[p], K = L.nc[p];
+        const T* G = F + L.soff[p];
+        const int* rows = L.list + L.ptr[p];
+        if (wave == 0) {
+            for (int k = 0; k < K; ++k) {
+                T zk = y[rows[k]];
+                if (TRANS) zk /= G[k * fs + k];                   // U11^T: non-unit
+                wave_sync();
+                if (TRANS && lane == 0) y[rows[k]] = zk;
+                for (int i = k + 1 + lane; i < K; i += 64)        // L11 (unit) / U11^T
+                    y[rows[i]] -= (TRANS ? G[k * fs + i] : G[i * fs + k]) * zk;
+                wave_sync();
+            }
+        }
+        __syncthreads();
+        for (int i = K + tid; i < fs; i += kSingleThreads) {      // contribution rows
+            T v = y[rows[i]];
+            for (int k = 0; k < K; ++k)
+                v -= (TRANS ? G[k * fs + i] : G[i * fs + k]) * y[rows[k]];
+            y[rows[i]] = v;
+        }
+        __syncthreads();
+    }
+    for (int p = sd.P - 1; p >= 0; --p) {                         // backward, reverse postorder
+        const int fs = L.fsz[p], K = L.nc[p];
+        const T* G = F + L.soff[p];
+        const int* rows = L.list + L.ptr[p];
+        for (int k = tid; k < K; k += kSingleThreads) {           // contribution rows' terms
+            T v = y[rows[k]];
+            for (int j = K; j < fs; ++j) v -= (TRANS ? G[j * fs + k] : G[k * fs + j]) * y[rows[j]];
+            y[rows[k]] = v;
+        }
+        __syncthreads();
+        if (wave == 0) {
+            for (int k = K - 1; k >= 0; --k) {
+                T xk = y[rows[k]];
+                if (!TRANS) xk /= G[k * fs + k];                  // U11: non-unit
+                wave_sync();
+                if (!TRANS && lane == 0) y[rows[k]] = xk;
+                for (int i = lane; i < k; i += 64)                // U11 / L11^T (unit)
+                    y[rows[i]] -= (TRANS ? G[k * fs + i] : G[i * fs + k]) * xk;
+                wave_sync();
+            }
+        }
+        __syncthreads();
+    }
+    for (int k = wave; k < sd.n; k += kSingleThreads / 64)
+        x[(long long)L.aloc[k] * Bp + lane] = TX(y[k]);
 }
 
 }  // namespace
@@ -2560,14 +2654,18 @@ void lu_solve_scatter(const LuDevice& d, const T* y, int Bp, TX* x, hipStream_t 
     CUPF_HIP(hipGetLastError());
 }
 
-size_t single_lds_bytes(long long total, int n, int elem) {
-    return size_t(total + n) * size_t(elem);
+size_t single_lds_bytes(const SingleDims& sd, int elem, bool solve) {
+    if (solve)
+        return size_t(sd.total + sd.n) * size_t(elem) +
+               sizeof(int) * size_t(3 * sd.P + sd.P + 1 + sd.nrows + sd.n);
+    return size_t(sd.total + sd.nnz) * size_t(elem) +
+           sizeof(int) * size_t(3 * sd.P + 3 * (sd.P + 1) + sd.nch + sd.nnz + sd.nasm);
 }
 
 template <typename T>
-void single_factor(const LuDevice& d, const long long* soff, long long total, const double* A,
-                   int Bp, T* store, double pivot_eps, int static_piv, int* sing, hipStream_t s) {
-    const size_t lds = single_lds_bytes(total, 0, int(sizeof(T)));
+void single_factor(const LuDevice& d, const SingleDims& sd, const double* A, int Bp, T* store,
+                   double pivot_eps, int static_piv, int* sing, hipStream_t s) {
+    const size_t lds = single_lds_bytes(sd, int(sizeof(T)), false);
     if (lds > kSingleLdsMax) throw std::runtime_error("single_factor: fronts exceed the LDS");
     static bool attr_set = false;
     if (!attr_set) {
@@ -2575,16 +2673,16 @@ void single_factor(const LuDevice& d, const long long* soff, long long total, co
                                      hipFuncAttributeMaxDynamicSharedMemorySize, int(kSingleLdsMax)));
         attr_set = true;
     }
-    hipLaunchKernelGGL(k_single_factor<T>, dim3(1), dim3(kLanes), lds, s, d, soff, total, A, Bp,
+    hipLaunchKernelGGL(k_single_factor<T>, dim3(1), dim3(kSingleThreads), lds, s, d, sd, A, Bp,
                        store, pivot_eps, static_piv, sing);
     CUPF_HIP(hipGetLastError());
 }
 
 template <typename T, typename TX>
-void single_solve(const LuDevice& d, const long long* soff, long long total, const T* store,
-                  const double* rhs, int Bp, TX* x, bool transpose, hipStream_t s) {
+void single_solve(const LuDevice& d, const SingleDims& sd, const T* store, const double* rhs,
+                  int Bp, TX* x, bool transpose, hipStream_t s) {
     if (Bp != kLanes) throw std::runtime_error("single_solve: the padded batch must be one wave");
-    const size_t lds = single_lds_bytes(total, d.n, int(sizeof(T)));
+    const size_t lds = single_lds_bytes(sd, int(sizeof(T)), true);
     if (lds > kSingleLdsMax) throw std::runtime_error("single_solve: fronts exceed the LDS");
     static bool attr_set = false;
     if (!attr_set) {
@@ -2595,20 +2693,20 @@ void single_solve(const LuDevice& d, const long long* soff, long long total, con
         attr_set = true;
     }
     if (transpose)
-        hipLaunchKernelGGL((k_single_solve<T, true, TX>), dim3(1), dim3(kLanes), lds, s, d, soff,
-                           total, store, rhs, Bp, x);
+        hipLaunchKernelGGL((k_single_solve<T, true, TX>), dim3(1), dim3(kSingleThreads), lds, s, d,
+                           sd, store, rhs, Bp, x);
     else
-        hipLaunchKernelGGL((k_single_solve<T, false, TX>), dim3(1), dim3(kLanes), lds, s, d, soff,
-                           total, store, rhs, Bp, x);
+        hipLaunchKernelGGL((k_single_solve<T, false, TX>), dim3(1), dim3(kSingleThreads), lds, s, d,
+                           sd, store, rhs, Bp, x);
     CUPF_HIP(hipGetLastError());
 }
 
 #define CUPF_SINGLE_INST(T)                                                                      \
-    template void single_factor<T>(const LuDevice&, const long long*, long long, const double*,   \
-                                   int, T*, double, int, int*, hipStream_t);                      \
-    template void single_solve<T, double>(const LuDevice&, const long long*, long long, const T*, \
+    template void single_factor<T>(const LuDevice&, const SingleDims&, const double*, int, T*,    \
+                                   double, int, int*, hipStream_t);                               \
+    template void single_solve<T, double>(const LuDevice&, const SingleDims&, const T*,           \
                                           const double*, int, double*, bool, hipStream_t);        \
-    template void single_solve<T, float>(const LuDevice&, const long long*, long long, const T*,  \
+    template void single_solve<T, float>(const LuDevice&, const SingleDims&, const T*,            \
                                          const double*, int, float*, bool, hipStream_t);
 
 CUPF_SINGLE_INST(double)
