@@ -47,11 +47,17 @@ HBM_PEAK_GBS = 8000.0
 FP32_MFMA_PEAK_TFLOPS = 157.3
 FP64_PEAK_TFLOPS = 78.6
 TRAFFIC_FILE = ROOT / "profiles" / "traffic.json"
-# the dense factor stage's kernel per form
-DENSE_KERNEL = {"fp64": "k_schur_tile<double,0>", "mixed": "k_schur<float,1>",
-                "bf16x3": "k_schur<float,2>", "learned": "k_schur<float,1>"}
-DENSE_KERNEL_10K = {"fp64": "k_schur_tile<double,0>", "mixed": "k_schur_tile<float,1>",
-                    "bf16x3": "k_schur_tile<float,2>"}
+# MFMA mode of each form's dense kernel (0 FP64, 1 FP32, 2 3xBF16)
+DENSE_MODE = {"fp64": 0, "mixed": 1, "bf16x3": 2, "learned": 1}
+
+
+def dense_kernel(form, kind):
+    """The timed dense stage's kernel: the solver's stats()["dense_kind"] (2
+    contribution-block tiles, 1 row-block Schur, 0 tier-2 LU groups) for the
+    form's front precision and MFMA mode."""
+    t = "double" if form == "fp64" else "float"
+    m = DENSE_MODE[form]
+    return {2: f"k_schur_tile<{t},{m}>", 1: f"k_schur<{t},{m}>"}.get(kind, "tier-2 LU groups")
 
 
 def parse():
@@ -493,7 +499,7 @@ def case10000(torch, cupf, syn, po, args):
              "stage_ms_per_step": {k: v[0] / steps for k, v in st.items()},
              "error_vs_oracle_1e-13": va_vm_err(Vout[:1].cpu().numpy(), Vo),
              "dense_roofline": dense_roofline(stats, st["factor_dense"], B, 8 if f == "fp64" else 4,
-                                              DENSE_KERNEL_10K[f])}
+                                              dense_kernel(f, stats["dense_kind"]))}
         out[f] = e
         log(f"case10000 {f}: {B / ms:.1f} solves/s, {it[-1]} iterations")
         del sv
@@ -606,7 +612,7 @@ def main():
     roofs = {}
     for f, r in res.items():
         rf = dense_roofline(r["stats"], r["stages"]["factor_dense"], B, 8 if f == "fp64" else 4,
-                            DENSE_KERNEL[f])
+                            dense_kernel(f, r["stats"]["dense_kind"]))
         if rf is not None:
             rf["traffic"] = traffic_for(rf["kernel"], workload)
         roofs[f] = rf

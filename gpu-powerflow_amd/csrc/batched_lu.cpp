@@ -187,27 +187,32 @@ void BatchedLU::analyze(int n, const int* rp, const int* ci, const std::vector<i
     lu.asm_cptr = d_acptr.as<int>(); lu.asm_chunks = d_achunks.as<int4>();
     lu.gat_ptr = d_gptr.as<int>(); lu.gat_src = d_gsrc.as<int>(); lu.rhs_idx = d_ridx.as<int>();
 
-    // Algorithmic work of the timed dense stage, per scenario.
-    //  pivot + Schur groups (the Schur kernel): flops 2 uc^2 K (update) +
-    //    2 uc K^2 (L21); bytes = the rows >= K written once (CB uc^2, L21
-    //    uc K) + the child contribution elements landing in them read once +
-    //    U12 (K uc) read once, at the front element size, + the values of
-    //    those rows at 8 B;
-    //  other tier-2 groups (assembly + dense LU): flops of the front LU;
-    //    bytes = each front element written and read once + its values.
+    // The timed dense stage is one kernel: the tile kernel when any level
+    // takes it, else the row-block Schur kernel, else the tier-2 LU groups.
+    // Its algorithmic work per scenario:
+    //  tile kernel: flops 2 uc^2 K + 2 uc K^2; bytes = C written once + the
+    //    children's contribution elements landing in rows and columns >= K
+    //    read once + L21 and U12 read once, at the front element size, + the
+    //    A22 entries at 8 B;
+    //  row-block Schur kernel: the same flops; bytes = the rows >= K written
+    //    once (CB uc^2, L21 uc K) + the child contribution elements landing in
+    //    them read once + U12 (K uc) read once + the values of those rows;
+    //  tier-2 LU groups: flops of the front LU; bytes = each front element
+    //    written and read once + its values.
     dense_flops_ = dense_bytes_ = 0.0;
     dense_groups_ = 0;
+    dense_kind_ = 0;
+    auto kind_of = [&](int level) { return tile_schur(level) ? 2 : cross_schur(level) ? 1 : 0; };
+    for (int L = 0; L < P.num_levels; ++L)
+        if (has_group(L, 2)) dense_kind_ = std::max(dense_kind_, kind_of(L));
     for (int p = 0; p < P.num_fronts; ++p) {
-        if (P.tier[p] != 2) continue;
+        if (P.tier[p] != 2 || kind_of(P.level[p]) != dense_kind_) continue;
         const int fsz = P.fsz[p], K = P.nc[p];
         const double f = fsz;
-        if (tile_schur(P.level[p])) {
-            // the tile kernel: C written once, the children's contribution
-            // elements landing in rows and columns >= K read once, L21 and
-            // U12 read once, the A22 entries
-            const double uc = f - K;
+        const double uc = f - K;
+        const int* rl = P.row_lb.data() + P.lb_ptr[p];
+        if (dense_kind_ == 2) {
             double child = 0.0;
-            const int* rl = P.row_lb.data() + P.lb_ptr[p];
             for (int q = P.child_ptr[p]; q < P.child_ptr[p + 1]; ++q) {
                 const int c = P.child_idx[q];
                 const int* cl = rl + (1 + q - P.child_ptr[p]) * (fsz + 1);
@@ -219,12 +224,8 @@ void BatchedLU::analyze(int n, const int* rp, const int* ci, const std::vector<i
                 if (P.ent_dst[e] % fsz >= K) ents += 1.0;
             dense_flops_ += 2.0 * uc * uc * K + 2.0 * uc * K * K;
             dense_bytes_ += double(front_bytes()) * (uc * uc + child + 2.0 * uc * K) + 8.0 * ents;
-            continue;
-        }
-        if (cross_schur(P.level[p])) {
-            const double uc = f - K;
+        } else if (dense_kind_ == 1) {
             double child = 0.0;
-            const int* rl = P.row_lb.data() + P.lb_ptr[p];
             for (int q = P.child_ptr[p]; q < P.child_ptr[p + 1]; ++q) {
                 const int c = P.child_idx[q];
                 const int* cl = rl + (1 + q - P.child_ptr[p]) * (fsz + 1);
@@ -234,19 +235,19 @@ void BatchedLU::analyze(int n, const int* rp, const int* ci, const std::vector<i
             const double ents = rl[fsz] - rl[K];
             dense_flops_ += 2.0 * uc * uc * K + 2.0 * uc * K * K;
             dense_bytes_ += double(front_bytes()) * (uc * uc + child + 2.0 * uc * K) + 8.0 * ents;
-            continue;
+        } else {
+            double fl = 0.0;
+            for (int k = 0; k < K; ++k) {
+                const double r = f - k - 1;
+                fl += 2.0 * r * r + r;
+            }
+            dense_flops_ += fl;
+            dense_bytes_ += 2.0 * double(front_bytes()) * f * f +
+                            8.0 * double(P.ent_ptr[p + 1] - P.ent_ptr[p]);
         }
-        double fl = 0.0;
-        for (int k = 0; k < K; ++k) {
-            const double r = f - k - 1;
-            fl += 2.0 * r * r + r;
-        }
-        dense_flops_ += fl;
-        dense_bytes_ += 2.0 * double(front_bytes()) * f * f +
-                        8.0 * double(P.ent_ptr[p + 1] - P.ent_ptr[p]);
     }
-    for (int g = 0; g < P.num_levels; ++g)
-        if (P.group_ptr[g * 3 + 3] > P.group_ptr[g * 3 + 2]) ++dense_groups_;
+    for (int L = 0; L < P.num_levels; ++L)
+        if (has_group(L, 2) && kind_of(L) == dense_kind_) ++dense_groups_;
 
     // batch buffers are sized lazily for the new structure
     capBp_ = 0;
@@ -417,8 +418,9 @@ bool BatchedLU::tile_schur(int level) const {
            (opt_.front64 || !kern::row_schur_fits(plan_.group_max_fsz[level * 3 + 2]));
 }
 
-// Stage `stage` times the dense tier-2 work: the Schur launch of a pivot +
-// Schur group, the whole group otherwise.
+// Stage `stage` times the launches of the dense kernel (dense_kind_): the tile
+// or row-block Schur launches of the pivot + Schur groups, else the tier-2 LU
+// groups.
 template <typename T>
 void BatchedLU::factor_t(const double* A, int Bp, hipStream_t s, StageTimer* timer, int stage) {
     const kern::Arenas<T> ar{static_cast<T*>(mem_.arena0), static_cast<T*>(mem_.arena1)};
@@ -433,18 +435,18 @@ void BatchedLU::factor_t(const double* A, int Bp, hipStream_t s, StageTimer* tim
             kern::cross_group<T>(lu_, f, cnt, mf, A, Bp, ar, uinv, opt_.pivot_eps, sp, mem_.sing,
                                  tiles, st);
             if (tiles) {
-                StageScope sc(timer, stage, st);
+                StageScope sc(timer, dense_kind_ == 2 ? stage : -1, st);
                 const int t0 = tile_ptr_[size_t(L)];
                 kern::schur_tile_group<T>(lu_, f, d_tiles.as<int2>() + t0,
                                           tile_ptr_[size_t(L) + 1] - t0, kTileScenGroup, A, Bp, ar,
                                           opt_.dense_mode, st);
                 return;
             }
-            StageScope sc(timer, stage, st);
+            StageScope sc(timer, dense_kind_ == 1 ? stage : -1, st);
             kern::schur_group<T>(lu_, f, cnt, mf, A, Bp, ar, uinv, opt_.dense_mode, st);
             return;
         }
-        StageScope sc(timer, t == 2 ? stage : -1, st);
+        StageScope sc(timer, t == 2 && dense_kind_ == 0 ? stage : -1, st);
         kern::lu_factor_group<T>(lu_, t, f, cnt, mf, A, Bp, ar, opt_.pivot_eps, sp, mem_.sing,
                                  opt_.dense_mode, st);
     };
@@ -534,11 +536,13 @@ void BatchedLU::solve(const double* A, const double* rhs, TX* x, TX* xtmp, bool 
                       int steps, int Bp, hipStream_t s) {
     if (Bp > memBp_) throw std::runtime_error("BatchedLU::solve(): batch exceeds the reserved size");
     solve_dispatch<TX>(rhs, x, transpose, Bp, s);
-    if (opt_.front64) return;
+    if (opt_.front64 || steps <= 0) return;
+    // the refinement's update of x runs under the refinement gate (its
+    // residual and correction solve write scratch only)
     for (int it = 0; it < steps; ++it) {
         residual<TX>(A, rhs, x, mem_.res, transpose, Bp, s);
         solve_dispatch<TX>(mem_.res, xtmp, transpose, Bp, s);
-        kern::axpy_update<TX>(xtmp, x, (long long)Bp * n_, s);
+        kern::axpy_update<TX>(xtmp, x, (long long)Bp * n_, s, gate_refine_);
     }
 }
 

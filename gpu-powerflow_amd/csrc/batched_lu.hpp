@@ -75,6 +75,16 @@ public:
     void set_single(bool on) { single_ = on && single_fits_; }
     bool single() const { return single_; }
 
+    // Gates of the following launches (a captured Newton step run under a
+    // device control word; null = open): the factor kernels set singular
+    // flags only while `step` is open, and the refinement steps of solve
+    // update x only while `refine` is open.
+    void set_gates(const int* step, const int* refine) {
+        gate_step_ = step;
+        gate_refine_ = refine;
+        lu_.gate = step;
+    }
+
     // Clear the per-scenario singular-pivot flags (a fill kernel).
     void clear_singular(int Bp, hipStream_t s);
     const int* singular() const { return mem_.sing; }
@@ -106,6 +116,8 @@ public:
     double dense_flops() const { return dense_flops_; }
     double dense_bytes() const { return dense_bytes_; }
     int dense_groups() const { return dense_groups_; }
+    // The timed dense kernel: 2 tile Schur, 1 row-block Schur, 0 tier-2 LU.
+    int dense_kind() const { return dense_kind_; }
     // residual work buffer [n][Bp] FP64 (free between calls)
     double* res() const { return mem_.res; }
 
@@ -131,8 +143,11 @@ private:
     int n_ = 0, nnz_ = 0;
     Symbolic sym_;
     LuPlan plan_;
+    const int* gate_step_ = nullptr;
+    const int* gate_refine_ = nullptr;
     double dense_flops_ = 0.0, dense_bytes_ = 0.0;
     int dense_groups_ = 0;
+    int dense_kind_ = 0;
 
     // device plan
     DeviceBuffer d_fsz, d_nc, d_tier, d_off, d_uoff, d_fptr, d_frows, d_cptr, d_cidx, d_aptr,

@@ -411,6 +411,8 @@ int cupf_solver_get_stats(cupf_solver* solver, cupf_solver_stats* out) {
         out->last_warm_iterations = s.last_warm_iterations;
         out->last_warm_fallbacks = s.last_warm_fallbacks;
         out->state_version = s.state_version;
+        out->dense_kind = s.dense_kind;
+        out->graph_launches = s.graph_launches;
     });
 }
 

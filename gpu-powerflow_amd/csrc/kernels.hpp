@@ -33,6 +33,12 @@ struct NrDevice {
     const int* map22 = nullptr;
     const int* row_bus = nullptr;     // J row -> bus [dim]
     const unsigned char* row_is_q = nullptr;  // J row -> 1 for a magnitude row
+    // set and nonzero: the residual and the voltage update do nothing (the
+    // Jacobian fill, which writes scratch only, runs regardless)
+    const int* gate = nullptr;
+    // FP32 profile: residual, Jacobian and voltage update computed in FP32,
+    // every value rounded to FP32 where it is stored (the buffers stay FP64)
+    int fp32_state = 0;
 };
 
 // ---- boundary layout changes ([B][stride] <-> [rows][Bp]) ------------------
@@ -47,9 +53,12 @@ void from_interleaved(const double* src, int rows, int B, int Bp, double* dst,
 
 // ---- NR stages (all arrays interleaved) ------------------------------------
 // Va = arg V, Vm = |V| (cuPF storage_convert.cu:22-38 semantics).
-void seed_polar(int n, int Bp, const double2* V, double* Va, double* Vm, hipStream_t s);
+void seed_polar(int n, int Bp, const double2* V, double* Va, double* Vm, hipStream_t s,
+                bool fp32_state = false);
 // normF must be zeroed first (zero_norm); per-scenario max |F| via atomics.
-void zero_norm(double* normF, int Bp, hipStream_t s);
+// `gate` (here and below): a control word flag of a captured Newton step;
+// when set and nonzero at run time the launch does nothing.
+void zero_norm(double* normF, int Bp, hipStream_t s, const int* gate = nullptr);
 // flags[0..n) = 0 by a fill kernel (capturable; no memset node)
 void zero_flags(int* flags, int n, hipStream_t s);
 void residual(const NrDevice& d, int Bp, const double2* V, const double2* S, double* F,
@@ -67,11 +76,31 @@ template <typename TX>
 void refine_residual(const int* rp, const int* ci, const int* vmap, const double* J, int dim,
                      int Bp, const double* F, const TX* x, double* r, hipStream_t s);
 template <typename TX>
-void axpy_update(const TX* ddx, TX* dx, long long count, hipStream_t s);
+void axpy_update(const TX* ddx, TX* dx, long long count, hipStream_t s, const int* gate = nullptr);
 // max over scenarios b < B of normF[b] -> out[0]
-void batch_max(const double* normF, int B, double* out, hipStream_t s);
+void batch_max(const double* normF, int B, double* out, hipStream_t s, const int* gate = nullptr);
 // the same over the finite entries only; -1 when none is finite
-void batch_max_finite(const double* normF, int B, double* out, hipStream_t s);
+void batch_max_finite(const double* normF, int B, double* out, hipStream_t s,
+                      const int* gate = nullptr);
+// Control word of a Newton loop run on the device (a captured solve), five
+// ints:
+//   ctl[0] stop: the gate of every launch of a refined step;
+//   ctl[1] 1 unless the next step wants refinement: the gate of the
+//          refinement launches;
+//   ctl[2] residual evaluations so far;
+//   ctl[3] stop or the next step wants refinement: the gate of every launch
+//          of an unrefined step (a step captured without refinement never
+//          stands in for one that wants it; a later refined step does);
+//   ctl[4] bit i set when the step after evaluation i + 1 wanted refinement.
+// Called after each residual's batch max (out[0] of batch_max /
+// batch_max_finite): counts the evaluation and stops when the max is within
+// tol, when it is not finite (plain loop) or negative (the tolerant pass: no
+// finite scenario), or when `limit` evaluations are done; otherwise the next
+// step wants refinement when refine > 0 and the max is at most refine_below.
+// `first` starts the count; otherwise a stopped word stays, and so does the
+// word when `gate` (the gate of the step just enqueued) is set.
+void nr_control(const double* bmax, double tol, double refine_below, int refine, bool tolerant,
+                int limit, bool first, int* ctl, hipStream_t s, const int* gate = nullptr);
 
 // Torch bridge (cuPF torch_bridge_kernels.cu:116-168): S = base - load and V0
 // from a batch-shared (Va, Vm); outputs Va/Vm cast to the tensor dtype.
@@ -128,6 +157,10 @@ struct LuDevice {
     const int* rhs_idx = nullptr;
     const int* asm_cptr = nullptr;    // chunk maps of the lists: (src, loc, rr, len) quads
     const int4* asm_chunks = nullptr;
+    // Device-side stop of a captured Newton step: when set and nonzero, the
+    // factor kernels set no singular-pivot flag (their fronts and work
+    // vectors are scratch; the step's state writes are gated elsewhere).
+    const int* gate = nullptr;
 };
 
 // Most pivots of a tier-1 front (the tier-1 factor kernel's register rows).

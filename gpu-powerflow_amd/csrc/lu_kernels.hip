@@ -183,6 +183,12 @@ __device__ inline void assemble_chunks(const LuDevice& d, const Arenas<T>& ar, i
     }
 }
 
+// A captured Newton step's launches carry its gate (the control word's flag,
+// LuDevice::gate): the factor kernels, whose fronts and work vectors are
+// scratch, run whatever it says, but set a singular-pivot flag only while
+// the gate is open (read only when a flag is to be set).
+__device__ __forceinline__ bool gate_closed(const int* gate) { return gate && *gate; }
+
 // ===========================================================================
 // Factor, tier 0 (lane per scenario)
 // ===========================================================================
@@ -222,7 +228,7 @@ __global__ __launch_bounds__(kLanes) void k_factor_t0(LuDevice d, const int* __r
             for (int j = k + 1; j < fs; ++j) G[(i * fs + j) * s] -= l * G[(k * fs + j) * s];
         }
     }
-    if (flag) atomicOr(sing + b, 1);
+    if (flag && !gate_closed(d.gate)) atomicOr(sing + b, 1);
 }
 
 // ===========================================================================
@@ -303,7 +309,7 @@ __global__ __launch_bounds__(kFactorBlock) void k_factor_t1(LuDevice d,
             for (int c = 0; c < kT1MaxK; ++c)
                 if (c < ncp) F[lane * fs + c] = row[c];
         }
-        if (flag) atomicOr(sing + b, 1);
+        if (flag && !gate_closed(d.gate)) atomicOr(sing + b, 1);
     }
     __syncthreads();
     if (tid < 128) {
@@ -498,7 +504,7 @@ __global__ __launch_bounds__(kFactorBlock) void k_factor_t2(LuDevice d,
         }
         __syncthreads();
     }
-    if (flag) atomicOr(sing + b, 1);
+    if (flag && !gate_closed(d.gate)) atomicOr(sing + b, 1);
 }
 
 // ===========================================================================
@@ -570,6 +576,7 @@ __global__ __launch_bounds__(kAsmThreads) void k_assemble_rows(LuDevice d,
 // register v of lane l holds row (v/4)*8 + (l/32)*4 + v%4, column l%32.
 // ===========================================================================
 typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef double doublex4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 constexpr int kMfmaF32 = 1;
@@ -843,7 +850,7 @@ __global__ __launch_bounds__(kDenseThreads) void k_factor_t2_mfma(LuDevice d,
             __syncthreads();
         }
     }
-    if (flag) atomicOr(sing + b, 1);
+    if (flag && !gate_closed(d.gate)) atomicOr(sing + b, 1);
 }
 
 // ===========================================================================
@@ -960,7 +967,7 @@ __global__ __launch_bounds__(pivot_threads<T>()) void k_pivot(LuDevice d, const 
         // done): PT[c][r] = (L\U)[r][c], zero outside K x K
 #pragma unroll
         for (int c = 0; c < kXMaxK; ++c) P[c * kPivLd + lane] = row[c];
-        if (flag) atomicOr(sing + b, 1);
+        if (flag && !gate_closed(d.gate)) atomicOr(sing + b, 1);
     }
     __syncthreads();
     if (wave == 0) {
@@ -1014,8 +1021,8 @@ __global__ __launch_bounds__(pivot_threads<T>()) void k_pivot(LuDevice d, const 
 
 // U12 = L11^-1 A12 for one chunk of kXChunk columns of one (front, scenario):
 // A12 chunk assembled in LDS (entries, then the chunk map),
-// L11^-1 from the side buffer, the product on MFMA (FP32) or vector FMA tiles
-// (FP64), written to the front's pivot rows.
+// L11^-1 from the side buffer, the product on FP32 or FP64 MFMA, written to
+// the front's pivot rows.
 template <typename T>
 __device__ __forceinline__ void u12_chunk(const LuDevice& d, const int* __restrict__ fronts,
                                           const double* __restrict__ A, int Bp,
@@ -1075,20 +1082,35 @@ __device__ __forceinline__ void u12_chunk(const LuDevice& d, const int* __restri
                 }
             }
         } else {
-            // FP64 on the vector units: row i = tid / 4, 16 columns per thread
-            const int i = tid >> 2, cb = (tid & 3) * 16;
-            if (i < K && cb < jn) {
-                T acc[16];
+            // FP64 MFMA: 64 x 64 output = 4 x 4 tiles of 16 x 16, wave w the
+            // 16-row strip w; L11^-1 is lower (zero above the diagonal, unit
+            // beyond K, where the chunk rows are zero), so k < 16 w + 16
+            if (16 * wave < K) {                           // wave-uniform
+                doublex4 acc[4];
 #pragma unroll
-                for (int c = 0; c < 16; ++c) acc[c] = T(0);
-                for (int k = 0; k <= i; ++k) {  // L11^-1 lower
-                    const T l = Li[i * kLinvLd + k];
+                for (int c = 0; c < 4; ++c) acc[c] = doublex4{0.0, 0.0, 0.0, 0.0};
+                const int r = 16 * wave + (lane & 15), q = lane >> 4;
+                const int kend = min(K, 16 * wave + 16);
+                for (int kk = 0; kk < kend; kk += 4) {
+                    const int k = kk + q;
+                    const double av = Li[r * kLinvLd + k];
 #pragma unroll
-                    for (int c = 0; c < 16; ++c) acc[c] += l * Ch[k * kXCld + cb + c];
+                    for (int c = 0; c < 4; ++c) {
+                        if (16 * c < jn) {                 // uniform
+                            const double bv = Ch[k * kXCld + 16 * c + (lane & 15)];
+                            acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[c], 0, 0, 0);
+                        }
+                    }
                 }
 #pragma unroll
-                for (int c = 0; c < 16; ++c)
-                    if (cb + c < jn) G[(long long)i * fs + j0 + cb + c] = acc[c];
+                for (int c = 0; c < 4; ++c) {
+                    const int jj = 16 * c + (lane & 15);
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int i = 16 * wave + q + 4 * v;
+                        if (i < K && jj < jn) G[(long long)i * fs + j0 + jj] = acc[c][v];
+                    }
+                }
             }
         }
     }
@@ -1360,7 +1382,6 @@ __global__ __launch_bounds__(kSchurThreads, 2) void k_schur(LuDevice d, const in
 // scenarios interleaved (they share the J values' cache lines), so a
 // front's L21 rows and U12 columns are re-read from that XCD's L2.
 // ===========================================================================
-typedef double doublex4 __attribute__((ext_vector_type(4)));
 
 constexpr int kTileRows = kSchurTileRows;
 constexpr int kTileCols = kSchurTileCols;
@@ -2320,7 +2341,8 @@ __global__ __launch_bounds__(kSingleThreads) void k_single_factor(
         __syncthreads();
     }
     for (long long e = tid; e < sd.total; e += kSingleThreads) store[e] = F[e];
-    if (flag && tid == 0) atomicOr(sing, 1);                  // flag is wave 0's (pivot block)
+    // the flag is wave 0's (pivot block)
+    if (flag && tid == 0 && !gate_closed(d.gate)) atomicOr(sing, 1);
 }
 
 template <typename T, bool TRANS, typename TX>

@@ -204,6 +204,11 @@ __global__ __launch_bounds__(256) void k_adamw(long long n, float* __restrict__ 
     if (o >= 0) wbf[o] = f2bf(wi);
 }
 
+__global__ __launch_bounds__(256) void k_fill_u16(long long n, uint16_t* __restrict__ out) {
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) out[i] = 0;
+}
+
 __global__ __launch_bounds__(256) void k_to_bf16(long long n, const float* __restrict__ in,
                                                  const long long* __restrict__ bf_map,
                                                  uint16_t* __restrict__ out) {
@@ -390,6 +395,12 @@ void adamw(long long n, float* w, const float* g, float* m, float* v, float lr, 
     const float bc1 = 1.f - std::pow(b1, float(step)), bc2 = 1.f - std::pow(b2, float(step));
     hipLaunchKernelGGL(k_adamw, grid1(n), dim3(256), 0, s, n, w, g, m, v, lr, b1, b2, eps, wd, bc1, bc2,
                        bf_map, wbf);
+    CUPF_HIP(hipGetLastError());
+}
+
+void zero_u16(long long n, uint16_t* out, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_fill_u16, grid1(n), dim3(256), 0, s, n, out);
     CUPF_HIP(hipGetLastError());
 }
 

@@ -35,6 +35,8 @@ void colsum_bf16(int M, int N, const uint16_t* in, int ld, float* out, hipStream
 void adamw(long long n, float* w, const float* g, float* m, float* v, float lr, float b1, float b2,
            float eps, float wd, int step, const long long* bf_map, uint16_t* wbf, hipStream_t s);
 void to_bf16(long long n, const float* in, const long long* bf_map, uint16_t* out, hipStream_t s);
+// out[0..n) = 0 by a fill kernel (capturable; no memset node)
+void zero_u16(long long n, uint16_t* out, hipStream_t s);
 
 // Data set: x = [Re S | Im S], y = [Va | Vm] per scenario; rows of width
 // w (2n rounded up to a multiple of 8, the extra columns zero), FP32.

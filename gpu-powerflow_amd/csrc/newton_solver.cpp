@@ -18,6 +18,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <stdexcept>
 #include <string>
 
@@ -73,6 +74,8 @@ struct NewtonSolver::Impl {
     int capBp = 0, capB = 0;
     int B = 0, Bp = 0;
     DeviceBuffer V, Va, Vm, S, F, normF, bmax, J, dx, ddx, lam, g, stage_a, stage_b, stage_c;
+    // control word of the device-run Newton loop (kern::nr_control)
+    DeviceBuffer ctl;
 
     // learned warm start (not owned)
     WarmStartMLP* warm = nullptr;
@@ -96,28 +99,54 @@ struct NewtonSolver::Impl {
         ~CallStream() { m.s = m.own; }
     };
 
-    // captured Newton step (use_cuda_graph): Jacobian, factor, solve with
-    // refinement, voltage update, then the next residual and its batch max.
-    // Captured on the private stream, launched on the call's stream; the key
-    // names everything the captured launches bake in.
+    // Device-run Newton loop (use_cuda_graph).  A captured solve holds the
+    // head (the warm-start inference, when a network is attached) and the
+    // residual at the initial V with its batch max and stop test, then the
+    // Newton steps the last solve of the same kind took (Jacobian, factor,
+    // solve, voltage update, the next residual, batch max, stop test), each
+    // captured with or without the refinement launches as that solve's step
+    // wanted them.  The launches that change the state (voltage update,
+    // residual, norms, batch max, stop test, refinement update, singular
+    // flags) are gated on the device by the control word; the Jacobian,
+    // factor and solve launches write scratch and run regardless, so their
+    // kernels pay no gate.  A step past convergence thus leaves the state as
+    // it is, a step captured without refinement yields to a later refined one
+    // when the device wants refinement, and the host reads the control word
+    // once per solve instead of the batch max once per iteration.  Steps
+    // beyond the captured ones run as one captured step each, refined or not
+    // as the control word read back says.  Captured on the private stream,
+    // launched on the call's stream; the keys name everything the captured
+    // launches bake in.
     struct GraphKey {
-        int B = -1, Bp = -1, tolerant = -1, refine = -1;
+        int B = -1, Bp = -1, tolerant = -1, refine = -1, limit = -1, steps = -1;
+        unsigned pattern = 0;            // bit k: step k captured with refinement
+        double tol = -1.0;
         const void* yb = nullptr;
-        unsigned long long gen = 0;
+        const void* head = nullptr;      // warm-start network of the head, if any
+        unsigned long long gen = 0, head_gen = 0;
         bool operator==(const GraphKey& o) const {
             return B == o.B && Bp == o.Bp && tolerant == o.tolerant && refine == o.refine &&
-                   yb == o.yb && gen == o.gen;
+                   limit == o.limit && steps == o.steps && pattern == o.pattern &&
+                   tol == o.tol && yb == o.yb && head == o.head && gen == o.gen &&
+                   head_gen == o.head_gen;
         }
     };
-    hipGraphExec_t step_graphs[2] = {nullptr, nullptr};   // without / with refinement
-    GraphKey graph_keys[2];
+    hipGraphExec_t solve_graphs[2] = {nullptr, nullptr};  // plain loop / tolerant warm pass
+    GraphKey solve_keys[2];
+    hipGraphExec_t step_graphs[2][2] = {};   // one step: [tolerant][refined]
+    GraphKey step_keys[2][2];
+    int pred_steps[2] = {0, 0};       // steps the last device-run loop of each kind took
+    unsigned pred_pattern[2] = {0, 0};  // and which of them wanted refinement
     unsigned long long buf_gen = 1;   // bumped whenever a device buffer the step uses moves
-    bool graph_warm = false;          // one eager step first (first-use kernel attributes)
+    bool graph_warm = false;          // first loop eager (first-use kernel attributes)
+    const void* head_warm = nullptr;  // the last head run eagerly (its first use likewise)
     bool capturing = false;
+    bool count_factor = true;         // factor() counts itself (not inside the device loop)
     void drop_graph() {
-        for (auto& gr : step_graphs) {
-            if (gr) (void)hipGraphExecDestroy(gr);
-            gr = nullptr;
+        for (hipGraphExec_t* gr : {&solve_graphs[0], &solve_graphs[1], &step_graphs[0][0],
+                                   &step_graphs[0][1], &step_graphs[1][0], &step_graphs[1][1]}) {
+            if (*gr) (void)hipGraphExecDestroy(*gr);
+            *gr = nullptr;
         }
     }
 
@@ -154,6 +183,7 @@ struct NewtonSolver::Impl {
             F.allocate(nb * js.dim * sizeof(double));
             normF.allocate(nb * sizeof(double));
             bmax.allocate(sizeof(double));
+            ctl.reserve(8 * sizeof(int));
             J.allocate(nb * js.nnz * sizeof(double));
             dx.allocate(nb * js.dim * xsize());
             ddx.allocate(nb * js.dim * xsize());
@@ -173,7 +203,7 @@ struct NewtonSolver::Impl {
     // ---- factor the current J values ------------------------------------
     void factor() {
         lu.factor(J.as<double>(), Bp, s, &timer, kStFactorDense);
-        if (!capturing) ++st.factorizations;   // a captured step counts at each launch
+        if (count_factor) ++st.factorizations;   // the device loop counts its steps itself
     }
 
     // rhs (FP64, [dim][Bp]) -> x, with FP64 iterative refinement when the
@@ -210,7 +240,7 @@ struct NewtonSolver::Impl {
 
     void eval_residual() {
         StageScope st_(timer, kStResidual, s);
-        kern::zero_norm(normF.as<double>(), Bp, s);
+        kern::zero_norm(normF.as<double>(), Bp, s, nr.gate);
         kern::residual(nr, Bp, V.as<double2>(), S.as<double2>(), F.as<double>(),
                        normF.as<double>(), s);
     }
@@ -232,11 +262,17 @@ struct NewtonSolver::Impl {
     // NR loop (reference newton_solver.cpp:497-558 order: residual and norm,
     // then Jacobian, factor, solve, update).  tolerant: the warm-start pass --
     // the test runs over the finite norms only, non-finite scenarios are left
-    // to the caller, and at most `budget` iterations run.
-    int run_nr(const NRConfig& cfg, bool tolerant = false, int budget = -1) {
+    // to the caller, and at most `budget` iterations run.  head: work that
+    // sets up the initial V (the warm-start inference), run first; head_id /
+    // head_gen name what it bakes in (the network and its buffers).
+    int run_nr(const NRConfig& cfg, bool tolerant = false, int budget = -1,
+               const std::function<void()>& head = {}, const void* head_id = nullptr,
+               unsigned long long head_gen = 0) {
         lu.clear_singular(Bp, s);
         const int limit = budget >= 0 ? std::min(budget, cfg.max_iter) : cfg.max_iter;
-        if (opt.use_cuda_graph && !timer.on && limit > 0) return run_nr_graph(cfg, tolerant, limit);
+        if (opt.use_cuda_graph && !timer.on && limit > 0)
+            return run_nr_device(cfg, tolerant, limit, head, head_id, head_gen);
+        if (head) head();
         int completed = 0;
         for (int it = 0; it < limit; ++it) {
             completed = it + 1;
@@ -260,96 +296,168 @@ struct NewtonSolver::Impl {
         return completed;
     }
 
-    // Stop test on a batch max: true when the loop ends here.
-    bool nr_stop(double m, const NRConfig& cfg, bool tolerant) const {
-        if (tolerant) {
-            if (m < 0.0) return true;               // nothing finite left
-        } else if (!std::isfinite(m)) {
-            throw std::runtime_error("NewtonSolver: mismatch norm is not finite");
-        }
-        return m <= cfg.tolerance;
-    }
-
     void enqueue_batch_max(bool tolerant) {
-        if (tolerant) kern::batch_max_finite(normF.as<double>(), B, bmax.as<double>(), s);
-        else kern::batch_max(normF.as<double>(), B, bmax.as<double>(), s);
+        if (tolerant) kern::batch_max_finite(normF.as<double>(), B, bmax.as<double>(), s, nr.gate);
+        else kern::batch_max(normF.as<double>(), B, bmax.as<double>(), s, nr.gate);
     }
 
-    double read_bmax() {
-        double h = 0.0;
-        CUPF_HIP(hipMemcpyAsync(&h, bmax.get(), sizeof(double), hipMemcpyDeviceToHost, s));
-        CUPF_HIP(hipStreamSynchronize(s));
-        return h;
-    }
-
-    // One Newton step and the residual it leads to (the captured body).
-    void step_and_residual(bool tolerant) {
-        eval_jacobian();
-        newton_step();
-        eval_residual();
-        enqueue_batch_max(tolerant);
-    }
-
-    // NR loop on a captured step (reference graph_iteration.hpp:85-146
-    // semantics): the residual at the initial V eagerly, then one graph
-    // launch per Newton step (step + next residual + device batch max) and a
-    // readback of the max; at most limit - 1 steps, completed = 1 + steps.
-    int run_nr_graph(const NRConfig& cfg, bool tolerant, int limit) {
-        eval_residual();
-        enqueue_batch_max(tolerant);
-        double m = read_bmax();
-        if (nr_stop(m, cfg, tolerant)) return 1;
-        int completed = 1;
-        for (int step = 0; step + 1 < limit; ++step) {
-            refine_now = refine_for(m);
-            if (!graph_warm) {
-                step_and_residual(tolerant);        // eager: first-use kernel set-up
-                graph_warm = true;
-            } else {
-                const GraphKey key{B, Bp, tolerant ? 1 : 0, refine_now, nr.yb, buf_gen};
-                hipGraphExec_t& step_graph = step_graphs[refine_now > 0 ? 1 : 0];
-                GraphKey& graph_key = graph_keys[refine_now > 0 ? 1 : 0];
-                if (!step_graph || !(key == graph_key)) {
-                    if (step_graph) (void)hipGraphExecDestroy(step_graph);
-                    step_graph = nullptr;
-                    const hipStream_t call = s;
-                    s = own;                        // capture on the private stream
-                    capturing = true;
-                    hipGraph_t g = nullptr;
-                    hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
-                    if (e == hipSuccess) {
-                        try {
-                            step_and_residual(tolerant);
-                        } catch (...) {             // end the capture, restore, rethrow
-                            (void)hipStreamEndCapture(s, &g);
-                            if (g) (void)hipGraphDestroy(g);
-                            capturing = false;
-                            s = call;
-                            throw;
-                        }
-                        e = hipStreamEndCapture(s, &g);
-                    }
-                    capturing = false;
-                    s = call;
-                    if (e != hipSuccess && g) (void)hipGraphDestroy(g);
-                    CUPF_HIP(e);
-                    const hipError_t ei = hipGraphInstantiate(&step_graph, g, nullptr, nullptr, 0);
-                    (void)hipGraphDestroy(g);
-                    CUPF_HIP(ei);
-                    graph_key = key;
-                }
-                CUPF_HIP(hipGraphLaunch(step_graph, s));
-                ++st.factorizations;
-            }
-            completed = step + 2;
-            m = read_bmax();
-            if (nr_stop(m, cfg, tolerant)) break;
+    // The state-changing launches enqueued while a Gated is alive carry the
+    // control word's flags: a refined step's under ctl[0] and its refinement
+    // update under ctl[1], an unrefined step's under ctl[3].
+    struct Gated {
+        Impl& m;
+        Gated(Impl& impl, bool refined) : m(impl) {
+            int* c = m.ctl.as<int>();
+            m.nr.gate = refined ? c : c + 3;
+            m.lu.set_gates(m.nr.gate, c + 1);
         }
-        return completed;
+        ~Gated() {
+            m.nr.gate = nullptr;
+            m.lu.set_gates(nullptr, nullptr);
+        }
+    };
+
+    // Head of a device-run loop: initial V, residual, batch max, stop test.
+    void loop_head(const std::function<void()>& head, const NRConfig& cfg, bool tolerant,
+                   int limit) {
+        if (head) head();
+        eval_residual();
+        enqueue_batch_max(tolerant);
+        kern::nr_control(bmax.as<double>(), cfg.tolerance, kRefineBelow, refine, tolerant, limit,
+                         true, ctl.as<int>(), s);
+    }
+
+    // One gated Newton step and the stop test on its residual.  A refined
+    // step is enqueued with the configured refinement steps, whose update
+    // ctl[1] skips on the device while the mismatch is above kRefineBelow;
+    // an unrefined step has none, and ctl[3] keeps its update, residual and
+    // stop test from running when the device wants refinement.
+    void loop_step(const NRConfig& cfg, bool tolerant, int limit, bool refined) {
+        refined = refined && refine > 0;
+        const int* step_gate = nullptr;
+        {
+            Gated gate(*this, refined);
+            step_gate = nr.gate;
+            refine_now = refined ? refine : 0;
+            eval_jacobian();
+            newton_step();
+            eval_residual();
+            enqueue_batch_max(tolerant);
+        }
+        kern::nr_control(bmax.as<double>(), cfg.tolerance, kRefineBelow, refine, tolerant, limit,
+                         false, ctl.as<int>(), s, step_gate);
+    }
+
+    // Control word and batch max of the loop so far (one readback).
+    void read_control(int c[5], double& m) {
+        CUPF_HIP(hipMemcpyAsync(c, ctl.get(), 5 * sizeof(int), hipMemcpyDeviceToHost, s));
+        CUPF_HIP(hipMemcpyAsync(&m, bmax.get(), sizeof(double), hipMemcpyDeviceToHost, s));
+        CUPF_HIP(hipStreamSynchronize(s));
+    }
+
+    // Capture body() on the private stream into an executable graph.
+    hipGraphExec_t capture(const std::function<void()>& body) {
+        const hipStream_t call = s;
+        s = own;
+        capturing = true;
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+            try {
+                body();
+            } catch (...) {                     // end the capture, restore, rethrow
+                (void)hipStreamEndCapture(s, &g);
+                if (g) (void)hipGraphDestroy(g);
+                capturing = false;
+                s = call;
+                throw;
+            }
+            e = hipStreamEndCapture(s, &g);
+        }
+        capturing = false;
+        s = call;
+        if (e != hipSuccess && g) (void)hipGraphDestroy(g);
+        CUPF_HIP(e);
+        hipGraphExec_t x = nullptr;
+        const hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        CUPF_HIP(ei);
+        return x;
+    }
+
+    // Device-run NR loop (the captured form of the reference's graph
+    // iteration, graph_iteration.hpp:85-146): the residual at the initial V
+    // and at most limit - 1 steps; completed = 1 + steps.  The first loop on
+    // a solver runs the same launches eagerly (first-use kernel attributes
+    // are set outside capture).
+    int run_nr_device(const NRConfig& cfg, bool tolerant, int limit,
+                      const std::function<void()>& head, const void* head_id,
+                      unsigned long long head_gen) {
+        struct CountBack {
+            bool& f;
+            ~CountBack() { f = true; }
+        } restore{count_factor};
+        count_factor = false;
+        const int ti = tolerant ? 1 : 0;
+        int c[5] = {0, 0, 0, 0, 0};
+        double m = 0.0;
+        if (!graph_warm || (head_id && head_id != head_warm)) {
+            loop_head(head, cfg, tolerant, limit);
+            read_control(c, m);
+            while (!c[0]) {
+                loop_step(cfg, tolerant, limit, c[1] == 0);
+                read_control(c, m);
+            }
+            graph_warm = true;
+            if (head_id) head_warm = head_id;
+        } else {
+            const int steps = std::min(pred_steps[ti], limit - 1);
+            const unsigned pattern =
+                refine > 0 ? pred_pattern[ti] & (steps >= 32 ? ~0u : (1u << steps) - 1u) : 0u;
+            GraphKey key{B, Bp, ti, refine, limit, steps, pattern, cfg.tolerance, nr.yb, head_id,
+                         buf_gen, head_gen};
+            if (!solve_graphs[ti] || !(key == solve_keys[ti])) {
+                if (solve_graphs[ti]) (void)hipGraphExecDestroy(solve_graphs[ti]);
+                solve_graphs[ti] = nullptr;
+                solve_graphs[ti] = capture([&] {
+                    loop_head(head, cfg, tolerant, limit);
+                    for (int k = 0; k < steps; ++k)
+                        loop_step(cfg, tolerant, limit, k < 32 && ((pattern >> k) & 1u));
+                });
+                solve_keys[ti] = key;
+            }
+            CUPF_HIP(hipGraphLaunch(solve_graphs[ti], s));
+            ++st.graph_launches;
+            read_control(c, m);
+            key.steps = 1;
+            key.head = nullptr;
+            key.head_gen = 0;
+            while (!c[0]) {
+                const int ri = c[1] == 0 && refine > 0 ? 1 : 0;
+                key.pattern = unsigned(ri);
+                hipGraphExec_t& sg = step_graphs[ti][ri];
+                if (!sg || !(key == step_keys[ti][ri])) {
+                    if (sg) (void)hipGraphExecDestroy(sg);
+                    sg = nullptr;
+                    sg = capture([&] { loop_step(cfg, tolerant, limit, ri == 1); });
+                    step_keys[ti][ri] = key;
+                }
+                CUPF_HIP(hipGraphLaunch(sg, s));
+                ++st.graph_launches;
+                read_control(c, m);
+            }
+        }
+        st.factorizations += c[2] - 1;
+        pred_steps[ti] = c[2] - 1;
+        pred_pattern[ti] = unsigned(c[4]);
+        if (!tolerant && !std::isfinite(m))
+            throw std::runtime_error("NewtonSolver: mismatch norm is not finite");
+        return c[2];
     }
 
     void ensure_warm(int batch) {
         if (batch <= warm_cap) return;
+        ++buf_gen;
         const size_t rows = size_t(batch) * n * sizeof(double2);
         warm_v0.allocate(rows);
         warm_s.allocate(rows);
@@ -390,7 +498,8 @@ struct NewtonSolver::Impl {
         load_ybus_rows(Yr, ys);
         if (!warm) {
             kern::to_interleaved(V0r, vs, n, B, Bp, V.as<double2>(), s);
-            kern::seed_polar(n, Bp, V.as<double2>(), Va.as<double>(), Vm.as<double>(), s);
+            kern::seed_polar(n, Bp, V.as<double2>(), Va.as<double>(), Vm.as<double>(), s,
+                             nr.fp32_state != 0);
             const int it = run_nr(cfg);
             if (cache_req) prepare_cache();
             kern::from_interleaved(V.as<double2>(), n, B, Bp, Vo, n, s);
@@ -400,13 +509,32 @@ struct NewtonSolver::Impl {
             return;
         }
         ensure_warm(Ball);
-        {
-            StageScope st_(timer, kStWarmStart, s);
-            warm->predict_v0(Sr, ss, V0r, vs, B, warm_v0.as<double2>(), n, s);
+        // The inference is the head of the loop, so a device-run loop
+        // captures it with the steps; it then reads the caller's rows from
+        // solver-owned copies, whose addresses the capture may bake in.
+        const double2* Sh = Sr;
+        const double2* Rh = V0r;
+        int64_t sh = ss, rh = vs;
+        if (opt.use_cuda_graph && !timer.on) {
+            const size_t row = size_t(n) * sizeof(double2);
+            CUPF_HIP(hipMemcpy2DAsync(warm_s.get(), row, Sr, size_t(ss) * sizeof(double2), row, B,
+                                      hipMemcpyDeviceToDevice, s));
+            CUPF_HIP(hipMemcpy2DAsync(warm_r.get(), row, V0r, size_t(vs) * sizeof(double2), row, B,
+                                      hipMemcpyDeviceToDevice, s));
+            Sh = warm_s.as<double2>();
+            Rh = warm_r.as<double2>();
+            sh = rh = n;
         }
-        kern::to_interleaved(warm_v0.as<double2>(), n, n, B, Bp, V.as<double2>(), s);
-        kern::seed_polar(n, Bp, V.as<double2>(), Va.as<double>(), Vm.as<double>(), s);
-        const int itw = run_nr(cfg, true, warm_budget);
+        const auto head = [&] {
+            {
+                StageScope st_(timer, kStWarmStart, s);
+                warm->predict_v0(Sh, sh, Rh, rh, B, warm_v0.as<double2>(), n, s);
+            }
+            kern::to_interleaved(warm_v0.as<double2>(), n, n, B, Bp, V.as<double2>(), s);
+            kern::seed_polar(n, Bp, V.as<double2>(), Va.as<double>(), Vm.as<double>(), s,
+                             nr.fp32_state != 0);
+        };
+        const int itw = run_nr(cfg, true, warm_budget, head, warm, warm->buffers_generation());
         kern::from_interleaved(V.as<double2>(), n, B, Bp, Vo, n, s);
         download_norms(nf, sg);
         iters.assign(B, itw);
@@ -435,7 +563,8 @@ struct NewtonSolver::Impl {
             if (Yr) load_ybus_rows(warm_y.as<double2>(), nnzY);
             kern::to_interleaved(warm_s.as<double2>(), n, n, B, Bp, S.as<double2>(), s);
             kern::to_interleaved(warm_r.as<double2>(), n, n, B, Bp, V.as<double2>(), s);
-            kern::seed_polar(n, Bp, V.as<double2>(), Va.as<double>(), Vm.as<double>(), s);
+            kern::seed_polar(n, Bp, V.as<double2>(), Va.as<double>(), Vm.as<double>(), s,
+                             nr.fp32_state != 0);
             const int itf = run_nr(cfg);
             kern::from_interleaved(V.as<double2>(), n, B, Bp, warm_o.as<double2>(), n, s);
             std::vector<double> nff;
@@ -670,6 +799,7 @@ void NewtonSolver::initialize(const YbusView& ybus, const int32_t* pv, int32_t n
     nr.map11 = m.d_m11.as<int>(); nr.map21 = m.d_m21.as<int>();
     nr.map12 = m.d_m12.as<int>(); nr.map22 = m.d_m22.as<int>();
     nr.row_bus = m.d_row_bus.as<int>(); nr.row_is_q = m.d_row_isq.as<unsigned char>();
+    nr.fp32_state = m.opt.compute == ComputePolicy::FP32 ? 1 : 0;
 
     // Every buffer sized by the previous structure goes: batch buffers are
     // sized lazily for the new one, the per-scenario Ybus values and the
@@ -686,6 +816,7 @@ void NewtonSolver::initialize(const YbusView& ybus, const int32_t* pv, int32_t n
     m.warm_cap = 0;
     m.drop_graph();
     m.graph_warm = false;
+    m.head_warm = nullptr;
     ++m.buf_gen;
     m.st = NewtonSolverStats{};
     m.bump();
@@ -697,6 +828,7 @@ void NewtonSolver::initialize(const YbusView& ybus, const int32_t* pv, int32_t n
     m.st.dense_flops = m.lu.dense_flops();
     m.st.dense_bytes = m.lu.dense_bytes();
     m.st.dense_groups = m.lu.dense_groups();
+    m.st.dense_kind = m.lu.dense_kind();
     m.initialized = true;
 }
 

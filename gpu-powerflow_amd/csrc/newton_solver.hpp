@@ -18,10 +18,13 @@ struct NewtonSolverStats {
     int32_t dim = 0, nnz_j = 0, num_fronts = 0, max_front = 0, max_pivots = 0;
     int64_t arena_elems = 0, lnz = 0;
     double factor_flops = 0.0;
-    // tier-2 (dense) fronts: flops and minimum bytes per scenario, and the
-    // number of tier-2 launches per factorisation
+    // the timed dense kernel (2 contribution-block tiles, 1 row-block Schur,
+    // 0 tier-2 LU groups): flops and minimum bytes of its fronts per
+    // scenario, and its launches per factorisation
     double dense_flops = 0.0, dense_bytes = 0.0;
-    int32_t dense_groups = 0;
+    int32_t dense_groups = 0, dense_kind = 0;
+    // captured-graph launches since initialize() (use_cuda_graph)
+    int64_t graph_launches = 0;
     // learned warm start: iterations of the warm pass and scenarios that fell
     // back to their own V0 in the last solve
     int32_t last_warm_iterations = 0, last_warm_fallbacks = 0;

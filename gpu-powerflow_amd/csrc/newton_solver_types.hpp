@@ -45,7 +45,11 @@ enum class BackendKind { CPU, CUDA };
 // Mixed: FP64 state/residual/Jacobian; FP32 fronts (precision FP32) or
 //        FP32 fronts with 3xBF16 trailing products (precision TF32), with
 //        FP64 iterative refinement of the Newton step.
-// FP32 : FP64 state, FP32 Jacobian/fronts/step, no refinement.
+// FP32 : FP32 state (the reference's CudaFp32CustomPipeline,
+//        pipeline.hpp:255-299): Ybus, V, Sbus, Ibus, F, the Jacobian and the
+//        voltage update computed in FP32 and rounded to FP32 where stored;
+//        FP32 fronts and step, no refinement.  Its mismatch floors near
+//        FP32 rounding of the power terms (about 1e-5 on a 118-bus grid).
 enum class ComputePolicy { FP64, FP32, Mixed };
 
 enum class CudaLinearSolverKind { CuDSS, Custom };   // only Custom is built

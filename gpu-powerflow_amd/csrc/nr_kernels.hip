@@ -79,6 +79,12 @@ __global__ __launch_bounds__(kBlock) void k_from_inter(const E* __restrict__ src
     }
 }
 
+// TS (here and in the residual, Jacobian and voltage-update kernels): the
+// arithmetic type of the state.  double for the FP64 and Mixed profiles; float
+// for the FP32 profile, whose every state value is computed in FP32 and
+// rounded to FP32 where it is stored (held in the FP64 buffers, which widen
+// it exactly).
+template <typename TS>
 __global__ __launch_bounds__(kBlock) void k_seed_polar(long long count,
                                                        const double2* __restrict__ V,
                                                        double* __restrict__ Va,
@@ -86,11 +92,20 @@ __global__ __launch_bounds__(kBlock) void k_seed_polar(long long count,
     const long long i = (long long)blockIdx.x * kBlock + threadIdx.x;
     if (i >= count) return;
     const double2 v = V[i];
-    Va[i] = atan2(v.y, v.x);
-    Vm[i] = hypot(v.x, v.y);
+    const TS x = TS(v.x), y = TS(v.y);
+    Va[i] = double(TS(atan2(y, x)));
+    Vm[i] = double(TS(hypot(x, y)));
 }
 
-__global__ __launch_bounds__(kBlock) void k_zero(double* __restrict__ p, int n) {
+// A captured Newton step's state writes carry the control word's gate: the
+// launches that change the state (voltage update, residual and its norms,
+// batch max, refinement update) do nothing while it is set; the Jacobian,
+// factor and solve launches write scratch only and run regardless.
+__device__ __forceinline__ bool gate_closed(const int* gate) { return gate && *gate; }
+
+__global__ __launch_bounds__(kBlock) void k_zero(double* __restrict__ p, int n,
+                                                 const int* __restrict__ gate) {
+    if (gate_closed(gate)) return;
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i < n) p[i] = 0.0;
 }
@@ -100,26 +115,40 @@ __global__ __launch_bounds__(kBlock) void k_zero_i32(int* __restrict__ p, int n)
     if (i < n) p[i] = 0;
 }
 
+template <typename TS>
+struct C2 {
+    TS x, y;
+};
+
+template <typename TS>
+__device__ inline C2<TS> load2(const double2* __restrict__ p, long long i) {
+    const double2 v = p[i];
+    return {TS(v.x), TS(v.y)};
+}
+
 // Ibus_i for this lane's scenario, accumulated in increasing column order.
-__device__ inline double2 ibus_row(const NrDevice& d, int Bp, const double2* __restrict__ V,
-                                   int i, int b) {
-    double ir = 0.0, ii = 0.0;
+template <typename TS>
+__device__ inline C2<TS> ibus_row(const NrDevice& d, int Bp, const double2* __restrict__ V,
+                                  int i, int b) {
+    TS ir = 0, ii = 0;
     for (int k = d.yp[i]; k < d.yp[i + 1]; ++k) {
-        const double2 yk = d.yb ? d.yb[(long long)k * Bp + b] : d.y[k];
-        const double2 vk = V[(long long)d.yi[k] * Bp + b];
-        const double tr = yk.x * vk.x - yk.y * vk.y;
-        const double ti = yk.x * vk.y + yk.y * vk.x;
+        const C2<TS> yk = d.yb ? load2<TS>(d.yb, (long long)k * Bp + b) : load2<TS>(d.y, k);
+        const C2<TS> vk = load2<TS>(V, (long long)d.yi[k] * Bp + b);
+        const TS tr = yk.x * vk.x - yk.y * vk.y;
+        const TS ti = yk.x * vk.y + yk.y * vk.x;
         ir += tr;
         ii += ti;
     }
-    return make_double2(ir, ii);
+    return {ir, ii};
 }
 
+template <typename TS>
 __global__ __launch_bounds__(kBlock) void k_residual(NrDevice d, int Bp,
                                                      const double2* __restrict__ V,
                                                      const double2* __restrict__ S,
                                                      double* __restrict__ F,
                                                      double* __restrict__ normF) {
+    if (gate_closed(d.gate)) return;
     const int lane = threadIdx.x & (kLanes - 1), w = threadIdx.x / kLanes;
     const int b = blockIdx.y * kLanes + lane;
     const int i0 = blockIdx.x * kResBusPerBlock + w * kResBusPerWave;
@@ -130,15 +159,15 @@ __global__ __launch_bounds__(kBlock) void k_residual(NrDevice d, int Bp,
         if (i >= d.n_bus) break;
         const int r1 = d.row_pvpq[i];
         if (r1 < 0) continue;
-        const double2 I = ibus_row(d, Bp, V, i, b);
-        const double2 v = V[(long long)i * Bp + b];
-        const double2 sb = S[(long long)i * Bp + b];
+        const C2<TS> I = ibus_row<TS>(d, Bp, V, i, b);
+        const C2<TS> v = load2<TS>(V, (long long)i * Bp + b);
+        const C2<TS> sb = load2<TS>(S, (long long)i * Bp + b);
         // V * conj(I) - S
-        const double nI = -I.y;
-        const double pr = v.x * I.x - v.y * nI;
-        const double pi = v.x * nI + v.y * I.x;
-        const double fr = pr - sb.x;
-        const double fi = pi - sb.y;
+        const TS nI = -I.y;
+        const TS pr = v.x * I.x - v.y * nI;
+        const TS pi = v.x * nI + v.y * I.x;
+        const double fr = double(pr - sb.x);
+        const double fi = double(pi - sb.y);
         F[(long long)r1 * Bp + b] = fr;
         m = fmax(m, fabs(fr));
         nan |= (fr != fr);
@@ -161,6 +190,7 @@ __global__ __launch_bounds__(kBlock) void k_residual(NrDevice d, int Bp,
 
 // Row-owned Jacobian fill: for each pv/pq bus i, every slot of J rows
 // row_pvpq[i] and row_pq[i] is written exactly once.
+template <typename TS>
 __global__ __launch_bounds__(kBlock) void k_jacobian(NrDevice d, int Bp,
                                                      const double2* __restrict__ V,
                                                      double* __restrict__ J,
@@ -175,37 +205,37 @@ __global__ __launch_bounds__(kBlock) void k_jacobian(NrDevice d, int Bp,
         const int i = i0 + t;
         if (i >= d.n_bus) break;
         if (d.row_pvpq[i] < 0) continue;
-        const double2 I = ibus_row(d, Bp, V, i, b);
-        const double2 vi = V[(long long)i * Bp + b];
-        const double vmi = fmax(hypot(vi.x, vi.y), 1e-8);
-        const double2 vni = make_double2(vi.x / vmi, vi.y / vmi);
+        const C2<TS> I = ibus_row<TS>(d, Bp, V, i, b);
+        const C2<TS> vi = load2<TS>(V, (long long)i * Bp + b);
+        const TS vmi = fmax(hypot(vi.x, vi.y), TS(1e-8));
+        const C2<TS> vni{vi.x / vmi, vi.y / vmi};
         // diagonal self terms: j V_i conj(I_i) and conj(I_i) Vn_i
-        const double nI = -I.y;
-        const double sr = vi.x * I.x - vi.y * nI;
-        const double si = vi.x * nI + vi.y * I.x;
-        const double da_r = 0.0 * sr - 1.0 * si;
-        const double da_i = 0.0 * si + 1.0 * sr;
-        const double dm_r = I.x * vni.x - nI * vni.y;
-        const double dm_i = I.x * vni.y + nI * vni.x;
+        const TS nI = -I.y;
+        const TS sr = vi.x * I.x - vi.y * nI;
+        const TS si = vi.x * nI + vi.y * I.x;
+        const TS da_r = TS(0) * sr - TS(1) * si;
+        const TS da_i = TS(0) * si + TS(1) * sr;
+        const TS dm_r = I.x * vni.x - nI * vni.y;
+        const TS dm_i = I.x * vni.y + nI * vni.x;
         // -j V_i as the reference forms it: (-0 - 1j) * V_i
-        const double ja_r = -0.0 * vi.x - (-1.0) * vi.y;
-        const double ja_i = -0.0 * vi.y + (-1.0) * vi.x;
+        const TS ja_r = TS(-0.0) * vi.x - TS(-1) * vi.y;
+        const TS ja_i = TS(-0.0) * vi.y + TS(-1) * vi.x;
         for (int k = d.yp[i]; k < d.yp[i + 1]; ++k) {
             const int j = d.yi[k];
-            const double2 yk = d.yb ? d.yb[(long long)k * Bp + b] : d.y[k];
-            const double2 vj = V[(long long)j * Bp + b];
-            const double cr = yk.x * vj.x - yk.y * vj.y;
-            const double ci = yk.x * vj.y + yk.y * vj.x;
-            const double nci = -ci;
-            double va_r = ja_r * cr - ja_i * nci;
-            double va_i = ja_r * nci + ja_i * cr;
-            const double vmj = fmax(hypot(vj.x, vj.y), 1e-8);
-            const double vnr = vj.x / vmj, vnj = vj.y / vmj;
-            const double wr = yk.x * vnr - yk.y * vnj;
-            const double wi = yk.x * vnj + yk.y * vnr;
-            const double nwi = -wi;
-            double vm_r = vi.x * wr - vi.y * nwi;
-            double vm_i = vi.x * nwi + vi.y * wr;
+            const C2<TS> yk = d.yb ? load2<TS>(d.yb, (long long)k * Bp + b) : load2<TS>(d.y, k);
+            const C2<TS> vj = load2<TS>(V, (long long)j * Bp + b);
+            const TS cr = yk.x * vj.x - yk.y * vj.y;
+            const TS ci = yk.x * vj.y + yk.y * vj.x;
+            const TS nci = -ci;
+            TS va_r = ja_r * cr - ja_i * nci;
+            TS va_i = ja_r * nci + ja_i * cr;
+            const TS vmj = fmax(hypot(vj.x, vj.y), TS(1e-8));
+            const TS vnr = vj.x / vmj, vnj = vj.y / vmj;
+            const TS wr = yk.x * vnr - yk.y * vnj;
+            const TS wi = yk.x * vnj + yk.y * vnr;
+            const TS nwi = -wi;
+            TS vm_r = vi.x * wr - vi.y * nwi;
+            TS vm_i = vi.x * nwi + vi.y * wr;
             if (j == i) {
                 va_r = va_r + da_r;
                 va_i = va_i + da_i;
@@ -213,32 +243,33 @@ __global__ __launch_bounds__(kBlock) void k_jacobian(NrDevice d, int Bp,
                 vm_i = vm_i + dm_i;
             }
             const int p11 = d.map11[k], p21 = d.map21[k], p12 = d.map12[k], p22 = d.map22[k];
-            if (p11 >= 0) J[(long long)p11 * Bp + b] = va_r;
-            if (p21 >= 0) J[(long long)p21 * Bp + b] = va_i;
-            if (p12 >= 0) J[(long long)p12 * Bp + b] = vm_r;
-            if (p22 >= 0) J[(long long)p22 * Bp + b] = vm_i;
+            if (p11 >= 0) J[(long long)p11 * Bp + b] = double(va_r);
+            if (p21 >= 0) J[(long long)p21 * Bp + b] = double(va_i);
+            if (p12 >= 0) J[(long long)p12 * Bp + b] = double(vm_r);
+            if (p22 >= 0) J[(long long)p22 * Bp + b] = double(vm_i);
         }
     }
 }
 
-template <typename TX>
+template <typename TX, typename TS>
 __global__ __launch_bounds__(kBlock) void k_vupdate(NrDevice d, int Bp, const TX* __restrict__ dx,
                                                     double* __restrict__ Va,
                                                     double* __restrict__ Vm,
                                                     double2* __restrict__ V) {
+    if (gate_closed(d.gate)) return;
     const int lane = threadIdx.x & (kLanes - 1), w = threadIdx.x / kLanes;
     const int i = blockIdx.x * kWaves + w;
     const int b = blockIdx.y * kLanes + lane;
     if (i >= d.n_bus) return;
     const long long o = (long long)i * Bp + b;
-    double va = Va[o], vm = Vm[o];
+    TS va = TS(Va[o]), vm = TS(Vm[o]);
     const int r1 = d.row_pvpq[i];
-    if (r1 >= 0) va -= double(dx[(long long)r1 * Bp + b]);
+    if (r1 >= 0) va -= TS(dx[(long long)r1 * Bp + b]);
     const int r2 = d.row_pq[i];
-    if (r2 >= 0) vm -= double(dx[(long long)r2 * Bp + b]);
-    Va[o] = va;
-    Vm[o] = vm;
-    V[o] = make_double2(vm * cos(va), vm * sin(va));
+    if (r2 >= 0) vm -= TS(dx[(long long)r2 * Bp + b]);
+    Va[o] = double(va);
+    Vm[o] = double(vm);
+    V[o] = make_double2(double(vm * cos(va)), double(vm * sin(va)));
 }
 
 template <typename TX>
@@ -265,13 +296,16 @@ __global__ __launch_bounds__(kBlock) void k_refine_residual(const int* __restric
 
 template <typename TX>
 __global__ __launch_bounds__(kBlock) void k_axpy(const TX* __restrict__ a, TX* __restrict__ x,
-                                                 long long count) {
+                                                 long long count, const int* __restrict__ gate) {
+    if (gate_closed(gate)) return;
     const long long i = (long long)blockIdx.x * kBlock + threadIdx.x;
     if (i < count) x[i] += a[i];
 }
 
 __global__ __launch_bounds__(kBlock) void k_batch_max(const double* __restrict__ v, int B,
-                                                      double* __restrict__ out) {
+                                                      double* __restrict__ out,
+                                                      const int* __restrict__ gate) {
+    if (gate_closed(gate)) return;
     unsigned long long m = 0;
     for (int i = threadIdx.x; i < B; i += kBlock) {
         const unsigned long long bits = (unsigned long long)__double_as_longlong(v[i]);
@@ -289,7 +323,9 @@ __global__ __launch_bounds__(kBlock) void k_batch_max(const double* __restrict__
 
 // max over the finite entries of v[0..B); -1 when none is finite
 __global__ __launch_bounds__(kBlock) void k_batch_max_finite(const double* __restrict__ v, int B,
-                                                             double* __restrict__ out) {
+                                                             double* __restrict__ out,
+                                                             const int* __restrict__ gate) {
+    if (gate_closed(gate)) return;
     unsigned long long m = 0;
     int any = 0;
     for (int i = threadIdx.x; i < B; i += kBlock) {
@@ -312,6 +348,33 @@ __global__ __launch_bounds__(kBlock) void k_batch_max_finite(const double* __res
         __syncthreads();
     }
     if (threadIdx.x == 0) out[0] = anyr[0] ? __longlong_as_double((long long)red[0]) : -1.0;
+}
+
+// The Newton loop's stop test on the device (control word layout: see
+// nr_control in the kernel header).  Counts the residual evaluation just
+// made; stops on convergence (batch max within tol), on a non-finite max
+// (plain loop) or no finite scenario left (tolerant warm-start pass,
+// max < 0), or when `limit` evaluations are done; otherwise the next step
+// wants refinement when refine > 0 and the max is at most refine_below.
+// `first` starts a loop; otherwise a stopped loop stays as it is, and so
+// does one whose step did not run (its gate set).
+__global__ __launch_bounds__(64) void k_nr_control(const double* __restrict__ bmax, double tol,
+                                                   double refine_below, int refine, int tolerant,
+                                                   int limit, int first, int* __restrict__ ctl,
+                                                   const int* __restrict__ gate) {
+    if (threadIdx.x != 0) return;
+    if (!first && (ctl[0] || gate_closed(gate))) return;
+    const double m = bmax[0];
+    const int completed = (first ? 0 : ctl[2]) + 1;
+    bool stop = tolerant ? m < 0.0 : !isfinite(m);
+    stop = stop || m <= tol || completed >= limit;
+    const bool want = !stop && refine > 0 && m <= refine_below;
+    const int hist = first ? 0 : ctl[4];
+    ctl[0] = stop ? 1 : 0;
+    ctl[1] = want ? 0 : 1;
+    ctl[2] = completed;
+    ctl[3] = (stop || want) ? 1 : 0;
+    ctl[4] = (want && completed <= 31) ? (hist | (1 << (completed - 1))) : hist;
 }
 
 template <typename TI>
@@ -430,15 +493,20 @@ void from_interleaved(const double* src, int rows, int B, int Bp, double* dst,
     from_interleaved_t(src, rows, B, Bp, dst, dst_stride, s);
 }
 
-void seed_polar(int n, int Bp, const double2* V, double* Va, double* Vm, hipStream_t s) {
+void seed_polar(int n, int Bp, const double2* V, double* Va, double* Vm, hipStream_t s,
+                bool fp32_state) {
     const long long count = (long long)n * Bp;
-    hipLaunchKernelGGL(k_seed_polar, dim3(unsigned((count + kBlock - 1) / kBlock)), dim3(kBlock), 0,
-                       s, count, V, Va, Vm);
+    const dim3 grid(unsigned((count + kBlock - 1) / kBlock));
+    if (fp32_state)
+        hipLaunchKernelGGL(k_seed_polar<float>, grid, dim3(kBlock), 0, s, count, V, Va, Vm);
+    else
+        hipLaunchKernelGGL(k_seed_polar<double>, grid, dim3(kBlock), 0, s, count, V, Va, Vm);
     CUPF_HIP(hipGetLastError());
 }
 
-void zero_norm(double* normF, int Bp, hipStream_t s) {
-    hipLaunchKernelGGL(k_zero, dim3((Bp + kBlock - 1) / kBlock), dim3(kBlock), 0, s, normF, Bp);
+void zero_norm(double* normF, int Bp, hipStream_t s, const int* gate) {
+    hipLaunchKernelGGL(k_zero, dim3((Bp + kBlock - 1) / kBlock), dim3(kBlock), 0, s, normF, Bp,
+                       gate);
     CUPF_HIP(hipGetLastError());
 }
 
@@ -451,22 +519,34 @@ void zero_flags(int* flags, int n, hipStream_t s) {
 void residual(const NrDevice& d, int Bp, const double2* V, const double2* S, double* F,
               double* normF, hipStream_t s) {
     dim3 grid((d.n_bus + kResBusPerBlock - 1) / kResBusPerBlock, Bp / kLanes);
-    hipLaunchKernelGGL(k_residual, grid, dim3(kBlock), 0, s, d, Bp, V, S, F, normF);
+    if (d.fp32_state)
+        hipLaunchKernelGGL(k_residual<float>, grid, dim3(kBlock), 0, s, d, Bp, V, S, F, normF);
+    else
+        hipLaunchKernelGGL(k_residual<double>, grid, dim3(kBlock), 0, s, d, Bp, V, S, F, normF);
     CUPF_HIP(hipGetLastError());
 }
 
 void jacobian(const NrDevice& d, int Bp, const double2* V, double* J, const int* zero_slots,
               int n_zero, hipStream_t s) {
     dim3 grid((d.n_bus + kJacBusPerBlock - 1) / kJacBusPerBlock, Bp / kLanes);
-    hipLaunchKernelGGL(k_jacobian, grid, dim3(kBlock), 0, s, d, Bp, V, J, zero_slots, n_zero);
+    if (d.fp32_state)
+        hipLaunchKernelGGL(k_jacobian<float>, grid, dim3(kBlock), 0, s, d, Bp, V, J, zero_slots,
+                           n_zero);
+    else
+        hipLaunchKernelGGL(k_jacobian<double>, grid, dim3(kBlock), 0, s, d, Bp, V, J, zero_slots,
+                           n_zero);
     CUPF_HIP(hipGetLastError());
 }
 
 template <typename TX>
 void voltage_update(const NrDevice& d, int Bp, const TX* dx, double* Va, double* Vm, double2* V,
                     hipStream_t s) {
-    hipLaunchKernelGGL(k_vupdate<TX>, rows_grid(d.n_bus, Bp), dim3(kBlock), 0, s, d, Bp, dx, Va,
-                       Vm, V);
+    if (d.fp32_state)
+        hipLaunchKernelGGL((k_vupdate<TX, float>), rows_grid(d.n_bus, Bp), dim3(kBlock), 0, s, d,
+                           Bp, dx, Va, Vm, V);
+    else
+        hipLaunchKernelGGL((k_vupdate<TX, double>), rows_grid(d.n_bus, Bp), dim3(kBlock), 0, s, d,
+                           Bp, dx, Va, Vm, V);
     CUPF_HIP(hipGetLastError());
 }
 
@@ -479,19 +559,27 @@ void refine_residual(const int* rp, const int* ci, const int* vmap, const double
 }
 
 template <typename TX>
-void axpy_update(const TX* ddx, TX* dx, long long count, hipStream_t s) {
+void axpy_update(const TX* ddx, TX* dx, long long count, hipStream_t s, const int* gate) {
     const long long blocks = (count + kBlock - 1) / kBlock;
-    hipLaunchKernelGGL(k_axpy<TX>, dim3((unsigned)blocks), dim3(kBlock), 0, s, ddx, dx, count);
+    hipLaunchKernelGGL(k_axpy<TX>, dim3((unsigned)blocks), dim3(kBlock), 0, s, ddx, dx, count,
+                       gate);
     CUPF_HIP(hipGetLastError());
 }
 
-void batch_max(const double* normF, int B, double* out, hipStream_t s) {
-    hipLaunchKernelGGL(k_batch_max, dim3(1), dim3(kBlock), 0, s, normF, B, out);
+void batch_max(const double* normF, int B, double* out, hipStream_t s, const int* gate) {
+    hipLaunchKernelGGL(k_batch_max, dim3(1), dim3(kBlock), 0, s, normF, B, out, gate);
     CUPF_HIP(hipGetLastError());
 }
 
-void batch_max_finite(const double* normF, int B, double* out, hipStream_t s) {
-    hipLaunchKernelGGL(k_batch_max_finite, dim3(1), dim3(kBlock), 0, s, normF, B, out);
+void batch_max_finite(const double* normF, int B, double* out, hipStream_t s, const int* gate) {
+    hipLaunchKernelGGL(k_batch_max_finite, dim3(1), dim3(kBlock), 0, s, normF, B, out, gate);
+    CUPF_HIP(hipGetLastError());
+}
+
+void nr_control(const double* bmax, double tol, double refine_below, int refine, bool tolerant,
+                int limit, bool first, int* ctl, hipStream_t s, const int* gate) {
+    hipLaunchKernelGGL(k_nr_control, dim3(1), dim3(64), 0, s, bmax, tol, refine_below, refine,
+                       tolerant ? 1 : 0, limit, first ? 1 : 0, ctl, gate);
     CUPF_HIP(hipGetLastError());
 }
 
@@ -536,8 +624,8 @@ template void refine_residual<double>(const int*, const int*, const int*, const 
                                       const double*, const double*, double*, hipStream_t);
 template void refine_residual<float>(const int*, const int*, const int*, const double*, int, int,
                                      const double*, const float*, double*, hipStream_t);
-template void axpy_update<double>(const double*, double*, long long, hipStream_t);
-template void axpy_update<float>(const float*, float*, long long, hipStream_t);
+template void axpy_update<double>(const double*, double*, long long, hipStream_t, const int*);
+template void axpy_update<float>(const float*, float*, long long, hipStream_t, const int*);
 template void torch_inputs<double>(int, int, int, const double*, const double*, const double*,
                                    const double*, const double*, const double*, double2*,
                                    double2*, double*, double*, hipStream_t);

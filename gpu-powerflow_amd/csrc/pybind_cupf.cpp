@@ -733,6 +733,8 @@ PYBIND11_MODULE(_cupf, m) {
                  d["dense_flops"] = s.dense_flops;
                  d["dense_bytes"] = s.dense_bytes;
                  d["dense_groups"] = s.dense_groups;
+                 d["dense_kind"] = s.dense_kind;
+                 d["graph_launches"] = s.graph_launches;
                  d["last_warm_iterations"] = s.last_warm_iterations;
                  d["last_warm_fallbacks"] = s.last_warm_fallbacks;
                  d["state_version"] = s.state_version;

@@ -115,6 +115,7 @@ void WarmStartMLP::ensure_batch(int batch) {
     partial_.allocate(size_t(mlp::mse_blocks((long long)batch * dims_[L])) * sizeof(float));
     loss_acc_.allocate(sizeof(float));
     cap_batch_ = batch;
+    ++buffers_gen_;
 }
 
 void WarmStartMLP::reserve_samples(int64_t capacity) {
@@ -280,8 +281,7 @@ void WarmStartMLP::predict_v0(const double2* S, int64_t s_stride, const double2*
         const int c = std::min(cap_batch_, count - c0);
         const int cp = (c + 7) / 8 * 8;      // GEMM rows: multiple of 8 (pad rows are zero inputs)
         if (cp != c)
-            CUPF_HIP(hipMemsetAsync(A_[0].as<uint16_t>() + size_t(c) * dims_[0], 0,
-                                    size_t(cp - c) * dims_[0] * sizeof(uint16_t), s));
+            mlp::zero_u16((long long)(cp - c) * dims_[0], A_[0].as<uint16_t>() + size_t(c) * dims_[0], s);
         mlp::features(n_, dims_[0], c, S + c0 * s_stride, s_stride, xm_.as<float>(), xs_.as<float>(),
                       A_[0].as<uint16_t>(), s);
         forward(cp, s);

@@ -43,6 +43,9 @@ public:
     int64_t num_samples() const { return n_samples_; }
     bool normalized() const { return normalized_; }
     bool has_skip() const { return skip_; }
+    // Bumped whenever a device buffer predict_v0 uses moves (a captured
+    // inference that baked the old addresses is stale).
+    uint64_t buffers_generation() const { return buffers_gen_; }
 
     // ---- data set (device pointers; complex128 rows with the given strides)
     void reserve_samples(int64_t capacity);
@@ -96,6 +99,7 @@ private:
     DeviceBuffer xm_, xs_, ym_, ys_;
     bool normalized_ = false;
     int step_ = 0;
+    uint64_t buffers_gen_ = 0;
 };
 
 }  // namespace cupf

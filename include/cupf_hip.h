@@ -111,13 +111,17 @@ typedef struct {
     int32_t dim, nnz_j, num_fronts, max_front, max_pivots;
     int64_t arena_elems, lnz;
     double factor_flops;
-    double dense_flops, dense_bytes;   /* tier-2 fronts, per scenario */
-    int32_t dense_groups;              /* tier-2 launches per factorisation */
+    double dense_flops, dense_bytes;   /* the timed dense kernel's fronts, per scenario */
+    int32_t dense_groups;              /* its launches per factorisation */
     int32_t last_warm_iterations;      /* warm-start pass of the last solve */
     int32_t last_warm_fallbacks;       /* scenarios re-solved from their V0 */
     int64_t state_version;             /* bumped by every call that replaces the
                                           batch state (initialize, forward solves);
                                           never reset */
+    int32_t dense_kind;                /* the timed dense kernel: 2 contribution-block
+                                          tiles, 1 row-block Schur, 0 tier-2 LU */
+    int64_t graph_launches;            /* captured-graph launches since initialize (use_cuda_graph:
+                                          one per solve once the step count is learned) */
 } cupf_solver_stats;
 
 const char* cupf_last_error(void);

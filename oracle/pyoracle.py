@@ -99,6 +99,40 @@ def mismatch(V, I, S, pv, pq):
     return F
 
 
+def mismatch_fp32(n, indptr, indices, data, V, S, pv, pq):
+    """The FP32 profile's residual for scenarios V, S [B][n]: every operand
+    rounded to float32 and every operation in float32, in the order of the
+    FP32 pipeline -- Ibus_i accumulated over Ybus row i in column order,
+    re += yr*vr - yi*vi, im += yr*vi + yi*vr (reference
+    cuPF/cpp/src/newton_solver/ops/ibus/compute_ibus.cu:59-93, instantiated
+    <float, float, float> at :133-142), then F = [Re(V conj I) - P at pv, pq |
+    Im(V conj I) - Q at pq] with Re = vr*ir + vi*ii, Im = vi*ir - vr*ii
+    (ops/mismatch/compute_mismatch_from_ibus.cu).  Returns F [B][dim] as
+    float64 (exact widening)."""
+    f = np.float32
+    V = np.atleast_2d(np.asarray(V, np.complex128))
+    S = np.atleast_2d(np.asarray(S, np.complex128))
+    vr, vi = V.real.astype(f), V.imag.astype(f)
+    yr, yi = np.real(data).astype(f), np.imag(data).astype(f)
+    B = V.shape[0]
+    ir = np.zeros((B, n), f)
+    ii = np.zeros((B, n), f)
+    indptr = np.asarray(indptr)
+    indices = np.asarray(indices)
+    deg = np.diff(indptr)
+    for t in range(int(deg.max()) if n else 0):
+        rows = np.nonzero(deg > t)[0]
+        k = indptr[rows] + t
+        c = indices[k]
+        a, b = yr[k], yi[k]
+        ir[:, rows] = ir[:, rows] + (a * vr[:, c] - b * vi[:, c])
+        ii[:, rows] = ii[:, rows] + (a * vi[:, c] + b * vr[:, c])
+    p = vr * ir + vi * ii - S.real.astype(f)
+    q = vi * ir - vr * ii - S.imag.astype(f)
+    pv, pq = np.asarray(pv), np.asarray(pq)
+    return np.concatenate([p[:, pv], p[:, pq], q[:, pq]], axis=1).astype(np.float64)
+
+
 def jacobian_values(n, indptr, indices, data, V, pv, pq, nnzJ):
     pv, pq = _i32(pv), _i32(pq)
     J = np.zeros(nnzJ, np.float64)

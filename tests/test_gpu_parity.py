@@ -5,7 +5,10 @@ reference CPU operator); Jacobian values within 4 ulp-scale relative error
 (|V| goes through the device hypot); linear solves to a relative residual
 1e-12; NR solutions within 1e-10 of the oracle at the same tolerance, with
 identical iteration counts.  Mixed path (FP32 fronts + FP64 refinement):
-solves to 1e-9 relative residual, NR solutions within 1e-9.
+solves to 1e-9 relative residual, NR solutions within 1e-9.  FP32 profile
+(FP32 state): residual bit-identical to the float32 restatement of the
+reference's FP32 pipeline; solutions are FP32 values within 1e-4 of the
+oracle.
 """
 
 from __future__ import annotations
@@ -133,6 +136,47 @@ def test_nr_matches_oracle(gpu, cupf, syn, oracle, g2000, compute, cap):
     V = r.V_numpy
     assert np.abs(np.angle(V) - np.angle(Vo)).max() <= 1e-10
     assert np.abs(np.abs(V) - np.abs(Vo)).max() <= 1e-10
+
+
+@pytest.mark.parametrize("which", ["g118", "g2000"])
+def test_fp32_profile_residual_is_float32(gpu, cupf, syn, oracle, request, which):
+    """ComputePolicy.FP32 keeps the state in FP32 (the reference's
+    CudaFp32CustomPipeline): its residual equals the float32 restatement of
+    the FP32 pipeline's Ibus and mismatch bit for bit."""
+    g = request.getfixturevalue(which)
+    s = make_solver(cupf, g, "FP32")
+    B = 4
+    V = random_states(g, B, seed=23)
+    S = g.sbus_scaled(syn.load_factors(B, g.n_bus, 0.7, 1.3, 4))
+    F, norm = capi.debug_residual(s.handle, S, V, g.dim)
+    Fo = oracle.mismatch_fp32(g.n_bus, g.indptr, g.indices, g.data, V, S, g.pv, g.pq)
+    np.testing.assert_array_equal(F, Fo)
+    np.testing.assert_array_equal(norm, np.abs(Fo).max(axis=1))
+
+
+def test_fp32_profile_solves_to_float_accuracy(gpu, cupf, syn, oracle, g118):
+    """The FP32 profile on the 118-bus grid: converges at a tolerance above
+    its FP32 floor (1e-4), returns voltages that are FP32 values, and lands
+    within 1e-4 of the FP64 oracle's solution."""
+    g = g118
+    B = 16
+    S = g.sbus_scaled(syn.load_factors(B, g.n_bus, 0.7, 1.3, 7))
+    V0 = g.flat_start(B)
+    Vo, io, no, so = oracle.nr_solve_batch(g.n_bus, g.indptr, g.indices, g.data, S, V0, g.pv, g.pq,
+                                           tol=1e-12)
+    s = make_solver(cupf, g, "FP32")
+    c = cupf.NRConfig()
+    c.tolerance = 1e-4
+    c.max_iter = 20
+    r = s.solve_batch(g.indptr, g.indices, g.data, g.n_bus, g.n_bus, S, V0, g.pv, g.pq, c)
+    print(f"[fp32 profile] iterations {r.iterations_numpy[0]} (FP64 oracle to 1e-12: {io.max()}), "
+          f"final mismatch {r.final_mismatch_numpy.max():.2e}", flush=True)
+    assert r.converged_numpy.all() and (r.final_mismatch_numpy <= 1e-4).all()
+    V = r.V_numpy
+    np.testing.assert_array_equal(V.real, V.real.astype(np.float32))
+    np.testing.assert_array_equal(V.imag, V.imag.astype(np.float32))
+    assert np.abs(np.angle(V) - np.angle(Vo)).max() <= 1e-4
+    assert np.abs(np.abs(V) - np.abs(Vo)).max() <= 1e-4
 
 
 def test_case118_single_scenario(gpu, cupf, oracle, g118):

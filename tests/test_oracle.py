@@ -8,7 +8,10 @@
 * the sparse LU against scipy's SuperLU and dense numpy on the reference's
   own J.mtx/F.mtx fixtures (custom_linear_solver/tests/datasets/power);
 * the Jacobian against central finite differences of F;
-* the adjoint against central finite differences of the converged solution.
+* the adjoint against central finite differences of the converged solution;
+* the FP32 profile's residual restatement (float32 operations): equal to the
+  FP64 residual where every intermediate is exact in float32 (small integer
+  data), and within float32 rounding of it on a grid.
 """
 
 from __future__ import annotations
@@ -215,3 +218,36 @@ def test_regression_fixture_grid30(oracle):
                                           max_iter=30)
     np.testing.assert_array_equal(it, d["iterations"])
     np.testing.assert_allclose(V, d["V"], rtol=0, atol=1e-13)
+
+
+def test_fp32_residual_exact_on_integer_data(ref):
+    """Integer admittances, voltages and injections keep every product and sum
+    exact in float32, so the float32 restatement must equal the FP64 oracle
+    bit for bit: a dropped term, a wrong sign or a transposed operand fails."""
+    rng = np.random.default_rng(4)
+    n = 40
+    A = sp.random(n, n, density=0.12, random_state=5, format="csr")
+    A = (A + A.T + sp.eye(n)).tocsr()
+    A.sort_indices()
+    data = rng.integers(-9, 10, A.nnz) + 1j * rng.integers(-9, 10, A.nnz)
+    pv = np.arange(1, 8, dtype=np.int32)
+    pq = np.arange(8, n, dtype=np.int32)
+    V = rng.integers(-4, 5, (3, n)) + 1j * rng.integers(-4, 5, (3, n))
+    S = rng.integers(-50, 51, (3, n)) + 1j * rng.integers(-50, 51, (3, n))
+    F32 = ref.mismatch_fp32(n, A.indptr, A.indices, data, V, S, pv, pq)
+    for b in range(3):
+        I = ref.ibus(n, A.indptr, A.indices, data, V[b])
+        np.testing.assert_array_equal(F32[b], ref.mismatch(V[b], I, S[b], pv, pq))
+
+
+def test_fp32_residual_within_float_rounding(syn, ref):
+    g = syn.workload_grid("case118")
+    S = g.sbus_scaled(syn.load_factors(4, g.n_bus, 0.7, 1.3, 6))
+    V = random_states(g, 4, seed=9)
+    F32 = ref.mismatch_fp32(g.n_bus, g.indptr, g.indices, g.data, V, S, g.pv, g.pq)
+    scale = np.abs(g.data).max() * np.abs(V).max() ** 2
+    for b in range(4):
+        I = ref.ibus(g.n_bus, g.indptr, g.indices, g.data, V[b])
+        F = ref.mismatch(V[b], I, S[b], g.pv, g.pq)
+        err = np.abs(F32[b] - F).max()
+        assert 0 < err <= 64 * np.finfo(np.float32).eps * scale, err
