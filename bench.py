@@ -87,6 +87,8 @@ def parse():
                    help="held-out evaluation scenarios (U(0.6, 1.4), seed 3)")
     p.add_argument("--epochs", type=int, default=60)
     p.add_argument("--warm-iters", type=int, default=4, help="warm-start NR budget before fallback")
+    p.add_argument("--learned-fronts", choices=["fp32", "bf16x3"], default="bf16x3",
+                   help="trailing-update MFMA form of the learned + mixed form's fronts")
     p.add_argument("--skip", action="store_true",
                    help="headline warm-start network with a linear input -> output path beside "
                         "the hidden layers (default: the plain MLP)")
@@ -173,11 +175,12 @@ def make_solver(cupf, grid, compute, precision, cap, refine, graph=False):
     return s
 
 
-def form_spec(cupf, refine):
+def form_spec(cupf, refine, learned_fronts="fp32"):
+    learned = cupf.CustomPrecision.TF32 if learned_fronts == "bf16x3" else cupf.CustomPrecision.FP32
     return {"fp64": (cupf.ComputePolicy.FP64, cupf.CustomPrecision.FP64, 0),
             "mixed": (cupf.ComputePolicy.Mixed, cupf.CustomPrecision.FP32, refine),
             "bf16x3": (cupf.ComputePolicy.Mixed, cupf.CustomPrecision.TF32, refine),
-            "learned": (cupf.ComputePolicy.Mixed, cupf.CustomPrecision.FP32, refine)}
+            "learned": (cupf.ComputePolicy.Mixed, learned, refine)}
 
 
 def timed_solves(torch, solver, S, V0, Vout, B, n, steps, warmup, cfg, stage_timing=False,
@@ -541,7 +544,8 @@ def main():
     cfg = cupf.NRConfig()
 
     forms = [f.strip() for f in args.forms.split(",") if f.strip()]
-    spec = form_spec(cupf, args.refine)
+    spec = form_spec(cupf, args.refine, args.learned_fronts)
+    DENSE_MODE["learned"] = 2 if args.learned_fronts == "bf16x3" else 1
     res = {}
     training = None
     mlp = None
@@ -640,8 +644,10 @@ def main():
             "batch": B, "n_bus": n, "dim": h["stats"]["dim"], "tolerance": cfg.tolerance,
             "panel_cap": args.cap, "load_scale": load_scale,
             "form": {"learned": f"learned MLP warm start (<= {args.warm_iters} NR iterations, "
-                                f"flat-start fallback) + FP32 fronts, FP32-MFMA dense trailing "
-                                f"updates, {args.refine} FP64 refinement step(s) per Newton step at ||F||inf <= 1e-5",
+                                f"flat-start fallback) + FP32 fronts, "
+                                f"{'3xBF16' if args.learned_fronts == 'bf16x3' else 'FP32'}-MFMA "
+                                f"dense trailing updates, {args.refine} FP64 refinement step(s) "
+                                f"per Newton step at ||F||inf <= 1e-5",
                      "mixed": f"FP32 fronts, FP32-MFMA dense trailing updates, "
                               f"{args.refine} FP64 refinement step(s) per Newton step at ||F||inf <= 1e-5",
                      "bf16x3": f"FP32 fronts, 3xBF16 MFMA dense trailing updates, "

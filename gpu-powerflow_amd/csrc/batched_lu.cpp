@@ -7,6 +7,14 @@
 
 namespace cupf {
 
+// Largest tier-1 front (rows): a whole FP32 front in 64 KiB of LDS, an FP64
+// one in 63 KiB.
+#ifndef CUPF_TIER1_ROWS32
+#define CUPF_TIER1_ROWS32 128
+#endif
+constexpr int kTier1MaxRows32 = CUPF_TIER1_ROWS32;
+constexpr int kTier1MaxRows64 = 90;
+
 // neighbouring scenarios that run each contribution-block tile together
 // (1, 4 and 16 measured alike on the 2,000-bus batch)
 constexpr int kTileScenGroup = 4;
@@ -40,7 +48,7 @@ void BatchedLU::analyze(int n, const int* rp, const int* ci, const std::vector<i
     sym_ = analyze_symbolic(n, rp, ci, perm, opt.cap);
     // tier 1 holds a whole front in 64 KiB of LDS
     static_assert(kern::kT1MaxK >= kTier1MaxPivots, "tier-1 fronts fit the tier-1 factor kernel");
-    const int tier1_max = opt.front64 ? 90 : 128;
+    const int tier1_max = opt.front64 ? kTier1MaxRows64 : kTier1MaxRows32;
     plan_ = lower_plan(sym_, n, rp, ci, 16, tier1_max);
     const int lds_max = opt.front64 ? 8192 : 16384;
     if (plan_.max_fsz > lds_max)
@@ -203,6 +211,11 @@ void BatchedLU::analyze(int n, const int* rp, const int* ci, const std::vector<i
     dense_groups_ = 0;
     dense_kind_ = 0;
     auto kind_of = [&](int level) { return tile_schur(level) ? 2 : cross_schur(level) ? 1 : 0; };
+    // pivot + Schur levels whose fronts have a contribution block (a level
+    // whose fronts are all pivots launches no Schur kernel)
+    level_cb_.assign(size_t(P.num_levels), 0);
+    for (int p = 0; p < P.num_fronts; ++p)
+        if (P.tier[p] == 2 && P.fsz[p] > P.nc[p]) level_cb_[size_t(P.level[p])] = 1;
     for (int L = 0; L < P.num_levels; ++L)
         if (has_group(L, 2)) dense_kind_ = std::max(dense_kind_, kind_of(L));
     for (int p = 0; p < P.num_fronts; ++p) {
@@ -247,7 +260,8 @@ void BatchedLU::analyze(int n, const int* rp, const int* ci, const std::vector<i
         }
     }
     for (int L = 0; L < P.num_levels; ++L)
-        if (has_group(L, 2) && kind_of(L) == dense_kind_) ++dense_groups_;
+        if (has_group(L, 2) && kind_of(L) == dense_kind_ && (dense_kind_ == 0 || level_cb_[size_t(L)]))
+            ++dense_groups_;
 
     // batch buffers are sized lazily for the new structure
     capBp_ = 0;
@@ -434,6 +448,7 @@ void BatchedLU::factor_t(const double* A, int Bp, hipStream_t s, StageTimer* tim
             const bool tiles = tile_schur(L);
             kern::cross_group<T>(lu_, f, cnt, mf, A, Bp, ar, uinv, opt_.pivot_eps, sp, mem_.sing,
                                  tiles, st);
+            if (!level_cb_[size_t(L)]) return;    // pivots only: no contribution block
             if (tiles) {
                 StageScope sc(timer, dense_kind_ == 2 ? stage : -1, st);
                 const int t0 = tile_ptr_[size_t(L)];

@@ -156,6 +156,7 @@ private:
     // contribution-block tiles of the tile-path levels: level L's at
     // tile_ptr_[L] .. tile_ptr_[L + 1] of d_tiles
     std::vector<int> tile_ptr_;
+    std::vector<char> level_cb_;      // per level: a tier-2 front with a contribution block
     // single-system arena: front p at d_soff[p]
     DeviceBuffer d_soff, single_store_;
     kern::SingleDims sdims_;

@@ -110,6 +110,11 @@ __device__ inline void put_entries(T* __restrict__ F, const int* __restrict__ ds
 }
 
 constexpr int kItemWaves = 4;              // owner waves of a block's rows (rr % 4)
+// Chunks a wave keeps in flight while it walks a chunk map.
+#ifndef CUPF_ASM_DEPTH
+#define CUPF_ASM_DEPTH 16
+#endif
+constexpr int kAsmDepth = CUPF_ASM_DEPTH;
 
 // Workgroup barrier that orders LDS accesses only: global loads in flight
 // stay in flight across it (__syncthreads() also waits for those).  For
@@ -243,7 +248,7 @@ __device__ inline void assemble_front(const LuDevice& d, const Arenas<T>& ar, co
     put_entries<NT>(F, d.ent_dst, d.ent_src, A, Bp, b, d.ent_ptr[p], d.ent_ptr[p + 1],
                     [](int dst) { return dst; });
     __syncthreads();
-    assemble_chunks<NT, 16>(d, ar, d.asm_base[p * kAsmKindCount + kAsmKFront], b, Bp, F,
+    assemble_chunks<NT, kAsmDepth>(d, ar, d.asm_base[p * kAsmKindCount + kAsmKFront], b, Bp, F,
                             [=](int r, int c) { return r * fs + c; });
     __syncthreads();
 }
@@ -553,7 +558,7 @@ __global__ __launch_bounds__(kAsmThreads) void k_assemble_rows(LuDevice d,
     put_entries<kAsmThreads>(F, d.ent_dst, d.ent_src, A, Bp, b, lo, hi,
                              [=](int dst) { return dst - i0 * fs; });
     __syncthreads();
-    assemble_chunks<kAsmThreads, 16>(d, ar, d.asm_base[p * kAsmKindCount + kAsmKRows] + rb, b, Bp, F,
+    assemble_chunks<kAsmThreads, kAsmDepth>(d, ar, d.asm_base[p * kAsmKindCount + kAsmKRows] + rb, b, Bp, F,
                                      [=](int rr, int c) { return rr * fs + c; });
     __syncthreads();
     T* G = ar.a1 + (long long)b * d.arena1 + d.off[p] + (long long)i0 * fs;
@@ -939,7 +944,7 @@ __global__ __launch_bounds__(pivot_threads<T>()) void k_pivot(LuDevice d, const 
         return c < K ? r * kPivLd + c : -1;
     });
     __syncthreads();
-    assemble_chunks<NT, 16>(d, ar, d.asm_base[p * kAsmKindCount + kAsmKPivot], b, Bp, P,
+    assemble_chunks<NT, kAsmDepth>(d, ar, d.asm_base[p * kAsmKindCount + kAsmKPivot], b, Bp, P,
                             [=](int r, int c) { return r * kPivLd + c; });
     __syncthreads();
 
@@ -1051,7 +1056,7 @@ __device__ __forceinline__ void u12_chunk(const LuDevice& d, const int* __restri
             return (c >= j0 && c < j0 + jn) ? rr * kXCld + c - j0 : -1;
         });
         lds_barrier();
-        assemble_chunks<kPivThreads, 16>(d, ar, d.asm_base[p * kAsmKindCount + kAsmKChunk] + ch, b, Bp, Ch,
+        assemble_chunks<kPivThreads, kAsmDepth>(d, ar, d.asm_base[p * kAsmKindCount + kAsmKChunk] + ch, b, Bp, Ch,
                                          [=](int r, int c) { return r * kXCld + c - j0; });
 #pragma unroll
         for (int u = 0; u < kUinvElems / kPivThreads; ++u) {
@@ -1188,7 +1193,7 @@ __global__ __launch_bounds__(kSchurThreads, 2) void k_schur(LuDevice d, const in
         return c < K ? rr * kSLd + c : RB * kSLd + rr * uc + c - K;
     });
     __syncthreads();
-    assemble_chunks<kSchurThreads, 16>(d, ar, d.asm_base[p * kAsmKindCount + kAsmKSchur] + rb, b, Bp, Lb,
+    assemble_chunks<kSchurThreads, kAsmDepth>(d, ar, d.asm_base[p * kAsmKindCount + kAsmKSchur] + rb, b, Bp, Lb,
                                       [=](int rr, int c) {
                                           return c < K ? rr * kSLd + c : RB * kSLd + rr * uc + c - K;
                                       });
@@ -1419,7 +1424,7 @@ __device__ __forceinline__ void l21_block(const LuDevice& d, const int* __restri
         return c < K ? (r - i0) * kL21Ld + c : -1;
     });
     lds_barrier();
-    assemble_chunks<256, 16>(d, ar, d.asm_base[p * kAsmKindCount + kAsmKL21] + blk, b, Bp, Ab,
+    assemble_chunks<256, kAsmDepth>(d, ar, d.asm_base[p * kAsmKindCount + kAsmKL21] + blk, b, Bp, Ab,
                              [=](int rr, int c) { return rr * kL21Ld + c; });
     lds_barrier();
     if constexpr (sizeof(T) == 4) {
@@ -1575,7 +1580,7 @@ __global__ __launch_bounds__(256) void k_schur_tile(LuDevice d, const int* __res
         return (c >= c0 && c < c0 + ncl) ? (r - r0) * LD + c - c0 : -1;
     });
     lds_barrier();
-    assemble_chunks<256, 16>(d, ar, d.asm_base[p * kAsmKindCount + kAsmKTile] + rb * nct + ct, b,
+    assemble_chunks<256, kAsmDepth>(d, ar, d.asm_base[p * kAsmKindCount + kAsmKTile] + rb * nct + ct, b,
                              Bp, At, [=](int rr, int c) { return rr * LD + c - c0; });
     lds_barrier();
     if (!busy) return;                           // no barrier below
