@@ -174,13 +174,30 @@ __global__ __launch_bounds__(kGemmThreads) void k_gemm(int M, int N, int K,
         }
 }
 
-// out[j] = sum_i in[i][j] (in BF16 [M][ld]); one thread per column, FP32.
-__global__ __launch_bounds__(256) void k_colsum_bf16(int M, int N, const uint16_t* __restrict__ in,
-                                                     int ld, float* __restrict__ out) {
+// Column sums out[j] = sum_i in[i][j] (in BF16 [M][ld], FP32 sums) in two
+// passes with a fixed order: partial sums of kColsumParts row ranges (a
+// wave per range and 64 columns, a lane per column: coalesced rows), then
+// the parts added in range order.
+constexpr int kColsumParts = 64;
+
+__global__ __launch_bounds__(256) void k_colsum_part(int M, int N, const uint16_t* __restrict__ in,
+                                                     int ld, float* __restrict__ part) {
+    const int j = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int r = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (j >= N) return;
+    const int rows = (M + kColsumParts - 1) / kColsumParts;
+    const int i1 = min(M, (r + 1) * rows);
+    float s = 0.f;
+    for (int i = r * rows; i < i1; ++i) s += bf2f(in[(long long)i * ld + j]);
+    part[(long long)r * N + j] = s;
+}
+
+__global__ __launch_bounds__(256) void k_colsum_final(int N, const float* __restrict__ part,
+                                                      float* __restrict__ out) {
     const int j = blockIdx.x * 256 + threadIdx.x;
     if (j >= N) return;
     float s = 0.f;
-    for (int i = 0; i < M; ++i) s += bf2f(in[(long long)i * ld + j]);
+    for (int r = 0; r < kColsumParts; ++r) s += part[(long long)r * N + j];
     out[j] = s;
 }
 
@@ -385,8 +402,13 @@ void gemm(bool ta, bool tb, int M, int N, int K, const uint16_t* A, int lda, con
     CUPF_HIP(hipGetLastError());
 }
 
-void colsum_bf16(int M, int N, const uint16_t* in, int ld, float* out, hipStream_t s) {
-    hipLaunchKernelGGL(k_colsum_bf16, dim3((N + 255) / 256), dim3(256), 0, s, M, N, in, ld, out);
+size_t colsum_part_elems(int N) { return size_t(kColsumParts) * size_t(N); }
+
+void colsum_bf16(int M, int N, const uint16_t* in, int ld, float* out, float* part, hipStream_t s) {
+    if (N <= 0) return;
+    hipLaunchKernelGGL(k_colsum_part, dim3((N + 63) / 64, kColsumParts / 4), dim3(256), 0, s, M, N,
+                       in, ld, part);
+    hipLaunchKernelGGL(k_colsum_final, dim3((N + 255) / 256), dim3(256), 0, s, N, part, out);
     CUPF_HIP(hipGetLastError());
 }
 

@@ -30,7 +30,10 @@ struct Epilogue {
 // tb: B stored [N][ldb] (else [K][ldb]).  Sizes and strides multiples of 8.
 void gemm(bool ta, bool tb, int M, int N, int K, const uint16_t* A, int lda, const uint16_t* B,
           int ldb, const Epilogue& ep, hipStream_t s);
-void colsum_bf16(int M, int N, const uint16_t* in, int ld, float* out, hipStream_t s);
+// out[j] = sum_i in[i][j] over a BF16 [M][ld] block (FP32, fixed order);
+// part: colsum_part_elems(N) floats of scratch.
+size_t colsum_part_elems(int N);
+void colsum_bf16(int M, int N, const uint16_t* in, int ld, float* out, float* part, hipStream_t s);
 // AdamW on n FP32 parameters; bf_map[i] >= 0 is the BF16 copy index of w[i].
 void adamw(long long n, float* w, const float* g, float* m, float* v, float lr, float b1, float b2,
            float eps, float wd, int step, const long long* bf_map, uint16_t* wbf, hipStream_t s);

@@ -114,6 +114,8 @@ void WarmStartMLP::ensure_batch(int batch) {
     T_batch_.allocate(size_t(batch) * dims_[L] * sizeof(float));
     partial_.allocate(size_t(mlp::mse_blocks((long long)batch * dims_[L])) * sizeof(float));
     loss_acc_.allocate(sizeof(float));
+    colsum_.allocate(mlp::colsum_part_elems(*std::max_element(dims_.begin(), dims_.end())) *
+                     sizeof(float));
     cap_batch_ = batch;
     ++buffers_gen_;
 }
@@ -216,7 +218,8 @@ void WarmStartMLP::backward(int count, hipStream_t s) {
         ew.ldc = in;
         mlp::gemm(true, false, out, in, count, dH_[l].as<uint16_t>(), out, A_[l - 1].as<uint16_t>(), in,
                   ew, s);
-        mlp::colsum_bf16(count, out, dH_[l].as<uint16_t>(), out, g_.as<float>() + b_off_[l - 1], s);
+        mlp::colsum_bf16(count, out, dH_[l].as<uint16_t>(), out, g_.as<float>() + b_off_[l - 1],
+                         colsum_.as<float>(), s);
         if (l > 1) {                         // dH_{l-1} = (dH_l W) * gelu'(H_{l-1})
             mlp::Epilogue ex;
             ex.mode = mlp::kEpGeluBwd;

@@ -92,7 +92,7 @@ private:
     DeviceBuffer kind_;
     // activations for cap_batch_ rows
     std::vector<DeviceBuffer> A_, H_, dH_;
-    DeviceBuffer T_batch_, partial_, loss_acc_, idx_;
+    DeviceBuffer T_batch_, partial_, loss_acc_, idx_, colsum_;
     // data set
     int64_t cap_samples_ = 0, n_samples_ = 0;
     DeviceBuffer X_, Xb_, Y_;
