@@ -78,4 +78,5 @@ else:
     CuPFFunction = _ta.CuPFFunction
     solve = _ta.solve
     from .warm_start import train_warm_start
+    from . import pinn
     __all__ += ["CuPFFunction", "solve", "train_warm_start"]
