@@ -3,6 +3,7 @@
 #include "batched_lu.hpp"
 
 #include <algorithm>
+#include <map>
 #include <stdexcept>
 
 namespace cupf {
@@ -50,6 +51,7 @@ void BatchedLU::analyze(int n, const int* rp, const int* ci, const std::vector<i
     static_assert(kern::kT1MaxK >= kTier1MaxPivots, "tier-1 fronts fit the tier-1 factor kernel");
     const int tier1_max = opt.front64 ? kTier1MaxRows64 : kTier1MaxRows32;
     plan_ = lower_plan(sym_, n, rp, ci, 16, tier1_max);
+    split_tier1();
     const int lds_max = opt.front64 ? 8192 : 16384;
     if (plan_.max_fsz > lds_max)
         throw std::runtime_error("BatchedLU::analyze(): a front exceeds the LDS panel budget");
@@ -195,6 +197,8 @@ void BatchedLU::analyze(int n, const int* rp, const int* ci, const std::vector<i
     lu.asm_cptr = d_acptr.as<int>(); lu.asm_chunks = d_achunks.as<int4>();
     lu.gat_ptr = d_gptr.as<int>(); lu.gat_src = d_gsrc.as<int>(); lu.rhs_idx = d_ridx.as<int>();
 
+    find_chains(s);
+
     // The timed dense stage is one kernel: the tile kernel when any level
     // takes it, else the row-block Schur kernel, else the tier-2 LU groups.
     // Its algorithmic work per scenario:
@@ -210,7 +214,10 @@ void BatchedLU::analyze(int n, const int* rp, const int* ci, const std::vector<i
     dense_flops_ = dense_bytes_ = 0.0;
     dense_groups_ = 0;
     dense_kind_ = 0;
-    auto kind_of = [&](int level) { return tile_schur(level) ? 2 : cross_schur(level) ? 1 : 0; };
+    auto kind_of = [&](int level) {
+        if (chain_level_[size_t(level)]) return -1;       // factored by the chain kernel
+        return tile_schur(level) ? 2 : cross_schur(level) ? 1 : 0;
+    };
     // pivot + Schur levels whose fronts have a contribution block (a level
     // whose fronts are all pivots launches no Schur kernel)
     level_cb_.assign(size_t(P.num_levels), 0);
@@ -269,6 +276,131 @@ void BatchedLU::analyze(int n, const int* rp, const int* ci, const std::vector<i
     use_reserved();
     n_ = n;
     nnz_ = nnz;
+}
+
+// Chains of nested fronts (FP32 fronts; kern::chain_factor).  A front q is
+// the nested parent of its child c when c's contribution rows are exactly
+// q's rows in order (fsz(c) - nc(c) == fsz(q), identity row map).  Levels are
+// walked upwards; a tier-2 front with at most 64 pivots alone on its level
+// whose nested child is already in a chain extends that chain, else it starts
+// one with the child as F0 (F0's own level factored as usual, its pivot +
+// Schur group writing the contribution block to its trailing block).  The
+// additions of each member (its entries, then its other children in child
+// order) are listed per element of F0's trailing block.
+void BatchedLU::find_chains(hipStream_t s) {
+    const LuPlan& P = plan_;
+    chains_.clear();
+    chain_at_.assign(size_t(P.num_levels), -1);
+    chain_level_.assign(size_t(P.num_levels), 0);
+    if (opt_.front64 || !opt_.chains) return;
+    std::vector<int> level_count(size_t(P.num_levels), 0), member(size_t(P.num_fronts), -1);
+    for (int g = 0; g < P.num_levels * 3; ++g) level_count[size_t(g / 3)] += P.group_ptr[g + 1] - P.group_ptr[g];
+    auto nested_child = [&](int q) {
+        for (int k = P.child_ptr[q]; k < P.child_ptr[q + 1]; ++k) {
+            const int c = P.child_idx[k];
+            if (P.tier[c] != 2 || P.fsz[c] - P.nc[c] != P.fsz[q]) continue;
+            bool id = true;
+            for (int a = 0; id && a < P.fsz[q]; ++a) id = P.asm_loc[size_t(P.asm_ptr[c] + a)] == a;
+            if (id) return c;
+        }
+        return -1;
+    };
+    std::vector<std::vector<int>> lists;
+    for (int L = 0; L < P.num_levels; ++L) {
+        if (level_count[size_t(L)] != 1 || !has_group(L, 2)) continue;
+        const int q = P.group_fronts[size_t(P.group_ptr[L * 3 + 2])];
+        if (!kern::cross_schur_path(P.fsz[q], P.nc[q])) continue;
+        const int c = nested_child(q);
+        if (c < 0 || !cross_schur(P.level[c])) continue;
+        if (member[size_t(c)] >= 0) {
+            member[size_t(q)] = member[size_t(c)];
+            lists[size_t(member[size_t(q)])].push_back(q);
+        } else {
+            member[size_t(q)] = int(lists.size());
+            lists.push_back({c, q});
+        }
+    }
+    for (const std::vector<int>& F : lists) {
+        Chain ch;
+        const int ldT = P.fsz[F[0]];
+        std::vector<int> eptr{0}, edst, esptr{0};
+        std::vector<int2> esrc;
+        int o = 0;
+        for (size_t k = 1; k < F.size(); ++k) {
+            const int q = F[k], fq = P.fsz[q];
+            std::map<int, std::vector<int2>> add;
+            auto at = [&](int r, int c) { return (o + r) * ldT + o + c; };
+            for (int e = P.ent_ptr[q]; e < P.ent_ptr[q + 1]; ++e)
+                add[at(P.ent_dst[e] / fq, P.ent_dst[e] % fq)].push_back(make_int2(kern::kChainSrcJ, P.ent_src[e]));
+            for (int kk = P.child_ptr[q]; kk < P.child_ptr[q + 1]; ++kk) {
+                const int c = P.child_idx[kk];
+                if (c == F[k - 1]) continue;
+                const int cfs = P.fsz[c], cnc = P.nc[c], cuc = cfs - cnc;
+                const int* loc = P.asm_loc.data() + P.asm_ptr[c];
+                const int kind = P.tier[c] == 0 ? kern::kChainSrcA0 : kern::kChainSrcA1;
+                for (int a = 0; a < cuc; ++a)
+                    for (int b = 0; b < cuc; ++b) {
+                        const long long idx = P.off[c] + (long long)(cnc + a) * cfs + cnc + b;
+                        if (idx >= (1LL << 31)) throw std::runtime_error("BatchedLU: arena too large for a chain");
+                        add[at(loc[a], loc[b])].push_back(make_int2(kind, int(idx)));
+                    }
+            }
+            for (const auto& kv : add) {
+                edst.push_back(kv.first);
+                for (const int2& v : kv.second) esrc.push_back(v);
+                esptr.push_back(int(esrc.size()));
+            }
+            eptr.push_back(int(edst.size()));
+            o += P.nc[q];
+        }
+        for (int f : F) {
+            ch.fsz.push_back(P.fsz[f]);
+            ch.nc.push_back(P.nc[f]);
+        }
+        ch.d_fronts = upload_vector(F, s);
+        ch.d_eptr = upload_vector(eptr, s);
+        ch.d_edst = upload_vector(edst, s);
+        ch.d_esptr = upload_vector(esptr, s);
+        ch.d_esrc = upload_vector(esrc, s);
+        ch.dev.fronts = ch.d_fronts.as<int>();
+        ch.dev.steps = int(F.size()) - 1;
+        ch.dev.ext_ptr = ch.d_eptr.as<int>();
+        ch.dev.ext_dst = ch.d_edst.as<int>();
+        ch.dev.ext_sptr = ch.d_esptr.as<int>();
+        ch.dev.ext_src = ch.d_esrc.as<int2>();
+        chain_at_[size_t(P.level[F[1]])] = int(chains_.size());
+        for (size_t k = 1; k < F.size(); ++k) chain_level_[size_t(P.level[F[k]])] = 1;
+        chains_.push_back(std::move(ch));
+    }
+}
+
+// Tier-1 groups are launched in parts of similar LDS need: a tier-1
+// workgroup holds its front in fsz^2 elements of LDS sized by the launch's
+// largest front, so one launch over fronts of 34 to 128 rows would run every
+// workgroup at the occupancy of the 128-row one.  The group's fronts are
+// sorted by size, largest first, and a new part starts at the first front
+// that fits at least twice as many workgroups per CU as the part's first.
+void BatchedLU::split_tier1() {
+    LuPlan& P = plan_;
+    const double lds_cu = 160.0 * 1024.0;
+    auto per_cu = [&](int f) {
+        const double b = double(P.fsz[f]) * P.fsz[f] * double(front_bytes());
+        return std::min(8, std::max(1, int(lds_cu / std::max(1.0, b))));
+    };
+    t1_parts_.assign(size_t(P.num_levels), {});
+    for (int L = 0; L < P.num_levels; ++L) {
+        const int g0 = P.group_ptr[L * 3 + 1], g1 = P.group_ptr[L * 3 + 2];
+        if (g0 == g1) continue;
+        std::stable_sort(P.group_fronts.begin() + g0, P.group_fronts.begin() + g1,
+                         [&](int a, int b) { return P.fsz[a] > P.fsz[b]; });
+        for (int st = g0; st < g1;) {
+            const int occ = per_cu(P.group_fronts[size_t(st)]);
+            int en = st + 1;
+            while (en < g1 && per_cu(P.group_fronts[size_t(en)]) < 2 * occ) ++en;
+            t1_parts_[size_t(L)].push_back({st - g0, en - st, P.fsz[P.group_fronts[size_t(st)]]});
+            st = en;
+        }
+    }
 }
 
 // Pivot side buffer: U11^-1 and L11^-1 of every front of the widest pivot +
@@ -461,11 +593,27 @@ void BatchedLU::factor_t(const double* A, int Bp, hipStream_t s, StageTimer* tim
             kern::schur_group<T>(lu_, f, cnt, mf, A, Bp, ar, uinv, opt_.dense_mode, st);
             return;
         }
+        if (t == 1) {
+            for (const T1Part& q : t1_parts_[size_t(L)])
+                kern::lu_factor_group<T>(lu_, 1, f + q.first, q.count, q.max_fsz, A, Bp, ar, opt_.pivot_eps,
+                                         sp, mem_.sing, opt_.dense_mode, st);
+            return;
+        }
         StageScope sc(timer, t == 2 && dense_kind_ == 0 ? stage : -1, st);
         kern::lu_factor_group<T>(lu_, t, f, cnt, mf, A, Bp, ar, opt_.pivot_eps, sp, mem_.sing,
                                  opt_.dense_mode, st);
     };
     for (int L = 0; L < plan_.num_levels; ++L) {
+        if (chain_level_[size_t(L)]) {
+            if constexpr (sizeof(T) == 4) {
+                const int c = chain_at_[size_t(L)];
+                if (c >= 0)
+                    kern::chain_factor(lu_, chains_[size_t(c)].dev, chains_[size_t(c)].fsz,
+                                       chains_[size_t(c)].nc, A, Bp, ar, uinv, opt_.pivot_eps, sp,
+                                       mem_.sing, opt_.dense_mode, s);
+            }
+            continue;
+        }
         const int t0 = first_tier(L);
         if (t0 > 2) continue;
         level_split(true, has_after(L, t0), [&](hipStream_t st) { run(L, t0, st); },

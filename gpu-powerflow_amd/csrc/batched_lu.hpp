@@ -39,6 +39,7 @@ struct LuOptions {
     int cap = 8;                // relaxed-panel cap
     double pivot_eps = 1e-8;    // static pivot threshold and replacement magnitude
     bool static_pivot = true;   // replace small pivots (StaticDiagonalShift)
+    bool chains = true;         // FP32 fronts: chains of nested fronts in one launch
 };
 
 class BatchedLU {
@@ -132,6 +133,8 @@ private:
     bool cross_schur(int level) const;
     bool tile_schur(int level) const;
     size_t uinv_elems(int Bp) const;
+    void find_chains(hipStream_t s);
+    void split_tier1();
     template <typename T>
     void factor_t(const double* A, int Bp, hipStream_t s, StageTimer* timer, int stage);
     template <typename T, typename TX>
@@ -157,6 +160,23 @@ private:
     // tile_ptr_[L] .. tile_ptr_[L + 1] of d_tiles
     std::vector<int> tile_ptr_;
     std::vector<char> level_cb_;      // per level: a tier-2 front with a contribution block
+    // chains of nested fronts (kern::chain_factor), launched at the level of
+    // their second front (chain_at_); the levels of F1..Fn (chain_level_) run
+    // no other launch
+    struct Chain {
+        DeviceBuffer d_fronts, d_eptr, d_edst, d_esptr, d_esrc;
+        kern::ChainDevice dev;
+        std::vector<int> fsz, nc;         // of F0..Fn
+    };
+    std::vector<Chain> chains_;
+    // per level: the tier-1 group's launches (fronts [first, first + count)
+    // of the group, the largest max_fsz rows)
+    struct T1Part {
+        int first, count, max_fsz;
+    };
+    std::vector<std::vector<T1Part>> t1_parts_;
+    std::vector<int> chain_at_;
+    std::vector<char> chain_level_;
     // single-system arena: front p at d_soff[p]
     DeviceBuffer d_soff, single_store_;
     kern::SingleDims sdims_;

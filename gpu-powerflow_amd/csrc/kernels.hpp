@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <vector>
 
 namespace cupf::kern {
 
@@ -204,6 +205,32 @@ void cross_group(const LuDevice& d, const int* fronts, int count, int max_fsz, c
 template <typename T>
 void schur_group(const LuDevice& d, const int* fronts, int count, int max_fsz, const double* A,
                  int Bp, Arenas<T> ar, const T* uinv, int dense_mode, hipStream_t s);
+
+// Chain of nested fronts F0 -> F1 -> ... -> Fn (each front's contribution
+// rows are exactly its parent's rows, in order; F1..Fn alone on their
+// levels; at most 64 pivots each): F1..Fn factored in one launch, one
+// workgroup per scenario, in place in F0's trailing block (row stride
+// fsz(F0)) once F0's level is done.  Additions of step k (F_k's entries and
+// its other children's contribution elements): ext_dst[e] for e in
+// [ext_ptr[k - 1], ext_ptr[k]) is the element's offset in F0's trailing
+// block, its sources ext_src[ext_sptr[e] .. ext_sptr[e + 1]) in order, each
+// (kChainSrcJ, J value slot), (kChainSrcA0, tier-0 arena element) or
+// (kChainSrcA1, tier-1/2 arena element).  FP32 fronts; the side buffer holds
+// one (front, scenario) slot per scenario.  dense_mode 1 FP32 MFMA, 2 3xBF16.
+// Three launches per front of F1..Fn.
+constexpr int kChainSrcJ = 0, kChainSrcA0 = 1, kChainSrcA1 = 2;
+struct ChainDevice {
+    const int* fronts = nullptr;      // F0, F1, ..., Fn
+    int steps = 0;                    // n
+    const int* ext_ptr = nullptr;     // [n + 1]
+    const int* ext_dst = nullptr;
+    const int* ext_sptr = nullptr;
+    const int2* ext_src = nullptr;
+};
+// fsz / nc: sizes and pivots of F0..Fn (host copies).
+void chain_factor(const LuDevice& d, const ChainDevice& c, const std::vector<int>& fsz,
+                  const std::vector<int>& nc, const double* A, int Bp, Arenas<float> ar, float* side,
+                  double pivot_eps, int static_piv, int* sing, int dense_mode, hipStream_t s);
 
 // Tile path of a group whose fronts have at most 64 pivots (any size):
 // cross_group with with_l21 also writes L21 = A21 U11^-1 by blocks of

@@ -920,35 +920,16 @@ constexpr size_t u12_lds() {
 template <typename T>
 constexpr int pivot_threads() { return 256; }
 
-template <typename T>
-__global__ __launch_bounds__(pivot_threads<T>()) void k_pivot(LuDevice d, const int* __restrict__ fronts,
-                                                        const double* __restrict__ A, int Bp, Arenas<T> ar,
-                                                        T* __restrict__ side, double eps, int static_piv,
-                                                        int* __restrict__ sing) {
-    extern __shared__ unsigned char smem_raw[];
-    T* P = reinterpret_cast<T*>(smem_raw);                         // A11: [64][kPivLd], then transposed
-    int fi, b;
-    decode_block(Bp, fi, b);
-    const int p = fronts[fi];
-    const int fs = d.fsz[p], K = d.nc[p];
-    constexpr int NT = pivot_threads<T>();
+// The pivot block A11 (K x K, in P with row stride kPivLd, zero outside K x K)
+// of one (front, scenario) factored by a workgroup of NT threads (see
+// k_pivot): L11\U11 to the front's pivot block G (row stride fs), U11^-1
+// [k][j] then L11^-1 [i][k] to the side buffer at Ui, a guarded pivot
+// flagged in *sing_b.  P is overwritten.
+template <typename T, int NT>
+__device__ __forceinline__ void pivot_factor(T* __restrict__ P, T* __restrict__ G, int fs, int K,
+                                             T* __restrict__ Ui, double eps, int static_piv,
+                                             int* __restrict__ sing_b, const int* gate) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    T* G = ar.a1 + (long long)b * d.arena1 + d.off[p];
-    const int* rl = d.row_lb + d.lb_ptr[p];
-
-    // ---- A11 -------------------------------------------------------------
-    for (int e = tid; e < 64 * kPivLd; e += NT) P[e] = T(0);
-    __syncthreads();
-    put_entries<NT>(P, d.ent_dst, d.ent_src, A, Bp, b, rl[0], rl[K], [=](int dst) {
-        const int r = dst / fs, c = dst - r * fs;
-        return c < K ? r * kPivLd + c : -1;
-    });
-    __syncthreads();
-    assemble_chunks<NT, kAsmDepth>(d, ar, d.asm_base[p * kAsmKindCount + kAsmKPivot], b, Bp, P,
-                            [=](int r, int c) { return r * kPivLd + c; });
-    __syncthreads();
-
-    T* Ui = side + ((long long)fi * Bp + b) * kSideElems;         // U11^-1 [k][j]
     T* Lo = Ui + kUinvElems;                                      // L11^-1 [i][k]
     T row[kXMaxK];                                                // lane's row of L\U
     if (wave == 0) {
@@ -972,7 +953,7 @@ __global__ __launch_bounds__(pivot_threads<T>()) void k_pivot(LuDevice d, const 
         // done): PT[c][r] = (L\U)[r][c], zero outside K x K
 #pragma unroll
         for (int c = 0; c < kXMaxK; ++c) P[c * kPivLd + lane] = row[c];
-        if (flag && !gate_closed(d.gate)) atomicOr(sing + b, 1);
+        if (flag && !gate_closed(gate)) atomicOr(sing_b, 1);
     }
     __syncthreads();
     if (wave == 0) {
@@ -1024,6 +1005,100 @@ __global__ __launch_bounds__(pivot_threads<T>()) void k_pivot(LuDevice d, const 
     }
 }
 
+template <typename T>
+__global__ __launch_bounds__(pivot_threads<T>()) void k_pivot(LuDevice d, const int* __restrict__ fronts,
+                                                        const double* __restrict__ A, int Bp, Arenas<T> ar,
+                                                        T* __restrict__ side, double eps, int static_piv,
+                                                        int* __restrict__ sing) {
+    extern __shared__ unsigned char smem_raw[];
+    T* P = reinterpret_cast<T*>(smem_raw);                         // A11: [64][kPivLd], then transposed
+    int fi, b;
+    decode_block(Bp, fi, b);
+    const int p = fronts[fi];
+    const int fs = d.fsz[p], K = d.nc[p];
+    constexpr int NT = pivot_threads<T>();
+    const int tid = threadIdx.x;
+    T* G = ar.a1 + (long long)b * d.arena1 + d.off[p];
+    const int* rl = d.row_lb + d.lb_ptr[p];
+
+    // ---- A11 -------------------------------------------------------------
+    for (int e = tid; e < 64 * kPivLd; e += NT) P[e] = T(0);
+    __syncthreads();
+    put_entries<NT>(P, d.ent_dst, d.ent_src, A, Bp, b, rl[0], rl[K], [=](int dst) {
+        const int r = dst / fs, c = dst - r * fs;
+        return c < K ? r * kPivLd + c : -1;
+    });
+    __syncthreads();
+    assemble_chunks<NT, kAsmDepth>(d, ar, d.asm_base[p * kAsmKindCount + kAsmKPivot], b, Bp, P,
+                            [=](int r, int c) { return r * kPivLd + c; });
+    __syncthreads();
+    pivot_factor<T, NT>(P, G, fs, K, side + ((long long)fi * Bp + b) * kSideElems, eps, static_piv,
+                        sing + b, d.gate);
+}
+
+// U12 chunk product: G[i][j0 + jj] = (L11^-1 Ch)[i][jj] for i < K, jj < jn,
+// Li = L11^-1 in LDS (row stride kLinvLd), Ch the A12 chunk in LDS (row
+// stride kXCld, zero beyond K rows), four waves (FP32 or FP64 MFMA).
+template <typename T>
+__device__ __forceinline__ void u12_product(const T* __restrict__ Li, const T* __restrict__ Ch,
+                                            T* __restrict__ G, int fs, int K, int j0, int jn) {
+    const int lane = int(threadIdx.x) & 63, wave = int(threadIdx.x) >> 6;
+    if constexpr (sizeof(T) == 4) {
+        // 64 x 64 output = 2 x 2 tiles of 32 x 32, one per wave; L11^-1 is
+        // lower triangular, so the top row half needs k < 32 only
+        const int rh = wave & 1, ct = wave >> 1;
+        if (32 * ct < jn && 32 * rh < K) {  // wave-uniform
+            floatx16 acc;
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[v] = 0.f;
+            const int kend = rh == 0 ? 32 : 64;
+            for (int kk = 0; kk < kend; kk += 2) {
+                const int k = kk + (lane >> 5);
+                const float av = Li[(32 * rh + (lane & 31)) * kLinvLd + k];
+                const float bv = Ch[k * kXCld + 32 * ct + (lane & 31)];
+                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv, acc, 0, 0, 0);
+            }
+            const int j = j0 + 32 * ct + (lane & 31);
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const int i = 32 * rh + (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
+                if (i < K && j < fs) G[(long long)i * fs + j] = acc[v];
+            }
+        }
+    } else {
+        // FP64 MFMA: 64 x 64 output = 4 x 4 tiles of 16 x 16, wave w the
+        // 16-row strip w; L11^-1 is lower (zero above the diagonal, unit
+        // beyond K, where the chunk rows are zero), so k < 16 w + 16
+        if (16 * wave < K) {                           // wave-uniform
+            doublex4 acc[4];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) acc[c] = doublex4{0.0, 0.0, 0.0, 0.0};
+            const int r = 16 * wave + (lane & 15), q = lane >> 4;
+            const int kend = min(K, 16 * wave + 16);
+            for (int kk = 0; kk < kend; kk += 4) {
+                const int k = kk + q;
+                const double av = Li[r * kLinvLd + k];
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    if (16 * c < jn) {                 // uniform
+                        const double bv = Ch[k * kXCld + 16 * c + (lane & 15)];
+                        acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[c], 0, 0, 0);
+                    }
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const int jj = 16 * c + (lane & 15);
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int i = 16 * wave + q + 4 * v;
+                    if (i < K && jj < jn) G[(long long)i * fs + j0 + jj] = acc[c][v];
+                }
+            }
+        }
+    }
+}
+
 // U12 = L11^-1 A12 for one chunk of kXChunk columns of one (front, scenario):
 // A12 chunk assembled in LDS (entries, then the chunk map),
 // L11^-1 from the side buffer, the product on FP32 or FP64 MFMA, written to
@@ -1040,7 +1115,7 @@ __device__ __forceinline__ void u12_chunk(const LuDevice& d, const int* __restri
     const int fs = d.fsz[p], K = d.nc[p];
     const int j0 = K + ch * kXChunk;
     if (j0 >= fs) return;                       // whole block exits together
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x;
     T* G = ar.a1 + (long long)b * d.arena1 + d.off[p];
     const int* rl = d.row_lb + d.lb_ptr[p];
     const T* Lo = side + ((long long)fi * Bp + b) * kSideElems + kUinvElems;
@@ -1064,60 +1139,7 @@ __device__ __forceinline__ void u12_chunk(const LuDevice& d, const int* __restri
             Li[(e >> 6) * kLinvLd + (e & 63)] = lv[u];
         }
         lds_barrier();
-        if constexpr (sizeof(T) == 4) {
-            // 64 x 64 output = 2 x 2 tiles of 32 x 32, one per wave; L11^-1 is
-            // lower triangular, so the top row half needs k < 32 only
-            const int rh = wave & 1, ct = wave >> 1;
-            if (32 * ct < jn && 32 * rh < K) {  // wave-uniform
-                floatx16 acc;
-#pragma unroll
-                for (int v = 0; v < 16; ++v) acc[v] = 0.f;
-                const int kend = rh == 0 ? 32 : 64;
-                for (int kk = 0; kk < kend; kk += 2) {
-                    const int k = kk + (lane >> 5);
-                    const float av = Li[(32 * rh + (lane & 31)) * kLinvLd + k];
-                    const float bv = Ch[k * kXCld + 32 * ct + (lane & 31)];
-                    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv, acc, 0, 0, 0);
-                }
-                const int j = j0 + 32 * ct + (lane & 31);
-#pragma unroll
-                for (int v = 0; v < 16; ++v) {
-                    const int i = 32 * rh + (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
-                    if (i < K && j < fs) G[(long long)i * fs + j] = acc[v];
-                }
-            }
-        } else {
-            // FP64 MFMA: 64 x 64 output = 4 x 4 tiles of 16 x 16, wave w the
-            // 16-row strip w; L11^-1 is lower (zero above the diagonal, unit
-            // beyond K, where the chunk rows are zero), so k < 16 w + 16
-            if (16 * wave < K) {                           // wave-uniform
-                doublex4 acc[4];
-#pragma unroll
-                for (int c = 0; c < 4; ++c) acc[c] = doublex4{0.0, 0.0, 0.0, 0.0};
-                const int r = 16 * wave + (lane & 15), q = lane >> 4;
-                const int kend = min(K, 16 * wave + 16);
-                for (int kk = 0; kk < kend; kk += 4) {
-                    const int k = kk + q;
-                    const double av = Li[r * kLinvLd + k];
-#pragma unroll
-                    for (int c = 0; c < 4; ++c) {
-                        if (16 * c < jn) {                 // uniform
-                            const double bv = Ch[k * kXCld + 16 * c + (lane & 15)];
-                            acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[c], 0, 0, 0);
-                        }
-                    }
-                }
-#pragma unroll
-                for (int c = 0; c < 4; ++c) {
-                    const int jj = 16 * c + (lane & 15);
-#pragma unroll
-                    for (int v = 0; v < 4; ++v) {
-                        const int i = 16 * wave + q + 4 * v;
-                        if (i < K && jj < jn) G[(long long)i * fs + j0 + jj] = acc[c][v];
-                    }
-                }
-            }
-        }
+        u12_product<T>(Li, Ch, G, fs, K, j0, jn);
     }
 }
 
@@ -1133,6 +1155,132 @@ constexpr int schur_rows() { return sizeof(T) == 4 ? 32 : 16; }
 template <typename T>
 inline size_t schur_lds(int max_fsz) {
     return (size_t(schur_rows<T>()) * kSLd + size_t(schur_rows<T>()) * max_fsz) * sizeof(T);
+}
+
+// acc0 / acc1 (the 32 x 32 halves of a 32 x 64 tile) -= L21 U12 over k < K:
+// L21 row lane % 32 at Lrow (LDS); the U12 operands of columns ja and ja + 32
+// in b0 / b1, B-operand k of slot i:
+//   FP32 (32x32x2):  k = 2 i + lane/32;
+//   3xBF16 (32x32x16): k = 16 (i / 8) + 8 (lane/32) + i % 8.
+template <int MODE>
+__device__ __forceinline__ void cb_tile_update(floatx16& acc0, floatx16& acc1,
+                                               const float (&b0)[kXMaxK / 2],
+                                               const float (&b1)[kXMaxK / 2],
+                                               const float* __restrict__ Lrow, int K) {
+    const int lane = int(threadIdx.x) & 63;
+    if (MODE == kMfmaF32) {
+#pragma unroll
+        for (int i = 0; i < kXMaxK / 2; ++i) {
+            if (2 * i < K) {                // uniform
+                const float a = -Lrow[2 * i + (lane >> 5)];
+                acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b0[i], acc0, 0, 0, 0);
+                acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b1[i], acc1, 0, 0, 0);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int kq = 0; kq < kXMaxK / 16; ++kq) {
+            if (16 * kq < K) {              // uniform
+                const int kb8 = 16 * kq + 8 * (lane >> 5);
+                bf16x8 ah, al, b0h, b0l, b1h, b1l;
+#pragma unroll
+                for (int t8 = 0; t8 < 8; ++t8) {
+                    __bf16 h, lo2;
+                    split_bf16(-Lrow[kb8 + t8], h, lo2);
+                    ah[t8] = h; al[t8] = lo2;
+                    split_bf16(b0[8 * kq + t8], h, lo2);
+                    b0h[t8] = h; b0l[t8] = lo2;
+                    split_bf16(b1[8 * kq + t8], h, lo2);
+                    b1h[t8] = h; b1l[t8] = lo2;
+                }
+                acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, b0h, acc0, 0, 0, 0);
+                acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, b0l, acc0, 0, 0, 0);
+                acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, b0h, acc0, 0, 0, 0);
+                acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, b1h, acc1, 0, 0, 0);
+                acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, b1l, acc1, 0, 0, 0);
+                acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, b1h, acc1, 0, 0, 0);
+            }
+        }
+    }
+}
+
+// U12 operands of one 32 x 32 tile held across several row strips: FP32
+// values (FP32 MFMA, B-operand k of slot i = 2 i + lane/32), or their BF16
+// high and low parts, split once (3xBF16, k of element t8 of group kq =
+// 16 kq + 8 (lane/32) + t8).  Columns j (< uc) of U12, whose row k starts at
+// U + k * fs; loads from indices clamped into the block, zeroed outside it.
+template <int MODE>
+struct U12Half {
+    float b[kXMaxK / 2];
+};
+template <>
+struct U12Half<kMfmaBf16x3> {
+    bf16x8 h[kXMaxK / 16], l[kXMaxK / 16];
+};
+
+// Load of base[e] by a 32-bit byte offset from a workgroup-uniform base
+// (one VGPR per address; e * 4 below 2^32 within a scenario's arena).
+__device__ __forceinline__ float ld_off(const float* base, int e) {
+    return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(base) + unsigned(e) * 4u);
+}
+__device__ __forceinline__ void st_off(float* base, int e, float v) {
+    *reinterpret_cast<float*>(reinterpret_cast<char*>(base) + unsigned(e) * 4u) = v;
+}
+
+template <int MODE>
+__device__ __forceinline__ void u12_half_load(U12Half<MODE>& o, const float* __restrict__ U, int j,
+                                              int uc, int fs, int K) {
+    const int lane = int(threadIdx.x) & 63;
+    const bool ok = j < uc;
+    const int cj = min(j, uc - 1);
+#pragma unroll
+    for (int i = 0; i < kXMaxK / 2; ++i) {
+        const int k = MODE == kMfmaF32 ? 2 * i + (lane >> 5) : 16 * (i >> 3) + 8 * (lane >> 5) + (i & 7);
+        const float v = ld_off(U, min(k, K - 1) * fs + cj);
+        const float x = (k < K && ok) ? v : 0.f;
+        if constexpr (MODE == kMfmaBf16x3) {
+            __bf16 h, lo2;
+            split_bf16(x, h, lo2);
+            o.h[i >> 3][i & 7] = h;
+            o.l[i >> 3][i & 7] = lo2;
+        } else {
+            o.b[i] = x;
+        }
+    }
+}
+
+// acc -= L21 U12 over k < K for one 32 x 32 tile with held operands: L21 row
+// lane % 32 at Lrow (LDS).
+template <int MODE>
+__device__ __forceinline__ void cb_half_update(floatx16& acc, const U12Half<MODE>& o,
+                                               const float* __restrict__ Lrow, int K) {
+    const int lane = int(threadIdx.x) & 63;
+    if constexpr (MODE == kMfmaBf16x3) {
+#pragma unroll
+        for (int kq = 0; kq < kXMaxK / 16; ++kq) {
+            if (16 * kq < K) {              // uniform
+                const int kb8 = 16 * kq + 8 * (lane >> 5);
+                bf16x8 ah, al;
+#pragma unroll
+                for (int t8 = 0; t8 < 8; ++t8) {
+                    __bf16 h, lo2;
+                    split_bf16(-Lrow[kb8 + t8], h, lo2);
+                    ah[t8] = h; al[t8] = lo2;
+                }
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, o.h[kq], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, o.l[kq], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, o.h[kq], acc, 0, 0, 0);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < kXMaxK / 2; ++i) {
+            if (2 * i < K) {                // uniform
+                const float a = -Lrow[2 * i + (lane >> 5)];
+                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, o.b[i], acc, 0, 0, 0);
+            }
+        }
+    }
 }
 
 template <typename T, int MODE>
@@ -1250,16 +1398,14 @@ __global__ __launch_bounds__(kSchurThreads, 2) void k_schur(LuDevice d, const in
         //   FP32 (32x32x2):  k = 2 i + lane/32;
         //   3xBF16 (32x32x16): k = 16 (i / 8) + 8 (lane/32) + i % 8.
         float b0[kXMaxK / 2], b1[kXMaxK / 2];
-        auto kslot = [&](int i) {
-            return MODE == kMfmaF32 ? 2 * i + (lane >> 5) : 16 * (i >> 3) + 8 * (lane >> 5) + (i & 7);
-        };
         auto load_u = [&](int t) {
             const int ja = t * 64 + (lane & 31);
             const bool oka = ja < uc, okb = ja + 32 < uc;
             const float* U = G + K + ja;        // U12 row k at U + k * fs
 #pragma unroll
             for (int i = 0; i < kXMaxK / 2; ++i) {
-                const int k = kslot(i);
+                const int k = MODE == kMfmaF32 ? 2 * i + (lane >> 5)
+                                               : 16 * (i >> 3) + 8 * (lane >> 5) + (i & 7);
                 b0[i] = (k < K && oka) ? U[(long long)k * fs] : 0.f;
                 b1[i] = (k < K && okb) ? U[(long long)k * fs + 32] : 0.f;
             }
@@ -1306,40 +1452,7 @@ __global__ __launch_bounds__(kSchurThreads, 2) void k_schur(LuDevice d, const in
                 acc0[v] = oka ? Ab[rr * uc + ja] : 0.f;
                 acc1[v] = okb ? Ab[rr * uc + ja + 32] : 0.f;
             }
-            if (MODE == kMfmaF32) {
-#pragma unroll
-                for (int i = 0; i < kXMaxK / 2; ++i) {
-                    if (2 * i < K) {                // uniform
-                        const float a = -Lb[(lane & 31) * kSLd + 2 * i + (lane >> 5)];
-                        acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b0[i], acc0, 0, 0, 0);
-                        acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b1[i], acc1, 0, 0, 0);
-                    }
-                }
-            } else {
-#pragma unroll
-                for (int kq = 0; kq < kXMaxK / 16; ++kq) {
-                    if (16 * kq < K) {              // uniform
-                        const int kb8 = 16 * kq + 8 * (lane >> 5);
-                        bf16x8 ah, al, b0h, b0l, b1h, b1l;
-#pragma unroll
-                        for (int t8 = 0; t8 < 8; ++t8) {
-                            __bf16 h, lo2;
-                            split_bf16(-Lb[(lane & 31) * kSLd + kb8 + t8], h, lo2);
-                            ah[t8] = h; al[t8] = lo2;
-                            split_bf16(b0[8 * kq + t8], h, lo2);
-                            b0h[t8] = h; b0l[t8] = lo2;
-                            split_bf16(b1[8 * kq + t8], h, lo2);
-                            b1h[t8] = h; b1l[t8] = lo2;
-                        }
-                        acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, b0h, acc0, 0, 0, 0);
-                        acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, b0l, acc0, 0, 0, 0);
-                        acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, b0h, acc0, 0, 0, 0);
-                        acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, b1h, acc1, 0, 0, 0);
-                        acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, b1l, acc1, 0, 0, 0);
-                        acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, b1h, acc1, 0, 0, 0);
-                    }
-                }
-            }
+            cb_tile_update<MODE>(acc0, acc1, b0, b1, Lb + (lane & 31) * kSLd, K);
             // next tile's U12 operands in flight during this tile's stores
             if (t + kSchurThreads / 64 < ntile) load_u(t + kSchurThreads / 64);
             float* Cw = G + (long long)i0 * fs + K + ja;
@@ -1351,6 +1464,255 @@ __global__ __launch_bounds__(kSchurThreads, 2) void k_schur(LuDevice d, const in
                     if (okb) Cw[(long long)rr * fs + 32] = acc1[v];
                 }
             }
+        }
+    }
+}
+
+// ===========================================================================
+// Chains of nested fronts.  When a front's contribution rows are exactly its
+// parent's rows, in order (the parent is the next panel of one supernode cut
+// by the panel cap), the parent front is the child's trailing block plus the
+// parent's own entries and other children.  In a chain F0 -> F1 -> ... -> Fn
+// of such fronts (F1..Fn each alone on its level), F1..Fn are factored in
+// place in F0's trailing block T (row stride fsz(F0)) once F0's level is
+// done: F_k occupies T's rows and columns [o_k, o_k + fsz(F_k)), o_k = the
+// pivots of F1..F_{k-1}.  Step k, three launches:
+//   k_chain_pivot (one workgroup per scenario): F_k's entries and its other
+//     children's contribution elements are added into T (one thread per
+//     element of T, its additions in a fixed, host-built order); then the
+//     pivot block is read from T and factored (pivot_factor: L11\U11 to
+//     F_k, U11^-1 and L11^-1 to the scenario's side buffer);
+//   k_chain_u12 (one workgroup per (64-column chunk, scenario)): U12 =
+//     L11^-1 A12 with the chunk of A12 read from T (u12_product), to F_k;
+//   k_chain_strips (one workgroup per (four 32-row strips, scenario)):
+//     L21 = A21 U11^-1 on FP32 MFMA, a strip per wave (A21 read from T, L21
+//     to F_k and to the strip's LDS block), then the strips' columns beyond
+//     the pivots -= L21 U12 in 32 x 32 tiles (cb_half_update, U12 read from
+//     F_k and held across the strips of one column tile), in place in T (for
+//     the last front, whose parent reads its contribution block from its
+//     arena, written there instead).
+// F_k's own arena holds its pivot block, U12 and L21 where the solves read
+// them.  Nothing of the trailing block goes through a chunk map.
+// ===========================================================================
+
+constexpr int kChainThreads = 256;
+constexpr int kChainLd = kXMaxK + 2;                 // L21 strip row stride (LDS)
+constexpr int kChainStrip = 32;                      // rows per strip
+constexpr int kChainStrips = kChainThreads / 64;     // strips per workgroup
+
+// Wave-local ordering of LDS writes before reads by other lanes of the wave.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+}
+
+// Block decode of a chain launch of `parts` workgroups per scenario: an XCD
+// owns a contiguous range of (scenario, part), the parts of one scenario
+// together (they share its U12 in L2).  The block count is a multiple of 8.
+__device__ inline void decode_chain(int parts, int& part, int& b) {
+    const long long lin = blockIdx.x;
+    const long long per = (long long)gridDim.x / kXcd;
+    const long long w = (lin % kXcd) * per + lin / kXcd;
+    part = int(w % parts);
+    b = int(w / parts);
+}
+
+// F0's trailing block of scenario b
+__device__ __forceinline__ float* chain_base(const LuDevice& d, const ChainDevice& ch, float* arena,
+                                             int& ldT) {
+    const int f0 = ch.fronts[0];
+    ldT = d.fsz[f0];
+    return arena + d.off[f0] + (long long)d.nc[f0] * ldT + d.nc[f0];
+}
+
+__global__ __launch_bounds__(kChainThreads) void k_chain_pivot(LuDevice d, ChainDevice ch, int k,
+                                                                int o, const double* __restrict__ A,
+                                                                int Bp, Arenas<float> ar,
+                                                                float* __restrict__ side, double eps,
+                                                                int static_piv,
+                                                                int* __restrict__ sing) {
+    extern __shared__ unsigned char smem_raw[];
+    float* P = reinterpret_cast<float*>(smem_raw);                 // [64][kPivLd]
+    const int b = int(blockIdx.x);
+    const int tid = threadIdx.x;
+    float* arena = ar.a1 + (long long)b * d.arena1;
+    int ldT;
+    float* Tb = chain_base(d, ch, arena, ldT);
+    const int p = ch.fronts[k];
+    const int fs = d.fsz[p], K = d.nc[p];
+    const float* Tk = Tb + (long long)o * ldT + o;                 // F_k's element (0, 0)
+    for (int e = ch.ext_ptr[k - 1] + tid; e < ch.ext_ptr[k]; e += kChainThreads) {
+        float* dst = Tb + ch.ext_dst[e];
+        float v = *dst;
+        for (int q = ch.ext_sptr[e]; q < ch.ext_sptr[e + 1]; ++q) {
+            const int2 sr = ch.ext_src[q];
+            v += sr.x == kChainSrcJ    ? float(A[(long long)sr.y * Bp + b])
+                 : sr.x == kChainSrcA0 ? ar.a0[(long long)sr.y * Bp + b]
+                                       : arena[sr.y];
+        }
+        *dst = v;
+    }
+    __syncthreads();                                               // T complete (global, this block)
+    for (int e = tid; e < 64 * kPivLd; e += kChainThreads) {
+        const int r = e / kPivLd, c = e - r * kPivLd;
+        P[e] = (r < K && c < K) ? Tk[(long long)r * ldT + c] : 0.f;
+    }
+    __syncthreads();
+    pivot_factor<float, kChainThreads>(P, arena + d.off[p], fs, K, side + (long long)b * kSideElems,
+                                       eps, static_piv, sing + b, d.gate);
+}
+
+__global__ __launch_bounds__(kPivThreads) void k_chain_u12(LuDevice d, ChainDevice ch, int k, int o,
+                                                           int Bp, Arenas<float> ar,
+                                                           const float* __restrict__ side, int nch) {
+    extern __shared__ unsigned char smem_raw[];
+    float* Li = reinterpret_cast<float*>(smem_raw);               // [64][kLinvLd]
+    float* Chk = Li + 64 * kLinvLd;                               // [64][kXCld]
+    int chn, b;
+    decode_chain(nch, chn, b);
+    const int p = ch.fronts[k];
+    const int fs = d.fsz[p], K = d.nc[p];
+    const int j0 = K + chn * kXChunk;
+    if (j0 >= fs || b >= Bp) return;            // whole block exits together
+    const int tid = threadIdx.x;
+    float* arena = ar.a1 + (long long)b * d.arena1;
+    int ldT;
+    const float* Tk = chain_base(d, ch, arena, ldT) + (long long)o * ldT + o;
+    const float* Lo = side + (long long)b * kSideElems + kUinvElems;
+    const int jn = min(kXChunk, fs - j0);
+    for (int e = tid; e < kUinvElems; e += kPivThreads) Li[(e >> 6) * kLinvLd + (e & 63)] = Lo[e];
+    for (int e = tid; e < 64 * kXCld; e += kPivThreads) {
+        const int r = e / kXCld, c = e - r * kXCld;
+        Chk[e] = (r < K && c < jn) ? Tk[(long long)r * ldT + j0 + c] : 0.f;
+    }
+    lds_barrier();
+    u12_product<float>(Li, Chk, arena + d.off[p], fs, K, j0, jn);
+}
+
+template <int MODE>
+__global__ __launch_bounds__(kChainThreads, 2) void k_chain_strips(LuDevice d, ChainDevice ch, int k,
+                                                                   int o, int Bp, Arenas<float> ar,
+                                                                   const float* __restrict__ side,
+                                                                   int ngrp, int last) {
+    extern __shared__ unsigned char smem_raw[];
+    float* Us = reinterpret_cast<float*>(smem_raw);               // U11^-1 [64][kLinvLd]
+    int g, b;
+    decode_chain(ngrp, g, b);
+    if (b >= Bp) return;                        // whole block exits together
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    float* Lw = Us + 64 * kLinvLd + wave * kChainStrip * kChainLd;  // this wave's strip
+    const int p = ch.fronts[k];
+    const int fs = d.fsz[p], K = d.nc[p], uc = fs - K;
+    float* arena = ar.a1 + (long long)b * d.arena1;
+    float* G = arena + d.off[p];
+    int ldT;
+    float* Tk = chain_base(d, ch, arena, ldT) + (long long)o * ldT + o;
+    const float* Ui = side + (long long)b * kSideElems;
+    for (int e = tid; e < kUinvElems; e += kChainThreads) Us[(e >> 6) * kLinvLd + (e & 63)] = Ui[e];
+    __syncthreads();
+    const int nstrip = (uc + kChainStrip - 1) / kChainStrip;
+    const int s0 = g * kChainStrips;                               // first strip of the block
+    const int ns = min(kChainStrips, nstrip - s0);
+    if (wave < ns) {
+        // ---- L21 = A21 U11^-1 for strip s0 + wave: A21 through the wave's LDS
+        // strip, A row lane % 32, k = 2 i + lane / 32; U11^-1 is upper, so the
+        // first 32 columns need k < 32 only
+        const int i0 = K + kChainStrip * (s0 + wave);
+        const int nr = min(kChainStrip, fs - i0);
+        for (int e = lane; e < kChainStrip * kXMaxK; e += 64) {
+            const int r = e >> 6, c = e & 63;
+            Lw[r * kChainLd + c] = (r < nr && c < K) ? Tk[(long long)(i0 + r) * ldT + c] : 0.f;
+        }
+        wave_lds_sync();
+        floatx16 l0, l1;
+#pragma unroll
+        for (int v = 0; v < 16; ++v) { l0[v] = 0.f; l1[v] = 0.f; }
+        {
+            const float* arow = Lw + (lane & 31) * kChainLd;
+#pragma unroll
+            for (int i = 0; i < kXMaxK / 2; ++i) {
+                if (2 * i < K) {                                   // uniform
+                    const int kk = 2 * i + (lane >> 5);
+                    const float av = arow[kk];
+                    if (i < 16)
+                        l0 = __builtin_amdgcn_mfma_f32_32x32x2f32(av, Us[kk * kLinvLd + (lane & 31)], l0,
+                                                                  0, 0, 0);
+                    if (K > 32)
+                        l1 = __builtin_amdgcn_mfma_f32_32x32x2f32(av, Us[kk * kLinvLd + 32 + (lane & 31)],
+                                                                  l1, 0, 0, 0);
+                }
+            }
+        }
+        wave_lds_sync();                                           // A21 reads done
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+            const int rr = (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
+            const int c0 = lane & 31, c1 = 32 + (lane & 31);
+            const float x0 = c0 < K ? l0[v] : 0.f;
+            const float x1 = c1 < K ? l1[v] : 0.f;
+            Lw[rr * kChainLd + c0] = x0;
+            Lw[rr * kChainLd + c1] = x1;
+            if (rr < nr) {
+                if (c0 < K) G[(long long)(i0 + rr) * fs + c0] = x0;
+                if (c1 < K) G[(long long)(i0 + rr) * fs + c1] = x1;
+            }
+        }
+    }
+    __syncthreads();                                               // every strip's L21 in LDS
+    // ---- C = A22 - L21 U12 over the block's strips, 32 x 32 tiles in place in
+    // T (the last front: to its own trailing block).  Wave w takes the column
+    // tiles w, w + 4, ...: a tile's U12 operands and its rows of two strips
+    // load together (all in flight at once), the two are updated and stored,
+    // then the next two.  Row and column indices are clamped into the strip:
+    // an element outside it holds some other element's value, which only its
+    // own (unstored) result depends on.
+    const int nct = (uc + 31) / 32;
+    const int ldC = last ? fs : ldT;
+    float* Cb = last ? G : Tk;
+    // strip q's rows (clamped: a strip past the block repeats the block's
+    // last strip, and stores nothing)
+    auto load_pair = [&](int sp, int cj, floatx16 (&acc)[2]) {
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int sq = min(sp + q, ns - 1);
+            const int i0 = K + kChainStrip * (s0 + sq);
+            const int nr = min(kChainStrip, fs - i0);
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const int rr = (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
+                acc[q][v] = ld_off(Tk, (i0 + min(rr, nr - 1)) * ldT + K + cj);
+            }
+        }
+    };
+    auto update_pair = [&](int sp, int j, const U12Half<MODE>& ops, floatx16 (&acc)[2]) {
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int sq = min(sp + q, ns - 1);
+            cb_half_update<MODE>(acc[q], ops,
+                                 Us + 64 * kLinvLd + sq * kChainStrip * kChainLd + (lane & 31) * kChainLd, K);
+            const int i0 = K + kChainStrip * (s0 + sp + q);
+            const int nr = sp + q < ns ? min(kChainStrip, fs - i0) : 0;
+            if (j < uc) {
+#pragma unroll
+                for (int v = 0; v < 16; ++v) {
+                    const int rr = (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
+                    if (rr < nr) st_off(Cb, (i0 + rr) * ldC + K + j, acc[q][v]);
+                }
+            }
+        }
+    };
+    for (int ct = wave; ct < nct; ct += kChainStrips) {
+        const int j = ct * 32 + (lane & 31);                       // CB column
+        const int cj = min(j, uc - 1);
+        U12Half<MODE> ops;
+        floatx16 acc[2];
+        load_pair(0, cj, acc);
+        u12_half_load<MODE>(ops, G + K, j, uc, fs, K);
+        update_pair(0, j, ops, acc);
+        if (ns > 2) {                                              // uniform
+            load_pair(2, cj, acc);
+            update_pair(2, j, ops, acc);
         }
     }
 }
@@ -2556,6 +2918,53 @@ void schur_tile_group(const LuDevice& d, const int* fronts, const int2* tiles, i
                            fronts, tiles, ntiles, scen_group, A, Bp, ar);
     }
     CUPF_HIP(hipGetLastError());
+}
+
+void chain_factor(const LuDevice& d, const ChainDevice& c, const std::vector<int>& fsz,
+                  const std::vector<int>& nc, const double* A, int Bp, Arenas<float> ar, float* side,
+                  double pivot_eps, int static_piv, int* sing, int dense_mode, hipStream_t s) {
+    if (c.steps <= 0) return;
+    if (int(fsz.size()) != c.steps + 1 || int(nc.size()) != c.steps + 1)
+        throw std::invalid_argument("chain_factor: front sizes do not match the chain");
+    const size_t piv_lds = size_t(64 * kPivLd) * sizeof(float);
+    const size_t u12_bytes = u12_lds<float>();
+    const size_t strip_lds = size_t(64 * kLinvLd + kChainStrips * kChainStrip * kChainLd) * sizeof(float);
+    static bool attr_set = false;
+    if (!attr_set) {
+        CUPF_HIP(hipFuncSetAttribute((const void*)k_chain_u12, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     int(u12_bytes)));
+        CUPF_HIP(hipFuncSetAttribute((const void*)k_chain_strips<kMfmaF32>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, int(strip_lds)));
+        CUPF_HIP(hipFuncSetAttribute((const void*)k_chain_strips<kMfmaBf16x3>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, int(strip_lds)));
+        attr_set = true;
+    }
+    int o = 0;
+    for (int k = 1; k <= c.steps; ++k) {
+        const int fs = fsz[size_t(k)], K = nc[size_t(k)], uc = fs - K;
+        if (K < 1 || K > kXMaxK || uc < 0 || o + fs > fsz[0] - nc[0])
+            throw std::invalid_argument("chain_factor: front outside the chain's trailing block");
+        hipLaunchKernelGGL(k_chain_pivot, dim3(unsigned(Bp)), dim3(kChainThreads), piv_lds, s, d, c, k, o,
+                           A, Bp, ar, side, pivot_eps, static_piv, sing);
+        CUPF_HIP(hipGetLastError());
+        if (uc > 0) {
+            const int nch = (uc + kXChunk - 1) / kXChunk;
+            hipLaunchKernelGGL(k_chain_u12, dim3(unsigned(nch) * unsigned(Bp)), dim3(kPivThreads), u12_bytes, s,
+                               d, c, k, o, Bp, ar, side, nch);
+            CUPF_HIP(hipGetLastError());
+            const int nstrip = (uc + kChainStrip - 1) / kChainStrip;
+            const int ngrp = (nstrip + kChainStrips - 1) / kChainStrips;
+            const int last = k == c.steps ? 1 : 0;
+            if (dense_mode == kMfmaBf16x3)
+                hipLaunchKernelGGL(k_chain_strips<kMfmaBf16x3>, dim3(unsigned(ngrp) * unsigned(Bp)),
+                                   dim3(kChainThreads), strip_lds, s, d, c, k, o, Bp, ar, side, ngrp, last);
+            else
+                hipLaunchKernelGGL(k_chain_strips<kMfmaF32>, dim3(unsigned(ngrp) * unsigned(Bp)),
+                                   dim3(kChainThreads), strip_lds, s, d, c, k, o, Bp, ar, side, ngrp, last);
+            CUPF_HIP(hipGetLastError());
+        }
+        o += K;
+    }
 }
 
 size_t pivot_side_elems(int count, int Bp) { return size_t(count) * Bp * kSideElems; }
