@@ -753,4 +753,36 @@ AsmLists build_asm_lists(const LuPlan& L, const std::vector<int>& kinds,
     return A;
 }
 
+std::vector<std::vector<int>> nested_chains(const LuPlan& P, int max_pivots) {
+    std::vector<int> level_count(size_t(P.num_levels), 0), member(size_t(P.num_fronts), -1);
+    for (int g = 0; g < P.num_levels * 3; ++g) level_count[size_t(g / 3)] += P.group_ptr[g + 1] - P.group_ptr[g];
+    auto nested_child = [&](int q) {
+        for (int k = P.child_ptr[q]; k < P.child_ptr[q + 1]; ++k) {
+            const int c = P.child_idx[k];
+            if (P.tier[c] != 2 || P.fsz[c] - P.nc[c] != P.fsz[q]) continue;
+            bool id = true;
+            for (int a = 0; id && a < P.fsz[q]; ++a) id = P.asm_loc[size_t(P.asm_ptr[c] + a)] == a;
+            if (id) return c;
+        }
+        return -1;
+    };
+    std::vector<std::vector<int>> lists;
+    for (int L = 0; L < P.num_levels; ++L) {
+        const int gi = L * 3 + 2;
+        if (level_count[size_t(L)] != 1 || P.group_ptr[gi + 1] == P.group_ptr[gi]) continue;
+        const int q = P.group_fronts[size_t(P.group_ptr[gi])];
+        if (P.nc[q] > max_pivots) continue;
+        const int c = nested_child(q);
+        if (c < 0 || P.group_max_nc[size_t(P.level[c] * 3 + 2)] > max_pivots) continue;
+        if (member[size_t(c)] >= 0) {
+            member[size_t(q)] = member[size_t(c)];
+            lists[size_t(member[size_t(q)])].push_back(q);
+        } else {
+            member[size_t(q)] = int(lists.size());
+            lists.push_back({c, q});
+        }
+    }
+    return lists;
+}
+
 }  // namespace cupf

@@ -142,6 +142,14 @@ struct LuPlan {
 LuPlan lower_plan(const Symbolic& s, int n, const int* rp, const int* ci, int tier0_max = 16,
                   int tier1_max = 128);
 
+// Chains of nested fronts [F0, F1, ..., Fn] of a plan.  q is the nested
+// parent of its tier-2 child c when c's contribution rows are exactly q's
+// rows in order (fsz(c) - nc(c) == fsz(q), identity row map).  Levels are
+// walked upwards; a tier-2 front with at most max_pivots pivots alone on its
+// level whose nested child has a level of at most max_pivots pivots per front
+// extends the child's chain, or starts one [child, front].
+std::vector<std::vector<int>> nested_chains(const LuPlan& P, int max_pivots);
+
 // ---------------------------------------------------------------------------
 // Assembly lists.  A factor kernel assembles a block of a front (a
 // rectangle of front rows x columns) from the children's contribution blocks.

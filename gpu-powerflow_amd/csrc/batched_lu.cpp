@@ -278,48 +278,18 @@ void BatchedLU::analyze(int n, const int* rp, const int* ci, const std::vector<i
     nnz_ = nnz;
 }
 
-// Chains of nested fronts (FP32 fronts; kern::chain_factor).  A front q is
-// the nested parent of its child c when c's contribution rows are exactly
-// q's rows in order (fsz(c) - nc(c) == fsz(q), identity row map).  Levels are
-// walked upwards; a tier-2 front with at most 64 pivots alone on its level
-// whose nested child is already in a chain extends that chain, else it starts
-// one with the child as F0 (F0's own level factored as usual, its pivot +
-// Schur group writing the contribution block to its trailing block).  The
-// additions of each member (its entries, then its other children in child
-// order) are listed per element of F0's trailing block.
+// Chains of nested fronts (FP32 fronts; kern::chain_factor, nested_chains).
+// F0's own level is factored as usual, its pivot + Schur group writing the
+// contribution block to its trailing block.  The additions of each member
+// (its entries, then its other children in child order) are listed per
+// element of F0's trailing block.
 void BatchedLU::find_chains(hipStream_t s) {
     const LuPlan& P = plan_;
     chains_.clear();
     chain_at_.assign(size_t(P.num_levels), -1);
     chain_level_.assign(size_t(P.num_levels), 0);
     if (opt_.front64 || !opt_.chains) return;
-    std::vector<int> level_count(size_t(P.num_levels), 0), member(size_t(P.num_fronts), -1);
-    for (int g = 0; g < P.num_levels * 3; ++g) level_count[size_t(g / 3)] += P.group_ptr[g + 1] - P.group_ptr[g];
-    auto nested_child = [&](int q) {
-        for (int k = P.child_ptr[q]; k < P.child_ptr[q + 1]; ++k) {
-            const int c = P.child_idx[k];
-            if (P.tier[c] != 2 || P.fsz[c] - P.nc[c] != P.fsz[q]) continue;
-            bool id = true;
-            for (int a = 0; id && a < P.fsz[q]; ++a) id = P.asm_loc[size_t(P.asm_ptr[c] + a)] == a;
-            if (id) return c;
-        }
-        return -1;
-    };
-    std::vector<std::vector<int>> lists;
-    for (int L = 0; L < P.num_levels; ++L) {
-        if (level_count[size_t(L)] != 1 || !has_group(L, 2)) continue;
-        const int q = P.group_fronts[size_t(P.group_ptr[L * 3 + 2])];
-        if (!kern::cross_schur_path(P.fsz[q], P.nc[q])) continue;
-        const int c = nested_child(q);
-        if (c < 0 || !cross_schur(P.level[c])) continue;
-        if (member[size_t(c)] >= 0) {
-            member[size_t(q)] = member[size_t(c)];
-            lists[size_t(member[size_t(q)])].push_back(q);
-        } else {
-            member[size_t(q)] = int(lists.size());
-            lists.push_back({c, q});
-        }
-    }
+    const std::vector<std::vector<int>> lists = nested_chains(P, kern::kCrossMaxK);
     for (const std::vector<int>& F : lists) {
         Chain ch;
         const int ldT = P.fsz[F[0]];

@@ -553,6 +553,26 @@ int cupf_plan_figures(int32_t n, const int32_t* row_ptr, const int32_t* col_idx,
     });
 }
 
+int cupf_plan_chains(int32_t n, const int32_t* row_ptr, const int32_t* col_idx,
+                     const int32_t* perm, int32_t cap, int32_t* sizes, int32_t max_sizes,
+                     int32_t* out2) {
+    return guarded([&] {
+        if (!row_ptr || !col_idx || !perm || !out2) throw std::invalid_argument("null argument");
+        const std::vector<int> pv(perm, perm + n);
+        const cupf::Symbolic s = cupf::analyze_symbolic(n, row_ptr, col_idx, pv, cap);
+        const cupf::LuPlan p = cupf::lower_plan(s, n, row_ptr, col_idx, 16, 128);
+        const std::vector<std::vector<int>> ch = cupf::nested_chains(p, cupf::kern::kCrossMaxK);
+        int k = 0;
+        for (const auto& F : ch)
+            for (int f : F) {
+                if (sizes && k < max_sizes) sizes[k] = p.fsz[f];
+                ++k;
+            }
+        out2[0] = int32_t(ch.size());
+        out2[1] = k;
+    });
+}
+
 int cupf_plan_maps_check(int32_t n, const int32_t* row_ptr, const int32_t* col_idx,
                          const int32_t* perm, int32_t cap, int64_t* out8) {
     return guarded([&] {

@@ -219,6 +219,8 @@ void schur_group(const LuDevice& d, const int* fronts, int count, int max_fsz, c
 // one (front, scenario) slot per scenario.  dense_mode 1 FP32 MFMA, 2 3xBF16.
 // Three launches per front of F1..Fn.
 constexpr int kChainSrcJ = 0, kChainSrcA0 = 1, kChainSrcA1 = 2;
+// most pivots of a front on the pivot + Schur path (cross_schur_path)
+constexpr int kCrossMaxK = 64;
 struct ChainDevice {
     const int* fronts = nullptr;      // F0, F1, ..., Fn
     int steps = 0;                    // n

@@ -887,7 +887,7 @@ __global__ __launch_bounds__(kDenseThreads) void k_factor_t2_mfma(LuDevice d,
 // ===========================================================================
 
 constexpr int kPivThreads = 256;
-constexpr int kXMaxK = 64;
+constexpr int kXMaxK = kCrossMaxK;
 constexpr int kXChunk = 64;                   // A12 columns per U12 step
 constexpr int kXCld = kXChunk + 4;            // chunk row stride (elements)
 constexpr int kL21Ld = kXMaxK + 2;            // A21 block row stride (elements)

@@ -268,6 +268,15 @@ int cupf_symbolic(int32_t n, const int32_t* row_ptr, const int32_t* col_idx,
 int cupf_plan_figures(int32_t n, const int32_t* row_ptr, const int32_t* col_idx,
                       const int32_t* perm, int32_t cap, double* out5);
 
+/* Chains of nested fronts of the lowered plan (FP32 fronts) for (pattern,
+ * perm, cap): fronts whose contribution rows are exactly their parent's rows,
+ * factored in place by the chain kernels (host analysis only; no reference
+ * counterpart).  out2 = {chains, fronts in them (F0..Fn each)}; sizes, when
+ * not NULL, gets the front sizes chain after chain, at most max_sizes. */
+int cupf_plan_chains(int32_t n, const int32_t* row_ptr, const int32_t* col_idx,
+                     const int32_t* perm, int32_t cap, int32_t* sizes, int32_t max_sizes,
+                     int32_t* out2);
+
 /* Coverage of the lowered plan's assembly chunk maps and solve gather maps
  * for (pattern, perm, cap) (host analysis only; no reference counterpart:
  * the reference assembles through its extend-add map directly):

@@ -108,6 +108,17 @@ def plan_figures(row_ptr, col_idx, perm, cap):
             "lnz": int(out[4])}
 
 
+def plan_chains(row_ptr, col_idx, perm, cap, max_sizes=256):
+    """Chains of nested fronts of the FP32 plan: counts and the front sizes."""
+    n = len(row_ptr) - 1
+    sizes = np.zeros(max_sizes, np.int32)
+    out = np.zeros(2, np.int32)
+    check(lib().cupf_plan_chains(ctypes.c_int32(n), _p(_i32(row_ptr)), _p(_i32(col_idx)),
+                                 _p(_i32(perm)), ctypes.c_int32(cap), _p(sizes),
+                                 ctypes.c_int32(max_sizes), _p(out)))
+    return {"chains": int(out[0]), "fronts": int(out[1]), "sizes": sizes[:min(int(out[1]), max_sizes)]}
+
+
 def exported_symbols():
     """Function names declared in include/cupf_hip.h."""
     import re

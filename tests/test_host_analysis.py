@@ -172,3 +172,22 @@ def test_plan_figures_are_consistent(jac):
     assert f8["lnz"] == fbig["lnz"]
     assert fbig["fronts"] <= f8["fronts"]
     assert fbig["sum_fsz2"] <= f8["sum_fsz2"]
+
+
+def test_nested_chain_of_the_2000_bus_plan(syn):
+    """At panel cap 64 the 2,000-bus plan's top separator is cut into 64-pivot
+    panels, each front's contribution rows being exactly the next front's
+    rows: one chain of nested fronts, factored in place by the chain kernels
+    (the GPU parity tests at cap 64 run it).  Nesting fixes the sizes: each
+    front is the previous one less its pivots (at most 64), down to the root.
+    Without a panel cap the separator stays one front and no chain forms."""
+    g = syn.workload_grid("case2000")
+    a = capi.analyze_jacobian(g.n_bus, g.indptr, g.indices, g.pv, g.pq)
+    perm = capi.order_jacobian(g.n_bus, g.indptr, g.indices, g.pv, g.pq, a["dim"])
+    ch = capi.plan_chains(a["row_ptr"], a["col_idx"], perm, 64)
+    assert ch["chains"] == 1 and ch["fronts"] >= 3
+    steps = -np.diff(ch["sizes"])
+    assert (steps > 0).all() and (steps <= 64).all()
+    fig = capi.plan_figures(a["row_ptr"], a["col_idx"], perm, 64)
+    assert ch["sizes"][0] == fig["max_fsz"]          # the top separator's first panel
+    assert capi.plan_chains(a["row_ptr"], a["col_idx"], perm, 1 << 20)["chains"] == 0
