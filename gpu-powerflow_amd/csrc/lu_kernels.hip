@@ -2336,31 +2336,32 @@ __global__ __launch_bounds__(kLanes) void k_bwd_t12(LuDevice d, const int* __res
 // (gather map: every position's loads independent).  Used for FP32 fronts;
 // FP64 fronts keep the one-wave sweeps above, which measure faster there.
 // ===========================================================================
-constexpr int kSolveK = 64;
-constexpr int kSolvePld = kSolveK + 1;          // pivot-block row stride in LDS
+constexpr int kSolveK = 64;                      // pivot-block capacity of the solve kernels
+// tier-1 fronts (at most 32 pivots): one wave per (front, scenario)
+constexpr int kSolveK1 = 32;
 
-// Pivot block of G (K x K, row stride fs) into registers, one row per wave
-// and instruction (row r = wave + waves * u, a lane per column, clamped),
-// then into LDS P[K][kSolvePld].
-template <int NT, typename T>
+// Pivot block of G (K x K, K <= KM, row stride fs) into registers, one row
+// per wave and instruction (row r = wave + waves * u, a lane per column,
+// clamped), then into LDS P[K][KM + 1].
+template <int NT, int KM, typename T>
 __device__ inline void load_pivot(const T* __restrict__ G, int fs, int K, T* pv) {
     constexpr int NW = NT / 64;
     const int lane = int(threadIdx.x) & 63, wave = int(threadIdx.x) >> 6;
 #pragma unroll
-    for (int u = 0; u < kSolveK / NW; ++u) {
+    for (int u = 0; u < KM / NW; ++u) {
         const int r = wave + NW * u;
         if (r < K) pv[u] = G[(long long)r * fs + min(lane, K - 1)];   // wave-uniform branch
     }
 }
 
-template <int NT, typename T>
+template <int NT, int KM, typename T>
 __device__ inline void store_pivot(int K, const T* pv, T* P) {
     constexpr int NW = NT / 64;
     const int lane = int(threadIdx.x) & 63, wave = int(threadIdx.x) >> 6;
 #pragma unroll
-    for (int u = 0; u < kSolveK / NW; ++u) {
+    for (int u = 0; u < KM / NW; ++u) {
         const int r = wave + NW * u;
-        if (r < K && lane < K) P[r * kSolvePld + lane] = pv[u];
+        if (r < K && lane < K) P[r * (KM + 1) + lane] = pv[u];
     }
 }
 
@@ -2397,13 +2398,14 @@ __device__ inline void l21_update(const T* __restrict__ G, int fs, int K, T* w) 
     }
 }
 
-template <typename T, bool TRANS, int NT>
+template <typename T, bool TRANS, int NT, int KM = kSolveK>
 __global__ __launch_bounds__(NT) void k_fwd_w(LuDevice d, const int* __restrict__ fronts,
                                               Arenas<T> ar, const double* __restrict__ rhs,
                                               T* __restrict__ y, T* __restrict__ upd, int Bp) {
+    constexpr int PLD = KM + 1;                            // pivot-block row stride in LDS
     extern __shared__ unsigned char smem_raw[];
-    T* P = reinterpret_cast<T*>(smem_raw);                 // [64][kSolvePld]
-    T* w = P + kSolveK * kSolvePld;                        // [fs]
+    T* P = reinterpret_cast<T*>(smem_raw);                 // [KM][PLD]
+    T* w = P + KM * PLD;                                   // [fs]
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     int fi, b;
     decode_block(Bp, fi, b);
@@ -2411,21 +2413,21 @@ __global__ __launch_bounds__(NT) void k_fwd_w(LuDevice d, const int* __restrict_
     const int fs = d.fsz[p], K = d.nc[p], uc = fs - K;
     const int* fr = d.front_rows + d.front_ptr[p];
     const T* G = ar.a1 + (long long)b * d.arena1 + d.off[p];
-    T pv[kSolveK * 64 / NT];
-    load_pivot<NT>(G, fs, K, pv);                          // in flight during the gather
+    T pv[KM * 64 / NT];
+    load_pivot<NT, KM>(G, fs, K, pv);                      // in flight during the gather
     gather_w<NT>(d, p, b, Bp, fs, K, rhs, upd, w);
-    store_pivot<NT>(K, pv, P);
+    store_pivot<NT, KM>(K, pv, P);
     __syncthreads();
     if (wave == 0) {
         T wi = lane < K ? w[lane] : T(0);
         for (int k = 0; k < K; ++k) {
             if (TRANS) {                                   // U11^T: lower, non-unit
-                if (lane == k) wi /= P[k * kSolvePld + k];
+                if (lane == k) wi /= P[k * PLD + k];
                 const T yk = bcast(wi, k);
-                if (lane > k && lane < K) wi -= P[k * kSolvePld + lane] * yk;
+                if (lane > k && lane < K) wi -= P[k * PLD + lane] * yk;
             } else {                                       // L11: unit lower
                 const T yk = bcast(wi, k);
-                if (lane > k && lane < K) wi -= P[lane * kSolvePld + k] * yk;
+                if (lane > k && lane < K) wi -= P[lane * PLD + k] * yk;
             }
         }
         if (lane < K) w[lane] = wi;
@@ -2451,13 +2453,14 @@ __global__ __launch_bounds__(NT) void k_fwd_w(LuDevice d, const int* __restrict_
     for (int a = tid; a < uc; a += NT) u[a] = w[K + a];
 }
 
-template <typename T, bool TRANS, int NT>
+template <typename T, bool TRANS, int NT, int KM = kSolveK>
 __global__ __launch_bounds__(NT) void k_bwd_w(LuDevice d, const int* __restrict__ fronts,
                                               Arenas<T> ar, T* __restrict__ y, int Bp) {
+    constexpr int PLD = KM + 1;                            // pivot-block row stride in LDS
     extern __shared__ unsigned char smem_raw[];
-    T* P = reinterpret_cast<T*>(smem_raw);                 // [64][kSolvePld]
-    T* part = P + kSolveK * kSolvePld;                     // [NW][64]
-    T* w = part + (NT / 64) * kSolveK;                     // [fs]
+    T* P = reinterpret_cast<T*>(smem_raw);                 // [KM][PLD]
+    T* part = P + KM * PLD;                                // [NW][KM]
+    T* w = part + (NT / 64) * KM;                          // [fs]
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     constexpr int NW = NT / 64;
     int fi, b;
@@ -2466,17 +2469,17 @@ __global__ __launch_bounds__(NT) void k_bwd_w(LuDevice d, const int* __restrict_
     const int fs = d.fsz[p], K = d.nc[p], uc = fs - K;
     const int* fr = d.front_rows + d.front_ptr[p];
     const T* G = ar.a1 + (long long)b * d.arena1 + d.off[p];
-    T pv[kSolveK * 64 / NT];
-    load_pivot<NT>(G, fs, K, pv);
+    T pv[KM * 64 / NT];
+    load_pivot<NT, KM>(G, fs, K, pv);
     for (int l = tid; l < fs; l += NT) w[l] = y[(long long)fr[l] * Bp + b];
-    store_pivot<NT>(K, pv, P);
+    store_pivot<NT, KM>(K, pv, P);
     __syncthreads();
     if (TRANS) {
         // w[k] -= sum_r L21[r][k] w[r]: lanes along k, rows split over waves
         T acc = T(0);
         if (lane < K)
             for (int r = K + wave; r < fs; r += NW) acc += G[(long long)r * fs + lane] * w[r];
-        part[wave * kSolveK + lane] = acc;
+        if (lane < KM) part[wave * KM + lane] = acc;
     } else {
         // w[k] -= U12[k] . w[K:fs]: 16 lanes per pivot row, 4 columns per lane
         // per step, 16 loads in flight, partial sums combined by shuffles
@@ -2510,18 +2513,18 @@ __global__ __launch_bounds__(NT) void k_bwd_w(LuDevice d, const int* __restrict_
         T wi = lane < K ? w[lane] : T(0);
         if (TRANS) {
 #pragma unroll
-            for (int q = 0; q < NW; ++q) wi -= part[q * kSolveK + lane];
+            for (int q = 0; q < NW; ++q) wi -= lane < K ? part[q * KM + lane] : T(0);
         } else {
             wi -= lane < K ? part[lane] : T(0);       // part[k] written for k < K
         }
         for (int k = K - 1; k >= 0; --k) {
             if (TRANS) {                                   // L11^T: unit upper
                 const T xk = bcast(wi, k);
-                if (lane < k) wi -= P[k * kSolvePld + lane] * xk;
+                if (lane < k) wi -= P[k * PLD + lane] * xk;
             } else {                                       // U11: upper, non-unit
-                if (lane == k) wi /= P[k * kSolvePld + k];
+                if (lane == k) wi /= P[k * PLD + k];
                 const T xk = bcast(wi, k);
-                if (lane < k) wi -= P[lane * kSolvePld + k] * xk;
+                if (lane < k) wi -= P[lane * PLD + k] * xk;
             }
         }
         if (lane < K) y[(long long)fr[lane] * Bp + b] = wi;
@@ -2529,8 +2532,8 @@ __global__ __launch_bounds__(NT) void k_bwd_w(LuDevice d, const int* __restrict_
 }
 
 template <int NT, typename T>
-inline size_t solve_w_lds(int max_fsz) {
-    return (size_t(kSolveK) * kSolvePld + size_t(NT / 64) * kSolveK + size_t(max_fsz)) * sizeof(T);
+inline size_t solve_w_lds(int max_fsz, int km = kSolveK) {
+    return (size_t(km) * (km + 1) + size_t(NT / 64) * km + size_t(max_fsz)) * sizeof(T);
 }
 
 template <typename T, typename TX>
@@ -3030,6 +3033,14 @@ void lu_solve_forward_group(const LuDevice& d, int tier, const int* fronts, int 
         else
             hipLaunchKernelGGL((k_fwd_t0<T, false>), dim3(count, Bp / kLanes), dim3(kLanes), 0, s,
                                d, fronts, ar, rhs, y, upd, Bp);
+    } else if (sizeof(T) == 4 && tier == 1 && max_nc <= kSolveK1) {
+        const size_t shmem = solve_w_lds<64, T>(max_fsz, kSolveK1);
+        if (transpose)
+            hipLaunchKernelGGL((k_fwd_w<T, true, 64, kSolveK1>), grid, dim3(64), shmem, s, d, fronts,
+                               ar, rhs, y, upd, Bp);
+        else
+            hipLaunchKernelGGL((k_fwd_w<T, false, 64, kSolveK1>), grid, dim3(64), shmem, s, d, fronts,
+                               ar, rhs, y, upd, Bp);
     } else if (sizeof(T) == 4 && max_nc <= kSolveK) {
         const size_t shmem = solve_w_lds<256, T>(max_fsz);
         if (transpose)
@@ -3063,6 +3074,14 @@ void lu_solve_backward_group(const LuDevice& d, int tier, const int* fronts, int
         else
             hipLaunchKernelGGL((k_bwd_t0<T, false>), dim3(count, Bp / kLanes), dim3(kLanes), 0, s,
                                d, fronts, ar, y, Bp);
+    } else if (sizeof(T) == 4 && tier == 1 && max_nc <= kSolveK1) {
+        const size_t shmem = solve_w_lds<64, T>(max_fsz, kSolveK1);
+        if (transpose)
+            hipLaunchKernelGGL((k_bwd_w<T, true, 64, kSolveK1>), grid, dim3(64), shmem, s, d, fronts,
+                               ar, y, Bp);
+        else
+            hipLaunchKernelGGL((k_bwd_w<T, false, 64, kSolveK1>), grid, dim3(64), shmem, s, d, fronts,
+                               ar, y, Bp);
     } else if (sizeof(T) == 4 && max_nc <= kSolveK) {
         const size_t shmem = solve_w_lds<256, T>(max_fsz);
         if (transpose)
