@@ -39,6 +39,11 @@ namespace cupf::kern {
 namespace {
 
 constexpr int kFactorBlock = 256;
+// tier-1 fronts of at most this many rows are factored by one wave
+#ifndef CUPF_T1_WAVE_ROWS
+#define CUPF_T1_WAVE_ROWS 48
+#endif
+constexpr int kT1WaveRows = CUPF_T1_WAVE_ROWS;
 constexpr int kLdsBytes = 64 * 1024;
 constexpr int kKbMax = 32;
 constexpr int kTile = 64;
@@ -267,21 +272,24 @@ __device__ __forceinline__ T bcast(T x, int lane) {
 }
 
 // Tier 1: the front (fsz <= 128, at most kT1MaxK pivots) is assembled and
-// factored in LDS: wave 0 factors the pivot block with one row per lane in
-// registers (the pivot row broadcast by v_readlane, guarded pivots); then,
-// at once, threads 0..127 form U12 = L11^-1 A12 (a column each) and threads
-// 128..255 L21 = A21 U11^-1 (a row each); the contribution block gets its
+// factored in LDS by NT threads: wave 0 factors the pivot block with one row
+// per lane in registers (the pivot row broadcast by v_readlane, guarded
+// pivots); then U12 = L11^-1 A12 (a column per thread) and L21 = A21 U11^-1
+// (a row per thread) -- with 256 threads at once, threads 0..127 and
+// 128..255, with fewer one after the other; the contribution block gets its
 // rank-K update C -= L21 U12 in 4 x 4 register tiles written straight to
-// the arena, then the pivot rows and columns follow from LDS.
+// the arena, then the pivot rows and columns follow from LDS.  Small fronts
+// run one wave per (front, scenario), so four times the scenarios are in
+// flight per CU.
 
-template <typename T>
-__global__ __launch_bounds__(kFactorBlock) void k_factor_t1(LuDevice d,
+template <typename T, int NT = kFactorBlock>
+__global__ __launch_bounds__(NT) void k_factor_t1(LuDevice d,
                                                             const int* __restrict__ fronts,
                                                             const double* __restrict__ A, int Bp,
                                                             Arenas<T> ar, double eps,
                                                             int static_piv,
                                                             int* __restrict__ sing) {
-    static_assert(kFactorBlock == 256, "two 128-thread halves for the triangular solves");
+    static_assert(NT == 256 || NT == 64, "two 128-thread halves, or one wave for everything");
     extern __shared__ unsigned char smem_raw[];
     T* F = reinterpret_cast<T*>(smem_raw);
     int fi, b;
@@ -289,7 +297,7 @@ __global__ __launch_bounds__(kFactorBlock) void k_factor_t1(LuDevice d,
     const int p = fronts[fi];
     const int fs = d.fsz[p], ncp = d.nc[p];
     const int tid = threadIdx.x, lane = tid & 63;
-    assemble_front(d, ar, A, Bp, p, b, fs, F);
+    assemble_front<T, NT>(d, ar, A, Bp, p, b, fs, F);
     __syncthreads();
     if (tid < 64) {
         // ---- LU of the pivot block, lane = row ----------------------------
@@ -317,10 +325,13 @@ __global__ __launch_bounds__(kFactorBlock) void k_factor_t1(LuDevice d,
         if (flag && !gate_closed(d.gate)) atomicOr(sing + b, 1);
     }
     __syncthreads();
-    if (tid < 128) {
+    // U12 columns j = ncp + c and L21 rows r = ncp + c for the thread's c
+    const int c_u = NT == 256 ? (tid < 128 ? tid : fs) : tid;
+    const int c_l = NT == 256 ? (tid < 128 ? fs : tid - 128) : tid;
+    for (int cu = c_u; ncp + cu < fs; cu += NT) {
         // ---- U12 column j: x_i = A12[i][j] - sum_{k<i} L_ik x_k ------------
-        const int j = ncp + tid;
-        if (j < fs) {
+        const int j = ncp + cu;
+        {
             T x[kT1MaxK];
 #pragma unroll
             for (int i = 0; i < kT1MaxK; ++i) {
@@ -333,10 +344,11 @@ __global__ __launch_bounds__(kFactorBlock) void k_factor_t1(LuDevice d,
                 }
             }
         }
-    } else {
+    }
+    for (int cl = c_l; ncp + cl < fs; cl += NT) {
         // ---- L21 row r: x_j = (A21[r][j] - sum_{k<j} x_k U_kj) / U_jj ------
-        const int r = ncp + tid - 128;
-        if (r < fs) {
+        const int r = ncp + cl;
+        {
             T x[kT1MaxK];
 #pragma unroll
             for (int jj = 0; jj < kT1MaxK; ++jj) {
@@ -357,7 +369,7 @@ __global__ __launch_bounds__(kFactorBlock) void k_factor_t1(LuDevice d,
     T* G = ar.a1 + (long long)b * d.arena1 + d.off[p];
     const int wc = fs - ncp;
     const int tn = (wc + 3) / 4;
-    for (int t = tid; t < tn * tn; t += kFactorBlock) {
+    for (int t = tid; t < tn * tn; t += NT) {
         const int ti = t / tn, tj = t - (t / tn) * tn;
         const int i0 = ncp + 4 * ti, j0 = ncp + 4 * tj;
         T acc[4][4];
@@ -384,8 +396,8 @@ __global__ __launch_bounds__(kFactorBlock) void k_factor_t1(LuDevice d,
                 if (i0 + r < fs && j0 + c < fs) G[(long long)(i0 + r) * fs + j0 + c] = acc[r][c];
     }
     // the pivot rows (L11\U11, U12) and pivot columns (L21) from LDS
-    for (int e = tid; e < ncp * fs; e += kFactorBlock) G[e] = F[e];
-    for (int e = tid; e < wc * ncp; e += kFactorBlock) {
+    for (int e = tid; e < ncp * fs; e += NT) G[e] = F[e];
+    for (int e = tid; e < wc * ncp; e += NT) {
         const int r = ncp + e / ncp, c = e - (e / ncp) * ncp;
         G[(long long)r * fs + c] = F[r * fs + c];
     }
@@ -3009,8 +3021,13 @@ void lu_factor_group(const LuDevice& d, int tier, const int* fronts, int count, 
     } else if (tier == 1) {
         const size_t shmem = size_t(max_fsz) * max_fsz * sizeof(T);
         if (shmem > size_t(kLdsBytes)) throw std::runtime_error("lu_factor_group: tier-1 front exceeds LDS");
-        hipLaunchKernelGGL(k_factor_t1<T>, dim3(unsigned(count) * unsigned(Bp)), dim3(kFactorBlock),
-                           shmem, s, d, fronts, A, Bp, ar, pivot_eps, static_piv, sing);
+        if (max_fsz <= kT1WaveRows)
+            hipLaunchKernelGGL((k_factor_t1<T, 64>), dim3(unsigned(count) * unsigned(Bp)), dim3(64),
+                               shmem, s, d, fronts, A, Bp, ar, pivot_eps, static_piv, sing);
+        else
+            hipLaunchKernelGGL((k_factor_t1<T, kFactorBlock>), dim3(unsigned(count) * unsigned(Bp)),
+                               dim3(kFactorBlock), shmem, s, d, fronts, A, Bp, ar, pivot_eps,
+                               static_piv, sing);
     } else {
         if (max_fsz > kLdsBytes / int(sizeof(T)))
             throw std::runtime_error("lu_factor_group: front exceeds the LDS panel budget");
