@@ -3050,7 +3050,7 @@ void lu_solve_forward_group(const LuDevice& d, int tier, const int* fronts, int 
         else
             hipLaunchKernelGGL((k_fwd_t0<T, false>), dim3(count, Bp / kLanes), dim3(kLanes), 0, s,
                                d, fronts, ar, rhs, y, upd, Bp);
-    } else if (sizeof(T) == 4 && tier == 1 && max_nc <= kSolveK1) {
+    } else if (tier == 1 && max_nc <= kSolveK1) {
         const size_t shmem = solve_w_lds<64, T>(max_fsz, kSolveK1);
         if (transpose)
             hipLaunchKernelGGL((k_fwd_w<T, true, 64, kSolveK1>), grid, dim3(64), shmem, s, d, fronts,
@@ -3058,7 +3058,7 @@ void lu_solve_forward_group(const LuDevice& d, int tier, const int* fronts, int 
         else
             hipLaunchKernelGGL((k_fwd_w<T, false, 64, kSolveK1>), grid, dim3(64), shmem, s, d, fronts,
                                ar, rhs, y, upd, Bp);
-    } else if (sizeof(T) == 4 && max_nc <= kSolveK) {
+    } else if (max_nc <= kSolveK) {
         const size_t shmem = solve_w_lds<256, T>(max_fsz);
         if (transpose)
             hipLaunchKernelGGL((k_fwd_w<T, true, 256>), grid, dim3(256), shmem, s, d, fronts, ar,
@@ -3091,7 +3091,7 @@ void lu_solve_backward_group(const LuDevice& d, int tier, const int* fronts, int
         else
             hipLaunchKernelGGL((k_bwd_t0<T, false>), dim3(count, Bp / kLanes), dim3(kLanes), 0, s,
                                d, fronts, ar, y, Bp);
-    } else if (sizeof(T) == 4 && tier == 1 && max_nc <= kSolveK1) {
+    } else if (tier == 1 && max_nc <= kSolveK1) {
         const size_t shmem = solve_w_lds<64, T>(max_fsz, kSolveK1);
         if (transpose)
             hipLaunchKernelGGL((k_bwd_w<T, true, 64, kSolveK1>), grid, dim3(64), shmem, s, d, fronts,
@@ -3099,7 +3099,7 @@ void lu_solve_backward_group(const LuDevice& d, int tier, const int* fronts, int
         else
             hipLaunchKernelGGL((k_bwd_w<T, false, 64, kSolveK1>), grid, dim3(64), shmem, s, d, fronts,
                                ar, y, Bp);
-    } else if (sizeof(T) == 4 && max_nc <= kSolveK) {
+    } else if (max_nc <= kSolveK) {
         const size_t shmem = solve_w_lds<256, T>(max_fsz);
         if (transpose)
             hipLaunchKernelGGL((k_bwd_w<T, true, 256>), grid, dim3(256), shmem, s, d, fronts, ar, y,
