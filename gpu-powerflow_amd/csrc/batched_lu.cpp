@@ -349,7 +349,8 @@ void BatchedLU::find_chains(hipStream_t s) {
 // largest front, so one launch over fronts of 34 to 128 rows would run every
 // workgroup at the occupancy of the 128-row one.  The group's fronts are
 // sorted by size, largest first, and a new part starts at the first front
-// that fits at least twice as many workgroups per CU as the part's first.
+// that fits at least twice as many workgroups per CU as the part's first, or
+// at the first front small enough for the one-wave kernel (kT1WaveRows).
 void BatchedLU::split_tier1() {
     LuPlan& P = plan_;
     const double lds_cu = 160.0 * 1024.0;
@@ -364,9 +365,14 @@ void BatchedLU::split_tier1() {
         std::stable_sort(P.group_fronts.begin() + g0, P.group_fronts.begin() + g1,
                          [&](int a, int b) { return P.fsz[a] > P.fsz[b]; });
         for (int st = g0; st < g1;) {
-            const int occ = per_cu(P.group_fronts[size_t(st)]);
+            const int first = P.group_fronts[size_t(st)];
+            const int occ = per_cu(first);
+            const bool wave = P.fsz[first] <= kern::kT1WaveRows;
             int en = st + 1;
-            while (en < g1 && per_cu(P.group_fronts[size_t(en)]) < 2 * occ) ++en;
+            // a part also ends where the fronts fit the one-wave kernel
+            while (en < g1 && per_cu(P.group_fronts[size_t(en)]) < 2 * occ &&
+                   (wave || P.fsz[P.group_fronts[size_t(en)]] > kern::kT1WaveRows))
+                ++en;
             t1_parts_[size_t(L)].push_back({st - g0, en - st, P.fsz[P.group_fronts[size_t(st)]]});
             st = en;
         }

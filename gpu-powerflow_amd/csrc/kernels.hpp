@@ -166,6 +166,9 @@ struct LuDevice {
 
 // Most pivots of a tier-1 front (the tier-1 factor kernel's register rows).
 constexpr int kT1MaxK = 32;
+// Tier-1 fronts of at most this many rows are factored by one wave per
+// (front, scenario) (lu_factor_group decides by the launch's largest front).
+constexpr int kT1WaveRows = 48;
 
 // Block height of the row-block assembly of a tier-2 group (elem = sizeof T).
 int assemble_rows_block(int max_fsz, int elem);

@@ -39,11 +39,7 @@ namespace cupf::kern {
 namespace {
 
 constexpr int kFactorBlock = 256;
-// tier-1 fronts of at most this many rows are factored by one wave
-#ifndef CUPF_T1_WAVE_ROWS
-#define CUPF_T1_WAVE_ROWS 48
-#endif
-constexpr int kT1WaveRows = CUPF_T1_WAVE_ROWS;
+
 constexpr int kLdsBytes = 64 * 1024;
 constexpr int kKbMax = 32;
 constexpr int kTile = 64;
