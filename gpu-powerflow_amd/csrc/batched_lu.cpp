@@ -344,10 +344,11 @@ void BatchedLU::find_chains(hipStream_t s) {
     }
 }
 
-// Tier-1 groups are launched in parts of similar LDS need: a tier-1
-// workgroup holds its front in fsz^2 elements of LDS sized by the launch's
-// largest front, so one launch over fronts of 34 to 128 rows would run every
-// workgroup at the occupancy of the 128-row one.  The group's fronts are
+// Tier-1 groups (and FP32 tier-0 groups) are launched in parts of similar LDS
+// need: a tier-1 workgroup holds its front in fsz^2 elements of LDS sized by
+// the launch's largest front (a tier-0 workgroup its 64 lanes' fronts), so one
+// launch over fronts of 34 to 128 rows would run every workgroup at the
+// occupancy of the 128-row one.  The group's fronts are
 // sorted by size, largest first, and a new part starts at the first front
 // that fits at least twice as many workgroups per CU as the part's first, or
 // at the first front small enough for the one-wave kernel (kT1WaveRows).
@@ -358,6 +359,27 @@ void BatchedLU::split_tier1() {
         const double b = double(P.fsz[f]) * P.fsz[f] * double(front_bytes());
         return std::min(8, std::max(1, int(lds_cu / std::max(1.0, b))));
     };
+    // tier 0 (FP32): 64 lanes' fronts of the launch's largest size in LDS per
+    // workgroup
+    auto per_cu0 = [&](int f) {
+        const double b = double(P.fsz[f]) * P.fsz[f] * 64.0 * double(front_bytes());
+        return std::min(8, std::max(1, int(lds_cu / std::max(1.0, b))));
+    };
+    t0_parts_.assign(size_t(P.num_levels), {});
+    for (int L = 0; L < P.num_levels; ++L) {
+        const int g0 = P.group_ptr[L * 3], g1 = P.group_ptr[L * 3 + 1];
+        if (g0 == g1) continue;
+        std::stable_sort(P.group_fronts.begin() + g0, P.group_fronts.begin() + g1,
+                         [&](int a, int b) { return P.fsz[a] > P.fsz[b]; });
+        for (int st = g0; st < g1;) {
+            const int occ = per_cu0(P.group_fronts[size_t(st)]);
+            int en = st + 1;
+            // FP64 tier-0 fronts above 11 rows do not fit the LDS kernel: one launch
+            while (en < g1 && (opt_.front64 || per_cu0(P.group_fronts[size_t(en)]) < 2 * occ)) ++en;
+            t0_parts_[size_t(L)].push_back({st - g0, en - st, P.fsz[P.group_fronts[size_t(st)]]});
+            st = en;
+        }
+    }
     t1_parts_.assign(size_t(P.num_levels), {});
     for (int L = 0; L < P.num_levels; ++L) {
         const int g0 = P.group_ptr[L * 3 + 1], g1 = P.group_ptr[L * 3 + 2];
@@ -569,9 +591,9 @@ void BatchedLU::factor_t(const double* A, int Bp, hipStream_t s, StageTimer* tim
             kern::schur_group<T>(lu_, f, cnt, mf, A, Bp, ar, uinv, opt_.dense_mode, st);
             return;
         }
-        if (t == 1) {
-            for (const T1Part& q : t1_parts_[size_t(L)])
-                kern::lu_factor_group<T>(lu_, 1, f + q.first, q.count, q.max_fsz, A, Bp, ar, opt_.pivot_eps,
+        if (t < 2) {
+            for (const T1Part& q : (t == 0 ? t0_parts_ : t1_parts_)[size_t(L)])
+                kern::lu_factor_group<T>(lu_, t, f + q.first, q.count, q.max_fsz, A, Bp, ar, opt_.pivot_eps,
                                          sp, mem_.sing, opt_.dense_mode, st);
             return;
         }

@@ -174,7 +174,7 @@ private:
     struct T1Part {
         int first, count, max_fsz;
     };
-    std::vector<std::vector<T1Part>> t1_parts_;
+    std::vector<std::vector<T1Part>> t0_parts_, t1_parts_;
     std::vector<int> chain_at_;
     std::vector<char> chain_level_;
     // single-system arena: front p at d_soff[p]

@@ -237,6 +237,62 @@ __global__ __launch_bounds__(kLanes) void k_factor_t0(LuDevice d, const int* __r
     if (flag && !gate_closed(d.gate)) atomicOr(sing + b, 1);
 }
 
+// Tier 0 by four waves over one LDS copy of 64 scenarios' fronts
+// (F[e * 64 + lane], lane = scenario): the zeroing, the entries and each
+// child's contribution elements are split over the waves (a child's elements
+// land on distinct positions; children one after another), then at each
+// pivot wave 0 guards the pivot and the rows below are split over the waves,
+// each row with k_factor_t0's arithmetic in its order; last, the factored
+// fronts go to the interleaved arena.  The lane's serial chain of k_factor_t0
+// becomes a quarter as long and waits on LDS instead of global memory.
+template <typename T>
+__global__ __launch_bounds__(256) void k_factor_t0_w4(LuDevice d, const int* __restrict__ fronts,
+                                                      const double* __restrict__ A, int Bp,
+                                                      Arenas<T> ar, double eps, int static_piv,
+                                                      int* __restrict__ sing) {
+    extern __shared__ unsigned char smem_raw[];
+    const int lane = int(threadIdx.x) & 63, wave = int(threadIdx.x) >> 6;
+    T* F = reinterpret_cast<T*>(smem_raw) + lane;          // element e at F[e * 64]
+    const int p = fronts[blockIdx.x];
+    const int b = blockIdx.y * kLanes + lane;
+    const int fs = d.fsz[p], ncp = d.nc[p];
+    for (int e = wave; e < fs * fs; e += 4) F[e * kLanes] = T(0);
+    __syncthreads();
+    for (int e = d.ent_ptr[p] + wave; e < d.ent_ptr[p + 1]; e += 4)
+        F[d.ent_dst[e] * kLanes] = T(A[(long long)d.ent_src[e] * Bp + b]);
+    __syncthreads();
+    for (int q = d.child_ptr[p]; q < d.child_ptr[p + 1]; ++q) {
+        const int c = d.child_idx[q];
+        const int cfs = d.fsz[c], cnc = d.nc[c], cuc = cfs - cnc;
+        long long cs;
+        const T* C = front_ptr(d, ar, c, b, Bp, cs);
+        const int* loc = d.asm_loc + d.asm_ptr[c];
+        for (int e = wave; e < cuc * cuc; e += 4) {
+            const int a = e / cuc, bb = e - a * cuc;
+            F[(loc[a] * fs + loc[bb]) * kLanes] += C[(long long)((cnc + a) * cfs + cnc + bb) * cs];
+        }
+        __syncthreads();
+    }
+    int flag = 0;
+    for (int k = 0; k < ncp; ++k) {
+        if (wave == 0) {
+            const T piv = F[(k * fs + k) * kLanes];
+            F[(k * fs + k) * kLanes] = guard_pivot(piv, eps, static_piv, flag);
+        }
+        __syncthreads();
+        const T g = F[(k * fs + k) * kLanes];
+        for (int i = k + 1 + wave; i < fs; i += 4) {
+            const T l = F[(i * fs + k) * kLanes] / g;
+            F[(i * fs + k) * kLanes] = l;
+            for (int j = k + 1; j < fs; ++j) F[(i * fs + j) * kLanes] -= l * F[(k * fs + j) * kLanes];
+        }
+        __syncthreads();
+    }
+    T* G = ar.a0 + d.off[p] * Bp + b;
+    for (int e = wave; e < fs * fs; e += 4) G[(long long)e * Bp] = F[e * kLanes];
+    if (wave == 0 && flag && !gate_closed(d.gate)) atomicOr(sing + b, 1);
+}
+
 // ===========================================================================
 // Factor, tier 1 (front in LDS) and tier 2 (blocked, in the arena)
 // ===========================================================================
@@ -3012,8 +3068,20 @@ void lu_factor_group(const LuDevice& d, int tier, const int* fronts, int count, 
     }
     if (tier == 0) {
         if (max_fsz > kTier0Max) throw std::runtime_error("lu_factor_group: tier-0 front too large");
-        hipLaunchKernelGGL(k_factor_t0<T>, dim3(count, Bp / kLanes), dim3(kLanes), 0, s, d, fronts,
-                           A, Bp, ar, pivot_eps, static_piv, sing);
+        const size_t shmem = size_t(max_fsz) * max_fsz * kLanes * sizeof(T);
+        if (shmem <= size_t(kLdsBytes)) {
+            static bool attr_set = false;
+            if (!attr_set) {
+                CUPF_HIP(hipFuncSetAttribute((const void*)k_factor_t0_w4<T>,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes));
+                attr_set = true;
+            }
+            hipLaunchKernelGGL(k_factor_t0_w4<T>, dim3(count, Bp / kLanes), dim3(256), shmem, s, d, fronts,
+                               A, Bp, ar, pivot_eps, static_piv, sing);
+        } else {
+            hipLaunchKernelGGL(k_factor_t0<T>, dim3(count, Bp / kLanes), dim3(kLanes), 0, s, d, fronts,
+                               A, Bp, ar, pivot_eps, static_piv, sing);
+        }
     } else if (tier == 1) {
         const size_t shmem = size_t(max_fsz) * max_fsz * sizeof(T);
         if (shmem > size_t(kLdsBytes)) throw std::runtime_error("lu_factor_group: tier-1 front exceeds LDS");
