@@ -380,6 +380,12 @@ void BatchedLU::split_tier1() {
             st = en;
         }
     }
+    t2_max_uc_.assign(size_t(P.num_levels), 0);
+    for (int L = 0; L < P.num_levels; ++L)
+        for (int q = P.group_ptr[L * 3 + 2]; q < P.group_ptr[L * 3 + 3]; ++q) {
+            const int f = P.group_fronts[size_t(q)];
+            t2_max_uc_[size_t(L)] = std::max(t2_max_uc_[size_t(L)], P.fsz[f] - P.nc[f]);
+        }
     t1_parts_.assign(size_t(P.num_levels), {});
     for (int L = 0; L < P.num_levels; ++L) {
         const int g0 = P.group_ptr[L * 3 + 1], g1 = P.group_ptr[L * 3 + 2];
@@ -588,7 +594,8 @@ void BatchedLU::factor_t(const double* A, int Bp, hipStream_t s, StageTimer* tim
                 return;
             }
             StageScope sc(timer, dense_kind_ == 1 ? stage : -1, st);
-            kern::schur_group<T>(lu_, f, cnt, mf, A, Bp, ar, uinv, opt_.dense_mode, st);
+            kern::schur_group<T>(lu_, f, cnt, plan_.group_max_nc[L * 3 + 2], t2_max_uc_[size_t(L)], A,
+                                 Bp, ar, uinv, opt_.dense_mode, st);
             return;
         }
         if (t < 2) {

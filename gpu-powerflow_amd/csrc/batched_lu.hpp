@@ -175,6 +175,8 @@ private:
         int first, count, max_fsz;
     };
     std::vector<std::vector<T1Part>> t0_parts_, t1_parts_;
+    // per level: the widest contribution block (fsz - nc) of the tier-2 group
+    std::vector<int> t2_max_uc_;
     std::vector<int> chain_at_;
     std::vector<char> chain_level_;
     // single-system arena: front p at d_soff[p]

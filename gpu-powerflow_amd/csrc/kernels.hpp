@@ -206,8 +206,9 @@ void cross_group(const LuDevice& d, const int* fronts, int count, int max_fsz, c
                  int Bp, Arenas<T> ar, T* uinv, double pivot_eps, int static_piv, int* sing,
                  bool with_l21, hipStream_t s);
 template <typename T>
-void schur_group(const LuDevice& d, const int* fronts, int count, int max_fsz, const double* A,
-                 int Bp, Arenas<T> ar, const T* uinv, int dense_mode, hipStream_t s);
+void schur_group(const LuDevice& d, const int* fronts, int count, int max_nc, int max_uc,
+                 const double* A, int Bp, Arenas<T> ar, const T* uinv, int dense_mode,
+                 hipStream_t s);
 
 // Chain of nested fronts F0 -> F1 -> ... -> Fn (each front's contribution
 // rows are exactly its parent's rows, in order; F1..Fn alone on their

@@ -1209,63 +1209,22 @@ __device__ __forceinline__ void u12_chunk(const LuDevice& d, const int* __restri
 
 constexpr int kSchurThreads = 256;
 constexpr int kSchurScen = 16;                 // scenarios interleaved inside an XCD range
-constexpr int kSLd = kXMaxK + 2;               // L21 block row stride (elements)
+constexpr int kSLd = kXMaxK + 2;               // L21 block row stride, fronts of 33-64 pivots
 constexpr int kMfmaNone = 0;                   // FP64: vector-unit FMA tiles
 
 // rows per Schur block: 32 (FP32 MFMA tiles), 16 (FP64, LDS budget)
 template <typename T>
 constexpr int schur_rows() { return sizeof(T) == 4 ? 32 : 16; }
 
-template <typename T>
-inline size_t schur_lds(int max_fsz) {
-    return (size_t(schur_rows<T>()) * kSLd + size_t(schur_rows<T>()) * max_fsz) * sizeof(T);
-}
+// L21 block row stride for at most max_nc pivots: the MFMA tiles read
+// L21 columns up to the next multiple of 32 (zeros beyond the pivots); two
+// more keep the rows' banks apart
+inline int schur_ld(int max_nc) { return max_nc <= 32 ? 34 : kSLd; }
 
-// acc0 / acc1 (the 32 x 32 halves of a 32 x 64 tile) -= L21 U12 over k < K:
-// L21 row lane % 32 at Lrow (LDS); the U12 operands of columns ja and ja + 32
-// in b0 / b1, B-operand k of slot i:
-//   FP32 (32x32x2):  k = 2 i + lane/32;
-//   3xBF16 (32x32x16): k = 16 (i / 8) + 8 (lane/32) + i % 8.
-template <int MODE>
-__device__ __forceinline__ void cb_tile_update(floatx16& acc0, floatx16& acc1,
-                                               const float (&b0)[kXMaxK / 2],
-                                               const float (&b1)[kXMaxK / 2],
-                                               const float* __restrict__ Lrow, int K) {
-    const int lane = int(threadIdx.x) & 63;
-    if (MODE == kMfmaF32) {
-#pragma unroll
-        for (int i = 0; i < kXMaxK / 2; ++i) {
-            if (2 * i < K) {                // uniform
-                const float a = -Lrow[2 * i + (lane >> 5)];
-                acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b0[i], acc0, 0, 0, 0);
-                acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b1[i], acc1, 0, 0, 0);
-            }
-        }
-    } else {
-#pragma unroll
-        for (int kq = 0; kq < kXMaxK / 16; ++kq) {
-            if (16 * kq < K) {              // uniform
-                const int kb8 = 16 * kq + 8 * (lane >> 5);
-                bf16x8 ah, al, b0h, b0l, b1h, b1l;
-#pragma unroll
-                for (int t8 = 0; t8 < 8; ++t8) {
-                    __bf16 h, lo2;
-                    split_bf16(-Lrow[kb8 + t8], h, lo2);
-                    ah[t8] = h; al[t8] = lo2;
-                    split_bf16(b0[8 * kq + t8], h, lo2);
-                    b0h[t8] = h; b0l[t8] = lo2;
-                    split_bf16(b1[8 * kq + t8], h, lo2);
-                    b1h[t8] = h; b1l[t8] = lo2;
-                }
-                acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, b0h, acc0, 0, 0, 0);
-                acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, b0l, acc0, 0, 0, 0);
-                acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, b0h, acc0, 0, 0, 0);
-                acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, b1h, acc1, 0, 0, 0);
-                acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, b1l, acc1, 0, 0, 0);
-                acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, b1h, acc1, 0, 0, 0);
-            }
-        }
-    }
+// LDS of a Schur block: the L21 block and rows of up to max_uc columns
+template <typename T>
+inline size_t schur_lds(int sld, int max_uc) {
+    return size_t(schur_rows<T>()) * size_t(sld + max_uc) * sizeof(T);
 }
 
 // U12 operands of one 32 x 32 tile held across several row strips: FP32
@@ -1348,14 +1307,15 @@ __device__ __forceinline__ void cb_half_update(floatx16& acc, const U12Half<MODE
 }
 
 template <typename T, int MODE>
-__global__ __launch_bounds__(kSchurThreads, 2) void k_schur(LuDevice d, const int* __restrict__ fronts,
+__global__ __launch_bounds__(kSchurThreads, 3) void k_schur(LuDevice d, const int* __restrict__ fronts,
                                                          const double* __restrict__ A, int Bp,
                                                          Arenas<T> ar,
-                                                         const T* __restrict__ uinv, int nrb) {
+                                                         const T* __restrict__ uinv, int nrb,
+                                                         int sld) {
     constexpr int RB = schur_rows<T>();
     extern __shared__ unsigned char smem_raw[];
-    T* Lb = reinterpret_cast<T*>(smem_raw);                       // [RB][kSLd]: A21, then L21
-    T* Ab = Lb + RB * kSLd;                                       // [RB][uc]
+    T* Lb = reinterpret_cast<T*>(smem_raw);                       // [RB][sld]: A21, then L21
+    T* Ab = Lb + RB * sld;                                        // [RB][uc]
     // lin -> (front, scenario group, row block, scenario in group): inside an
     // XCD's contiguous range, kSchurScen neighbouring scenarios run together
     // (they share the cache lines of the interleaved J values) over the row
@@ -1397,17 +1357,17 @@ __global__ __launch_bounds__(kSchurThreads, 2) void k_schur(LuDevice d, const in
     }
 
     // ---- rows [i0, i1) of A21 (-> Lb) and A22 (-> Ab) -----------------------
-    for (int e = tid; e < RB * (kSLd + uc); e += kSchurThreads) Lb[e] = T(0);
+    for (int e = tid; e < RB * (sld + uc); e += kSchurThreads) Lb[e] = T(0);
     const int lo = rl[i0], hi = rl[i1];
     __syncthreads();
     put_entries<kSchurThreads>(Lb, d.ent_dst, d.ent_src, A, Bp, b, lo, hi, [=](int dst) {
         const int rr = dst / fs - i0, c = dst - (dst / fs) * fs;
-        return c < K ? rr * kSLd + c : RB * kSLd + rr * uc + c - K;
+        return c < K ? rr * sld + c : RB * sld + rr * uc + c - K;
     });
     __syncthreads();
     assemble_chunks<kSchurThreads, kAsmDepth>(d, ar, d.asm_base[p * kAsmKindCount + kAsmKSchur] + rb, b, Bp, Lb,
                                       [=](int rr, int c) {
-                                          return c < K ? rr * kSLd + c : RB * kSLd + rr * uc + c - K;
+                                          return c < K ? rr * sld + c : RB * sld + rr * uc + c - K;
                                       });
     __syncthreads();
     const int ntile = (uc + 63) / 64;
@@ -1420,7 +1380,7 @@ __global__ __launch_bounds__(kSchurThreads, 2) void k_schur(LuDevice d, const in
             if (cb < K) {
 #pragma unroll 4
                 for (int k = 0; k < K; ++k) {
-                    const T a = Lb[r * kSLd + k];
+                    const T a = Lb[r * sld + k];
 #pragma unroll
                     for (int c = 0; c < 4; ++c)
                         if (k <= cb + c) acc[c] += a * Ui[k * kXMaxK + cb + c];   // U11^-1 upper
@@ -1432,7 +1392,7 @@ __global__ __launch_bounds__(kSchurThreads, 2) void k_schur(LuDevice d, const in
                 for (int c = 0; c < 4; ++c) {
                     const int j = cb + c;
                     const T x = j < K ? acc[c] : T(0);
-                    Lb[r * kSLd + j] = x;
+                    Lb[r * sld + j] = x;
                     if (r < nr && j < K) G[(long long)(i0 + r) * fs + j] = x;
                 }
             }
@@ -1450,33 +1410,18 @@ __global__ __launch_bounds__(kSchurThreads, 2) void k_schur(LuDevice d, const in
             for (int k = 0; k < K; ++k) {
                 const T u = U[(long long)k * fs];
 #pragma unroll
-                for (int rr = 0; rr < RB; ++rr) acc[rr] -= Lb[rr * kSLd + k] * u;
+                for (int rr = 0; rr < RB; ++rr) acc[rr] -= Lb[rr * sld + k] * u;
             }
 #pragma unroll
             for (int rr = 0; rr < RB; ++rr)
                 if (rr < nr) G[(long long)(i0 + rr) * fs + K + j] = acc[rr];
         }
     } else {
-        // ---- U12 operands (B of the update), per wave and tile.  B-operand k of
-        // slot i:
-        //   FP32 (32x32x2):  k = 2 i + lane/32;
-        //   3xBF16 (32x32x16): k = 16 (i / 8) + 8 (lane/32) + i % 8.
-        float b0[kXMaxK / 2], b1[kXMaxK / 2];
-        auto load_u = [&](int t) {
-            const int ja = t * 64 + (lane & 31);
-            const bool oka = ja < uc, okb = ja + 32 < uc;
-            const float* U = G + K + ja;        // U12 row k at U + k * fs
-#pragma unroll
-            for (int i = 0; i < kXMaxK / 2; ++i) {
-                const int k = MODE == kMfmaF32 ? 2 * i + (lane >> 5)
-                                               : 16 * (i >> 3) + 8 * (lane >> 5) + (i & 7);
-                b0[i] = (k < K && oka) ? U[(long long)k * fs] : 0.f;
-                b1[i] = (k < K && okb) ? U[(long long)k * fs + 32] : 0.f;
-            }
-        };
-
-        // the first tile's U12 operands load while L21 is formed
-        if (wave < ntile) load_u(wave);
+        // ---- U12 operands (B of the update) of one 32 x 32 tile per wave;
+        // the first tile's load while L21 is formed
+        const int nt32 = (uc + 31) / 32;
+        U12Half<MODE> ops;
+        if (wave < nt32) u12_half_load<MODE>(ops, G + K, wave * 32 + (lane & 31), uc, fs, K);
 
         // ---- L21 = A21 U11^-1 (two 32 x 32 tiles, waves 0 and 1) -----------
         {
@@ -1487,7 +1432,7 @@ __global__ __launch_bounds__(kSchurThreads, 2) void k_schur(LuDevice d, const in
 #pragma unroll
                 for (int i = 0; i < kXMaxK / 2; ++i) {
                     if (2 * i < 32 * wave + 32) {         // U11^-1 upper: k <= j
-                        const float av = Lb[(lane & 31) * kSLd + 2 * i + (lane >> 5)];
+                        const float av = Lb[(lane & 31) * sld + 2 * i + (lane >> 5)];
                         acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av, ub[i], acc, 0, 0, 0);
                     }
                 }
@@ -1498,35 +1443,32 @@ __global__ __launch_bounds__(kSchurThreads, 2) void k_schur(LuDevice d, const in
                 for (int v = 0; v < 16; ++v) {
                     const int rr = (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
                     const float x = jc < K ? acc[v] : 0.f;
-                    Lb[rr * kSLd + jc] = x;
+                    Lb[rr * sld + jc] = x;
                     if (rr < nr && jc < K) G[(long long)(i0 + rr) * fs + jc] = x;
                 }
             }
             __syncthreads();
         }
 
-        // ---- C = A22 - L21 U12, 32 x 64 tiles, one wave each ----------------
-        for (int t = wave; t < ntile; t += kSchurThreads / 64) {
-            const int ja = t * 64 + (lane & 31);    // CB column of acc0 (acc1: + 32)
-            const bool oka = ja < uc, okb = ja + 32 < uc;
-            floatx16 acc0, acc1;
+        // ---- C = A22 - L21 U12, 32 x 32 tiles, one wave each ----------------
+        for (int t = wave; t < nt32; t += kSchurThreads / 64) {
+            const int ja = t * 32 + (lane & 31);    // CB column of this lane
+            const bool oka = ja < uc;
+            floatx16 acc;
 #pragma unroll
             for (int v = 0; v < 16; ++v) {
                 const int rr = (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
-                acc0[v] = oka ? Ab[rr * uc + ja] : 0.f;
-                acc1[v] = okb ? Ab[rr * uc + ja + 32] : 0.f;
+                acc[v] = oka ? Ab[rr * uc + ja] : 0.f;
             }
-            cb_tile_update<MODE>(acc0, acc1, b0, b1, Lb + (lane & 31) * kSLd, K);
+            cb_half_update<MODE>(acc, ops, Lb + (lane & 31) * sld, K);
             // next tile's U12 operands in flight during this tile's stores
-            if (t + kSchurThreads / 64 < ntile) load_u(t + kSchurThreads / 64);
+            if (t + kSchurThreads / 64 < nt32)
+                u12_half_load<MODE>(ops, G + K, ja + 32 * (kSchurThreads / 64), uc, fs, K);
             float* Cw = G + (long long)i0 * fs + K + ja;
 #pragma unroll
             for (int v = 0; v < 16; ++v) {
                 const int rr = (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
-                if (rr < nr) {
-                    if (oka) Cw[(long long)rr * fs] = acc0[v];
-                    if (okb) Cw[(long long)rr * fs + 32] = acc1[v];
-                }
+                if (rr < nr && oka) Cw[(long long)rr * fs] = acc[v];
             }
         }
     }
@@ -2924,42 +2866,44 @@ void cross_group(const LuDevice& d, const int* fronts, int count, int max_fsz, c
 }
 
 template <typename T>
-void schur_group(const LuDevice& d, const int* fronts, int count, int max_fsz, const double* A,
-                 int Bp, Arenas<T> ar, const T* uinv, int dense_mode, hipStream_t s) {
-    if (count <= 0) return;
-    if (max_fsz > kDenseMaxFsz || Bp % kSchurScen)
+void schur_group(const LuDevice& d, const int* fronts, int count, int max_nc, int max_uc,
+                 const double* A, int Bp, Arenas<T> ar, const T* uinv, int dense_mode,
+                 hipStream_t s) {
+    if (count <= 0 || max_uc <= 0) return;
+    if (max_uc > kDenseMaxFsz || max_nc > kXMaxK || Bp % kSchurScen)
         throw std::runtime_error("schur_group: front or batch outside the kernel's limits");
-    const int nrb = (max_fsz + schur_rows<T>() - 1) / schur_rows<T>();
+    const int nrb = (max_uc + schur_rows<T>() - 1) / schur_rows<T>();
     const long long blocks = (long long)count * nrb * Bp;
     if (blocks >= (1LL << 31)) throw std::runtime_error("schur_group: grid too large");
-    const size_t shmem = schur_lds<T>(max_fsz);
+    const int sld = schur_ld(max_nc);
+    const size_t shmem = schur_lds<T>(sld, max_uc);
     if constexpr (sizeof(T) == 8) {
         static bool attr_set = false;
         if (!attr_set) {
             CUPF_HIP(hipFuncSetAttribute((const void*)k_schur<double, kMfmaNone>,
                                          hipFuncAttributeMaxDynamicSharedMemorySize,
-                                         int(schur_lds<double>(kDenseMaxFsz))));
+                                         int(schur_lds<double>(kSLd, kDenseMaxFsz))));
             attr_set = true;
         }
         hipLaunchKernelGGL((k_schur<double, kMfmaNone>), dim3(unsigned(blocks)), dim3(kSchurThreads), shmem,
-                           s, d, fronts, A, Bp, ar, uinv, nrb);
+                           s, d, fronts, A, Bp, ar, uinv, nrb, sld);
     } else {
         static bool attr_set = false;
         if (!attr_set) {
             CUPF_HIP(hipFuncSetAttribute((const void*)k_schur<float, kMfmaF32>,
                                          hipFuncAttributeMaxDynamicSharedMemorySize,
-                                         int(schur_lds<float>(kDenseMaxFsz))));
+                                         int(schur_lds<float>(kSLd, kDenseMaxFsz))));
             CUPF_HIP(hipFuncSetAttribute((const void*)k_schur<float, kMfmaBf16x3>,
                                          hipFuncAttributeMaxDynamicSharedMemorySize,
-                                         int(schur_lds<float>(kDenseMaxFsz))));
+                                         int(schur_lds<float>(kSLd, kDenseMaxFsz))));
             attr_set = true;
         }
         if (dense_mode == kMfmaBf16x3)
             hipLaunchKernelGGL((k_schur<float, kMfmaBf16x3>), dim3(unsigned(blocks)), dim3(kSchurThreads),
-                               shmem, s, d, fronts, A, Bp, ar, uinv, nrb);
+                               shmem, s, d, fronts, A, Bp, ar, uinv, nrb, sld);
         else
             hipLaunchKernelGGL((k_schur<float, kMfmaF32>), dim3(unsigned(blocks)), dim3(kSchurThreads),
-                               shmem, s, d, fronts, A, Bp, ar, uinv, nrb);
+                               shmem, s, d, fronts, A, Bp, ar, uinv, nrb, sld);
     }
     CUPF_HIP(hipGetLastError());
 }
@@ -3266,7 +3210,7 @@ CUPF_LU_INST(float)
 #define CUPF_XS_INST(T)                                                                          \
     template void cross_group<T>(const LuDevice&, const int*, int, int, const double*, int,       \
                                  Arenas<T>, T*, double, int, int*, bool, hipStream_t);            \
-    template void schur_group<T>(const LuDevice&, const int*, int, int, const double*, int,       \
+    template void schur_group<T>(const LuDevice&, const int*, int, int, int, const double*, int,  \
                                  Arenas<T>, const T*, int, hipStream_t);
 
 CUPF_XS_INST(double)
