@@ -1227,11 +1227,12 @@ inline size_t schur_lds(int sld, int max_uc) {
     return size_t(schur_rows<T>()) * size_t(sld + max_uc) * sizeof(T);
 }
 
-// U12 operands of one 32 x 32 tile held across several row strips: FP32
-// values (FP32 MFMA, B-operand k of slot i = 2 i + lane/32), or their BF16
-// high and low parts, split once (3xBF16, k of element t8 of group kq =
-// 16 kq + 8 (lane/32) + t8).  Columns j (< uc) of U12, whose row k starts at
-// U + k * fs; loads from indices clamped into the block, zeroed outside it.
+// U12 operands of one 32 x 32 tile (held across several row strips by the
+// chain kernel): FP32 values (FP32 MFMA, B-operand k of slot i = 2 i +
+// lane/32), or their BF16 high and low parts, split once (3xBF16, k of
+// element t8 of group kq = 16 kq + 8 (lane/32) + t8).  Columns j (< uc) of
+// U12, whose row k starts at U + k * fs; loads from indices clamped into the
+// block, zeroed outside it.
 template <int MODE>
 struct U12Half {
     float b[kXMaxK / 2];
@@ -1306,6 +1307,9 @@ __device__ __forceinline__ void cb_half_update(floatx16& acc, const U12Half<MODE
     }
 }
 
+// Three workgroups per CU: the 32 x 32 update tiles keep the kernel within
+// 168 VGPRs; the LDS (schur_lds) allows three up to 392 contribution columns
+// at <= 32 pivots.
 template <typename T, int MODE>
 __global__ __launch_bounds__(kSchurThreads, 3) void k_schur(LuDevice d, const int* __restrict__ fronts,
                                                          const double* __restrict__ A, int Bp,
