@@ -1739,10 +1739,13 @@ __global__ __launch_bounds__(kChainThreads, 2) void k_chain_strips(LuDevice d, C
 //   k_schur_tile (one workgroup per (tile, scenario), 256 lanes): one
 //     64 x 64 tile of the contribution block C = A22 - L21 U12.  The tile of
 //     A22 is assembled in LDS (entries, then the tile's chunk map) while each
-//     wave's operands -- 32 rows of L21 and 32 columns of U12, read from the
-//     front -- are in flight; each wave then forms its 32 x 32 quarter on
-//     MFMA (FP32 32x32x2, 3xBF16 32x32x16, or FP64 16x16x4 as 2 x 2 tiles)
-//     starting from the assembled values, and writes it once.
+//     wave's operands -- 32 rows of L21 and (FP32) 32 columns of U12, read
+//     from the front -- are in flight; each wave then forms its 32 x 32
+//     quarter on MFMA (FP32 32x32x2, 3xBF16 32x32x16, or FP64 16x16x4 as
+//     2 x 2 tiles, a 16-column half at a time with that half's U12 operands
+//     loaded after the assembly) starting from the assembled values, and
+//     writes it once.  Three workgroups per CU: FP64 holds only its L21
+//     operands through the assembly, which keeps it within 168 VGPRs.
 //
 // The operands' k index is split by lane group so every lane reads a run of
 // contiguous L21 elements: in an FP32 32x32x2 instruction i, lane half h
@@ -1885,7 +1888,7 @@ __global__ __launch_bounds__(256) void k_l21(LuDevice d, const int* __restrict__
 }
 
 template <typename T, int MODE>
-__global__ __launch_bounds__(256) void k_schur_tile(LuDevice d, const int* __restrict__ fronts,
+__global__ __launch_bounds__(256, 3) void k_schur_tile(LuDevice d, const int* __restrict__ fronts,
                                                     const int2* __restrict__ tiles, int ntiles,
                                                     int scen_group, const double* __restrict__ A,
                                                     int Bp, Arenas<T> ar) {
@@ -1938,7 +1941,8 @@ __global__ __launch_bounds__(256) void k_schur_tile(LuDevice d, const int* __res
             for (int i = 0; i < NOP; ++i) {
                 const int k = full ? KW * g + i : min(KW * g + i, K - 1);   // in [0, K)
                 a[s][i] = *reinterpret_cast<const T*>(Gb + unsigned(r * fs + k) * ES);
-                u[s][i] = *reinterpret_cast<const T*>(Gb + unsigned(k * fs + c) * ES);
+                if constexpr (sizeof(T) == 4)
+                    u[s][i] = *reinterpret_cast<const T*>(Gb + unsigned(k * fs + c) * ES);
             }
         }
     }
@@ -2006,37 +2010,43 @@ __global__ __launch_bounds__(256) void k_schur_tile(LuDevice d, const int* __res
             if (rr < nr && cl < ncl) Cw[(long long)rr * fs] = acc[v];
         }
     } else {
+        // one 16-column sub-tile at a time: its U12 operands load, then the
+        // two 16 x 16 row tiles update and store
         const int q = lane >> 4, li = lane & 15;
-        doublex4 acc[NT][NT];
+        const char* Gb = reinterpret_cast<const char*>(G);
+        const bool full = K == kXMaxK;                           // uniform
 #pragma unroll
-        for (int s = 0; s < NT; ++s)
+        for (int c = 0; c < NT; ++c) {
+            const int col = min(c0 + 32 * wc + 16 * c + li, fs - 1);
+            T uc[NOP];
 #pragma unroll
-            for (int c = 0; c < NT; ++c)
+            for (int i = 0; i < NOP; ++i) {
+                const int k = full ? 16 * q + i : min(16 * q + i, K - 1);   // in [0, K)
+                uc[i] = *reinterpret_cast<const T*>(Gb + unsigned(k * fs + col) * 8u);
+            }
+            doublex4 acc[NT];
+#pragma unroll
+            for (int s = 0; s < NT; ++s)
 #pragma unroll
                 for (int v = 0; v < 4; ++v)
-                    acc[s][c][v] = At[(32 * wr + 16 * s + q + 4 * v) * LD + 32 * wc + 16 * c + li];
+                    acc[s][v] = At[(32 * wr + 16 * s + q + 4 * v) * LD + 32 * wc + 16 * c + li];
 #pragma unroll
-        for (int i = 0; i < NOP; ++i) {
-            if (i < K) {                                 // uniform
+            for (int i = 0; i < NOP; ++i) {
+                if (i < K) {                             // uniform
 #pragma unroll
-                for (int s = 0; s < NT; ++s)
-#pragma unroll
-                    for (int c = 0; c < NT; ++c)
-                        acc[s][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a[s][i], u[c][i], acc[s][c],
-                                                                         0, 0, 0);
+                    for (int s = 0; s < NT; ++s)
+                        acc[s] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a[s][i], uc[i], acc[s], 0, 0, 0);
+                }
             }
-        }
+            const int cl = 32 * wc + 16 * c + li;
 #pragma unroll
-        for (int s = 0; s < NT; ++s)
-#pragma unroll
-            for (int c = 0; c < NT; ++c) {
-                const int cl = 32 * wc + 16 * c + li;
+            for (int s = 0; s < NT; ++s)
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
                     const int rr = 32 * wr + 16 * s + q + 4 * v;
-                    if (rr < nr && cl < ncl) G[(long long)(r0 + rr) * fs + c0 + cl] = acc[s][c][v];
+                    if (rr < nr && cl < ncl) G[(long long)(r0 + rr) * fs + c0 + cl] = acc[s][v];
                 }
-            }
+        }
     }
 }
 
