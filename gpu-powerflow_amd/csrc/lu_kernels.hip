@@ -1739,13 +1739,13 @@ __global__ __launch_bounds__(kChainThreads, 2) void k_chain_strips(LuDevice d, C
 //   k_schur_tile (one workgroup per (tile, scenario), 256 lanes): one
 //     64 x 64 tile of the contribution block C = A22 - L21 U12.  The tile of
 //     A22 is assembled in LDS (entries, then the tile's chunk map) while each
-//     wave's operands -- 32 rows of L21 and (FP32) 32 columns of U12, read
-//     from the front -- are in flight; each wave then forms its 32 x 32
-//     quarter on MFMA (FP32 32x32x2, 3xBF16 32x32x16, or FP64 16x16x4 as
-//     2 x 2 tiles, a 16-column half at a time with that half's U12 operands
-//     loaded after the assembly) starting from the assembled values, and
-//     writes it once.  Three workgroups per CU: FP64 holds only its L21
-//     operands through the assembly, which keeps it within 168 VGPRs.
+//     wave's L21 operands (its 32 rows, read from the front) are in flight;
+//     each wave then loads its U12 operands (32 columns) and forms its
+//     32 x 32 quarter on MFMA (FP32 32x32x2, 3xBF16 32x32x16, or FP64
+//     16x16x4 as 2 x 2 tiles, a 16-column half at a time with that half's
+//     U12 operands) starting from the assembled values, and writes it once.
+//     Holding only the L21 operands through the assembly keeps FP64 within
+//     168 VGPRs (three workgroups per CU) and FP32 within 97.
 //
 // The operands' k index is split by lane group so every lane reads a run of
 // contiguous L21 elements: in an FP32 32x32x2 instruction i, lane half h
@@ -1915,7 +1915,7 @@ __global__ __launch_bounds__(256, 3) void k_schur_tile(LuDevice d, const int* __
     T* G = ar.a1 + (long long)b * d.arena1 + d.off[p];
     const int* rl = d.row_lb + d.lb_ptr[p];
 
-    // ---- operands, in flight during the assembly ----------------------------
+    // ---- L21 operands, in flight during the assembly -------------------------
     constexpr int NOP = sizeof(T) == 4 ? 32 : 16;
     constexpr int NT = sizeof(T) == 4 ? 1 : 2;                  // 16 x 16 sub-tiles per side (FP64)
     T a[NT][NOP], u[NT][NOP];
@@ -1936,13 +1936,10 @@ __global__ __launch_bounds__(256, 3) void k_schur_tile(LuDevice d, const int* __
 #pragma unroll
         for (int s = 0; s < NT; ++s) {
             const int r = min(r0 + 32 * wr + 16 * s + li, fs - 1);
-            const int c = min(c0 + 32 * wc + 16 * s + li, fs - 1);
 #pragma unroll
             for (int i = 0; i < NOP; ++i) {
                 const int k = full ? KW * g + i : min(KW * g + i, K - 1);   // in [0, K)
                 a[s][i] = *reinterpret_cast<const T*>(Gb + unsigned(r * fs + k) * ES);
-                if constexpr (sizeof(T) == 4)
-                    u[s][i] = *reinterpret_cast<const T*>(Gb + unsigned(k * fs + c) * ES);
             }
         }
     }
@@ -1972,6 +1969,16 @@ __global__ __launch_bounds__(256, 3) void k_schur_tile(LuDevice d, const int* __
 
     if constexpr (sizeof(T) == 4) {
         const int h = lane >> 5, li = lane & 31;
+        {
+            const int c = min(c0 + 32 * wc + li, fs - 1);
+            const bool full = K == kXMaxK;
+            const char* Gb = reinterpret_cast<const char*>(G);
+#pragma unroll
+            for (int i = 0; i < NOP; ++i) {
+                const int k = full ? 32 * h + i : min(32 * h + i, K - 1);
+                u[0][i] = *reinterpret_cast<const T*>(Gb + unsigned(k * fs + c) * 4u);
+            }
+        }
         floatx16 acc;
 #pragma unroll
         for (int v = 0; v < 16; ++v) {
