@@ -194,8 +194,9 @@ void lu_factor_group(const LuDevice& d, int tier, const int* fronts, int count, 
 // pivot_side_elems(count, Bp) elements, then U12 by 64-column chunks) and
 // schur_group (L21 and the
 // contribution block C = A22 - L21 U12, assembled and updated in one pass,
-// schur_block_rows(sizeof(T)) rows per workgroup).  FP32 dense_mode 1: FP32
-// MFMA, 2: 3xBF16 MFMA; FP64 runs on the vector units.
+// schur_block_rows(sizeof(T)) rows per workgroup; max_nc and max_uc, the
+// group's most pivots and widest contribution block, size its LDS).  FP32
+// dense_mode 1: FP32 MFMA, 2: 3xBF16 MFMA; FP64 runs on the vector units.
 bool cross_schur_path(int max_fsz, int max_nc);
 // schur_group (row blocks) handles fronts of at most 512 rows
 bool row_schur_fits(int max_fsz);
