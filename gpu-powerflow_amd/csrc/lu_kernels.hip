@@ -26,10 +26,11 @@
 // GuardedPivot, custom_linear_solver/src/factorize/front_ops.cuh:146-160; the
 // flag is returned to the caller here).
 //
-// Workgroup-per-scenario grids are 1-D and decoded so that each of the 8 XCDs
-// (blocks are dispatched round-robin over XCDs) owns a contiguous range of
-// scenarios of a front: neighbouring scenarios share the cache lines of the
-// interleaved arrays inside one XCD's L2.
+// Workgroup-per-scenario grids are 1-D and decoded in launch order, the
+// scenario (or a small group of scenarios) fastest: neighbouring workgroups
+// take neighbouring scenarios, which share the cache lines of the interleaved
+// arrays.  (A remap giving each of the 8 XCDs a contiguous range of the grid
+// measured 5-6 % slower on the factorisation and 3-5 % on the solve.)
 #include "kernels.hpp"
 #include "hip_common.hpp"
 
@@ -44,7 +45,6 @@ constexpr int kLdsBytes = 64 * 1024;
 constexpr int kKbMax = 32;
 constexpr int kTile = 64;
 constexpr int kTier0Max = 16;
-constexpr int kXcd = 8;
 
 template <typename T>
 __device__ inline T guard_pivot(T piv, double eps, int static_piv, int& flag) {
@@ -61,9 +61,7 @@ __device__ inline T guard_pivot(T piv, double eps, int static_piv, int& flag) {
 
 // Flat block id -> (front index in the group, scenario); see file comment.
 __device__ inline void decode_block(int Bp, int& fi, int& b) {
-    const long long lin = blockIdx.x;
-    const long long per = (long long)gridDim.x / kXcd;   // gridDim.x = count * Bp, Bp % 64 == 0
-    const long long w = (lin % kXcd) * per + lin / kXcd;
+    const long long w = blockIdx.x;                      // gridDim.x = count * Bp
     fi = int(w / Bp);
     b = int(w % Bp);
 }
@@ -600,10 +598,8 @@ __global__ __launch_bounds__(kAsmThreads) void k_assemble_rows(LuDevice d,
                                                                 int rb_rows) {
     extern __shared__ unsigned char smem_raw[];
     T* F = reinterpret_cast<T*>(smem_raw);   // [rb_rows][fs]
-    // lin -> (front, row block, scenario), scenario fastest inside one XCD
-    const long long lin = blockIdx.x;
-    const long long per = (long long)gridDim.x / kXcd;
-    const long long w = (lin % kXcd) * per + lin / kXcd;
+    // lin -> (front, row block, scenario), scenario fastest
+    const long long w = blockIdx.x;
     const int b = int(w % Bp);
     const long long rest = w / Bp;
     const int rb = int(rest % nrb);
@@ -1208,7 +1204,7 @@ __device__ __forceinline__ void u12_chunk(const LuDevice& d, const int* __restri
 }
 
 constexpr int kSchurThreads = 256;
-constexpr int kSchurScen = 16;                 // scenarios interleaved inside an XCD range
+constexpr int kSchurScen = 16;                 // scenarios interleaved over a front's row blocks
 constexpr int kSLd = kXMaxK + 2;               // L21 block row stride, fronts of 33-64 pivots
 constexpr int kMfmaNone = 0;                   // FP64: vector-unit FMA tiles
 
@@ -1320,13 +1316,11 @@ __global__ __launch_bounds__(kSchurThreads, 3) void k_schur(LuDevice d, const in
     extern __shared__ unsigned char smem_raw[];
     T* Lb = reinterpret_cast<T*>(smem_raw);                       // [RB][sld]: A21, then L21
     T* Ab = Lb + RB * sld;                                        // [RB][uc]
-    // lin -> (front, scenario group, row block, scenario in group): inside an
-    // XCD's contiguous range, kSchurScen neighbouring scenarios run together
-    // (they share the cache lines of the interleaved J values) over the row
-    // blocks of one front (they share that scenario's U12 in L2)
-    const long long lin = blockIdx.x;
-    const long long per = (long long)gridDim.x / kXcd;
-    long long w = (lin % kXcd) * per + lin / kXcd;
+    // lin -> (front, scenario group, row block, scenario in group):
+    // kSchurScen neighbouring scenarios run together (they share the cache
+    // lines of the interleaved J values) over the row blocks of one front
+    // (they share that scenario's U12 in L2)
+    long long w = blockIdx.x;
     const int bl = int(w % kSchurScen);
     w /= kSchurScen;
     const int rb = int(w % nrb);
@@ -1517,13 +1511,10 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
 }
 
-// Block decode of a chain launch of `parts` workgroups per scenario: an XCD
-// owns a contiguous range of (scenario, part), the parts of one scenario
-// together (they share its U12 in L2).  The block count is a multiple of 8.
+// Block decode of a chain launch of `parts` workgroups per scenario: the
+// parts of one scenario are neighbours (they share its U12 in L2).
 __device__ inline void decode_chain(int parts, int& part, int& b) {
-    const long long lin = blockIdx.x;
-    const long long per = (long long)gridDim.x / kXcd;
-    const long long w = (lin % kXcd) * per + lin / kXcd;
+    const long long w = blockIdx.x;
     part = int(w % parts);
     b = int(w / parts);
 }
@@ -1757,10 +1748,10 @@ __global__ __launch_bounds__(kChainThreads, 2) void k_chain_strips(LuDevice d, C
 // assignment shared by both operands gives the same product.
 //
 // Tiles are listed per group on the host (front index in the group, row
-// block << 16 | column tile), row-major per front; blocks are decoded so that
-// an XCD runs its scenarios' tiles in list order, a few neighbouring
+// block << 16 | column tile), row-major per front; blocks are decoded so
+// that a scenario group's tiles run in list order, a few neighbouring
 // scenarios interleaved (they share the J values' cache lines), so a
-// front's L21 rows and U12 columns are re-read from that XCD's L2.
+// front's L21 rows and U12 columns are re-read from L2.
 // ===========================================================================
 
 constexpr int kTileRows = kSchurTileRows;
@@ -1856,12 +1847,9 @@ __device__ __forceinline__ void l21_block(const LuDevice& d, const int* __restri
 }
 
 // Block decode of the pivot-row chunk and pivot-column block launches:
-// lin -> (front, part, scenario), an XCD owning a contiguous range of
-// scenarios (the block count is a multiple of 8).
+// lin -> (front, part, scenario), scenario fastest.
 __device__ inline void decode_part(int Bp, int parts, int& fi, int& part, int& b) {
-    const long long lin = blockIdx.x;
-    const long long per = (long long)gridDim.x / kXcd;
-    long long w = (lin % kXcd) * per + lin / kXcd;
+    long long w = blockIdx.x;
     b = int(w % Bp);
     w /= Bp;
     part = int(w % parts);
@@ -1895,9 +1883,7 @@ __global__ __launch_bounds__(256, 3) void k_schur_tile(LuDevice d, const int* __
     constexpr int LD = tile_ld<T>();
     extern __shared__ unsigned char smem_raw[];
     T* At = reinterpret_cast<T*>(smem_raw);                        // [kTileRows][LD]
-    const long long lin = blockIdx.x;
-    const long long per = (long long)gridDim.x / kXcd;
-    long long w = (lin % kXcd) * per + lin / kXcd;
+    long long w = blockIdx.x;
     const int bl = int(w % scen_group);
     w /= scen_group;
     const int t = int(w % ntiles);
